@@ -1,0 +1,184 @@
+"""Benchmark of the batched gate-bootstrapping workload on one MI355X.
+
+Workload (one step): 4096 independent gate bootstraps on seeded random bit
+pairs, 2048 NAND + 2048 XOR, n=636, N=1024, BSK 2^7 x 3, KS 2^2 x 8, FP64
+negacyclic FFT, one persistent blind-rotation launch per gate batch followed by
+one batched key switch.  Keys are generated on the GPU from seed 1; inputs are
+resident in HBM before the timed region.
+
+Prints one JSON line (see the README of the driver contract).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "gpu-circuitbootstrapping-aestransciphering_amd")
+sys.path.insert(0, PKG)
+
+METRIC = "AES-128 blocks/sec + bootstraps/sec at 1 MI355X, batched FP32-split vs plain FP64"
+BATCH = 4096
+N_LWE, N_RING, BK_L, M_FFT = 636, 1024, 3, 512
+# algorithmic FLOPs of one external product: (k+1)(l+1) FFTs of M points (5 M log2 M each)
+# plus the Fourier-domain MAC 8 (k+1)^2 l M, k = 1
+F_EP = 2 * (BK_L + 1) * 5 * M_FFT * 9 + 8 * 4 * BK_L * M_FFT
+F_PBS = N_LWE * F_EP  # 148.49 MFLOP per blind rotation
+BSK_BYTES = N_LWE * 2 * BK_L * 2 * M_FFT * 16  # Fourier BSK streamed once per pass (62.5 MB)
+PEAK_FP64_TFLOPS = 78.6
+
+
+def cpu_baseline(sample=24):
+    """The oracle's single-thread gate bootstrap on the host cores (a bounded sample)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    keys = O.GateKeys(1)
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, 2, sample)
+    b = rng.integers(0, 2, sample)
+    c1 = [keys.encrypt(int(a[k]), 2 * k) for k in range(sample)]
+    c2 = [keys.encrypt(int(b[k]), 2 * k + 1) for k in range(sample)]
+    t0 = time.perf_counter()
+    for k in range(sample):
+        keys.bootstrap(O.NAND if k % 2 == 0 else O.XOR, c1[k], c2[k])
+    dt = time.perf_counter() - t0
+    return {"value": round(sample / dt, 3), "unit": "bootstraps/s", "cores": 1, "kind": "port",
+            "sample": f"{sample} gate bootstraps (NAND/XOR alternating) of the same parameter set, "
+                      f"oracle C restatement, FP64 radix-2 FFT, one thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=3, help="plain per-bootstrap form samples")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              f"(bootstraps are independent: a multi-GPU run would shard the batch, see DESIGN.md)")
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import tfhe_amd
+
+    torch.cuda.init()
+    stream = torch.cuda.current_stream()
+    sptr = stream.cuda_stream
+
+    ctx = tfhe_amd.GateContext(max_batch=BATCH)
+    sk = tfhe_amd.lwe_keygen(N_LWE, 1)
+    ctx.bt_keygen(sk, 1)
+    rng = np.random.default_rng(1)
+    a = rng.integers(0, 2, BATCH)
+    b = rng.integers(0, 2, BATCH)
+    c1 = np.stack([tfhe_amd.lwe_encrypt(sk, 2, 2 * k, int(a[k])) for k in range(BATCH)])
+    c2 = np.stack([tfhe_amd.lwe_encrypt(sk, 2, 2 * k + 1, int(b[k])) for k in range(BATCH)])
+    d1 = torch.from_numpy(c1.view(np.int32).copy()).cuda()
+    d2 = torch.from_numpy(c2.view(np.int32).copy()).cuda()
+    d_out = torch.empty_like(d1)
+    d_lweN = torch.empty((BATCH, N_RING + 1), dtype=torch.int32, device="cuda")
+    half = BATCH // 2
+    L = tfhe_amd.lib()
+
+    def ptr(t):
+        return tfhe_amd._ptr(t)
+
+    def step(events=None):
+        # NAND on the first half, XOR on the second half
+        for g, lo in ((tfhe_amd.NAND, 0), (tfhe_amd.XOR, half)):
+            sl = slice(lo, lo + half)
+            if events is not None:
+                events.append(torch.cuda.Event(enable_timing=True))
+                events[-1].record(stream)
+            tfhe_amd.check(L.tfhe_gate_blind_rotate_batch(ctx.h, g, ptr(d1[sl]), ptr(d2[sl]), half,
+                                                          ptr(d_lweN[sl]), sptr))
+            if events is not None:
+                events.append(torch.cuda.Event(enable_timing=True))
+                events[-1].record(stream)
+            tfhe_amd.check(L.tfhe_gate_keyswitch_batch(ctx.h, ptr(d_lweN[sl]), half, ptr(d_out[sl]), sptr))
+            if events is not None:
+                events.append(torch.cuda.Event(enable_timing=True))
+                events[-1].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    events = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(events)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    # correctness of the timed output (last step)
+    got = d_out.cpu().numpy().view(np.uint32)
+    want = np.concatenate([1 - (a[:half] & b[:half]), a[half:] ^ b[half:]])
+    s64 = sk.astype(np.uint64)
+    ph = (got[:, -1].astype(np.uint64) - (got[:, :-1].astype(np.uint64) @ s64)) % 2 ** 32
+    dec = ((ph + (1 << 29)) >> 30) & 3
+    failures = int(np.count_nonzero(dec != want))
+
+    br_ms, ks_ms = [], []
+    for k in range(0, len(events), 3):
+        br_ms.append(events[k].elapsed_time(events[k + 1]))
+        ks_ms.append(events[k + 1].elapsed_time(events[k + 2]))
+    br_avg = sum(br_ms) / len(br_ms)
+    ks_avg = sum(ks_ms) / len(ks_ms)
+    achieved_tflops = F_PBS * half / (br_avg * 1e-3) / 1e12
+
+    # plain per-bootstrap form (reference launch structure), a few single gates
+    plain_rate = None
+    if args.plain_steps > 0:
+        one_out = torch.empty((1, N_LWE + 1), dtype=torch.int32, device="cuda")
+        ctx.eval_plain(tfhe_amd.NAND, d1[0:1], d2[0:1], one_out, stream=sptr)
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        for k in range(args.plain_steps):
+            ctx.eval_plain(tfhe_amd.NAND, d1[k:k + 1], d2[k:k + 1], one_out, stream=sptr)
+        torch.cuda.synchronize()
+        plain_rate = args.plain_steps / (time.perf_counter() - tp)
+
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "gate_blind_rotate_c1_traffic.json")
+    if os.path.exists(prof):
+        traffic = json.load(open(prof)).get("hbm_bytes_per_launch")
+
+    ms_per_step = elapsed * 1e3 / args.steps
+    line = {
+        "metric": METRIC,
+        "value": round(BATCH * args.steps / elapsed, 1),
+        "unit": "bootstraps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "batched gate bootstraps (configs[1]): 4096 = 2048 NAND + 2048 XOR per step",
+                   "n": N_LWE, "N": N_RING, "bsk_gadget": "2^7 x 3", "ks_gadget": "2^2 x 8",
+                   "torus_bits": 32, "fft": "fp64 negacyclic, wavefront-resident", "batch": BATCH},
+        "roofline": {"bound": "valu", "kernel": "k_gate_blind_rotate",
+                     "achieved": round(achieved_tflops, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved_tflops / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
+                     "algorithmic_flop_per_launch": F_PBS * half,
+                     "avg_launch_ms": round(br_avg, 4)},
+        "breakdown_ms": {"blind_rotate_per_2048": round(br_avg, 4), "keyswitch_per_2048": round(ks_avg, 4)},
+        "plain_fp64_per_bootstrap_rate": None if plain_rate is None else round(plain_rate, 2),
+        "decrypt_failures": failures,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,58 @@
+// capi_common.hpp -- error reporting, device allocation and FFT twiddle tables
+// shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/tfhe_amd.h"
+#include "device_common.hpp"
+
+namespace tfhe {
+
+int fail(int code, const std::string& msg);
+int hipcheck(hipError_t e, const char* what);
+int check_launch(const char* what);
+int dmalloc(void** p, size_t bytes);
+
+// Twiddle tables of the wavefront FFT (see device_common.hpp for the layout).
+struct DeviceTables {
+    FFTTables T{};
+    cplx* d_buf = nullptr;
+    int init(int N) {
+        if (N != 1024) return fail(TFHE_E_PARAM, "FFT tables: only N = 1024 is built");
+        std::vector<cplx> h(3 * 512);
+        const double PI = 3.14159265358979323846264338327950288;
+        for (int a = 0; a < 8; ++a)
+            for (int b = 0; b < 64; ++b) {
+                const double ang = PI * (64.0 * a + b) / N;
+                h[a * 64 + b] = {std::cos(ang), std::sin(ang)};
+            }
+        for (int k = 0; k < 8; ++k)
+            for (int b = 0; b < 64; ++b) {
+                const double ang = -2.0 * PI * (double)((b * k) % 512) / 512.0;
+                h[512 + k * 64 + b] = {std::cos(ang), std::sin(ang)};
+            }
+        for (int e = 0; e < 8; ++e)
+            for (int l = 0; l < 64; ++l) {
+                const double ang = -2.0 * PI * (double)(((l & 7) * e) % 64) / 64.0;
+                h[1024 + e * 64 + l] = {std::cos(ang), std::sin(ang)};
+            }
+        int rc = dmalloc((void**)&d_buf, h.size() * sizeof(cplx));
+        if (rc) return rc;
+        rc = hipcheck(hipMemcpy(d_buf, h.data(), h.size() * sizeof(cplx), hipMemcpyHostToDevice), "tables");
+        if (rc) return rc;
+        T.twist = d_buf;
+        T.p1 = d_buf + 512;
+        T.p2 = d_buf + 1024;
+        return TFHE_OK;
+    }
+    void release() {
+        if (d_buf) (void)hipFree(d_buf);
+        d_buf = nullptr;
+    }
+};
+
+}  // namespace tfhe

@@ -1,0 +1,221 @@
+// device_common.hpp -- device-side building blocks shared by every gfx950 kernel:
+// the Philox4x32-10 key-material sampler, signed gadget decomposition, complex
+// helpers and the wavefront-resident negacyclic FFT.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tfhe {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).  Counter = (j, id, sub, domain), key = seed.
+// ---------------------------------------------------------------------------
+struct u32x4 { uint32_t x, y, z, w; };
+
+__host__ __device__ inline u32x4 philox4x32_10(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2,
+                                               uint32_t c3) {
+    uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    }
+    return {c0, c1, c2, c3};
+}
+
+enum : uint32_t {
+    DOM_LWE_SK = 1, DOM_RLWE1_SK = 2, DOM_RLWE2_SK = 3, DOM_BSK1 = 4, DOM_KSK1 = 5,
+    DOM_BSK2 = 6, DOM_PFKS = 7, DOM_KSK2 = 8, DOM_ENC = 16
+};
+
+__host__ __device__ inline uint32_t rand32(uint64_t seed, uint32_t dom, uint32_t id, uint32_t j) {
+    return philox4x32_10(seed, j, id, 0, dom).x;
+}
+
+__host__ __device__ inline uint64_t rand64(uint64_t seed, uint32_t dom, uint32_t id, uint32_t j) {
+    u32x4 w = philox4x32_10(seed, j, id, 0, dom);
+    return (uint64_t)w.x | ((uint64_t)w.y << 32);
+}
+
+// Irwin-Hall(12) noise with unit variance scaled by 2^32, rounded to 2^-nshift of that.
+__host__ __device__ inline int64_t noise(uint64_t seed, uint32_t dom, uint32_t id, uint32_t j, int nshift) {
+    uint64_t s = 0;
+    for (uint32_t sub = 1; sub <= 3; ++sub) {
+        u32x4 w = philox4x32_10(seed, j, id, sub, dom);
+        s += (uint64_t)w.x + w.y + w.z + w.w;
+    }
+    int64_t z = (int64_t)s - (int64_t)6 * ((int64_t)1 << 32);
+    if (nshift > 0) return (z + ((int64_t)1 << (nshift - 1))) >> nshift;
+    return z * ((int64_t)1 << (-nshift));
+}
+
+__host__ __device__ inline int32_t secret_bit(uint64_t seed, uint32_t dom, uint32_t i) {
+    return (int32_t)(philox4x32_10(seed, i, 0, 0, dom).x & 1u);
+}
+
+// ---------------------------------------------------------------------------
+// Signed gadget decomposition on a 32-bit torus: centre, drop the low
+// 32 - l*logB bits with rounding, then l signed digits in [-B/2, B/2), least
+// significant first.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int32_t sext32(int32_t v, int bits) {
+    return (v << (32 - bits)) >> (32 - bits);
+}
+
+template <int LOGB, int L>
+__device__ __forceinline__ void decompose32(uint32_t t, int32_t (&dig)[L]) {
+    constexpr int IGN = 32 - L * LOGB;
+    int32_t d = (int32_t)t;
+    if constexpr (IGN > 0) {
+        int32_t r = sext32(d, IGN);
+        d = (d >> IGN) + (r < 0 ? 1 : 0);
+    }
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        int32_t r = sext32(d, LOGB);
+        dig[j] = r;
+        d = (d >> LOGB) + (r < 0 ? 1 : 0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Complex FP64 helpers
+// ---------------------------------------------------------------------------
+struct cplx { double re, im; };
+
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cplx csub(cplx a, cplx b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+    return {fma(a.re, b.re, -a.im * b.im), fma(a.re, b.im, a.im * b.re)};
+}
+__device__ __forceinline__ cplx cmulc(cplx a, cplx b) {  // a * conj(b)
+    return {fma(a.re, b.re, a.im * b.im), fma(a.im, b.re, -a.re * b.im)};
+}
+__device__ __forceinline__ void cmac(cplx& acc, cplx a, cplx b) {
+    acc.re = fma(a.re, b.re, acc.re);
+    acc.re = fma(-a.im, b.im, acc.re);
+    acc.im = fma(a.re, b.im, acc.im);
+    acc.im = fma(a.im, b.re, acc.im);
+}
+
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ cplx mul_mi(cplx a) {
+    if constexpr (INV) return {-a.im, a.re};
+    else return {a.im, -a.re};
+}
+
+// 8-point DFT in registers, natural order in and out; forward uses exp(-2 pi i/8).
+template <bool INV>
+__device__ __forceinline__ void dft8(cplx (&x)[8]) {
+    constexpr double R = 0.70710678118654752440084436210485;
+    // DFT4 on even and odd halves
+    cplx e0 = cadd(x[0], x[4]), e1 = csub(x[0], x[4]);
+    cplx e2 = cadd(x[2], x[6]), e3 = mul_mi<INV>(csub(x[2], x[6]));
+    cplx E0 = cadd(e0, e2), E2 = csub(e0, e2), E1 = cadd(e1, e3), E3 = csub(e1, e3);
+    cplx o0 = cadd(x[1], x[5]), o1 = csub(x[1], x[5]);
+    cplx o2 = cadd(x[3], x[7]), o3 = mul_mi<INV>(csub(x[3], x[7]));
+    cplx O0 = cadd(o0, o2), O2 = csub(o0, o2), O1 = cadd(o1, o3), O3 = csub(o1, o3);
+    // twiddles W8^k on the odd half
+    cplx T1, T3;
+    if constexpr (INV) {
+        T1 = {R * (O1.re - O1.im), R * (O1.re + O1.im)};
+        T3 = {-R * (O3.re + O3.im), R * (O3.re - O3.im)};
+    } else {
+        T1 = {R * (O1.re + O1.im), R * (O1.im - O1.re)};
+        T3 = {R * (O3.im - O3.re), -R * (O3.re + O3.im)};
+    }
+    cplx T2 = mul_mi<INV>(O2);
+    x[0] = cadd(E0, O0); x[4] = csub(E0, O0);
+    x[1] = cadd(E1, T1); x[5] = csub(E1, T1);
+    x[2] = cadd(E2, T2); x[6] = csub(E2, T2);
+    x[3] = cadd(E3, T3); x[7] = csub(E3, T3);
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront-resident negacyclic FFT for N = 1024 (M = 512 complex points).
+//
+// Data distribution (lane l of a 64-wide wavefront, register slot s):
+//   coefficient domain : lane b, slot a (a < 8) holds folded point z[64a + b],
+//                        i.e. coefficients 64a+b (real) and 64a+b+512 (imag).
+//   frequency domain   : lane 8*k0 + e, slot f holds X[k0 + 8e + 64f].
+// Forward = radix-8 over slots, twiddle, LDS transpose, radix-8, twiddle, LDS
+// transpose, radix-8 (decimation in frequency, digit-reversed output order).
+// The inverse runs the exact reverse; the frequency order never leaves the
+// kernel family (Fourier keys are stored in the same order).
+//
+// Twiddle tables (global memory, [slot][lane] so every load is coalesced):
+//   tw_twist[a][b] = exp(i pi (64a+b) / N)        (negacyclic twist)
+//   tw_p1[k0][b]   = exp(-2 pi i b k0 / 512)
+//   tw_p2[e][l]    = exp(-2 pi i (l%8) e / 64)
+// LDS: one exchange buffer of 8*72 cplx (9216 B) per wavefront.
+// ---------------------------------------------------------------------------
+constexpr int FFT512_LDS_CPLX = 8 * 72;
+
+struct FFTTables {
+    const cplx* twist;  // [8][64]
+    const cplx* p1;     // [8][64]
+    const cplx* p2;     // [8][64]
+};
+
+__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); __builtin_amdgcn_s_waitcnt(0xc07f); }
+
+// forward: x in coefficient distribution (already twisted), out in frequency distribution
+__device__ __forceinline__ void fft512_fwd(cplx (&x)[8], cplx* lds, const FFTTables& T, int lane) {
+    dft8<false>(x);
+#pragma unroll
+    for (int k = 1; k < 8; ++k) x[k] = cmul(x[k], T.p1[k * 64 + lane]);
+    // exchange 1: lane b writes y[b][k0] at k0*64 + b
+#pragma unroll
+    for (int k = 0; k < 8; ++k) lds[k * 64 + lane] = x[k];
+    wave_sync();
+    const int k0 = lane >> 3, d = lane & 7;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) x[c] = lds[k0 * 64 + 8 * c + d];
+    wave_sync();
+    dft8<false>(x);
+#pragma unroll
+    for (int e = 1; e < 8; ++e) x[e] = cmul(x[e], T.p2[e * 64 + lane]);
+    // exchange 2: lane (k0,d) writes u[e] at k0*72 + 9e + d
+#pragma unroll
+    for (int e = 0; e < 8; ++e) lds[k0 * 72 + 9 * e + d] = x[e];
+    wave_sync();
+    const int e2 = lane & 7;
+#pragma unroll
+    for (int dd = 0; dd < 8; ++dd) x[dd] = lds[k0 * 72 + 9 * e2 + dd];
+    wave_sync();
+    dft8<false>(x);
+}
+
+// inverse (unscaled): x in frequency distribution, out in coefficient distribution
+__device__ __forceinline__ void fft512_inv(cplx (&x)[8], cplx* lds, const FFTTables& T, int lane) {
+    dft8<true>(x);
+    const int k0 = lane >> 3, e = lane & 7;
+#pragma unroll
+    for (int dd = 0; dd < 8; ++dd) lds[k0 * 72 + 9 * e + dd] = x[dd];
+    wave_sync();
+    const int d = lane & 7;
+#pragma unroll
+    for (int ee = 0; ee < 8; ++ee) x[ee] = lds[k0 * 72 + 9 * ee + d];
+    wave_sync();
+#pragma unroll
+    for (int ee = 1; ee < 8; ++ee) x[ee] = cmulc(x[ee], T.p2[ee * 64 + lane]);
+    dft8<true>(x);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) lds[k0 * 64 + 8 * c + d] = x[c];
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) x[k] = lds[k * 64 + lane];
+    wave_sync();
+#pragma unroll
+    for (int k = 1; k < 8; ++k) x[k] = cmulc(x[k], T.p1[k * 64 + lane]);
+    dft8<true>(x);
+}
+
+}  // namespace tfhe

@@ -1,0 +1,450 @@
+// gate.hip -- gfx950 kernels of the TFHE gate-bootstrapping path (N = 1024, torus-32):
+// key generation, BSK Fourier conversion, the persistent batched blind rotation,
+// the plain per-step form, sample extraction and the batched LWE key switch.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace tfhe {
+
+// ===========================================================================
+// Key generation (replaces the OpenFHE fork's KeySwitchGen and the reference's
+// GPUKeyGenAcc, bitwise/src/binfhe-base-scheme.cu:18-97, rgsw-acc.cu:13-122).
+// ===========================================================================
+
+// BSK row (i, r): RLWE_K(msg) with msg = s_i * g_j placed in component u = r & 1,
+// j = r >> 1, g_j = 2^(32 - l*logB + j*logB).  Coefficient layout [n][2l][2][N].
+__global__ __launch_bounds__(256) void k_gen_bsk32(uint64_t seed, int n, int N, int logB, int l,
+                                                   int nshift, const int32_t* __restrict__ s,
+                                                   const int32_t* __restrict__ K,
+                                                   uint32_t* __restrict__ bsk) {
+    extern __shared__ uint32_t sh[];
+    uint32_t* a = sh;             // [N]
+    int32_t* Ks = (int32_t*)(sh + N);  // [N]
+    const int rows = 2 * l;
+    const int row = blockIdx.x;   // i*rows + r
+    const int i = row / rows, r = row % rows;
+    const int j = r >> 1, u = r & 1;
+    for (int c = threadIdx.x; c < N; c += blockDim.x) {
+        a[c] = rand32(seed, DOM_BSK1, (uint32_t)row, (uint32_t)c);
+        Ks[c] = K[c];
+    }
+    __syncthreads();
+    const uint32_t g = 1u << (32 - l * logB + j * logB);
+    const uint32_t msg0 = (uint32_t)s[i] * g;
+    uint32_t* outa = bsk + (size_t)row * 2 * N;
+    uint32_t* outb = outa + N;
+    for (int c = threadIdx.x; c < N; c += blockDim.x) {
+        uint32_t acc = 0;
+        for (int t = 0; t < N; ++t) {
+            if (!Ks[t]) continue;
+            acc += (t <= c) ? a[c - t] : (0u - a[N + c - t]);
+        }
+        acc += (uint32_t)noise(seed, DOM_BSK1, (uint32_t)row, (uint32_t)c, nshift);
+        uint32_t av = a[c];
+        if (c == 0) {
+            if (u == 0) av += msg0; else acc += msg0;
+        }
+        outa[c] = av;
+        outb[c] = acc;
+    }
+}
+
+// KSK entry (i, j, v) = LWE_s(v * K_i * 2^(32 - l*logB + j*logB)), layout [N][l][B][n+1].
+__global__ __launch_bounds__(256) void k_gen_ksk32(uint64_t seed, int n, int N, int logB, int l,
+                                                   int nshift, const int32_t* __restrict__ s,
+                                                   const int32_t* __restrict__ K,
+                                                   uint32_t* __restrict__ ksk) {
+    __shared__ uint32_t red[256];
+    const int B = 1 << logB;
+    const uint32_t id = blockIdx.x;  // ((i*l + j)*B + v)
+    const int v = id % B;
+    const int j = (id / B) % l;
+    const int i = id / (B * l);
+    uint32_t* ct = ksk + (size_t)id * (n + 1);
+    uint32_t part = 0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        uint32_t ak = rand32(seed, DOM_KSK1, id, (uint32_t)k);
+        ct[k] = ak;
+        part += ak * (uint32_t)s[k];
+    }
+    red[threadIdx.x] = part;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int kbits = l * logB;
+        uint32_t mu = (uint32_t)v * (uint32_t)K[i] * (1u << (32 - kbits + j * logB));
+        ct[n] = red[0] + (uint32_t)noise(seed, DOM_KSK1, id, 0, nshift) + mu;
+    }
+}
+
+// ===========================================================================
+// Fourier conversion of N=1024 torus-32 polynomials into the kernel frequency
+// layout [poly][slot 8][lane 64] (one wavefront per polynomial).
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_to_fourier1024(const uint32_t* __restrict__ coef, size_t npoly,
+                                                       FFTTables T, cplx* __restrict__ out) {
+    __shared__ cplx lds_all[4][FFT512_LDS_CPLX];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const size_t q = (size_t)blockIdx.x * 4 + w;
+    if (q >= npoly) return;
+    cplx* lds = lds_all[w];
+    const uint32_t* p = coef + q * 1024;
+    cplx x[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        double re = (double)(int32_t)p[64 * a + lane];
+        double im = (double)(int32_t)p[64 * a + lane + 512];
+        x[a] = cmul({re, im}, T.twist[a * 64 + lane]);
+    }
+    fft512_fwd(x, lds, T, lane);
+    cplx* o = out + q * 512;
+#pragma unroll
+    for (int f = 0; f < 8; ++f) o[f * 64 + lane] = x[f];
+}
+
+// ===========================================================================
+// Gate preparation and modulus switch (reference binfhe-base-scheme.cu:151-176,
+// rgsw-acc.cu:597-598): prep = ct1 + ct2 (doubled for XOR/XNOR), then
+// round(v * 2N / 2^32) mod 2N on every coefficient.
+// ===========================================================================
+__device__ __forceinline__ uint32_t modswitch_2N(uint32_t v, int log2N2) {
+    return (uint32_t)((((uint64_t)v + (1ull << (31 - log2N2))) >> (32 - log2N2)) & ((1u << log2N2) - 1));
+}
+
+// Test vector value at coefficient c (reference binfhe-base-scheme.cu:103-138, q = 2N).
+__device__ __forceinline__ uint32_t gate_tv(uint32_t q1, uint32_t q2, uint32_t bt, uint32_t c, uint32_t q) {
+    const uint32_t Q2p = (1u << 29) + 1u, Q2pNeg = 0u - Q2p;
+    uint32_t temp = (bt + q - c) & (q - 1);
+    if (q1 < q2) return (temp >= q1 && temp < q2) ? Q2pNeg : Q2p;
+    return (temp >= q2 && temp < q1) ? Q2p : Q2pNeg;
+}
+
+// ===========================================================================
+// Persistent batched gate bootstrap: one wavefront owns one ciphertext for all
+// n blind-rotation steps.  Per step: signed decomposition of the register-
+// resident accumulator, 6 forward FFTs with the Fourier-domain MAC against the
+// streamed BSK row block, 2 inverse FFTs, exact rounding mod 2^32 and the CGGI
+// monic update acc += X^m t - t with m = -a~_i mod 2N (reference
+// rgsw-acc.cu:606-708 fused into one launch).  Ends with sample extraction
+// (coefficient 0) and the + Q/8 + 1 offset; the key switch runs separately.
+// ===========================================================================
+constexpr int GATE_N = 1024, GATE_LOGB = 7, GATE_L = 3, GATE_ROWS = 6;
+
+__global__ __launch_bounds__(256) void k_gate_blind_rotate(
+    const uint32_t* __restrict__ ct1, const uint32_t* __restrict__ ct2, uint32_t batch, int n,
+    int doubled, uint32_t q1, const cplx* __restrict__ bskf, FFTTables T,
+    uint32_t* __restrict__ lweN) {
+    constexpr int N = GATE_N, M = N / 2;
+    __shared__ cplx lds_all[4][FFT512_LDS_CPLX];
+    __shared__ uint16_t at_all[4][1024];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t cidx = blockIdx.x * 4 + w;
+    if (cidx >= batch) return;
+    cplx* lds = lds_all[w];
+    uint32_t* ldsu = reinterpret_cast<uint32_t*>(lds);
+    uint16_t* at = at_all[w];
+    const uint32_t* c1 = ct1 + (size_t)cidx * (n + 1);
+    const uint32_t* c2 = ct2 + (size_t)cidx * (n + 1);
+    constexpr int LOG2N2 = 11;
+    for (int k = lane; k < n; k += 64) {
+        uint32_t v = c1[k] + c2[k];
+        if (doubled) v *= 2u;
+        at[k] = (uint16_t)modswitch_2N(v, LOG2N2);
+    }
+    uint32_t vb = c1[n] + c2[n];
+    if (doubled) vb *= 2u;
+    const uint32_t bt = modswitch_2N(vb, LOG2N2);
+    const uint32_t q = 2u * N;
+    const uint32_t q2 = (q1 + N) & (q - 1);
+
+    uint32_t acc[2][16];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        acc[0][s] = 0u;
+        acc[1][s] = gate_tv(q1, q2, bt, (uint32_t)(64 * s + lane), q);
+    }
+    wave_sync();
+
+    for (int i = 0; i < n; ++i) {
+        const uint32_t mrot = (q - (uint32_t)at[i]) & (q - 1);
+        if (mrot == 0) continue;
+        const cplx* key = bskf + (size_t)i * GATE_ROWS * 2 * M;
+        cplx macc[2][8];
+#pragma unroll
+        for (int o = 0; o < 2; ++o)
+#pragma unroll
+            for (int f = 0; f < 8; ++f) macc[o][f] = {0.0, 0.0};
+
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            // full decomposition of this component's 16 coefficients, digits packed as bytes
+            uint32_t pk[GATE_L][4];
+#pragma unroll
+            for (int j = 0; j < GATE_L; ++j)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) pk[j][t] = 0u;
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                int32_t dg[GATE_L];
+                decompose32<GATE_LOGB, GATE_L>(acc[u][s], dg);
+#pragma unroll
+                for (int j = 0; j < GATE_L; ++j) pk[j][s >> 2] |= ((uint32_t)dg[j] & 0xFFu) << (8 * (s & 3));
+            }
+#pragma unroll
+            for (int j = 0; j < GATE_L; ++j) {
+                const int r = 2 * j + u;
+                cplx x[8];
+#pragma unroll
+                for (int a = 0; a < 8; ++a) {
+                    const int sr = a, si = a + 8;
+                    double re = (double)__builtin_amdgcn_sbfe(pk[j][sr >> 2], 8 * (sr & 3), 8);
+                    double im = (double)__builtin_amdgcn_sbfe(pk[j][si >> 2], 8 * (si & 3), 8);
+                    x[a] = cmul({re, im}, T.twist[a * 64 + lane]);
+                }
+                fft512_fwd(x, lds, T, lane);
+                const cplx* kr = key + (size_t)r * 2 * M;
+#pragma unroll
+                for (int o = 0; o < 2; ++o)
+#pragma unroll
+                    for (int f = 0; f < 8; ++f) cmac(macc[o][f], x[f], kr[(o * 8 + f) * 64 + lane]);
+            }
+        }
+
+        const int kshift = (int)(mrot & (N - 1));
+        const bool hi = mrot >= (uint32_t)N;
+#pragma unroll
+        for (int o = 0; o < 2; ++o) {
+            cplx x[8];
+#pragma unroll
+            for (int f = 0; f < 8; ++f) x[f] = macc[o][f];
+            fft512_inv(x, lds, T, lane);
+            uint32_t tv[16];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                cplx y = cmulc(x[a], T.twist[a * 64 + lane]);
+                tv[a] = (uint32_t)(int64_t)__builtin_rint(y.re * (1.0 / M));
+                tv[a + 8] = (uint32_t)(int64_t)__builtin_rint(y.im * (1.0 / M));
+            }
+#pragma unroll
+            for (int s = 0; s < 16; ++s) ldsu[64 * s + lane] = tv[s];
+            wave_sync();
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                int idx = 64 * s + lane - kshift;
+                const bool wrapped = idx < 0;
+                idx += wrapped ? N : 0;
+                uint32_t v = ldsu[idx];
+                if (hi ^ wrapped) v = 0u - v;
+                acc[o][s] += v - tv[s];
+            }
+            wave_sync();
+        }
+    }
+
+    // sample extraction at coefficient 0 (X -> X^{2N-1} automorphism, bitwise/src/kernel.cu:241-250)
+    uint32_t* out = lweN + (size_t)cidx * (N + 1);
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        const int c = 64 * s + lane;
+        const uint32_t v = acc[0][s];
+        out[(N - c) & (N - 1)] = c ? 0u - v : v;
+    }
+    if (lane == 0) out[N] = acc[1][0] + (1u << 29) + 1u;
+}
+
+// ===========================================================================
+// Plain per-step form (the reference's launch structure, rgsw-acc.cu:606-708):
+// per blind-rotation step a decompose+FFT launch (one wavefront per digit row),
+// a MAC+inverse-FFT launch (one wavefront per output polynomial) and a monic
+// update launch, for one ciphertext at a time.
+// ===========================================================================
+__global__ __launch_bounds__(64) void k_plain_decompose_fft(const uint32_t* __restrict__ acc, FFTTables T,
+                                                           cplx* __restrict__ digf) {
+    __shared__ cplx lds[FFT512_LDS_CPLX];
+    const int lane = threadIdx.x;
+    const int r = blockIdx.x, j = r >> 1, u = r & 1;
+    const uint32_t* src = acc + u * GATE_N;
+    cplx x[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        int32_t d0[GATE_L], d1[GATE_L];
+        decompose32<GATE_LOGB, GATE_L>(src[64 * a + lane], d0);
+        decompose32<GATE_LOGB, GATE_L>(src[64 * a + lane + 512], d1);
+        x[a] = cmul({(double)d0[j], (double)d1[j]}, T.twist[a * 64 + lane]);
+    }
+    fft512_fwd(x, lds, T, lane);
+#pragma unroll
+    for (int f = 0; f < 8; ++f) digf[(r * 8 + f) * 64 + lane] = x[f];
+}
+
+__global__ __launch_bounds__(64) void k_plain_mac_ifft(const cplx* __restrict__ digf, const cplx* __restrict__ key,
+                                                      FFTTables T, uint32_t* __restrict__ t) {
+    __shared__ cplx lds[FFT512_LDS_CPLX];
+    const int lane = threadIdx.x, o = blockIdx.x;
+    cplx x[8];
+#pragma unroll
+    for (int f = 0; f < 8; ++f) x[f] = {0.0, 0.0};
+    for (int r = 0; r < GATE_ROWS; ++r)
+#pragma unroll
+        for (int f = 0; f < 8; ++f)
+            cmac(x[f], digf[(r * 8 + f) * 64 + lane], key[((r * 2 + o) * 8 + f) * 64 + lane]);
+    fft512_inv(x, lds, T, lane);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        cplx y = cmulc(x[a], T.twist[a * 64 + lane]);
+        t[o * GATE_N + 64 * a + lane] = (uint32_t)(int64_t)__builtin_rint(y.re * (1.0 / 512));
+        t[o * GATE_N + 64 * a + lane + 512] = (uint32_t)(int64_t)__builtin_rint(y.im * (1.0 / 512));
+    }
+}
+
+// acc += X^m t - t (reference kernels_fft.cu:90-113)
+__global__ __launch_bounds__(256) void k_plain_monic(uint32_t* __restrict__ acc, const uint32_t* __restrict__ t,
+                                                    uint32_t m) {
+    const int o = blockIdx.x;
+    const int k = (int)(m & (GATE_N - 1));
+    const bool hi = m >= (uint32_t)GATE_N;
+    for (int c = threadIdx.x; c < GATE_N; c += blockDim.x) {
+        int idx = c - k;
+        const bool wrapped = idx < 0;
+        if (wrapped) idx += GATE_N;
+        uint32_t v = t[o * GATE_N + idx];
+        if (hi ^ wrapped) v = 0u - v;
+        acc[o * GATE_N + c] += v - t[o * GATE_N + c];
+    }
+}
+
+__global__ void k_plain_init(uint32_t* __restrict__ acc, uint32_t q1, uint32_t bt) {
+    const uint32_t q = 2u * GATE_N, q2 = (q1 + GATE_N) & (q - 1);
+    for (int c = threadIdx.x; c < GATE_N; c += blockDim.x) {
+        acc[c] = 0u;
+        acc[GATE_N + c] = gate_tv(q1, q2, bt, (uint32_t)c, q);
+    }
+}
+
+__global__ void k_plain_extract(const uint32_t* __restrict__ acc, uint32_t* __restrict__ lweN) {
+    for (int c = threadIdx.x; c < GATE_N; c += blockDim.x) {
+        const uint32_t v = acc[c];
+        lweN[(GATE_N - c) & (GATE_N - 1)] = c ? 0u - v : v;
+    }
+    if (threadIdx.x == 0) lweN[GATE_N] = acc[GATE_N] + (1u << 29) + 1u;
+}
+
+// ===========================================================================
+// Batched LWE key switch N -> n on a 32-bit torus (reference
+// bitwise/src/kernel.cu:259-279 with the host-side B loop of
+// binfhe-base-scheme.cu:225-246 moved onto the GPU):
+//   out = (0, b) - sum_{i<N} sum_{j<l} KSK[i][j][digit_j(a~_i)]
+// where a~_i keeps the top l*logB bits of a_i (rounded) and digits are
+// unsigned, least significant first.  A block owns 256 output columns of
+// KS_CT ciphertexts; each key row read is reused across the ciphertext tile.
+// ===========================================================================
+constexpr int KS_CT = 16, KS_ICHUNK = 128;
+
+template <int LOGB, int L>
+__global__ __launch_bounds__(256) void k_keyswitch32(const uint32_t* __restrict__ lweN, uint32_t batch, int N,
+                                                    int n, const uint32_t* __restrict__ ksk,
+                                                    uint32_t* __restrict__ out) {
+    constexpr int B = 1 << LOGB, KBITS = L * LOGB;
+    __shared__ uint32_t atil[KS_CT][KS_ICHUNK];
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t c0 = blockIdx.y * KS_CT;
+    const bool active = col <= n;
+    uint32_t acc[KS_CT];
+#pragma unroll
+    for (int c = 0; c < KS_CT; ++c) acc[c] = 0u;
+    const size_t rowstride = (size_t)(n + 1);
+    for (int i0 = 0; i0 < N; i0 += KS_ICHUNK) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < KS_CT * KS_ICHUNK; e += 256) {
+            const int c = e / KS_ICHUNK, ii = e % KS_ICHUNK;
+            uint32_t v = 0u;
+            if (c0 + c < batch) {
+                uint32_t a = lweN[(size_t)(c0 + c) * (N + 1) + i0 + ii];
+                v = (uint32_t)(((uint64_t)a + (1ull << (31 - KBITS))) >> (32 - KBITS));
+            }
+            atil[c][ii] = v;
+        }
+        __syncthreads();
+        if (!active) continue;
+        for (int ii = 0; ii < KS_ICHUNK; ++ii) {
+            const uint32_t* kr = ksk + (size_t)(i0 + ii) * L * B * rowstride + col;
+#pragma unroll
+            for (int j = 0; j < L; ++j) {
+                uint32_t r[B];
+#pragma unroll
+                for (int v = 0; v < B; ++v) r[v] = kr[(size_t)(j * B + v) * rowstride];
+#pragma unroll
+                for (int c = 0; c < KS_CT; ++c) {
+                    const uint32_t d = (atil[c][ii] >> (j * LOGB)) & (B - 1);
+                    uint32_t sel = r[0];
+#pragma unroll
+                    for (int v = 1; v < B; ++v) sel = (d == (uint32_t)v) ? r[v] : sel;
+                    acc[c] -= sel;
+                }
+            }
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int c = 0; c < KS_CT; ++c) {
+        if (c0 + c >= batch) break;
+        uint32_t v = acc[c];
+        if (col == n) v += lweN[(size_t)(c0 + c) * (N + 1) + N];
+        out[(size_t)(c0 + c) * (n + 1) + col] = v;
+    }
+}
+
+// ===========================================================================
+// Host launchers
+// ===========================================================================
+void launch_gen_bsk32(uint64_t seed, int n, int N, int logB, int l, int nshift, const int32_t* s,
+                      const int32_t* K, uint32_t* bsk, hipStream_t st) {
+    k_gen_bsk32<<<n * 2 * l, 256, 2 * N * sizeof(uint32_t), st>>>(seed, n, N, logB, l, nshift, s, K, bsk);
+}
+
+void launch_gen_ksk32(uint64_t seed, int n, int N, int logB, int l, int nshift, const int32_t* s,
+                      const int32_t* K, uint32_t* ksk, hipStream_t st) {
+    k_gen_ksk32<<<N * l * (1 << logB), 256, 0, st>>>(seed, n, N, logB, l, nshift, s, K, ksk);
+}
+
+void launch_to_fourier1024(const uint32_t* coef, size_t npoly, const FFTTables& T, cplx* out, hipStream_t st) {
+    k_to_fourier1024<<<(unsigned)((npoly + 3) / 4), 256, 0, st>>>(coef, npoly, T, out);
+}
+
+void launch_gate_blind_rotate(const uint32_t* ct1, const uint32_t* ct2, uint32_t batch, int n, int doubled,
+                              uint32_t q1, const cplx* bskf, const FFTTables& T, uint32_t* lweN,
+                              hipStream_t st) {
+    k_gate_blind_rotate<<<(batch + 3) / 4, 256, 0, st>>>(ct1, ct2, batch, n, doubled, q1, bskf, T, lweN);
+}
+
+void launch_plain_init(uint32_t* acc, uint32_t q1, uint32_t bt, hipStream_t st) {
+    k_plain_init<<<1, 256, 0, st>>>(acc, q1, bt);
+}
+
+void launch_plain_step(uint32_t* acc, cplx* digf, uint32_t* t, const cplx* key, uint32_t m,
+                       const FFTTables& T, hipStream_t st) {
+    k_plain_decompose_fft<<<GATE_ROWS, 64, 0, st>>>(acc, T, digf);
+    k_plain_mac_ifft<<<2, 64, 0, st>>>(digf, key, T, t);
+    k_plain_monic<<<2, 256, 0, st>>>(acc, t, m);
+}
+
+void launch_plain_extract(const uint32_t* acc, uint32_t* lweN, hipStream_t st) {
+    k_plain_extract<<<1, 256, 0, st>>>(acc, lweN);
+}
+
+int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int logB, int l,
+                       const uint32_t* ksk, uint32_t* out, hipStream_t st) {
+    dim3 grid((n + 1 + 255) / 256, (batch + KS_CT - 1) / KS_CT);
+    if (N % KS_ICHUNK) return -1;
+    if (logB == 2 && l == 8) {
+        k_keyswitch32<2, 8><<<grid, 256, 0, st>>>(lweN, batch, N, n, ksk, out);
+        return 0;
+    }
+    return -1;
+}
+
+}  // namespace tfhe

@@ -1,0 +1,26 @@
+// kernels.hpp -- host launchers of the gfx950 kernels (defined in the .hip files).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace tfhe {
+struct cplx;
+struct FFTTables;
+
+// gate level (N = 1024, torus-32)
+void launch_gen_bsk32(uint64_t seed, int n, int N, int logB, int l, int nshift, const int32_t* s,
+                      const int32_t* K, uint32_t* bsk, hipStream_t st);
+void launch_gen_ksk32(uint64_t seed, int n, int N, int logB, int l, int nshift, const int32_t* s,
+                      const int32_t* K, uint32_t* ksk, hipStream_t st);
+void launch_to_fourier1024(const uint32_t* coef, size_t npoly, const FFTTables& T, cplx* out, hipStream_t st);
+void launch_gate_blind_rotate(const uint32_t* ct1, const uint32_t* ct2, uint32_t batch, int n, int doubled,
+                              uint32_t q1, const cplx* bskf, const FFTTables& T, uint32_t* lweN,
+                              hipStream_t st);
+void launch_plain_init(uint32_t* acc, uint32_t q1, uint32_t bt, hipStream_t st);
+void launch_plain_step(uint32_t* acc, cplx* digf, uint32_t* t, const cplx* key, uint32_t m,
+                       const FFTTables& T, hipStream_t st);
+void launch_plain_extract(const uint32_t* acc, uint32_t* lweN, hipStream_t st);
+int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int logB, int l,
+                       const uint32_t* ksk, uint32_t* out, hipStream_t st);
+}  // namespace tfhe

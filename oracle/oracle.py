@@ -1,0 +1,224 @@
+"""ctypes bindings for the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product library.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+DOM_LWE_SK, DOM_RLWE1_SK, DOM_RLWE2_SK = 1, 2, 3
+DOM_BSK1, DOM_KSK1, DOM_BSK2, DOM_PFKS, DOM_KSK2, DOM_ENC = 4, 5, 6, 7, 8, 16
+
+# OpenFHE BINGATE numbering
+OR, AND, NOR, NAND, XOR_FAST, XNOR_FAST = 0, 1, 2, 3, 4, 5
+XOR, XNOR = 11, 12
+GATE_FUNCS = {
+    OR: lambda a, b: a | b, AND: lambda a, b: a & b, NOR: lambda a, b: 1 - (a | b),
+    NAND: lambda a, b: 1 - (a & b), XOR_FAST: lambda a, b: a ^ b, XNOR_FAST: lambda a, b: 1 - (a ^ b),
+    XOR: lambda a, b: a ^ b, XNOR: lambda a, b: 1 - (a ^ b),
+}
+
+
+class GateParams(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("N", ctypes.c_int), ("bk_logB", ctypes.c_int),
+                ("bk_l", ctypes.c_int), ("ks_logB", ctypes.c_int), ("ks_l", ctypes.c_int),
+                ("lwe_nshift", ctypes.c_int), ("rlwe_nshift", ctypes.c_int)]
+
+
+def gate_params(n=636, N=1024, bk_logB=7, bk_l=3, ks_logB=2, ks_l=8, lwe_nshift=15, rlwe_nshift=25):
+    """The TFHE gate parameter set: n=636, N=1024, BSK 2^7 x 3, KS 2^2 x 8,
+    LWE sigma 2^-15, RLWE sigma 2^-25 (torus-32)."""
+    return GateParams(n, N, bk_logB, bk_l, ks_logB, ks_l, lwe_nshift, rlwe_nshift)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.POINTER(GateParams)
+        sig = {
+            "orc_philox4x32_10": (None, [_u32p, _u32p, _u32p]),
+            "orc_rand32": (ctypes.c_uint32, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+            "orc_rand64": (ctypes.c_uint64, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+            "orc_noise": (ctypes.c_int64, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]),
+            "orc_nc_mul_exact32": (None, [_i32p, _u32p, _u32p, ctypes.c_int]),
+            "orc_nc_mul_exact64": (None, [_i32p, _u64p, _u64p, ctypes.c_int]),
+            "orc_nc_mul_fft": (ctypes.c_double, [_i64p, _i64p, _i64p, ctypes.c_int]),
+            "orc_decompose32": (None, [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, _i32p]),
+            "orc_decompose64": (None, [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _i32p]),
+            "orc_gen_lwe_sk": (None, [ctypes.c_uint64, ctypes.c_int, _i32p]),
+            "orc_gen_rlwe_sk": (None, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, _i32p]),
+            "orc_lwe_encrypt32": (None, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _i32p, ctypes.c_uint32, ctypes.c_int, _u32p]),
+            "orc_lwe_phase32": (ctypes.c_uint32, [ctypes.c_int, _i32p, _u32p]),
+            "orc_gen_bsk32": (None, [ctypes.c_uint64, P, _i32p, _i32p, _u32p]),
+            "orc_gen_ksk32": (None, [ctypes.c_uint64, P, _i32p, _i32p, _u32p]),
+            "orc_bsk_to_fourier": (None, [P, _u32p, _f64p]),
+            "orc_modswitch_2N": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_int]),
+            "orc_gate_test_vector": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_uint32, _u32p]),
+            "orc_external_product32": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, _u32p, _u32p]),
+            "orc_monic_update32": (None, [ctypes.c_int, ctypes.c_uint32, _u32p, _u32p]),
+            "orc_blind_rotate32": (ctypes.c_double, [P, _f64p, _u32p, _u32p]),
+            "orc_sample_extract32": (None, [ctypes.c_int, _u32p, ctypes.c_uint32, _u32p]),
+            "orc_keyswitch32": (None, [P, _u32p, _u32p, _u32p]),
+            "orc_gate_prep": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _u32p, _u32p, _u32p]),
+            "orc_gate_bootstrap": (ctypes.c_double, [P, _f64p, _u32p, ctypes.c_int, _u32p, _u32p, _u32p]),
+            "orc_aes_sbox": (ctypes.c_uint8, [ctypes.c_uint8]),
+            "orc_aes128_expand": (None, [_u8p, _u8p]),
+            "orc_aes128_encrypt": (None, [_u8p, _u8p, _u8p]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+# ---------------------------------------------------------------- helpers
+def philox(key, ctr):
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(np.asarray(key, np.uint32), np.asarray(ctr, np.uint32), out)
+    return out
+
+
+def decompose32(t, logB, l):
+    d = np.zeros(l, np.int32)
+    lib().orc_decompose32(int(t) & 0xFFFFFFFF, logB, l, d)
+    return d
+
+
+def decompose64(t, logB, l):
+    d = np.zeros(l, np.int32)
+    lib().orc_decompose64(int(t) & 0xFFFFFFFFFFFFFFFF, logB, l, d)
+    return d
+
+
+def nc_mul_exact32(x, y):
+    x = np.ascontiguousarray(x, np.int32)
+    y = np.ascontiguousarray(y, np.uint32)
+    out = np.zeros(len(y), np.uint32)
+    lib().orc_nc_mul_exact32(x, y, out, len(y))
+    return out
+
+
+def nc_mul_fft(x, y):
+    x = np.ascontiguousarray(x, np.int64)
+    y = np.ascontiguousarray(y, np.int64)
+    out = np.zeros(len(y), np.int64)
+    err = lib().orc_nc_mul_fft(x, y, out, len(y))
+    return out, err
+
+
+class GateKeys:
+    """Gate-level key material generated by the oracle from a seed."""
+
+    def __init__(self, seed, p=None, with_fourier=True):
+        self.p = p if p is not None else gate_params()
+        p = self.p
+        L = lib()
+        self.seed = seed
+        self.s = np.zeros(p.n, np.int32)
+        self.K = np.zeros(p.N, np.int32)
+        L.orc_gen_lwe_sk(seed, p.n, self.s)
+        L.orc_gen_rlwe_sk(seed, DOM_RLWE1_SK, p.N, self.K)
+        rows = 2 * p.bk_l
+        self.bsk = np.zeros(p.n * rows * 2 * p.N, np.uint32)
+        L.orc_gen_bsk32(seed, ctypes.byref(p), self.s, self.K, self.bsk)
+        B = 1 << p.ks_logB
+        self.ksk = np.zeros(p.N * p.ks_l * B * (p.n + 1), np.uint32)
+        L.orc_gen_ksk32(seed, ctypes.byref(p), self.s, self.K, self.ksk)
+        self.bsk_f = None
+        if with_fourier:
+            self.bsk_f = np.zeros(p.n * rows * 2 * p.N, np.float64)
+            L.orc_bsk_to_fourier(ctypes.byref(p), self.bsk, self.bsk_f)
+
+    def encrypt(self, bit, id_, dom=DOM_ENC, seed=None):
+        """LWE encryption of bit * 2^30 (OpenFHE p = 4 encoding)."""
+        p = self.p
+        ct = np.zeros(p.n + 1, np.uint32)
+        lib().orc_lwe_encrypt32(self.seed if seed is None else seed, dom, id_, p.n, self.s,
+                                (int(bit) << 30) & 0xFFFFFFFF, p.lwe_nshift, ct)
+        return ct
+
+    def phase(self, ct):
+        return int(lib().orc_lwe_phase32(self.p.n, self.s, np.ascontiguousarray(ct, np.uint32)))
+
+    def decrypt(self, ct):
+        """round(phase * 4 / 2^32) mod 4, the OpenFHE p = 4 decryption."""
+        ph = self.phase(ct)
+        return ((ph + (1 << 29)) >> 30) & 3
+
+    def bootstrap(self, gate, ct1, ct2):
+        p = self.p
+        out = np.zeros(p.n + 1, np.uint32)
+        err = lib().orc_gate_bootstrap(ctypes.byref(p), self.bsk_f, self.ksk, gate,
+                                       np.ascontiguousarray(ct1, np.uint32),
+                                       np.ascontiguousarray(ct2, np.uint32), out)
+        return out, err
+
+
+def aes_sbox(x):
+    return int(lib().orc_aes_sbox(x))
+
+
+def aes128_expand(key):
+    rk = np.zeros(176, np.uint8)
+    lib().orc_aes128_expand(np.frombuffer(bytes(key), np.uint8).copy(), rk)
+    return rk
+
+
+def aes128_encrypt(key, block):
+    rk = aes128_expand(key)
+    out = np.zeros(16, np.uint8)
+    lib().orc_aes128_encrypt(rk, np.frombuffer(bytes(block), np.uint8).copy(), out)
+    return bytes(out)
+
+
+def gate_blind_rotate_extract(keys, gate, ct1, ct2):
+    """Oracle blind rotation + sample extraction (+Q/8+1), i.e. the LWE under the
+    RLWE key before key switching; returns (lweN, max FFT rounding error)."""
+    L = lib()
+    p = keys.p
+    n, N = p.n, p.N
+    prep = np.zeros(n + 1, np.uint32)
+    if L.orc_gate_prep(gate, n, np.ascontiguousarray(ct1, np.uint32), np.ascontiguousarray(ct2, np.uint32), prep):
+        raise ValueError("unsupported gate")
+    at = np.array([L.orc_modswitch_2N(int(v), N) for v in prep[:n]], np.uint32)
+    bt = L.orc_modswitch_2N(int(prep[n]), N)
+    acc = np.zeros(2 * N, np.uint32)
+    tv = np.zeros(N, np.uint32)
+    L.orc_gate_test_vector(N, gate, bt, tv)
+    acc[N:] = tv
+    err = L.orc_blind_rotate32(ctypes.byref(p), keys.bsk_f, at, acc)
+    lwe = np.zeros(N + 1, np.uint32)
+    L.orc_sample_extract32(N, acc, 0, lwe)
+    lwe[N] = (int(lwe[N]) + (1 << 29) + 1) & 0xFFFFFFFF
+    return lwe, err
+
+
+def keyswitch32(keys, lweN):
+    out = np.zeros(keys.p.n + 1, np.uint32)
+    lib().orc_keyswitch32(ctypes.byref(keys.p), keys.ksk, np.ascontiguousarray(lweN, np.uint32), out)
+    return out

@@ -1,0 +1,96 @@
+/*
+ * tfhe_oracle.h -- CPU restatement of the reference's TFHE bootstrapping path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it.  Nothing in the shipped library links, calls or executes it.
+ *
+ * Every function cites the reference file:line whose behaviour it restates.
+ * The reference's CPU half (the OpenFHE fork) is absent from the reference tree,
+ * so the arithmetic domain is the TFHE torus (Z/2^32 for the gate path and the
+ * level-0 LWE, Z/2^64 for the N=2048 circuit-bootstrapping level); the
+ * reference's formulas (decomposition, CGGI accumulator update, sample
+ * extraction, key switching, CMux) are followed in that domain.
+ */
+#ifndef TFHE_ORACLE_H
+#define TFHE_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Philox4x32-10 (Salmon et al., SC'11) and the key-material sampler ---- */
+void orc_philox4x32_10(const uint32_t key[2], const uint32_t ctr[4], uint32_t out[4]);
+uint32_t orc_rand32(uint64_t seed, uint32_t dom, uint32_t id, uint32_t sub, uint32_t j);
+uint64_t orc_rand64(uint64_t seed, uint32_t dom, uint32_t id, uint32_t j);
+/* Irwin-Hall(12) integer noise: round(z * 2^-nshift) with z = sum of 12 u32 - 6*2^32. */
+int64_t orc_noise(uint64_t seed, uint32_t dom, uint32_t id, uint32_t j, int nshift);
+int32_t orc_secret_bit(uint64_t seed, uint32_t dom, uint32_t i);
+
+/* sampler domains: the same numbering is used by the product's key generator */
+enum {
+    ORC_DOM_LWE_SK = 1, ORC_DOM_RLWE1_SK = 2, ORC_DOM_RLWE2_SK = 3,
+    ORC_DOM_BSK1 = 4, ORC_DOM_KSK1 = 5, ORC_DOM_BSK2 = 6, ORC_DOM_PFKS = 7,
+    ORC_DOM_KSK2 = 8, ORC_DOM_ENC = 16
+};
+
+/* ---- parameter sets ---- */
+typedef struct {
+    int n;            /* LWE dimension */
+    int N;            /* ring dimension */
+    int bk_logB, bk_l;   /* blind-rotation gadget */
+    int ks_logB, ks_l;   /* key-switching gadget */
+    int lwe_nshift;      /* LWE noise: sigma = 2^-(32 - (32 - nshift)) -> see orc_noise */
+    int rlwe_nshift;     /* RLWE (BSK) noise shift */
+} orc_gate_params;
+
+/* ---- negacyclic products ---- */
+void orc_nc_mul_exact32(const int32_t* x, const uint32_t* y, uint32_t* out, int N);
+void orc_nc_mul_exact64(const int32_t* x, const uint64_t* y, uint64_t* out, int N);
+double orc_nc_mul_fft(const int64_t* x, const int64_t* y, int64_t* out, int N);
+void orc_fft_forward(const double* x /*[N]*/, double* re, double* im, int N);
+void orc_fft_inverse(const double* re, const double* im, double* x /*[N]*/, int N);
+
+/* ---- gadget decomposition ---- */
+void orc_decompose32(uint32_t t, int logB, int l, int32_t* digits);
+void orc_decompose64(uint64_t t, int logB, int l, int32_t* digits);
+
+/* ---- key generation (torus-32 gate level) ---- */
+void orc_gen_lwe_sk(uint64_t seed, int n, int32_t* s);
+void orc_gen_rlwe_sk(uint64_t seed, uint32_t dom, int N, int32_t* K);
+void orc_rlwe_encrypt32(uint64_t seed, uint32_t dom, uint32_t id, int N, const int32_t* K,
+                        const uint32_t* msg_a, const uint32_t* msg_b, int nshift, uint32_t* ct);
+void orc_lwe_encrypt32(uint64_t seed, uint32_t dom, uint32_t id, int n, const int32_t* s,
+                       uint32_t mu, int nshift, uint32_t* ct);
+uint32_t orc_lwe_phase32(int n, const int32_t* s, const uint32_t* ct);
+void orc_gen_bsk32(uint64_t seed, const orc_gate_params* p, const int32_t* s, const int32_t* K,
+                   uint32_t* bsk);
+void orc_gen_ksk32(uint64_t seed, const orc_gate_params* p, const int32_t* s, const int32_t* K,
+                   uint32_t* ksk);
+
+/* ---- gate bootstrap pieces ---- */
+void orc_bsk_to_fourier(const orc_gate_params* p, const uint32_t* bsk, double* bsk_f);
+uint32_t orc_modswitch_2N(uint32_t v, int N);
+void orc_gate_test_vector(int N, int gate, uint32_t b_tilde, uint32_t* tv);
+double orc_external_product32(int N, int logB, int l, const double* rgsw_f, const uint32_t* acc,
+                              uint32_t* out);
+void orc_monic_update32(int N, uint32_t m, const uint32_t* t, uint32_t* acc);
+double orc_blind_rotate32(const orc_gate_params* p, const double* bsk_f, const uint32_t* a_tilde,
+                          uint32_t* acc);
+void orc_sample_extract32(int N, const uint32_t* acc, uint32_t idx, uint32_t* lwe);
+void orc_keyswitch32(const orc_gate_params* p, const uint32_t* ksk, const uint32_t* lweN,
+                     uint32_t* out);
+int orc_gate_prep(int gate, int n, const uint32_t* ct1, const uint32_t* ct2, uint32_t* prep);
+double orc_gate_bootstrap(const orc_gate_params* p, const double* bsk_f, const uint32_t* ksk,
+                          int gate, const uint32_t* ct1, const uint32_t* ct2, uint32_t* out);
+
+/* ---- plain AES-128 (FIPS-197) ---- */
+uint8_t orc_aes_sbox(uint8_t x);
+void orc_aes128_expand(const uint8_t key[16], uint8_t rk[176]);
+void orc_aes128_encrypt(const uint8_t rk[176], const uint8_t in[16], uint8_t out[16]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
