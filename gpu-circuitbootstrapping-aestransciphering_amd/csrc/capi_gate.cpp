@@ -243,6 +243,30 @@ extern "C" int tfhe_gate_eval_plain(tfhe_gate_ctx* c, int gate, const uint32_t* 
     return tfhe_gate_keyswitch_batch(c, c->d_lweN, 1, d_out, stream);
 }
 
+extern "C" int tfhe_polymul1024_batch(const int32_t* d_x, const uint32_t* d_y, uint32_t batch, uint32_t* d_out,
+                                     void* stream) {
+    if (!d_x || !d_y || !d_out) return fail(TFHE_E_PARAM, "polymul: NULL argument");
+    if (batch == 0) return TFHE_OK;
+    static DeviceTables tables;  // process-lifetime twiddle tables for the stateless entry point
+    if (!tables.d_buf) {
+        int rc = tables.init(1024);
+        if (rc) return rc;
+    }
+    launch_polymul1024(d_x, d_y, batch, tables.T, d_out, (hipStream_t)stream);
+    return check_launch("polymul1024");
+}
+
+extern "C" int tfhe_gate_external_product(tfhe_gate_ctx* c, uint32_t key_index, uint32_t* d_acc, uint32_t* d_out,
+                                          void* stream) {
+    if (!c || !d_acc || !d_out) return fail(TFHE_E_PARAM, "external_product: NULL argument");
+    if (!c->has_keys) return fail(TFHE_E_STATE, "external_product: no bootstrapping key");
+    if (key_index >= (uint32_t)c->p.n) return fail(TFHE_E_PARAM, "external_product: key index out of range");
+    const size_t key_stride = (size_t)2 * c->p.bk_l * 2 * 512;
+    launch_external_product1024(d_acc, c->d_digf, d_out, c->d_bskf + key_index * key_stride, c->tables.T,
+                                (hipStream_t)stream);
+    return check_launch("external_product1024");
+}
+
 // ---------------------------------------------------------------------------
 // Host helpers
 // ---------------------------------------------------------------------------

@@ -164,7 +164,14 @@ struct FFTTables {
     const cplx* p2;     // [8][64]
 };
 
-__device__ __forceinline__ void wave_sync() { __builtin_amdgcn_wave_barrier(); __builtin_amdgcn_s_waitcnt(0xc07f); }
+// Orders LDS traffic between the lanes of one wavefront: the fences stop the
+// compiler from moving LDS accesses across the point, the wave barrier keeps the
+// lanes converged.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // forward: x in coefficient distribution (already twisted), out in frequency distribution
 __device__ __forceinline__ void fft512_fwd(cplx (&x)[8], cplx* lds, const FFTTables& T, int lane) {

@@ -201,8 +201,8 @@ __global__ __launch_bounds__(256) void k_gate_blind_rotate(
 #pragma unroll
                 for (int a = 0; a < 8; ++a) {
                     const int sr = a, si = a + 8;
-                    double re = (double)__builtin_amdgcn_sbfe(pk[j][sr >> 2], 8 * (sr & 3), 8);
-                    double im = (double)__builtin_amdgcn_sbfe(pk[j][si >> 2], 8 * (si & 3), 8);
+                    double re = (double)((int32_t)(pk[j][sr >> 2] << (24 - 8 * (sr & 3))) >> 24);
+                    double im = (double)((int32_t)(pk[j][si >> 2] << (24 - 8 * (si & 3))) >> 24);
                     x[a] = cmul({re, im}, T.twist[a * 64 + lane]);
                 }
                 fft512_fwd(x, lds, T, lane);
@@ -399,6 +399,40 @@ __global__ __launch_bounds__(256) void k_keyswitch32(const uint32_t* __restrict_
 }
 
 // ===========================================================================
+// Exact negacyclic product of small-integer polynomials x by torus-32
+// polynomials y through the wavefront FFT (one wavefront per product).
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_polymul1024(const int32_t* __restrict__ x, const uint32_t* __restrict__ y,
+                                                    uint32_t batch, FFTTables T, uint32_t* __restrict__ out) {
+    __shared__ cplx lds_all[4][FFT512_LDS_CPLX];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t q = blockIdx.x * 4 + w;
+    if (q >= batch) return;
+    cplx* lds = lds_all[w];
+    const int32_t* xp = x + (size_t)q * 1024;
+    const uint32_t* yp = y + (size_t)q * 1024;
+    cplx X[8], Y[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        X[a] = cmul({(double)xp[64 * a + lane], (double)xp[64 * a + lane + 512]}, T.twist[a * 64 + lane]);
+        Y[a] = cmul({(double)(int32_t)yp[64 * a + lane], (double)(int32_t)yp[64 * a + lane + 512]},
+                    T.twist[a * 64 + lane]);
+    }
+    fft512_fwd(X, lds, T, lane);
+    fft512_fwd(Y, lds, T, lane);
+#pragma unroll
+    for (int f = 0; f < 8; ++f) X[f] = cmul(X[f], Y[f]);
+    fft512_inv(X, lds, T, lane);
+    uint32_t* op = out + (size_t)q * 1024;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        cplx z = cmulc(X[a], T.twist[a * 64 + lane]);
+        op[64 * a + lane] = (uint32_t)(int64_t)__builtin_rint(z.re * (1.0 / 512));
+        op[64 * a + lane + 512] = (uint32_t)(int64_t)__builtin_rint(z.im * (1.0 / 512));
+    }
+}
+
+// ===========================================================================
 // Host launchers
 // ===========================================================================
 void launch_gen_bsk32(uint64_t seed, int n, int N, int logB, int l, int nshift, const int32_t* s,
@@ -434,6 +468,17 @@ void launch_plain_step(uint32_t* acc, cplx* digf, uint32_t* t, const cplx* key, 
 
 void launch_plain_extract(const uint32_t* acc, uint32_t* lweN, hipStream_t st) {
     k_plain_extract<<<1, 256, 0, st>>>(acc, lweN);
+}
+
+void launch_polymul1024(const int32_t* x, const uint32_t* y, uint32_t batch, const FFTTables& T, uint32_t* out,
+                        hipStream_t st) {
+    k_polymul1024<<<(batch + 3) / 4, 256, 0, st>>>(x, y, batch, T, out);
+}
+
+void launch_external_product1024(uint32_t* acc, cplx* digf, uint32_t* t, const cplx* key, const FFTTables& T,
+                                 hipStream_t st) {
+    k_plain_decompose_fft<<<GATE_ROWS, 64, 0, st>>>(acc, T, digf);
+    k_plain_mac_ifft<<<2, 64, 0, st>>>(digf, key, T, t);
 }
 
 int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int logB, int l,

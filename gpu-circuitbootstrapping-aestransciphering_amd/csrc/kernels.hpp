@@ -21,6 +21,10 @@ void launch_plain_init(uint32_t* acc, uint32_t q1, uint32_t bt, hipStream_t st);
 void launch_plain_step(uint32_t* acc, cplx* digf, uint32_t* t, const cplx* key, uint32_t m,
                        const FFTTables& T, hipStream_t st);
 void launch_plain_extract(const uint32_t* acc, uint32_t* lweN, hipStream_t st);
+void launch_polymul1024(const int32_t* x, const uint32_t* y, uint32_t batch, const FFTTables& T, uint32_t* out,
+                        hipStream_t st);
+void launch_external_product1024(uint32_t* acc, cplx* digf, uint32_t* t, const cplx* key, const FFTTables& T,
+                                 hipStream_t st);
 int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int logB, int l,
                        const uint32_t* ksk, uint32_t* out, hipStream_t st);
 }  // namespace tfhe

@@ -57,6 +57,8 @@ SIGNATURES = {
     "tfhe_gate_eval_plain": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]),
     "tfhe_gate_blind_rotate_batch": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_gate_keyswitch_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
+    "tfhe_polymul1024_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
+    "tfhe_gate_external_product": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp, _vp, _vp]),
     "tfhe_gen_bsk32": (ctypes.c_int, [ctypes.POINTER(GateParams), ctypes.c_uint64, _vp, _vp, _vp, _vp]),
     "tfhe_gen_ksk32": (ctypes.c_int, [ctypes.POINTER(GateParams), ctypes.c_uint64, _vp, _vp, _vp, _vp]),
     "tfhe_lwe_keygen": (None, [ctypes.c_int, ctypes.c_uint64, _i32p]),
@@ -92,6 +94,10 @@ def _ptr(t):
 
 def _stream(s):
     return ctypes.c_void_p(0 if s is None else s)
+
+
+def polymul1024_batch(d_x, d_y, d_out, stream=None):
+    check(lib().tfhe_polymul1024_batch(_ptr(d_x), _ptr(d_y), d_x.shape[0], _ptr(d_out), _stream(stream)))
 
 
 # ------------------------------------------------------------------ host helpers
@@ -154,6 +160,9 @@ class GateContext:
     def blind_rotate_batch(self, gate, d_ct1, d_ct2, d_lweN, stream=None):
         check(lib().tfhe_gate_blind_rotate_batch(self.h, gate, _ptr(d_ct1), _ptr(d_ct2), d_ct1.shape[0],
                                                  _ptr(d_lweN), _stream(stream)))
+
+    def external_product(self, key_index, d_acc, d_out, stream=None):
+        check(lib().tfhe_gate_external_product(self.h, key_index, _ptr(d_acc), _ptr(d_out), _stream(stream)))
 
     def keyswitch_batch(self, d_lweN, d_out, stream=None):
         check(lib().tfhe_gate_keyswitch_batch(self.h, _ptr(d_lweN), d_lweN.shape[0], _ptr(d_out), _stream(stream)))

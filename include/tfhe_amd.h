@@ -101,6 +101,18 @@ int tfhe_gate_blind_rotate_batch(tfhe_gate_ctx* ctx, int gate, const uint32_t* d
 int tfhe_gate_keyswitch_batch(tfhe_gate_ctx* ctx, const uint32_t* d_lweN, uint32_t batch, uint32_t* d_out,
                               void* stream);
 
+/* Exact negacyclic products through the wavefront FP64 FFT, N = 1024:
+ * d_out[k] = d_x[k] * d_y[k] mod (X^N + 1, 2^32) for small-integer d_x (|x| < 2^10)
+ * and torus-32 d_y, all [batch][1024].  The FFT of the reference
+ * (bitwise/src/fft.cu:3797-3875, cuFFTDxWrapper) in one self-contained call. */
+int tfhe_polymul1024_batch(const int32_t* d_x, const uint32_t* d_y, uint32_t batch, uint32_t* d_out, void* stream);
+
+/* One external product t = dec(acc) (*) BSK[key_index] in the reference's
+ * per-step form (kernel_SignedDigitDecompose + FFT + kernel_EvalAccCoreCGGI_fft +
+ * inverse, bitwise/src/rgsw-acc.cu:614-668): d_acc, d_out [2][N] uint32. */
+int tfhe_gate_external_product(tfhe_gate_ctx* ctx, uint32_t key_index, uint32_t* d_acc, uint32_t* d_out,
+                               void* stream);
+
 /* Stateless key generators (the GPU side of keygen parity):
  * d_bsk [n][2*bk_l][2][N], d_ksk [N][ks_l][2^ks_logB][n+1]. */
 int tfhe_gen_bsk32(const tfhe_gate_params* p, uint64_t seed, const int32_t* d_lwe_sk, const int32_t* d_rlwe_sk,

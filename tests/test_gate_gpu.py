@@ -47,6 +47,40 @@ def test_keygen_parity_with_oracle(gpu, okeys):
     assert np.array_equal(from_dev(d_ksk), okeys.ksk)
 
 
+def test_polymul1024_exact(gpu):
+    """Minimum slice: the wavefront FFT negacyclic product equals the schoolbook product."""
+    import torch
+    import tfhe_amd
+    rng = np.random.default_rng(17)
+    batch = 9
+    x = rng.integers(-64, 64, (batch, 1024)).astype(np.int32)
+    y = rng.integers(0, 2 ** 32, (batch, 1024), dtype=np.uint64).astype(np.uint32)
+    d_out = torch.empty((batch, 1024), dtype=torch.int32, device="cuda")
+    tfhe_amd.polymul1024_batch(to_dev(x), to_dev(y), d_out)
+    torch.cuda.synchronize()
+    got = from_dev(d_out)
+    for k in range(batch):
+        assert np.array_equal(got[k], O.nc_mul_exact32(x[k], y[k])), k
+
+
+def test_external_product_bit_exact(ctx, okeys):
+    import ctypes
+    import torch
+    p = okeys.p
+    N, M, rows = p.N, p.N // 2, 2 * p.bk_l
+    rng = np.random.default_rng(23)
+    acc = rng.integers(0, 2 ** 32, 2 * N, dtype=np.uint64).astype(np.uint32)
+    for i in (0, 5, p.n - 1):
+        d_out = torch.empty(2 * N, dtype=torch.int32, device="cuda")
+        ctx.external_product(i, to_dev(acc), d_out)
+        torch.cuda.synchronize()
+        key = np.ascontiguousarray(okeys.bsk_f.reshape(p.n, rows * 2 * 2 * M)[i])
+        want = np.zeros(2 * N, np.uint32)
+        err = O.lib().orc_external_product32(N, p.bk_logB, p.bk_l, key, acc, want)
+        assert err < 0.25
+        assert np.array_equal(from_dev(d_out), want), i
+
+
 @pytest.mark.parametrize("gate", [O.NAND, O.XOR, O.AND, O.OR, O.NOR, O.XNOR])
 def test_blind_rotate_extract_bit_exact(ctx, okeys, gate):
     """Blind rotation + sample extraction equals the oracle's LWE_N bit for bit."""
@@ -171,3 +205,4 @@ def test_error_paths(ctx, okeys):
         ctx.eval_batch(O.NAND, big, big.clone(), big.clone())
     assert e.value.code == tfhe_amd.TFHE_E_CAPACITY
     ctx.eval_batch(O.NAND, d, d.clone(), out, batch=0)  # empty batch is a no-op
+
