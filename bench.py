@@ -56,6 +56,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=3, help="plain per-bootstrap form samples")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run three 512 MiB device copies (the HBM-counter calibration kernel)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -142,6 +144,13 @@ def main():
             ctx.eval_plain(tfhe_amd.NAND, d1[k:k + 1], d2[k:k + 1], one_out, stream=sptr)
         torch.cuda.synchronize()
         plain_rate = args.plain_steps / (time.perf_counter() - tp)
+
+    if args.copy_calibration:
+        src = torch.empty(512 << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
 
     traffic = None
     prof = os.path.join(ROOT, "profiles", "gate_blind_rotate_c1_traffic.json")

@@ -23,30 +23,25 @@ struct DeviceTables {
     cplx* d_buf = nullptr;
     int init(int N) {
         if (N != 1024) return fail(TFHE_E_PARAM, "FFT tables: only N = 1024 is built");
-        std::vector<cplx> h(3 * 512);
+        std::vector<cplx> h(2 * 512);
         const double PI = 3.14159265358979323846264338327950288;
-        for (int a = 0; a < 8; ++a)
-            for (int b = 0; b < 64; ++b) {
-                const double ang = PI * (64.0 * a + b) / N;
-                h[a * 64 + b] = {std::cos(ang), std::sin(ang)};
-            }
         for (int k = 0; k < 8; ++k)
             for (int b = 0; b < 64; ++b) {
-                const double ang = -2.0 * PI * (double)((b * k) % 512) / 512.0;
-                h[512 + k * 64 + b] = {std::cos(ang), std::sin(ang)};
+                // w^b * exp(-2 pi i b k / 512), w = exp(i pi / N)
+                const double ang = PI * b / N - 2.0 * PI * (double)((b * k) % 512) / 512.0;
+                h[k * 64 + b] = {std::cos(ang), std::sin(ang)};
             }
         for (int e = 0; e < 8; ++e)
             for (int l = 0; l < 64; ++l) {
                 const double ang = -2.0 * PI * (double)(((l & 7) * e) % 64) / 64.0;
-                h[1024 + e * 64 + l] = {std::cos(ang), std::sin(ang)};
+                h[512 + e * 64 + l] = {std::cos(ang), std::sin(ang)};
             }
         int rc = dmalloc((void**)&d_buf, h.size() * sizeof(cplx));
         if (rc) return rc;
         rc = hipcheck(hipMemcpy(d_buf, h.data(), h.size() * sizeof(cplx), hipMemcpyHostToDevice), "tables");
         if (rc) return rc;
-        T.twist = d_buf;
-        T.p1 = d_buf + 512;
-        T.p2 = d_buf + 1024;
+        T.p1t = d_buf;
+        T.p2 = d_buf + 512;
         return TFHE_OK;
     }
     void release() {

@@ -142,49 +142,69 @@ __device__ __forceinline__ void dft8(cplx (&x)[8]) {
 // Wavefront-resident negacyclic FFT for N = 1024 (M = 512 complex points).
 //
 // Data distribution (lane l of a 64-wide wavefront, register slot s):
-//   coefficient domain : lane b, slot a (a < 8) holds folded point z[64a + b],
-//                        i.e. coefficients 64a+b (real) and 64a+b+512 (imag).
+//   coefficient domain : lane b, slot a (a < 8) holds the folded pair
+//                        (c[64a + b], c[64a + b + 512]) as (re, im).
 //   frequency domain   : lane 8*k0 + e, slot f holds X[k0 + 8e + 64f].
-// Forward = radix-8 over slots, twiddle, LDS transpose, radix-8, twiddle, LDS
-// transpose, radix-8 (decimation in frequency, digit-reversed output order).
-// The inverse runs the exact reverse; the frequency order never leaves the
-// kernel family (Fourier keys are stored in the same order).
+// Both LDS transposes use rows of 72 entries so that neighbouring 8-lane groups
+// land on different bank halves.
+// The negacyclic twist w^(64a+b), w = exp(i pi / N), splits into the
+// compile-time factor w^(64a) (applied per slot before the first radix-8) and
+// the per-lane factor w^b, which commutes with the in-register radix-8 and is
+// folded into the pass-1 twiddle table.  Forward = radix-8 over slots, twiddle,
+// LDS transpose, radix-8, twiddle, LDS transpose, radix-8 (decimation in
+// frequency, digit-reversed output).  The inverse runs the exact reverse and
+// returns 512 x the folded pairs.  The frequency order never leaves the kernel
+// family (Fourier keys are stored in the same order).
 //
 // Twiddle tables (global memory, [slot][lane] so every load is coalesced):
-//   tw_twist[a][b] = exp(i pi (64a+b) / N)        (negacyclic twist)
-//   tw_p1[k0][b]   = exp(-2 pi i b k0 / 512)
-//   tw_p2[e][l]    = exp(-2 pi i (l%8) e / 64)
+//   p1t[k0][b] = w^b * exp(-2 pi i b k0 / 512)
+//   p2[e][l]   = exp(-2 pi i (l%8) e / 64)
 // LDS: one exchange buffer of 8*72 cplx (9216 B) per wavefront.
 // ---------------------------------------------------------------------------
 constexpr int FFT512_LDS_CPLX = 8 * 72;
 
 struct FFTTables {
-    const cplx* twist;  // [8][64]
-    const cplx* p1;     // [8][64]
-    const cplx* p2;     // [8][64]
+    const cplx* p1t;  // [8][64]
+    const cplx* p2;   // [8][64]
 };
 
-// Orders LDS traffic between the lanes of one wavefront: the fences stop the
-// compiler from moving LDS accesses across the point, the wave barrier keeps the
-// lanes converged.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// w^(64a) = exp(i pi a / 16), a = 0..7
+__device__ __forceinline__ cplx twist_const(int a) {
+    constexpr double C[8][2] = {
+        {1.0, 0.0},
+        {0.98078528040323043, 0.19509032201612825},
+        {0.92387953251128674, 0.38268343236508978},
+        {0.83146961230254524, 0.55557023301960218},
+        {0.70710678118654757, 0.70710678118654746},
+        {0.55557023301960229, 0.83146961230254524},
+        {0.38268343236508984, 0.92387953251128674},
+        {0.19509032201612833, 0.98078528040323043}};
+    return {C[a][0], C[a][1]};
 }
 
-// forward: x in coefficient distribution (already twisted), out in frequency distribution
+// Orders LDS traffic between the lanes of one wavefront: the LDS-only fences
+// stop the compiler from moving LDS accesses across the point (global loads
+// stay free to be hoisted), the wave barrier keeps the lanes converged.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// forward: x = folded coefficient pairs (untwisted), out in frequency distribution
 __device__ __forceinline__ void fft512_fwd(cplx (&x)[8], cplx* lds, const FFTTables& T, int lane) {
+#pragma unroll
+    for (int a = 1; a < 8; ++a) x[a] = cmul(x[a], twist_const(a));
     dft8<false>(x);
 #pragma unroll
-    for (int k = 1; k < 8; ++k) x[k] = cmul(x[k], T.p1[k * 64 + lane]);
-    // exchange 1: lane b writes y[b][k0] at k0*64 + b
+    for (int k = 0; k < 8; ++k) x[k] = cmul(x[k], T.p1t[k * 64 + lane]);
+    // exchange 1: lane b writes y[b][k0] at k0*72 + b (72: bank-staggered rows)
 #pragma unroll
-    for (int k = 0; k < 8; ++k) lds[k * 64 + lane] = x[k];
+    for (int k = 0; k < 8; ++k) lds[k * 72 + lane] = x[k];
     wave_sync();
     const int k0 = lane >> 3, d = lane & 7;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) x[c] = lds[k0 * 64 + 8 * c + d];
+    for (int c = 0; c < 8; ++c) x[c] = lds[k0 * 72 + 8 * c + d];
     wave_sync();
     dft8<false>(x);
 #pragma unroll
@@ -200,7 +220,7 @@ __device__ __forceinline__ void fft512_fwd(cplx (&x)[8], cplx* lds, const FFTTab
     dft8<false>(x);
 }
 
-// inverse (unscaled): x in frequency distribution, out in coefficient distribution
+// inverse: x in frequency distribution, out = 512 x folded coefficient pairs
 __device__ __forceinline__ void fft512_inv(cplx (&x)[8], cplx* lds, const FFTTables& T, int lane) {
     dft8<true>(x);
     const int k0 = lane >> 3, e = lane & 7;
@@ -215,14 +235,40 @@ __device__ __forceinline__ void fft512_inv(cplx (&x)[8], cplx* lds, const FFTTab
     for (int ee = 1; ee < 8; ++ee) x[ee] = cmulc(x[ee], T.p2[ee * 64 + lane]);
     dft8<true>(x);
 #pragma unroll
-    for (int c = 0; c < 8; ++c) lds[k0 * 64 + 8 * c + d] = x[c];
+    for (int c = 0; c < 8; ++c) lds[k0 * 72 + 8 * c + d] = x[c];
     wave_sync();
 #pragma unroll
-    for (int k = 0; k < 8; ++k) x[k] = lds[k * 64 + lane];
+    for (int k = 0; k < 8; ++k) x[k] = lds[k * 72 + lane];
     wave_sync();
 #pragma unroll
-    for (int k = 1; k < 8; ++k) x[k] = cmulc(x[k], T.p1[k * 64 + lane]);
+    for (int k = 0; k < 8; ++k) x[k] = cmulc(x[k], T.p1t[k * 64 + lane]);
     dft8<true>(x);
+#pragma unroll
+    for (int a = 1; a < 8; ++a) x[a] = cmulc(x[a], twist_const(a));
+}
+
+// round(x * scale) mod 2^32 for |x * scale| < 2^51: adding 1.5 * 2^52 leaves the
+// rounded integer (ties to even, as rint) in the low mantissa bits.
+__device__ __forceinline__ uint32_t round_mod32(double x, double scale) {
+    const double y = fma(x, scale, 6755399441055744.0);
+    return (uint32_t)__double_as_longlong(y);
+}
+
+// Signed digits by the offset method: with off = 2^(IGN-1) + sum_j (B/2) 2^(IGN + j logB),
+// digit_j = ((t + off) >> (IGN + j logB) & (B-1)) - B/2.  This is the same balanced
+// representation as decompose32 (round half up, digits in [-B/2, B/2)).
+template <int LOGB, int L>
+__host__ __device__ constexpr uint32_t digit_offset32() {
+    constexpr int IGN = 32 - L * LOGB;
+    uint32_t off = IGN > 0 ? (1u << (IGN - 1)) : 0u;
+    for (int j = 0; j < L; ++j) off += (1u << (LOGB - 1)) << (IGN + j * LOGB);
+    return off;
+}
+
+template <int LOGB, int L>
+__device__ __forceinline__ int32_t digit32(uint32_t t_plus_off, int j) {
+    constexpr int IGN = 32 - L * LOGB;
+    return (int32_t)((t_plus_off >> (IGN + j * LOGB)) & ((1u << LOGB) - 1)) - (1 << (LOGB - 1));
 }
 
 }  // namespace tfhe

@@ -95,9 +95,7 @@ __global__ __launch_bounds__(256) void k_to_fourier1024(const uint32_t* __restri
     cplx x[8];
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-        double re = (double)(int32_t)p[64 * a + lane];
-        double im = (double)(int32_t)p[64 * a + lane + 512];
-        x[a] = cmul({re, im}, T.twist[a * 64 + lane]);
+        x[a] = {(double)(int32_t)p[64 * a + lane], (double)(int32_t)p[64 * a + lane + 512]};
     }
     fft512_fwd(x, lds, T, lane);
     cplx* o = out + q * 512;
@@ -135,14 +133,23 @@ constexpr int GATE_N = 1024, GATE_LOGB = 7, GATE_L = 3, GATE_ROWS = 6;
 
 __global__ __launch_bounds__(256) void k_gate_blind_rotate(
     const uint32_t* __restrict__ ct1, const uint32_t* __restrict__ ct2, uint32_t batch, int n,
-    int doubled, uint32_t q1, const cplx* __restrict__ bskf, FFTTables T,
+    int doubled, uint32_t q1, const cplx* __restrict__ bskf, FFTTables Tg,
     uint32_t* __restrict__ lweN) {
     constexpr int N = GATE_N, M = N / 2;
     __shared__ cplx lds_all[4][FFT512_LDS_CPLX];
     __shared__ uint16_t at_all[4][1024];
+    __shared__ cplx tw_lds[1024];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t cidx = blockIdx.x * 4 + w;
+    // twiddle tables staged in LDS: per-lane table reads through the vector memory
+    // path stall the FFT passes far more than LDS reads do
+    for (int k = threadIdx.x; k < 512; k += blockDim.x) {
+        tw_lds[k] = Tg.p1t[k];
+        tw_lds[512 + k] = Tg.p2[k];
+    }
+    __syncthreads();
+    const FFTTables T{tw_lds, tw_lds + 512};
     if (cidx >= batch) return;
     cplx* lds = lds_all[w];
     uint32_t* ldsu = reinterpret_cast<uint32_t*>(lds);
@@ -179,32 +186,20 @@ __global__ __launch_bounds__(256) void k_gate_blind_rotate(
 #pragma unroll
             for (int f = 0; f < 8; ++f) macc[o][f] = {0.0, 0.0};
 
+        constexpr uint32_t OFF = digit_offset32<GATE_LOGB, GATE_L>();
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-            // full decomposition of this component's 16 coefficients, digits packed as bytes
-            uint32_t pk[GATE_L][4];
+            uint32_t tp[16];
 #pragma unroll
-            for (int j = 0; j < GATE_L; ++j)
-#pragma unroll
-                for (int t = 0; t < 4; ++t) pk[j][t] = 0u;
-#pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                int32_t dg[GATE_L];
-                decompose32<GATE_LOGB, GATE_L>(acc[u][s], dg);
-#pragma unroll
-                for (int j = 0; j < GATE_L; ++j) pk[j][s >> 2] |= ((uint32_t)dg[j] & 0xFFu) << (8 * (s & 3));
-            }
+            for (int s = 0; s < 16; ++s) tp[s] = acc[u][s] + OFF;
 #pragma unroll
             for (int j = 0; j < GATE_L; ++j) {
                 const int r = 2 * j + u;
                 cplx x[8];
 #pragma unroll
-                for (int a = 0; a < 8; ++a) {
-                    const int sr = a, si = a + 8;
-                    double re = (double)((int32_t)(pk[j][sr >> 2] << (24 - 8 * (sr & 3))) >> 24);
-                    double im = (double)((int32_t)(pk[j][si >> 2] << (24 - 8 * (si & 3))) >> 24);
-                    x[a] = cmul({re, im}, T.twist[a * 64 + lane]);
-                }
+                for (int a = 0; a < 8; ++a)
+                    x[a] = {(double)digit32<GATE_LOGB, GATE_L>(tp[a], j),
+                            (double)digit32<GATE_LOGB, GATE_L>(tp[a + 8], j)};
                 fft512_fwd(x, lds, T, lane);
                 const cplx* kr = key + (size_t)r * 2 * M;
 #pragma unroll
@@ -225,9 +220,8 @@ __global__ __launch_bounds__(256) void k_gate_blind_rotate(
             uint32_t tv[16];
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
-                cplx y = cmulc(x[a], T.twist[a * 64 + lane]);
-                tv[a] = (uint32_t)(int64_t)__builtin_rint(y.re * (1.0 / M));
-                tv[a + 8] = (uint32_t)(int64_t)__builtin_rint(y.im * (1.0 / M));
+                tv[a] = round_mod32(x[a].re, 1.0 / M);
+                tv[a + 8] = round_mod32(x[a].im, 1.0 / M);
             }
 #pragma unroll
             for (int s = 0; s < 16; ++s) ldsu[64 * s + lane] = tv[s];
@@ -274,7 +268,7 @@ __global__ __launch_bounds__(64) void k_plain_decompose_fft(const uint32_t* __re
         int32_t d0[GATE_L], d1[GATE_L];
         decompose32<GATE_LOGB, GATE_L>(src[64 * a + lane], d0);
         decompose32<GATE_LOGB, GATE_L>(src[64 * a + lane + 512], d1);
-        x[a] = cmul({(double)d0[j], (double)d1[j]}, T.twist[a * 64 + lane]);
+        x[a] = {(double)d0[j], (double)d1[j]};
     }
     fft512_fwd(x, lds, T, lane);
 #pragma unroll
@@ -295,9 +289,8 @@ __global__ __launch_bounds__(64) void k_plain_mac_ifft(const cplx* __restrict__ 
     fft512_inv(x, lds, T, lane);
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-        cplx y = cmulc(x[a], T.twist[a * 64 + lane]);
-        t[o * GATE_N + 64 * a + lane] = (uint32_t)(int64_t)__builtin_rint(y.re * (1.0 / 512));
-        t[o * GATE_N + 64 * a + lane + 512] = (uint32_t)(int64_t)__builtin_rint(y.im * (1.0 / 512));
+        t[o * GATE_N + 64 * a + lane] = round_mod32(x[a].re, 1.0 / 512);
+        t[o * GATE_N + 64 * a + lane + 512] = round_mod32(x[a].im, 1.0 / 512);
     }
 }
 
@@ -339,18 +332,18 @@ __global__ void k_plain_extract(const uint32_t* __restrict__ acc, uint32_t* __re
 // binfhe-base-scheme.cu:225-246 moved onto the GPU):
 //   out = (0, b) - sum_{i<N} sum_{j<l} KSK[i][j][digit_j(a~_i)]
 // where a~_i keeps the top l*logB bits of a_i (rounded) and digits are
-// unsigned, least significant first.  A block owns 256 output columns of
+// unsigned, least significant first.  A block owns 128 output columns of
 // KS_CT ciphertexts; each key row read is reused across the ciphertext tile.
 // ===========================================================================
-constexpr int KS_CT = 16, KS_ICHUNK = 128;
+constexpr int KS_CT = 16, KS_ICHUNK = 128, KS_COLS = 128;
 
 template <int LOGB, int L>
-__global__ __launch_bounds__(256) void k_keyswitch32(const uint32_t* __restrict__ lweN, uint32_t batch, int N,
+__global__ __launch_bounds__(KS_COLS) void k_keyswitch32(const uint32_t* __restrict__ lweN, uint32_t batch, int N,
                                                     int n, const uint32_t* __restrict__ ksk,
                                                     uint32_t* __restrict__ out) {
     constexpr int B = 1 << LOGB, KBITS = L * LOGB;
     __shared__ uint32_t atil[KS_CT][KS_ICHUNK];
-    const int col = blockIdx.x * 256 + threadIdx.x;
+    const int col = blockIdx.x * KS_COLS + threadIdx.x;
     const uint32_t c0 = blockIdx.y * KS_CT;
     const bool active = col <= n;
     uint32_t acc[KS_CT];
@@ -359,7 +352,7 @@ __global__ __launch_bounds__(256) void k_keyswitch32(const uint32_t* __restrict_
     const size_t rowstride = (size_t)(n + 1);
     for (int i0 = 0; i0 < N; i0 += KS_ICHUNK) {
         __syncthreads();
-        for (int e = threadIdx.x; e < KS_CT * KS_ICHUNK; e += 256) {
+        for (int e = threadIdx.x; e < KS_CT * KS_ICHUNK; e += KS_COLS) {
             const int c = e / KS_ICHUNK, ii = e % KS_ICHUNK;
             uint32_t v = 0u;
             if (c0 + c < batch) {
@@ -414,9 +407,8 @@ __global__ __launch_bounds__(256) void k_polymul1024(const int32_t* __restrict__
     cplx X[8], Y[8];
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-        X[a] = cmul({(double)xp[64 * a + lane], (double)xp[64 * a + lane + 512]}, T.twist[a * 64 + lane]);
-        Y[a] = cmul({(double)(int32_t)yp[64 * a + lane], (double)(int32_t)yp[64 * a + lane + 512]},
-                    T.twist[a * 64 + lane]);
+        X[a] = {(double)xp[64 * a + lane], (double)xp[64 * a + lane + 512]};
+        Y[a] = {(double)(int32_t)yp[64 * a + lane], (double)(int32_t)yp[64 * a + lane + 512]};
     }
     fft512_fwd(X, lds, T, lane);
     fft512_fwd(Y, lds, T, lane);
@@ -426,9 +418,8 @@ __global__ __launch_bounds__(256) void k_polymul1024(const int32_t* __restrict__
     uint32_t* op = out + (size_t)q * 1024;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-        cplx z = cmulc(X[a], T.twist[a * 64 + lane]);
-        op[64 * a + lane] = (uint32_t)(int64_t)__builtin_rint(z.re * (1.0 / 512));
-        op[64 * a + lane + 512] = (uint32_t)(int64_t)__builtin_rint(z.im * (1.0 / 512));
+        op[64 * a + lane] = round_mod32(X[a].re, 1.0 / 512);
+        op[64 * a + lane + 512] = round_mod32(X[a].im, 1.0 / 512);
     }
 }
 
@@ -483,10 +474,10 @@ void launch_external_product1024(uint32_t* acc, cplx* digf, uint32_t* t, const c
 
 int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int logB, int l,
                        const uint32_t* ksk, uint32_t* out, hipStream_t st) {
-    dim3 grid((n + 1 + 255) / 256, (batch + KS_CT - 1) / KS_CT);
+    dim3 grid((n + 1 + KS_COLS - 1) / KS_COLS, (batch + KS_CT - 1) / KS_CT);
     if (N % KS_ICHUNK) return -1;
     if (logB == 2 && l == 8) {
-        k_keyswitch32<2, 8><<<grid, 256, 0, st>>>(lweN, batch, N, n, ksk, out);
+        k_keyswitch32<2, 8><<<grid, KS_COLS, 0, st>>>(lweN, batch, N, n, ksk, out);
         return 0;
     }
     return -1;
