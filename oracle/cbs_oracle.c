@@ -1,0 +1,380 @@
+/*
+ * cbs_oracle.c -- CPU restatement of the circuit-bootstrapping level (N = 2048,
+ * torus-64): multi-value programmable bootstrap, private functional key
+ * switching, RGSW assembly, CMux / vertical-packing LUT and the 64 -> 32 key
+ * switch back to the level-0 LWE key.
+ *
+ * TEST INFRASTRUCTURE ONLY (see tfhe_oracle.h).
+ *
+ * Exactness: every negacyclic product of a signed digit polynomial (|d| <= 2^8)
+ * with a torus-64 polynomial is computed as two FP64 FFT products, one per
+ * signed 32-bit limb of the torus polynomial, rounded to integers and
+ * recombined mod 2^64 (the limb scheme of the reference's split-FFT backend,
+ * bitwise/src/cirbts/cirbts-init.cu:1096-1120, cirbts-fft.cu:159-226,345-355,
+ * with 32-bit limbs of a 64-bit torus instead of limbs of a 54-bit prime).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "tfhe_oracle.h"
+
+/* ------------------------------------------------------------------------- */
+/* limb helpers                                                               */
+/* ------------------------------------------------------------------------- */
+/* v = hi * 2^32 + lo with lo, hi signed 32-bit */
+static inline void split64(uint64_t v, int32_t* lo, int32_t* hi) {
+    int32_t l = (int32_t)(uint32_t)v;
+    *lo = l;
+    *hi = (int32_t)(uint32_t)((v - (uint64_t)(int64_t)l) >> 32);
+}
+
+static inline uint64_t rnd64(double x, double* maxerr) {
+    double r = nearbyint(x);
+    double e = fabs(x - r);
+    if (e > *maxerr) *maxerr = e;
+    return (uint64_t)(int64_t)r;
+}
+
+/* Fourier form of one torus-64 polynomial: [limb][re M | im M] (4*M doubles). */
+void orc_to_fourier64(const uint64_t* poly, int N, double* out) {
+    int M = N / 2;
+    double* x0 = (double*)malloc(sizeof(double) * N);
+    double* x1 = (double*)malloc(sizeof(double) * N);
+    for (int c = 0; c < N; ++c) {
+        int32_t lo, hi;
+        split64(poly[c], &lo, &hi);
+        x0[c] = lo;
+        x1[c] = hi;
+    }
+    orc_fft_forward(x0, out, out + M, N);
+    orc_fft_forward(x1, out + 2 * M, out + 3 * M, N);
+    free(x0);
+    free(x1);
+}
+
+/* exact a*K mod 2^64 for binary K: two limb products through the FFT
+ * (|limb| * N/2 <= 2^41, far inside the FP64 exactness range) */
+static void mul_binary64(const int32_t* K, const uint64_t* a, uint64_t* out, int N) {
+    int64_t* x = (int64_t*)malloc(sizeof(int64_t) * N);
+    int64_t* l0 = (int64_t*)malloc(sizeof(int64_t) * N);
+    int64_t* l1 = (int64_t*)malloc(sizeof(int64_t) * N);
+    int64_t* p0 = (int64_t*)malloc(sizeof(int64_t) * N);
+    int64_t* p1 = (int64_t*)malloc(sizeof(int64_t) * N);
+    for (int c = 0; c < N; ++c) {
+        int32_t lo, hi;
+        split64(a[c], &lo, &hi);
+        l0[c] = lo;
+        l1[c] = hi;
+        x[c] = K[c];
+    }
+    orc_nc_mul_fft(x, l0, p0, N);
+    orc_nc_mul_fft(x, l1, p1, N);
+    for (int c = 0; c < N; ++c) out[c] = (uint64_t)p0[c] + ((uint64_t)p1[c] << 32);
+    free(x); free(l0); free(l1); free(p0); free(p1);
+}
+
+/* Fresh RGSW_K(m) with gadget 2^(64 - l logB + j logB), rows r = 2j + u ([2l][2][N]);
+ * ids id0 .. id0 + 2l - 1 of the sampler domain dom. */
+void orc_rgsw_encrypt64(uint64_t seed, uint32_t dom, uint32_t id0, int N, const int32_t* K, int m, int logB,
+                        int l, int nshift, uint64_t* rgsw) {
+    uint64_t* msg = (uint64_t*)calloc((size_t)N, sizeof(uint64_t));
+    for (int r = 0; r < 2 * l; ++r) {
+        int j = r >> 1, u = r & 1;
+        memset(msg, 0, sizeof(uint64_t) * (size_t)N);
+        msg[0] = (uint64_t)m << (64 - l * logB + j * logB);
+        orc_rlwe_encrypt64(seed, dom, id0 + (uint32_t)r, N, K, u == 0 ? msg : NULL, u == 1 ? msg : NULL, nshift,
+                           rgsw + (size_t)r * 2 * N);
+    }
+    free(msg);
+}
+
+/* ct = (a + msg_a, a*K + e + msg_b) over Z/2^64; msg pointers may be NULL. */
+void orc_rlwe_encrypt64(uint64_t seed, uint32_t dom, uint32_t id, int N, const int32_t* K,
+                        const uint64_t* msg_a, const uint64_t* msg_b, int nshift, uint64_t* ct) {
+    uint64_t* a = ct;
+    uint64_t* b = ct + N;
+    for (int j = 0; j < N; ++j) a[j] = orc_rand64(seed, dom, id, (uint32_t)j);
+    mul_binary64(K, a, b, N);
+    for (int j = 0; j < N; ++j) {
+        b[j] += (uint64_t)orc_noise(seed, dom, id, (uint32_t)j, nshift);
+        if (msg_b) b[j] += msg_b[j];
+        if (msg_a) a[j] += msg_a[j];
+    }
+}
+
+/* BSK2 [n][2l][2][N] u64: row r = 2j + u adds s_i * 2^(64 - l logB + j logB) to component u. */
+void orc_gen_bsk64(uint64_t seed, const orc_cbs_params* p, const int32_t* s, const int32_t* K2,
+                   uint64_t* bsk) {
+    int N = p->N, rows = 2 * p->bk_l;
+    uint64_t* msg = (uint64_t*)calloc((size_t)N, sizeof(uint64_t));
+    for (int i = 0; i < p->n; ++i)
+        for (int r = 0; r < rows; ++r) {
+            int j = r >> 1, u = r & 1;
+            uint64_t g = (uint64_t)1 << (64 - p->bk_l * p->bk_logB + j * p->bk_logB);
+            memset(msg, 0, sizeof(uint64_t) * (size_t)N);
+            msg[0] = (uint64_t)s[i] * g;
+            uint32_t id = (uint32_t)(i * rows + r);
+            orc_rlwe_encrypt64(seed, ORC_DOM_BSK2, id, N, K2, u == 0 ? msg : NULL, u == 1 ? msg : NULL,
+                               p->rlwe_nshift, bsk + (size_t)id * 2 * N);
+        }
+    free(msg);
+}
+
+/* Fourier BSK2: [n][2l][2 out][4M] (two limbs per polynomial). */
+void orc_bsk64_to_fourier(const orc_cbs_params* p, const uint64_t* bsk, double* bsk_f) {
+    int N = p->N, M = N / 2;
+    size_t npoly = (size_t)p->n * 2 * p->bk_l * 2;
+    for (size_t q = 0; q < npoly; ++q) orc_to_fourier64(bsk + q * N, N, bsk_f + q * 4 * M);
+}
+
+/* t = sum_r dec_r(acc) (*) C_r over Z/2^64, exact through two limb products.
+ * rgsw_f: [2l rows][2 out][4M]; digit poly r = 2j + u holds digit j of component u. */
+double orc_external_product64(int N, int logB, int l, const double* rgsw_f, const uint64_t* acc,
+                              uint64_t* out) {
+    int M = N / 2, rows = 2 * l;
+    double* x = (double*)malloc(sizeof(double) * N);
+    double* dr = (double*)malloc(sizeof(double) * M);
+    double* di = (double*)malloc(sizeof(double) * M);
+    double* sr = (double*)calloc((size_t)4 * M, sizeof(double)); /* [o][limb][M] */
+    double* si = (double*)calloc((size_t)4 * M, sizeof(double));
+    int32_t* dig = (int32_t*)malloc(sizeof(int32_t) * (size_t)rows * N);
+    int32_t tmp[64];
+    for (int u = 0; u < 2; ++u)
+        for (int c = 0; c < N; ++c) {
+            orc_decompose64(acc[u * N + c], logB, l, tmp);
+            for (int j = 0; j < l; ++j) dig[(size_t)(2 * j + u) * N + c] = tmp[j];
+        }
+    for (int r = 0; r < rows; ++r) {
+        for (int c = 0; c < N; ++c) x[c] = (double)dig[(size_t)r * N + c];
+        orc_fft_forward(x, dr, di, N);
+        for (int o = 0; o < 2; ++o)
+            for (int lb = 0; lb < 2; ++lb) {
+                const double* kr = rgsw_f + ((size_t)r * 2 + o) * 4 * M + (size_t)lb * 2 * M;
+                const double* ki = kr + M;
+                double* ar = sr + (size_t)(o * 2 + lb) * M;
+                double* ai = si + (size_t)(o * 2 + lb) * M;
+                for (int k = 0; k < M; ++k) {
+                    ar[k] += dr[k] * kr[k] - di[k] * ki[k];
+                    ai[k] += dr[k] * ki[k] + di[k] * kr[k];
+                }
+            }
+    }
+    double maxerr = 0.0;
+    double* y = (double*)malloc(sizeof(double) * N);
+    for (int o = 0; o < 2; ++o) {
+        orc_fft_inverse(sr + (size_t)(o * 2) * M, si + (size_t)(o * 2) * M, x, N);
+        orc_fft_inverse(sr + (size_t)(o * 2 + 1) * M, si + (size_t)(o * 2 + 1) * M, y, N);
+        for (int c = 0; c < N; ++c) {
+            uint64_t p0 = rnd64(x[c], &maxerr);
+            uint64_t p1 = rnd64(y[c], &maxerr);
+            out[o * N + c] = p0 + (p1 << 32);
+        }
+    }
+    free(x); free(y); free(dr); free(di); free(sr); free(si); free(dig);
+    return maxerr;
+}
+
+/* acc += X^m t - t on both components (reference kernels_fft.cu:90-113, over Z/2^64). */
+void orc_monic_update64(int N, uint32_t m, const uint64_t* t, uint64_t* acc) {
+    int k = (int)(m & (uint32_t)(N - 1));
+    int hi = m >= (uint32_t)N;
+    for (int o = 0; o < 2; ++o)
+        for (int c = 0; c < N; ++c) {
+            int idx = c - k;
+            int wrapped = idx < 0;
+            if (wrapped) idx += N;
+            uint64_t v = t[o * N + idx];
+            if (hi ^ wrapped) v = 0 - v;
+            acc[o * N + c] += v - t[o * N + c];
+        }
+}
+
+double orc_blind_rotate64(const orc_cbs_params* p, const double* bsk_f, const uint32_t* a_tilde,
+                          uint64_t* acc) {
+    int N = p->N, M = N / 2, rows = 2 * p->bk_l;
+    uint64_t* t = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N);
+    double maxerr = 0.0;
+    for (int i = 0; i < p->n; ++i) {
+        uint32_t m = (2u * (uint32_t)N - a_tilde[i]) & (2u * (uint32_t)N - 1u);
+        if (m == 0) continue;
+        const double* key = bsk_f + (size_t)i * rows * 2 * 4 * M;
+        double e = orc_external_product64(N, p->bk_logB, p->bk_l, key, acc, t);
+        if (e > maxerr) maxerr = e;
+        orc_monic_update64(N, m, t, acc);
+    }
+    free(t);
+    return maxerr;
+}
+
+/* LWE of coefficient idx over Z/2^64 (extract.cuh:22-31). */
+void orc_sample_extract64(int N, const uint64_t* acc, uint32_t idx, uint64_t* lwe) {
+    for (uint32_t j = 0; j < (uint32_t)N; ++j) lwe[j] = j <= idx ? acc[idx - j] : 0 - acc[(uint32_t)N + idx - j];
+    lwe[N] = acc[N + idx];
+}
+
+/* round(v * 2N / (2^32 * 2^bw)) * 2^bw mod 2N: the multi-value modulus switch
+ * (reference gpu_SpecilMS, bitwise/src/cirbts/cirbts-utils.cu:52-61, restated in
+ * exact integer arithmetic, ties up). */
+uint32_t orc_special_ms(uint32_t v, int N, int bw) {
+    int lg = 0;
+    while ((1 << lg) < 2 * N) ++lg;
+    int sh = 32 - lg + bw;
+    uint64_t r = ((uint64_t)v + ((uint64_t)1 << (sh - 1))) >> sh;
+    return (uint32_t)((r << bw) & (uint64_t)(2 * N - 1));
+}
+
+/* Circuit-bootstrapping PBS with both CB levels from one blind rotation
+ * (the reference's ManyLUT, cirbts-bootstrap.cu:753-839 + kernel_GenMVRLWEs):
+ * input LWE_s (torus-32) of b * 2^31; b += 2^30; test vector TV[c] = -mu_{c mod 2}
+ * with mu_j = 2^(64 - cb_l cb_logB + j cb_logB - 1); outputs out[j] = LWE_K2 of
+ * b * 2^(64 - cb_l cb_logB + j cb_logB) (coefficient j plus mu_j), j = 0, 1. */
+double orc_pbs_manylut(const orc_cbs_params* p, const double* bsk_f, const uint32_t* ct32, uint64_t* out) {
+    int n = p->n, N = p->N;
+    uint32_t* at = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    uint64_t* acc = (uint64_t*)calloc((size_t)2 * N, sizeof(uint64_t));
+    for (int i = 0; i < n; ++i) at[i] = orc_special_ms(ct32[i], N, 1);
+    uint32_t bt = orc_special_ms(ct32[n] + (1u << 30), N, 1);
+    uint64_t mu[2];
+    for (int j = 0; j < 2; ++j) mu[j] = (uint64_t)1 << (64 - p->cb_l * p->cb_logB + j * p->cb_logB - 1);
+    /* b-part: the rotation X^{-<a~,s>} of the blind rotation selects coefficient
+     * <a~,s> of the test vector, so b~ is folded in as in the gate test vector
+     * (binfhe-base-scheme.cu:120-131): TV[k] = F_{k mod 2}(b~ - (k - k mod 2))
+     * with F_j(x) = -mu_j for x mod 2N in [0, N) and +mu_j otherwise. */
+    for (int k = 0; k < N; ++k) {
+        uint32_t arg = (bt + 2u * (uint32_t)N - (uint32_t)(k & ~1)) & (2u * (uint32_t)N - 1u);
+        uint64_t v = 0 - mu[k & 1];
+        if (arg >= (uint32_t)N) v = 0 - v;
+        acc[N + k] = v;
+    }
+    double err = orc_blind_rotate64(p, bsk_f, at, acc);
+    for (int j = 0; j < 2; ++j) {
+        uint64_t* lwe = out + (size_t)j * (N + 1);
+        orc_sample_extract64(N, acc, (uint32_t)j, lwe);
+        lwe[N] += mu[j];
+    }
+    free(at);
+    free(acc);
+    return err;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Private functional key switching LWE_K2 (dim N) -> RLWE_K2 (CGGI 2020,     */
+/* PrivKS): out = - sum_{i<=N} sum_t KS[u][i][t][digit_t(a~_i)], with          */
+/* a_N = b and kappa_N = -1, a~_i = round(a_i / 2^(64 - pf_l pf_logB)), digits  */
+/* unsigned, least significant first, value 0 contributing nothing.           */
+/* KS[u][i][t][v] = RLWE_K2(f_u(v * kappa_i * 2^(64 - pf_l pf_logB + t pf_logB)))*/
+/* with f_0(x) = -K2(X) * x and f_1(x) = x.                                    */
+/* ------------------------------------------------------------------------- */
+uint32_t orc_pfks_id(const orc_cbs_params* p, int u, int i, int t, int v) {
+    int B = 1 << p->pf_logB;
+    return (uint32_t)((((size_t)u * (p->N + 1) + i) * p->pf_l + t) * B + v);
+}
+
+void orc_gen_pfks_entry(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, int u, int i, int t,
+                        int v, uint64_t* ct) {
+    int N = p->N;
+    uint64_t kappa = i < N ? (uint64_t)(int64_t)K2[i] : (uint64_t)(int64_t)-1;
+    uint64_t c = (uint64_t)v * kappa * ((uint64_t)1 << (64 - p->pf_l * p->pf_logB + t * p->pf_logB));
+    uint64_t* msg = (uint64_t*)calloc((size_t)N, sizeof(uint64_t));
+    if (u == 1) {
+        msg[0] = c;
+    } else {
+        for (int k = 0; k < N; ++k) msg[k] = 0 - (uint64_t)(int64_t)K2[k] * c;
+    }
+    orc_rlwe_encrypt64(seed, ORC_DOM_PFKS, orc_pfks_id(p, u, i, t, v), N, K2, NULL, msg, p->rlwe_nshift, ct);
+    free(msg);
+}
+
+void orc_pfks(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, int u, const uint64_t* lwe,
+              uint64_t* out) {
+    int N = p->N, B = 1 << p->pf_logB, kb = p->pf_l * p->pf_logB;
+    uint64_t* ct = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N);
+    memset(out, 0, sizeof(uint64_t) * 2 * N);
+    for (int i = 0; i <= N; ++i) {
+        uint64_t at = (lwe[i] + ((uint64_t)1 << (63 - kb))) >> (64 - kb);
+        for (int t = 0; t < p->pf_l; ++t) {
+            int v = (int)((at >> (t * p->pf_logB)) & (uint64_t)(B - 1));
+            if (!v) continue;
+            orc_gen_pfks_entry(seed, p, K2, u, i, t, v, ct);
+            for (int k = 0; k < 2 * N; ++k) out[k] -= ct[k];
+        }
+    }
+    free(ct);
+}
+
+/* RGSW row r = 2j + u = PFKS_u(PBS_j(ct)): [2 cb_l][2][N] u64 (the circuit bootstrap). */
+double orc_cbs(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, const double* bsk_f,
+               const uint32_t* ct32, uint64_t* rgsw) {
+    int N = p->N;
+    uint64_t* lwes = (uint64_t*)malloc(sizeof(uint64_t) * 2 * (N + 1));
+    double err = orc_pbs_manylut(p, bsk_f, ct32, lwes);
+    for (int j = 0; j < p->cb_l; ++j)
+        for (int u = 0; u < 2; ++u) orc_pfks(seed, p, K2, u, lwes + (size_t)j * (N + 1), rgsw + (size_t)(2 * j + u) * 2 * N);
+    free(lwes);
+    return err;
+}
+
+/* Fourier RGSW: [2 cb_l rows][2 out][4M]. */
+void orc_rgsw_to_fourier64(int N, int rows, const uint64_t* rgsw, double* rgsw_f) {
+    int M = N / 2;
+    for (int q = 0; q < rows * 2; ++q) orc_to_fourier64(rgsw + (size_t)q * N, N, rgsw_f + (size_t)q * 4 * M);
+}
+
+/* CMux(C, d0, d1) = d0 + C (*) (d1 - d0) (reference rlwe_tables.cuh:79-91). */
+double orc_cmux64(int N, int logB, int l, const double* rgsw_f, const uint64_t* d0, const uint64_t* d1,
+                  uint64_t* out) {
+    uint64_t* diff = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N);
+    uint64_t* t = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N);
+    for (int k = 0; k < 2 * N; ++k) diff[k] = d1[k] - d0[k];
+    double e = orc_external_product64(N, logB, l, rgsw_f, diff, t);
+    for (int k = 0; k < 2 * N; ++k) out[k] = d0[k] + t[k];
+    free(diff);
+    free(t);
+    return e;
+}
+
+/* Vertical-packing LUT: acc = (0, table_poly); for bit i = 0..7 (least significant
+ * first, as EvalLUT8x32_RLWE_CPU, rlwe_tables.cuh:93-108):
+ * acc = CMux(C_i, acc, X^{-stride 2^i} acc).  After the loop coefficient j holds
+ * table_poly[stride x + j].  rgsw_f: [8][2 cb_l][2][4M]. */
+double orc_vertical_packing(const orc_cbs_params* p, int stride, const double* rgsw_f, const uint64_t* table_poly,
+                            uint64_t* acc) {
+    int N = p->N, M = N / 2, rows = 2 * p->cb_l;
+    uint64_t* rot = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N);
+    double maxerr = 0.0;
+    memset(acc, 0, sizeof(uint64_t) * N);
+    memcpy(acc + N, table_poly, sizeof(uint64_t) * N);
+    for (int i = 0; i < 8; ++i) {
+        uint32_t sh = (uint32_t)(stride << i); /* rotate by X^{-sh} */
+        for (int o = 0; o < 2; ++o)
+            for (int c = 0; c < N; ++c) {
+                uint32_t src = (uint32_t)c + sh; /* (X^{-sh} a)_c = a_{c+sh} (negated past N) */
+                uint64_t v = acc[o * N + (src % (uint32_t)N)];
+                if ((src / (uint32_t)N) & 1) v = 0 - v;
+                rot[o * N + c] = v;
+            }
+        double e = orc_cmux64(N, p->cb_logB, p->cb_l, rgsw_f + (size_t)i * rows * 2 * 4 * M, acc, rot, acc);
+        if (e > maxerr) maxerr = e;
+    }
+    free(rot);
+    return maxerr;
+}
+
+/* KS2: LWE_K2 (torus-64, dim N) -> LWE_s (torus-32, dim n): modulus switch each
+ * word 2^64 -> 2^32 (round half up) and key switch with the torus-32 key
+ * KSK2 [N][ks_l][B][n+1] (the reference's Q -> qKS then KS order,
+ * extract.cuh:66-75, aes_gpu_ks.cuh:341-370). */
+void orc_ks2(const orc_cbs_params* p, const uint32_t* ksk2, const uint64_t* lwe64, uint32_t* out) {
+    int N = p->N;
+    uint32_t* l32 = (uint32_t*)malloc(sizeof(uint32_t) * (N + 1));
+    for (int i = 0; i <= N; ++i) l32[i] = (uint32_t)((lwe64[i] + ((uint64_t)1 << 31)) >> 32);
+    orc_gate_params g = {p->n, p->N, 0, 0, p->ks_logB, p->ks_l, p->lwe_nshift, 0};
+    orc_keyswitch32(&g, ksk2, l32, out);
+    free(l32);
+}
+
+void orc_gen_ksk2(uint64_t seed, const orc_cbs_params* p, const int32_t* s, const int32_t* K2, uint32_t* ksk) {
+    orc_gate_params g = {p->n, p->N, 0, 0, p->ks_logB, p->ks_l, p->lwe_nshift, 0};
+    orc_gen_ksk32_dom(seed, ORC_DOM_KSK2, &g, s, K2, ksk);
+}

@@ -222,3 +222,196 @@ def keyswitch32(keys, lweN):
     out = np.zeros(keys.p.n + 1, np.uint32)
     lib().orc_keyswitch32(ctypes.byref(keys.p), keys.ksk, np.ascontiguousarray(lweN, np.uint32), out)
     return out
+
+
+# ====================================================================== level 2
+class CbsParams(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("N", ctypes.c_int), ("bk_logB", ctypes.c_int), ("bk_l", ctypes.c_int),
+                ("cb_logB", ctypes.c_int), ("cb_l", ctypes.c_int), ("pf_logB", ctypes.c_int), ("pf_l", ctypes.c_int),
+                ("ks_logB", ctypes.c_int), ("ks_l", ctypes.c_int), ("lwe_nshift", ctypes.c_int),
+                ("rlwe_nshift", ctypes.c_int)]
+
+
+def cbs_params(n=636, N=2048):
+    """Circuit-bootstrapping level: N=2048 torus-64, PBS 2^9 x 4, CB 2^8 x 2, PFKS 2^4 x 8,
+    KS 2^2 x 8 to the level-0 key; level-2 sigma 2^-51, level-0 sigma 2^-15."""
+    return CbsParams(n, N, 9, 4, 8, 2, 4, 8, 2, 8, 15, 19)
+
+
+_cbs_sigs_done = False
+
+
+def _cbs_lib():
+    global _cbs_sigs_done
+    L = lib()
+    if not _cbs_sigs_done:
+        P = ctypes.POINTER(CbsParams)
+        G = ctypes.POINTER(GateParams)
+        sig = {
+            "orc_to_fourier64": (None, [_u64p, ctypes.c_int, _f64p]),
+            "orc_rlwe_encrypt64": (None, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _i32p,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _u64p]),
+            "orc_rgsw_encrypt64": (None, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _i32p,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _u64p]),
+            "orc_gen_bsk64": (None, [ctypes.c_uint64, P, _i32p, _i32p, _u64p]),
+            "orc_bsk64_to_fourier": (None, [P, _u64p, _f64p]),
+            "orc_external_product64": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, _u64p, _u64p]),
+            "orc_monic_update64": (None, [ctypes.c_int, ctypes.c_uint32, _u64p, _u64p]),
+            "orc_blind_rotate64": (ctypes.c_double, [P, _f64p, _u32p, _u64p]),
+            "orc_sample_extract64": (None, [ctypes.c_int, _u64p, ctypes.c_uint32, _u64p]),
+            "orc_special_ms": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_int, ctypes.c_int]),
+            "orc_pbs_manylut": (ctypes.c_double, [P, _f64p, _u32p, _u64p]),
+            "orc_pfks_id": (ctypes.c_uint32, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "orc_gen_pfks_entry": (None, [ctypes.c_uint64, P, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, _u64p]),
+            "orc_pfks": (None, [ctypes.c_uint64, P, _i32p, ctypes.c_int, _u64p, _u64p]),
+            "orc_cbs": (ctypes.c_double, [ctypes.c_uint64, P, _i32p, _f64p, _u32p, _u64p]),
+            "orc_rgsw_to_fourier64": (None, [ctypes.c_int, ctypes.c_int, _u64p, _f64p]),
+            "orc_cmux64": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, _u64p, _u64p, _u64p]),
+            "orc_vertical_packing": (ctypes.c_double, [P, ctypes.c_int, _f64p, _u64p, _u64p]),
+            "orc_ks2": (None, [P, _u32p, _u64p, _u32p]),
+            "orc_gen_ksk2": (None, [ctypes.c_uint64, P, _i32p, _i32p, _u32p]),
+            "orc_gen_ksk32_dom": (None, [ctypes.c_uint64, ctypes.c_uint32, G, _i32p, _i32p, _u32p]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _cbs_sigs_done = True
+    return L
+
+
+def nc_mul_exact64(x, y):
+    x = np.ascontiguousarray(x, np.int32)
+    y = np.ascontiguousarray(y, np.uint64)
+    out = np.zeros(len(y), np.uint64)
+    lib().orc_nc_mul_exact64(x, y, out, len(y))
+    return out
+
+
+def rlwe_phase64(K, ct):
+    """b - a*K over Z/2^64 for ct = [a | b]."""
+    N = len(K)
+    return (ct[N:] - nc_mul_exact64(K, ct[:N])).astype(np.uint64)
+
+
+def to_fourier64(poly):
+    N = len(poly)
+    out = np.zeros(2 * N, np.float64)
+    _cbs_lib().orc_to_fourier64(np.ascontiguousarray(poly, np.uint64), N, out)
+    return out
+
+
+def signed64(v):
+    """uint64 array -> signed int64 array (two's complement view) as Python-friendly ints."""
+    return np.asarray(v, np.uint64).view(np.int64)
+
+
+class CbsKeys:
+    """Circuit-bootstrapping key material from a seed (the PFKS key is never
+    materialised: its entries are regenerated on demand by orc_pfks)."""
+
+    def __init__(self, seed, p=None, s=None, with_bsk=True):
+        self.p = p if p is not None else cbs_params()
+        p = self.p
+        L = _cbs_lib()
+        self.seed = seed
+        self.s = np.zeros(p.n, np.int32)
+        if s is None:
+            L.orc_gen_lwe_sk(seed, p.n, self.s)
+        else:
+            self.s[:] = s
+        self.K2 = np.zeros(p.N, np.int32)
+        L.orc_gen_rlwe_sk(seed, DOM_RLWE2_SK, p.N, self.K2)
+        self.bsk = self.bsk_f = None
+        if with_bsk:
+            rows = 2 * p.bk_l
+            self.bsk = np.zeros(p.n * rows * 2 * p.N, np.uint64)
+            L.orc_gen_bsk64(seed, ctypes.byref(p), self.s, self.K2, self.bsk)
+            self.bsk_f = np.zeros(p.n * rows * 2 * 2 * p.N, np.float64)
+            L.orc_bsk64_to_fourier(ctypes.byref(p), self.bsk, self.bsk_f)
+        self._ksk2 = None
+
+    @property
+    def ksk2(self):
+        if self._ksk2 is None:
+            p = self.p
+            B = 1 << p.ks_logB
+            self._ksk2 = np.zeros(p.N * p.ks_l * B * (p.n + 1), np.uint32)
+            _cbs_lib().orc_gen_ksk2(self.seed, ctypes.byref(p), self.s, self.K2, self._ksk2)
+        return self._ksk2
+
+    def encrypt_bit(self, bit, id_, seed=None):
+        """Level-0 LWE of bit * 2^31 (p = 2)."""
+        p = self.p
+        ct = np.zeros(p.n + 1, np.uint32)
+        lib().orc_lwe_encrypt32(self.seed if seed is None else seed, DOM_ENC, id_, p.n, self.s,
+                                (int(bit) << 31) & 0xFFFFFFFF, p.lwe_nshift, ct)
+        return ct
+
+    def phase32(self, ct):
+        return int(lib().orc_lwe_phase32(self.p.n, self.s, np.ascontiguousarray(ct, np.uint32)))
+
+    def decrypt_bit(self, ct):
+        return ((self.phase32(ct) + (1 << 30)) >> 31) & 1
+
+    def lwe64_phase(self, lwe):
+        N = self.p.N
+        return (int(lwe[N]) - int(np.dot(np.asarray(lwe[:N], np.uint64), self.K2.astype(np.uint64)))) % 2 ** 64
+
+    def pbs(self, ct32):
+        p = self.p
+        out = np.zeros(2 * (p.N + 1), np.uint64)
+        err = _cbs_lib().orc_pbs_manylut(ctypes.byref(p), self.bsk_f, np.ascontiguousarray(ct32, np.uint32), out)
+        return out.reshape(2, p.N + 1), err
+
+    def pfks(self, u, lwe64):
+        p = self.p
+        out = np.zeros(2 * p.N, np.uint64)
+        _cbs_lib().orc_pfks(self.seed, ctypes.byref(p), self.K2, u, np.ascontiguousarray(lwe64, np.uint64), out)
+        return out
+
+    def pfks_entry(self, u, i, t, v):
+        p = self.p
+        out = np.zeros(2 * p.N, np.uint64)
+        _cbs_lib().orc_gen_pfks_entry(self.seed, ctypes.byref(p), self.K2, u, i, t, v, out)
+        return out
+
+    def cbs(self, ct32):
+        p = self.p
+        rgsw = np.zeros(2 * p.cb_l * 2 * p.N, np.uint64)
+        err = _cbs_lib().orc_cbs(self.seed, ctypes.byref(p), self.K2, self.bsk_f,
+                                 np.ascontiguousarray(ct32, np.uint32), rgsw)
+        return rgsw, err
+
+    def rgsw_encrypt(self, m, id0, logB=None, l=None):
+        p = self.p
+        logB = p.cb_logB if logB is None else logB
+        l = p.cb_l if l is None else l
+        rgsw = np.zeros(2 * l * 2 * p.N, np.uint64)
+        _cbs_lib().orc_rgsw_encrypt64(self.seed, DOM_ENC, id0, p.N, self.K2, int(m), logB, l, p.rlwe_nshift, rgsw)
+        return rgsw
+
+    def rgsw_fourier(self, rgsw):
+        p = self.p
+        rows = len(rgsw) // (2 * p.N)
+        out = np.zeros(rows * 2 * 2 * p.N, np.float64)
+        _cbs_lib().orc_rgsw_to_fourier64(p.N, rows, np.ascontiguousarray(rgsw, np.uint64), out)
+        return out
+
+    def vertical_packing(self, rgsw_fs, table_poly, stride=8):
+        p = self.p
+        acc = np.zeros(2 * p.N, np.uint64)
+        err = _cbs_lib().orc_vertical_packing(ctypes.byref(p), stride, np.ascontiguousarray(np.concatenate(rgsw_fs)),
+                                              np.ascontiguousarray(table_poly, np.uint64), acc)
+        return acc, err
+
+    def ks2(self, lwe64):
+        out = np.zeros(self.p.n + 1, np.uint32)
+        _cbs_lib().orc_ks2(ctypes.byref(self.p), self.ksk2, np.ascontiguousarray(lwe64, np.uint64), out)
+        return out
+
+    def extract64(self, acc, idx):
+        lwe = np.zeros(self.p.N + 1, np.uint64)
+        _cbs_lib().orc_sample_extract64(self.p.N, np.ascontiguousarray(acc, np.uint64), idx, lwe)
+        return lwe

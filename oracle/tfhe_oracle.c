@@ -107,7 +107,34 @@ void orc_nc_mul_exact64(const int32_t* x, const uint64_t* y, uint64_t* out, int 
  * DFT.  The inverse undoes the DFT, multiplies by conj(w^j) and unfolds the real and
  * imaginary parts back into coefficients j and j+N/2.
  */
+/* twiddle tables cached per transform size (M <= 4096) */
+typedef struct { int M; double* wr; double* wi; double* tc; double* ts; } fft_tab;
+static fft_tab g_tabs[8];
+
+static const fft_tab* get_tab(int M) {
+    for (int i = 0; i < 8; ++i)
+        if (g_tabs[i].M == M) return &g_tabs[i];
+    for (int i = 0; i < 8; ++i)
+        if (g_tabs[i].M == 0) {
+            fft_tab* t = &g_tabs[i];
+            t->wr = (double*)malloc(sizeof(double) * M);
+            t->wi = (double*)malloc(sizeof(double) * M);
+            t->tc = (double*)malloc(sizeof(double) * M);
+            t->ts = (double*)malloc(sizeof(double) * M);
+            for (int k = 0; k < M; ++k) {
+                t->wr[k] = cos(2.0 * M_PI * k / M);
+                t->wi[k] = sin(2.0 * M_PI * k / M);
+                t->tc[k] = cos(M_PI * k / (2.0 * M)); /* twist w^k, w = exp(i pi / N), N = 2M */
+                t->ts[k] = sin(M_PI * k / (2.0 * M));
+            }
+            t->M = M;
+            return t;
+        }
+    return NULL;
+}
+
 static void dft_inplace(double* re, double* im, int M, int inverse) {
+    const fft_tab* T = get_tab(M);
     /* iterative radix-2, bit-reversed input permutation */
     for (int i = 1, j = 0; i < M; ++i) {
         int bit = M >> 1;
@@ -118,11 +145,11 @@ static void dft_inplace(double* re, double* im, int M, int inverse) {
             t = im[i]; im[i] = im[j]; im[j] = t;
         }
     }
+    const double sgn = inverse ? 1.0 : -1.0;
     for (int len = 2; len <= M; len <<= 1) {
-        double ang = 2.0 * M_PI / len * (inverse ? 1.0 : -1.0);
-        int half = len >> 1;
+        int half = len >> 1, step = M / len;
         for (int k = 0; k < half; ++k) {
-            double wr = cos(ang * k), wi = sin(ang * k);
+            double wr = T->wr[k * step], wi = sgn * T->wi[k * step];
             for (int s = 0; s < M; s += len) {
                 int a = s + k, b = s + k + half;
                 double xr = re[b] * wr - im[b] * wi;
@@ -136,10 +163,10 @@ static void dft_inplace(double* re, double* im, int M, int inverse) {
 
 void orc_fft_forward(const double* x, double* re, double* im, int N) {
     int M = N / 2;
+    const fft_tab* T = get_tab(M);
     for (int j = 0; j < M; ++j) {
         double a = x[j], b = x[j + M];
-        double ang = M_PI * j / N;
-        double c = cos(ang), s = sin(ang);
+        double c = T->tc[j], s = T->ts[j];
         re[j] = a * c - b * s;
         im[j] = a * s + b * c;
     }
@@ -153,9 +180,9 @@ void orc_fft_inverse(const double* re_in, const double* im_in, double* x, int N)
     memcpy(re, re_in, sizeof(double) * M);
     memcpy(im, im_in, sizeof(double) * M);
     dft_inplace(re, im, M, 1);
+    const fft_tab* T = get_tab(M);
     for (int j = 0; j < M; ++j) {
-        double ang = -M_PI * j / N;
-        double c = cos(ang), s = sin(ang);
+        double c = T->tc[j], s = -T->ts[j];
         double r = (re[j] * c - im[j] * s) / M;
         double i = (re[j] * s + im[j] * c) / M;
         x[j] = r;
@@ -311,6 +338,11 @@ void orc_gen_bsk32(uint64_t seed, const orc_gate_params* p, const int32_t* s, co
 /* KSK layout [N][ks_l][B][n+1]: entry (i, j, v) = LWE_s(v * K_i * 2^(32 - ks_l*ks_logB + j*ks_logB)). */
 void orc_gen_ksk32(uint64_t seed, const orc_gate_params* p, const int32_t* s, const int32_t* K,
                    uint32_t* ksk) {
+    orc_gen_ksk32_dom(seed, ORC_DOM_KSK1, p, s, K, ksk);
+}
+
+void orc_gen_ksk32_dom(uint64_t seed, uint32_t dom, const orc_gate_params* p, const int32_t* s,
+                       const int32_t* K, uint32_t* ksk) {
     int B = 1 << p->ks_logB;
     int kbits = p->ks_l * p->ks_logB;
     for (int i = 0; i < p->N; ++i)
@@ -318,7 +350,7 @@ void orc_gen_ksk32(uint64_t seed, const orc_gate_params* p, const int32_t* s, co
             for (int v = 0; v < B; ++v) {
                 uint32_t id = (uint32_t)(((size_t)i * p->ks_l + j) * B + v);
                 uint32_t mu = (uint32_t)v * (uint32_t)K[i] * ((uint32_t)1 << (32 - kbits + j * p->ks_logB));
-                orc_lwe_encrypt32(seed, ORC_DOM_KSK1, id, p->n, s, mu, p->lwe_nshift,
+                orc_lwe_encrypt32(seed, dom, id, p->n, s, mu, p->lwe_nshift,
                                   ksk + (size_t)id * (p->n + 1));
             }
 }

@@ -85,6 +85,47 @@ int orc_gate_prep(int gate, int n, const uint32_t* ct1, const uint32_t* ct2, uin
 double orc_gate_bootstrap(const orc_gate_params* p, const double* bsk_f, const uint32_t* ksk,
                           int gate, const uint32_t* ct1, const uint32_t* ct2, uint32_t* out);
 
+void orc_gen_ksk32_dom(uint64_t seed, uint32_t dom, const orc_gate_params* p, const int32_t* s,
+                       const int32_t* K, uint32_t* ksk);
+
+/* ---- circuit-bootstrapping level (N = 2048, torus-64), cbs_oracle.c ---- */
+typedef struct {
+    int n, N;             /* 636, 2048 */
+    int bk_logB, bk_l;    /* PBS gadget 2^9 x 4 */
+    int cb_logB, cb_l;    /* circuit-bootstrapping level gadget 2^8 x 2 */
+    int pf_logB, pf_l;    /* private functional KS 2^4 x 8 */
+    int ks_logB, ks_l;    /* N -> n key switch 2^2 x 8 (torus-32) */
+    int lwe_nshift;       /* level-0 LWE noise (torus-32): 15 */
+    int rlwe_nshift;      /* level-2 noise (torus-64): 19, i.e. sigma = 2^-51 */
+} orc_cbs_params;
+
+void orc_to_fourier64(const uint64_t* poly, int N, double* out);
+void orc_rlwe_encrypt64(uint64_t seed, uint32_t dom, uint32_t id, int N, const int32_t* K,
+                        const uint64_t* msg_a, const uint64_t* msg_b, int nshift, uint64_t* ct);
+void orc_rgsw_encrypt64(uint64_t seed, uint32_t dom, uint32_t id0, int N, const int32_t* K, int m, int logB,
+                        int l, int nshift, uint64_t* rgsw);
+void orc_gen_bsk64(uint64_t seed, const orc_cbs_params* p, const int32_t* s, const int32_t* K2, uint64_t* bsk);
+void orc_bsk64_to_fourier(const orc_cbs_params* p, const uint64_t* bsk, double* bsk_f);
+double orc_external_product64(int N, int logB, int l, const double* rgsw_f, const uint64_t* acc, uint64_t* out);
+void orc_monic_update64(int N, uint32_t m, const uint64_t* t, uint64_t* acc);
+double orc_blind_rotate64(const orc_cbs_params* p, const double* bsk_f, const uint32_t* a_tilde, uint64_t* acc);
+void orc_sample_extract64(int N, const uint64_t* acc, uint32_t idx, uint64_t* lwe);
+uint32_t orc_special_ms(uint32_t v, int N, int bw);
+double orc_pbs_manylut(const orc_cbs_params* p, const double* bsk_f, const uint32_t* ct32, uint64_t* out);
+uint32_t orc_pfks_id(const orc_cbs_params* p, int u, int i, int t, int v);
+void orc_gen_pfks_entry(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, int u, int i, int t, int v,
+                        uint64_t* ct);
+void orc_pfks(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, int u, const uint64_t* lwe, uint64_t* out);
+double orc_cbs(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, const double* bsk_f,
+               const uint32_t* ct32, uint64_t* rgsw);
+void orc_rgsw_to_fourier64(int N, int rows, const uint64_t* rgsw, double* rgsw_f);
+double orc_cmux64(int N, int logB, int l, const double* rgsw_f, const uint64_t* d0, const uint64_t* d1,
+                  uint64_t* out);
+double orc_vertical_packing(const orc_cbs_params* p, int stride, const double* rgsw_f, const uint64_t* table_poly,
+                            uint64_t* acc);
+void orc_ks2(const orc_cbs_params* p, const uint32_t* ksk2, const uint64_t* lwe64, uint32_t* out);
+void orc_gen_ksk2(uint64_t seed, const orc_cbs_params* p, const int32_t* s, const int32_t* K2, uint32_t* ksk);
+
 /* ---- plain AES-128 (FIPS-197) ---- */
 uint8_t orc_aes_sbox(uint8_t x);
 void orc_aes128_expand(const uint8_t key[16], uint8_t rk[176]);

@@ -1,0 +1,173 @@
+"""Pins of the CPU oracle's circuit-bootstrapping level (N = 2048, torus-64):
+exact limb products against schoolbook, the multi-value PBS, private functional
+key switching, CMux / vertical packing and the 64 -> 32 key switch, each
+checked by decryption under the secret keys (no GPU needed)."""
+import random
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+@pytest.fixture(scope="module")
+def keys():
+    return O.CbsKeys(2)
+
+
+def centered64(v):
+    return ((int(v) + 2 ** 63) % 2 ** 64) - 2 ** 63
+
+
+def test_external_product64_exact_against_schoolbook():
+    """sum_r dec_r(acc) * C_r computed with two FP64 limb products equals the
+    exact integer result mod 2^64 (random torus-64 RGSW rows, N=2048, 2^8 x 2)."""
+    L = O._cbs_lib()
+    N, logB, l = 2048, 8, 2
+    rng = np.random.default_rng(31)
+    rows = 2 * l
+    rgsw = rng.integers(0, 2 ** 64, rows * 2 * N, dtype=np.uint64)
+    acc = rng.integers(0, 2 ** 64, 2 * N, dtype=np.uint64)
+    rgsw_f = np.zeros(rows * 2 * 2 * N, np.float64)
+    L.orc_rgsw_to_fourier64(N, rows, rgsw, rgsw_f)
+    out = np.zeros(2 * N, np.uint64)
+    err = L.orc_external_product64(N, logB, l, rgsw_f, acc, out)
+    assert err < 0.25
+    want = np.zeros(2 * N, np.uint64)
+    for u in range(2):
+        digs = np.array([O.decompose64(int(t), logB, l) for t in acc[u * N:(u + 1) * N]])  # [N][l]
+        for j in range(l):
+            r = 2 * j + u
+            for o in range(2):
+                want[o * N:(o + 1) * N] += O.nc_mul_exact64(digs[:, j], rgsw[(r * 2 + o) * N:(r * 2 + o + 1) * N])
+    assert np.array_equal(out, want)
+
+
+def test_special_ms_closed_form():
+    L = O._cbs_lib()
+    rng = random.Random(8)
+    for v in [0, 2 ** 32 - 1, 1 << 20, (1 << 20) - 1, 3 << 20] + [rng.getrandbits(32) for _ in range(3000)]:
+        for N, bw in ((2048, 1), (2048, 0), (1024, 1)):
+            lg = (2 * N).bit_length() - 1
+            want = (((v * 2 * N * 2 + (2 ** 32 << bw)) // (2 * 2 ** 32 << bw)) << bw) % (2 * N)
+            assert L.orc_special_ms(v, N, bw) == want, (v, N, bw)
+
+
+def test_rlwe64_encryption_and_bsk_rows(keys):
+    p = keys.p
+    rows = 2 * p.bk_l
+    bsk = keys.bsk.reshape(p.n, rows, 2, p.N)
+    for i in (0, p.n - 1):
+        for r in range(rows):
+            j, u = r >> 1, r & 1
+            g = 1 << (64 - p.bk_l * p.bk_logB + j * p.bk_logB)
+            ph = O.rlwe_phase64(keys.K2, bsk[i, r].reshape(-1))
+            if u == 1:
+                want = np.zeros(p.N, np.uint64)
+                want[0] = (int(keys.s[i]) * g) % 2 ** 64
+            else:
+                m = np.zeros(p.N, np.uint64)
+                m[0] = (int(keys.s[i]) * g) % 2 ** 64
+                want = (-O.nc_mul_exact64(keys.K2, m)).astype(np.uint64)
+            err = O.signed64((ph - want).astype(np.uint64))
+            assert np.abs(err).max() < 7 * 2 ** 13
+
+
+@pytest.mark.parametrize("bit", [0, 1])
+def test_pbs_manylut_levels(keys, bit):
+    """One blind rotation yields both circuit-bootstrapping levels:
+    LWE_K2(bit * 2^48) and LWE_K2(bit * 2^56)."""
+    for rep in range(2):
+        ct = keys.encrypt_bit(bit, 100 + 2 * rep + bit)
+        out, err = keys.pbs(ct)
+        assert err < 0.25
+        for j in range(2):
+            ph = keys.lwe64_phase(out[j])
+            e = centered64(ph - bit * 2 ** (48 + 8 * j))
+            assert abs(e) < 2 ** 40  # noise ~2^-28 of the torus, bound 2^-24
+
+
+def test_pfks_entries_and_functions(keys):
+    """Key entries decrypt to f_u(v kappa_i g_t); the switched RLWE decrypts to f_u(phase)."""
+    p = keys.p
+    N = p.N
+    for (u, i, t, v) in [(1, 0, 0, 1), (1, N, 7, 15), (0, 5, 3, 9), (0, N, 0, 2)]:
+        ct = keys.pfks_entry(u, i, t, v)
+        kappa = int(keys.K2[i]) if i < N else -1
+        c = (v * kappa * 2 ** (32 + 4 * t)) % 2 ** 64
+        if u == 1:
+            want = np.zeros(N, np.uint64)
+            want[0] = c
+        else:
+            m = np.zeros(N, np.uint64)
+            m[0] = c
+            want = (-O.nc_mul_exact64(keys.K2, m)).astype(np.uint64)
+        err = O.signed64((O.rlwe_phase64(keys.K2, ct) - want).astype(np.uint64))
+        assert np.abs(err).max() < 7 * 2 ** 13
+    # a random LWE_K2 of mu = 2^56
+    rng = np.random.default_rng(3)
+    a = rng.integers(0, 2 ** 64, N, dtype=np.uint64)
+    mu = 2 ** 56
+    b = (int(np.dot(a, keys.K2.astype(np.uint64))) + mu + 999) % 2 ** 64
+    lwe = np.concatenate([a, np.array([b], np.uint64)])
+    for u in (0, 1):
+        r = keys.pfks(u, lwe)
+        ph = O.rlwe_phase64(keys.K2, r)
+        m = np.zeros(N, np.uint64)
+        m[0] = mu
+        want = m if u == 1 else (-O.nc_mul_exact64(keys.K2, m)).astype(np.uint64)
+        err = O.signed64((ph - want).astype(np.uint64))
+        assert np.abs(err).max() < 2 ** 40
+
+
+@pytest.mark.parametrize("m", [0, 1])
+def test_cmux_selects(keys, m):
+    p = keys.p
+    N = p.N
+    rng = np.random.default_rng(10 + m)
+    C = keys.rgsw_fourier(keys.rgsw_encrypt(m, 500))
+    msgs = [rng.integers(0, 2, N).astype(np.uint64) << np.uint64(63) for _ in range(2)]
+    ds = []
+    for k, msg in enumerate(msgs):
+        a = rng.integers(0, 2 ** 64, N, dtype=np.uint64)
+        b = (O.nc_mul_exact64(keys.K2, a) + msg).astype(np.uint64)
+        ds.append(np.concatenate([a, b]))
+    out = np.zeros(2 * N, np.uint64)
+    err = O._cbs_lib().orc_cmux64(N, p.cb_logB, p.cb_l, C, ds[0], ds[1], out)
+    assert err < 0.25
+    ph = O.rlwe_phase64(keys.K2, out)
+    e = O.signed64((ph - msgs[m]).astype(np.uint64))
+    assert np.abs(e).max() < 2 ** 52  # CMux noise ~2^-13 of the torus
+
+
+def test_vertical_packing_table_lookup(keys):
+    """Eight CMux rotations by X^{-8 * 2^i} select table[x] (8 output bits) from a
+    table packed as coefficient 8x + j = bit j of table[x]."""
+    p = keys.p
+    rng = np.random.default_rng(77)
+    table = rng.integers(0, 256, 256)
+    poly = np.zeros(p.N, np.uint64)
+    for x in range(256):
+        for j in range(8):
+            poly[8 * x + j] = np.uint64(((int(table[x]) >> j) & 1) << 63)
+    for x in (0, 0xA7, 255):
+        fs = [keys.rgsw_fourier(keys.rgsw_encrypt((x >> i) & 1, 1000 + 8 * i + 64 * x)) for i in range(8)]
+        acc, err = keys.vertical_packing(fs, poly)
+        assert err < 0.25
+        ph = O.rlwe_phase64(keys.K2, acc)
+        got = [int(((int(ph[j]) + 2 ** 62) >> 63) & 1) for j in range(8)]
+        assert got == [(int(table[x]) >> j) & 1 for j in range(8)], x
+
+
+def test_ks2_preserves_phase(keys):
+    p = keys.p
+    N = p.N
+    rng = np.random.default_rng(5)
+    for bit in (0, 1):
+        a = rng.integers(0, 2 ** 64, N, dtype=np.uint64)
+        b = (int(np.dot(a, keys.K2.astype(np.uint64))) + (bit << 63) + 12345) % 2 ** 64
+        lwe = np.concatenate([a, np.array([b], np.uint64)])
+        out = keys.ks2(lwe)
+        assert keys.decrypt_bit(out) == bit
+        e = ((keys.phase32(out) - (bit << 31) + 2 ** 31) % 2 ** 32) - 2 ** 31
+        assert abs(e) < 2 ** 32 / 16
