@@ -1,0 +1,439 @@
+// capi_cbs.cpp -- C ABI of the circuit-bootstrapping context and the AES-128
+// transciphering pipeline built on it (include/tfhe_amd.h).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/tfhe_amd.h"
+#include "capi_common.hpp"
+#include "kernels.hpp"
+
+using namespace tfhe;
+
+namespace {
+constexpr int kN = 2048;
+
+bool cbs_params_supported(const tfhe_cbs_params* p) {
+    return p && p->N == kN && p->bk_logB == 9 && p->bk_l == 4 && p->cb_logB == 8 && p->cb_l == 2 && p->pf_logB == 4 &&
+           p->pf_l == 8 && p->ks_logB == 2 && p->ks_l == 8 && p->n > 0 && p->n <= 1024 && p->lwe_nshift >= 1 &&
+           p->lwe_nshift <= 31 && p->rlwe_nshift >= -31 && p->rlwe_nshift <= 31;
+}
+
+size_t bsk2_cplx(const tfhe_cbs_params& p) { return (size_t)p.n * 8 * 2 * 2 * 1024; }
+size_t pfks_words_per_u() { return (size_t)(kN + 1) * 8 * 15 * 2 * kN; }
+size_t ksk2_words(const tfhe_cbs_params& p) { return (size_t)kN * 8 * 4 * (p.n + 1); }
+
+// ---- AES-128 pieces computed by the library itself (client-side helpers and
+// the bit-level linear layer); the S-box is GF(2^8) inversion + affine map ----
+uint8_t gmul(uint8_t a, uint8_t b) {
+    uint8_t r = 0;
+    while (b) {
+        if (b & 1) r ^= a;
+        a = (uint8_t)((a << 1) ^ ((a & 0x80) ? 0x1b : 0));
+        b >>= 1;
+    }
+    return r;
+}
+uint8_t sbox(uint8_t x) {
+    uint8_t inv = 0;
+    if (x) {
+        uint8_t r = 1, b = x;
+        for (int e = 254; e; e >>= 1) {
+            if (e & 1) r = gmul(r, b);
+            b = gmul(b, b);
+        }
+        inv = r;
+    }
+    uint8_t res = 0x63, s = inv;
+    for (int i = 0; i < 5; ++i) {
+        res ^= s;
+        s = (uint8_t)((s << 1) | (s >> 7));
+    }
+    return res;
+}
+void key_expand(const uint8_t key[16], uint8_t rk[176]) {
+    std::memcpy(rk, key, 16);
+    uint8_t rcon = 1;
+    for (int i = 4; i < 44; ++i) {
+        uint8_t t[4];
+        std::memcpy(t, rk + 4 * (i - 1), 4);
+        if (i % 4 == 0) {
+            const uint8_t t0 = t[0];
+            t[0] = (uint8_t)(sbox(t[1]) ^ rcon);
+            t[1] = sbox(t[2]);
+            t[2] = sbox(t[3]);
+            t[3] = sbox(t0);
+            rcon = gmul(rcon, 2);
+        }
+        for (int k = 0; k < 4; ++k) rk[4 * i + k] = (uint8_t)(rk[4 * (i - 4) + k] ^ t[k]);
+    }
+}
+// ShiftRows then (optionally) MixColumns on 16 bytes, byte index = row + 4 col
+void shift_mix(const uint8_t in[16], uint8_t out[16], bool mix) {
+    uint8_t t[16];
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) t[r + 4 * c] = in[r + 4 * ((c + r) % 4)];
+    if (!mix) {
+        std::memcpy(out, t, 16);
+        return;
+    }
+    for (int c = 0; c < 4; ++c) {
+        const uint8_t* a = t + 4 * c;
+        out[4 * c + 0] = (uint8_t)(gmul(a[0], 2) ^ gmul(a[1], 3) ^ a[2] ^ a[3]);
+        out[4 * c + 1] = (uint8_t)(a[0] ^ gmul(a[1], 2) ^ gmul(a[2], 3) ^ a[3]);
+        out[4 * c + 2] = (uint8_t)(a[0] ^ a[1] ^ gmul(a[2], 2) ^ gmul(a[3], 3));
+        out[4 * c + 3] = (uint8_t)(gmul(a[0], 3) ^ a[1] ^ a[2] ^ gmul(a[3], 2));
+    }
+}
+constexpr int kLinSrc = 8;  // max sources per output bit of ShiftRows+MixColumns
+}  // namespace
+
+struct tfhe_cbs_ctx {
+    tfhe_cbs_params p{};
+    uint32_t max_batch = 0;  // LWE inputs per circuit-bootstrapping call
+    bool has_keys = false;
+    std::vector<int32_t> lwe_sk, K2;
+    uint64_t seed = 0;
+    int32_t* d_K2 = nullptr;
+    cplx* d_bsk2f = nullptr;
+    uint64_t* d_pfks = nullptr;  // [2][N+1][8][15][2N]
+    uint32_t* d_ksk2 = nullptr;
+    uint64_t* d_lwe64 = nullptr;  // [max_batch][2][N+1]
+    DeviceTables1024 tables;
+    // AES state
+    uint32_t max_blocks = 0;
+    uint64_t* d_rgsw = nullptr;   // [128 max_blocks][4][2][N]
+    cplx* d_rgsw_f = nullptr;     // [128 max_blocks][8][2][1024]
+    uint32_t* d_lut32 = nullptr;  // [128 max_blocks][N+1]
+    uint32_t* d_state[2] = {nullptr, nullptr};  // [max_blocks][128][n+1]
+    uint64_t* d_sbox_poly = nullptr;            // [N]
+    int16_t* d_lin_mix = nullptr;               // [128][kLinSrc]
+    int16_t* d_lin_last = nullptr;
+    uint32_t* d_rk = nullptr;                   // [11][128][n+1] encrypted round-key bits
+    uint8_t* d_ctr = nullptr;                   // [max_blocks][16]
+    bool has_round_keys = false;
+};
+
+extern "C" int tfhe_cbs_ctx_destroy(tfhe_cbs_ctx* c) {
+    if (!c) return TFHE_OK;
+    void* bufs[] = {c->d_K2, c->d_bsk2f, c->d_pfks, c->d_ksk2, c->d_lwe64, c->d_rgsw, c->d_rgsw_f, c->d_lut32,
+                    c->d_state[0], c->d_state[1], c->d_sbox_poly, c->d_lin_mix, c->d_lin_last, c->d_rk, c->d_ctr};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    c->tables.release();
+    delete c;
+    return TFHE_OK;
+}
+
+extern "C" int tfhe_cbs_ctx_create(const tfhe_cbs_params* params, uint32_t max_blocks, tfhe_cbs_ctx** out) {
+    if (!out) return fail(TFHE_E_PARAM, "tfhe_cbs_ctx_create: out is NULL");
+    *out = nullptr;
+    if (!cbs_params_supported(params))
+        return fail(TFHE_E_PARAM, "tfhe_cbs_ctx_create: only N=2048, PBS 2^9x4, CB 2^8x2, PFKS 2^4x8, KS 2^2x8 is built");
+    if (max_blocks == 0) max_blocks = 1;
+    auto* c = new tfhe_cbs_ctx();
+    c->p = *params;
+    c->max_blocks = max_blocks;
+    c->max_batch = 128 * max_blocks;
+    const auto& p = c->p;
+    const size_t B = c->max_batch;
+    int rc = TFHE_OK;
+    rc = rc ? rc : dmalloc((void**)&c->d_K2, kN * sizeof(int32_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_bsk2f, bsk2_cplx(p) * sizeof(cplx));
+    rc = rc ? rc : dmalloc((void**)&c->d_ksk2, ksk2_words(p) * sizeof(uint32_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_lwe64, B * 2 * (kN + 1) * sizeof(uint64_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_rgsw, B * 4 * 2 * kN * sizeof(uint64_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_rgsw_f, B * 8 * 2 * 1024 * sizeof(cplx));
+    rc = rc ? rc : dmalloc((void**)&c->d_lut32, B * (kN + 1) * sizeof(uint32_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_state[0], B * (p.n + 1) * sizeof(uint32_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_state[1], B * (p.n + 1) * sizeof(uint32_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_sbox_poly, kN * sizeof(uint64_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_lin_mix, 128 * kLinSrc * sizeof(int16_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_lin_last, 128 * kLinSrc * sizeof(int16_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_rk, (size_t)11 * 128 * (p.n + 1) * sizeof(uint32_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_ctr, (size_t)max_blocks * 16);
+    rc = rc ? rc : c->tables.init();
+    if (!rc) {
+        // S-box table packed for vertical packing: coefficient 8x + j = bit j of S(x), times 2^63
+        std::vector<uint64_t> poly(kN, 0);
+        for (int x = 0; x < 256; ++x)
+            for (int j = 0; j < 8; ++j) poly[8 * x + j] = (uint64_t)((sbox((uint8_t)x) >> j) & 1) << 63;
+        rc = hipcheck(hipMemcpy(c->d_sbox_poly, poly.data(), kN * sizeof(uint64_t), hipMemcpyHostToDevice), "sbox");
+    }
+    if (!rc) {
+        // source bit lists of the GF(2)-linear ShiftRows(+MixColumns) map, bit = 8*byte + j
+        for (int mix = 0; mix < 2 && !rc; ++mix) {
+            std::vector<int16_t> src(128 * kLinSrc, -1);
+            std::vector<int> cnt(128, 0);
+            for (int ib = 0; ib < 128; ++ib) {
+                uint8_t in[16] = {0}, o[16];
+                in[ib / 8] = (uint8_t)(1u << (ib % 8));
+                shift_mix(in, o, mix == 1);
+                for (int ob = 0; ob < 128; ++ob)
+                    if ((o[ob / 8] >> (ob % 8)) & 1) {
+                        if (cnt[ob] >= kLinSrc) { rc = fail(TFHE_E_PARAM, "linear layer: too many sources"); break; }
+                        src[ob * kLinSrc + cnt[ob]++] = (int16_t)ib;
+                    }
+            }
+            if (!rc)
+                rc = hipcheck(hipMemcpy(mix ? c->d_lin_mix : c->d_lin_last, src.data(), src.size() * sizeof(int16_t),
+                                        hipMemcpyHostToDevice), "linear tables");
+        }
+    }
+    if (rc) {
+        tfhe_cbs_ctx_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return TFHE_OK;
+}
+
+extern "C" int tfhe_gen_bsk64(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_lwe_sk, const int32_t* d_rlwe_sk,
+                              uint64_t* d_bsk, void* stream) {
+    if (!cbs_params_supported(p) || !d_lwe_sk || !d_rlwe_sk || !d_bsk) return fail(TFHE_E_PARAM, "tfhe_gen_bsk64: bad argument");
+    static DeviceTables1024 tables;
+    if (!tables.d_buf) {
+        int rc = tables.init();
+        if (rc) return rc;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    cplx* d_K2f = nullptr;
+    uint64_t* d_K64 = nullptr;
+    int rc = dmalloc((void**)&d_K2f, 2 * 1024 * sizeof(cplx));
+    rc = rc ? rc : dmalloc((void**)&d_K64, kN * sizeof(uint64_t));
+    std::vector<int32_t> hK(kN);
+    std::vector<uint64_t> hK64(kN);
+    if (!rc) rc = hipcheck(hipMemcpy(hK.data(), d_rlwe_sk, kN * sizeof(int32_t), hipMemcpyDeviceToHost), "K2");
+    for (int i = 0; i < kN; ++i) hK64[i] = (uint64_t)(int64_t)hK[i];
+    if (!rc) rc = hipcheck(hipMemcpy(d_K64, hK64.data(), kN * sizeof(uint64_t), hipMemcpyHostToDevice), "K64");
+    if (!rc) launch_to_fourier2048(d_K64, 1, tables.T, d_K2f, st);
+    if (!rc) launch_gen_bsk64(seed, p->n, p->rlwe_nshift, d_lwe_sk, d_rlwe_sk, d_K2f, tables.T, d_bsk, st);
+    if (!rc) rc = check_launch("gen_bsk64");
+    if (!rc) rc = hipcheck(hipStreamSynchronize(st), "gen_bsk64 sync");
+    (void)hipFree(d_K2f);
+    (void)hipFree(d_K64);
+    return rc;
+}
+
+extern "C" int tfhe_cbs_keygen(tfhe_cbs_ctx* c, const int32_t* lwe_sk, uint64_t seed) {
+    if (!c || !lwe_sk) return fail(TFHE_E_PARAM, "tfhe_cbs_keygen: bad argument");
+    const auto& p = c->p;
+    c->seed = seed;
+    c->lwe_sk.assign(lwe_sk, lwe_sk + p.n);
+    c->K2.resize(kN);
+    tfhe_rlwe_keygen(kN, seed, DOM_RLWE2_SK, c->K2.data());
+    int rc = TFHE_OK;
+    if (!c->d_pfks) rc = dmalloc((void**)&c->d_pfks, 2 * pfks_words_per_u() * sizeof(uint64_t));
+    int32_t* d_s = nullptr;
+    uint64_t *d_bsk = nullptr, *d_K64 = nullptr;
+    cplx* d_K2f = nullptr;
+    rc = rc ? rc : dmalloc((void**)&d_s, p.n * sizeof(int32_t));
+    rc = rc ? rc : dmalloc((void**)&d_bsk, (size_t)p.n * 8 * 2 * kN * sizeof(uint64_t));
+    rc = rc ? rc : dmalloc((void**)&d_K64, kN * sizeof(uint64_t));
+    rc = rc ? rc : dmalloc((void**)&d_K2f, 2 * 1024 * sizeof(cplx));
+    std::vector<uint64_t> hK64(kN);
+    for (int i = 0; i < kN; ++i) hK64[i] = (uint64_t)(int64_t)c->K2[i];
+    if (!rc) rc = hipcheck(hipMemcpy(d_s, c->lwe_sk.data(), p.n * sizeof(int32_t), hipMemcpyHostToDevice), "sk");
+    if (!rc) rc = hipcheck(hipMemcpy(c->d_K2, c->K2.data(), kN * sizeof(int32_t), hipMemcpyHostToDevice), "K2");
+    if (!rc) rc = hipcheck(hipMemcpy(d_K64, hK64.data(), kN * sizeof(uint64_t), hipMemcpyHostToDevice), "K64");
+    if (!rc) {
+        launch_to_fourier2048(d_K64, 1, c->tables.T, d_K2f, nullptr);
+        launch_gen_bsk64(seed, p.n, p.rlwe_nshift, d_s, c->d_K2, d_K2f, c->tables.T, d_bsk, nullptr);
+        launch_to_fourier2048(d_bsk, (size_t)p.n * 8 * 2, c->tables.T, c->d_bsk2f, nullptr);
+        launch_gen_pfks(seed, p.rlwe_nshift, c->d_K2, d_K2f, c->tables.T, c->d_pfks, nullptr);
+        launch_gen_ksk32(seed, p.n, kN, p.ks_logB, p.ks_l, p.lwe_nshift, d_s, c->d_K2, c->d_ksk2, nullptr, DOM_KSK2);
+        rc = check_launch("cbs keygen");
+    }
+    if (!rc) rc = hipcheck(hipDeviceSynchronize(), "cbs keygen sync");
+    (void)hipFree(d_s);
+    (void)hipFree(d_bsk);
+    (void)hipFree(d_K64);
+    (void)hipFree(d_K2f);
+    if (!rc) c->has_keys = true;
+    return rc;
+}
+
+extern "C" int tfhe_cbs_get_rlwe_sk(const tfhe_cbs_ctx* c, int32_t* out) {
+    if (!c || !out || c->K2.empty()) return fail(TFHE_E_STATE, "tfhe_cbs_get_rlwe_sk: no keys");
+    std::memcpy(out, c->K2.data(), kN * sizeof(int32_t));
+    return TFHE_OK;
+}
+
+extern "C" int tfhe_cbs_get_pfks_entry(const tfhe_cbs_ctx* c, int u, int i, int t, int v, uint64_t* out) {
+    if (!c || !out || !c->has_keys) return fail(TFHE_E_STATE, "tfhe_cbs_get_pfks_entry: no keys");
+    if (u < 0 || u > 1 || i < 0 || i > kN || t < 0 || t > 7 || v < 1 || v > 15)
+        return fail(TFHE_E_PARAM, "tfhe_cbs_get_pfks_entry: index out of range");
+    const size_t row = (((size_t)u * (kN + 1) + i) * 8 + t) * 15 + (v - 1);
+    return hipcheck(hipMemcpy(out, c->d_pfks + row * 2 * kN, 2 * kN * sizeof(uint64_t), hipMemcpyDeviceToHost),
+                    "pfks entry");
+}
+
+static int check_cbs(const tfhe_cbs_ctx* c, uint32_t batch, uint32_t cap) {
+    if (!c) return fail(TFHE_E_PARAM, "cbs call: ctx is NULL");
+    if (!c->has_keys) return fail(TFHE_E_STATE, "cbs call: no keys (call tfhe_cbs_keygen)");
+    if (batch > cap) return fail(TFHE_E_CAPACITY, "cbs call: batch exceeds the context capacity");
+    return TFHE_OK;
+}
+
+extern "C" int tfhe_cbs_pbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t batch, uint64_t* d_out, void* stream) {
+    int rc = check_cbs(c, batch, c ? c->max_batch : 0);
+    if (rc) return rc;
+    if (!d_lwe || !d_out) return fail(TFHE_E_PARAM, "pbs: NULL buffer");
+    if (!batch) return TFHE_OK;
+    launch_pbs2(d_lwe, batch, c->p.n, c->d_bsk2f, c->tables.T, d_out, (hipStream_t)stream);
+    return check_launch("pbs2");
+}
+
+extern "C" int tfhe_cbs_pfks_batch(tfhe_cbs_ctx* c, int u, const uint64_t* d_lwe64, uint32_t count, size_t out_stride,
+                                   uint64_t* d_out, void* stream) {
+    int rc = check_cbs(c, count, c ? 2 * c->max_batch : 0);
+    if (rc) return rc;
+    if (u < 0 || u > 1 || !d_lwe64 || !d_out || out_stride < 2 * (size_t)kN)
+        return fail(TFHE_E_PARAM, "pfks: bad argument");
+    if (!count) return TFHE_OK;
+    launch_pfks(d_lwe64, count, c->d_pfks + (size_t)u * pfks_words_per_u(), d_out, out_stride, (hipStream_t)stream);
+    return check_launch("pfks");
+}
+
+extern "C" int tfhe_cbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t batch, uint64_t* d_rgsw, void* stream) {
+    int rc = check_cbs(c, batch, c ? c->max_batch : 0);
+    if (rc) return rc;
+    if (!d_lwe || !d_rgsw) return fail(TFHE_E_PARAM, "cbs: NULL buffer");
+    if (!batch) return TFHE_OK;
+    rc = tfhe_cbs_pbs_batch(c, d_lwe, batch, c->d_lwe64, stream);
+    // LWE L = 2 input + j -> RGSW row 2j + u of input: out stride 4N u64 (two rows)
+    for (int u = 0; u < 2 && !rc; ++u)
+        rc = tfhe_cbs_pfks_batch(c, u, c->d_lwe64, 2 * batch, 4 * (size_t)kN, d_rgsw + (size_t)u * 2 * kN, stream);
+    return rc;
+}
+
+extern "C" int tfhe_rgsw_to_fourier(tfhe_cbs_ctx* c, const uint64_t* d_rgsw, uint32_t count, double* d_rgsw_f,
+                                    void* stream) {
+    if (!c || !d_rgsw || !d_rgsw_f) return fail(TFHE_E_PARAM, "rgsw_to_fourier: bad argument");
+    if (!count) return TFHE_OK;
+    launch_to_fourier2048(d_rgsw, (size_t)count * 8, c->tables.T, reinterpret_cast<cplx*>(d_rgsw_f), (hipStream_t)stream);
+    return check_launch("to_fourier2048");
+}
+
+extern "C" int tfhe_lut8_eval_batch(tfhe_cbs_ctx* c, const double* d_rgsw_f, uint32_t nlut, const uint64_t* d_table,
+                                    size_t table_stride, int out_bits, uint32_t* d_out32, void* stream) {
+    if (!c || !d_rgsw_f || !d_table || !d_out32 || out_bits < 1 || out_bits > 8)
+        return fail(TFHE_E_PARAM, "lut8: bad argument");
+    if (!nlut) return TFHE_OK;
+    launch_lut_vertical(reinterpret_cast<const cplx*>(d_rgsw_f), nlut, d_table, table_stride, 8, out_bits, c->tables.T,
+                        d_out32, (hipStream_t)stream);
+    return check_launch("lut_vertical");
+}
+
+extern "C" int tfhe_cbs_keyswitch_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe32, uint32_t count, uint32_t* d_out,
+                                        void* stream) {
+    if (!c || !c->has_keys || !d_lwe32 || !d_out) return fail(TFHE_E_PARAM, "cbs keyswitch: bad argument");
+    if (!count) return TFHE_OK;
+    if (launch_keyswitch32(d_lwe32, count, kN, c->p.n, c->p.ks_logB, c->p.ks_l, c->d_ksk2, d_out, (hipStream_t)stream))
+        return fail(TFHE_E_PARAM, "cbs keyswitch: unsupported shape");
+    return check_launch("keyswitch32 N=2048");
+}
+
+extern "C" int tfhe_polymul2048_batch(const int32_t* d_x, const uint64_t* d_y, uint32_t batch, uint64_t* d_out,
+                                      void* stream) {
+    if (!d_x || !d_y || !d_out) return fail(TFHE_E_PARAM, "polymul2048: NULL argument");
+    if (!batch) return TFHE_OK;
+    static DeviceTables1024 tables;
+    if (!tables.d_buf) {
+        int rc = tables.init();
+        if (rc) return rc;
+    }
+    launch_polymul2048(d_x, d_y, batch, tables.T, d_out, (hipStream_t)stream);
+    return check_launch("polymul2048");
+}
+
+// ---------------------------------------------------------------------------
+// AES-128 transciphering
+// ---------------------------------------------------------------------------
+extern "C" void tfhe_aes_encrypt_key_bits(int n, const int32_t* sk, uint64_t seed, const uint8_t key[16], int nshift,
+                                          uint32_t* out) {
+    uint8_t rk[176];
+    key_expand(key, rk);
+    for (int r = 0; r < 11; ++r)
+        for (int b = 0; b < 128; ++b) {
+            const uint32_t bit = (rk[16 * r + b / 8] >> (b % 8)) & 1u;
+            tfhe_lwe_encrypt(n, sk, seed, (uint32_t)(r * 128 + b), bit, 2, nshift, out + ((size_t)r * 128 + b) * (n + 1));
+        }
+}
+
+extern "C" void tfhe_aes128_encrypt_plain(const uint8_t key[16], const uint8_t in[16], uint8_t out[16]) {
+    uint8_t rk[176], s[16];
+    key_expand(key, rk);
+    for (int i = 0; i < 16; ++i) s[i] = (uint8_t)(in[i] ^ rk[i]);
+    for (int round = 1; round <= 10; ++round) {
+        for (int i = 0; i < 16; ++i) s[i] = sbox(s[i]);
+        uint8_t t[16];
+        shift_mix(s, t, round != 10);
+        for (int i = 0; i < 16; ++i) s[i] = (uint8_t)(t[i] ^ rk[16 * round + i]);
+    }
+    std::memcpy(out, s, 16);
+}
+
+extern "C" int tfhe_aes_set_round_keys(tfhe_cbs_ctx* c, const uint32_t* rk_bits) {
+    if (!c || !rk_bits) return fail(TFHE_E_PARAM, "aes_set_round_keys: bad argument");
+    int rc = hipcheck(hipMemcpy(c->d_rk, rk_bits, (size_t)11 * 128 * (c->p.n + 1) * sizeof(uint32_t),
+                                hipMemcpyHostToDevice), "round keys");
+    if (!rc) c->has_round_keys = true;
+    return rc;
+}
+
+// One AES round (1..10) on device LWE states [blocks][128][n+1]:
+// SubBytes (circuit bootstrap of every bit, S-box vertical packing, key switch),
+// then ShiftRows, MixColumns (rounds 1..9) and AddRoundKey as LWE additions.
+extern "C" int tfhe_aes_round_batch(tfhe_cbs_ctx* c, int round, const uint32_t* d_in, uint32_t blocks, uint32_t* d_out,
+                                    void* stream) {
+    if (!c || !d_in || !d_out || round < 1 || round > 10) return fail(TFHE_E_PARAM, "aes_round: bad argument");
+    if (!c->has_round_keys) return fail(TFHE_E_STATE, "aes_round: no round keys");
+    if (blocks > c->max_blocks) return fail(TFHE_E_CAPACITY, "aes_round: blocks exceed the context capacity");
+    if (!blocks) return TFHE_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t nb = 128 * blocks;
+    const int n = c->p.n;
+    int rc = tfhe_cbs_batch(c, d_in, nb, c->d_rgsw, stream);
+    if (!rc) rc = tfhe_rgsw_to_fourier(c, c->d_rgsw, nb, reinterpret_cast<double*>(c->d_rgsw_f), stream);
+    if (!rc) rc = tfhe_lut8_eval_batch(c, reinterpret_cast<const double*>(c->d_rgsw_f), 16 * blocks, c->d_sbox_poly, 0, 8,
+                                       c->d_lut32, stream);
+    // SubBytes output into d_out as scratch, then the linear layer into d_out via the spare state buffer
+    uint32_t* sb = (d_out == c->d_state[0]) ? c->d_state[1] : c->d_state[0];
+    if (!rc) rc = tfhe_cbs_keyswitch_batch(c, c->d_lut32, nb, sb, stream);
+    if (!rc) {
+        launch_aes_linear(sb, c->d_rk + (size_t)round * 128 * (n + 1), round == 10 ? c->d_lin_last : c->d_lin_mix,
+                          kLinSrc, blocks, n, d_out, st);
+        rc = check_launch("aes_linear");
+    }
+    return rc;
+}
+
+extern "C" int tfhe_aes_ctr_keystream_batch(tfhe_cbs_ctx* c, const uint8_t* h_ctr, uint32_t blocks, uint32_t* d_out,
+                                            void* stream) {
+    if (!c || !h_ctr || !d_out) return fail(TFHE_E_PARAM, "aes_ctr: bad argument");
+    if (!c->has_round_keys) return fail(TFHE_E_STATE, "aes_ctr: no round keys");
+    if (blocks > c->max_blocks) return fail(TFHE_E_CAPACITY, "aes_ctr: blocks exceed the context capacity");
+    if (!blocks) return TFHE_OK;
+    hipStream_t st = (hipStream_t)stream;
+    int rc = hipcheck(hipMemcpyAsync(c->d_ctr, h_ctr, (size_t)blocks * 16, hipMemcpyHostToDevice, st), "ctr");
+    if (rc) return rc;
+    launch_aes_init(c->d_rk, c->d_ctr, blocks, c->p.n, c->d_state[0], st);
+    rc = check_launch("aes_init");
+    int cur = 0;
+    for (int round = 1; round <= 10 && !rc; ++round) {
+        uint32_t* dst = (round == 10) ? d_out : c->d_state[cur ^ 1];
+        rc = tfhe_aes_round_batch(c, round, c->d_state[cur], blocks, dst, stream);
+        cur ^= 1;
+    }
+    return rc;
+}
+
+extern "C" int tfhe_lwe_xor_public(uint32_t* d_ct, const uint8_t* d_pub, uint32_t nbits, int n, void* stream) {
+    if (!d_ct || !d_pub) return fail(TFHE_E_PARAM, "xor_public: bad argument");
+    if (!nbits) return TFHE_OK;
+    launch_xor_public(d_ct, d_pub, nbits, n, (hipStream_t)stream);
+    return check_launch("xor_public");
+}

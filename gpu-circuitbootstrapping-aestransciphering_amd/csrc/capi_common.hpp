@@ -9,6 +9,7 @@
 
 #include "../../include/tfhe_amd.h"
 #include "device_common.hpp"
+#include "fft1024.hpp"
 
 namespace tfhe {
 
@@ -42,6 +43,46 @@ struct DeviceTables {
         if (rc) return rc;
         T.p1t = d_buf;
         T.p2 = d_buf + 512;
+        return TFHE_OK;
+    }
+    void release() {
+        if (d_buf) (void)hipFree(d_buf);
+        d_buf = nullptr;
+    }
+};
+
+// Twiddle tables of the 4-wave N = 2048 FFT (fft1024.hpp): p1t, p2, p3 [4][64],
+// tr [4][4][64]; 1792 complex values.
+struct DeviceTables1024 {
+    FFT1024Tables T{};
+    cplx* d_buf = nullptr;
+    static constexpr int kCount = 1792;
+    int init() {
+        std::vector<cplx> h(kCount);
+        const double PI = 3.14159265358979323846264338327950288;
+        const int N = 2048;
+        auto put = [&](int idx, double ang) { h[idx] = {std::cos(ang), std::sin(ang)}; };
+        for (int k0 = 0; k0 < 4; ++k0)
+            for (int b = 0; b < 64; ++b)  // (w^4)^b * W256^(b k0)
+                put(k0 * 64 + b, PI * 4.0 * b / N - 2.0 * PI * (double)((b * k0) % 256) / 256.0);
+        for (int e = 0; e < 4; ++e)
+            for (int L = 0; L < 64; ++L) put(256 + e * 64 + L, -2.0 * PI * (double)(((L & 15) * e) % 64) / 64.0);
+        for (int p = 0; p < 4; ++p)
+            for (int L = 0; L < 64; ++L) put(512 + p * 64 + L, -2.0 * PI * (double)(((L & 3) * p) % 16) / 16.0);
+        for (int r = 0; r < 4; ++r)
+            for (int s = 0; s < 4; ++s)
+                for (int l = 0; l < 64; ++l) {
+                    const int k = (l >> 4) + 4 * ((l >> 2) & 3) + 16 * (l & 3) + 64 * s;
+                    put(768 + (r * 4 + s) * 64 + l, PI * r / N - 2.0 * PI * (double)((r * k) % 1024) / 1024.0);
+                }
+        int rc = dmalloc((void**)&d_buf, h.size() * sizeof(cplx));
+        if (rc) return rc;
+        rc = hipcheck(hipMemcpy(d_buf, h.data(), h.size() * sizeof(cplx), hipMemcpyHostToDevice), "tables1024");
+        if (rc) return rc;
+        T.p1t = d_buf;
+        T.p2 = d_buf + 256;
+        T.p3 = d_buf + 512;
+        T.tr = d_buf + 768;
         return TFHE_OK;
     }
     void release() {

@@ -1,0 +1,692 @@
+// cbs.hip -- gfx950 kernels of the circuit-bootstrapping level (N = 2048,
+// torus-64): 2-limb Fourier conversion, key generation, the multi-value
+// programmable bootstrap, private functional key switching, the CMux vertical
+// packing LUT and the level-2 -> level-0 key switch.  All products of a digit
+// polynomial with a torus-64 polynomial are two FP64 products (one per signed
+// 32-bit limb), rounded and recombined mod 2^64, which makes them exact.
+#include "fft1024.hpp"
+#include "kernels.hpp"
+
+namespace tfhe {
+
+constexpr int CB_N = 2048, CB_M = 1024;
+
+// ---------------------------------------------------------------------------
+// shared helpers
+// ---------------------------------------------------------------------------
+// p1t, p2, p3 staged in LDS (768 cplx); the per-wave cross twiddles tr[w] live
+// in registers (4 cplx per lane).
+struct StagedTables {
+    FFT1024Tables T;
+};
+
+__device__ __forceinline__ FFT1024Tables stage_tables(const FFT1024Tables& Tg, cplx* tl) {
+    for (int k = threadIdx.x; k < 256; k += blockDim.x) {
+        tl[k] = Tg.p1t[k];
+        tl[256 + k] = Tg.p2[k];
+        tl[512 + k] = Tg.p3[k];
+    }
+    __syncthreads();
+    return {tl, tl + 256, tl + 512, Tg.tr};
+}
+
+__device__ __forceinline__ void split_limbs(uint64_t v, double& lo, double& hi) {
+    const int32_t l = (int32_t)(uint32_t)v;
+    lo = (double)l;
+    hi = (double)(int32_t)(uint32_t)((v - (uint64_t)(int64_t)l) >> 32);
+}
+
+// FFT1024 forward/inverse with the cross twiddles taken from registers.
+__device__ __forceinline__ void f1024_fwd(cplx (&x)[4], cplx* lds, cplx* xbuf, const FFT1024Tables& T,
+                                          const cplx (&trw)[4], int w, int lane) {
+    fft256_fwd(x, lds, T, lane);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) xbuf[(w * 4 + s) * 64 + lane] = cmul(x[s], trw[s]);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        cplx y0 = xbuf[(0 * 4 + s) * 64 + lane], y1 = xbuf[(1 * 4 + s) * 64 + lane];
+        cplx y2 = xbuf[(2 * 4 + s) * 64 + lane], y3 = xbuf[(3 * 4 + s) * 64 + lane];
+        cplx s02 = cadd(y0, y2), d02 = csub(y0, y2), s13 = cadd(y1, y3), d13 = mul_mi<false>(csub(y1, y3));
+        x[s] = (w == 0) ? cadd(s02, s13) : (w == 1) ? cadd(d02, d13) : (w == 2) ? csub(s02, s13) : csub(d02, d13);
+    }
+}
+
+__device__ __forceinline__ void f1024_inv(cplx (&x)[4], cplx* lds, cplx* xbuf, const FFT1024Tables& T,
+                                          const cplx (&trw)[4], int w, int lane) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) xbuf[(w * 4 + s) * 64 + lane] = x[s];
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        cplx z0 = xbuf[(0 * 4 + s) * 64 + lane], z1 = xbuf[(1 * 4 + s) * 64 + lane];
+        cplx z2 = xbuf[(2 * 4 + s) * 64 + lane], z3 = xbuf[(3 * 4 + s) * 64 + lane];
+        cplx s02 = cadd(z0, z2), d02 = csub(z0, z2), s13 = cadd(z1, z3), d13 = mul_mi<true>(csub(z1, z3));
+        cplx y = (w == 0) ? cadd(s02, s13) : (w == 1) ? cadd(d02, d13) : (w == 2) ? csub(s02, s13) : csub(d02, d13);
+        x[s] = cmulc(y, trw[s]);
+    }
+    fft256_inv(x, lds, T, lane);
+}
+
+__device__ __forceinline__ void load_trw(const FFT1024Tables& Tg, int w, int lane, cplx (&trw)[4]) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) trw[s] = Tg.tr[(w * 4 + s) * 64 + lane];
+}
+
+// coefficient owned by (wave w, lane, slot a): 4 (64 a + lane) + w; slot a + 4 is +1024
+__device__ __forceinline__ int own_coeff(int w, int lane, int a) { return 4 * (64 * a + lane) + w; }
+
+// ---------------------------------------------------------------------------
+// Fourier conversion: u64 polynomials -> [poly][limb][q][s][lane] (1024 cplx per limb)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_to_fourier2048(const uint64_t* __restrict__ coef, size_t npoly,
+                                                       FFT1024Tables Tg, cplx* __restrict__ out) {
+    __shared__ cplx tl[768];
+    __shared__ cplx lds_all[4][F256_LDS_CPLX];
+    __shared__ cplx xbuf[2][1024];
+    const FFT1024Tables T = stage_tables(Tg, tl);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    cplx trw[4];
+    load_trw(Tg, w, lane, trw);
+    int buf = 0;
+    for (size_t q = blockIdx.x; q < npoly; q += gridDim.x) {
+        const uint64_t* p = coef + q * CB_N;
+        cplx x0[4], x1[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int c = own_coeff(w, lane, a);
+            double l0r, l1r, l0i, l1i;
+            split_limbs(p[c], l0r, l1r);
+            split_limbs(p[c + CB_M], l0i, l1i);
+            x0[a] = {l0r, l0i};
+            x1[a] = {l1r, l1i};
+        }
+        f1024_fwd(x0, lds_all[w], xbuf[buf], T, trw, w, lane);
+        buf ^= 1;
+        f1024_fwd(x1, lds_all[w], xbuf[buf], T, trw, w, lane);
+        buf ^= 1;
+        cplx* o = out + q * 2 * CB_M;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            o[(w * 4 + s) * 64 + lane] = x0[s];
+            o[CB_M + (w * 4 + s) * 64 + lane] = x1[s];
+        }
+    }
+}
+
+// Exact negacyclic product of small-integer x by torus-64 y (N = 2048), one block per product.
+__global__ __launch_bounds__(256) void k_polymul2048(const int32_t* __restrict__ x, const uint64_t* __restrict__ y,
+                                                    uint32_t batch, FFT1024Tables Tg, uint64_t* __restrict__ out) {
+    __shared__ cplx tl[768];
+    __shared__ cplx lds_all[4][F256_LDS_CPLX];
+    __shared__ cplx xbuf[2][1024];
+    const FFT1024Tables T = stage_tables(Tg, tl);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    cplx trw[4];
+    load_trw(Tg, w, lane, trw);
+    int buf = 0;
+    for (uint32_t q = blockIdx.x; q < batch; q += gridDim.x) {
+        const int32_t* xp = x + (size_t)q * CB_N;
+        const uint64_t* yp = y + (size_t)q * CB_N;
+        cplx X[4], Y0[4], Y1[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int c = own_coeff(w, lane, a);
+            X[a] = {(double)xp[c], (double)xp[c + CB_M]};
+            double l0r, l1r, l0i, l1i;
+            split_limbs(yp[c], l0r, l1r);
+            split_limbs(yp[c + CB_M], l0i, l1i);
+            Y0[a] = {l0r, l0i};
+            Y1[a] = {l1r, l1i};
+        }
+        f1024_fwd(X, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        f1024_fwd(Y0, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        f1024_fwd(Y1, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            Y0[s] = cmul(Y0[s], X[s]);
+            Y1[s] = cmul(Y1[s], X[s]);
+        }
+        f1024_inv(Y0, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        f1024_inv(Y1, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        uint64_t* op = out + (size_t)q * CB_N;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int c = own_coeff(w, lane, a);
+            op[c] = (uint64_t)round_i64(Y0[a].re, 1.0 / CB_M) + ((uint64_t)round_i64(Y1[a].re, 1.0 / CB_M) << 32);
+            op[c + CB_M] = (uint64_t)round_i64(Y0[a].im, 1.0 / CB_M) + ((uint64_t)round_i64(Y1[a].im, 1.0 / CB_M) << 32);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Key generation over Z/2^64: one block per RLWE row; a * K2 is exact through
+// two limb products with the precomputed Fourier transform of K2.
+//   KIND_BSK : row id = i*8 + r; msg = s_i * 2^(28 + 9j) on component u of coefficient 0
+//   KIND_PFKS: row id = ((u*(N+1) + i)*8 + t)*16 + v (v >= 1);
+//              msg_b = f_u(v * kappa_i * 2^(32 + 4t)), kappa_N = -1,
+//              f_1(x) = x (coefficient 0), f_0(x) = -K2(X) x
+// Output row layout [2][N] (a then b); PFKS rows are stored densely over v = 1..15.
+// ---------------------------------------------------------------------------
+enum { KIND_BSK = 0, KIND_PFKS = 1 };
+
+template <int KIND>
+__global__ __launch_bounds__(256) void k_gen_rlwe64(uint64_t seed, uint32_t nrows, int nshift,
+                                                   const int32_t* __restrict__ s, const int32_t* __restrict__ K2,
+                                                   const cplx* __restrict__ K2f, FFT1024Tables Tg,
+                                                   uint64_t* __restrict__ out) {
+    __shared__ cplx tl[768];
+    __shared__ cplx lds_all[4][F256_LDS_CPLX];
+    __shared__ cplx xbuf[2][1024];
+    const FFT1024Tables T = stage_tables(Tg, tl);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    cplx trw[4], kf[4];
+    load_trw(Tg, w, lane, trw);
+#pragma unroll
+    for (int sl = 0; sl < 4; ++sl) kf[sl] = K2f[(w * 4 + sl) * 64 + lane];
+    int buf = 0;
+    for (uint32_t row = blockIdx.x; row < nrows; row += gridDim.x) {
+        uint32_t id, dom;
+        int msg_comp = 1;
+        uint64_t cval = 0;
+        bool poly_msg = false;  // f_0: msg_b = -K2 * cval
+        if constexpr (KIND == KIND_BSK) {
+            const int i = row / 8, r = row % 8, j = r >> 1;
+            msg_comp = r & 1;
+            cval = (uint64_t)s[i] << (28 + 9 * j);
+            id = row;
+            dom = DOM_BSK2;
+        } else {
+            const uint32_t v1 = row % 15, rest = row / 15;  // v = v1 + 1
+            const int t = rest % 8;
+            const uint32_t rest2 = rest / 8;
+            const int i = rest2 % (CB_N + 1);
+            const int u = rest2 / (CB_N + 1);
+            const uint64_t kappa = i < CB_N ? (uint64_t)(int64_t)K2[i] : ~0ull;
+            cval = (uint64_t)(v1 + 1) * kappa * (1ull << (32 + 4 * t));
+            poly_msg = (u == 0);
+            id = ((rest2 * 8) + t) * 16 + (v1 + 1);
+            dom = DOM_PFKS;
+        }
+        uint64_t av[8];
+        cplx A0[4], A1[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int c = own_coeff(w, lane, a);
+            av[a] = rand64(seed, dom, id, (uint32_t)c);
+            av[a + 4] = rand64(seed, dom, id, (uint32_t)(c + CB_M));
+            double l0r, l1r, l0i, l1i;
+            split_limbs(av[a], l0r, l1r);
+            split_limbs(av[a + 4], l0i, l1i);
+            A0[a] = {l0r, l0i};
+            A1[a] = {l1r, l1i};
+        }
+        f1024_fwd(A0, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        f1024_fwd(A1, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+#pragma unroll
+        for (int sl = 0; sl < 4; ++sl) {
+            A0[sl] = cmul(A0[sl], kf[sl]);
+            A1[sl] = cmul(A1[sl], kf[sl]);
+        }
+        f1024_inv(A0, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        f1024_inv(A1, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        uint64_t* oa = out + (size_t)row * 2 * CB_N;
+        uint64_t* ob = oa + CB_N;
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+            const int a = h & 3;
+            const int c = own_coeff(w, lane, a) + (h >= 4 ? CB_M : 0);
+            const double re0 = h < 4 ? A0[a].re : A0[a].im, re1 = h < 4 ? A1[a].re : A1[a].im;
+            uint64_t b = (uint64_t)round_i64(re0, 1.0 / CB_M) + ((uint64_t)round_i64(re1, 1.0 / CB_M) << 32);
+            b += (uint64_t)noise(seed, dom, id, (uint32_t)c, nshift);
+            uint64_t aval = av[h];
+            if constexpr (KIND == KIND_BSK) {
+                if (c == 0) {
+                    if (msg_comp == 0) aval += cval; else b += cval;
+                }
+            } else {
+                if (poly_msg) b -= (uint64_t)(int64_t)K2[c] * cval;
+                else if (c == 0) b += cval;
+            }
+            oa[c] = aval;
+            ob[c] = b;
+        }
+    }
+}
+
+// ===========================================================================
+// Multi-value programmable bootstrap (the circuit bootstrap's blind rotation):
+// one 4-wave workgroup per ciphertext, accumulator in registers in the
+// FFT1024 coefficient ownership.  Input LWE_s (torus-32) of b * 2^31.
+// Steps (reference ManyLUT path, cirbts-batch.cu:3293-4006, restated for the
+// torus-64 CGGI accumulator): b += 2^30; a~_i = special_ms(a_i, bw = 1);
+// test vector with b~ folded in; n CMux steps acc += (X^m - 1)(dec(acc) (*) BSK_i),
+// m = -a~_i mod 2N; extract coefficients 0 and 1 and add mu_j = 2^(47 + 8j).
+// Output [batch][2][N+1] u64: LWE_K2 of b * 2^(48 + 8j).
+// ===========================================================================
+constexpr int PBS_LOGB = 9, PBS_L = 4, PBS_ROWS = 8, PBS_IGN = 28;
+
+__device__ __forceinline__ uint64_t pbs_digit_offset() {
+    uint64_t off = 1ull << (PBS_IGN - 1);
+#pragma unroll
+    for (int j = 0; j < PBS_L; ++j) off += (1ull << (PBS_LOGB - 1)) << (PBS_IGN + j * PBS_LOGB);
+    return off;
+}
+
+__device__ __forceinline__ double pbs_digit(uint64_t tp, int j) {
+    return (double)((int32_t)((tp >> (PBS_IGN + j * PBS_LOGB)) & ((1u << PBS_LOGB) - 1)) - (1 << (PBS_LOGB - 1)));
+}
+
+// round(v * 2N / (2^32 2^bw)) * 2^bw mod 2N with N = 2048, bw = 1
+__device__ __forceinline__ uint32_t special_ms_2048(uint32_t v) {
+    return (uint32_t)((((uint64_t)v + (1ull << 20)) >> 21) << 1) & 4095u;
+}
+
+__global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict__ lwe, uint32_t batch, int n,
+                                                     const cplx* __restrict__ bskf, FFT1024Tables Tg,
+                                                     uint64_t* __restrict__ out) {
+    __shared__ cplx tl[768];
+    __shared__ cplx lds_all[4][F256_LDS_CPLX];
+    __shared__ cplx xbuf[2][1024];
+    __shared__ uint16_t at[1024];
+    const FFT1024Tables T = stage_tables(Tg, tl);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    cplx trw[4];
+    load_trw(Tg, w, lane, trw);
+    cplx* lds = lds_all[w];
+    uint64_t* x64 = reinterpret_cast<uint64_t*>(&xbuf[0][0]);  // [2][2048] u64 scratch for the monic exchange
+    const uint64_t OFF = pbs_digit_offset();
+    constexpr uint32_t N = CB_N, Q2 = 2 * CB_N;
+    const uint64_t mu0 = 1ull << 47, mu1 = 1ull << 55;
+
+    for (uint32_t cidx = blockIdx.x; cidx < batch; cidx += gridDim.x) {
+        const uint32_t* ct = lwe + (size_t)cidx * (n + 1);
+        __syncthreads();
+        for (int k = threadIdx.x; k < n; k += blockDim.x) at[k] = (uint16_t)special_ms_2048(ct[k]);
+        const uint32_t bt = special_ms_2048(ct[n] + (1u << 30));
+        __syncthreads();
+
+        uint64_t acc[2][8];
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+            const uint32_t c = (uint32_t)own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0);
+            const uint32_t arg = (bt + Q2 - (c & ~1u)) & (Q2 - 1);
+            uint64_t v = 0 - ((c & 1) ? mu1 : mu0);
+            if (arg >= N) v = 0 - v;
+            acc[0][h] = 0;
+            acc[1][h] = v;
+        }
+        int buf = 0;
+        for (int i = 0; i < n; ++i) {
+            const uint32_t mrot = (Q2 - (uint32_t)at[i]) & (Q2 - 1);
+            if (mrot == 0) continue;
+            const cplx* key = bskf + (size_t)i * PBS_ROWS * 2 * 2 * CB_M;
+            cplx macc[2][2][4];
+#pragma unroll
+            for (int o = 0; o < 2; ++o)
+#pragma unroll
+                for (int lb = 0; lb < 2; ++lb)
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) macc[o][lb][s] = {0.0, 0.0};
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                uint64_t tp[8];
+#pragma unroll
+                for (int h = 0; h < 8; ++h) tp[h] = acc[u][h] + OFF;
+#pragma unroll
+                for (int j = 0; j < PBS_L; ++j) {
+                    const int r = 2 * j + u;
+                    cplx x[4];
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) x[a] = {pbs_digit(tp[a], j), pbs_digit(tp[a + 4], j)};
+                    f1024_fwd(x, lds, xbuf[buf], T, trw, w, lane);
+                    buf ^= 1;
+                    const cplx* kr = key + (size_t)r * 4 * CB_M + (w * 4) * 64 + lane;
+#pragma unroll
+                    for (int o = 0; o < 2; ++o)
+#pragma unroll
+                        for (int lb = 0; lb < 2; ++lb)
+#pragma unroll
+                            for (int s = 0; s < 4; ++s)
+                                cmac(macc[o][lb][s], x[s], kr[(o * 2 + lb) * CB_M + s * 64]);
+                }
+            }
+            uint64_t t[2][8];
+#pragma unroll
+            for (int o = 0; o < 2; ++o) {
+                cplx y0[4], y1[4];
+#pragma unroll
+                for (int s = 0; s < 4; ++s) { y0[s] = macc[o][0][s]; y1[s] = macc[o][1][s]; }
+                f1024_inv(y0, lds, xbuf[buf], T, trw, w, lane);
+                buf ^= 1;
+                f1024_inv(y1, lds, xbuf[buf], T, trw, w, lane);
+                buf ^= 1;
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    t[o][a] = (uint64_t)round_i64(y0[a].re, 1.0 / CB_M) + ((uint64_t)round_i64(y1[a].re, 1.0 / CB_M) << 32);
+                    t[o][a + 4] = (uint64_t)round_i64(y0[a].im, 1.0 / CB_M) + ((uint64_t)round_i64(y1[a].im, 1.0 / CB_M) << 32);
+                }
+            }
+            // acc += X^mrot t - t, rotation across the four waves through LDS
+            __syncthreads();
+#pragma unroll
+            for (int o = 0; o < 2; ++o)
+#pragma unroll
+                for (int h = 0; h < 8; ++h) x64[o * CB_N + own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0)] = t[o][h];
+            __syncthreads();
+            const int kshift = (int)(mrot & (N - 1));
+            const bool hi = mrot >= N;
+#pragma unroll
+            for (int o = 0; o < 2; ++o)
+#pragma unroll
+                for (int h = 0; h < 8; ++h) {
+                    int idx = own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0) - kshift;
+                    const bool wrapped = idx < 0;
+                    idx += wrapped ? (int)N : 0;
+                    uint64_t v = x64[o * CB_N + idx];
+                    if (hi ^ wrapped) v = 0 - v;
+                    acc[o][h] += v - t[o][h];
+                }
+            __syncthreads();
+        }
+        // sample extraction of coefficients 0 and 1
+        uint64_t* o0 = out + (size_t)cidx * 2 * (N + 1);
+        uint64_t* o1 = o0 + (N + 1);
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+            const uint32_t c = (uint32_t)own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0);
+            const uint64_t v = acc[0][h];
+            o0[(0u - c) & (N - 1)] = (c == 0) ? v : 0 - v;
+            o1[(1u - c) & (N - 1)] = (c <= 1) ? v : 0 - v;
+        }
+        if (lane == 0 && w == 0) o0[N] = acc[1][0] + mu0;
+        if (lane == 0 && w == 1) o1[N] = acc[1][0] + mu1;
+    }
+}
+
+// ===========================================================================
+// Private functional key switching LWE_K2 -> RLWE_K2 (CGGI PrivKS) for one
+// function u:  out = - sum_{i<=N} sum_{t<8} KS_u[i][t][digit_t(a~_i)], a_N = b,
+// a~_i = round(a_i / 2^32) (top 32 bits), unsigned base-16 digits, least
+// significant first; digit 0 contributes nothing.  Key layout
+// [i][t][v-1][2N] u64 (v = 1..15).  A block is 4 waves over the same 64 output
+// coefficients, each wave owning PF_CT inputs; the 16 candidate rows of an
+// (i, t) pair sit in an LDS slot private to each lane (row 0 is zero), so the
+// per-input select is one indexed LDS read.  The four waves read the same key
+// rows, which the L1 cache serves once per block.
+// ===========================================================================
+constexpr int PF_CT = 32, PF_ICH = 64;
+
+__global__ __launch_bounds__(256) void k_pfks(const uint64_t* __restrict__ lwe64, uint32_t B,
+                                             const uint64_t* __restrict__ key, uint64_t* __restrict__ out,
+                                             size_t out_stride) {
+    __shared__ uint64_t rows[4][16][64];
+    __shared__ uint32_t atl[4][PF_CT][PF_ICH + 1];
+    constexpr int N = CB_N, COLS = 2 * CB_N;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int col = blockIdx.x * 64 + lane;
+    const uint32_t cbase = blockIdx.y * (4 * PF_CT) + w * PF_CT;
+    if (cbase >= B) return;  // whole wave idle; no block-level barriers below
+    uint64_t acc[PF_CT];
+#pragma unroll
+    for (int k = 0; k < PF_CT; ++k) acc[k] = 0;
+    rows[w][0][lane] = 0;
+    for (int i0 = 0; i0 <= N; i0 += PF_ICH) {
+        const int ich = (N + 1 - i0) < PF_ICH ? (N + 1 - i0) : PF_ICH;
+        wave_sync();
+        for (int e = lane; e < PF_CT * PF_ICH; e += 64) {
+            const int k = e / PF_ICH, ii = e % PF_ICH;
+            uint32_t v = 0;
+            if (cbase + k < B && ii < ich) {
+                const uint64_t a = lwe64[(size_t)(cbase + k) * (N + 1) + i0 + ii];
+                v = (uint32_t)((a + (1ull << 31)) >> 32);
+            }
+            atl[w][k][ii] = v;
+        }
+        wave_sync();
+        for (int ii = 0; ii < ich; ++ii) {
+            const uint64_t* kr = key + (size_t)(i0 + ii) * 8 * 15 * COLS + col;
+#pragma unroll 1
+            for (int t = 0; t < 8; ++t) {
+#pragma unroll
+                for (int v = 0; v < 15; ++v) rows[w][v + 1][lane] = kr[(size_t)(t * 15 + v) * COLS];
+#pragma unroll
+                for (int k = 0; k < PF_CT; ++k) {
+                    const uint32_t d = (atl[w][k][ii] >> (4 * t)) & 15u;
+                    acc[k] -= rows[w][d][lane];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PF_CT; ++k)
+        if (cbase + k < B) out[(size_t)(cbase + k) * out_stride + col] = acc[k];
+}
+
+// ===========================================================================
+// Vertical-packing LUT (reference CMux semantics, rlwe_tables.cuh:79-108):
+// acc = (0, table); for bit i = 0..7: acc = CMux(C_i, acc, X^{-stride 2^i} acc)
+// = acc + C_i (*) (X^{-stride 2^i} acc - acc), gadget 2^8 x 2 on torus-64.
+// Then coefficients 0..out_bits-1 are sample-extracted and switched to the
+// 32-bit torus (round half up): out [lut][out_bits][N+1] u32, ready for the
+// N -> n key switch.  rgsw_f [lut][8][4 rows][2 out][2 limbs][1024] cplx.
+// ===========================================================================
+constexpr int CB_LOGB = 8, CB_L = 2, CB_ROWS = 4, CB_IGN = 48;
+
+__device__ __forceinline__ uint64_t cb_digit_offset() {
+    uint64_t off = 1ull << (CB_IGN - 1);
+#pragma unroll
+    for (int j = 0; j < CB_L; ++j) off += (1ull << (CB_LOGB - 1)) << (CB_IGN + j * CB_LOGB);
+    return off;
+}
+
+__global__ __launch_bounds__(256) void k_lut_vertical(const cplx* __restrict__ rgsw_f, uint32_t nlut,
+                                                     const uint64_t* __restrict__ table, size_t table_stride,
+                                                     int stride, int out_bits, FFT1024Tables Tg,
+                                                     uint32_t* __restrict__ out) {
+    __shared__ cplx tl[768];
+    __shared__ cplx lds_all[4][F256_LDS_CPLX];
+    __shared__ cplx xbuf[2][1024];
+    const FFT1024Tables T = stage_tables(Tg, tl);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    cplx trw[4];
+    load_trw(Tg, w, lane, trw);
+    cplx* lds = lds_all[w];
+    uint64_t* x64 = reinterpret_cast<uint64_t*>(&xbuf[0][0]);
+    const uint64_t OFF = cb_digit_offset();
+    constexpr uint32_t N = CB_N;
+
+    for (uint32_t lut = blockIdx.x; lut < nlut; lut += gridDim.x) {
+        const uint64_t* tab = table + lut * table_stride;
+        uint64_t acc[2][8];
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+            acc[0][h] = 0;
+            acc[1][h] = tab[own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0)];
+        }
+        int buf = 0;
+        for (int bit = 0; bit < 8; ++bit) {
+            const uint32_t sh = (uint32_t)(stride << bit);
+            // diff = X^{-sh} acc - acc, rotation through LDS
+            __syncthreads();
+#pragma unroll
+            for (int o = 0; o < 2; ++o)
+#pragma unroll
+                for (int h = 0; h < 8; ++h) x64[o * CB_N + own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0)] = acc[o][h];
+            __syncthreads();
+            uint64_t tp[2][8];
+#pragma unroll
+            for (int o = 0; o < 2; ++o)
+#pragma unroll
+                for (int h = 0; h < 8; ++h) {
+                    const uint32_t src = (uint32_t)(own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0)) + sh;
+                    uint64_t v = x64[o * CB_N + (src & (N - 1))];
+                    if ((src / N) & 1) v = 0 - v;
+                    tp[o][h] = v - acc[o][h] + OFF;
+                }
+            __syncthreads();
+            const cplx* key = rgsw_f + ((size_t)lut * 8 + bit) * CB_ROWS * 4 * CB_M;
+            cplx macc[2][2][4];
+#pragma unroll
+            for (int o = 0; o < 2; ++o)
+#pragma unroll
+                for (int lb = 0; lb < 2; ++lb)
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) macc[o][lb][s] = {0.0, 0.0};
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int j = 0; j < CB_L; ++j) {
+                    const int r = 2 * j + u;
+                    cplx x[4];
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        const int sh2 = CB_IGN + j * CB_LOGB;
+                        x[a] = {(double)((int32_t)((tp[u][a] >> sh2) & 255u) - 128),
+                                (double)((int32_t)((tp[u][a + 4] >> sh2) & 255u) - 128)};
+                    }
+                    f1024_fwd(x, lds, xbuf[buf], T, trw, w, lane);
+                    buf ^= 1;
+                    const cplx* kr = key + (size_t)r * 4 * CB_M + (w * 4) * 64 + lane;
+#pragma unroll
+                    for (int o = 0; o < 2; ++o)
+#pragma unroll
+                        for (int lb = 0; lb < 2; ++lb)
+#pragma unroll
+                            for (int s = 0; s < 4; ++s) cmac(macc[o][lb][s], x[s], kr[(o * 2 + lb) * CB_M + s * 64]);
+                }
+#pragma unroll
+            for (int o = 0; o < 2; ++o) {
+                cplx y0[4], y1[4];
+#pragma unroll
+                for (int s = 0; s < 4; ++s) { y0[s] = macc[o][0][s]; y1[s] = macc[o][1][s]; }
+                f1024_inv(y0, lds, xbuf[buf], T, trw, w, lane);
+                buf ^= 1;
+                f1024_inv(y1, lds, xbuf[buf], T, trw, w, lane);
+                buf ^= 1;
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    acc[o][a] += (uint64_t)round_i64(y0[a].re, 1.0 / CB_M) + ((uint64_t)round_i64(y1[a].re, 1.0 / CB_M) << 32);
+                    acc[o][a + 4] += (uint64_t)round_i64(y0[a].im, 1.0 / CB_M) + ((uint64_t)round_i64(y1[a].im, 1.0 / CB_M) << 32);
+                }
+            }
+        }
+        // extract coefficients 0..out_bits-1, switch 2^64 -> 2^32
+        uint32_t* ob = out + (size_t)lut * out_bits * (N + 1);
+        for (int idx = 0; idx < out_bits; ++idx) {
+            uint32_t* o = ob + (size_t)idx * (N + 1);
+#pragma unroll
+            for (int h = 0; h < 8; ++h) {
+                const uint32_t c = (uint32_t)own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0);
+                const uint64_t v = (c <= (uint32_t)idx) ? acc[0][h] : 0 - acc[0][h];
+                o[((uint32_t)idx - c) & (N - 1)] = (uint32_t)((v + (1ull << 31)) >> 32);
+            }
+            if (lane == idx / 4 && w == idx % 4) o[N] = (uint32_t)((acc[1][0] + (1ull << 31)) >> 32);
+        }
+    }
+}
+
+// ===========================================================================
+// AES transciphering linear pieces on level-0 LWE bits (b * 2^31):
+// state init  = Enc(rk0 bit) + (0, counter bit * 2^31)  (reference
+//               kernel_InitLWEStateFromRoundKey, gpu_lut.cuh:425-452)
+// linear layer = ShiftRows + MixColumns + AddRoundKey as LWE additions:
+//               out[bit] = rk[bit] + sum_{s in src[bit]} in[s]  (XOR = addition
+//               of p = 2 encodings, rlwe_tables.cuh:26-35)
+// ===========================================================================
+__global__ void k_aes_init(const uint32_t* __restrict__ rk0, const uint8_t* __restrict__ ctr, uint32_t blocks, int n,
+                           uint32_t* __restrict__ state) {
+    const uint32_t blk = blockIdx.y, bit = blockIdx.x;
+    if (blk >= blocks) return;
+    const uint32_t pb = (ctr[blk * 16 + bit / 8] >> (bit % 8)) & 1u;
+    const uint32_t* src = rk0 + (size_t)bit * (n + 1);
+    uint32_t* dst = state + ((size_t)blk * 128 + bit) * (n + 1);
+    for (int k = threadIdx.x; k <= n; k += blockDim.x) dst[k] = src[k] + (k == n ? (pb << 31) : 0u);
+}
+
+__global__ void k_aes_linear(const uint32_t* __restrict__ in, const uint32_t* __restrict__ rk,
+                             const int16_t* __restrict__ src, int nsrc, uint32_t blocks, int n,
+                             uint32_t* __restrict__ out) {
+    const uint32_t blk = blockIdx.y, bit = blockIdx.x;
+    if (blk >= blocks) return;
+    const uint32_t* base = in + (size_t)blk * 128 * (n + 1);
+    uint32_t* dst = out + ((size_t)blk * 128 + bit) * (n + 1);
+    for (int k = threadIdx.x; k <= n; k += blockDim.x) {
+        uint32_t v = rk[(size_t)bit * (n + 1) + k];
+        for (int q = 0; q < nsrc; ++q) {
+            const int s = src[bit * nsrc + q];
+            if (s >= 0) v += base[(size_t)s * (n + 1) + k];
+        }
+        dst[k] = v;
+    }
+}
+
+// public bits XOR: ct[bit] += (0, pub_bit * 2^31)
+__global__ void k_xor_public(uint32_t* __restrict__ ct, const uint8_t* __restrict__ pub, uint32_t nbits, int n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nbits) return;
+    ct[(size_t)i * (n + 1) + n] += ((uint32_t)((pub[i / 8] >> (i % 8)) & 1u)) << 31;
+}
+
+// ===========================================================================
+// Host launchers
+// ===========================================================================
+static unsigned grid_for(size_t work, size_t cap) { return (unsigned)(work < cap ? work : cap); }
+
+void launch_lut_vertical(const cplx* rgsw_f, uint32_t nlut, const uint64_t* table, size_t table_stride, int stride,
+                         int out_bits, const FFT1024Tables& T, uint32_t* out, hipStream_t st) {
+    k_lut_vertical<<<grid_for(nlut, 2048), 256, 0, st>>>(rgsw_f, nlut, table, table_stride, stride, out_bits, T, out);
+}
+
+void launch_aes_init(const uint32_t* rk0, const uint8_t* ctr, uint32_t blocks, int n, uint32_t* state, hipStream_t st) {
+    k_aes_init<<<dim3(128, blocks), 256, 0, st>>>(rk0, ctr, blocks, n, state);
+}
+
+void launch_aes_linear(const uint32_t* in, const uint32_t* rk, const int16_t* src, int nsrc, uint32_t blocks, int n,
+                       uint32_t* out, hipStream_t st) {
+    k_aes_linear<<<dim3(128, blocks), 256, 0, st>>>(in, rk, src, nsrc, blocks, n, out);
+}
+
+void launch_xor_public(uint32_t* ct, const uint8_t* pub, uint32_t nbits, int n, hipStream_t st) {
+    k_xor_public<<<(nbits + 255) / 256, 256, 0, st>>>(ct, pub, nbits, n);
+}
+
+void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, const FFT1024Tables& T, uint64_t* out,
+                 hipStream_t st) {
+    k_pbs2_manylut<<<grid_for(batch, 2048), 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+}
+
+void launch_pfks(const uint64_t* lwe64, uint32_t B, const uint64_t* key_u, uint64_t* out, size_t out_stride,
+                 hipStream_t st) {
+    dim3 grid(2 * CB_N / 64, (B + 4 * PF_CT - 1) / (4 * PF_CT));
+    k_pfks<<<grid, 256, 0, st>>>(lwe64, B, key_u, out, out_stride);
+}
+
+void launch_to_fourier2048(const uint64_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st) {
+    k_to_fourier2048<<<grid_for(npoly, 4096), 256, 0, st>>>(coef, npoly, T, out);
+}
+
+void launch_polymul2048(const int32_t* x, const uint64_t* y, uint32_t batch, const FFT1024Tables& T, uint64_t* out,
+                        hipStream_t st) {
+    k_polymul2048<<<grid_for(batch, 4096), 256, 0, st>>>(x, y, batch, T, out);
+}
+
+void launch_gen_bsk64(uint64_t seed, int n, int nshift, const int32_t* s, const int32_t* K2, const cplx* K2f,
+                      const FFT1024Tables& T, uint64_t* bsk, hipStream_t st) {
+    const uint32_t rows = (uint32_t)n * 8;
+    k_gen_rlwe64<KIND_BSK><<<grid_for(rows, 8192), 256, 0, st>>>(seed, rows, nshift, s, K2, K2f, T, bsk);
+}
+
+void launch_gen_pfks(uint64_t seed, int nshift, const int32_t* K2, const cplx* K2f, const FFT1024Tables& T,
+                     uint64_t* key, hipStream_t st) {
+    const uint32_t rows = 2u * (CB_N + 1) * 8 * 15;
+    k_gen_rlwe64<KIND_PFKS><<<grid_for(rows, 8192), 256, 0, st>>>(seed, rows, nshift, nullptr, K2, K2f, T, key);
+}
+
+}  // namespace tfhe

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(256) void k_gen_bsk32(uint64_t seed, int n, int N, 
 }
 
 // KSK entry (i, j, v) = LWE_s(v * K_i * 2^(32 - l*logB + j*logB)), layout [N][l][B][n+1].
-__global__ __launch_bounds__(256) void k_gen_ksk32(uint64_t seed, int n, int N, int logB, int l,
+__global__ __launch_bounds__(256) void k_gen_ksk32(uint64_t seed, uint32_t dom, int n, int N, int logB, int l,
                                                    int nshift, const int32_t* __restrict__ s,
                                                    const int32_t* __restrict__ K,
                                                    uint32_t* __restrict__ ksk) {
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void k_gen_ksk32(uint64_t seed, int n, int N, 
     uint32_t* ct = ksk + (size_t)id * (n + 1);
     uint32_t part = 0;
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        uint32_t ak = rand32(seed, DOM_KSK1, id, (uint32_t)k);
+        uint32_t ak = rand32(seed, dom, id, (uint32_t)k);
         ct[k] = ak;
         part += ak * (uint32_t)s[k];
     }
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(256) void k_gen_ksk32(uint64_t seed, int n, int N, 
     if (threadIdx.x == 0) {
         const int kbits = l * logB;
         uint32_t mu = (uint32_t)v * (uint32_t)K[i] * (1u << (32 - kbits + j * logB));
-        ct[n] = red[0] + (uint32_t)noise(seed, DOM_KSK1, id, 0, nshift) + mu;
+        ct[n] = red[0] + (uint32_t)noise(seed, dom, id, 0, nshift) + mu;
     }
 }
 
@@ -432,8 +432,8 @@ void launch_gen_bsk32(uint64_t seed, int n, int N, int logB, int l, int nshift, 
 }
 
 void launch_gen_ksk32(uint64_t seed, int n, int N, int logB, int l, int nshift, const int32_t* s,
-                      const int32_t* K, uint32_t* ksk, hipStream_t st) {
-    k_gen_ksk32<<<N * l * (1 << logB), 256, 0, st>>>(seed, n, N, logB, l, nshift, s, K, ksk);
+                      const int32_t* K, uint32_t* ksk, hipStream_t st, uint32_t dom) {
+    k_gen_ksk32<<<N * l * (1 << logB), 256, 0, st>>>(seed, dom, n, N, logB, l, nshift, s, K, ksk);
 }
 
 void launch_to_fourier1024(const uint32_t* coef, size_t npoly, const FFTTables& T, cplx* out, hipStream_t st) {

@@ -12,7 +12,7 @@ struct FFTTables;
 void launch_gen_bsk32(uint64_t seed, int n, int N, int logB, int l, int nshift, const int32_t* s,
                       const int32_t* K, uint32_t* bsk, hipStream_t st);
 void launch_gen_ksk32(uint64_t seed, int n, int N, int logB, int l, int nshift, const int32_t* s,
-                      const int32_t* K, uint32_t* ksk, hipStream_t st);
+                      const int32_t* K, uint32_t* ksk, hipStream_t st, uint32_t dom = 5);
 void launch_to_fourier1024(const uint32_t* coef, size_t npoly, const FFTTables& T, cplx* out, hipStream_t st);
 void launch_gate_blind_rotate(const uint32_t* ct1, const uint32_t* ct2, uint32_t batch, int n, int doubled,
                               uint32_t q1, const cplx* bskf, const FFTTables& T, uint32_t* lweN,
@@ -27,4 +27,23 @@ void launch_external_product1024(uint32_t* acc, cplx* digf, uint32_t* t, const c
                                  hipStream_t st);
 int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int logB, int l,
                        const uint32_t* ksk, uint32_t* out, hipStream_t st);
+// circuit-bootstrapping level (N = 2048, torus-64)
+struct FFT1024Tables;
+void launch_to_fourier2048(const uint64_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st);
+void launch_polymul2048(const int32_t* x, const uint64_t* y, uint32_t batch, const FFT1024Tables& T, uint64_t* out,
+                        hipStream_t st);
+void launch_gen_bsk64(uint64_t seed, int n, int nshift, const int32_t* s, const int32_t* K2, const cplx* K2f,
+                      const FFT1024Tables& T, uint64_t* bsk, hipStream_t st);
+void launch_gen_pfks(uint64_t seed, int nshift, const int32_t* K2, const cplx* K2f, const FFT1024Tables& T,
+                     uint64_t* key, hipStream_t st);
+void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, const FFT1024Tables& T, uint64_t* out,
+                 hipStream_t st);
+void launch_pfks(const uint64_t* lwe64, uint32_t B, const uint64_t* key_u, uint64_t* out, size_t out_stride,
+                 hipStream_t st);
+void launch_lut_vertical(const cplx* rgsw_f, uint32_t nlut, const uint64_t* table, size_t table_stride, int stride,
+                         int out_bits, const FFT1024Tables& T, uint32_t* out, hipStream_t st);
+void launch_aes_init(const uint32_t* rk0, const uint8_t* ctr, uint32_t blocks, int n, uint32_t* state, hipStream_t st);
+void launch_aes_linear(const uint32_t* in, const uint32_t* rk, const int16_t* src, int nsrc, uint32_t blocks, int n,
+                       uint32_t* out, hipStream_t st);
+void launch_xor_public(uint32_t* ct, const uint8_t* pub, uint32_t nbits, int n, hipStream_t st);
 }  // namespace tfhe

@@ -35,6 +35,19 @@ def gate_params():
     return GateParams(636, 1024, 7, 3, 2, 8, 15, 25)
 
 
+class CbsParams(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("N", ctypes.c_int), ("bk_logB", ctypes.c_int), ("bk_l", ctypes.c_int),
+                ("cb_logB", ctypes.c_int), ("cb_l", ctypes.c_int), ("pf_logB", ctypes.c_int), ("pf_l", ctypes.c_int),
+                ("ks_logB", ctypes.c_int), ("ks_l", ctypes.c_int), ("lwe_nshift", ctypes.c_int),
+                ("rlwe_nshift", ctypes.c_int)]
+
+
+def cbs_params():
+    """n=636, N=2048 torus-64, PBS 2^9 x 4, CB 2^8 x 2, PFKS 2^4 x 8, KS 2^2 x 8,
+    level-0 sigma 2^-15, level-2 sigma 2^-51."""
+    return CbsParams(636, 2048, 9, 4, 8, 2, 4, 8, 2, 8, 15, 19)
+
+
 def build():
     """Compile libtfhe_amd.so for gfx950 in-tree (hipcc cross-compiles without a GPU)."""
     subprocess.run(["make", "-s", "-j8", "-C", PKG_DIR], check=True)
@@ -66,6 +79,30 @@ SIGNATURES = {
     "tfhe_lwe_encrypt": (None, [ctypes.c_int, _i32p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                 ctypes.c_uint32, ctypes.c_int, _u32p]),
     "tfhe_lwe_decrypt": (ctypes.c_uint32, [ctypes.c_int, _i32p, _u32p, ctypes.c_uint32]),
+    # circuit-bootstrapping level and AES
+    "tfhe_cbs_ctx_create": (ctypes.c_int, [ctypes.POINTER(CbsParams), ctypes.c_uint32, ctypes.POINTER(_vp)]),
+    "tfhe_cbs_ctx_destroy": (ctypes.c_int, [_vp]),
+    "tfhe_cbs_keygen": (ctypes.c_int, [_vp, _i32p, ctypes.c_uint64]),
+    "tfhe_cbs_get_rlwe_sk": (ctypes.c_int, [_vp, _i32p]),
+    "tfhe_cbs_get_pfks_entry": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")]),
+    "tfhe_gen_bsk64": (ctypes.c_int, [ctypes.POINTER(CbsParams), ctypes.c_uint64, _vp, _vp, _vp, _vp]),
+    "tfhe_cbs_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
+    "tfhe_cbs_pbs_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
+    "tfhe_cbs_pfks_batch": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_uint32, ctypes.c_size_t, _vp, _vp]),
+    "tfhe_rgsw_to_fourier": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
+    "tfhe_lut8_eval_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_size_t, ctypes.c_int, _vp, _vp]),
+    "tfhe_cbs_keyswitch_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
+    "tfhe_polymul2048_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
+    "tfhe_aes_encrypt_key_bits": (None, [ctypes.c_int, _i32p, ctypes.c_uint64,
+                                         np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
+                                         ctypes.c_int, _u32p]),
+    "tfhe_aes128_encrypt_plain": (None, [np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")] * 3),
+    "tfhe_aes_set_round_keys": (ctypes.c_int, [_vp, _u32p]),
+    "tfhe_aes_round_batch": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_uint32, _vp, _vp]),
+    "tfhe_aes_ctr_keystream_batch": (ctypes.c_int, [_vp, np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
+                                                    ctypes.c_uint32, _vp, _vp]),
+    "tfhe_lwe_xor_public": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_int, _vp]),
 }
 
 
@@ -166,3 +203,88 @@ class GateContext:
 
     def keyswitch_batch(self, d_lweN, d_out, stream=None):
         check(lib().tfhe_gate_keyswitch_batch(self.h, _ptr(d_lweN), d_lweN.shape[0], _ptr(d_out), _stream(stream)))
+
+
+def polymul2048_batch(d_x, d_y, d_out, stream=None):
+    check(lib().tfhe_polymul2048_batch(_ptr(d_x), _ptr(d_y), d_x.shape[0], _ptr(d_out), _stream(stream)))
+
+
+def aes_encrypt_key_bits(sk, seed, key, nshift=15):
+    n = len(sk)
+    out = np.zeros(11 * 128 * (n + 1), np.uint32)
+    lib().tfhe_aes_encrypt_key_bits(n, np.ascontiguousarray(sk, np.int32), seed,
+                                    np.frombuffer(bytes(key), np.uint8).copy(), nshift, out)
+    return out.reshape(11, 128, n + 1)
+
+
+def aes128_encrypt_plain(key, block):
+    out = np.zeros(16, np.uint8)
+    lib().tfhe_aes128_encrypt_plain(np.frombuffer(bytes(key), np.uint8).copy(),
+                                    np.frombuffer(bytes(block), np.uint8).copy(), out)
+    return bytes(out)
+
+
+class CbsContext:
+    """Circuit-bootstrapping + AES transciphering context (GPUCirBTSContext counterpart)."""
+
+    def __init__(self, max_blocks=1, params=None):
+        self.p = params if params is not None else cbs_params()
+        h = ctypes.c_void_p()
+        check(lib().tfhe_cbs_ctx_create(ctypes.byref(self.p), max_blocks, ctypes.byref(h)))
+        self.h = h
+        self.max_blocks = max_blocks
+
+    def close(self):
+        if self.h:
+            lib().tfhe_cbs_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def keygen(self, lwe_sk, seed):
+        check(lib().tfhe_cbs_keygen(self.h, np.ascontiguousarray(lwe_sk, np.int32), seed))
+
+    def rlwe_sk(self):
+        K = np.zeros(self.p.N, np.int32)
+        check(lib().tfhe_cbs_get_rlwe_sk(self.h, K))
+        return K
+
+    def pfks_entry(self, u, i, t, v):
+        out = np.zeros(2 * self.p.N, np.uint64)
+        check(lib().tfhe_cbs_get_pfks_entry(self.h, u, i, t, v, out))
+        return out
+
+    def pbs_batch(self, d_lwe, d_out, batch=None, stream=None):
+        b = d_lwe.shape[0] if batch is None else batch
+        check(lib().tfhe_cbs_pbs_batch(self.h, _ptr(d_lwe), b, _ptr(d_out), _stream(stream)))
+
+    def pfks_batch(self, u, d_lwe64, count, out_stride, d_out, stream=None):
+        check(lib().tfhe_cbs_pfks_batch(self.h, u, _ptr(d_lwe64), count, out_stride, _ptr(d_out), _stream(stream)))
+
+    def cbs_batch(self, d_lwe, d_rgsw, batch=None, stream=None):
+        b = d_lwe.shape[0] if batch is None else batch
+        check(lib().tfhe_cbs_batch(self.h, _ptr(d_lwe), b, _ptr(d_rgsw), _stream(stream)))
+
+    def rgsw_to_fourier(self, d_rgsw, count, d_rgsw_f, stream=None):
+        check(lib().tfhe_rgsw_to_fourier(self.h, _ptr(d_rgsw), count, _ptr(d_rgsw_f), _stream(stream)))
+
+    def lut8(self, d_rgsw_f, nlut, d_table, table_stride, out_bits, d_out32, stream=None):
+        check(lib().tfhe_lut8_eval_batch(self.h, _ptr(d_rgsw_f), nlut, _ptr(d_table), table_stride, out_bits,
+                                         _ptr(d_out32), _stream(stream)))
+
+    def keyswitch_batch(self, d_lwe32, count, d_out, stream=None):
+        check(lib().tfhe_cbs_keyswitch_batch(self.h, _ptr(d_lwe32), count, _ptr(d_out), _stream(stream)))
+
+    def set_round_keys(self, rk_bits):
+        check(lib().tfhe_aes_set_round_keys(self.h, np.ascontiguousarray(rk_bits, np.uint32).reshape(-1)))
+
+    def aes_round(self, rnd, d_in, blocks, d_out, stream=None):
+        check(lib().tfhe_aes_round_batch(self.h, rnd, _ptr(d_in), blocks, _ptr(d_out), _stream(stream)))
+
+    def aes_ctr_keystream(self, ctr_blocks, d_out, stream=None):
+        ctr = np.ascontiguousarray(np.frombuffer(b"".join(ctr_blocks), np.uint8))
+        check(lib().tfhe_aes_ctr_keystream_batch(self.h, ctr, len(ctr_blocks), _ptr(d_out), _stream(stream)))

@@ -121,6 +121,99 @@ int tfhe_gen_ksk32(const tfhe_gate_params* p, uint64_t seed, const int32_t* d_lw
                    uint32_t* d_ksk, void* stream);
 
 /* ------------------------------------------------------------------------- */
+/* Circuit-bootstrapping context (level 2: N = 2048, torus-64).  Replaces     */
+/* GPUCirBTSContext (reference bitwise/include/cirbts/cirbts.cuh:31-500).     */
+/* LWE inputs are level-0 ciphertexts (uint32, n + 1) encoding bit * 2^31.    */
+/* Outputs are RGSW_K2(bit) in the coefficient domain, uint64                 */
+/* [2*cb_l rows][2][N]: row 2j + u is the private functional key switch f_u   */
+/* of LWE_K2(bit * 2^(48 + 8j)), f_0 = -K2 * x, f_1 = x.                     */
+/* ------------------------------------------------------------------------- */
+typedef struct tfhe_cbs_params {
+    int n;            /* level-0 LWE dimension (636) */
+    int N;            /* ring dimension (2048) */
+    int bk_logB, bk_l;  /* PBS gadget 2^9 x 4 */
+    int cb_logB, cb_l;  /* circuit-bootstrapping gadget 2^8 x 2 */
+    int pf_logB, pf_l;  /* private functional key switching 2^4 x 8 */
+    int ks_logB, ks_l;  /* N -> n key switching 2^2 x 8 */
+    int lwe_nshift;     /* level-0 noise 2^-15 */
+    int rlwe_nshift;    /* level-2 noise: sigma = 2^-(32 + rlwe_nshift) of the 64-bit torus (19) */
+} tfhe_cbs_params;
+
+typedef struct tfhe_cbs_ctx tfhe_cbs_ctx;
+
+/* GPUCirBTSContext(cc, maxBatchSize) (cirbts.cuh:420-426): max_blocks AES blocks,
+ * i.e. 128 * max_blocks circuit bootstraps per call. */
+int tfhe_cbs_ctx_create(const tfhe_cbs_params* params, uint32_t max_blocks, tfhe_cbs_ctx** out);
+int tfhe_cbs_ctx_destroy(tfhe_cbs_ctx* ctx);
+
+/* Key generation on the GPU from seed (replaces the fork's CirBTKeyGen,
+ * main.cu:232-235, and gpu_init_cirbsk, cirbts-init.cu:200-865): RLWE secret
+ * K2, BSK2 (Fourier, 2 limbs), the PFKS key (2 x 8.05 GB) and KSK2. */
+int tfhe_cbs_keygen(tfhe_cbs_ctx* ctx, const int32_t* lwe_sk, uint64_t seed);
+int tfhe_cbs_get_rlwe_sk(const tfhe_cbs_ctx* ctx, int32_t* out);
+/* host copy of PFKS key entry (u, i, t, v), 2N uint64 (keygen parity) */
+int tfhe_cbs_get_pfks_entry(const tfhe_cbs_ctx* ctx, int u, int i, int t, int v, uint64_t* out);
+/* stateless BSK2 generator into d_bsk [n][8][2][N] uint64 (keygen parity) */
+int tfhe_gen_bsk64(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_lwe_sk, const int32_t* d_rlwe_sk,
+                   uint64_t* d_bsk, void* stream);
+
+/* gpu_CircuitBootstrappingBatchToDeviceFromDeviceLWE (cirbts.cuh:466-474,
+ * cirbts-batch.cu:1416-1879): d_lwe [batch][n+1] -> d_rgsw [batch][4][2][N].
+ * Pieces: the multi-value PBS (gpu_BootstrapLUT_inplace_batch_device,
+ * cirbts-batch.cu:3293) into d_out [batch][2][N+1] (LWE_K2 of bit * 2^48 and
+ * bit * 2^56), and the private functional key switch of count LWE_K2
+ * ciphertexts [count][N+1] into rows spaced out_stride uint64 apart. */
+int tfhe_cbs_batch(tfhe_cbs_ctx* ctx, const uint32_t* d_lwe, uint32_t batch, uint64_t* d_rgsw, void* stream);
+int tfhe_cbs_pbs_batch(tfhe_cbs_ctx* ctx, const uint32_t* d_lwe, uint32_t batch, uint64_t* d_out, void* stream);
+int tfhe_cbs_pfks_batch(tfhe_cbs_ctx* ctx, int u, const uint64_t* d_lwe64, uint32_t count, size_t out_stride,
+                        uint64_t* d_out, void* stream);
+
+/* RGSW (coefficient, [count][4][2][N] uint64) -> Fourier, 2 limbs per polynomial:
+ * d_rgsw_f [count][8][2][1024] complex doubles. */
+int tfhe_rgsw_to_fourier(tfhe_cbs_ctx* ctx, const uint64_t* d_rgsw, uint32_t count, double* d_rgsw_f, void* stream);
+
+/* 8-input CMux vertical-packing LUT (the S-box LUT; CPU semantics of
+ * EvalLUT8x32_RLWE_CPU, rlwe_tables.cuh:93-108, GPU form gpu_lut.cuh:1783):
+ * for lut k, control RGSWs d_rgsw_f[8k .. 8k+7] (bit 0 first), table polynomial
+ * d_table + k * table_stride (coefficient 8x + j = output bit j of entry x,
+ * times 2^63); out [nlut][out_bits][N+1] uint32 = extracted coefficients switched
+ * to the 32-bit torus. */
+int tfhe_lut8_eval_batch(tfhe_cbs_ctx* ctx, const double* d_rgsw_f, uint32_t nlut, const uint64_t* d_table,
+                         size_t table_stride, int out_bits, uint32_t* d_out32, void* stream);
+
+/* N -> n key switch of [count][N+1] uint32 (32-bit torus) LWE_K2 ciphertexts
+ * (reference kernel_KeySwitchU16, aes_gpu_ks.cuh:341-370). */
+int tfhe_cbs_keyswitch_batch(tfhe_cbs_ctx* ctx, const uint32_t* d_lwe32, uint32_t count, uint32_t* d_out, void* stream);
+
+/* Exact negacyclic product through the 4-wave FP64 FFT, N = 2048:
+ * d_out = d_x * d_y mod (X^N + 1, 2^64), small-integer d_x, torus-64 d_y. */
+int tfhe_polymul2048_batch(const int32_t* d_x, const uint64_t* d_y, uint32_t batch, uint64_t* d_out, void* stream);
+
+/* ------------------------------------------------------------------------- */
+/* AES-128 transciphering (reference bitwise/examples/aes_transcipher/main.cu */
+/* RunTranscipherBatch :1316-1619).  State bit 8*byte + j (byte = row + 4 col)*/
+/* is a level-0 LWE of that bit * 2^31.                                       */
+/* ------------------------------------------------------------------------- */
+/* client side: the 11 round keys' bits encrypted under sk, [11][128][n+1]
+ * (reference BuildEncryptedKeyMaterial, key_material.cuh:12-47) */
+void tfhe_aes_encrypt_key_bits(int n, const int32_t* sk, uint64_t seed, const uint8_t key[16], int nshift,
+                               uint32_t* out);
+/* plain AES-128 block encryption (the decryption check of the pipeline) */
+void tfhe_aes128_encrypt_plain(const uint8_t key[16], const uint8_t in[16], uint8_t out[16]);
+int tfhe_aes_set_round_keys(tfhe_cbs_ctx* ctx, const uint32_t* rk_bits);
+/* one round (1..10): SubBytes by circuit bootstrap + S-box LUT + key switch,
+ * then ShiftRows, MixColumns (rounds 1-9) and AddRoundKey as LWE additions
+ * (reference run_round_batch, main.cu:1438-1554); [blocks][128][n+1] */
+int tfhe_aes_round_batch(tfhe_cbs_ctx* ctx, int round, const uint32_t* d_in, uint32_t blocks, uint32_t* d_out,
+                         void* stream);
+/* CTR keystream: out [blocks][128][n+1] = LWE bits of AES_k(h_ctr[blk]) with the
+ * public counter blocks h_ctr [blocks][16] (host) */
+int tfhe_aes_ctr_keystream_batch(tfhe_cbs_ctx* ctx, const uint8_t* h_ctr, uint32_t blocks, uint32_t* d_out,
+                                 void* stream);
+/* XOR of public bits into LWE bits: d_ct[i] += (0, bit_i * 2^31), d_pub packed bits */
+int tfhe_lwe_xor_public(uint32_t* d_ct, const uint8_t* d_pub, uint32_t nbits, int n, void* stream);
+
+/* ------------------------------------------------------------------------- */
 /* Host helpers standing in for the OpenFHE fork's KeyGen/Encrypt/Decrypt     */
 /* (reference bitwise/examples/aes_transcipher/main.cu:232-235, 626, 1287).   */
 /* ------------------------------------------------------------------------- */

@@ -1,0 +1,230 @@
+"""GPU parity of the circuit-bootstrapping level (N = 2048, torus-64) and the
+AES-128 transciphering pipeline against the CPU oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import from_dev
+
+pytestmark = pytest.mark.gpu
+
+SEED = 21
+
+
+def dev(a, dtype):
+    import torch
+    a = np.ascontiguousarray(a)
+    t = torch.from_numpy(a.view(dtype).copy())
+    return t.to("cuda")
+
+
+def to_dev32(a):
+    import torch
+    return dev(np.asarray(a, np.uint32), np.int32)
+
+
+def to_dev64(a):
+    import torch
+    return dev(np.asarray(a, np.uint64), np.int64)
+
+
+@pytest.fixture(scope="module")
+def okeys():
+    return O.CbsKeys(SEED)
+
+
+@pytest.fixture(scope="module")
+def ctx(gpu, okeys):
+    import tfhe_amd
+    c = tfhe_amd.CbsContext(max_blocks=2)
+    c.keygen(okeys.s, SEED)
+    yield c
+    c.close()
+
+
+def test_polymul2048_exact(gpu):
+    import torch
+    import tfhe_amd
+    rng = np.random.default_rng(4)
+    batch = 5
+    x = rng.integers(-256, 256, (batch, 2048)).astype(np.int32)
+    y = rng.integers(0, 2 ** 64, (batch, 2048), dtype=np.uint64)
+    d_out = torch.empty((batch, 2048), dtype=torch.int64, device="cuda")
+    tfhe_amd.polymul2048_batch(to_dev32(x.view(np.uint32)), to_dev64(y), d_out)
+    torch.cuda.synchronize()
+    got = from_dev(d_out, np.uint64)
+    for k in range(batch):
+        assert np.array_equal(got[k], O.nc_mul_exact64(x[k], y[k])), k
+
+
+def test_keygen_parity(ctx, okeys):
+    """GPU key generation reproduces the oracle's K2, BSK2 and PFKS entries bit for bit."""
+    import torch
+    import tfhe_amd
+    assert np.array_equal(ctx.rlwe_sk(), okeys.K2)
+    p = tfhe_amd.cbs_params()
+    d_bsk = torch.empty(okeys.bsk.size, dtype=torch.int64, device="cuda")
+    tfhe_amd.check(tfhe_amd.lib().tfhe_gen_bsk64(p, SEED, tfhe_amd._ptr(to_dev32(okeys.s.view(np.uint32))),
+                                                 tfhe_amd._ptr(to_dev32(okeys.K2.view(np.uint32))),
+                                                 tfhe_amd._ptr(d_bsk), None))
+    torch.cuda.synchronize()
+    assert np.array_equal(from_dev(d_bsk, np.uint64), okeys.bsk)
+    for (u, i, t, v) in [(0, 0, 0, 1), (1, 0, 0, 1), (0, 2048, 7, 15), (1, 1000, 3, 9), (0, 17, 5, 4)]:
+        assert np.array_equal(ctx.pfks_entry(u, i, t, v), okeys.pfks_entry(u, i, t, v)), (u, i, t, v)
+
+
+def _bits_cts(okeys, bits, base_id):
+    return np.stack([okeys.encrypt_bit(int(b), base_id + k) for k, b in enumerate(bits)])
+
+
+def test_pbs_manylut_bit_exact(ctx, okeys):
+    import torch
+    bits = [0, 1, 1, 0]
+    cts = _bits_cts(okeys, bits, 300)
+    d_out = torch.empty((len(bits), 2, 2049), dtype=torch.int64, device="cuda")
+    ctx.pbs_batch(to_dev32(cts), d_out)
+    torch.cuda.synchronize()
+    got = from_dev(d_out, np.uint64)
+    for k, b in enumerate(bits):
+        want, err = okeys.pbs(cts[k])
+        assert err < 0.25
+        assert np.array_equal(got[k], want), k
+
+
+def test_pfks_bit_exact(ctx, okeys):
+    import torch
+    cts = _bits_cts(okeys, [1], 400)
+    d_lwe64 = torch.empty((1, 2, 2049), dtype=torch.int64, device="cuda")
+    ctx.pbs_batch(to_dev32(cts), d_lwe64)
+    torch.cuda.synchronize()
+    lwes = from_dev(d_lwe64, np.uint64)[0]
+    for u in (0, 1):
+        d_out = torch.empty((2, 4096), dtype=torch.int64, device="cuda")
+        ctx.pfks_batch(u, d_lwe64, 2, 4096, d_out)
+        torch.cuda.synchronize()
+        got = from_dev(d_out, np.uint64)
+        for j in range(2):
+            assert np.array_equal(got[j], okeys.pfks(u, lwes[j])), (u, j)
+
+
+def test_cbs_rgsw_rows_decrypt(ctx, okeys):
+    """Circuit bootstrap of 6 bits: row 2j+u decrypts to f_u(bit * 2^(48+8j))."""
+    import torch
+    bits = [1, 0, 1, 1, 0, 1]
+    cts = _bits_cts(okeys, bits, 500)
+    d_rgsw = torch.empty((len(bits), 4, 2, 2048), dtype=torch.int64, device="cuda")
+    ctx.cbs_batch(to_dev32(cts), d_rgsw)
+    torch.cuda.synchronize()
+    rg = from_dev(d_rgsw, np.uint64)
+    for k, b in enumerate(bits):
+        for j in range(2):
+            mu = np.zeros(2048, np.uint64)
+            mu[0] = np.uint64((b << (48 + 8 * j)) % 2 ** 64)
+            for u in range(2):
+                ph = O.rlwe_phase64(okeys.K2, rg[k, 2 * j + u].reshape(-1))
+                want = mu if u == 1 else (-O.nc_mul_exact64(okeys.K2, mu)).astype(np.uint64)
+                err = O.signed64((ph - want).astype(np.uint64))
+                assert np.abs(err).max() < 2 ** 40, (k, j, u)
+
+
+def _sbox_poly():
+    S = [O.aes_sbox(x) for x in range(256)]
+    poly = np.zeros(2048, np.uint64)
+    for x in range(256):
+        for j in range(8):
+            poly[8 * x + j] = np.uint64(((S[x] >> j) & 1) << 63)
+    return poly, S
+
+
+def test_lut_bit_exact(ctx, okeys):
+    """Vertical packing on fresh RGSWs equals the oracle (extracted, switched to 32 bits)."""
+    import torch
+    poly, S = _sbox_poly()
+    xs = [0x53, 0x00]
+    rgsws = [okeys.rgsw_encrypt((x >> i) & 1, 2000 + 16 * k + 2 * i) for k, x in enumerate(xs) for i in range(8)]
+    d_rgsw = to_dev64(np.concatenate(rgsws))
+    d_f = torch.empty(len(rgsws) * 8 * 2 * 1024 * 2, dtype=torch.float64, device="cuda")
+    ctx.rgsw_to_fourier(d_rgsw, len(rgsws), d_f)
+    d_out = torch.empty((len(xs), 8, 2049), dtype=torch.int32, device="cuda")
+    ctx.lut8(d_f, len(xs), to_dev64(poly), 0, 8, d_out)
+    torch.cuda.synchronize()
+    got = from_dev(d_out)
+    for k, x in enumerate(xs):
+        fs = [okeys.rgsw_fourier(r) for r in rgsws[8 * k:8 * k + 8]]
+        acc, err = okeys.vertical_packing(fs, poly)
+        assert err < 0.25
+        for j in range(8):
+            lwe = okeys.extract64(acc, j)
+            want = ((lwe + np.uint64(1 << 31)) >> np.uint64(32)).astype(np.uint32)
+            assert np.array_equal(got[k, j], want), (x, j)
+
+
+def test_ks2_bit_exact(ctx, okeys):
+    import torch
+    rng = np.random.default_rng(9)
+    count = 19
+    lwe64 = rng.integers(0, 2 ** 64, (count, 2049), dtype=np.uint64)
+    l32 = ((lwe64 + np.uint64(1 << 31)) >> np.uint64(32)).astype(np.uint32)
+    d_out = torch.empty((count, 637), dtype=torch.int32, device="cuda")
+    ctx.keyswitch_batch(to_dev32(l32), count, d_out)
+    torch.cuda.synchronize()
+    got = from_dev(d_out)
+    for k in range(count):
+        assert np.array_equal(got[k], okeys.ks2(lwe64[k])), k
+
+
+def test_sbox_end_to_end(ctx, okeys):
+    """Encrypted bytes -> circuit bootstrap of each bit -> S-box LUT -> key switch
+    decrypts to S(x) for every probed byte."""
+    import torch
+    poly, S = _sbox_poly()
+    xs = [0x00, 0x53, 0xFF, 0x1C, 0xA7, 0x80]
+    bits = [(x >> i) & 1 for x in xs for i in range(8)]
+    cts = _bits_cts(okeys, bits, 700)
+    nb = len(bits)
+    d_rgsw = torch.empty((nb, 4, 2, 2048), dtype=torch.int64, device="cuda")
+    ctx.cbs_batch(to_dev32(cts), d_rgsw)
+    d_f = torch.empty(nb * 8 * 2 * 1024 * 2, dtype=torch.float64, device="cuda")
+    ctx.rgsw_to_fourier(d_rgsw, nb, d_f)
+    d_l = torch.empty((len(xs), 8, 2049), dtype=torch.int32, device="cuda")
+    ctx.lut8(d_f, len(xs), to_dev64(poly), 0, 8, d_l)
+    d_out = torch.empty((len(xs) * 8, 637), dtype=torch.int32, device="cuda")
+    ctx.keyswitch_batch(d_l, len(xs) * 8, d_out)
+    torch.cuda.synchronize()
+    got = from_dev(d_out)
+    for k, x in enumerate(xs):
+        val = sum(okeys.decrypt_bit(got[8 * k + j]) << j for j in range(8))
+        assert val == S[x], (hex(x), hex(val), hex(S[x]))
+
+
+def _decrypt_blocks(okeys, bits_ct, blocks):
+    out = []
+    for b in range(blocks):
+        v = bytearray(16)
+        for bit in range(128):
+            v[bit // 8] |= okeys.decrypt_bit(bits_ct[b, bit]) << (bit % 8)
+        out.append(bytes(v))
+    return out
+
+
+def test_aes128_fips197_and_random_blocks(ctx, okeys):
+    """Homomorphic AES-128 (CTR keystream with the counter block as input):
+    the FIPS-197 C.1 vector and a seeded random block, both bit-exact after
+    decryption; every output phase inside the p = 2 decision margin."""
+    import torch
+    import tfhe_amd
+    key = bytes(range(16))
+    rk = tfhe_amd.aes_encrypt_key_bits(okeys.s, 77, key)
+    ctx.set_round_keys(rk)
+    blocks = [bytes.fromhex("00112233445566778899aabbccddeeff"), bytes(np.random.default_rng(3).integers(0, 256, 16, dtype=np.uint8))]
+    d_ks = torch.empty((2, 128, 637), dtype=torch.int32, device="cuda")
+    ctx.aes_ctr_keystream(blocks, d_ks)
+    torch.cuda.synchronize()
+    got = from_dev(d_ks)
+    dec = _decrypt_blocks(okeys, got, 2)
+    assert dec[0].hex() == "69c4e0d86a7b0430d8cdb78070b4c55a"
+    assert dec[1] == O.aes128_encrypt(key, blocks[1]) == tfhe_amd.aes128_encrypt_plain(key, blocks[1])
+    s64 = okeys.s.astype(np.uint64)
+    ph = (got[:, :, -1].astype(np.uint64) - (got[:, :, :-1].astype(np.uint64) @ s64)) % 2 ** 32
+    err = ((ph + 2 ** 30) % 2 ** 31).astype(np.int64) - 2 ** 30
+    assert np.abs(err).max() < 2 ** 29  # within half of the 2^30 margin
