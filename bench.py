@@ -61,7 +61,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              f"(bootstraps are independent: a multi-GPU run would shard the batch, see DESIGN.md)")
+              f"(bootstraps are independent: a multi-GPU run would give each GPU a key replica and a slice "
+              f"of the batch)")
         sys.exit(2)
 
     import numpy as np

@@ -341,6 +341,10 @@ template <int LOGB, int L>
 __global__ __launch_bounds__(KS_COLS) void k_keyswitch32(const uint32_t* __restrict__ lweN, uint32_t batch, int N,
                                                     int n, const uint32_t* __restrict__ ksk,
                                                     uint32_t* __restrict__ out) {
+    // blockIdx.z selects a slice of the N input coefficients; slices accumulate into
+    // the zero-filled output with 32-bit atomic adds (addition mod 2^32 is order-free)
+    const int islice = N / (int)gridDim.z;
+    const int ibeg = (int)blockIdx.z * islice, iend = ibeg + islice;
     constexpr int B = 1 << LOGB, KBITS = L * LOGB;
     __shared__ uint32_t atil[KS_CT][KS_ICHUNK];
     const int col = blockIdx.x * KS_COLS + threadIdx.x;
@@ -350,7 +354,7 @@ __global__ __launch_bounds__(KS_COLS) void k_keyswitch32(const uint32_t* __restr
 #pragma unroll
     for (int c = 0; c < KS_CT; ++c) acc[c] = 0u;
     const size_t rowstride = (size_t)(n + 1);
-    for (int i0 = 0; i0 < N; i0 += KS_ICHUNK) {
+    for (int i0 = ibeg; i0 < iend; i0 += KS_ICHUNK) {
         __syncthreads();
         for (int e = threadIdx.x; e < KS_CT * KS_ICHUNK; e += KS_COLS) {
             const int c = e / KS_ICHUNK, ii = e % KS_ICHUNK;
@@ -386,9 +390,14 @@ __global__ __launch_bounds__(KS_COLS) void k_keyswitch32(const uint32_t* __restr
     for (int c = 0; c < KS_CT; ++c) {
         if (c0 + c >= batch) break;
         uint32_t v = acc[c];
-        if (col == n) v += lweN[(size_t)(c0 + c) * (N + 1) + N];
-        out[(size_t)(c0 + c) * (n + 1) + col] = v;
+        if (col == n && blockIdx.z == 0) v += lweN[(size_t)(c0 + c) * (N + 1) + N];
+        atomicAdd(&out[(size_t)(c0 + c) * (n + 1) + col], v);
     }
+}
+
+__global__ void k_fill_zero(uint32_t* __restrict__ p, size_t count) {
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += (size_t)gridDim.x * blockDim.x)
+        p[k] = 0u;
 }
 
 // ===========================================================================
@@ -474,9 +483,12 @@ void launch_external_product1024(uint32_t* acc, cplx* digf, uint32_t* t, const c
 
 int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int logB, int l,
                        const uint32_t* ksk, uint32_t* out, hipStream_t st) {
-    dim3 grid((n + 1 + KS_COLS - 1) / KS_COLS, (batch + KS_CT - 1) / KS_CT);
-    if (N % KS_ICHUNK) return -1;
+    constexpr int KS_SPLIT = 4;
+    dim3 grid((n + 1 + KS_COLS - 1) / KS_COLS, (batch + KS_CT - 1) / KS_CT, KS_SPLIT);
+    if (N % (KS_ICHUNK * KS_SPLIT)) return -1;
     if (logB == 2 && l == 8) {
+        const size_t count = (size_t)batch * (n + 1);
+        k_fill_zero<<<(unsigned)((count + 255) / 256 < 4096 ? (count + 255) / 256 : 4096), 256, 0, st>>>(out, count);
         k_keyswitch32<2, 8><<<grid, KS_COLS, 0, st>>>(lweN, batch, N, n, ksk, out);
         return 0;
     }

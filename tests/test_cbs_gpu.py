@@ -64,8 +64,9 @@ def test_keygen_parity(ctx, okeys):
     assert np.array_equal(ctx.rlwe_sk(), okeys.K2)
     p = tfhe_amd.cbs_params()
     d_bsk = torch.empty(okeys.bsk.size, dtype=torch.int64, device="cuda")
-    tfhe_amd.check(tfhe_amd.lib().tfhe_gen_bsk64(p, SEED, tfhe_amd._ptr(to_dev32(okeys.s.view(np.uint32))),
-                                                 tfhe_amd._ptr(to_dev32(okeys.K2.view(np.uint32))),
+    d_s = to_dev32(okeys.s.view(np.uint32))  # keep the device buffers alive across the call
+    d_K2 = to_dev32(okeys.K2.view(np.uint32))
+    tfhe_amd.check(tfhe_amd.lib().tfhe_gen_bsk64(p, SEED, tfhe_amd._ptr(d_s), tfhe_amd._ptr(d_K2),
                                                  tfhe_amd._ptr(d_bsk), None))
     torch.cuda.synchronize()
     assert np.array_equal(from_dev(d_bsk, np.uint64), okeys.bsk)
