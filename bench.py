@@ -49,6 +49,48 @@ def cpu_baseline(sample=24):
                       f"oracle C restatement, FP64 radix-2 FFT, one thread"}
 
 
+def aes_ctr_bench(batches, reps=1):
+    """AES-128 CTR keystream: circuit bootstrap + S-box LUT per round,
+    linear layers as LWE additions; every block decrypted and compared with plain AES."""
+    import numpy as np
+    import torch
+    import tfhe_amd
+    maxb = max(batches)
+    ctx = tfhe_amd.CbsContext(max_blocks=maxb)
+    sk = tfhe_amd.lwe_keygen(N_LWE, 3)
+    t0 = time.perf_counter()
+    ctx.keygen(sk, 3)
+    keygen_s = time.perf_counter() - t0
+    rng = np.random.default_rng(11)
+    key = bytes(rng.integers(0, 256, 16, dtype=np.uint8))
+    ctx.set_round_keys(tfhe_amd.aes_encrypt_key_bits(sk, 5, key))
+    nonce = bytes(rng.integers(0, 256, 8, dtype=np.uint8))
+    out = {}
+    d_ks = torch.empty((maxb, 128, N_LWE + 1), dtype=torch.int32, device="cuda")
+    s64 = sk.astype(np.uint64)
+    for blocks in batches:
+        ctrs = [nonce + int(b).to_bytes(8, "big") for b in range(blocks)]
+        ctx.aes_ctr_keystream(ctrs, d_ks)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.aes_ctr_keystream(ctrs, d_ks)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        got = d_ks[:blocks].cpu().numpy().view(np.uint32)
+        ph = (got[:, :, -1].astype(np.uint64) - (got[:, :, :-1].astype(np.uint64) @ s64)) % 2 ** 32
+        bits = ((ph + (1 << 30)) >> 31) & 1
+        bad = 0
+        for b in range(blocks):
+            want = tfhe_amd.aes128_encrypt_plain(key, ctrs[b])
+            v = bytes(int(sum(int(bits[b, 8 * k + j]) << j for j in range(8))) for k in range(16))
+            bad += v != want
+        out[str(blocks)] = {"blocks_per_s": round(blocks / dt, 3), "s_per_launch": round(dt, 4),
+                            "circuit_bootstraps_per_s": round(blocks * 1280 / dt, 1), "blocks_wrong": int(bad)}
+    ctx.close()
+    return {"form": "fp64 (torus-64, 2-limb exact FFT)", "keygen_s": round(keygen_s, 2), "batches": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +100,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3, help="plain per-bootstrap form samples")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run three 512 MiB device copies (the HBM-counter calibration kernel)")
+    ap.add_argument("--aes-blocks", default="1,17",
+                    help="comma-separated AES-128 CTR batch sizes to time ('' to skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -184,10 +228,12 @@ def main():
         "plain_fp64_per_bootstrap_rate": None if plain_rate is None else round(plain_rate, 2),
         "decrypt_failures": failures,
     }
+    ctx.close()
+    if args.aes_blocks:
+        line["aes128_ctr"] = aes_ctr_bench([int(b) for b in args.aes_blocks.split(",") if b])
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
-    ctx.close()
 
 
 if __name__ == "__main__":

@@ -413,13 +413,17 @@ __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict
 // function u:  out = - sum_{i<=N} sum_{t<8} KS_u[i][t][digit_t(a~_i)], a_N = b,
 // a~_i = round(a_i / 2^32) (top 32 bits), unsigned base-16 digits, least
 // significant first; digit 0 contributes nothing.  Key layout
-// [i][t][v-1][2N] u64 (v = 1..15).  A block is 4 waves over the same 64 output
-// coefficients, each wave owning PF_CT inputs; the 16 candidate rows of an
-// (i, t) pair sit in an LDS slot private to each lane (row 0 is zero), so the
-// per-input select is one indexed LDS read.  The four waves read the same key
-// rows, which the L1 cache serves once per block.
+// [i][t][v-1][2N] u64 (v = 1..15).
+// Work split: a block is 4 waves over the same 64 output coefficients, each
+// wave owning PF_CT inputs, and blockIdx.z selects a slice of the i range.
+// Slices accumulate into the zero-filled output with 64-bit atomic adds
+// (addition mod 2^64 is order-free), so the key is still read once per
+// (coefficient tile, input group) while the grid grows by the slice count.
+// The 16 candidate rows of an (i, t) pair sit in an LDS slot private to each
+// lane (row 0 is zero), so the per-input select is one indexed LDS read; the
+// four waves read the same key rows, which the L1 cache serves once.
 // ===========================================================================
-constexpr int PF_CT = 32, PF_ICH = 64;
+constexpr int PF_CT = 32, PF_ICH = 64, PF_SLICES = 8;
 
 __global__ __launch_bounds__(256) void k_pfks(const uint64_t* __restrict__ lwe64, uint32_t B,
                                              const uint64_t* __restrict__ key, uint64_t* __restrict__ out,
@@ -427,17 +431,20 @@ __global__ __launch_bounds__(256) void k_pfks(const uint64_t* __restrict__ lwe64
     __shared__ uint64_t rows[4][16][64];
     __shared__ uint32_t atl[4][PF_CT][PF_ICH + 1];
     constexpr int N = CB_N, COLS = 2 * CB_N;
+    constexpr int ISLICE = (N + 1 + PF_SLICES - 1) / PF_SLICES;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int col = blockIdx.x * 64 + lane;
     const uint32_t cbase = blockIdx.y * (4 * PF_CT) + w * PF_CT;
+    const int ibeg = blockIdx.z * ISLICE;
+    const int iend = (ibeg + ISLICE) < (N + 1) ? (ibeg + ISLICE) : (N + 1);
     if (cbase >= B) return;  // whole wave idle; no block-level barriers below
     uint64_t acc[PF_CT];
 #pragma unroll
     for (int k = 0; k < PF_CT; ++k) acc[k] = 0;
     rows[w][0][lane] = 0;
-    for (int i0 = 0; i0 <= N; i0 += PF_ICH) {
-        const int ich = (N + 1 - i0) < PF_ICH ? (N + 1 - i0) : PF_ICH;
+    for (int i0 = ibeg; i0 < iend; i0 += PF_ICH) {
+        const int ich = (iend - i0) < PF_ICH ? (iend - i0) : PF_ICH;
         wave_sync();
         for (int e = lane; e < PF_CT * PF_ICH; e += 64) {
             const int k = e / PF_ICH, ii = e % PF_ICH;
@@ -451,21 +458,29 @@ __global__ __launch_bounds__(256) void k_pfks(const uint64_t* __restrict__ lwe64
         wave_sync();
         for (int ii = 0; ii < ich; ++ii) {
             const uint64_t* kr = key + (size_t)(i0 + ii) * 8 * 15 * COLS + col;
+            uint32_t dw[PF_CT];
+#pragma unroll
+            for (int k = 0; k < PF_CT; ++k) dw[k] = atl[w][k][ii];
 #pragma unroll 1
             for (int t = 0; t < 8; ++t) {
 #pragma unroll
                 for (int v = 0; v < 15; ++v) rows[w][v + 1][lane] = kr[(size_t)(t * 15 + v) * COLS];
 #pragma unroll
-                for (int k = 0; k < PF_CT; ++k) {
-                    const uint32_t d = (atl[w][k][ii] >> (4 * t)) & 15u;
-                    acc[k] -= rows[w][d][lane];
-                }
+                for (int k = 0; k < PF_CT; ++k) acc[k] -= rows[w][(dw[k] >> (4 * t)) & 15u][lane];
             }
         }
     }
 #pragma unroll
     for (int k = 0; k < PF_CT; ++k)
-        if (cbase + k < B) out[(size_t)(cbase + k) * out_stride + col] = acc[k];
+        if (cbase + k < B) atomicAdd((unsigned long long*)&out[(size_t)(cbase + k) * out_stride + col],
+                                     (unsigned long long)acc[k]);
+}
+
+// zero `count` rows of `width` u64 spaced `stride` apart
+__global__ void k_zero_rows64(uint64_t* __restrict__ out, uint32_t count, size_t stride, uint32_t width) {
+    const size_t total = (size_t)count * width;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x)
+        out[(e / width) * stride + (e % width)] = 0;
 }
 
 // ===========================================================================
@@ -664,7 +679,9 @@ void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, c
 
 void launch_pfks(const uint64_t* lwe64, uint32_t B, const uint64_t* key_u, uint64_t* out, size_t out_stride,
                  hipStream_t st) {
-    dim3 grid(2 * CB_N / 64, (B + 4 * PF_CT - 1) / (4 * PF_CT));
+    const size_t total = (size_t)B * 2 * CB_N;
+    k_zero_rows64<<<grid_for((total + 255) / 256, 4096), 256, 0, st>>>(out, B, out_stride, 2 * CB_N);
+    dim3 grid(2 * CB_N / 64, (B + 4 * PF_CT - 1) / (4 * PF_CT), PF_SLICES);
     k_pfks<<<grid, 256, 0, st>>>(lwe64, B, key_u, out, out_stride);
 }
 
