@@ -50,7 +50,7 @@ def cpu_baseline(sample=24):
 
 
 def aes_ctr_bench(batches, reps=1):
-    """AES-128 CTR keystream (configs[4]): circuit bootstrap + S-box LUT per round,
+    """AES-128 CTR keystream: circuit bootstrap + S-box LUT per round,
     linear layers as LWE additions; every block decrypted and compared with plain AES."""
     import numpy as np
     import torch
@@ -198,7 +198,7 @@ def main():
         torch.cuda.synchronize()
 
     traffic = None
-    prof = os.path.join(ROOT, "profiles", "gate_blind_rotate_c1_traffic.json")
+    prof = os.path.join(ROOT, "profiles", "gate_blind_rotate_4096_nand_xor_traffic.json")
     if os.path.exists(prof):
         traffic = json.load(open(prof)).get("hbm_bytes_per_launch")
 
@@ -216,7 +216,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "batched gate bootstraps (configs[1]): 4096 = 2048 NAND + 2048 XOR per step",
+        "config": {"workload": "batched gate bootstraps: 4096 = 2048 NAND + 2048 XOR per step",
                    "n": N_LWE, "N": N_RING, "bsk_gadget": "2^7 x 3", "ks_gadget": "2^2 x 8",
                    "torus_bits": 32, "fft": "fp64 negacyclic, wavefront-resident", "batch": BATCH},
         "roofline": {"bound": "valu", "kernel": "k_gate_blind_rotate",
