@@ -18,7 +18,8 @@ constexpr int kN = 2048;
 bool cbs_params_supported(const tfhe_cbs_params* p) {
     return p && p->N == kN && p->bk_logB == 9 && p->bk_l == 4 && p->cb_logB == 8 && p->cb_l == 2 && p->pf_logB == 4 &&
            p->pf_l == 8 && p->ks_logB == 2 && p->ks_l == 8 && p->n > 0 && p->n <= 1024 && p->lwe_nshift >= 1 &&
-           p->lwe_nshift <= 31 && p->rlwe_nshift >= -31 && p->rlwe_nshift <= 31;
+           p->lwe_nshift <= 31 && p->rlwe_nshift >= -31 && p->rlwe_nshift <= 31 && p->pf_nshift >= 1 &&
+           p->pf_nshift <= 31;
 }
 
 size_t bsk2_cplx(const tfhe_cbs_params& p) { return (size_t)p.n * 8 * 2 * 2 * 1024; }
@@ -98,17 +99,18 @@ struct tfhe_cbs_ctx {
     uint64_t seed = 0;
     int32_t* d_K2 = nullptr;
     cplx* d_bsk2f = nullptr;
-    uint64_t* d_pfks = nullptr;  // [2][N+1][8][15][2N]
+    uint32_t* d_pfks = nullptr;  // [2][N+1][8][15][2N] torus-32
     uint32_t* d_ksk2 = nullptr;
     uint64_t* d_lwe64 = nullptr;  // [max_batch][2][N+1]
+    uint32_t* d_at = nullptr;     // [N+1][pfks_bpad(2 max_batch)] rounded, transposed PFKS inputs
     DeviceTables1024 tables;
     // AES state
     uint32_t max_blocks = 0;
-    uint64_t* d_rgsw = nullptr;   // [128 max_blocks][4][2][N]
-    cplx* d_rgsw_f = nullptr;     // [128 max_blocks][8][2][1024]
+    uint32_t* d_rgsw = nullptr;   // [128 max_blocks][4][2][N] torus-32
+    cplx* d_rgsw_f = nullptr;     // [128 max_blocks][8][1024]
     uint32_t* d_lut32 = nullptr;  // [128 max_blocks][N+1]
     uint32_t* d_state[2] = {nullptr, nullptr};  // [max_blocks][128][n+1]
-    uint64_t* d_sbox_poly = nullptr;            // [N]
+    uint32_t* d_sbox_poly = nullptr;            // [N]
     int16_t* d_lin_mix = nullptr;               // [128][kLinSrc]
     int16_t* d_lin_last = nullptr;
     uint32_t* d_rk = nullptr;                   // [11][128][n+1] encrypted round-key bits
@@ -118,7 +120,7 @@ struct tfhe_cbs_ctx {
 
 extern "C" int tfhe_cbs_ctx_destroy(tfhe_cbs_ctx* c) {
     if (!c) return TFHE_OK;
-    void* bufs[] = {c->d_K2, c->d_bsk2f, c->d_pfks, c->d_ksk2, c->d_lwe64, c->d_rgsw, c->d_rgsw_f, c->d_lut32,
+    void* bufs[] = {c->d_K2, c->d_bsk2f, c->d_pfks, c->d_ksk2, c->d_lwe64, c->d_at, c->d_rgsw, c->d_rgsw_f, c->d_lut32,
                     c->d_state[0], c->d_state[1], c->d_sbox_poly, c->d_lin_mix, c->d_lin_last, c->d_rk, c->d_ctr};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -144,23 +146,24 @@ extern "C" int tfhe_cbs_ctx_create(const tfhe_cbs_params* params, uint32_t max_b
     rc = rc ? rc : dmalloc((void**)&c->d_bsk2f, bsk2_cplx(p) * sizeof(cplx));
     rc = rc ? rc : dmalloc((void**)&c->d_ksk2, ksk2_words(p) * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_lwe64, B * 2 * (kN + 1) * sizeof(uint64_t));
-    rc = rc ? rc : dmalloc((void**)&c->d_rgsw, B * 4 * 2 * kN * sizeof(uint64_t));
-    rc = rc ? rc : dmalloc((void**)&c->d_rgsw_f, B * 8 * 2 * 1024 * sizeof(cplx));
+    rc = rc ? rc : dmalloc((void**)&c->d_at, (size_t)(kN + 1) * pfks_bpad(2 * (uint32_t)B) * sizeof(uint32_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_rgsw, B * 4 * 2 * kN * sizeof(uint32_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_rgsw_f, B * 8 * 1024 * sizeof(cplx));
     rc = rc ? rc : dmalloc((void**)&c->d_lut32, B * (kN + 1) * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_state[0], B * (p.n + 1) * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_state[1], B * (p.n + 1) * sizeof(uint32_t));
-    rc = rc ? rc : dmalloc((void**)&c->d_sbox_poly, kN * sizeof(uint64_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_sbox_poly, kN * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_lin_mix, 128 * kLinSrc * sizeof(int16_t));
     rc = rc ? rc : dmalloc((void**)&c->d_lin_last, 128 * kLinSrc * sizeof(int16_t));
     rc = rc ? rc : dmalloc((void**)&c->d_rk, (size_t)11 * 128 * (p.n + 1) * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_ctr, (size_t)max_blocks * 16);
     rc = rc ? rc : c->tables.init();
     if (!rc) {
-        // S-box table packed for vertical packing: coefficient 8x + j = bit j of S(x), times 2^63
-        std::vector<uint64_t> poly(kN, 0);
+        // S-box table packed for vertical packing: coefficient 8x + j = bit j of S(x), times 2^31
+        std::vector<uint32_t> poly(kN, 0);
         for (int x = 0; x < 256; ++x)
-            for (int j = 0; j < 8; ++j) poly[8 * x + j] = (uint64_t)((sbox((uint8_t)x) >> j) & 1) << 63;
-        rc = hipcheck(hipMemcpy(c->d_sbox_poly, poly.data(), kN * sizeof(uint64_t), hipMemcpyHostToDevice), "sbox");
+            for (int j = 0; j < 8; ++j) poly[8 * x + j] = (uint32_t)((sbox((uint8_t)x) >> j) & 1) << 31;
+        rc = hipcheck(hipMemcpy(c->d_sbox_poly, poly.data(), kN * sizeof(uint32_t), hipMemcpyHostToDevice), "sbox");
     }
     if (!rc) {
         // source bit lists of the GF(2)-linear ShiftRows(+MixColumns) map, bit = 8*byte + j
@@ -225,7 +228,7 @@ extern "C" int tfhe_cbs_keygen(tfhe_cbs_ctx* c, const int32_t* lwe_sk, uint64_t 
     c->K2.resize(kN);
     tfhe_rlwe_keygen(kN, seed, DOM_RLWE2_SK, c->K2.data());
     int rc = TFHE_OK;
-    if (!c->d_pfks) rc = dmalloc((void**)&c->d_pfks, 2 * pfks_words_per_u() * sizeof(uint64_t));
+    if (!c->d_pfks) rc = dmalloc((void**)&c->d_pfks, 2 * pfks_words_per_u() * sizeof(uint32_t));
     int32_t* d_s = nullptr;
     uint64_t *d_bsk = nullptr, *d_K64 = nullptr;
     cplx* d_K2f = nullptr;
@@ -242,7 +245,7 @@ extern "C" int tfhe_cbs_keygen(tfhe_cbs_ctx* c, const int32_t* lwe_sk, uint64_t 
         launch_to_fourier2048(d_K64, 1, c->tables.T, d_K2f, nullptr);
         launch_gen_bsk64(seed, p.n, p.rlwe_nshift, d_s, c->d_K2, d_K2f, c->tables.T, d_bsk, nullptr);
         launch_to_fourier2048(d_bsk, (size_t)p.n * 8 * 2, c->tables.T, c->d_bsk2f, nullptr);
-        launch_gen_pfks(seed, p.rlwe_nshift, c->d_K2, d_K2f, c->tables.T, c->d_pfks, nullptr);
+        launch_gen_pfks(seed, p.pf_nshift, c->d_K2, d_K2f, c->tables.T, c->d_pfks, nullptr);
         launch_gen_ksk32(seed, p.n, kN, p.ks_logB, p.ks_l, p.lwe_nshift, d_s, c->d_K2, c->d_ksk2, nullptr, DOM_KSK2);
         rc = check_launch("cbs keygen");
     }
@@ -261,12 +264,12 @@ extern "C" int tfhe_cbs_get_rlwe_sk(const tfhe_cbs_ctx* c, int32_t* out) {
     return TFHE_OK;
 }
 
-extern "C" int tfhe_cbs_get_pfks_entry(const tfhe_cbs_ctx* c, int u, int i, int t, int v, uint64_t* out) {
+extern "C" int tfhe_cbs_get_pfks_entry(const tfhe_cbs_ctx* c, int u, int i, int t, int v, uint32_t* out) {
     if (!c || !out || !c->has_keys) return fail(TFHE_E_STATE, "tfhe_cbs_get_pfks_entry: no keys");
     if (u < 0 || u > 1 || i < 0 || i > kN || t < 0 || t > 7 || v < 1 || v > 15)
         return fail(TFHE_E_PARAM, "tfhe_cbs_get_pfks_entry: index out of range");
     const size_t row = (((size_t)u * (kN + 1) + i) * 8 + t) * 15 + (v - 1);
-    return hipcheck(hipMemcpy(out, c->d_pfks + row * 2 * kN, 2 * kN * sizeof(uint64_t), hipMemcpyDeviceToHost),
+    return hipcheck(hipMemcpy(out, c->d_pfks + row * 2 * kN, 2 * kN * sizeof(uint32_t), hipMemcpyDeviceToHost),
                     "pfks entry");
 }
 
@@ -287,37 +290,46 @@ extern "C" int tfhe_cbs_pbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32
 }
 
 extern "C" int tfhe_cbs_pfks_batch(tfhe_cbs_ctx* c, int u, const uint64_t* d_lwe64, uint32_t count, size_t out_stride,
-                                   uint64_t* d_out, void* stream) {
+                                   uint32_t* d_out, void* stream) {
     int rc = check_cbs(c, count, c ? 2 * c->max_batch : 0);
     if (rc) return rc;
     if (u < 0 || u > 1 || !d_lwe64 || !d_out || out_stride < 2 * (size_t)kN)
         return fail(TFHE_E_PARAM, "pfks: bad argument");
     if (!count) return TFHE_OK;
-    launch_pfks(d_lwe64, count, c->d_pfks + (size_t)u * pfks_words_per_u(), d_out, out_stride, (hipStream_t)stream);
+    hipStream_t st = (hipStream_t)stream;
+    launch_pfks_digits(d_lwe64, count, c->d_at, st);
+    launch_zero_rows32(d_out, count, out_stride, 2 * kN, st);
+    launch_pfks32(c->d_at, count, c->d_pfks, pfks_words_per_u(), u, 1, d_out, out_stride, 0, st);
     return check_launch("pfks");
 }
 
-extern "C" int tfhe_cbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t batch, uint64_t* d_rgsw, void* stream) {
+extern "C" int tfhe_cbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t batch, uint32_t* d_rgsw, void* stream) {
     int rc = check_cbs(c, batch, c ? c->max_batch : 0);
     if (rc) return rc;
     if (!d_lwe || !d_rgsw) return fail(TFHE_E_PARAM, "cbs: NULL buffer");
     if (!batch) return TFHE_OK;
+    hipStream_t st = (hipStream_t)stream;
     rc = tfhe_cbs_pbs_batch(c, d_lwe, batch, c->d_lwe64, stream);
-    // LWE L = 2 input + j -> RGSW row 2j + u of input: out stride 4N u64 (two rows)
-    for (int u = 0; u < 2 && !rc; ++u)
-        rc = tfhe_cbs_pfks_batch(c, u, c->d_lwe64, 2 * batch, 4 * (size_t)kN, d_rgsw + (size_t)u * 2 * kN, stream);
-    return rc;
+    if (rc) return rc;
+    // LWE L = 2 input + j -> RGSW row 2j + u of that input, at L * 4N + u * 2N:
+    // both functions in one launch over the rounded, transposed inputs
+    const uint32_t count = 2 * batch;
+    launch_pfks_digits(c->d_lwe64, count, c->d_at, st);
+    launch_zero_rows32(d_rgsw, 1, 0, (uint32_t)((size_t)count * 4 * kN), st);
+    launch_pfks32(c->d_at, count, c->d_pfks, pfks_words_per_u(), 0, 2, d_rgsw, 4 * (size_t)kN, 2 * (size_t)kN, st);
+    return check_launch("cbs pfks");
 }
 
-extern "C" int tfhe_rgsw_to_fourier(tfhe_cbs_ctx* c, const uint64_t* d_rgsw, uint32_t count, double* d_rgsw_f,
+extern "C" int tfhe_rgsw_to_fourier(tfhe_cbs_ctx* c, const uint32_t* d_rgsw, uint32_t count, double* d_rgsw_f,
                                     void* stream) {
     if (!c || !d_rgsw || !d_rgsw_f) return fail(TFHE_E_PARAM, "rgsw_to_fourier: bad argument");
     if (!count) return TFHE_OK;
-    launch_to_fourier2048(d_rgsw, (size_t)count * 8, c->tables.T, reinterpret_cast<cplx*>(d_rgsw_f), (hipStream_t)stream);
-    return check_launch("to_fourier2048");
+    launch_to_fourier2048_32(d_rgsw, (size_t)count * 8, c->tables.T, reinterpret_cast<cplx*>(d_rgsw_f),
+                             (hipStream_t)stream);
+    return check_launch("to_fourier2048_32");
 }
 
-extern "C" int tfhe_lut8_eval_batch(tfhe_cbs_ctx* c, const double* d_rgsw_f, uint32_t nlut, const uint64_t* d_table,
+extern "C" int tfhe_lut8_eval_batch(tfhe_cbs_ctx* c, const double* d_rgsw_f, uint32_t nlut, const uint32_t* d_table,
                                     size_t table_stride, int out_bits, uint32_t* d_out32, void* stream) {
     if (!c || !d_rgsw_f || !d_table || !d_out32 || out_bits < 1 || out_bits > 8)
         return fail(TFHE_E_PARAM, "lut8: bad argument");

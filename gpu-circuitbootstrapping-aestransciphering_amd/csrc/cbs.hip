@@ -1,9 +1,10 @@
-// cbs.hip -- gfx950 kernels of the circuit-bootstrapping level (N = 2048,
-// torus-64): 2-limb Fourier conversion, key generation, the multi-value
-// programmable bootstrap, private functional key switching, the CMux vertical
-// packing LUT and the level-2 -> level-0 key switch.  All products of a digit
-// polynomial with a torus-64 polynomial are two FP64 products (one per signed
-// 32-bit limb), rounded and recombined mod 2^64, which makes them exact.
+// cbs.hip -- gfx950 kernels of the circuit-bootstrapping level (N = 2048):
+// Fourier conversions, key generation, the multi-value programmable bootstrap
+// (torus-64), private functional key switching to torus-32 RLWE, the torus-32
+// CMux vertical-packing LUT and the AES linear layers.  A product of a digit
+// polynomial with a torus-64 polynomial is two FP64 products (one per signed
+// 32-bit limb), rounded and recombined mod 2^64; with a torus-32 polynomial it
+// is one FP64 product.  Both are exact.
 #include "fft1024.hpp"
 #include "kernels.hpp"
 
@@ -115,6 +116,34 @@ __global__ __launch_bounds__(256) void k_to_fourier2048(const uint64_t* __restri
     }
 }
 
+// Fourier conversion of torus-32 polynomials: u32 -> [poly][q][s][lane] (1024 cplx)
+__global__ __launch_bounds__(256) void k_to_fourier2048_32(const uint32_t* __restrict__ coef, size_t npoly,
+                                                          FFT1024Tables Tg, cplx* __restrict__ out) {
+    __shared__ cplx tl[768];
+    __shared__ cplx lds_all[4][F256_LDS_CPLX];
+    __shared__ cplx xbuf[2][1024];
+    const FFT1024Tables T = stage_tables(Tg, tl);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    cplx trw[4];
+    load_trw(Tg, w, lane, trw);
+    int buf = 0;
+    for (size_t q = blockIdx.x; q < npoly; q += gridDim.x) {
+        const uint32_t* p = coef + q * CB_N;
+        cplx x[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int c = own_coeff(w, lane, a);
+            x[a] = {(double)(int32_t)p[c], (double)(int32_t)p[c + CB_M]};
+        }
+        f1024_fwd(x, lds_all[w], xbuf[buf], T, trw, w, lane);
+        buf ^= 1;
+        cplx* o = out + q * CB_M;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) o[(w * 4 + s) * 64 + lane] = x[s];
+    }
+}
+
 // Exact negacyclic product of small-integer x by torus-64 y (N = 2048), one block per product.
 __global__ __launch_bounds__(256) void k_polymul2048(const int32_t* __restrict__ x, const uint64_t* __restrict__ y,
                                                     uint32_t batch, FFT1024Tables Tg, uint64_t* __restrict__ out) {
@@ -162,21 +191,14 @@ __global__ __launch_bounds__(256) void k_polymul2048(const int32_t* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Key generation over Z/2^64: one block per RLWE row; a * K2 is exact through
+// BSK2 generation over Z/2^64: one block per RLWE row; a * K2 is exact through
 // two limb products with the precomputed Fourier transform of K2.
-//   KIND_BSK : row id = i*8 + r; msg = s_i * 2^(28 + 9j) on component u of coefficient 0
-//   KIND_PFKS: row id = ((u*(N+1) + i)*8 + t)*16 + v (v >= 1);
-//              msg_b = f_u(v * kappa_i * 2^(32 + 4t)), kappa_N = -1,
-//              f_1(x) = x (coefficient 0), f_0(x) = -K2(X) x
-// Output row layout [2][N] (a then b); PFKS rows are stored densely over v = 1..15.
+// Row id = i*8 + r; msg = s_i * 2^(28 + 9j) on component u of coefficient 0
+// (r = 2j + u).  Output row layout [2][N] (a then b).
 // ---------------------------------------------------------------------------
-enum { KIND_BSK = 0, KIND_PFKS = 1 };
-
-template <int KIND>
-__global__ __launch_bounds__(256) void k_gen_rlwe64(uint64_t seed, uint32_t nrows, int nshift,
-                                                   const int32_t* __restrict__ s, const int32_t* __restrict__ K2,
-                                                   const cplx* __restrict__ K2f, FFT1024Tables Tg,
-                                                   uint64_t* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_gen_bsk64(uint64_t seed, uint32_t nrows, int nshift,
+                                                  const int32_t* __restrict__ s, const cplx* __restrict__ K2f,
+                                                  FFT1024Tables Tg, uint64_t* __restrict__ out) {
     __shared__ cplx tl[768];
     __shared__ cplx lds_all[4][F256_LDS_CPLX];
     __shared__ cplx xbuf[2][1024];
@@ -189,35 +211,17 @@ __global__ __launch_bounds__(256) void k_gen_rlwe64(uint64_t seed, uint32_t nrow
     for (int sl = 0; sl < 4; ++sl) kf[sl] = K2f[(w * 4 + sl) * 64 + lane];
     int buf = 0;
     for (uint32_t row = blockIdx.x; row < nrows; row += gridDim.x) {
-        uint32_t id, dom;
-        int msg_comp = 1;
-        uint64_t cval = 0;
-        bool poly_msg = false;  // f_0: msg_b = -K2 * cval
-        if constexpr (KIND == KIND_BSK) {
-            const int i = row / 8, r = row % 8, j = r >> 1;
-            msg_comp = r & 1;
-            cval = (uint64_t)s[i] << (28 + 9 * j);
-            id = row;
-            dom = DOM_BSK2;
-        } else {
-            const uint32_t v1 = row % 15, rest = row / 15;  // v = v1 + 1
-            const int t = rest % 8;
-            const uint32_t rest2 = rest / 8;
-            const int i = rest2 % (CB_N + 1);
-            const int u = rest2 / (CB_N + 1);
-            const uint64_t kappa = i < CB_N ? (uint64_t)(int64_t)K2[i] : ~0ull;
-            cval = (uint64_t)(v1 + 1) * kappa * (1ull << (32 + 4 * t));
-            poly_msg = (u == 0);
-            id = ((rest2 * 8) + t) * 16 + (v1 + 1);
-            dom = DOM_PFKS;
-        }
+        const int i = row / 8, r = row % 8, j = r >> 1;
+        const int msg_comp = r & 1;
+        const uint64_t cval = (uint64_t)s[i] << (28 + 9 * j);
+        const uint32_t id = row;
         uint64_t av[8];
         cplx A0[4], A1[4];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
             const int c = own_coeff(w, lane, a);
-            av[a] = rand64(seed, dom, id, (uint32_t)c);
-            av[a + 4] = rand64(seed, dom, id, (uint32_t)(c + CB_M));
+            av[a] = rand64(seed, DOM_BSK2, id, (uint32_t)c);
+            av[a + 4] = rand64(seed, DOM_BSK2, id, (uint32_t)(c + CB_M));
             double l0r, l1r, l0i, l1i;
             split_limbs(av[a], l0r, l1r);
             split_limbs(av[a + 4], l0i, l1i);
@@ -241,17 +245,71 @@ __global__ __launch_bounds__(256) void k_gen_rlwe64(uint64_t seed, uint32_t nrow
             const int c = own_coeff(w, lane, a) + (h >= 4 ? CB_M : 0);
             const double re0 = h < 4 ? A0[a].re : A0[a].im, re1 = h < 4 ? A1[a].re : A1[a].im;
             uint64_t b = (uint64_t)round_i64(re0, 1.0 / CB_M) + ((uint64_t)round_i64(re1, 1.0 / CB_M) << 32);
-            b += (uint64_t)noise(seed, dom, id, (uint32_t)c, nshift);
+            b += (uint64_t)noise(seed, DOM_BSK2, id, (uint32_t)c, nshift);
             uint64_t aval = av[h];
-            if constexpr (KIND == KIND_BSK) {
-                if (c == 0) {
-                    if (msg_comp == 0) aval += cval; else b += cval;
-                }
-            } else {
-                if (poly_msg) b -= (uint64_t)(int64_t)K2[c] * cval;
-                else if (c == 0) b += cval;
+            if (c == 0) {
+                if (msg_comp == 0) aval += cval; else b += cval;
             }
             oa[c] = aval;
+            ob[c] = b;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PFKS key generation over Z/2^32 (one block per RLWE row): row index
+// ((u*(N+1) + i)*8 + t)*15 + (v-1), v = 1..15, Philox id ((u*(N+1) + i)*8 + t)*16 + v;
+// a = rand32, b = a*K2 + noise + msg_b with msg_b = f_u(v * kappa_i * 2^(4t)),
+// kappa_N = -1, f_1(x) = x (coefficient 0), f_0(x) = -K2(X) x.  a*K2 is one
+// exact FP64 product (|a| < 2^31, K2 binary).  Output row layout [2][N] u32.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gen_pfks32(uint64_t seed, uint32_t nrows, int nshift,
+                                                   const int32_t* __restrict__ K2, const cplx* __restrict__ K2f,
+                                                   FFT1024Tables Tg, uint32_t* __restrict__ out) {
+    __shared__ cplx tl[768];
+    __shared__ cplx lds_all[4][F256_LDS_CPLX];
+    __shared__ cplx xbuf[2][1024];
+    const FFT1024Tables T = stage_tables(Tg, tl);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    cplx trw[4], kf[4];
+    load_trw(Tg, w, lane, trw);
+#pragma unroll
+    for (int sl = 0; sl < 4; ++sl) kf[sl] = K2f[(w * 4 + sl) * 64 + lane];
+    int buf = 0;
+    for (uint32_t row = blockIdx.x; row < nrows; row += gridDim.x) {
+        const uint32_t v1 = row % 15, rest = row / 15;  // v = v1 + 1
+        const int t = rest % 8;
+        const uint32_t rest2 = rest / 8;
+        const int i = rest2 % (CB_N + 1);
+        const int u = rest2 / (CB_N + 1);
+        const uint32_t kappa = i < CB_N ? (uint32_t)K2[i] : 0xFFFFFFFFu;
+        const uint32_t cval = ((v1 + 1) * kappa) << (4 * t);
+        const uint32_t id = ((rest2 * 8) + t) * 16 + (v1 + 1);
+        uint32_t av[8];
+        cplx A[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int c = own_coeff(w, lane, a);
+            av[a] = rand32(seed, DOM_PFKS, id, (uint32_t)c);
+            av[a + 4] = rand32(seed, DOM_PFKS, id, (uint32_t)(c + CB_M));
+            A[a] = {(double)(int32_t)av[a], (double)(int32_t)av[a + 4]};
+        }
+        f1024_fwd(A, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+#pragma unroll
+        for (int sl = 0; sl < 4; ++sl) A[sl] = cmul(A[sl], kf[sl]);
+        f1024_inv(A, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        uint32_t* oa = out + (size_t)row * 2 * CB_N;
+        uint32_t* ob = oa + CB_N;
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+            const int a = h & 3;
+            const int c = own_coeff(w, lane, a) + (h >= 4 ? CB_M : 0);
+            uint32_t b = (uint32_t)round_i64(h < 4 ? A[a].re : A[a].im, 1.0 / CB_M);
+            b += (uint32_t)noise(seed, DOM_PFKS, id, (uint32_t)c, nshift);
+            if (u == 0) b -= (uint32_t)K2[c] * cval;
+            else if (c == 0) b += cval;
+            oa[c] = av[h];
             ob[c] = b;
         }
     }
@@ -409,99 +467,141 @@ __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict
 }
 
 // ===========================================================================
-// Private functional key switching LWE_K2 -> RLWE_K2 (CGGI PrivKS) for one
-// function u:  out = - sum_{i<=N} sum_{t<8} KS_u[i][t][digit_t(a~_i)], a_N = b,
-// a~_i = round(a_i / 2^32) (top 32 bits), unsigned base-16 digits, least
+// Private functional key switching LWE_K2 (torus-64) -> RLWE_K2 (torus-32),
+// CGGI PrivKS, for function u:
+//   out = - sum_{i<=N} sum_{t<8} KS_u[i][t][digit_t(a~_i)]  (mod 2^32),
+// a_N = b, a~_i = round(a_i / 2^32), unsigned base-16 digits, least
 // significant first; digit 0 contributes nothing.  Key layout
-// [i][t][v-1][2N] u64 (v = 1..15).
-// Work split: a block is 4 waves over the same 64 output coefficients, each
-// wave owning PF_CT inputs, and blockIdx.z selects a slice of the i range.
-// Slices accumulate into the zero-filled output with 64-bit atomic adds
-// (addition mod 2^64 is order-free), so the key is still read once per
-// (coefficient tile, input group) while the grid grows by the slice count.
-// The 16 candidate rows of an (i, t) pair sit in an LDS slot private to each
-// lane (row 0 is zero), so the per-input select is one indexed LDS read; the
-// four waves read the same key rows, which the L1 cache serves once.
+// [u][i][t][v-1][2N] u32 (v = 1..15).
+//
+// k_pfks_digits rounds the inputs and transposes them to [i][Bpad] so that
+// one i-row of a wave's PF_CT inputs is a uniform (scalar-load) read.
+// k_pfks32: a workgroup of PF_WAVES waves shares one 64-column tile of the
+// key; per (i, t) the 15 candidate rows are loaded once by the workgroup into
+// an LDS table (row 0 = 0, double-buffered, one barrier per step) and every
+// wave selects, for each of its PF_CT inputs, the row named by that input's
+// digit: one LDS read and one subtraction per (input, column).  blockIdx.z
+// picks an i-slice and the function; slices add into the zero-filled output
+// with 32-bit atomics (addition mod 2^32 is order-free, so the result is
+// exact).  The key is read once per (column tile, input group) from L2, and
+// the workgroups of one key tile share an XCD (their linear ids differ by
+// multiples of 64), so HBM sees it about once.
 // ===========================================================================
-constexpr int PF_CT = 32, PF_ICH = 64, PF_SLICES = 8;
+constexpr int PF_WAVES = 8, PF_CT = 32, PF_SLICES = 8, PF_GROUP = PF_WAVES * PF_CT;
 
-__global__ __launch_bounds__(256) void k_pfks(const uint64_t* __restrict__ lwe64, uint32_t B,
-                                             const uint64_t* __restrict__ key, uint64_t* __restrict__ out,
-                                             size_t out_stride) {
-    __shared__ uint64_t rows[4][16][64];
-    __shared__ uint32_t atl[4][PF_CT][PF_ICH + 1];
-    constexpr int N = CB_N, COLS = 2 * CB_N;
-    constexpr int ISLICE = (N + 1 + PF_SLICES - 1) / PF_SLICES;
-    const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int col = blockIdx.x * 64 + lane;
-    const uint32_t cbase = blockIdx.y * (4 * PF_CT) + w * PF_CT;
-    const int ibeg = blockIdx.z * ISLICE;
-    const int iend = (ibeg + ISLICE) < (N + 1) ? (ibeg + ISLICE) : (N + 1);
-    if (cbase >= B) return;  // whole wave idle; no block-level barriers below
-    uint64_t acc[PF_CT];
-#pragma unroll
-    for (int k = 0; k < PF_CT; ++k) acc[k] = 0;
-    rows[w][0][lane] = 0;
-    for (int i0 = ibeg; i0 < iend; i0 += PF_ICH) {
-        const int ich = (iend - i0) < PF_ICH ? (iend - i0) : PF_ICH;
-        wave_sync();
-        for (int e = lane; e < PF_CT * PF_ICH; e += 64) {
-            const int k = e / PF_ICH, ii = e % PF_ICH;
-            uint32_t v = 0;
-            if (cbase + k < B && ii < ich) {
-                const uint64_t a = lwe64[(size_t)(cbase + k) * (N + 1) + i0 + ii];
-                v = (uint32_t)((a + (1ull << 31)) >> 32);
-            }
-            atl[w][k][ii] = v;
-        }
-        wave_sync();
-        for (int ii = 0; ii < ich; ++ii) {
-            const uint64_t* kr = key + (size_t)(i0 + ii) * 8 * 15 * COLS + col;
-            uint32_t dw[PF_CT];
-#pragma unroll
-            for (int k = 0; k < PF_CT; ++k) dw[k] = atl[w][k][ii];
-#pragma unroll 1
-            for (int t = 0; t < 8; ++t) {
-#pragma unroll
-                for (int v = 0; v < 15; ++v) rows[w][v + 1][lane] = kr[(size_t)(t * 15 + v) * COLS];
-#pragma unroll
-                for (int k = 0; k < PF_CT; ++k) acc[k] -= rows[w][(dw[k] >> (4 * t)) & 15u][lane];
-            }
-        }
+__global__ __launch_bounds__(256) void k_pfks_digits(const uint64_t* __restrict__ lwe64, uint32_t B, uint32_t Bpad,
+                                                    uint32_t* __restrict__ at) {
+    __shared__ uint32_t tile[64][65];
+    constexpr int N = CB_N;
+    const int i0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) {  // r = input offset, tx = i offset
+        const uint32_t k = k0 + r;
+        const int i = i0 + tx;
+        uint32_t v = 0;
+        if (k < B && i <= N) v = (uint32_t)((lwe64[(size_t)k * (N + 1) + i] + (1ull << 31)) >> 32);
+        tile[r][tx] = v;
     }
-#pragma unroll
-    for (int k = 0; k < PF_CT; ++k)
-        if (cbase + k < B) atomicAdd((unsigned long long*)&out[(size_t)(cbase + k) * out_stride + col],
-                                     (unsigned long long)acc[k]);
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {  // r = i offset, tx = input offset
+        const int i = i0 + r;
+        const uint32_t k = k0 + tx;
+        if (i <= N && k < Bpad) at[(size_t)i * Bpad + k] = tile[tx][r];
+    }
 }
 
-// zero `count` rows of `width` u64 spaced `stride` apart
-__global__ void k_zero_rows64(uint64_t* __restrict__ out, uint32_t count, size_t stride, uint32_t width) {
+__global__ __launch_bounds__(64 * PF_WAVES) void k_pfks32(const uint32_t* __restrict__ at, uint32_t B, uint32_t Bpad,
+                                                         const uint32_t* __restrict__ key, size_t key_u_stride,
+                                                         int u0, uint32_t* __restrict__ out, size_t out_stride,
+                                                         size_t out_u_offset) {
+    __shared__ uint32_t tab[2][16][64];
+    constexpr int N = CB_N, COLS = 2 * CB_N;
+    constexpr int ISLICE = (N + 1 + PF_SLICES - 1) / PF_SLICES;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int slice = blockIdx.z % PF_SLICES;
+    const int u = u0 + (int)(blockIdx.z / PF_SLICES);
+    const int ibeg = slice * ISLICE;
+    const int iend = (ibeg + ISLICE) < (N + 1) ? (ibeg + ISLICE) : (N + 1);
+    const uint32_t cbase = blockIdx.y * PF_GROUP + w * PF_CT;
+    const bool active = cbase < B;  // idle waves still take part in the barriers
+    // this thread's share of a tile: candidate v = 1 + w (all waves) and v = 9 + w (waves 0..6)
+    const uint32_t* kcol = key + (size_t)u * key_u_stride + blockIdx.x * 64 + lane;
+    const bool second = w < 7;
+    if (tid < 64) {
+        tab[0][0][tid] = 0;
+        tab[1][0][tid] = 0;
+    }
+    uint32_t acc[PF_CT];
+#pragma unroll
+    for (int k = 0; k < PF_CT; ++k) acc[k] = 0;
+    uint32_t kv0 = 0, kv1 = 0;
+    auto load_tile = [&](int i, int t) {
+        const uint32_t* kr = kcol + ((size_t)(i * 8 + t) * 15 + w) * COLS;
+        kv0 = kr[0];
+        if (second) kv1 = kr[(size_t)8 * COLS];
+    };
+    auto store_tile = [&](int b) {
+        tab[b][1 + w][lane] = kv0;
+        if (second) tab[b][9 + w][lane] = kv1;
+    };
+    load_tile(ibeg, 0);
+    store_tile(0);
+    __syncthreads();
+    for (int i = ibeg; i < iend; ++i) {
+        uint32_t a[PF_CT];
+        if (active) {
+            const uint32_t* ar = at + (size_t)i * Bpad + cbase;
+#pragma unroll
+            for (int k = 0; k < PF_CT; ++k) a[k] = ar[k];
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const bool more = (t < 7) || (i + 1 < iend);
+            if (more) load_tile(t < 7 ? i : i + 1, (t + 1) & 7);
+            if (active) {
+#pragma unroll
+                for (int k = 0; k < PF_CT; ++k) acc[k] -= tab[t & 1][(a[k] >> (4 * t)) & 15u][lane];
+            }
+            if (more) store_tile((t + 1) & 1);
+            __syncthreads();
+        }
+    }
+    if (active) {
+        uint32_t* o = out + (size_t)(u - u0) * out_u_offset + blockIdx.x * 64 + lane;
+#pragma unroll
+        for (int k = 0; k < PF_CT; ++k)
+            if (cbase + k < B) atomicAdd(&o[(size_t)(cbase + k) * out_stride], acc[k]);
+    }
+}
+
+// zero `count` rows of `width` u32 spaced `stride` apart
+__global__ void k_zero_rows32(uint32_t* __restrict__ out, uint32_t count, size_t stride, uint32_t width) {
     const size_t total = (size_t)count * width;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x)
         out[(e / width) * stride + (e % width)] = 0;
 }
 
 // ===========================================================================
-// Vertical-packing LUT (reference CMux semantics, rlwe_tables.cuh:79-108):
-// acc = (0, table); for bit i = 0..7: acc = CMux(C_i, acc, X^{-stride 2^i} acc)
-// = acc + C_i (*) (X^{-stride 2^i} acc - acc), gadget 2^8 x 2 on torus-64.
-// Then coefficients 0..out_bits-1 are sample-extracted and switched to the
-// 32-bit torus (round half up): out [lut][out_bits][N+1] u32, ready for the
-// N -> n key switch.  rgsw_f [lut][8][4 rows][2 out][2 limbs][1024] cplx.
+// Vertical-packing LUT on the 32-bit torus (reference CMux semantics,
+// rlwe_tables.cuh:79-108): acc = (0, table); for bit i = 0..7:
+// acc = CMux(C_i, acc, X^{-stride 2^i} acc) = acc + C_i (*) (X^{-stride 2^i} acc - acc),
+// gadget 2^8 x 2 (16 bits dropped).  Each digit polynomial times a torus-32
+// RGSW row is one exact FP64 product.  Then coefficients 0..out_bits-1 are
+// sample-extracted: out [lut][out_bits][N+1] u32, ready for the N -> n key
+// switch.  rgsw_f [lut][8][4 rows][2 out][1024] cplx.
 // ===========================================================================
-constexpr int CB_LOGB = 8, CB_L = 2, CB_ROWS = 4, CB_IGN = 48;
+constexpr int CB_LOGB = 8, CB_L = 2, CB_ROWS = 4, CB_IGN = 16;
 
-__device__ __forceinline__ uint64_t cb_digit_offset() {
-    uint64_t off = 1ull << (CB_IGN - 1);
+__device__ __forceinline__ uint32_t cb_digit_offset() {
+    uint32_t off = 1u << (CB_IGN - 1);
 #pragma unroll
-    for (int j = 0; j < CB_L; ++j) off += (1ull << (CB_LOGB - 1)) << (CB_IGN + j * CB_LOGB);
+    for (int j = 0; j < CB_L; ++j) off += (1u << (CB_LOGB - 1)) << (CB_IGN + j * CB_LOGB);
     return off;
 }
 
 __global__ __launch_bounds__(256) void k_lut_vertical(const cplx* __restrict__ rgsw_f, uint32_t nlut,
-                                                     const uint64_t* __restrict__ table, size_t table_stride,
+                                                     const uint32_t* __restrict__ table, size_t table_stride,
                                                      int stride, int out_bits, FFT1024Tables Tg,
                                                      uint32_t* __restrict__ out) {
     __shared__ cplx tl[768];
@@ -513,13 +613,13 @@ __global__ __launch_bounds__(256) void k_lut_vertical(const cplx* __restrict__ r
     cplx trw[4];
     load_trw(Tg, w, lane, trw);
     cplx* lds = lds_all[w];
-    uint64_t* x64 = reinterpret_cast<uint64_t*>(&xbuf[0][0]);
-    const uint64_t OFF = cb_digit_offset();
+    uint32_t* x32 = reinterpret_cast<uint32_t*>(&xbuf[0][0]);
+    const uint32_t OFF = cb_digit_offset();
     constexpr uint32_t N = CB_N;
 
     for (uint32_t lut = blockIdx.x; lut < nlut; lut += gridDim.x) {
-        const uint64_t* tab = table + lut * table_stride;
-        uint64_t acc[2][8];
+        const uint32_t* tab = table + lut * table_stride;
+        uint32_t acc[2][8];
 #pragma unroll
         for (int h = 0; h < 8; ++h) {
             acc[0][h] = 0;
@@ -533,76 +633,65 @@ __global__ __launch_bounds__(256) void k_lut_vertical(const cplx* __restrict__ r
 #pragma unroll
             for (int o = 0; o < 2; ++o)
 #pragma unroll
-                for (int h = 0; h < 8; ++h) x64[o * CB_N + own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0)] = acc[o][h];
+                for (int h = 0; h < 8; ++h) x32[o * CB_N + own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0)] = acc[o][h];
             __syncthreads();
-            uint64_t tp[2][8];
+            uint32_t tp[2][8];
 #pragma unroll
             for (int o = 0; o < 2; ++o)
 #pragma unroll
                 for (int h = 0; h < 8; ++h) {
                     const uint32_t src = (uint32_t)(own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0)) + sh;
-                    uint64_t v = x64[o * CB_N + (src & (N - 1))];
-                    if ((src / N) & 1) v = 0 - v;
+                    uint32_t v = x32[o * CB_N + (src & (N - 1))];
+                    if ((src / N) & 1) v = 0u - v;
                     tp[o][h] = v - acc[o][h] + OFF;
                 }
             __syncthreads();
-            const cplx* key = rgsw_f + ((size_t)lut * 8 + bit) * CB_ROWS * 4 * CB_M;
-            cplx macc[2][2][4];
+            const cplx* key = rgsw_f + ((size_t)lut * 8 + bit) * CB_ROWS * 2 * CB_M;
+            cplx macc[2][4];
 #pragma unroll
             for (int o = 0; o < 2; ++o)
 #pragma unroll
-                for (int lb = 0; lb < 2; ++lb)
-#pragma unroll
-                    for (int s = 0; s < 4; ++s) macc[o][lb][s] = {0.0, 0.0};
+                for (int s = 0; s < 4; ++s) macc[o][s] = {0.0, 0.0};
 #pragma unroll
             for (int u = 0; u < 2; ++u)
 #pragma unroll
                 for (int j = 0; j < CB_L; ++j) {
                     const int r = 2 * j + u;
+                    const int sh2 = CB_IGN + j * CB_LOGB;
                     cplx x[4];
 #pragma unroll
-                    for (int a = 0; a < 4; ++a) {
-                        const int sh2 = CB_IGN + j * CB_LOGB;
+                    for (int a = 0; a < 4; ++a)
                         x[a] = {(double)((int32_t)((tp[u][a] >> sh2) & 255u) - 128),
                                 (double)((int32_t)((tp[u][a + 4] >> sh2) & 255u) - 128)};
-                    }
                     f1024_fwd(x, lds, xbuf[buf], T, trw, w, lane);
                     buf ^= 1;
-                    const cplx* kr = key + (size_t)r * 4 * CB_M + (w * 4) * 64 + lane;
+                    const cplx* kr = key + (size_t)r * 2 * CB_M + (w * 4) * 64 + lane;
 #pragma unroll
                     for (int o = 0; o < 2; ++o)
 #pragma unroll
-                        for (int lb = 0; lb < 2; ++lb)
-#pragma unroll
-                            for (int s = 0; s < 4; ++s) cmac(macc[o][lb][s], x[s], kr[(o * 2 + lb) * CB_M + s * 64]);
+                        for (int s = 0; s < 4; ++s) cmac(macc[o][s], x[s], kr[o * CB_M + s * 64]);
                 }
 #pragma unroll
             for (int o = 0; o < 2; ++o) {
-                cplx y0[4], y1[4];
-#pragma unroll
-                for (int s = 0; s < 4; ++s) { y0[s] = macc[o][0][s]; y1[s] = macc[o][1][s]; }
-                f1024_inv(y0, lds, xbuf[buf], T, trw, w, lane);
-                buf ^= 1;
-                f1024_inv(y1, lds, xbuf[buf], T, trw, w, lane);
+                f1024_inv(macc[o], lds, xbuf[buf], T, trw, w, lane);
                 buf ^= 1;
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
-                    acc[o][a] += (uint64_t)round_i64(y0[a].re, 1.0 / CB_M) + ((uint64_t)round_i64(y1[a].re, 1.0 / CB_M) << 32);
-                    acc[o][a + 4] += (uint64_t)round_i64(y0[a].im, 1.0 / CB_M) + ((uint64_t)round_i64(y1[a].im, 1.0 / CB_M) << 32);
+                    acc[o][a] += (uint32_t)round_i64(macc[o][a].re, 1.0 / CB_M);
+                    acc[o][a + 4] += (uint32_t)round_i64(macc[o][a].im, 1.0 / CB_M);
                 }
             }
         }
-        // extract coefficients 0..out_bits-1, switch 2^64 -> 2^32
+        // extract coefficients 0..out_bits-1
         uint32_t* ob = out + (size_t)lut * out_bits * (N + 1);
         for (int idx = 0; idx < out_bits; ++idx) {
             uint32_t* o = ob + (size_t)idx * (N + 1);
 #pragma unroll
             for (int h = 0; h < 8; ++h) {
                 const uint32_t c = (uint32_t)own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0);
-                const uint64_t v = (c <= (uint32_t)idx) ? acc[0][h] : 0 - acc[0][h];
-                o[((uint32_t)idx - c) & (N - 1)] = (uint32_t)((v + (1ull << 31)) >> 32);
+                o[((uint32_t)idx - c) & (N - 1)] = (c <= (uint32_t)idx) ? acc[0][h] : 0u - acc[0][h];
             }
-            if (lane == idx / 4 && w == idx % 4) o[N] = (uint32_t)((acc[1][0] + (1ull << 31)) >> 32);
+            if (lane == idx / 4 && w == idx % 4) o[N] = acc[1][0];
         }
     }
 }
@@ -654,7 +743,7 @@ __global__ void k_xor_public(uint32_t* __restrict__ ct, const uint8_t* __restric
 // ===========================================================================
 static unsigned grid_for(size_t work, size_t cap) { return (unsigned)(work < cap ? work : cap); }
 
-void launch_lut_vertical(const cplx* rgsw_f, uint32_t nlut, const uint64_t* table, size_t table_stride, int stride,
+void launch_lut_vertical(const cplx* rgsw_f, uint32_t nlut, const uint32_t* table, size_t table_stride, int stride,
                          int out_bits, const FFT1024Tables& T, uint32_t* out, hipStream_t st) {
     k_lut_vertical<<<grid_for(nlut, 2048), 256, 0, st>>>(rgsw_f, nlut, table, table_stride, stride, out_bits, T, out);
 }
@@ -677,16 +766,31 @@ void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, c
     k_pbs2_manylut<<<grid_for(batch, 2048), 256, 0, st>>>(lwe, batch, n, bskf, T, out);
 }
 
-void launch_pfks(const uint64_t* lwe64, uint32_t B, const uint64_t* key_u, uint64_t* out, size_t out_stride,
-                 hipStream_t st) {
-    const size_t total = (size_t)B * 2 * CB_N;
-    k_zero_rows64<<<grid_for((total + 255) / 256, 4096), 256, 0, st>>>(out, B, out_stride, 2 * CB_N);
-    dim3 grid(2 * CB_N / 64, (B + 4 * PF_CT - 1) / (4 * PF_CT), PF_SLICES);
-    k_pfks<<<grid, 256, 0, st>>>(lwe64, B, key_u, out, out_stride);
+uint32_t pfks_bpad(uint32_t B) { return (B + PF_GROUP - 1) / PF_GROUP * PF_GROUP; }
+
+void launch_pfks_digits(const uint64_t* lwe64, uint32_t B, uint32_t* at, hipStream_t st) {
+    const uint32_t Bpad = pfks_bpad(B);
+    k_pfks_digits<<<dim3((CB_N + 1 + 63) / 64, Bpad / 64), 256, 0, st>>>(lwe64, B, Bpad, at);
+}
+
+void launch_zero_rows32(uint32_t* out, uint32_t count, size_t stride, uint32_t width, hipStream_t st) {
+    const size_t total = (size_t)count * width;
+    k_zero_rows32<<<grid_for((total + 255) / 256, 8192), 256, 0, st>>>(out, count, stride, width);
+}
+
+void launch_pfks32(const uint32_t* at, uint32_t B, const uint32_t* key, size_t key_u_stride, int u0, int nu,
+                   uint32_t* out, size_t out_stride, size_t out_u_offset, hipStream_t st) {
+    dim3 grid(2 * CB_N / 64, pfks_bpad(B) / PF_GROUP, PF_SLICES * nu);
+    k_pfks32<<<grid, 64 * PF_WAVES, 0, st>>>(at, B, pfks_bpad(B), key, key_u_stride, u0, out, out_stride,
+                                              out_u_offset);
 }
 
 void launch_to_fourier2048(const uint64_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st) {
     k_to_fourier2048<<<grid_for(npoly, 4096), 256, 0, st>>>(coef, npoly, T, out);
+}
+
+void launch_to_fourier2048_32(const uint32_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st) {
+    k_to_fourier2048_32<<<grid_for(npoly, 8192), 256, 0, st>>>(coef, npoly, T, out);
 }
 
 void launch_polymul2048(const int32_t* x, const uint64_t* y, uint32_t batch, const FFT1024Tables& T, uint64_t* out,
@@ -696,14 +800,15 @@ void launch_polymul2048(const int32_t* x, const uint64_t* y, uint32_t batch, con
 
 void launch_gen_bsk64(uint64_t seed, int n, int nshift, const int32_t* s, const int32_t* K2, const cplx* K2f,
                       const FFT1024Tables& T, uint64_t* bsk, hipStream_t st) {
+    (void)K2;
     const uint32_t rows = (uint32_t)n * 8;
-    k_gen_rlwe64<KIND_BSK><<<grid_for(rows, 8192), 256, 0, st>>>(seed, rows, nshift, s, K2, K2f, T, bsk);
+    k_gen_bsk64<<<grid_for(rows, 8192), 256, 0, st>>>(seed, rows, nshift, s, K2f, T, bsk);
 }
 
 void launch_gen_pfks(uint64_t seed, int nshift, const int32_t* K2, const cplx* K2f, const FFT1024Tables& T,
-                     uint64_t* key, hipStream_t st) {
+                     uint32_t* key, hipStream_t st) {
     const uint32_t rows = 2u * (CB_N + 1) * 8 * 15;
-    k_gen_rlwe64<KIND_PFKS><<<grid_for(rows, 8192), 256, 0, st>>>(seed, rows, nshift, nullptr, K2, K2f, T, key);
+    k_gen_pfks32<<<grid_for(rows, 8192), 256, 0, st>>>(seed, rows, nshift, K2, K2f, T, key);
 }
 
 }  // namespace tfhe

@@ -27,7 +27,7 @@ void launch_external_product1024(uint32_t* acc, cplx* digf, uint32_t* t, const c
                                  hipStream_t st);
 int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int logB, int l,
                        const uint32_t* ksk, uint32_t* out, hipStream_t st);
-// circuit-bootstrapping level (N = 2048, torus-64)
+// circuit-bootstrapping level (N = 2048; PBS torus-64, PFKS / RGSW / LUT torus-32)
 struct FFT1024Tables;
 void launch_to_fourier2048(const uint64_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st);
 void launch_polymul2048(const int32_t* x, const uint64_t* y, uint32_t batch, const FFT1024Tables& T, uint64_t* out,
@@ -35,12 +35,18 @@ void launch_polymul2048(const int32_t* x, const uint64_t* y, uint32_t batch, con
 void launch_gen_bsk64(uint64_t seed, int n, int nshift, const int32_t* s, const int32_t* K2, const cplx* K2f,
                       const FFT1024Tables& T, uint64_t* bsk, hipStream_t st);
 void launch_gen_pfks(uint64_t seed, int nshift, const int32_t* K2, const cplx* K2f, const FFT1024Tables& T,
-                     uint64_t* key, hipStream_t st);
+                     uint32_t* key, hipStream_t st);
+void launch_to_fourier2048_32(const uint32_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st);
 void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, const FFT1024Tables& T, uint64_t* out,
                  hipStream_t st);
-void launch_pfks(const uint64_t* lwe64, uint32_t B, const uint64_t* key_u, uint64_t* out, size_t out_stride,
-                 hipStream_t st);
-void launch_lut_vertical(const cplx* rgsw_f, uint32_t nlut, const uint64_t* table, size_t table_stride, int stride,
+// PFKS: inputs rounded and transposed to at [N+1][pfks_bpad(B)] u32, then the key switch for
+// functions u0 .. u0+nu-1 added into zero-filled outputs (out + (u-u0)*out_u_offset, rows out_stride apart)
+uint32_t pfks_bpad(uint32_t B);
+void launch_pfks_digits(const uint64_t* lwe64, uint32_t B, uint32_t* at, hipStream_t st);
+void launch_zero_rows32(uint32_t* out, uint32_t count, size_t stride, uint32_t width, hipStream_t st);
+void launch_pfks32(const uint32_t* at, uint32_t B, const uint32_t* key, size_t key_u_stride, int u0, int nu,
+                   uint32_t* out, size_t out_stride, size_t out_u_offset, hipStream_t st);
+void launch_lut_vertical(const cplx* rgsw_f, uint32_t nlut, const uint32_t* table, size_t table_stride, int stride,
                          int out_bits, const FFT1024Tables& T, uint32_t* out, hipStream_t st);
 void launch_aes_init(const uint32_t* rk0, const uint8_t* ctr, uint32_t blocks, int n, uint32_t* state, hipStream_t st);
 void launch_aes_linear(const uint32_t* in, const uint32_t* rk, const int16_t* src, int nsrc, uint32_t blocks, int n,

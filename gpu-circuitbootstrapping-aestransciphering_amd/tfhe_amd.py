@@ -39,13 +39,14 @@ class CbsParams(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("N", ctypes.c_int), ("bk_logB", ctypes.c_int), ("bk_l", ctypes.c_int),
                 ("cb_logB", ctypes.c_int), ("cb_l", ctypes.c_int), ("pf_logB", ctypes.c_int), ("pf_l", ctypes.c_int),
                 ("ks_logB", ctypes.c_int), ("ks_l", ctypes.c_int), ("lwe_nshift", ctypes.c_int),
-                ("rlwe_nshift", ctypes.c_int)]
+                ("rlwe_nshift", ctypes.c_int), ("pf_nshift", ctypes.c_int)]
 
 
 def cbs_params():
-    """n=636, N=2048 torus-64, PBS 2^9 x 4, CB 2^8 x 2, PFKS 2^4 x 8, KS 2^2 x 8,
-    level-0 sigma 2^-15, level-2 sigma 2^-51."""
-    return CbsParams(636, 2048, 9, 4, 8, 2, 4, 8, 2, 8, 15, 19)
+    """n=636, N=2048; PBS 2^9 x 4 on the 64-bit torus (sigma 2^-51), PFKS 2^4 x 8 with
+    a torus-32 key (sigma 2^-30), CB 2^8 x 2 (torus-32 RGSW), KS 2^2 x 8 to the
+    level-0 key (sigma 2^-15)."""
+    return CbsParams(636, 2048, 9, 4, 8, 2, 4, 8, 2, 8, 15, 19, 30)
 
 
 def build():
@@ -85,7 +86,7 @@ SIGNATURES = {
     "tfhe_cbs_keygen": (ctypes.c_int, [_vp, _i32p, ctypes.c_uint64]),
     "tfhe_cbs_get_rlwe_sk": (ctypes.c_int, [_vp, _i32p]),
     "tfhe_cbs_get_pfks_entry": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                               np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")]),
+                                               np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")]),
     "tfhe_gen_bsk64": (ctypes.c_int, [ctypes.POINTER(CbsParams), ctypes.c_uint64, _vp, _vp, _vp, _vp]),
     "tfhe_cbs_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_cbs_pbs_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
@@ -254,7 +255,7 @@ class CbsContext:
         return K
 
     def pfks_entry(self, u, i, t, v):
-        out = np.zeros(2 * self.p.N, np.uint64)
+        out = np.zeros(2 * self.p.N, np.uint32)
         check(lib().tfhe_cbs_get_pfks_entry(self.h, u, i, t, v, out))
         return out
 

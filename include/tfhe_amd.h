@@ -124,9 +124,10 @@ int tfhe_gen_ksk32(const tfhe_gate_params* p, uint64_t seed, const int32_t* d_lw
 /* Circuit-bootstrapping context (level 2: N = 2048, torus-64).  Replaces     */
 /* GPUCirBTSContext (reference bitwise/include/cirbts/cirbts.cuh:31-500).     */
 /* LWE inputs are level-0 ciphertexts (uint32, n + 1) encoding bit * 2^31.    */
-/* Outputs are RGSW_K2(bit) in the coefficient domain, uint64                 */
-/* [2*cb_l rows][2][N]: row 2j + u is the private functional key switch f_u   */
-/* of LWE_K2(bit * 2^(48 + 8j)), f_0 = -K2 * x, f_1 = x.                     */
+/* Outputs are RGSW_K2(bit) on the 32-bit torus in the coefficient domain,   */
+/* uint32 [2*cb_l rows][2][N]: row 2j + u is the private functional key       */
+/* switch f_u of the torus-64 LWE_K2(bit * 2^(48 + 8j)), i.e. of bit * 2^(16+8j)*/
+/* on the 32-bit torus, f_0 = -K2 * x, f_1 = x.                              */
 /* ------------------------------------------------------------------------- */
 typedef struct tfhe_cbs_params {
     int n;            /* level-0 LWE dimension (636) */
@@ -136,7 +137,8 @@ typedef struct tfhe_cbs_params {
     int pf_logB, pf_l;  /* private functional key switching 2^4 x 8 */
     int ks_logB, ks_l;  /* N -> n key switching 2^2 x 8 */
     int lwe_nshift;     /* level-0 noise 2^-15 */
-    int rlwe_nshift;    /* level-2 noise: sigma = 2^-(32 + rlwe_nshift) of the 64-bit torus (19) */
+    int rlwe_nshift;    /* BSK2 noise: sigma = 2^-(32 + rlwe_nshift) of the 64-bit torus (19) */
+    int pf_nshift;      /* PFKS key noise: sigma = 2^-pf_nshift of the 32-bit torus (30) */
 } tfhe_cbs_params;
 
 typedef struct tfhe_cbs_ctx tfhe_cbs_ctx;
@@ -148,37 +150,39 @@ int tfhe_cbs_ctx_destroy(tfhe_cbs_ctx* ctx);
 
 /* Key generation on the GPU from seed (replaces the fork's CirBTKeyGen,
  * main.cu:232-235, and gpu_init_cirbsk, cirbts-init.cu:200-865): RLWE secret
- * K2, BSK2 (Fourier, 2 limbs), the PFKS key (2 x 8.05 GB) and KSK2. */
+ * K2, BSK2 (torus-64, Fourier, 2 limbs), the torus-32 PFKS key (2 x 4.03 GB)
+ * and KSK2. */
 int tfhe_cbs_keygen(tfhe_cbs_ctx* ctx, const int32_t* lwe_sk, uint64_t seed);
 int tfhe_cbs_get_rlwe_sk(const tfhe_cbs_ctx* ctx, int32_t* out);
-/* host copy of PFKS key entry (u, i, t, v), 2N uint64 (keygen parity) */
-int tfhe_cbs_get_pfks_entry(const tfhe_cbs_ctx* ctx, int u, int i, int t, int v, uint64_t* out);
+/* host copy of PFKS key entry (u, i, t, v), 2N uint32 (keygen parity) */
+int tfhe_cbs_get_pfks_entry(const tfhe_cbs_ctx* ctx, int u, int i, int t, int v, uint32_t* out);
 /* stateless BSK2 generator into d_bsk [n][8][2][N] uint64 (keygen parity) */
 int tfhe_gen_bsk64(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_lwe_sk, const int32_t* d_rlwe_sk,
                    uint64_t* d_bsk, void* stream);
 
 /* gpu_CircuitBootstrappingBatchToDeviceFromDeviceLWE (cirbts.cuh:466-474,
- * cirbts-batch.cu:1416-1879): d_lwe [batch][n+1] -> d_rgsw [batch][4][2][N].
+ * cirbts-batch.cu:1416-1879): d_lwe [batch][n+1] -> d_rgsw [batch][4][2][N] uint32.
  * Pieces: the multi-value PBS (gpu_BootstrapLUT_inplace_batch_device,
- * cirbts-batch.cu:3293) into d_out [batch][2][N+1] (LWE_K2 of bit * 2^48 and
- * bit * 2^56), and the private functional key switch of count LWE_K2
- * ciphertexts [count][N+1] into rows spaced out_stride uint64 apart. */
-int tfhe_cbs_batch(tfhe_cbs_ctx* ctx, const uint32_t* d_lwe, uint32_t batch, uint64_t* d_rgsw, void* stream);
+ * cirbts-batch.cu:3293) into d_out [batch][2][N+1] uint64 (LWE_K2 of bit * 2^48
+ * and bit * 2^56 on the 64-bit torus), and the private functional key switch of
+ * count torus-64 LWE_K2 ciphertexts [count][N+1] into torus-32 RLWE rows of 2N
+ * uint32 spaced out_stride uint32 apart. */
+int tfhe_cbs_batch(tfhe_cbs_ctx* ctx, const uint32_t* d_lwe, uint32_t batch, uint32_t* d_rgsw, void* stream);
 int tfhe_cbs_pbs_batch(tfhe_cbs_ctx* ctx, const uint32_t* d_lwe, uint32_t batch, uint64_t* d_out, void* stream);
 int tfhe_cbs_pfks_batch(tfhe_cbs_ctx* ctx, int u, const uint64_t* d_lwe64, uint32_t count, size_t out_stride,
-                        uint64_t* d_out, void* stream);
+                        uint32_t* d_out, void* stream);
 
-/* RGSW (coefficient, [count][4][2][N] uint64) -> Fourier, 2 limbs per polynomial:
- * d_rgsw_f [count][8][2][1024] complex doubles. */
-int tfhe_rgsw_to_fourier(tfhe_cbs_ctx* ctx, const uint64_t* d_rgsw, uint32_t count, double* d_rgsw_f, void* stream);
+/* RGSW (torus-32 coefficients, [count][4][2][N] uint32) -> Fourier, one
+ * exact FP64 transform per polynomial: d_rgsw_f [count][8][1024] complex doubles. */
+int tfhe_rgsw_to_fourier(tfhe_cbs_ctx* ctx, const uint32_t* d_rgsw, uint32_t count, double* d_rgsw_f, void* stream);
 
-/* 8-input CMux vertical-packing LUT (the S-box LUT; CPU semantics of
- * EvalLUT8x32_RLWE_CPU, rlwe_tables.cuh:93-108, GPU form gpu_lut.cuh:1783):
- * for lut k, control RGSWs d_rgsw_f[8k .. 8k+7] (bit 0 first), table polynomial
- * d_table + k * table_stride (coefficient 8x + j = output bit j of entry x,
- * times 2^63); out [nlut][out_bits][N+1] uint32 = extracted coefficients switched
- * to the 32-bit torus. */
-int tfhe_lut8_eval_batch(tfhe_cbs_ctx* ctx, const double* d_rgsw_f, uint32_t nlut, const uint64_t* d_table,
+/* 8-input CMux vertical-packing LUT on the 32-bit torus (the S-box LUT; CPU
+ * semantics of EvalLUT8x32_RLWE_CPU, rlwe_tables.cuh:93-108, GPU form
+ * gpu_lut.cuh:1783): for lut k, control RGSWs d_rgsw_f[8k .. 8k+7] (bit 0
+ * first), table polynomial d_table + k * table_stride uint32 (coefficient
+ * 8x + j = output bit j of entry x, times 2^31); out [nlut][out_bits][N+1]
+ * uint32 = the extracted coefficients. */
+int tfhe_lut8_eval_batch(tfhe_cbs_ctx* ctx, const double* d_rgsw_f, uint32_t nlut, const uint32_t* d_table,
                          size_t table_stride, int out_bits, uint32_t* d_out32, void* stream);
 
 /* N -> n key switch of [count][N+1] uint32 (32-bit torus) LWE_K2 ciphertexts

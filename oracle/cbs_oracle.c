@@ -1,8 +1,8 @@
 /*
- * cbs_oracle.c -- CPU restatement of the circuit-bootstrapping level (N = 2048,
- * torus-64): multi-value programmable bootstrap, private functional key
- * switching, RGSW assembly, CMux / vertical-packing LUT and the 64 -> 32 key
- * switch back to the level-0 LWE key.
+ * cbs_oracle.c -- CPU restatement of the circuit-bootstrapping level (N = 2048):
+ * multi-value programmable bootstrap (torus-64), private functional key
+ * switching to torus-32 RLWE, RGSW assembly, CMux / vertical-packing LUT
+ * (torus-32) and the key switch back to the level-0 LWE key.
  *
  * TEST INFRASTRUCTURE ONLY (see tfhe_oracle.h).
  *
@@ -259,12 +259,15 @@ double orc_pbs_manylut(const orc_cbs_params* p, const double* bsk_f, const uint3
 }
 
 /* ------------------------------------------------------------------------- */
-/* Private functional key switching LWE_K2 (dim N) -> RLWE_K2 (CGGI 2020,     */
-/* PrivKS): out = - sum_{i<=N} sum_t KS[u][i][t][digit_t(a~_i)], with          */
-/* a_N = b and kappa_N = -1, a~_i = round(a_i / 2^(64 - pf_l pf_logB)), digits  */
-/* unsigned, least significant first, value 0 contributing nothing.           */
-/* KS[u][i][t][v] = RLWE_K2(f_u(v * kappa_i * 2^(64 - pf_l pf_logB + t pf_logB)))*/
-/* with f_0(x) = -K2(X) * x and f_1(x) = x.                                    */
+/* Private functional key switching LWE_K2 (dim N, torus-64) -> RLWE_K2        */
+/* (torus-32) (CGGI 2020, PrivKS):                                             */
+/*   out = - sum_{i<=N} sum_t KS[u][i][t][digit_t(a~_i)]  (mod 2^32)           */
+/* with a_N = b and kappa_N = -1, a~_i = round(a_i / 2^(64 - pf_l pf_logB)),   */
+/* digits unsigned, least significant first, value 0 contributing nothing.     */
+/* KS[u][i][t][v] = RLWE_K2(f_u(v * kappa_i * 2^(32 - pf_l pf_logB + t pf_logB)))*/
+/* on the 32-bit torus (sigma 2^-pf_nshift), f_0(x) = -K2(X) * x, f_1(x) = x.  */
+/* The output RLWE, the RGSW rows built from it and the CMux LUT all live on  */
+/* the 32-bit torus; pf_l pf_logB = 32 keeps every bit the key can resolve.   */
 /* ------------------------------------------------------------------------- */
 uint32_t orc_pfks_id(const orc_cbs_params* p, int u, int i, int t, int v) {
     int B = 1 << p->pf_logB;
@@ -272,40 +275,79 @@ uint32_t orc_pfks_id(const orc_cbs_params* p, int u, int i, int t, int v) {
 }
 
 void orc_gen_pfks_entry(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, int u, int i, int t,
-                        int v, uint64_t* ct) {
+                        int v, uint32_t* ct) {
     int N = p->N;
-    uint64_t kappa = i < N ? (uint64_t)(int64_t)K2[i] : (uint64_t)(int64_t)-1;
-    uint64_t c = (uint64_t)v * kappa * ((uint64_t)1 << (64 - p->pf_l * p->pf_logB + t * p->pf_logB));
-    uint64_t* msg = (uint64_t*)calloc((size_t)N, sizeof(uint64_t));
+    uint32_t kappa = i < N ? (uint32_t)K2[i] : 0xFFFFFFFFu;
+    uint32_t c = (uint32_t)v * kappa * ((uint32_t)1 << (32 - p->pf_l * p->pf_logB + t * p->pf_logB));
+    uint32_t* msg = (uint32_t*)calloc((size_t)N, sizeof(uint32_t));
     if (u == 1) {
         msg[0] = c;
     } else {
-        for (int k = 0; k < N; ++k) msg[k] = 0 - (uint64_t)(int64_t)K2[k] * c;
+        for (int k = 0; k < N; ++k) msg[k] = 0u - (uint32_t)K2[k] * c;
     }
-    orc_rlwe_encrypt64(seed, ORC_DOM_PFKS, orc_pfks_id(p, u, i, t, v), N, K2, NULL, msg, p->rlwe_nshift, ct);
+    orc_rlwe_encrypt32(seed, ORC_DOM_PFKS, orc_pfks_id(p, u, i, t, v), N, K2, NULL, msg, p->pf_nshift, ct);
     free(msg);
 }
 
+/* Sum of the selected entries, restated by linearity: every entry is
+ * (a_e, a_e * K2 + e_e + msg_e) with a_e, e_e from Philox (orc_rlwe_encrypt32),
+ * so the sum is (A, A * K2 + E + sum msg_e) with A = sum a_e, E = sum e_e; the
+ * messages are c_e (coefficient 0, f_1) or -K2 c_e (f_0) with scalar c_e.  All
+ * of it is exact mod 2^32, so the result equals adding the ciphertexts of
+ * orc_gen_pfks_entry one by one, at one polynomial product instead of one per
+ * entry.  OpenMP over i with per-thread partial sums (order-free mod 2^32). */
 void orc_pfks(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, int u, const uint64_t* lwe,
-              uint64_t* out) {
+              uint32_t* out) {
     int N = p->N, B = 1 << p->pf_logB, kb = p->pf_l * p->pf_logB;
-    uint64_t* ct = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N);
-    memset(out, 0, sizeof(uint64_t) * 2 * N);
-    for (int i = 0; i <= N; ++i) {
-        uint64_t at = (lwe[i] + ((uint64_t)1 << (63 - kb))) >> (64 - kb);
-        for (int t = 0; t < p->pf_l; ++t) {
-            int v = (int)((at >> (t * p->pf_logB)) & (uint64_t)(B - 1));
-            if (!v) continue;
-            orc_gen_pfks_entry(seed, p, K2, u, i, t, v, ct);
-            for (int k = 0; k < 2 * N; ++k) out[k] -= ct[k];
+    uint32_t* A = (uint32_t*)calloc((size_t)N, sizeof(uint32_t));
+    uint32_t* E = (uint32_t*)calloc((size_t)N, sizeof(uint32_t));
+    uint32_t C = 0;
+#pragma omp parallel
+    {
+        uint32_t* al = (uint32_t*)calloc((size_t)N, sizeof(uint32_t));
+        uint32_t* el = (uint32_t*)calloc((size_t)N, sizeof(uint32_t));
+        uint32_t cl = 0;
+#pragma omp for schedule(static)
+        for (int i = 0; i <= N; ++i) {
+            uint64_t at = (lwe[i] + ((uint64_t)1 << (63 - kb))) >> (64 - kb);
+            uint32_t kappa = i < N ? (uint32_t)K2[i] : 0xFFFFFFFFu;
+            for (int t = 0; t < p->pf_l; ++t) {
+                int v = (int)((at >> (t * p->pf_logB)) & (uint64_t)(B - 1));
+                if (!v) continue;
+                uint32_t id = orc_pfks_id(p, u, i, t, v);
+                for (int j = 0; j < N; ++j) {
+                    al[j] += orc_rand32(seed, ORC_DOM_PFKS, id, 0, (uint32_t)j);
+                    el[j] += (uint32_t)orc_noise(seed, ORC_DOM_PFKS, id, (uint32_t)j, p->pf_nshift);
+                }
+                cl += (uint32_t)v * kappa * ((uint32_t)1 << (32 - kb + t * p->pf_logB));
+            }
         }
+#pragma omp critical
+        {
+            for (int j = 0; j < N; ++j) {
+                A[j] += al[j];
+                E[j] += el[j];
+            }
+            C += cl;
+        }
+        free(al);
+        free(el);
     }
-    free(ct);
+    uint32_t* b = (uint32_t*)malloc(sizeof(uint32_t) * N);
+    orc_nc_mul_exact32(K2, A, b, N);
+    for (int j = 0; j < N; ++j) {
+        uint32_t m = u == 1 ? (j == 0 ? C : 0u) : 0u - (uint32_t)K2[j] * C;
+        out[j] = 0u - A[j];
+        out[N + j] = 0u - (b[j] + E[j] + m);
+    }
+    free(b);
+    free(A);
+    free(E);
 }
 
-/* RGSW row r = 2j + u = PFKS_u(PBS_j(ct)): [2 cb_l][2][N] u64 (the circuit bootstrap). */
+/* RGSW row r = 2j + u = PFKS_u(PBS_j(ct)): [2 cb_l][2][N] u32 (the circuit bootstrap). */
 double orc_cbs(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, const double* bsk_f,
-               const uint32_t* ct32, uint64_t* rgsw) {
+               const uint32_t* ct32, uint32_t* rgsw) {
     int N = p->N;
     uint64_t* lwes = (uint64_t*)malloc(sizeof(uint64_t) * 2 * (N + 1));
     double err = orc_pbs_manylut(p, bsk_f, ct32, lwes);
@@ -315,46 +357,54 @@ double orc_cbs(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, const 
     return err;
 }
 
-/* Fourier RGSW: [2 cb_l rows][2 out][4M]. */
-void orc_rgsw_to_fourier64(int N, int rows, const uint64_t* rgsw, double* rgsw_f) {
+/* Fourier RGSW (torus-32, one limb): [2 cb_l rows][2 out][re M | im M]. */
+void orc_rgsw_to_fourier32(int N, int rows, const uint32_t* rgsw, double* rgsw_f) {
     int M = N / 2;
-    for (int q = 0; q < rows * 2; ++q) orc_to_fourier64(rgsw + (size_t)q * N, N, rgsw_f + (size_t)q * 4 * M);
+    double* x = (double*)malloc(sizeof(double) * N);
+    for (int q = 0; q < rows * 2; ++q) {
+        for (int c = 0; c < N; ++c) x[c] = (double)(int32_t)rgsw[(size_t)q * N + c];
+        orc_fft_forward(x, rgsw_f + (size_t)q * 2 * M, rgsw_f + (size_t)q * 2 * M + M, N);
+    }
+    free(x);
 }
 
-/* CMux(C, d0, d1) = d0 + C (*) (d1 - d0) (reference rlwe_tables.cuh:79-91). */
-double orc_cmux64(int N, int logB, int l, const double* rgsw_f, const uint64_t* d0, const uint64_t* d1,
-                  uint64_t* out) {
-    uint64_t* diff = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N);
-    uint64_t* t = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N);
+/* CMux(C, d0, d1) = d0 + C (*) (d1 - d0) on the 32-bit torus (reference
+ * rlwe_tables.cuh:79-91); the external product is the exact one-limb FFT
+ * product of orc_external_product32 (|digit| <= 2^7, |key| <= 2^31, N = 2048). */
+double orc_cmux32(int N, int logB, int l, const double* rgsw_f, const uint32_t* d0, const uint32_t* d1,
+                  uint32_t* out) {
+    uint32_t* diff = (uint32_t*)malloc(sizeof(uint32_t) * 2 * N);
+    uint32_t* t = (uint32_t*)malloc(sizeof(uint32_t) * 2 * N);
     for (int k = 0; k < 2 * N; ++k) diff[k] = d1[k] - d0[k];
-    double e = orc_external_product64(N, logB, l, rgsw_f, diff, t);
+    double e = orc_external_product32(N, logB, l, rgsw_f, diff, t);
     for (int k = 0; k < 2 * N; ++k) out[k] = d0[k] + t[k];
     free(diff);
     free(t);
     return e;
 }
 
-/* Vertical-packing LUT: acc = (0, table_poly); for bit i = 0..7 (least significant
- * first, as EvalLUT8x32_RLWE_CPU, rlwe_tables.cuh:93-108):
- * acc = CMux(C_i, acc, X^{-stride 2^i} acc).  After the loop coefficient j holds
- * table_poly[stride x + j].  rgsw_f: [8][2 cb_l][2][4M]. */
-double orc_vertical_packing(const orc_cbs_params* p, int stride, const double* rgsw_f, const uint64_t* table_poly,
-                            uint64_t* acc) {
+/* Vertical-packing LUT on the 32-bit torus: acc = (0, table_poly); for bit
+ * i = 0..7 (least significant first, as EvalLUT8x32_RLWE_CPU,
+ * rlwe_tables.cuh:93-108): acc = CMux(C_i, acc, X^{-stride 2^i} acc).  After
+ * the loop coefficient j holds table_poly[stride x + j].
+ * rgsw_f: [8][2 cb_l][2][2M] (orc_rgsw_to_fourier32 per bit). */
+double orc_vertical_packing(const orc_cbs_params* p, int stride, const double* rgsw_f, const uint32_t* table_poly,
+                            uint32_t* acc) {
     int N = p->N, M = N / 2, rows = 2 * p->cb_l;
-    uint64_t* rot = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N);
+    uint32_t* rot = (uint32_t*)malloc(sizeof(uint32_t) * 2 * N);
     double maxerr = 0.0;
-    memset(acc, 0, sizeof(uint64_t) * N);
-    memcpy(acc + N, table_poly, sizeof(uint64_t) * N);
+    memset(acc, 0, sizeof(uint32_t) * N);
+    memcpy(acc + N, table_poly, sizeof(uint32_t) * N);
     for (int i = 0; i < 8; ++i) {
         uint32_t sh = (uint32_t)(stride << i); /* rotate by X^{-sh} */
         for (int o = 0; o < 2; ++o)
             for (int c = 0; c < N; ++c) {
                 uint32_t src = (uint32_t)c + sh; /* (X^{-sh} a)_c = a_{c+sh} (negated past N) */
-                uint64_t v = acc[o * N + (src % (uint32_t)N)];
-                if ((src / (uint32_t)N) & 1) v = 0 - v;
+                uint32_t v = acc[o * N + (src % (uint32_t)N)];
+                if ((src / (uint32_t)N) & 1) v = 0u - v;
                 rot[o * N + c] = v;
             }
-        double e = orc_cmux64(N, p->cb_logB, p->cb_l, rgsw_f + (size_t)i * rows * 2 * 4 * M, acc, rot, acc);
+        double e = orc_cmux32(N, p->cb_logB, p->cb_l, rgsw_f + (size_t)i * rows * 2 * 2 * M, acc, rot, acc);
         if (e > maxerr) maxerr = e;
     }
     free(rot);

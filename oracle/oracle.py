@@ -229,13 +229,14 @@ class CbsParams(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("N", ctypes.c_int), ("bk_logB", ctypes.c_int), ("bk_l", ctypes.c_int),
                 ("cb_logB", ctypes.c_int), ("cb_l", ctypes.c_int), ("pf_logB", ctypes.c_int), ("pf_l", ctypes.c_int),
                 ("ks_logB", ctypes.c_int), ("ks_l", ctypes.c_int), ("lwe_nshift", ctypes.c_int),
-                ("rlwe_nshift", ctypes.c_int)]
+                ("rlwe_nshift", ctypes.c_int), ("pf_nshift", ctypes.c_int)]
 
 
 def cbs_params(n=636, N=2048):
-    """Circuit-bootstrapping level: N=2048 torus-64, PBS 2^9 x 4, CB 2^8 x 2, PFKS 2^4 x 8,
-    KS 2^2 x 8 to the level-0 key; level-2 sigma 2^-51, level-0 sigma 2^-15."""
-    return CbsParams(n, N, 9, 4, 8, 2, 4, 8, 2, 8, 15, 19)
+    """Circuit-bootstrapping level: N=2048, PBS 2^9 x 4 (torus-64, sigma 2^-51), PFKS 2^4 x 8
+    (torus-32 key, sigma 2^-30), CB 2^8 x 2 (torus-32 RGSW), KS 2^2 x 8 to the level-0 key
+    (sigma 2^-15)."""
+    return CbsParams(n, N, 9, 4, 8, 2, 4, 8, 2, 8, 15, 19, 30)
 
 
 _cbs_sigs_done = False
@@ -263,12 +264,12 @@ def _cbs_lib():
             "orc_pbs_manylut": (ctypes.c_double, [P, _f64p, _u32p, _u64p]),
             "orc_pfks_id": (ctypes.c_uint32, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "orc_gen_pfks_entry": (None, [ctypes.c_uint64, P, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                          ctypes.c_int, _u64p]),
-            "orc_pfks": (None, [ctypes.c_uint64, P, _i32p, ctypes.c_int, _u64p, _u64p]),
-            "orc_cbs": (ctypes.c_double, [ctypes.c_uint64, P, _i32p, _f64p, _u32p, _u64p]),
-            "orc_rgsw_to_fourier64": (None, [ctypes.c_int, ctypes.c_int, _u64p, _f64p]),
-            "orc_cmux64": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, _u64p, _u64p, _u64p]),
-            "orc_vertical_packing": (ctypes.c_double, [P, ctypes.c_int, _f64p, _u64p, _u64p]),
+                                          ctypes.c_int, _u32p]),
+            "orc_pfks": (None, [ctypes.c_uint64, P, _i32p, ctypes.c_int, _u64p, _u32p]),
+            "orc_cbs": (ctypes.c_double, [ctypes.c_uint64, P, _i32p, _f64p, _u32p, _u32p]),
+            "orc_rgsw_to_fourier32": (None, [ctypes.c_int, ctypes.c_int, _u32p, _f64p]),
+            "orc_cmux32": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, _u32p, _u32p, _u32p]),
+            "orc_vertical_packing": (ctypes.c_double, [P, ctypes.c_int, _f64p, _u32p, _u32p]),
             "orc_ks2": (None, [P, _u32p, _u64p, _u32p]),
             "orc_gen_ksk2": (None, [ctypes.c_uint64, P, _i32p, _i32p, _u32p]),
             "orc_gen_ksk32_dom": (None, [ctypes.c_uint64, ctypes.c_uint32, G, _i32p, _i32p, _u32p]),
@@ -287,6 +288,13 @@ def nc_mul_exact64(x, y):
     out = np.zeros(len(y), np.uint64)
     lib().orc_nc_mul_exact64(x, y, out, len(y))
     return out
+
+
+def rlwe_phase32(K, ct):
+    """b - a*K over Z/2^32 for ct = [a | b]."""
+    N = len(K)
+    ct = np.asarray(ct, np.uint32)
+    return (ct[N:] - nc_mul_exact32(K, ct[:N])).astype(np.uint32)
 
 
 def rlwe_phase64(K, ct):
@@ -366,50 +374,57 @@ class CbsKeys:
         return out.reshape(2, p.N + 1), err
 
     def pfks(self, u, lwe64):
+        """Torus-32 RLWE_K2 of f_u(phase of lwe64)."""
         p = self.p
-        out = np.zeros(2 * p.N, np.uint64)
+        out = np.zeros(2 * p.N, np.uint32)
         _cbs_lib().orc_pfks(self.seed, ctypes.byref(p), self.K2, u, np.ascontiguousarray(lwe64, np.uint64), out)
         return out
 
     def pfks_entry(self, u, i, t, v):
         p = self.p
-        out = np.zeros(2 * p.N, np.uint64)
+        out = np.zeros(2 * p.N, np.uint32)
         _cbs_lib().orc_gen_pfks_entry(self.seed, ctypes.byref(p), self.K2, u, i, t, v, out)
         return out
 
     def cbs(self, ct32):
         p = self.p
-        rgsw = np.zeros(2 * p.cb_l * 2 * p.N, np.uint64)
+        rgsw = np.zeros(2 * p.cb_l * 2 * p.N, np.uint32)
         err = _cbs_lib().orc_cbs(self.seed, ctypes.byref(p), self.K2, self.bsk_f,
                                  np.ascontiguousarray(ct32, np.uint32), rgsw)
         return rgsw, err
 
     def rgsw_encrypt(self, m, id0, logB=None, l=None):
+        """Torus-32 RGSW_K2(m): a torus-64 encryption rounded to its top 32 bits."""
         p = self.p
         logB = p.cb_logB if logB is None else logB
         l = p.cb_l if l is None else l
         rgsw = np.zeros(2 * l * 2 * p.N, np.uint64)
         _cbs_lib().orc_rgsw_encrypt64(self.seed, DOM_ENC, id0, p.N, self.K2, int(m), logB, l, p.rlwe_nshift, rgsw)
-        return rgsw
+        return ((rgsw + np.uint64(1 << 31)) >> np.uint64(32)).astype(np.uint32)
 
     def rgsw_fourier(self, rgsw):
         p = self.p
         rows = len(rgsw) // (2 * p.N)
-        out = np.zeros(rows * 2 * 2 * p.N, np.float64)
-        _cbs_lib().orc_rgsw_to_fourier64(p.N, rows, np.ascontiguousarray(rgsw, np.uint64), out)
+        out = np.zeros(rows * 2 * p.N, np.float64)
+        _cbs_lib().orc_rgsw_to_fourier32(p.N, rows, np.ascontiguousarray(rgsw, np.uint32), out)
         return out
 
     def vertical_packing(self, rgsw_fs, table_poly, stride=8):
         p = self.p
-        acc = np.zeros(2 * p.N, np.uint64)
+        acc = np.zeros(2 * p.N, np.uint32)
         err = _cbs_lib().orc_vertical_packing(ctypes.byref(p), stride, np.ascontiguousarray(np.concatenate(rgsw_fs)),
-                                              np.ascontiguousarray(table_poly, np.uint64), acc)
+                                              np.ascontiguousarray(table_poly, np.uint32), acc)
         return acc, err
 
     def ks2(self, lwe64):
         out = np.zeros(self.p.n + 1, np.uint32)
         _cbs_lib().orc_ks2(ctypes.byref(self.p), self.ksk2, np.ascontiguousarray(lwe64, np.uint64), out)
         return out
+
+    def extract32(self, acc, idx):
+        lwe = np.zeros(self.p.N + 1, np.uint32)
+        lib().orc_sample_extract32(self.p.N, np.ascontiguousarray(acc, np.uint32), idx, lwe)
+        return lwe
 
     def extract64(self, acc, idx):
         lwe = np.zeros(self.p.N + 1, np.uint64)

@@ -88,7 +88,7 @@ double orc_gate_bootstrap(const orc_gate_params* p, const double* bsk_f, const u
 void orc_gen_ksk32_dom(uint64_t seed, uint32_t dom, const orc_gate_params* p, const int32_t* s,
                        const int32_t* K, uint32_t* ksk);
 
-/* ---- circuit-bootstrapping level (N = 2048, torus-64), cbs_oracle.c ---- */
+/* ---- circuit-bootstrapping level (N = 2048; PBS torus-64, PFKS/RGSW/LUT torus-32), cbs_oracle.c ---- */
 typedef struct {
     int n, N;             /* 636, 2048 */
     int bk_logB, bk_l;    /* PBS gadget 2^9 x 4 */
@@ -97,6 +97,7 @@ typedef struct {
     int ks_logB, ks_l;    /* N -> n key switch 2^2 x 8 (torus-32) */
     int lwe_nshift;       /* level-0 LWE noise (torus-32): 15 */
     int rlwe_nshift;      /* level-2 noise (torus-64): 19, i.e. sigma = 2^-51 */
+    int pf_nshift;        /* PFKS key noise (torus-32): 30, i.e. sigma = 2^-30 (4 units of 2^-32) */
 } orc_cbs_params;
 
 void orc_to_fourier64(const uint64_t* poly, int N, double* out);
@@ -114,15 +115,15 @@ uint32_t orc_special_ms(uint32_t v, int N, int bw);
 double orc_pbs_manylut(const orc_cbs_params* p, const double* bsk_f, const uint32_t* ct32, uint64_t* out);
 uint32_t orc_pfks_id(const orc_cbs_params* p, int u, int i, int t, int v);
 void orc_gen_pfks_entry(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, int u, int i, int t, int v,
-                        uint64_t* ct);
-void orc_pfks(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, int u, const uint64_t* lwe, uint64_t* out);
+                        uint32_t* ct);
+void orc_pfks(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, int u, const uint64_t* lwe, uint32_t* out);
 double orc_cbs(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, const double* bsk_f,
-               const uint32_t* ct32, uint64_t* rgsw);
-void orc_rgsw_to_fourier64(int N, int rows, const uint64_t* rgsw, double* rgsw_f);
-double orc_cmux64(int N, int logB, int l, const double* rgsw_f, const uint64_t* d0, const uint64_t* d1,
-                  uint64_t* out);
-double orc_vertical_packing(const orc_cbs_params* p, int stride, const double* rgsw_f, const uint64_t* table_poly,
-                            uint64_t* acc);
+               const uint32_t* ct32, uint32_t* rgsw);
+void orc_rgsw_to_fourier32(int N, int rows, const uint32_t* rgsw, double* rgsw_f);
+double orc_cmux32(int N, int logB, int l, const double* rgsw_f, const uint32_t* d0, const uint32_t* d1,
+                  uint32_t* out);
+double orc_vertical_packing(const orc_cbs_params* p, int stride, const double* rgsw_f, const uint32_t* table_poly,
+                            uint32_t* acc);
 void orc_ks2(const orc_cbs_params* p, const uint32_t* ksk2, const uint64_t* lwe64, uint32_t* out);
 void orc_gen_ksk2(uint64_t seed, const orc_cbs_params* p, const int32_t* s, const int32_t* K2, uint32_t* ksk);
 

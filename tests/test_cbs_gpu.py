@@ -1,5 +1,6 @@
-"""GPU parity of the circuit-bootstrapping level (N = 2048, torus-64) and the
-AES-128 transciphering pipeline against the CPU oracle, through the C ABI."""
+"""GPU parity of the circuit-bootstrapping level (N = 2048: PBS on the 64-bit
+torus, PFKS / RGSW / LUT on the 32-bit torus) and the AES-128 transciphering
+pipeline against the CPU oracle, through the C ABI."""
 import numpy as np
 import pytest
 
@@ -93,47 +94,63 @@ def test_pbs_manylut_bit_exact(ctx, okeys):
 
 
 def test_pfks_bit_exact(ctx, okeys):
+    """Both functions on the two PBS outputs, plus random torus-64 inputs whose
+    rounding hits the 2^32 wrap (a_i >= 2^64 - 2^31) and every digit value; the
+    output rows are strided (out_stride > 2N) and the gap stays untouched."""
     import torch
     cts = _bits_cts(okeys, [1], 400)
-    d_lwe64 = torch.empty((1, 2, 2049), dtype=torch.int64, device="cuda")
-    ctx.pbs_batch(to_dev32(cts), d_lwe64)
+    d_pbs = torch.empty((1, 2, 2049), dtype=torch.int64, device="cuda")
+    ctx.pbs_batch(to_dev32(cts), d_pbs)
     torch.cuda.synchronize()
-    lwes = from_dev(d_lwe64, np.uint64)[0]
+    lwes = from_dev(d_pbs, np.uint64)[0]
+    rng = np.random.default_rng(12)
+    extra = rng.integers(0, 2 ** 64, (3, 2049), dtype=np.uint64)
+    extra[0, :64] = np.uint64(2 ** 64 - 2 ** 31 + 5)
+    extra[1, :16] = np.array([(v * 0x11111111) << 32 for v in range(16)], np.uint64)
+    lwes = np.concatenate([lwes, extra])
+    d_lwe64 = to_dev64(lwes)
+    count, stride = len(lwes), 4096 + 64
     for u in (0, 1):
-        d_out = torch.empty((2, 4096), dtype=torch.int64, device="cuda")
-        ctx.pfks_batch(u, d_lwe64, 2, 4096, d_out)
+        d_out = torch.full((count, stride), 7, dtype=torch.int32, device="cuda")
+        ctx.pfks_batch(u, d_lwe64, count, stride, d_out)
         torch.cuda.synchronize()
-        got = from_dev(d_out, np.uint64)
-        for j in range(2):
-            assert np.array_equal(got[j], okeys.pfks(u, lwes[j])), (u, j)
+        got = from_dev(d_out)
+        for j in range(count):
+            assert np.array_equal(got[j, :4096], okeys.pfks(u, lwes[j])), (u, j)
+        assert (got[:, 4096:] == 7).all()
 
 
 def test_cbs_rgsw_rows_decrypt(ctx, okeys):
-    """Circuit bootstrap of 6 bits: row 2j+u decrypts to f_u(bit * 2^(48+8j))."""
+    """Circuit bootstrap of 6 bits: row 2j+u decrypts to f_u(bit * 2^(16+8j)) on
+    the 32-bit torus; the first input is also checked bit for bit against
+    the oracle's circuit bootstrap."""
     import torch
     bits = [1, 0, 1, 1, 0, 1]
     cts = _bits_cts(okeys, bits, 500)
-    d_rgsw = torch.empty((len(bits), 4, 2, 2048), dtype=torch.int64, device="cuda")
+    d_rgsw = torch.empty((len(bits), 4, 2, 2048), dtype=torch.int32, device="cuda")
     ctx.cbs_batch(to_dev32(cts), d_rgsw)
     torch.cuda.synchronize()
-    rg = from_dev(d_rgsw, np.uint64)
+    rg = from_dev(d_rgsw)
+    want0, err0 = okeys.cbs(cts[0])
+    assert err0 < 0.25
+    assert np.array_equal(rg[0].reshape(-1), want0)
     for k, b in enumerate(bits):
         for j in range(2):
-            mu = np.zeros(2048, np.uint64)
-            mu[0] = np.uint64((b << (48 + 8 * j)) % 2 ** 64)
+            mu = np.zeros(2048, np.uint32)
+            mu[0] = np.uint32(b << (16 + 8 * j))
             for u in range(2):
-                ph = O.rlwe_phase64(okeys.K2, rg[k, 2 * j + u].reshape(-1))
-                want = mu if u == 1 else (-O.nc_mul_exact64(okeys.K2, mu)).astype(np.uint64)
-                err = O.signed64((ph - want).astype(np.uint64))
-                assert np.abs(err).max() < 2 ** 40, (k, j, u)
+                ph = O.rlwe_phase32(okeys.K2, rg[k, 2 * j + u].reshape(-1))
+                want = mu if u == 1 else (-O.nc_mul_exact32(okeys.K2, mu)).astype(np.uint32)
+                err = ((ph - want).astype(np.uint32)).view(np.int32)
+                assert np.abs(err).max() < 2 ** 16, (k, j, u)  # ~2^12 expected
 
 
 def _sbox_poly():
     S = [O.aes_sbox(x) for x in range(256)]
-    poly = np.zeros(2048, np.uint64)
+    poly = np.zeros(2048, np.uint32)
     for x in range(256):
         for j in range(8):
-            poly[8 * x + j] = np.uint64(((S[x] >> j) & 1) << 63)
+            poly[8 * x + j] = np.uint32(((S[x] >> j) & 1) << 31)
     return poly, S
 
 
@@ -143,11 +160,11 @@ def test_lut_bit_exact(ctx, okeys):
     poly, S = _sbox_poly()
     xs = [0x53, 0x00]
     rgsws = [okeys.rgsw_encrypt((x >> i) & 1, 2000 + 16 * k + 2 * i) for k, x in enumerate(xs) for i in range(8)]
-    d_rgsw = to_dev64(np.concatenate(rgsws))
-    d_f = torch.empty(len(rgsws) * 8 * 2 * 1024 * 2, dtype=torch.float64, device="cuda")
+    d_rgsw = to_dev32(np.concatenate(rgsws))
+    d_f = torch.empty(len(rgsws) * 8 * 1024 * 2, dtype=torch.float64, device="cuda")
     ctx.rgsw_to_fourier(d_rgsw, len(rgsws), d_f)
     d_out = torch.empty((len(xs), 8, 2049), dtype=torch.int32, device="cuda")
-    ctx.lut8(d_f, len(xs), to_dev64(poly), 0, 8, d_out)
+    ctx.lut8(d_f, len(xs), to_dev32(poly), 0, 8, d_out)
     torch.cuda.synchronize()
     got = from_dev(d_out)
     for k, x in enumerate(xs):
@@ -155,9 +172,7 @@ def test_lut_bit_exact(ctx, okeys):
         acc, err = okeys.vertical_packing(fs, poly)
         assert err < 0.25
         for j in range(8):
-            lwe = okeys.extract64(acc, j)
-            want = ((lwe + np.uint64(1 << 31)) >> np.uint64(32)).astype(np.uint32)
-            assert np.array_equal(got[k, j], want), (x, j)
+            assert np.array_equal(got[k, j], okeys.extract32(acc, j)), (x, j)
 
 
 def test_ks2_bit_exact(ctx, okeys):
@@ -183,19 +198,24 @@ def test_sbox_end_to_end(ctx, okeys):
     bits = [(x >> i) & 1 for x in xs for i in range(8)]
     cts = _bits_cts(okeys, bits, 700)
     nb = len(bits)
-    d_rgsw = torch.empty((nb, 4, 2, 2048), dtype=torch.int64, device="cuda")
+    d_rgsw = torch.empty((nb, 4, 2, 2048), dtype=torch.int32, device="cuda")
     ctx.cbs_batch(to_dev32(cts), d_rgsw)
-    d_f = torch.empty(nb * 8 * 2 * 1024 * 2, dtype=torch.float64, device="cuda")
+    d_f = torch.empty(nb * 8 * 1024 * 2, dtype=torch.float64, device="cuda")
     ctx.rgsw_to_fourier(d_rgsw, nb, d_f)
     d_l = torch.empty((len(xs), 8, 2049), dtype=torch.int32, device="cuda")
-    ctx.lut8(d_f, len(xs), to_dev64(poly), 0, 8, d_l)
+    ctx.lut8(d_f, len(xs), to_dev32(poly), 0, 8, d_l)
     d_out = torch.empty((len(xs) * 8, 637), dtype=torch.int32, device="cuda")
     ctx.keyswitch_batch(d_l, len(xs) * 8, d_out)
     torch.cuda.synchronize()
+    lut = from_dev(d_l)
     got = from_dev(d_out)
     for k, x in enumerate(xs):
         val = sum(okeys.decrypt_bit(got[8 * k + j]) << j for j in range(8))
         assert val == S[x], (hex(x), hex(val), hex(S[x]))
+        for j in range(8):  # LUT output noise (torus-32 phase under K2) well inside the 2^30 margin
+            ph = (int(lut[k, j, -1]) - int(np.dot(lut[k, j, :-1].astype(np.uint64), okeys.K2.astype(np.uint64)))) % 2 ** 32
+            e = ((ph - (((S[x] >> j) & 1) << 31) + 2 ** 31) % 2 ** 32) - 2 ** 31
+            assert abs(e) < 2 ** 28, (hex(x), j, e)
 
 
 def _decrypt_blocks(okeys, bits_ct, blocks):

@@ -1,7 +1,8 @@
-"""Pins of the CPU oracle's circuit-bootstrapping level (N = 2048, torus-64):
-exact limb products against schoolbook, the multi-value PBS, private functional
-key switching, CMux / vertical packing and the 64 -> 32 key switch, each
-checked by decryption under the secret keys (no GPU needed)."""
+"""Pins of the CPU oracle's circuit-bootstrapping level (N = 2048): exact
+torus-64 limb products and torus-32 products against schoolbook, the
+multi-value PBS (torus-64), private functional key switching to torus-32,
+CMux / vertical packing (torus-32) and the 64 -> 32 key switch, each checked
+by decryption under the secret keys (no GPU needed)."""
 import random
 
 import numpy as np
@@ -28,8 +29,7 @@ def test_external_product64_exact_against_schoolbook():
     rows = 2 * l
     rgsw = rng.integers(0, 2 ** 64, rows * 2 * N, dtype=np.uint64)
     acc = rng.integers(0, 2 ** 64, 2 * N, dtype=np.uint64)
-    rgsw_f = np.zeros(rows * 2 * 2 * N, np.float64)
-    L.orc_rgsw_to_fourier64(N, rows, rgsw, rgsw_f)
+    rgsw_f = np.concatenate([O.to_fourier64(rgsw[q * N:(q + 1) * N]) for q in range(rows * 2)])
     out = np.zeros(2 * N, np.uint64)
     err = L.orc_external_product64(N, logB, l, rgsw_f, acc, out)
     assert err < 0.25
@@ -40,6 +40,31 @@ def test_external_product64_exact_against_schoolbook():
             r = 2 * j + u
             for o in range(2):
                 want[o * N:(o + 1) * N] += O.nc_mul_exact64(digs[:, j], rgsw[(r * 2 + o) * N:(r * 2 + o + 1) * N])
+    assert np.array_equal(out, want)
+
+
+def test_external_product32_n2048_exact_against_schoolbook():
+    """The torus-32 CMux product at N = 2048 (one FP64 FFT product per digit
+    polynomial, 2^8 x 2 digits times full 32-bit RGSW words) equals the exact
+    integer result mod 2^32; also pins orc_rgsw_to_fourier32's layout."""
+    L = O._cbs_lib()
+    N, logB, l = 2048, 8, 2
+    rng = np.random.default_rng(32)
+    rows = 2 * l
+    rgsw = rng.integers(0, 2 ** 32, rows * 2 * N, dtype=np.uint64).astype(np.uint32)
+    acc = rng.integers(0, 2 ** 32, 2 * N, dtype=np.uint64).astype(np.uint32)
+    rgsw_f = np.zeros(rows * 2 * N, np.float64)
+    L.orc_rgsw_to_fourier32(N, rows, rgsw, rgsw_f)
+    out = np.zeros(2 * N, np.uint32)
+    err = L.orc_external_product32(N, logB, l, rgsw_f, acc, out)
+    assert err < 0.25
+    want = np.zeros(2 * N, np.uint32)
+    for u in range(2):
+        digs = np.array([O.decompose32(int(t), logB, l) for t in acc[u * N:(u + 1) * N]])  # [N][l]
+        for j in range(l):
+            r = 2 * j + u
+            for o in range(2):
+                want[o * N:(o + 1) * N] += O.nc_mul_exact32(digs[:, j], rgsw[(r * 2 + o) * N:(r * 2 + o + 1) * N])
     assert np.array_equal(out, want)
 
 
@@ -88,22 +113,20 @@ def test_pbs_manylut_levels(keys, bit):
 
 
 def test_pfks_entries_and_functions(keys):
-    """Key entries decrypt to f_u(v kappa_i g_t); the switched RLWE decrypts to f_u(phase)."""
+    """Key entries (torus-32) decrypt to f_u(v kappa_i 2^(4t)) with sigma 2^-30
+    noise; the switched RLWE decrypts to f_u(phase) of the torus-64 input."""
     p = keys.p
     N = p.N
     for (u, i, t, v) in [(1, 0, 0, 1), (1, N, 7, 15), (0, 5, 3, 9), (0, N, 0, 2)]:
         ct = keys.pfks_entry(u, i, t, v)
         kappa = int(keys.K2[i]) if i < N else -1
-        c = (v * kappa * 2 ** (32 + 4 * t)) % 2 ** 64
-        if u == 1:
-            want = np.zeros(N, np.uint64)
-            want[0] = c
-        else:
-            m = np.zeros(N, np.uint64)
-            m[0] = c
-            want = (-O.nc_mul_exact64(keys.K2, m)).astype(np.uint64)
-        err = O.signed64((O.rlwe_phase64(keys.K2, ct) - want).astype(np.uint64))
-        assert np.abs(err).max() < 7 * 2 ** 13
+        c = (v * kappa * 2 ** (4 * t)) % 2 ** 32
+        m = np.zeros(N, np.uint32)
+        m[0] = c
+        want = m if u == 1 else (-O.nc_mul_exact32(keys.K2, m)).astype(np.uint32)
+        err = ((O.rlwe_phase32(keys.K2, ct) - want).astype(np.uint32)).view(np.int32)
+        assert np.abs(err).max() <= 7 * 4  # sigma = 4 units of 2^-32; Irwin-Hall tail is +-6 sigma
+        assert np.abs(err).max() > 0
     # a random LWE_K2 of mu = 2^56
     rng = np.random.default_rng(3)
     a = rng.integers(0, 2 ** 64, N, dtype=np.uint64)
@@ -112,12 +135,32 @@ def test_pfks_entries_and_functions(keys):
     lwe = np.concatenate([a, np.array([b], np.uint64)])
     for u in (0, 1):
         r = keys.pfks(u, lwe)
-        ph = O.rlwe_phase64(keys.K2, r)
-        m = np.zeros(N, np.uint64)
-        m[0] = mu
-        want = m if u == 1 else (-O.nc_mul_exact64(keys.K2, m)).astype(np.uint64)
-        err = O.signed64((ph - want).astype(np.uint64))
-        assert np.abs(err).max() < 2 ** 40
+        ph = O.rlwe_phase32(keys.K2, r)
+        m = np.zeros(N, np.uint32)
+        m[0] = mu >> 32
+        want = m if u == 1 else (-O.nc_mul_exact32(keys.K2, m)).astype(np.uint32)
+        err = ((ph - want).astype(np.uint32)).view(np.int32)
+        # sqrt(sum over 16392 digits of v^2) * 4 ~ 2^12 units (2^-20 of the torus)
+        assert np.abs(err).max() < 2 ** 16
+
+
+def test_pfks_sum_equals_entry_by_entry(keys):
+    """orc_pfks sums the selected entries by linearity (one mask product); on a
+    sparse input it equals subtracting each orc_gen_pfks_entry ciphertext, for
+    both functions, including the b row (i = N) and a digit-15 word."""
+    l2 = np.zeros(keys.p.N + 1, np.uint64)
+    l2[3] = np.uint64(0x5 << 32)
+    l2[keys.p.N] = np.uint64(0xA3 << 32)
+    l2[700] = np.uint64((0xF0000001 << 32) - (1 << 31))  # rounds up to 0xF0000001
+    for u in (0, 1):
+        want = np.zeros(2 * keys.p.N, np.uint32)
+        for i in (3, keys.p.N, 700):
+            at = ((int(l2[i]) + (1 << 31)) >> 32) & 0xFFFFFFFF
+            for t in range(8):
+                v = (at >> (4 * t)) & 15
+                if v:
+                    want = (want - keys.pfks_entry(u, i, t, v)).astype(np.uint32)
+        assert np.array_equal(keys.pfks(u, l2), want), u
 
 
 @pytest.mark.parametrize("m", [0, 1])
@@ -126,19 +169,19 @@ def test_cmux_selects(keys, m):
     N = p.N
     rng = np.random.default_rng(10 + m)
     C = keys.rgsw_fourier(keys.rgsw_encrypt(m, 500))
-    msgs = [rng.integers(0, 2, N).astype(np.uint64) << np.uint64(63) for _ in range(2)]
+    msgs = [(rng.integers(0, 2, N).astype(np.uint32) << np.uint32(31)).astype(np.uint32) for _ in range(2)]
     ds = []
     for k, msg in enumerate(msgs):
-        a = rng.integers(0, 2 ** 64, N, dtype=np.uint64)
-        b = (O.nc_mul_exact64(keys.K2, a) + msg).astype(np.uint64)
+        a = rng.integers(0, 2 ** 32, N, dtype=np.uint64).astype(np.uint32)
+        b = (O.nc_mul_exact32(keys.K2, a) + msg).astype(np.uint32)
         ds.append(np.concatenate([a, b]))
-    out = np.zeros(2 * N, np.uint64)
-    err = O._cbs_lib().orc_cmux64(N, p.cb_logB, p.cb_l, C, ds[0], ds[1], out)
+    out = np.zeros(2 * N, np.uint32)
+    err = O._cbs_lib().orc_cmux32(N, p.cb_logB, p.cb_l, C, ds[0], ds[1], out)
     assert err < 0.25
-    ph = O.rlwe_phase64(keys.K2, out)
-    e = O.signed64((ph - msgs[m]).astype(np.uint64))
-    # decomposition keeps 16 of 64 bits: error ~ sqrt(N/6) 2^47 ~ 2^51 typical
-    assert np.abs(e).max() < 2 ** 56
+    ph = O.rlwe_phase32(keys.K2, out)
+    e = ((ph - msgs[m]).astype(np.uint32)).view(np.int32)
+    # decomposition keeps 16 of 32 bits: error ~ sqrt(N/6) 2^15 ~ 2^19 typical
+    assert np.abs(e).max() < 2 ** 24
 
 
 def test_vertical_packing_table_lookup(keys):
@@ -147,16 +190,16 @@ def test_vertical_packing_table_lookup(keys):
     p = keys.p
     rng = np.random.default_rng(77)
     table = rng.integers(0, 256, 256)
-    poly = np.zeros(p.N, np.uint64)
+    poly = np.zeros(p.N, np.uint32)
     for x in range(256):
         for j in range(8):
-            poly[8 * x + j] = np.uint64(((int(table[x]) >> j) & 1) << 63)
+            poly[8 * x + j] = np.uint32(((int(table[x]) >> j) & 1) << 31)
     for x in (0, 0xA7, 255):
         fs = [keys.rgsw_fourier(keys.rgsw_encrypt((x >> i) & 1, 1000 + 8 * i + 64 * x)) for i in range(8)]
         acc, err = keys.vertical_packing(fs, poly)
         assert err < 0.25
-        ph = O.rlwe_phase64(keys.K2, acc)
-        got = [int(((int(ph[j]) + 2 ** 62) >> 63) & 1) for j in range(8)]
+        ph = O.rlwe_phase32(keys.K2, acc)
+        got = [int(((int(ph[j]) + 2 ** 30) >> 31) & 1) for j in range(8)]
         assert got == [(int(table[x]) >> j) & 1 for j in range(8)], x
 
 
