@@ -103,6 +103,7 @@ struct tfhe_cbs_ctx {
     uint32_t* d_ksk2 = nullptr;
     uint64_t* d_lwe64 = nullptr;  // [max_batch][2][N+1]
     uint32_t* d_at = nullptr;     // [N+1][pfks_bpad(2 max_batch)] rounded, transposed PFKS inputs
+    uint32_t* d_ks_at = nullptr;  // [N][ks_bpad(max_batch)] key-switch digits scratch
     DeviceTables1024 tables;
     // AES state
     uint32_t max_blocks = 0;
@@ -120,7 +121,7 @@ struct tfhe_cbs_ctx {
 
 extern "C" int tfhe_cbs_ctx_destroy(tfhe_cbs_ctx* c) {
     if (!c) return TFHE_OK;
-    void* bufs[] = {c->d_K2, c->d_bsk2f, c->d_pfks, c->d_ksk2, c->d_lwe64, c->d_at, c->d_rgsw, c->d_rgsw_f, c->d_lut32,
+    void* bufs[] = {c->d_K2, c->d_bsk2f, c->d_pfks, c->d_ksk2, c->d_lwe64, c->d_at, c->d_ks_at, c->d_rgsw, c->d_rgsw_f, c->d_lut32,
                     c->d_state[0], c->d_state[1], c->d_sbox_poly, c->d_lin_mix, c->d_lin_last, c->d_rk, c->d_ctr};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -147,6 +148,7 @@ extern "C" int tfhe_cbs_ctx_create(const tfhe_cbs_params* params, uint32_t max_b
     rc = rc ? rc : dmalloc((void**)&c->d_ksk2, ksk2_words(p) * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_lwe64, B * 2 * (kN + 1) * sizeof(uint64_t));
     rc = rc ? rc : dmalloc((void**)&c->d_at, (size_t)(kN + 1) * pfks_bpad(2 * (uint32_t)B) * sizeof(uint32_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_ks_at, (size_t)kN * ks_bpad((uint32_t)B) * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_rgsw, B * 4 * 2 * kN * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_rgsw_f, B * 8 * 1024 * sizeof(cplx));
     rc = rc ? rc : dmalloc((void**)&c->d_lut32, B * (kN + 1) * sizeof(uint32_t));
@@ -342,8 +344,10 @@ extern "C" int tfhe_lut8_eval_batch(tfhe_cbs_ctx* c, const double* d_rgsw_f, uin
 extern "C" int tfhe_cbs_keyswitch_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe32, uint32_t count, uint32_t* d_out,
                                         void* stream) {
     if (!c || !c->has_keys || !d_lwe32 || !d_out) return fail(TFHE_E_PARAM, "cbs keyswitch: bad argument");
+    if (count > c->max_batch) return fail(TFHE_E_CAPACITY, "cbs keyswitch: count exceeds the context capacity");
     if (!count) return TFHE_OK;
-    if (launch_keyswitch32(d_lwe32, count, kN, c->p.n, c->p.ks_logB, c->p.ks_l, c->d_ksk2, d_out, (hipStream_t)stream))
+    if (launch_keyswitch32(d_lwe32, count, kN, c->p.n, c->p.ks_logB, c->p.ks_l, c->d_ksk2, c->d_ks_at, d_out,
+                           (hipStream_t)stream))
         return fail(TFHE_E_PARAM, "cbs keyswitch: unsupported shape");
     return check_launch("keyswitch32 N=2048");
 }

@@ -22,6 +22,7 @@ struct tfhe_gate_ctx {
     cplx* d_bskf = nullptr;       // [n][2l][2][8][64]
     uint32_t* d_ksk = nullptr;    // [N][ks_l][B][n+1]
     uint32_t* d_lweN = nullptr;   // [max_batch][N+1]
+    uint32_t* d_ks_at = nullptr;  // [N][ks_bpad(max_batch)] key-switch digits scratch
     // plain-form scratch
     uint32_t* d_acc = nullptr;    // [2][N]
     cplx* d_digf = nullptr;       // [2l][8][64]
@@ -72,6 +73,7 @@ extern "C" int tfhe_gate_ctx_create(const tfhe_gate_params* params, uint32_t max
     rc = rc ? rc : dmalloc((void**)&c->d_bskf, bsk_words(p) / 2 * sizeof(cplx));
     rc = rc ? rc : dmalloc((void**)&c->d_ksk, ksk_words(p) * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_lweN, (size_t)max_batch * (p.N + 1) * sizeof(uint32_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_ks_at, (size_t)p.N * ks_bpad(max_batch) * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_acc, 2 * (size_t)p.N * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_digf, (size_t)2 * p.bk_l * 512 * sizeof(cplx));
     rc = rc ? rc : dmalloc((void**)&c->d_t, 2 * (size_t)p.N * sizeof(uint32_t));
@@ -89,6 +91,7 @@ extern "C" int tfhe_gate_ctx_destroy(tfhe_gate_ctx* c) {
     (void)hipFree(c->d_bskf);
     (void)hipFree(c->d_ksk);
     (void)hipFree(c->d_lweN);
+    (void)hipFree(c->d_ks_at);
     (void)hipFree(c->d_acc);
     (void)hipFree(c->d_digf);
     (void)hipFree(c->d_t);
@@ -189,8 +192,9 @@ extern "C" int tfhe_gate_keyswitch_batch(tfhe_gate_ctx* c, const uint32_t* d_lwe
                                          void* stream) {
     if (!c || !d_lweN || !d_out) return fail(TFHE_E_PARAM, "keyswitch: bad argument");
     if (!c->has_keys) return fail(TFHE_E_STATE, "keyswitch: no key-switching key");
+    if (batch > c->max_batch) return fail(TFHE_E_CAPACITY, "keyswitch: batch exceeds the context capacity");
     if (batch == 0) return TFHE_OK;
-    if (launch_keyswitch32(d_lweN, batch, c->p.N, c->p.n, c->p.ks_logB, c->p.ks_l, c->d_ksk, d_out,
+    if (launch_keyswitch32(d_lweN, batch, c->p.N, c->p.n, c->p.ks_logB, c->p.ks_l, c->d_ksk, c->d_ks_at, d_out,
                            (hipStream_t)stream))
         return fail(TFHE_E_PARAM, "keyswitch: unsupported shape");
     return check_launch("keyswitch32");

@@ -327,71 +327,107 @@ __global__ void k_plain_extract(const uint32_t* __restrict__ acc, uint32_t* __re
 }
 
 // ===========================================================================
-// Batched LWE key switch N -> n on a 32-bit torus (reference
-// bitwise/src/kernel.cu:259-279 with the host-side B loop of
-// binfhe-base-scheme.cu:225-246 moved onto the GPU):
-//   out = (0, b) - sum_{i<N} sum_{j<l} KSK[i][j][digit_j(a~_i)]
-// where a~_i keeps the top l*logB bits of a_i (rounded) and digits are
-// unsigned, least significant first.  A block owns 128 output columns of
-// KS_CT ciphertexts; each key row read is reused across the ciphertext tile.
+// LWE key switching N -> n over Z/2^32 (reference kernel_LWEKeySwitch_modSwitch,
+// bitwise/src/kernel.cu:259-279, with the host half of binfhe-base-scheme.cu:225-246
+// folded in): out = (0, b) - sum_{i<N} sum_{j<L} KSK[i][j][digit_j(a~_i)], where
+// a~_i = round(a_i / 2^(32 - L LOGB)) and the digits are unsigned base 2^LOGB,
+// least significant first (digit 0 selects the key's encryption of zero).
+// k_ks_digits rounds and transposes the inputs to [i][Bpad], so that one
+// coefficient of a wave's KS_CT ciphertexts is a uniform (scalar) read.
+// k_keyswitch32: a workgroup of KS_WAVES waves covers 64 output columns; per
+// coefficient i the L * 2^LOGB candidate rows of those columns are staged once
+// in a double-buffered LDS tile (one barrier per i) and each wave subtracts,
+// for each of its ciphertexts and digits, the row the digit names: one LDS read
+// and one subtraction per (ciphertext, digit, column).  blockIdx.z picks a
+// slice of i; slices add into the zero-filled output with 32-bit atomics
+// (order-free mod 2^32).
 // ===========================================================================
-constexpr int KS_CT = 16, KS_ICHUNK = 128, KS_COLS = 128;
+constexpr int KS_WAVES = 8, KS_CT = 16, KS_GROUP = KS_WAVES * KS_CT;
+
+template <int KBITS>
+__global__ __launch_bounds__(256) void k_ks_digits(const uint32_t* __restrict__ lweN, uint32_t batch, int N,
+                                                  uint32_t Bpad, uint32_t* __restrict__ at) {
+    __shared__ uint32_t tile[64][65];
+    const int i0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) {  // r = ciphertext offset, tx = coefficient offset
+        const uint32_t k = k0 + r;
+        const int i = i0 + tx;
+        uint32_t v = 0;
+        if (k < batch && i < N)
+            v = (uint32_t)(((uint64_t)lweN[(size_t)k * (N + 1) + i] + (1ull << (31 - KBITS))) >> (32 - KBITS));
+        tile[r][tx] = v;
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {  // r = coefficient offset, tx = ciphertext offset
+        const int i = i0 + r;
+        const uint32_t k = k0 + tx;
+        if (i < N && k < Bpad) at[(size_t)i * Bpad + k] = tile[tx][r];
+    }
+}
 
 template <int LOGB, int L>
-__global__ __launch_bounds__(KS_COLS) void k_keyswitch32(const uint32_t* __restrict__ lweN, uint32_t batch, int N,
-                                                    int n, const uint32_t* __restrict__ ksk,
-                                                    uint32_t* __restrict__ out) {
-    // blockIdx.z selects a slice of the N input coefficients; slices accumulate into
-    // the zero-filled output with 32-bit atomic adds (addition mod 2^32 is order-free)
+__global__ __launch_bounds__(64 * KS_WAVES) void k_keyswitch32(const uint32_t* __restrict__ at, uint32_t batch,
+                                                              uint32_t Bpad, int N, int n,
+                                                              const uint32_t* __restrict__ lweN,
+                                                              const uint32_t* __restrict__ ksk,
+                                                              uint32_t* __restrict__ out) {
+    constexpr int B = 1 << LOGB, ROWS = L * B, PER = ROWS / KS_WAVES;
+    static_assert(ROWS % KS_WAVES == 0, "tile rows must split evenly over the waves");
+    __shared__ uint32_t tab[2][ROWS][64];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int col = blockIdx.x * 64 + lane;
+    const bool colok = col <= n;
     const int islice = N / (int)gridDim.z;
     const int ibeg = (int)blockIdx.z * islice, iend = ibeg + islice;
-    constexpr int B = 1 << LOGB, KBITS = L * LOGB;
-    __shared__ uint32_t atil[KS_CT][KS_ICHUNK];
-    const int col = blockIdx.x * KS_COLS + threadIdx.x;
-    const uint32_t c0 = blockIdx.y * KS_CT;
-    const bool active = col <= n;
+    const uint32_t cbase = blockIdx.y * KS_GROUP + w * KS_CT;
+    const bool active = cbase < batch;  // idle waves still take part in the barriers
+    const size_t rs = (size_t)(n + 1);
+    uint32_t kv[PER];
+    auto load_i = [&](int i) {  // this thread's rows w + KS_WAVES q of coefficient i
+        uint64_t off = (uint64_t)i * ROWS * rs;
+        asm volatile("" : "+s"(off));  // keep the prefetch where it is written (see k_gate_blind_rotate)
+        const uint32_t* base = ksk + off + (colok ? col : 0);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) kv[q] = colok ? base[(size_t)(w + KS_WAVES * q) * rs] : 0u;
+    };
+    auto store = [&](int b) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) tab[b][w + KS_WAVES * q][lane] = kv[q];
+    };
     uint32_t acc[KS_CT];
 #pragma unroll
-    for (int c = 0; c < KS_CT; ++c) acc[c] = 0u;
-    const size_t rowstride = (size_t)(n + 1);
-    for (int i0 = ibeg; i0 < iend; i0 += KS_ICHUNK) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < KS_CT * KS_ICHUNK; e += KS_COLS) {
-            const int c = e / KS_ICHUNK, ii = e % KS_ICHUNK;
-            uint32_t v = 0u;
-            if (c0 + c < batch) {
-                uint32_t a = lweN[(size_t)(c0 + c) * (N + 1) + i0 + ii];
-                v = (uint32_t)(((uint64_t)a + (1ull << (31 - KBITS))) >> (32 - KBITS));
-            }
-            atil[c][ii] = v;
+    for (int k = 0; k < KS_CT; ++k) acc[k] = 0u;
+    load_i(ibeg);
+    store(0);
+    if (ibeg + 1 < iend) load_i(ibeg + 1);
+    __syncthreads();
+    for (int i = ibeg; i < iend; ++i) {
+        const int b = (i - ibeg) & 1;
+        if (i + 1 < iend) store(b ^ 1);
+        if (i + 2 < iend) load_i(i + 2);
+        if (active) {
+            uint32_t a[KS_CT];
+            const uint32_t* ar = at + (size_t)i * Bpad + cbase;
+#pragma unroll
+            for (int k = 0; k < KS_CT; ++k) a[k] = ar[k];
+#pragma unroll
+            for (int j = 0; j < L; ++j)
+#pragma unroll
+                for (int k = 0; k < KS_CT; ++k) acc[k] -= tab[b][j * B + ((a[k] >> (j * LOGB)) & (B - 1))][lane];
         }
+#pragma unroll
+        for (int k = 0; k < KS_CT; ++k) asm volatile("" : "+v"(acc[k]));  // subtractions stay before the barrier
         __syncthreads();
-        if (!active) continue;
-        for (int ii = 0; ii < KS_ICHUNK; ++ii) {
-            const uint32_t* kr = ksk + (size_t)(i0 + ii) * L * B * rowstride + col;
-#pragma unroll
-            for (int j = 0; j < L; ++j) {
-                uint32_t r[B];
-#pragma unroll
-                for (int v = 0; v < B; ++v) r[v] = kr[(size_t)(j * B + v) * rowstride];
-#pragma unroll
-                for (int c = 0; c < KS_CT; ++c) {
-                    const uint32_t d = (atil[c][ii] >> (j * LOGB)) & (B - 1);
-                    uint32_t sel = r[0];
-#pragma unroll
-                    for (int v = 1; v < B; ++v) sel = (d == (uint32_t)v) ? r[v] : sel;
-                    acc[c] -= sel;
-                }
-            }
-        }
     }
-    if (!active) return;
+    if (!active || !colok) return;
 #pragma unroll
-    for (int c = 0; c < KS_CT; ++c) {
-        if (c0 + c >= batch) break;
-        uint32_t v = acc[c];
-        if (col == n && blockIdx.z == 0) v += lweN[(size_t)(c0 + c) * (N + 1) + N];
-        atomicAdd(&out[(size_t)(c0 + c) * (n + 1) + col], v);
+    for (int k = 0; k < KS_CT; ++k) {
+        if (cbase + k >= batch) break;
+        uint32_t v = acc[k];
+        if (col == n && blockIdx.z == 0) v += lweN[(size_t)(cbase + k) * (N + 1) + N];
+        atomicAdd(&out[(size_t)(cbase + k) * rs + col], v);
     }
 }
 
@@ -481,18 +517,21 @@ void launch_external_product1024(uint32_t* acc, cplx* digf, uint32_t* t, const c
     k_plain_mac_ifft<<<2, 64, 0, st>>>(digf, key, T, t);
 }
 
-int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int logB, int l,
-                       const uint32_t* ksk, uint32_t* out, hipStream_t st) {
-    constexpr int KS_SPLIT = 4;
-    dim3 grid((n + 1 + KS_COLS - 1) / KS_COLS, (batch + KS_CT - 1) / KS_CT, KS_SPLIT);
-    if (N % (KS_ICHUNK * KS_SPLIT)) return -1;
-    if (logB == 2 && l == 8) {
-        const size_t count = (size_t)batch * (n + 1);
-        k_fill_zero<<<(unsigned)((count + 255) / 256 < 4096 ? (count + 255) / 256 : 4096), 256, 0, st>>>(out, count);
-        k_keyswitch32<2, 8><<<grid, KS_COLS, 0, st>>>(lweN, batch, N, n, ksk, out);
-        return 0;
-    }
-    return -1;
+uint32_t ks_bpad(uint32_t batch) { return (batch + KS_GROUP - 1) / KS_GROUP * KS_GROUP; }
+
+int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int logB, int l, const uint32_t* ksk,
+                       uint32_t* at, uint32_t* out, hipStream_t st) {
+    if (!(logB == 2 && l == 8) || N % 64 || n + 1 > 64 * 1024) return -1;
+    const uint32_t Bpad = ks_bpad(batch);
+    const unsigned cols = (unsigned)(n + 1 + 63) / 64, groups = Bpad / KS_GROUP;
+    // i-slices: enough workgroups to fill the chip (about 512), each slice >= 32 coefficients
+    unsigned slices = 1;
+    while (cols * groups * slices < 512 && (unsigned)N / (slices * 2) >= 32) slices *= 2;
+    const size_t count = (size_t)batch * (n + 1);
+    k_fill_zero<<<(unsigned)((count + 255) / 256 < 4096 ? (count + 255) / 256 : 4096), 256, 0, st>>>(out, count);
+    k_ks_digits<16><<<dim3((unsigned)N / 64, Bpad / 64), 256, 0, st>>>(lweN, batch, N, Bpad, at);
+    k_keyswitch32<2, 8><<<dim3(cols, groups, slices), 64 * KS_WAVES, 0, st>>>(at, batch, Bpad, N, n, lweN, ksk, out);
+    return 0;
 }
 
 }  // namespace tfhe

@@ -25,8 +25,10 @@ void launch_polymul1024(const int32_t* x, const uint32_t* y, uint32_t batch, con
                         hipStream_t st);
 void launch_external_product1024(uint32_t* acc, cplx* digf, uint32_t* t, const cplx* key, const FFTTables& T,
                                  hipStream_t st);
-int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int logB, int l,
-                       const uint32_t* ksk, uint32_t* out, hipStream_t st);
+// key switch N -> n; at: scratch of N * ks_bpad(batch) u32 (rounded, transposed inputs)
+uint32_t ks_bpad(uint32_t batch);
+int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int logB, int l, const uint32_t* ksk,
+                       uint32_t* at, uint32_t* out, hipStream_t st);
 // circuit-bootstrapping level (N = 2048; PBS torus-64, PFKS / RGSW / LUT torus-32)
 struct FFT1024Tables;
 void launch_to_fourier2048(const uint64_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st);

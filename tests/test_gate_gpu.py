@@ -204,5 +204,9 @@ def test_error_paths(ctx, okeys):
     with pytest.raises(tfhe_amd.TfheError) as e:
         ctx.eval_batch(O.NAND, big, big.clone(), big.clone())
     assert e.value.code == tfhe_amd.TFHE_E_CAPACITY
+    with pytest.raises(tfhe_amd.TfheError) as e:  # key switch beyond the context's scratch capacity
+        tfhe_amd.check(tfhe_amd.lib().tfhe_gate_keyswitch_batch(ctx.h, tfhe_amd._ptr(big), 4097,
+                                                                tfhe_amd._ptr(big), None))
+    assert e.value.code == tfhe_amd.TFHE_E_CAPACITY
     ctx.eval_batch(O.NAND, d, d.clone(), out, batch=0)  # empty batch is a no-op
 
