@@ -37,7 +37,7 @@ def okeys():
 @pytest.fixture(scope="module")
 def ctx(gpu, okeys):
     import tfhe_amd
-    c = tfhe_amd.CbsContext(max_blocks=2)
+    c = tfhe_amd.CbsContext(max_blocks=3)  # 384 inputs: room for the large-batch PBS form
     c.keygen(okeys.s, SEED)
     yield c
     c.close()
@@ -91,6 +91,33 @@ def test_pbs_manylut_bit_exact(ctx, okeys):
         want, err = okeys.pbs(cts[k])
         assert err < 0.25
         assert np.array_equal(got[k], want), k
+
+
+def test_pbs_large_batch_form_matches(ctx, okeys):
+    """A 300-ciphertext PBS (more workgroups than fit at once) equals the same
+    inputs bootstrapped in small sub-batches and the oracle on sampled inputs,
+    bit for bit."""
+    import torch
+    import tfhe_amd
+    batch = 300
+    rng = np.random.default_rng(17)
+    bits = rng.integers(0, 2, batch)
+    cts = np.stack([tfhe_amd.lwe_encrypt(okeys.s, 31, k, int(bits[k]), p=2) for k in range(batch)])
+    d_cts = to_dev32(cts)
+    d_big = torch.empty((batch, 2, 2049), dtype=torch.int64, device="cuda")
+    ctx.pbs_batch(d_cts, d_big)
+    torch.cuda.synchronize()
+    big = from_dev(d_big, np.uint64)
+    for lo in (0, 150, 296):
+        hi = min(lo + 4, batch)
+        d_small = torch.empty((hi - lo, 2, 2049), dtype=torch.int64, device="cuda")
+        ctx.pbs_batch(d_cts[lo:hi].contiguous(), d_small)
+        torch.cuda.synchronize()
+        assert np.array_equal(from_dev(d_small, np.uint64), big[lo:hi]), lo
+    for k in (0, 299):
+        want, err = okeys.pbs(cts[k])
+        assert err < 0.25
+        assert np.array_equal(big[k], want), k
 
 
 def test_pfks_bit_exact(ctx, okeys):
