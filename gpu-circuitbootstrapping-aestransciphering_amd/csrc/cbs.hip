@@ -37,7 +37,14 @@ __device__ __forceinline__ void split_limbs(uint64_t v, double& lo, double& hi) 
     hi = (double)(int32_t)(uint32_t)((v - (uint64_t)(int64_t)l) >> 32);
 }
 
-// FFT1024 forward/inverse with the cross twiddles taken from registers.
+// FFT1024 forward/inverse with the cross twiddles taken from registers.  The
+// cross-wave radix-4 output of wave w is either selected on w (BF = false:
+// uniform branches, which also bound the scheduler's reordering and keep the
+// kernel within 256 registers, i.e. two waves per SIMD) or computed branch-free
+// (BF = true: fewer instructions and a longer straight-line schedule, but more
+// registers; faster where there is one wave per SIMD anyway).  Both round
+// at the same points, so the outputs are identical.
+template <bool BF>
 __device__ __forceinline__ void f1024_fwd(cplx (&x)[4], cplx* lds, cplx* xbuf, const FFT1024Tables& T,
                                           const cplx (&trw)[4], int w, int lane) {
     fft256_fwd(x, lds, T, lane);
@@ -48,11 +55,16 @@ __device__ __forceinline__ void f1024_fwd(cplx (&x)[4], cplx* lds, cplx* xbuf, c
     for (int s = 0; s < 4; ++s) {
         cplx y0 = xbuf[(0 * 4 + s) * 64 + lane], y1 = xbuf[(1 * 4 + s) * 64 + lane];
         cplx y2 = xbuf[(2 * 4 + s) * 64 + lane], y3 = xbuf[(3 * 4 + s) * 64 + lane];
-        cplx s02 = cadd(y0, y2), d02 = csub(y0, y2), s13 = cadd(y1, y3), d13 = mul_mi<false>(csub(y1, y3));
-        x[s] = (w == 0) ? cadd(s02, s13) : (w == 1) ? cadd(d02, d13) : (w == 2) ? csub(s02, s13) : csub(d02, d13);
+        if constexpr (BF) {
+            x[s] = xwave_dft4<false>(y0, y1, y2, y3, w);
+        } else {
+            cplx s02 = cadd(y0, y2), d02 = csub(y0, y2), s13 = cadd(y1, y3), d13 = mul_mi<false>(csub(y1, y3));
+            x[s] = (w == 0) ? cadd(s02, s13) : (w == 1) ? cadd(d02, d13) : (w == 2) ? csub(s02, s13) : csub(d02, d13);
+        }
     }
 }
 
+template <bool BF>
 __device__ __forceinline__ void f1024_inv(cplx (&x)[4], cplx* lds, cplx* xbuf, const FFT1024Tables& T,
                                           const cplx (&trw)[4], int w, int lane) {
 #pragma unroll
@@ -62,8 +74,13 @@ __device__ __forceinline__ void f1024_inv(cplx (&x)[4], cplx* lds, cplx* xbuf, c
     for (int s = 0; s < 4; ++s) {
         cplx z0 = xbuf[(0 * 4 + s) * 64 + lane], z1 = xbuf[(1 * 4 + s) * 64 + lane];
         cplx z2 = xbuf[(2 * 4 + s) * 64 + lane], z3 = xbuf[(3 * 4 + s) * 64 + lane];
-        cplx s02 = cadd(z0, z2), d02 = csub(z0, z2), s13 = cadd(z1, z3), d13 = mul_mi<true>(csub(z1, z3));
-        cplx y = (w == 0) ? cadd(s02, s13) : (w == 1) ? cadd(d02, d13) : (w == 2) ? csub(s02, s13) : csub(d02, d13);
+        cplx y;
+        if constexpr (BF) {
+            y = xwave_dft4<true>(z0, z1, z2, z3, w);
+        } else {
+            cplx s02 = cadd(z0, z2), d02 = csub(z0, z2), s13 = cadd(z1, z3), d13 = mul_mi<true>(csub(z1, z3));
+            y = (w == 0) ? cadd(s02, s13) : (w == 1) ? cadd(d02, d13) : (w == 2) ? csub(s02, s13) : csub(d02, d13);
+        }
         x[s] = cmulc(y, trw[s]);
     }
     fft256_inv(x, lds, T, lane);
@@ -103,9 +120,9 @@ __global__ __launch_bounds__(256) void k_to_fourier2048(const uint64_t* __restri
             x0[a] = {l0r, l0i};
             x1[a] = {l1r, l1i};
         }
-        f1024_fwd(x0, lds_all[w], xbuf[buf], T, trw, w, lane);
+        f1024_fwd<false>(x0, lds_all[w], xbuf[buf], T, trw, w, lane);
         buf ^= 1;
-        f1024_fwd(x1, lds_all[w], xbuf[buf], T, trw, w, lane);
+        f1024_fwd<false>(x1, lds_all[w], xbuf[buf], T, trw, w, lane);
         buf ^= 1;
         cplx* o = out + q * 2 * CB_M;
 #pragma unroll
@@ -136,7 +153,7 @@ __global__ __launch_bounds__(256) void k_to_fourier2048_32(const uint32_t* __res
             const int c = own_coeff(w, lane, a);
             x[a] = {(double)(int32_t)p[c], (double)(int32_t)p[c + CB_M]};
         }
-        f1024_fwd(x, lds_all[w], xbuf[buf], T, trw, w, lane);
+        f1024_fwd<false>(x, lds_all[w], xbuf[buf], T, trw, w, lane);
         buf ^= 1;
         cplx* o = out + q * CB_M;
 #pragma unroll
@@ -170,16 +187,16 @@ __global__ __launch_bounds__(256) void k_polymul2048(const int32_t* __restrict__
             Y0[a] = {l0r, l0i};
             Y1[a] = {l1r, l1i};
         }
-        f1024_fwd(X, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
-        f1024_fwd(Y0, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
-        f1024_fwd(Y1, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        f1024_fwd<false>(X, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        f1024_fwd<false>(Y0, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        f1024_fwd<false>(Y1, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             Y0[s] = cmul(Y0[s], X[s]);
             Y1[s] = cmul(Y1[s], X[s]);
         }
-        f1024_inv(Y0, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
-        f1024_inv(Y1, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        f1024_inv<false>(Y0, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        f1024_inv<false>(Y1, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
         uint64_t* op = out + (size_t)q * CB_N;
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
@@ -228,15 +245,15 @@ __global__ __launch_bounds__(256) void k_gen_bsk64(uint64_t seed, uint32_t nrows
             A0[a] = {l0r, l0i};
             A1[a] = {l1r, l1i};
         }
-        f1024_fwd(A0, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
-        f1024_fwd(A1, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        f1024_fwd<false>(A0, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        f1024_fwd<false>(A1, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
 #pragma unroll
         for (int sl = 0; sl < 4; ++sl) {
             A0[sl] = cmul(A0[sl], kf[sl]);
             A1[sl] = cmul(A1[sl], kf[sl]);
         }
-        f1024_inv(A0, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
-        f1024_inv(A1, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        f1024_inv<false>(A0, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        f1024_inv<false>(A1, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
         uint64_t* oa = out + (size_t)row * 2 * CB_N;
         uint64_t* ob = oa + CB_N;
 #pragma unroll
@@ -295,10 +312,10 @@ __global__ __launch_bounds__(256) void k_gen_pfks32(uint64_t seed, uint32_t nrow
             av[a + 4] = rand32(seed, DOM_PFKS, id, (uint32_t)(c + CB_M));
             A[a] = {(double)(int32_t)av[a], (double)(int32_t)av[a + 4]};
         }
-        f1024_fwd(A, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        f1024_fwd<false>(A, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
 #pragma unroll
         for (int sl = 0; sl < 4; ++sl) A[sl] = cmul(A[sl], kf[sl]);
-        f1024_inv(A, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        f1024_inv<false>(A, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
         uint32_t* oa = out + (size_t)row * 2 * CB_N;
         uint32_t* ob = oa + CB_N;
 #pragma unroll
@@ -343,6 +360,7 @@ __device__ __forceinline__ uint32_t special_ms_2048(uint32_t v) {
     return (uint32_t)((((uint64_t)v + (1ull << 20)) >> 21) << 1) & 4095u;
 }
 
+template <bool BF>
 __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict__ lwe, uint32_t batch, int n,
                                                      const cplx* __restrict__ bskf, FFT1024Tables Tg,
                                                      uint64_t* __restrict__ out) {
@@ -401,7 +419,7 @@ __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict
                     cplx x[4];
 #pragma unroll
                     for (int a = 0; a < 4; ++a) x[a] = {pbs_digit(tp[a], j), pbs_digit(tp[a + 4], j)};
-                    f1024_fwd(x, lds, xbuf[buf], T, trw, w, lane);
+                    f1024_fwd<BF>(x, lds, xbuf[buf], T, trw, w, lane);
                     buf ^= 1;
                     const cplx* kr = key + (size_t)r * 4 * CB_M + (w * 4) * 64 + lane;
 #pragma unroll
@@ -419,9 +437,9 @@ __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict
                 cplx y0[4], y1[4];
 #pragma unroll
                 for (int s = 0; s < 4; ++s) { y0[s] = macc[o][0][s]; y1[s] = macc[o][1][s]; }
-                f1024_inv(y0, lds, xbuf[buf], T, trw, w, lane);
+                f1024_inv<BF>(y0, lds, xbuf[buf], T, trw, w, lane);
                 buf ^= 1;
-                f1024_inv(y1, lds, xbuf[buf], T, trw, w, lane);
+                f1024_inv<BF>(y1, lds, xbuf[buf], T, trw, w, lane);
                 buf ^= 1;
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
@@ -476,7 +494,7 @@ __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict
 //
 // k_pfks_digits rounds the inputs and transposes them to [i][Bpad] so that
 // one i-row of a wave's PF_CT inputs is a uniform (scalar-load) read.
-// k_pfks32: a workgroup of PF_WAVES waves shares one 64-column tile of the
+// k_pfks32: a workgroup of PF_WAVES waves shares one 128-column tile of the
 // key; per (i, t) the 15 candidate rows are loaded once by the workgroup into
 // an LDS table (row 0 = 0, double-buffered, one barrier per step) and every
 // wave selects, for each of its PF_CT inputs, the row named by that input's
@@ -514,7 +532,8 @@ __global__ __launch_bounds__(64 * PF_WAVES) void k_pfks32(const uint32_t* __rest
                                                          const uint32_t* __restrict__ key, size_t key_u_stride,
                                                          int u0, uint32_t* __restrict__ out, size_t out_stride,
                                                          size_t out_u_offset) {
-    __shared__ uint32_t tab[2][16][64];
+    // two adjacent columns per lane: one LDS address and one 8-byte read feed two subtractions
+    __shared__ uint2 tab[2][16][64];
     constexpr int N = CB_N, COLS = 2 * CB_N;
     constexpr int ISLICE = (N + 1 + PF_SLICES - 1) / PF_SLICES;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -526,20 +545,22 @@ __global__ __launch_bounds__(64 * PF_WAVES) void k_pfks32(const uint32_t* __rest
     const uint32_t cbase = blockIdx.y * PF_GROUP + w * PF_CT;
     const bool active = cbase < B;  // idle waves still take part in the barriers
     // this thread's share of a tile: candidate v = 1 + w (all waves) and v = 9 + w (waves 0..6)
-    const uint32_t* kcol = key + (size_t)u * key_u_stride + blockIdx.x * 64 + lane;
+    const uint2* kcol = reinterpret_cast<const uint2*>(key + (size_t)u * key_u_stride + blockIdx.x * 128) + lane;
     const bool second = w < 7;
     if (tid < 64) {
-        tab[0][0][tid] = 0;
-        tab[1][0][tid] = 0;
+        tab[0][0][tid] = make_uint2(0, 0);
+        tab[1][0][tid] = make_uint2(0, 0);
     }
-    uint32_t acc[PF_CT];
+    uint32_t acc[PF_CT][2];
 #pragma unroll
-    for (int k = 0; k < PF_CT; ++k) acc[k] = 0;
-    uint32_t kv0 = 0, kv1 = 0;
+    for (int k = 0; k < PF_CT; ++k) acc[k][0] = acc[k][1] = 0;
+    uint2 kv0 = make_uint2(0, 0), kv1 = make_uint2(0, 0);
     auto load_tile = [&](int i, int t) {
-        const uint32_t* kr = kcol + ((size_t)(i * 8 + t) * 15 + w) * COLS;
+        uint64_t off = ((uint64_t)(i * 8 + t) * 15 + w) * (COLS / 2);
+        asm volatile("" : "+s"(off));  // keep the prefetch where it is written
+        const uint2* kr = kcol + off;
         kv0 = kr[0];
-        if (second) kv1 = kr[(size_t)8 * COLS];
+        if (second) kv1 = kr[(size_t)8 * (COLS / 2)];
     };
     auto store_tile = [&](int b) {
         tab[b][1 + w][lane] = kv0;
@@ -561,17 +582,26 @@ __global__ __launch_bounds__(64 * PF_WAVES) void k_pfks32(const uint32_t* __rest
             if (more) load_tile(t < 7 ? i : i + 1, (t + 1) & 7);
             if (active) {
 #pragma unroll
-                for (int k = 0; k < PF_CT; ++k) acc[k] -= tab[t & 1][(a[k] >> (4 * t)) & 15u][lane];
+                for (int k = 0; k < PF_CT; ++k) {
+                    const uint2 v = tab[t & 1][(a[k] >> (4 * t)) & 15u][lane];
+                    acc[k][0] -= v.x;
+                    acc[k][1] -= v.y;
+                }
             }
+#pragma unroll
+            for (int k = 0; k < PF_CT; ++k) asm volatile("" : "+v"(acc[k][0]), "+v"(acc[k][1]));
             if (more) store_tile((t + 1) & 1);
             __syncthreads();
         }
     }
     if (active) {
-        uint32_t* o = out + (size_t)(u - u0) * out_u_offset + blockIdx.x * 64 + lane;
+        uint32_t* o = out + (size_t)(u - u0) * out_u_offset + blockIdx.x * 128 + 2 * lane;
 #pragma unroll
         for (int k = 0; k < PF_CT; ++k)
-            if (cbase + k < B) atomicAdd(&o[(size_t)(cbase + k) * out_stride], acc[k]);
+            if (cbase + k < B) {
+                atomicAdd(&o[(size_t)(cbase + k) * out_stride], acc[k][0]);
+                atomicAdd(&o[(size_t)(cbase + k) * out_stride + 1], acc[k][1]);
+            }
     }
 }
 
@@ -663,7 +693,7 @@ __global__ __launch_bounds__(256) void k_lut_vertical(const cplx* __restrict__ r
                     for (int a = 0; a < 4; ++a)
                         x[a] = {(double)((int32_t)((tp[u][a] >> sh2) & 255u) - 128),
                                 (double)((int32_t)((tp[u][a + 4] >> sh2) & 255u) - 128)};
-                    f1024_fwd(x, lds, xbuf[buf], T, trw, w, lane);
+                    f1024_fwd<false>(x, lds, xbuf[buf], T, trw, w, lane);
                     buf ^= 1;
                     const cplx* kr = key + (size_t)r * 2 * CB_M + (w * 4) * 64 + lane;
 #pragma unroll
@@ -673,7 +703,7 @@ __global__ __launch_bounds__(256) void k_lut_vertical(const cplx* __restrict__ r
                 }
 #pragma unroll
             for (int o = 0; o < 2; ++o) {
-                f1024_inv(macc[o], lds, xbuf[buf], T, trw, w, lane);
+                f1024_inv<false>(macc[o], lds, xbuf[buf], T, trw, w, lane);
                 buf ^= 1;
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
@@ -763,7 +793,12 @@ void launch_xor_public(uint32_t* ct, const uint8_t* pub, uint32_t nbits, int n, 
 
 void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, const FFT1024Tables& T, uint64_t* out,
                  hipStream_t st) {
-    k_pbs2_manylut<<<grid_for(batch, 2048), 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+    // one workgroup per ciphertext: up to 256 they run one wave per SIMD and the
+    // branch-free FFT form is faster; beyond, the two-waves-per-SIMD form is
+    if (batch <= 256)
+        k_pbs2_manylut<true><<<batch, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+    else
+        k_pbs2_manylut<false><<<grid_for(batch, 2048), 256, 0, st>>>(lwe, batch, n, bskf, T, out);
 }
 
 uint32_t pfks_bpad(uint32_t B) { return (B + PF_GROUP - 1) / PF_GROUP * PF_GROUP; }
@@ -780,7 +815,7 @@ void launch_zero_rows32(uint32_t* out, uint32_t count, size_t stride, uint32_t w
 
 void launch_pfks32(const uint32_t* at, uint32_t B, const uint32_t* key, size_t key_u_stride, int u0, int nu,
                    uint32_t* out, size_t out_stride, size_t out_u_offset, hipStream_t st) {
-    dim3 grid(2 * CB_N / 64, pfks_bpad(B) / PF_GROUP, PF_SLICES * nu);
+    dim3 grid(2 * CB_N / 128, pfks_bpad(B) / PF_GROUP, PF_SLICES * nu);
     k_pfks32<<<grid, 64 * PF_WAVES, 0, st>>>(at, B, pfks_bpad(B), key, key_u_stride, u0, out, out_stride,
                                               out_u_offset);
 }

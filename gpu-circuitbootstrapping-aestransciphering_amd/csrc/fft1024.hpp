@@ -121,52 +121,22 @@ __device__ __forceinline__ void fft256_inv(cplx (&x)[4], cplx* lds, const FFT102
     for (int a = 1; a < 4; ++a) x[a] = cmulc(x[a], twist4_const(a));
 }
 
-// Cross-wave forward step.  xbuf: [4 waves][4 slots][64] cplx in LDS.  Caller
-// guarantees xbuf is free (no wave still reading it).  On return x holds
-// Z[k + 256 q] for this wave q = w.
-__device__ __forceinline__ void fft1024_fwd(cplx (&x)[4], cplx* lds, cplx* xbuf, const FFT1024Tables& T, int w,
-                                            int lane) {
-    fft256_fwd(x, lds, T, lane);
-#pragma unroll
-    for (int s = 0; s < 4; ++s) xbuf[(w * 4 + s) * 64 + lane] = cmul(x[s], T.tr[(w * 4 + s) * 64 + lane]);
-    __syncthreads();
-    // Z_q = sum_r W4^(r q) Y'_r ; W4 = -i
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        cplx y0 = xbuf[(0 * 4 + s) * 64 + lane], y1 = xbuf[(1 * 4 + s) * 64 + lane];
-        cplx y2 = xbuf[(2 * 4 + s) * 64 + lane], y3 = xbuf[(3 * 4 + s) * 64 + lane];
-        cplx s02 = cadd(y0, y2), d02 = csub(y0, y2), s13 = cadd(y1, y3), d13 = mul_mi<false>(csub(y1, y3));
-        cplx z;
-        if (w == 0) z = cadd(s02, s13);
-        else if (w == 1) z = cadd(d02, d13);
-        else if (w == 2) z = csub(s02, s13);
-        else z = csub(d02, d13);
-        x[s] = z;
-    }
-}
-
-// Cross-wave inverse step: x = Z[k + 256 q] of this wave q = w on entry; on
-// return x holds 1024 x the folded points of subsequence r = w.  Caller
-// guarantees xbuf is free.
-__device__ __forceinline__ void fft1024_inv(cplx (&x)[4], cplx* lds, cplx* xbuf, const FFT1024Tables& T, int w,
-                                            int lane) {
-#pragma unroll
-    for (int s = 0; s < 4; ++s) xbuf[(w * 4 + s) * 64 + lane] = x[s];
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        cplx z0 = xbuf[(0 * 4 + s) * 64 + lane], z1 = xbuf[(1 * 4 + s) * 64 + lane];
-        cplx z2 = xbuf[(2 * 4 + s) * 64 + lane], z3 = xbuf[(3 * 4 + s) * 64 + lane];
-        // Y'_r = sum_q W4^(-r q) Z_q ; W4^-1 = +i
-        cplx s02 = cadd(z0, z2), d02 = csub(z0, z2), s13 = cadd(z1, z3), d13 = mul_mi<true>(csub(z1, z3));
-        cplx y;
-        if (w == 0) y = cadd(s02, s13);
-        else if (w == 1) y = cadd(d02, d13);
-        else if (w == 2) y = csub(s02, s13);
-        else y = csub(d02, d13);
-        x[s] = cmulc(y, T.tr[(w * 4 + s) * 64 + lane]);
-    }
-    fft256_inv(x, lds, T, lane);
+// Output w of the 4-point DFT across the four waves of an FFT1024,
+// X_w = sum_r y_r W^(r w) with W = -i (forward) or +i (inverse), written as
+// X_w = (y0 + e y2) + W^w (y1 + e y3), e = (-1)^w, W^w = c + i t.  The wave index
+// is uniform, so a select on it compiles to scalar branches that cut the FFT into
+// basic blocks; the constants e, c, t in {0, +-1} make it straight-line FMAs
+// with the same rounding points as the add/subtract form (products by 0 and +-1
+// are exact).
+template <bool INV>
+__device__ __forceinline__ cplx xwave_dft4(cplx y0, cplx y1, cplx y2, cplx y3, int w) {
+    const double e = (w & 1) ? -1.0 : 1.0;
+    const double c = (w == 0) ? 1.0 : (w == 2) ? -1.0 : 0.0;
+    const double tf = (w == 1) ? -1.0 : (w == 3) ? 1.0 : 0.0;  // sin(-pi w / 2)
+    const double t = INV ? -tf : tf;
+    const cplx A = {fma(e, y2.re, y0.re), fma(e, y2.im, y0.im)};
+    const cplx B = {fma(e, y3.re, y1.re), fma(e, y3.im, y1.im)};
+    return {fma(-t, B.im, fma(c, B.re, A.re)), fma(t, B.re, fma(c, B.im, A.im))};
 }
 
 // round(x * scale) mod 2^64 through two FP64 limbs is done by the callers;

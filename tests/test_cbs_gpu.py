@@ -94,9 +94,9 @@ def test_pbs_manylut_bit_exact(ctx, okeys):
 
 
 def test_pbs_large_batch_form_matches(ctx, okeys):
-    """A 300-ciphertext PBS (more workgroups than fit at once) equals the same
-    inputs bootstrapped in small sub-batches and the oracle on sampled inputs,
-    bit for bit."""
+    """Batches above 256 run the two-waves-per-SIMD PBS form, small ones the
+    branch-free form: 300 bootstraps equal the same inputs bootstrapped in small
+    sub-batches and the oracle on sampled inputs, bit for bit."""
     import torch
     import tfhe_amd
     batch = 300
