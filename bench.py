@@ -89,6 +89,16 @@ def aes_ctr_bench(batches, reps=1):
             bad += v != want
         out[str(blocks)] = {"blocks_per_s": round(blocks / dt, 3), "s_per_launch": round(dt, 4),
                             "circuit_bootstraps_per_s": round(blocks * 1280 / dt, 1), "blocks_wrong": int(bad)}
+        if blocks <= 32:  # the same keystream replayed from its captured hipGraph
+            eager = d_ks[:blocks].clone()
+            ctx.aes_ctr_keystream(ctrs, d_ks, graph=True)  # capture + first replay
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ctx.aes_ctr_keystream(ctrs, d_ks, graph=True)
+            torch.cuda.synchronize()
+            gdt = time.perf_counter() - t0
+            out[str(blocks)]["graph_s_per_launch"] = round(gdt, 4)
+            out[str(blocks)]["graph_equals_eager"] = bool(torch.equal(d_ks[:blocks], eager))
     ctx.close()
     return {"form": "fp64 FFT: PBS on the 64-bit torus (2-limb exact), PFKS / CMux LUT on the 32-bit torus "
                     "(1-limb exact)", "keygen_s": round(keygen_s, 2), "batches": out}

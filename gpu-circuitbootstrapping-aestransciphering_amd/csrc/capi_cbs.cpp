@@ -117,6 +117,16 @@ struct tfhe_cbs_ctx {
     uint32_t* d_rk = nullptr;                   // [11][128][n+1] encrypted round-key bits
     uint8_t* d_ctr = nullptr;                   // [max_blocks][16]
     bool has_round_keys = false;
+    // captured CTR keystreams (aes_init + 10 rounds), one per (blocks, output buffer),
+    // replayed on the context's own stream, ordered after / before the caller's stream by events
+    struct AesGraph {
+        uint32_t blocks;
+        uint32_t* d_out;
+        hipGraphExec_t exec;
+    };
+    std::vector<AesGraph> graphs;
+    hipStream_t gstream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
 };
 
 extern "C" int tfhe_cbs_ctx_destroy(tfhe_cbs_ctx* c) {
@@ -125,6 +135,10 @@ extern "C" int tfhe_cbs_ctx_destroy(tfhe_cbs_ctx* c) {
                     c->d_state[0], c->d_state[1], c->d_sbox_poly, c->d_lin_mix, c->d_lin_last, c->d_rk, c->d_ctr};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    if (c->ev_in) (void)hipEventDestroy(c->ev_in);
+    if (c->ev_out) (void)hipEventDestroy(c->ev_out);
+    if (c->gstream) (void)hipStreamDestroy(c->gstream);
     c->tables.release();
     delete c;
     return TFHE_OK;
@@ -427,23 +441,71 @@ extern "C" int tfhe_aes_round_batch(tfhe_cbs_ctx* c, int round, const uint32_t* 
     return rc;
 }
 
-extern "C" int tfhe_aes_ctr_keystream_batch(tfhe_cbs_ctx* c, const uint8_t* h_ctr, uint32_t blocks, uint32_t* d_out,
-                                            void* stream) {
-    if (!c || !h_ctr || !d_out) return fail(TFHE_E_PARAM, "aes_ctr: bad argument");
-    if (!c->has_round_keys) return fail(TFHE_E_STATE, "aes_ctr: no round keys");
-    if (blocks > c->max_blocks) return fail(TFHE_E_CAPACITY, "aes_ctr: blocks exceed the context capacity");
-    if (!blocks) return TFHE_OK;
-    hipStream_t st = (hipStream_t)stream;
-    int rc = hipcheck(hipMemcpyAsync(c->d_ctr, h_ctr, (size_t)blocks * 16, hipMemcpyHostToDevice, st), "ctr");
-    if (rc) return rc;
+// the keystream's kernels after the counter upload: state init from round key 0
+// and the counter blocks in d_ctr, then rounds 1..10 ping-ponging the state buffers
+static int aes_keystream_kernels(tfhe_cbs_ctx* c, uint32_t blocks, uint32_t* d_out, hipStream_t st) {
     launch_aes_init(c->d_rk, c->d_ctr, blocks, c->p.n, c->d_state[0], st);
-    rc = check_launch("aes_init");
+    int rc = check_launch("aes_init");
     int cur = 0;
     for (int round = 1; round <= 10 && !rc; ++round) {
         uint32_t* dst = (round == 10) ? d_out : c->d_state[cur ^ 1];
-        rc = tfhe_aes_round_batch(c, round, c->d_state[cur], blocks, dst, stream);
+        rc = tfhe_aes_round_batch(c, round, c->d_state[cur], blocks, dst, st);
         cur ^= 1;
     }
+    return rc;
+}
+
+static int aes_ctr_check(const tfhe_cbs_ctx* c, const uint8_t* h_ctr, uint32_t blocks, const uint32_t* d_out) {
+    if (!c || !h_ctr || !d_out) return fail(TFHE_E_PARAM, "aes_ctr: bad argument");
+    if (!c->has_round_keys) return fail(TFHE_E_STATE, "aes_ctr: no round keys");
+    if (blocks > c->max_blocks) return fail(TFHE_E_CAPACITY, "aes_ctr: blocks exceed the context capacity");
+    return TFHE_OK;
+}
+
+extern "C" int tfhe_aes_ctr_keystream_batch(tfhe_cbs_ctx* c, const uint8_t* h_ctr, uint32_t blocks, uint32_t* d_out,
+                                            void* stream) {
+    int rc = aes_ctr_check(c, h_ctr, blocks, d_out);
+    if (rc || !blocks) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    rc = hipcheck(hipMemcpyAsync(c->d_ctr, h_ctr, (size_t)blocks * 16, hipMemcpyHostToDevice, st), "ctr");
+    if (rc) return rc;
+    return aes_keystream_kernels(c, blocks, d_out, st);
+}
+
+extern "C" int tfhe_aes_ctr_keystream_graph(tfhe_cbs_ctx* c, const uint8_t* h_ctr, uint32_t blocks, uint32_t* d_out,
+                                            void* stream) {
+    int rc = aes_ctr_check(c, h_ctr, blocks, d_out);
+    if (rc || !blocks) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    if (!c->gstream) {
+        rc = hipcheck(hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking), "graph stream");
+        if (!rc) rc = hipcheck(hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming), "graph event");
+        if (!rc) rc = hipcheck(hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming), "graph event");
+        if (rc) return rc;
+    }
+    // the counter upload stays on the caller's stream, outside the graph
+    rc = hipcheck(hipMemcpyAsync(c->d_ctr, h_ctr, (size_t)blocks * 16, hipMemcpyHostToDevice, st), "ctr");
+    if (!rc) rc = hipcheck(hipEventRecord(c->ev_in, st), "event");
+    if (!rc) rc = hipcheck(hipStreamWaitEvent(c->gstream, c->ev_in, 0), "event wait");
+    if (rc) return rc;
+    hipGraphExec_t exec = nullptr;
+    for (const auto& g : c->graphs)
+        if (g.blocks == blocks && g.d_out == d_out) exec = g.exec;
+    if (!exec) {
+        rc = hipcheck(hipStreamBeginCapture(c->gstream, hipStreamCaptureModeThreadLocal), "begin capture");
+        if (rc) return rc;
+        rc = aes_keystream_kernels(c, blocks, d_out, c->gstream);
+        hipGraph_t graph = nullptr;
+        const int rc_end = hipcheck(hipStreamEndCapture(c->gstream, &graph), "end capture");
+        if (!rc) rc = rc_end;
+        if (!rc) rc = hipcheck(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc) return rc;
+        c->graphs.push_back({blocks, d_out, exec});
+    }
+    rc = hipcheck(hipGraphLaunch(exec, c->gstream), "graph launch");
+    if (!rc) rc = hipcheck(hipEventRecord(c->ev_out, c->gstream), "event");
+    if (!rc) rc = hipcheck(hipStreamWaitEvent(st, c->ev_out, 0), "event wait");
     return rc;
 }
 

@@ -214,6 +214,16 @@ int tfhe_aes_round_batch(tfhe_cbs_ctx* ctx, int round, const uint32_t* d_in, uin
  * public counter blocks h_ctr [blocks][16] (host) */
 int tfhe_aes_ctr_keystream_batch(tfhe_cbs_ctx* ctx, const uint8_t* h_ctr, uint32_t blocks, uint32_t* d_out,
                                  void* stream);
+/* The same keystream replayed from a captured hipGraph (the reference captures
+ * its LUT and circuit-bootstrap launch sequences as CUDA graphs with thread-local
+ * capture, gpu_lut.cuh:1859-2039, cirbts-batch.cu:559-581): the first
+ * call for a (blocks, d_out) pair captures state init + the 10 rounds on the
+ * context's own stream, later calls upload the counters on `stream` and replay,
+ * ordered after and before `stream` by events.  Results equal
+ * tfhe_aes_ctr_keystream_batch bit for bit.  Not thread-safe (the context caches
+ * the graphs). */
+int tfhe_aes_ctr_keystream_graph(tfhe_cbs_ctx* ctx, const uint8_t* h_ctr, uint32_t blocks, uint32_t* d_out,
+                                 void* stream);
 /* XOR of public bits into LWE bits: d_ct[i] += (0, bit_i * 2^31), d_pub packed bits */
 int tfhe_lwe_xor_public(uint32_t* d_ct, const uint8_t* d_pub, uint32_t nbits, int n, void* stream);
 

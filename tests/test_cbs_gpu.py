@@ -276,3 +276,23 @@ def test_aes128_fips197_and_random_blocks(ctx, okeys):
     ph = (got[:, :, -1].astype(np.uint64) - (got[:, :, :-1].astype(np.uint64) @ s64)) % 2 ** 32
     err = ((ph + 2 ** 30) % 2 ** 31).astype(np.int64) - 2 ** 30
     assert np.abs(err).max() < 2 ** 29  # within half of the 2^30 margin
+
+
+def test_aes_keystream_graph_replays_match_eager(ctx, okeys):
+    """The hipGraph-captured keystream equals the eager launch sequence on every
+    replay, with eager work and new counter blocks in between replays."""
+    import torch
+    import tfhe_amd
+    key = bytes(range(100, 116))
+    ctx.set_round_keys(tfhe_amd.aes_encrypt_key_bits(okeys.s, 79, key))
+    rng = np.random.default_rng(8)
+    d_eager = torch.empty((2, 128, 637), dtype=torch.int32, device="cuda")
+    d_graph = torch.empty((2, 128, 637), dtype=torch.int32, device="cuda")
+    for rep in range(3):
+        ctrs = [bytes(rng.integers(0, 256, 16, dtype=np.uint8)) for _ in range(2)]
+        ctx.aes_ctr_keystream(ctrs, d_eager)
+        torch.cuda.synchronize()
+        ctx.aes_ctr_keystream(ctrs, d_graph, graph=True)
+        torch.cuda.synchronize()
+        assert np.array_equal(from_dev(d_graph), from_dev(d_eager)), rep
+        assert _decrypt_blocks(okeys, from_dev(d_graph), 2) == [O.aes128_encrypt(key, c) for c in ctrs]
