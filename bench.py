@@ -113,6 +113,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3, help="plain per-bootstrap form samples")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run three 512 MiB device copies (the HBM-counter calibration kernel)")
+    ap.add_argument("--valu-rates", action="store_true",
+                    help="also measure the FP64 and (unpacked) FP32 FMA throughput of the library's probe kernel")
     ap.add_argument("--aes-blocks", default="1,17,256",
                     help="comma-separated AES-128 CTR batch sizes to time ('' to skip)")
     args = ap.parse_args()
@@ -242,6 +244,9 @@ def main():
         "decrypt_failures": failures,
     }
     ctx.close()
+    if args.valu_rates:
+        line["valu_fma_tflops"] = {"fp64": round(tfhe_amd.probe_fma_tflops(1), 2),
+                                   "fp32_unpacked": round(tfhe_amd.probe_fma_tflops(0), 2)}
     if args.aes_blocks:
         line["aes128_ctr"] = aes_ctr_bench([int(b) for b in args.aes_blocks.split(",") if b])
     if not args.no_cpu_baseline:

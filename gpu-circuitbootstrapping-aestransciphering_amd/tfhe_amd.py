@@ -103,6 +103,7 @@ SIGNATURES = {
     "tfhe_aes_round_batch": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_aes_ctr_keystream_batch": (ctypes.c_int, [_vp, np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
                                                     ctypes.c_uint32, _vp, _vp]),
+    "tfhe_probe_fma_tflops": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "tfhe_aes_ctr_keystream_graph": (ctypes.c_int, [_vp, np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
                                                     ctypes.c_uint32, _vp, _vp]),
     "tfhe_lwe_xor_public": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_int, _vp]),
@@ -294,3 +295,10 @@ class CbsContext:
         ctr = np.ascontiguousarray(np.frombuffer(b"".join(ctr_blocks), np.uint8))
         fn = lib().tfhe_aes_ctr_keystream_graph if graph else lib().tfhe_aes_ctr_keystream_batch
         check(fn(self.h, ctr, len(ctr_blocks), _ptr(d_out), _stream(stream)))
+
+
+def probe_fma_tflops(fp64):
+    """Measured FMA throughput (TFLOP/s) of the library's FP64 or FP32 VALU chain kernel."""
+    v = ctypes.c_double(0.0)
+    check(lib().tfhe_probe_fma_tflops(int(fp64), ctypes.byref(v)))
+    return v.value

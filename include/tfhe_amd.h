@@ -193,6 +193,12 @@ int tfhe_cbs_keyswitch_batch(tfhe_cbs_ctx* ctx, const uint32_t* d_lwe32, uint32_
  * d_out = d_x * d_y mod (X^N + 1, 2^64), small-integer d_x, torus-64 d_y. */
 int tfhe_polymul2048_batch(const int32_t* d_x, const uint64_t* d_y, uint32_t batch, uint64_t* d_out, void* stream);
 
+/* Measured VALU FMA throughput (TFLOP/s) of a chain kernel in FP64 (fp64 = 1)
+ * or FP32 (fp64 = 0), built like the rest of the library (no packed FP32).  No
+ * counterpart in the reference: it is the measured basis of the choice of FP64
+ * over split-precision FP32 for the external product on this GPU. */
+int tfhe_probe_fma_tflops(int fp64, double* tflops);
+
 /* ------------------------------------------------------------------------- */
 /* AES-128 transciphering (reference bitwise/examples/aes_transcipher/main.cu */
 /* RunTranscipherBatch :1316-1619).  State bit 8*byte + j (byte = row + 4 col)*/
