@@ -24,7 +24,7 @@ struct DeviceTables {
     cplx* d_buf = nullptr;
     int init(int N) {
         if (N != 1024) return fail(TFHE_E_PARAM, "FFT tables: only N = 1024 is built");
-        std::vector<cplx> h(2 * 512);
+        std::vector<cplx> h(2 * 512 + 2 * N);
         const double PI = 3.14159265358979323846264338327950288;
         for (int k = 0; k < 8; ++k)
             for (int b = 0; b < 64; ++b) {
@@ -37,12 +37,17 @@ struct DeviceTables {
                 const double ang = -2.0 * PI * (double)(((l & 7) * e) % 64) / 64.0;
                 h[512 + e * 64 + l] = {std::cos(ang), std::sin(ang)};
             }
+        for (int j = 0; j < 2 * N; ++j) {  // w^j, the evaluation points' powers
+            const double ang = PI * j / N;
+            h[1024 + j] = {std::cos(ang), std::sin(ang)};
+        }
         int rc = dmalloc((void**)&d_buf, h.size() * sizeof(cplx));
         if (rc) return rc;
         rc = hipcheck(hipMemcpy(d_buf, h.data(), h.size() * sizeof(cplx), hipMemcpyHostToDevice), "tables");
         if (rc) return rc;
         T.p1t = d_buf;
         T.p2 = d_buf + 512;
+        T.rot = d_buf + 1024;
         return TFHE_OK;
     }
     void release() {

@@ -166,6 +166,7 @@ constexpr int FFT512_LDS_CPLX = 8 * 72;
 struct FFTTables {
     const cplx* p1t;  // [8][64]
     const cplx* p2;   // [8][64]
+    const cplx* rot;  // [2N] w^j, w = exp(i pi / N): X[k] of the transform is p(w^(1 - 4k))
 };
 
 // w^(64a) = exp(i pi a / 16), a = 0..7

@@ -149,10 +149,9 @@ __global__ __launch_bounds__(256) void k_gate_blind_rotate(
         tw_lds[512 + k] = Tg.p2[k];
     }
     __syncthreads();
-    const FFTTables T{tw_lds, tw_lds + 512};
+    const FFTTables T{tw_lds, tw_lds + 512, Tg.rot};
     if (cidx >= batch) return;
     cplx* lds = lds_all[w];
-    uint32_t* ldsu = reinterpret_cast<uint32_t*>(lds);
     uint16_t* at = at_all[w];
     const uint32_t* c1 = ct1 + (size_t)cidx * (n + 1);
     const uint32_t* c2 = ct2 + (size_t)cidx * (n + 1);
@@ -209,33 +208,31 @@ __global__ __launch_bounds__(256) void k_gate_blind_rotate(
             }
         }
 
-        const int kshift = (int)(mrot & (N - 1));
-        const bool hi = mrot >= (uint32_t)N;
+        // CGGI monic update acc += (X^m - 1) t in the Fourier domain: X[k] is the
+        // polynomial at zeta_k = w^(1 - 4k), so X^m multiplies it by w^(m (1 - 4k));
+        // one rounding of (X^m - 1) t gives the same integers as rotating the rounded t
+        cplx rf[8];
+        {
+            const uint32_t kl = (uint32_t)((lane >> 3) + 8 * (lane & 7));  // k = kl + 64 f
+#pragma unroll
+            for (int f = 0; f < 8; ++f) {
+                const uint32_t k = kl + 64u * f;
+                const uint32_t ex = (mrot * ((2u * N + 1u - 4u * k) & (2u * N - 1u))) & (2u * N - 1u);
+                const cplx r = T.rot[ex];
+                rf[f] = {r.re - 1.0, r.im};
+            }
+        }
 #pragma unroll
         for (int o = 0; o < 2; ++o) {
             cplx x[8];
 #pragma unroll
-            for (int f = 0; f < 8; ++f) x[f] = macc[o][f];
+            for (int f = 0; f < 8; ++f) x[f] = cmul(macc[o][f], rf[f]);
             fft512_inv(x, lds, T, lane);
-            uint32_t tv[16];
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
-                tv[a] = round_mod32(x[a].re, 1.0 / M);
-                tv[a + 8] = round_mod32(x[a].im, 1.0 / M);
+                acc[o][a] += round_mod32(x[a].re, 1.0 / M);
+                acc[o][a + 8] += round_mod32(x[a].im, 1.0 / M);
             }
-#pragma unroll
-            for (int s = 0; s < 16; ++s) ldsu[64 * s + lane] = tv[s];
-            wave_sync();
-#pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                int idx = 64 * s + lane - kshift;
-                const bool wrapped = idx < 0;
-                idx += wrapped ? N : 0;
-                uint32_t v = ldsu[idx];
-                if (hi ^ wrapped) v = 0u - v;
-                acc[o][s] += v - tv[s];
-            }
-            wave_sync();
         }
     }
 
