@@ -505,7 +505,7 @@ __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict
 // the workgroups of one key tile share an XCD (their linear ids differ by
 // multiples of 64), so HBM sees it about once.
 // ===========================================================================
-constexpr int PF_WAVES = 8, PF_CT = 32, PF_SLICES = 8, PF_GROUP = PF_WAVES * PF_CT;
+constexpr int PF_WAVES = 8, PF_CT = 32, PF_GROUP = PF_WAVES * PF_CT;
 
 __global__ __launch_bounds__(256) void k_pfks_digits(const uint64_t* __restrict__ lwe64, uint32_t B, uint32_t Bpad,
                                                     uint32_t* __restrict__ at) {
@@ -530,18 +530,19 @@ __global__ __launch_bounds__(256) void k_pfks_digits(const uint64_t* __restrict_
 
 __global__ __launch_bounds__(64 * PF_WAVES) void k_pfks32(const uint32_t* __restrict__ at, uint32_t B, uint32_t Bpad,
                                                          const uint32_t* __restrict__ key, size_t key_u_stride,
-                                                         int u0, uint32_t* __restrict__ out, size_t out_stride,
-                                                         size_t out_u_offset) {
+                                                         int u0, int slices, uint32_t* __restrict__ out,
+                                                         size_t out_stride, size_t out_u_offset) {
     // two adjacent columns per lane: one LDS address and one 8-byte read feed two subtractions
     __shared__ uint2 tab[2][16][64];
     constexpr int N = CB_N, COLS = 2 * CB_N;
-    constexpr int ISLICE = (N + 1 + PF_SLICES - 1) / PF_SLICES;
+    const int ISLICE = (N + 1 + slices - 1) / slices;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int slice = blockIdx.z % PF_SLICES;
-    const int u = u0 + (int)(blockIdx.z / PF_SLICES);
+    const int slice = blockIdx.z % slices;
+    const int u = u0 + (int)(blockIdx.z / slices);
     const int ibeg = slice * ISLICE;
     const int iend = (ibeg + ISLICE) < (N + 1) ? (ibeg + ISLICE) : (N + 1);
+    if (ibeg >= iend) return;  // empty slice (uniform over the workgroup)
     const uint32_t cbase = blockIdx.y * PF_GROUP + w * PF_CT;
     const bool active = cbase < B;  // idle waves still take part in the barriers
     // this thread's share of a tile: candidate v = 1 + w (all waves) and v = 9 + w (waves 0..6)
@@ -815,8 +816,13 @@ void launch_zero_rows32(uint32_t* out, uint32_t count, size_t stride, uint32_t w
 
 void launch_pfks32(const uint32_t* at, uint32_t B, const uint32_t* key, size_t key_u_stride, int u0, int nu,
                    uint32_t* out, size_t out_stride, size_t out_u_offset, hipStream_t st) {
-    dim3 grid(2 * CB_N / 128, pfks_bpad(B) / PF_GROUP, PF_SLICES * nu);
-    k_pfks32<<<grid, 64 * PF_WAVES, 0, st>>>(at, B, pfks_bpad(B), key, key_u_stride, u0, out, out_stride,
+    // i-slices: enough workgroups to keep the key stream busy at small batches (about 1536),
+    // few at large ones (each slice adds one atomic pass over the outputs)
+    const unsigned tiles = 2 * CB_N / 128, groups = pfks_bpad(B) / PF_GROUP;
+    const unsigned want = (1536 + tiles * groups * nu - 1) / (tiles * groups * nu);
+    const int slices = (int)(want < 4 ? 4 : want > 32 ? 32 : want);
+    dim3 grid(tiles, groups, (unsigned)(slices * nu));
+    k_pfks32<<<grid, 64 * PF_WAVES, 0, st>>>(at, B, pfks_bpad(B), key, key_u_stride, u0, slices, out, out_stride,
                                               out_u_offset);
 }
 
