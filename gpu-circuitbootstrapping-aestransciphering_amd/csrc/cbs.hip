@@ -555,20 +555,27 @@ __global__ __launch_bounds__(64 * PF_WAVES) void k_pfks32(const uint32_t* __rest
     uint32_t acc[PF_CT][2];
 #pragma unroll
     for (int k = 0; k < PF_CT; ++k) acc[k][0] = acc[k][1] = 0;
-    uint2 kv0 = make_uint2(0, 0), kv1 = make_uint2(0, 0);
-    auto load_tile = [&](int i, int t) {
+    // Tile m (m = 8 (i - ibeg) + t) is loaded into register slot m % PF_RING at step
+    // m - PF_RING, stored into tab[m & 1] at the start of step m - 1 and read at step m:
+    // PF_RING - 1 steps of latency cover the key stream.  With the t loop unrolled by 8
+    // every slot index is a compile-time constant.
+    constexpr int PF_RING = 4;
+    uint2 kv[PF_RING][2];
+    auto load_tile = [&](int slot, int i, int t) {
         uint64_t off = ((uint64_t)(i * 8 + t) * 15 + w) * (COLS / 2);
         asm volatile("" : "+s"(off));  // keep the prefetch where it is written
         const uint2* kr = kcol + off;
-        kv0 = kr[0];
-        if (second) kv1 = kr[(size_t)8 * (COLS / 2)];
+        kv[slot][0] = kr[0];
+        kv[slot][1] = second ? kr[(size_t)8 * (COLS / 2)] : make_uint2(0, 0);
     };
-    auto store_tile = [&](int b) {
-        tab[b][1 + w][lane] = kv0;
-        if (second) tab[b][9 + w][lane] = kv1;
+    auto store_tile = [&](int slot, int b) {
+        tab[b][1 + w][lane] = kv[slot][0];
+        if (second) tab[b][9 + w][lane] = kv[slot][1];
     };
-    load_tile(ibeg, 0);
-    store_tile(0);
+#pragma unroll
+    for (int m = 0; m < PF_RING; ++m)
+        if (ibeg + m / 8 < iend) load_tile(m, ibeg + m / 8, m % 8);
+    store_tile(0, 0);
     __syncthreads();
     for (int i = ibeg; i < iend; ++i) {
         uint32_t a[PF_CT];
@@ -579,8 +586,10 @@ __global__ __launch_bounds__(64 * PF_WAVES) void k_pfks32(const uint32_t* __rest
         }
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
-            const bool more = (t < 7) || (i + 1 < iend);
-            if (more) load_tile(t < 7 ? i : i + 1, (t + 1) & 7);
+            // store tile m + 1 (loaded PF_RING - 1 steps ago), refill its slot with tile m + PF_RING
+            const int t1 = t + 1, tr = t + PF_RING;
+            if (i + t1 / 8 < iend) store_tile(t1 % PF_RING, t1 & 1);
+            if (i + tr / 8 < iend) load_tile(tr % PF_RING, i + tr / 8, tr % 8);
             if (active) {
 #pragma unroll
                 for (int k = 0; k < PF_CT; ++k) {
@@ -591,7 +600,6 @@ __global__ __launch_bounds__(64 * PF_WAVES) void k_pfks32(const uint32_t* __rest
             }
 #pragma unroll
             for (int k = 0; k < PF_CT; ++k) asm volatile("" : "+v"(acc[k][0]), "+v"(acc[k][1]));
-            if (more) store_tile((t + 1) & 1);
             __syncthreads();
         }
     }
