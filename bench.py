@@ -113,6 +113,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3, help="plain per-bootstrap form samples")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run three 512 MiB device copies (the HBM-counter calibration kernel)")
+    ap.add_argument("--campaign", type=int, default=1_000_000,
+                    help="gate bootstraps in the decryption-failure campaign (0 to skip)")
     ap.add_argument("--valu-rates", action="store_true",
                     help="also measure the FP64 and (unpacked) FP32 FMA throughput of the library's probe kernel")
     ap.add_argument("--aes-blocks", default="1,17,256",
@@ -185,6 +187,37 @@ def main():
     dec = ((ph + (1 << 29)) >> 30) & 3
     failures = int(np.count_nonzero(dec != want))
 
+    # decryption-failure campaign: batches chained through their own outputs (inputs of
+    # batch r+1 are the outputs of batch r and a rotation of them), every output decrypted
+    campaign = None
+    if args.campaign > 0:
+        cur, cur_bits = d_out.clone(), want.copy()
+        d_new = torch.empty_like(d_out)
+        done = fails = 0
+        worst = 0
+        while done < args.campaign:
+            d1.copy_(cur)
+            d2.copy_(torch.roll(cur, 1, 0))
+            bits_a, bits_b = cur_bits, np.roll(cur_bits, 1)
+            for g, lo in ((tfhe_amd.NAND, 0), (tfhe_amd.XOR, half)):
+                sl = slice(lo, lo + half)
+                tfhe_amd.check(L.tfhe_gate_blind_rotate_batch(ctx.h, g, ptr(d1[sl]), ptr(d2[sl]), half,
+                                                              ptr(d_lweN[sl]), sptr))
+                tfhe_amd.check(L.tfhe_gate_keyswitch_batch(ctx.h, ptr(d_lweN[sl]), half, ptr(d_new[sl]), sptr))
+            exp = np.concatenate([1 - (bits_a[:half] & bits_b[:half]), bits_a[half:] ^ bits_b[half:]])
+            g_np = d_new.cpu().numpy().view(np.uint32)
+            ph_c = (g_np[:, -1].astype(np.uint64) - (g_np[:, :-1].astype(np.uint64) @ s64)) % 2 ** 32
+            dec_c = ((ph_c + (1 << 29)) >> 30) & 3
+            fails += int(np.count_nonzero(dec_c != exp))
+            err = ((ph_c - exp.astype(np.uint64) * (1 << 30) + 2 ** 31) % 2 ** 32).astype(np.int64) - 2 ** 31
+            worst = max(worst, int(np.abs(err).max()))
+            cur, d_new = d_new, cur
+            cur_bits = exp
+            done += BATCH
+        campaign = {"bootstraps": done, "failures": fails,
+                    "max_abs_phase_error_log2": round(float(np.log2(max(worst, 1))), 2), "margin_log2": 29.0,
+                    "inputs": "chained: each batch's inputs are the previous outputs and a rotation of them"}
+
     br_ms, ks_ms = [], []
     for k in range(0, len(events), 3):
         br_ms.append(events[k].elapsed_time(events[k + 1]))
@@ -243,6 +276,8 @@ def main():
         "plain_fp64_per_bootstrap_rate": None if plain_rate is None else round(plain_rate, 2),
         "decrypt_failures": failures,
     }
+    if campaign is not None:
+        line["failure_campaign"] = campaign
     ctx.close()
     if args.valu_rates:
         line["valu_fma_tflops"] = {"fp64": round(tfhe_amd.probe_fma_tflops(1), 2),
