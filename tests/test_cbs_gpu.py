@@ -296,3 +296,40 @@ def test_aes_keystream_graph_replays_match_eager(ctx, okeys):
         torch.cuda.synchronize()
         assert np.array_equal(from_dev(d_graph), from_dev(d_eager)), rep
         assert _decrypt_blocks(okeys, from_dev(d_graph), 2) == [O.aes128_encrypt(key, c) for c in ctrs]
+
+
+def test_cbs_error_paths(gpu):
+    """Capacity, argument and state errors come back as codes (the reference throws,
+    cirbts-batch.cu:97-100); nothing is launched."""
+    import torch
+    import tfhe_amd
+    L = tfhe_amd.lib()
+    c = tfhe_amd.CbsContext(max_blocks=1)
+    try:
+        d = torch.zeros((4, 637), dtype=torch.int32, device="cuda")
+        out = torch.zeros((4, 4, 2, 2048), dtype=torch.int32, device="cuda")
+        with pytest.raises(tfhe_amd.TfheError) as e:  # no keys yet
+            c.cbs_batch(d, out)
+        assert e.value.code == tfhe_amd.TFHE_E_STATE
+        with pytest.raises(tfhe_amd.TfheError) as e:  # no round keys
+            c.aes_ctr_keystream([bytes(16)], torch.zeros((1, 128, 637), dtype=torch.int32, device="cuda"))
+        assert e.value.code == tfhe_amd.TFHE_E_STATE
+        c.keygen(tfhe_amd.lwe_keygen(636, 4), 4)
+        with pytest.raises(tfhe_amd.TfheError) as e:  # 129 inputs in a 1-block (128) context
+            tfhe_amd.check(L.tfhe_cbs_batch(c.h, tfhe_amd._ptr(d), 129, tfhe_amd._ptr(out), None))
+        assert e.value.code == tfhe_amd.TFHE_E_CAPACITY
+        with pytest.raises(tfhe_amd.TfheError) as e:
+            tfhe_amd.check(L.tfhe_cbs_keyswitch_batch(c.h, tfhe_amd._ptr(d), 129, tfhe_amd._ptr(d), None))
+        assert e.value.code == tfhe_amd.TFHE_E_CAPACITY
+        with pytest.raises(tfhe_amd.TfheError) as e:  # function index out of range
+            tfhe_amd.check(L.tfhe_cbs_pfks_batch(c.h, 2, tfhe_amd._ptr(d), 1, 4096, tfhe_amd._ptr(out), None))
+        assert e.value.code == tfhe_amd.TFHE_E_PARAM
+        with pytest.raises(tfhe_amd.TfheError) as e:  # rows closer than 2N apart
+            tfhe_amd.check(L.tfhe_cbs_pfks_batch(c.h, 0, tfhe_amd._ptr(d), 1, 4095, tfhe_amd._ptr(out), None))
+        assert e.value.code == tfhe_amd.TFHE_E_PARAM
+        with pytest.raises(tfhe_amd.TfheError) as e:
+            tfhe_amd.check(L.tfhe_aes_round_batch(c.h, 11, tfhe_amd._ptr(d), 1, tfhe_amd._ptr(d), None))
+        assert e.value.code == tfhe_amd.TFHE_E_PARAM
+        c.cbs_batch(d, out, batch=0)  # empty batch is a no-op
+    finally:
+        c.close()
