@@ -331,7 +331,7 @@ __global__ void k_plain_extract(const uint32_t* __restrict__ acc, uint32_t* __re
 // least significant first (digit 0 selects the key's encryption of zero).
 // k_ks_digits rounds and transposes the inputs to [i][Bpad], so that one
 // coefficient of a wave's KS_CT ciphertexts is a uniform (scalar) read.
-// k_keyswitch32: a workgroup of KS_WAVES waves covers 64 output columns; per
+// k_keyswitch32: a workgroup of KS_WAVES waves covers 128 output columns; per
 // coefficient i the L * 2^LOGB candidate rows of those columns are staged once
 // in a double-buffered LDS tile (one barrier per i) and each wave subtracts,
 // for each of its ciphertexts and digits, the row the digit names: one LDS read
@@ -369,33 +369,39 @@ __global__ __launch_bounds__(64 * KS_WAVES) void k_keyswitch32(const uint32_t* _
                                                               const uint32_t* __restrict__ lweN,
                                                               const uint32_t* __restrict__ ksk,
                                                               uint32_t* __restrict__ out) {
+    // two adjacent columns per lane (128 per workgroup): one LDS address and one 8-byte
+    // read feed two subtractions.  Key rows are n + 1 = 637 words long, so the global
+    // loads stay 32-bit (odd rows are not 8-byte aligned).
     constexpr int B = 1 << LOGB, ROWS = L * B, PER = ROWS / KS_WAVES;
     static_assert(ROWS % KS_WAVES == 0, "tile rows must split evenly over the waves");
-    __shared__ uint32_t tab[2][ROWS][64];
+    __shared__ uint2 tab[2][ROWS][64];
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int col = blockIdx.x * 64 + lane;
-    const bool colok = col <= n;
+    const int col0 = blockIdx.x * 128 + 2 * lane;
+    const bool ok0 = col0 <= n, ok1 = col0 + 1 <= n;
     const int islice = N / (int)gridDim.z;
     const int ibeg = (int)blockIdx.z * islice, iend = ibeg + islice;
     const uint32_t cbase = blockIdx.y * KS_GROUP + w * KS_CT;
     const bool active = cbase < batch;  // idle waves still take part in the barriers
     const size_t rs = (size_t)(n + 1);
-    uint32_t kv[PER];
+    uint2 kv[PER];
     auto load_i = [&](int i) {  // this thread's rows w + KS_WAVES q of coefficient i
         uint64_t off = (uint64_t)i * ROWS * rs;
-        asm volatile("" : "+s"(off));  // keep the prefetch where it is written (see k_gate_blind_rotate)
-        const uint32_t* base = ksk + off + (colok ? col : 0);
+        asm volatile("" : "+s"(off));  // keep the prefetch where it is written
+        const uint32_t* base = ksk + off;
 #pragma unroll
-        for (int q = 0; q < PER; ++q) kv[q] = colok ? base[(size_t)(w + KS_WAVES * q) * rs] : 0u;
+        for (int q = 0; q < PER; ++q) {
+            const uint32_t* r = base + (size_t)(w + KS_WAVES * q) * rs;
+            kv[q] = make_uint2(ok0 ? r[col0] : 0u, ok1 ? r[col0 + 1] : 0u);
+        }
     };
     auto store = [&](int b) {
 #pragma unroll
         for (int q = 0; q < PER; ++q) tab[b][w + KS_WAVES * q][lane] = kv[q];
     };
-    uint32_t acc[KS_CT];
+    uint32_t acc[KS_CT][2];
 #pragma unroll
-    for (int k = 0; k < KS_CT; ++k) acc[k] = 0u;
+    for (int k = 0; k < KS_CT; ++k) acc[k][0] = acc[k][1] = 0u;
     load_i(ibeg);
     store(0);
     if (ibeg + 1 < iend) load_i(ibeg + 1);
@@ -412,19 +418,24 @@ __global__ __launch_bounds__(64 * KS_WAVES) void k_keyswitch32(const uint32_t* _
 #pragma unroll
             for (int j = 0; j < L; ++j)
 #pragma unroll
-                for (int k = 0; k < KS_CT; ++k) acc[k] -= tab[b][j * B + ((a[k] >> (j * LOGB)) & (B - 1))][lane];
+                for (int k = 0; k < KS_CT; ++k) {
+                    const uint2 v = tab[b][j * B + ((a[k] >> (j * LOGB)) & (B - 1))][lane];
+                    acc[k][0] -= v.x;
+                    acc[k][1] -= v.y;
+                }
         }
 #pragma unroll
-        for (int k = 0; k < KS_CT; ++k) asm volatile("" : "+v"(acc[k]));  // subtractions stay before the barrier
+        for (int k = 0; k < KS_CT; ++k) asm volatile("" : "+v"(acc[k][0]), "+v"(acc[k][1]));  // before the barrier
         __syncthreads();
     }
-    if (!active || !colok) return;
+    if (!active) return;
 #pragma unroll
     for (int k = 0; k < KS_CT; ++k) {
         if (cbase + k >= batch) break;
-        uint32_t v = acc[k];
-        if (col == n && blockIdx.z == 0) v += lweN[(size_t)(cbase + k) * (N + 1) + N];
-        atomicAdd(&out[(size_t)(cbase + k) * rs + col], v);
+        uint32_t* o = out + (size_t)(cbase + k) * rs;
+        const uint32_t bterm = (blockIdx.z == 0) ? lweN[(size_t)(cbase + k) * (N + 1) + N] : 0u;
+        if (ok0) atomicAdd(&o[col0], acc[k][0] + (col0 == n ? bterm : 0u));
+        if (ok1) atomicAdd(&o[col0 + 1], acc[k][1] + (col0 + 1 == n ? bterm : 0u));
     }
 }
 
@@ -520,7 +531,7 @@ int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int l
                        uint32_t* at, uint32_t* out, hipStream_t st) {
     if (!(logB == 2 && l == 8) || N % 64 || n + 1 > 64 * 1024) return -1;
     const uint32_t Bpad = ks_bpad(batch);
-    const unsigned cols = (unsigned)(n + 1 + 63) / 64, groups = Bpad / KS_GROUP;
+    const unsigned cols = (unsigned)(n + 1 + 127) / 128, groups = Bpad / KS_GROUP;
     // i-slices: enough workgroups to fill the chip (about 512), each slice >= 32 coefficients
     unsigned slices = 1;
     while (cols * groups * slices < 512 && (unsigned)N / (slices * 2) >= 32) slices *= 2;
