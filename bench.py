@@ -49,14 +49,15 @@ def cpu_baseline(sample=24):
                       f"oracle C restatement, FP64 radix-2 FFT, one thread"}
 
 
-def aes_ctr_bench(batches, reps=1):
+def aes_ctr_bench(batches, reps=1, pbs_approx=0):
     """AES-128 CTR keystream: circuit bootstrap + S-box LUT per round,
-    linear layers as LWE additions; every block decrypted and compared with plain AES."""
+    linear layers as LWE additions; every block decrypted and compared with plain AES.
+    pbs_approx=1 runs the one-product FP64 PBS instead of the exact two-limb one."""
     import numpy as np
     import torch
     import tfhe_amd
     maxb = max(batches)
-    ctx = tfhe_amd.CbsContext(max_blocks=maxb)
+    ctx = tfhe_amd.CbsContext(max_blocks=maxb, params=tfhe_amd.cbs_params(pbs_approx))
     sk = tfhe_amd.lwe_keygen(N_LWE, 3)
     t0 = time.perf_counter()
     ctx.keygen(sk, 3)
@@ -100,7 +101,9 @@ def aes_ctr_bench(batches, reps=1):
             out[str(blocks)]["graph_s_per_launch"] = round(gdt, 4)
             out[str(blocks)]["graph_equals_eager"] = bool(torch.equal(d_ks[:blocks], eager))
     ctx.close()
-    return {"form": "fp64 FFT: PBS on the 64-bit torus (2-limb exact), PFKS / CMux LUT on the 32-bit torus "
+    pbs = ("one FP64 product per digit polynomial, rounding error ~2^-33" if pbs_approx
+           else "2-limb exact")
+    return {"form": f"fp64 FFT: PBS on the 64-bit torus ({pbs}), PFKS / CMux LUT on the 32-bit torus "
                     "(1-limb exact)", "keygen_s": round(keygen_s, 2), "batches": out}
 
 
@@ -119,6 +122,8 @@ def main():
                     help="also measure the FP64 and (unpacked) FP32 FMA throughput of the library's probe kernel")
     ap.add_argument("--aes-blocks", default="1,17,256",
                     help="comma-separated AES-128 CTR batch sizes to time ('' to skip)")
+    ap.add_argument("--aes-pbs", choices=("both", "exact", "approx"), default="both",
+                    help="PBS precision forms of the AES runs: exact two-limb, one-product FP64, or both")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -283,7 +288,11 @@ def main():
         line["valu_fma_tflops"] = {"fp64": round(tfhe_amd.probe_fma_tflops(1), 2),
                                    "fp32_unpacked": round(tfhe_amd.probe_fma_tflops(0), 2)}
     if args.aes_blocks:
-        line["aes128_ctr"] = aes_ctr_bench([int(b) for b in args.aes_blocks.split(",") if b])
+        sizes = [int(b) for b in args.aes_blocks.split(",") if b]
+        if args.aes_pbs in ("both", "exact"):
+            line["aes128_ctr"] = aes_ctr_bench(sizes)
+        if args.aes_pbs in ("both", "approx"):
+            line["aes128_ctr_one_product_pbs"] = aes_ctr_bench(sizes, pbs_approx=1)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

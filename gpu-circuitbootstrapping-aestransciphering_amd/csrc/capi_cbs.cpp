@@ -19,10 +19,11 @@ bool cbs_params_supported(const tfhe_cbs_params* p) {
     return p && p->N == kN && p->bk_logB == 9 && p->bk_l == 4 && p->cb_logB == 8 && p->cb_l == 2 && p->pf_logB == 4 &&
            p->pf_l == 8 && p->ks_logB == 2 && p->ks_l == 8 && p->n > 0 && p->n <= 1024 && p->lwe_nshift >= 1 &&
            p->lwe_nshift <= 31 && p->rlwe_nshift >= -31 && p->rlwe_nshift <= 31 && p->pf_nshift >= 1 &&
-           p->pf_nshift <= 31;
+           p->pf_nshift <= 31 && (p->pbs_approx == 0 || p->pbs_approx == 1);
 }
 
-size_t bsk2_cplx(const tfhe_cbs_params& p) { return (size_t)p.n * 8 * 2 * 2 * 1024; }
+int pbs_limbs(const tfhe_cbs_params& p) { return p.pbs_approx ? 1 : 2; }
+size_t bsk2_cplx(const tfhe_cbs_params& p) { return (size_t)p.n * 8 * 2 * pbs_limbs(p) * 1024; }
 size_t pfks_words_per_u() { return (size_t)(kN + 1) * 8 * 15 * 2 * kN; }
 size_t ksk2_words(const tfhe_cbs_params& p) { return (size_t)kN * 8 * 4 * (p.n + 1); }
 
@@ -260,7 +261,10 @@ extern "C" int tfhe_cbs_keygen(tfhe_cbs_ctx* c, const int32_t* lwe_sk, uint64_t 
     if (!rc) {
         launch_to_fourier2048(d_K64, 1, c->tables.T, d_K2f, nullptr);
         launch_gen_bsk64(seed, p.n, p.rlwe_nshift, d_s, c->d_K2, d_K2f, c->tables.T, d_bsk, nullptr);
-        launch_to_fourier2048(d_bsk, (size_t)p.n * 8 * 2, c->tables.T, c->d_bsk2f, nullptr);
+        if (pbs_limbs(p) == 2)
+            launch_to_fourier2048(d_bsk, (size_t)p.n * 8 * 2, c->tables.T, c->d_bsk2f, nullptr);
+        else
+            launch_to_fourier2048_t64(d_bsk, (size_t)p.n * 8 * 2, c->tables.T, c->d_bsk2f, nullptr);
         launch_gen_pfks(seed, p.pf_nshift, c->d_K2, d_K2f, c->tables.T, c->d_pfks, nullptr);
         launch_gen_ksk32(seed, p.n, kN, p.ks_logB, p.ks_l, p.lwe_nshift, d_s, c->d_K2, c->d_ksk2, nullptr, DOM_KSK2);
         rc = check_launch("cbs keygen");
@@ -301,7 +305,7 @@ extern "C" int tfhe_cbs_pbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32
     if (rc) return rc;
     if (!d_lwe || !d_out) return fail(TFHE_E_PARAM, "pbs: NULL buffer");
     if (!batch) return TFHE_OK;
-    launch_pbs2(d_lwe, batch, c->p.n, c->d_bsk2f, c->tables.T, d_out, (hipStream_t)stream);
+    launch_pbs2(d_lwe, batch, c->p.n, c->d_bsk2f, pbs_limbs(c->p), c->tables.T, d_out, (hipStream_t)stream);
     return check_launch("pbs2");
 }
 

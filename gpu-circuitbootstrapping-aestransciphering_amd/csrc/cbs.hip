@@ -161,6 +161,38 @@ __global__ __launch_bounds__(256) void k_to_fourier2048_32(const uint32_t* __res
     }
 }
 
+// Fourier conversion of torus-64 polynomials in one limb: coefficient v (signed
+// 64-bit) becomes the double v * 2^-64 / M, so that an inverse transform of a
+// product with a digit polynomial lands directly in torus units (the 1/M of the
+// inverse folded in).  Output [poly][q][s][lane] (1024 cplx).
+__global__ __launch_bounds__(256) void k_to_fourier2048_t64(const uint64_t* __restrict__ coef, size_t npoly,
+                                                           FFT1024Tables Tg, cplx* __restrict__ out) {
+    __shared__ cplx tl[768];
+    __shared__ cplx lds_all[4][F256_LDS_CPLX];
+    __shared__ cplx xbuf[2][1024];
+    const FFT1024Tables T = stage_tables(Tg, tl);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    constexpr double SCALE = 0x1p-74;  // 2^-64 / M
+    cplx trw[4];
+    load_trw(Tg, w, lane, trw);
+    int buf = 0;
+    for (size_t q = blockIdx.x; q < npoly; q += gridDim.x) {
+        const uint64_t* p = coef + q * CB_N;
+        cplx x[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int c = own_coeff(w, lane, a);
+            x[a] = {(double)(int64_t)p[c] * SCALE, (double)(int64_t)p[c + CB_M] * SCALE};
+        }
+        f1024_fwd<false>(x, lds_all[w], xbuf[buf], T, trw, w, lane);
+        buf ^= 1;
+        cplx* o = out + q * CB_M;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) o[(w * 4 + s) * 64 + lane] = x[s];
+    }
+}
+
 // Exact negacyclic product of small-integer x by torus-64 y (N = 2048), one block per product.
 __global__ __launch_bounds__(256) void k_polymul2048(const int32_t* __restrict__ x, const uint64_t* __restrict__ y,
                                                     uint32_t batch, FFT1024Tables Tg, uint64_t* __restrict__ out) {
@@ -360,7 +392,26 @@ __device__ __forceinline__ uint32_t special_ms_2048(uint32_t v) {
     return (uint32_t)((((uint64_t)v + (1ull << 20)) >> 21) << 1) & 4095u;
 }
 
-template <bool BF>
+// round(y * 2^64) mod 2^64 for a torus value held as a double.  f = y - rint(y)
+// in [-1/2, 1/2] is exact; its top 32 bits are rint(f 2^32), taken with the
+// 1.5 * 2^52 rounding constant, and the exact residual r = f 2^32 - rint(f 2^32)
+// gives the low 32 bits the same way (borrowing one from the top half when
+// rint(r 2^32) < 0, i.e. l < MAGIC).
+__device__ __forceinline__ uint64_t torus64_from_double(double y) {
+    constexpr double MAGIC = 6755399441055744.0, TWO32 = 4294967296.0;
+    const double f = y - rint(y);
+    const double h = fma(f, TWO32, MAGIC);
+    const double r = fma(f, TWO32, -(h - MAGIC));
+    const double l = fma(r, TWO32, MAGIC);
+    const uint32_t hi = (uint32_t)__double_as_longlong(h) - (l < MAGIC ? 1u : 0u);
+    return ((uint64_t)hi << 32) | (uint32_t)__double_as_longlong(l);
+}
+
+// LIMBS = 2: BSK2 Fourier rows hold the two signed 32-bit limbs of each torus-64
+// coefficient and every product is rounded exactly.  LIMBS = 1: one Fourier row
+// of the coefficients scaled to the torus (k_to_fourier2048_t64); the product is
+// rounded once, to about 2^-36 of the torus per step.
+template <bool BF, int LIMBS>
 __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict__ lwe, uint32_t batch, int n,
                                                      const cplx* __restrict__ bskf, FFT1024Tables Tg,
                                                      uint64_t* __restrict__ out) {
@@ -400,12 +451,12 @@ __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict
         for (int i = 0; i < n; ++i) {
             const uint32_t mrot = (Q2 - (uint32_t)at[i]) & (Q2 - 1);
             if (mrot == 0) continue;
-            const cplx* key = bskf + (size_t)i * PBS_ROWS * 2 * 2 * CB_M;
-            cplx macc[2][2][4];
+            const cplx* key = bskf + (size_t)i * PBS_ROWS * 2 * LIMBS * CB_M;
+            cplx macc[2][LIMBS][4];
 #pragma unroll
             for (int o = 0; o < 2; ++o)
 #pragma unroll
-                for (int lb = 0; lb < 2; ++lb)
+                for (int lb = 0; lb < LIMBS; ++lb)
 #pragma unroll
                     for (int s = 0; s < 4; ++s) macc[o][lb][s] = {0.0, 0.0};
 #pragma unroll
@@ -421,30 +472,45 @@ __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict
                     for (int a = 0; a < 4; ++a) x[a] = {pbs_digit(tp[a], j), pbs_digit(tp[a + 4], j)};
                     f1024_fwd<BF>(x, lds, xbuf[buf], T, trw, w, lane);
                     buf ^= 1;
-                    const cplx* kr = key + (size_t)r * 4 * CB_M + (w * 4) * 64 + lane;
+                    const cplx* kr = key + (size_t)r * 2 * LIMBS * CB_M + (w * 4) * 64 + lane;
 #pragma unroll
                     for (int o = 0; o < 2; ++o)
 #pragma unroll
-                        for (int lb = 0; lb < 2; ++lb)
+                        for (int lb = 0; lb < LIMBS; ++lb)
 #pragma unroll
                             for (int s = 0; s < 4; ++s)
-                                cmac(macc[o][lb][s], x[s], kr[(o * 2 + lb) * CB_M + s * 64]);
+                                cmac(macc[o][lb][s], x[s], kr[(o * LIMBS + lb) * CB_M + s * 64]);
                 }
             }
             uint64_t t[2][8];
 #pragma unroll
             for (int o = 0; o < 2; ++o) {
-                cplx y0[4], y1[4];
+                if constexpr (LIMBS == 2) {
+                    cplx y0[4], y1[4];
 #pragma unroll
-                for (int s = 0; s < 4; ++s) { y0[s] = macc[o][0][s]; y1[s] = macc[o][1][s]; }
-                f1024_inv<BF>(y0, lds, xbuf[buf], T, trw, w, lane);
-                buf ^= 1;
-                f1024_inv<BF>(y1, lds, xbuf[buf], T, trw, w, lane);
-                buf ^= 1;
+                    for (int s = 0; s < 4; ++s) { y0[s] = macc[o][0][s]; y1[s] = macc[o][1][s]; }
+                    f1024_inv<BF>(y0, lds, xbuf[buf], T, trw, w, lane);
+                    buf ^= 1;
+                    f1024_inv<BF>(y1, lds, xbuf[buf], T, trw, w, lane);
+                    buf ^= 1;
 #pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    t[o][a] = (uint64_t)round_i64(y0[a].re, 1.0 / CB_M) + ((uint64_t)round_i64(y1[a].re, 1.0 / CB_M) << 32);
-                    t[o][a + 4] = (uint64_t)round_i64(y0[a].im, 1.0 / CB_M) + ((uint64_t)round_i64(y1[a].im, 1.0 / CB_M) << 32);
+                    for (int a = 0; a < 4; ++a) {
+                        t[o][a] = (uint64_t)round_i64(y0[a].re, 1.0 / CB_M) +
+                                  ((uint64_t)round_i64(y1[a].re, 1.0 / CB_M) << 32);
+                        t[o][a + 4] = (uint64_t)round_i64(y0[a].im, 1.0 / CB_M) +
+                                      ((uint64_t)round_i64(y1[a].im, 1.0 / CB_M) << 32);
+                    }
+                } else {
+                    cplx y0[4];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) y0[s] = macc[o][0][s];
+                    f1024_inv<BF>(y0, lds, xbuf[buf], T, trw, w, lane);
+                    buf ^= 1;
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        t[o][a] = torus64_from_double(y0[a].re);
+                        t[o][a + 4] = torus64_from_double(y0[a].im);
+                    }
                 }
             }
             // acc += X^mrot t - t, rotation across the four waves through LDS
@@ -800,14 +866,22 @@ void launch_xor_public(uint32_t* ct, const uint8_t* pub, uint32_t nbits, int n, 
     k_xor_public<<<(nbits + 255) / 256, 256, 0, st>>>(ct, pub, nbits, n);
 }
 
-void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, const FFT1024Tables& T, uint64_t* out,
-                 hipStream_t st) {
+void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, int limbs, const FFT1024Tables& T,
+                 uint64_t* out, hipStream_t st) {
     // one workgroup per ciphertext: up to 256 they run one wave per SIMD and the
     // branch-free FFT form is faster; beyond, the two-waves-per-SIMD form is
-    if (batch <= 256)
-        k_pbs2_manylut<true><<<batch, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
-    else
-        k_pbs2_manylut<false><<<grid_for(batch, 2048), 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+    const unsigned big = grid_for(batch, 2048);
+    if (limbs == 2) {
+        if (batch <= 256)
+            k_pbs2_manylut<true, 2><<<batch, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+        else
+            k_pbs2_manylut<false, 2><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+    } else {
+        if (batch <= 256)
+            k_pbs2_manylut<true, 1><<<batch, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+        else
+            k_pbs2_manylut<false, 1><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+    }
 }
 
 uint32_t pfks_bpad(uint32_t B) { return (B + PF_GROUP - 1) / PF_GROUP * PF_GROUP; }
@@ -836,6 +910,10 @@ void launch_pfks32(const uint32_t* at, uint32_t B, const uint32_t* key, size_t k
 
 void launch_to_fourier2048(const uint64_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st) {
     k_to_fourier2048<<<grid_for(npoly, 4096), 256, 0, st>>>(coef, npoly, T, out);
+}
+
+void launch_to_fourier2048_t64(const uint64_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st) {
+    k_to_fourier2048_t64<<<grid_for(npoly, 8192), 256, 0, st>>>(coef, npoly, T, out);
 }
 
 void launch_to_fourier2048_32(const uint32_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st) {

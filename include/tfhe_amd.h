@@ -139,6 +139,16 @@ typedef struct tfhe_cbs_params {
     int lwe_nshift;     /* level-0 noise 2^-15 */
     int rlwe_nshift;    /* BSK2 noise: sigma = 2^-(32 + rlwe_nshift) of the 64-bit torus (19) */
     int pf_nshift;      /* PFKS key noise: sigma = 2^-pf_nshift of the 32-bit torus (30) */
+    /* Precision form of the PBS's digit-polynomial x torus-64 key products (the
+     * reference picks limb splits for its FFT backend, cirbts-init.cu:1096-1120):
+     * 0 = two signed 32-bit limbs, each product rounded exactly, so the PBS output
+     *     is bit-identical to exact integer arithmetic (default);
+     * 1 = one FP64 product of the key scaled to the torus: half the inverse
+     *     transforms, MACs and key bytes.  The rounding adds about 2^-33 of the
+     *     torus per bootstrap to a PBS noise of about 2^-28, so outputs decrypt
+     *     as the exact form's do, but they are not the same words (a rounding
+     *     that moves a gadget digit brings in another key row). */
+    int pbs_approx;
 } tfhe_cbs_params;
 
 typedef struct tfhe_cbs_ctx tfhe_cbs_ctx;
@@ -150,7 +160,7 @@ int tfhe_cbs_ctx_destroy(tfhe_cbs_ctx* ctx);
 
 /* Key generation on the GPU from seed (replaces the fork's CirBTKeyGen,
  * main.cu:232-235, and gpu_init_cirbsk, cirbts-init.cu:200-865): RLWE secret
- * K2, BSK2 (torus-64, Fourier, 2 limbs), the torus-32 PFKS key (2 x 4.03 GB)
+ * K2, BSK2 (torus-64, Fourier, 2 limbs or 1 with pbs_approx), the torus-32 PFKS key (2 x 4.03 GB)
  * and KSK2. */
 int tfhe_cbs_keygen(tfhe_cbs_ctx* ctx, const int32_t* lwe_sk, uint64_t seed);
 int tfhe_cbs_get_rlwe_sk(const tfhe_cbs_ctx* ctx, int32_t* out);

@@ -298,12 +298,89 @@ def test_aes_keystream_graph_replays_match_eager(ctx, okeys):
         assert _decrypt_blocks(okeys, from_dev(d_graph), 2) == [O.aes128_encrypt(key, c) for c in ctrs]
 
 
+@pytest.fixture(scope="module")
+def actx(gpu, okeys):
+    """The same keys with the one-product FP64 PBS (pbs_approx = 1)."""
+    import tfhe_amd
+    c = tfhe_amd.CbsContext(max_blocks=3, params=tfhe_amd.cbs_params(pbs_approx=1))
+    c.keygen(okeys.s, SEED)
+    yield c
+    c.close()
+
+
+def _pbs_noise(K2, out, bit):
+    """|phase - bit * 2^(48 + 8j)| of both levels of a PBS output under K2 (torus-64)."""
+    K = K2.astype(np.int64).astype(np.uint64)
+    errs = []
+    for j in range(2):
+        ph = out[j, -1] - np.dot(out[j, :-1], K)
+        errs.append(abs(int((ph - np.uint64(bit << (48 + 8 * j))).view(np.int64))))
+    return max(errs)
+
+
+def test_pbs_approx_within_noise_bound(actx, okeys):
+    """The one-product PBS decrypts like the exact one.  Its FP64 rounding (about
+    2^-33 of the torus per bootstrap) can move a gadget digit by one, after which
+    the ciphertext words differ from the exact form's (another BSK row enters) while
+    the phase stays within the PBS noise; so both forms are compared by
+    decryption: every level of every output (300 inputs: the two-waves-per-SIMD
+    form; 4 inputs: the branch-free form) lies within 2^-24 of the torus of its
+    message, as do the oracle's exact outputs.  A CPU emulation measured phase
+    differences between the two forms of 2^-27.8 rms (both PBS noises)."""
+    import torch
+    import tfhe_amd
+    batch = 300
+    rng = np.random.default_rng(29)
+    bits = rng.integers(0, 2, batch)
+    cts = np.stack([tfhe_amd.lwe_encrypt(okeys.s, 33, k, int(bits[k]), p=2) for k in range(batch)])
+    d_cts = to_dev32(cts)
+    d_big = torch.empty((batch, 2, 2049), dtype=torch.int64, device="cuda")
+    actx.pbs_batch(d_cts, d_big)
+    d_small = torch.empty((4, 2, 2049), dtype=torch.int64, device="cuda")
+    actx.pbs_batch(d_cts[:4].contiguous(), d_small)
+    torch.cuda.synchronize()
+    big, small = from_dev(d_big, np.uint64), from_dev(d_small, np.uint64)
+    worst = max(_pbs_noise(okeys.K2, big[k], int(bits[k])) for k in range(batch))
+    worst = max([worst] + [_pbs_noise(okeys.K2, small[k], int(bits[k])) for k in range(4)])
+    exact = 0
+    for k in (0, 1, 150, 299):
+        want, err = okeys.pbs(cts[k])
+        assert err < 0.25
+        exact = max(exact, _pbs_noise(okeys.K2, want, int(bits[k])))
+    print(f"PBS noise: one-product 2^{np.log2(worst):.2f}, exact 2^{np.log2(exact):.2f} of 2^64")
+    assert worst < 2 ** 40 and exact < 2 ** 40
+
+
+def test_aes128_approx_pbs_form(actx, okeys):
+    """Homomorphic AES-128 with the one-product PBS: the FIPS-197 C.1 vector and a
+    seeded random block decrypt bit-exactly, phases inside half the margin."""
+    import torch
+    import tfhe_amd
+    key = bytes(range(16))
+    actx.set_round_keys(tfhe_amd.aes_encrypt_key_bits(okeys.s, 77, key))
+    blocks = [bytes.fromhex("00112233445566778899aabbccddeeff"), bytes(np.random.default_rng(5).integers(0, 256, 16, dtype=np.uint8))]
+    d_ks = torch.empty((2, 128, 637), dtype=torch.int32, device="cuda")
+    actx.aes_ctr_keystream(blocks, d_ks)
+    torch.cuda.synchronize()
+    got = from_dev(d_ks)
+    dec = _decrypt_blocks(okeys, got, 2)
+    assert dec[0].hex() == "69c4e0d86a7b0430d8cdb78070b4c55a"
+    assert dec[1] == O.aes128_encrypt(key, blocks[1])
+    s64 = okeys.s.astype(np.uint64)
+    ph = (got[:, :, -1].astype(np.uint64) - (got[:, :, :-1].astype(np.uint64) @ s64)) % 2 ** 32
+    err = ((ph + 2 ** 30) % 2 ** 31).astype(np.int64) - 2 ** 30
+    assert np.abs(err).max() < 2 ** 29
+
+
 def test_cbs_error_paths(gpu):
     """Capacity, argument and state errors come back as codes (the reference throws,
     cirbts-batch.cu:97-100); nothing is launched."""
     import torch
     import tfhe_amd
     L = tfhe_amd.lib()
+    with pytest.raises(tfhe_amd.TfheError) as e:  # no such PBS precision form
+        tfhe_amd.CbsContext(max_blocks=1, params=tfhe_amd.cbs_params(pbs_approx=2))
+    assert e.value.code == tfhe_amd.TFHE_E_PARAM
     c = tfhe_amd.CbsContext(max_blocks=1)
     try:
         d = torch.zeros((4, 637), dtype=torch.int32, device="cuda")
