@@ -868,8 +868,10 @@ void launch_xor_public(uint32_t* ct, const uint8_t* pub, uint32_t nbits, int n, 
 
 void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, int limbs, const FFT1024Tables& T,
                  uint64_t* out, hipStream_t st) {
-    // one workgroup per ciphertext: up to 256 they run one wave per SIMD and the
-    // branch-free FFT form is faster; beyond, the two-waves-per-SIMD form is
+    // one workgroup per ciphertext.  Two limbs: the branch-free FFT form needs 267
+    // registers (one wave per SIMD), which pays only up to 256 ciphertexts; beyond,
+    // the branchy form keeps two waves per SIMD.  One limb: the branch-free form
+    // fits two waves per SIMD (220 registers) and is the faster one at every size.
     const unsigned big = grid_for(batch, 2048);
     if (limbs == 2) {
         if (batch <= 256)
@@ -877,10 +879,7 @@ void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, i
         else
             k_pbs2_manylut<false, 2><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
     } else {
-        if (batch <= 256)
-            k_pbs2_manylut<true, 1><<<batch, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
-        else
-            k_pbs2_manylut<false, 1><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+        k_pbs2_manylut<true, 1><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
     }
 }
 

@@ -323,9 +323,8 @@ def test_pbs_approx_within_noise_bound(actx, okeys):
     2^-33 of the torus per bootstrap) can move a gadget digit by one, after which
     the ciphertext words differ from the exact form's (another BSK row enters) while
     the phase stays within the PBS noise; so both forms are compared by
-    decryption: every level of every output (300 inputs: the two-waves-per-SIMD
-    form; 4 inputs: the branch-free form) lies within 2^-24 of the torus of its
-    message, as do the oracle's exact outputs.  A CPU emulation measured phase
+    decryption: every level of every output (batches of 300 and of 4) lies within 2^-24 of the torus of
+    its message, as do the oracle's exact outputs.  A CPU emulation measured phase
     differences between the two forms of 2^-27.8 rms (both PBS noises)."""
     import torch
     import tfhe_amd
