@@ -335,9 +335,11 @@ __global__ void k_plain_extract(const uint32_t* __restrict__ acc, uint32_t* __re
 // coefficient i the L * 2^LOGB candidate rows of those columns are staged once
 // in a double-buffered LDS tile (one barrier per i) and each wave subtracts,
 // for each of its ciphertexts and digits, the row the digit names: one LDS read
-// and one subtraction per (ciphertext, digit, column).  blockIdx.z picks a
-// slice of i; slices add into the zero-filled output with 32-bit atomics
-// (order-free mod 2^32).
+// and one subtraction per (ciphertext, digit, column).  blockIdx.x = column
+// tile + cols * i-slice and blockIdx.y = ciphertext group, so the groups that
+// read one key tile have linear workgroup ids cols * slices apart; with that a
+// multiple of 8 they run on one XCD and share its L2.  Slices add into the
+// zero-filled output with 32-bit atomics (order-free mod 2^32).
 // ===========================================================================
 constexpr int KS_WAVES = 8, KS_CT = 16, KS_GROUP = KS_WAVES * KS_CT;
 
@@ -367,8 +369,8 @@ template <int LOGB, int L>
 __global__ __launch_bounds__(64 * KS_WAVES) void k_keyswitch32(const uint32_t* __restrict__ at, uint32_t batch,
                                                               uint32_t Bpad, int N, int n,
                                                               const uint32_t* __restrict__ lweN,
-                                                              const uint32_t* __restrict__ ksk,
-                                                              uint32_t* __restrict__ out) {
+                                                              const uint32_t* __restrict__ ksk, int cols,
+                                                              int slices, uint32_t* __restrict__ out) {
     // two adjacent columns per lane (128 per workgroup): one LDS address and one 8-byte
     // read feed two subtractions.  Key rows are n + 1 = 637 words long, so the global
     // loads stay 32-bit (odd rows are not 8-byte aligned).
@@ -377,10 +379,11 @@ __global__ __launch_bounds__(64 * KS_WAVES) void k_keyswitch32(const uint32_t* _
     __shared__ uint2 tab[2][ROWS][64];
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int col0 = blockIdx.x * 128 + 2 * lane;
+    const int tile = (int)blockIdx.x % cols, slice = (int)blockIdx.x / cols;
+    const int col0 = tile * 128 + 2 * lane;
     const bool ok0 = col0 <= n, ok1 = col0 + 1 <= n;
-    const int islice = N / (int)gridDim.z;
-    const int ibeg = (int)blockIdx.z * islice, iend = ibeg + islice;
+    const int islice = N / slices;
+    const int ibeg = slice * islice, iend = ibeg + islice;
     const uint32_t cbase = blockIdx.y * KS_GROUP + w * KS_CT;
     const bool active = cbase < batch;  // idle waves still take part in the barriers
     const size_t rs = (size_t)(n + 1);
@@ -433,7 +436,7 @@ __global__ __launch_bounds__(64 * KS_WAVES) void k_keyswitch32(const uint32_t* _
     for (int k = 0; k < KS_CT; ++k) {
         if (cbase + k >= batch) break;
         uint32_t* o = out + (size_t)(cbase + k) * rs;
-        const uint32_t bterm = (blockIdx.z == 0) ? lweN[(size_t)(cbase + k) * (N + 1) + N] : 0u;
+        const uint32_t bterm = (slice == 0) ? lweN[(size_t)(cbase + k) * (N + 1) + N] : 0u;
         if (ok0) atomicAdd(&o[col0], acc[k][0] + (col0 == n ? bterm : 0u));
         if (ok1) atomicAdd(&o[col0 + 1], acc[k][1] + (col0 + 1 == n ? bterm : 0u));
     }
@@ -538,7 +541,8 @@ int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int l
     const size_t count = (size_t)batch * (n + 1);
     k_fill_zero<<<(unsigned)((count + 255) / 256 < 4096 ? (count + 255) / 256 : 4096), 256, 0, st>>>(out, count);
     k_ks_digits<16><<<dim3((unsigned)N / 64, Bpad / 64), 256, 0, st>>>(lweN, batch, N, Bpad, at);
-    k_keyswitch32<2, 8><<<dim3(cols, groups, slices), 64 * KS_WAVES, 0, st>>>(at, batch, Bpad, N, n, lweN, ksk, out);
+    k_keyswitch32<2, 8><<<dim3(cols * slices, groups), 64 * KS_WAVES, 0, st>>>(at, batch, Bpad, N, n, lweN, ksk,
+                                                                               (int)cols, (int)slices, out);
     return 0;
 }
 
