@@ -57,11 +57,11 @@ struct DeviceTables {
 };
 
 // Twiddle tables of the 4-wave N = 2048 FFT (fft1024.hpp): p1t, p2, p3 [4][64],
-// tr [4][4][64]; 1792 complex values.
+// tr [4][4][64], rot [2N]; 5888 complex values.
 struct DeviceTables1024 {
     FFT1024Tables T{};
     cplx* d_buf = nullptr;
-    static constexpr int kCount = 1792;
+    static constexpr int kCount = 1792 + 4096;
     int init() {
         std::vector<cplx> h(kCount);
         const double PI = 3.14159265358979323846264338327950288;
@@ -80,6 +80,7 @@ struct DeviceTables1024 {
                     const int k = (l >> 4) + 4 * ((l >> 2) & 3) + 16 * (l & 3) + 64 * s;
                     put(768 + (r * 4 + s) * 64 + l, PI * r / N - 2.0 * PI * (double)((r * k) % 1024) / 1024.0);
                 }
+        for (int j = 0; j < 2 * N; ++j) put(1792 + j, PI * j / N);
         int rc = dmalloc((void**)&d_buf, h.size() * sizeof(cplx));
         if (rc) return rc;
         rc = hipcheck(hipMemcpy(d_buf, h.data(), h.size() * sizeof(cplx), hipMemcpyHostToDevice), "tables1024");
@@ -88,6 +89,7 @@ struct DeviceTables1024 {
         T.p2 = d_buf + 256;
         T.p3 = d_buf + 512;
         T.tr = d_buf + 768;
+        T.rot = d_buf + 1792;
         return TFHE_OK;
     }
     void release() {

@@ -28,7 +28,7 @@ __device__ __forceinline__ FFT1024Tables stage_tables(const FFT1024Tables& Tg, c
         tl[512 + k] = Tg.p3[k];
     }
     __syncthreads();
-    return {tl, tl + 256, tl + 512, Tg.tr};
+    return {tl, tl + 256, tl + 512, Tg.tr, Tg.rot};
 }
 
 __device__ __forceinline__ void split_limbs(uint64_t v, double& lo, double& hi) {
@@ -408,9 +408,11 @@ __device__ __forceinline__ uint64_t torus64_from_double(double y) {
 }
 
 // LIMBS = 2: BSK2 Fourier rows hold the two signed 32-bit limbs of each torus-64
-// coefficient and every product is rounded exactly.  LIMBS = 1: one Fourier row
-// of the coefficients scaled to the torus (k_to_fourier2048_t64); the product is
-// rounded once, to about 2^-36 of the torus per step.
+// coefficient and every product is rounded exactly; the monic update rotates the
+// rounded product across the four waves through LDS.  LIMBS = 1: one Fourier row
+// of the coefficients scaled to the torus (k_to_fourier2048_t64); the monic factor
+// is applied pointwise before the inverse transform and the product is rounded
+// once, to about 2^-36 of the torus per step.
 template <bool BF, int LIMBS>
 __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict__ lwe, uint32_t batch, int n,
                                                      const cplx* __restrict__ bskf, FFT1024Tables Tg,
@@ -429,6 +431,10 @@ __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict
     const uint64_t OFF = pbs_digit_offset();
     constexpr uint32_t N = CB_N, Q2 = 2 * CB_N;
     const uint64_t mu0 = 1ull << 47, mu1 = 1ull << 55;
+    uint32_t ek[4];  // evaluation exponents 1 - 4k (mod 2N) of this lane's frequencies
+    const uint32_t kq = (uint32_t)((lane >> 4) + 4 * ((lane >> 2) & 3) + 16 * (lane & 3) + 256 * w);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) ek[s] = (1u - 4u * (kq + 64u * (uint32_t)s)) & (Q2 - 1);
 
     for (uint32_t cidx = blockIdx.x; cidx < batch; cidx += gridDim.x) {
         const uint32_t* ct = lwe + (size_t)cidx * (n + 1);
@@ -482,10 +488,33 @@ __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict
                                 cmac(macc[o][lb][s], x[s], kr[(o * LIMBS + lb) * CB_M + s * 64]);
                 }
             }
-            uint64_t t[2][8];
+            if constexpr (LIMBS == 1) {
+                // acc += (X^mrot - 1) t in the Fourier domain: frequency k is the polynomial
+                // at w^(1 - 4k), so t's transform is scaled by w^(mrot (1 - 4k)) - 1 before
+                // the inverse (the product at most doubles; one rounding per coefficient)
+                cplx rf[4];
 #pragma unroll
-            for (int o = 0; o < 2; ++o) {
-                if constexpr (LIMBS == 2) {
+                for (int s = 0; s < 4; ++s) {
+                    const cplx z = T.rot[(mrot * ek[s]) & (Q2 - 1)];
+                    rf[s] = {z.re - 1.0, z.im};
+                }
+#pragma unroll
+                for (int o = 0; o < 2; ++o) {
+                    cplx y0[4];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) y0[s] = cmul(macc[o][0][s], rf[s]);
+                    f1024_inv<BF>(y0, lds, xbuf[buf], T, trw, w, lane);
+                    buf ^= 1;
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        acc[o][a] += torus64_from_double(y0[a].re);
+                        acc[o][a + 4] += torus64_from_double(y0[a].im);
+                    }
+                }
+            } else {
+                uint64_t t[2][8];
+#pragma unroll
+                for (int o = 0; o < 2; ++o) {
                     cplx y0[4], y1[4];
 #pragma unroll
                     for (int s = 0; s < 4; ++s) { y0[s] = macc[o][0][s]; y1[s] = macc[o][1][s]; }
@@ -500,40 +529,30 @@ __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict
                         t[o][a + 4] = (uint64_t)round_i64(y0[a].im, 1.0 / CB_M) +
                                       ((uint64_t)round_i64(y1[a].im, 1.0 / CB_M) << 32);
                     }
-                } else {
-                    cplx y0[4];
+                }
+                // acc += X^mrot t - t, rotation across the four waves through LDS
+                __syncthreads();
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) y0[s] = macc[o][0][s];
-                    f1024_inv<BF>(y0, lds, xbuf[buf], T, trw, w, lane);
-                    buf ^= 1;
+                for (int o = 0; o < 2; ++o)
 #pragma unroll
-                    for (int a = 0; a < 4; ++a) {
-                        t[o][a] = torus64_from_double(y0[a].re);
-                        t[o][a + 4] = torus64_from_double(y0[a].im);
+                    for (int h = 0; h < 8; ++h)
+                        x64[o * CB_N + own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0)] = t[o][h];
+                __syncthreads();
+                const int kshift = (int)(mrot & (N - 1));
+                const bool hi = mrot >= N;
+#pragma unroll
+                for (int o = 0; o < 2; ++o)
+#pragma unroll
+                    for (int h = 0; h < 8; ++h) {
+                        int idx = own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0) - kshift;
+                        const bool wrapped = idx < 0;
+                        idx += wrapped ? (int)N : 0;
+                        uint64_t v = x64[o * CB_N + idx];
+                        if (hi ^ wrapped) v = 0 - v;
+                        acc[o][h] += v - t[o][h];
                     }
-                }
+                __syncthreads();
             }
-            // acc += X^mrot t - t, rotation across the four waves through LDS
-            __syncthreads();
-#pragma unroll
-            for (int o = 0; o < 2; ++o)
-#pragma unroll
-                for (int h = 0; h < 8; ++h) x64[o * CB_N + own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0)] = t[o][h];
-            __syncthreads();
-            const int kshift = (int)(mrot & (N - 1));
-            const bool hi = mrot >= N;
-#pragma unroll
-            for (int o = 0; o < 2; ++o)
-#pragma unroll
-                for (int h = 0; h < 8; ++h) {
-                    int idx = own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0) - kshift;
-                    const bool wrapped = idx < 0;
-                    idx += wrapped ? (int)N : 0;
-                    uint64_t v = x64[o * CB_N + idx];
-                    if (hi ^ wrapped) v = 0 - v;
-                    acc[o][h] += v - t[o][h];
-                }
-            __syncthreads();
         }
         // sample extraction of coefficients 0 and 1
         uint64_t* o0 = out + (size_t)cidx * 2 * (N + 1);

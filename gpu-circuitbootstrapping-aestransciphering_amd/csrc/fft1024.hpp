@@ -23,6 +23,8 @@ struct FFT1024Tables {
     const cplx* p2;   // [4 e][64 L]   W64^((L mod 16) e)
     const cplx* p3;   // [4 p][64 L]   W16^((L mod 4) p)
     const cplx* tr;   // [4 r][4 s][64 lane]  w^r W1024^(r k(lane, s))
+    const cplx* rot;  // [2N] w^j: frequency k (wave q, slot s, lane) is the polynomial at w^(1 - 4k),
+                      // k = (lane >> 4) + 4 ((lane >> 2) & 3) + 16 (lane & 3) + 64 s + 256 q
 };
 
 // (w^4)^(64 a) = exp(i pi a / 8), a < 4
