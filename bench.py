@@ -90,6 +90,8 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0):
             bad += v != want
         out[str(blocks)] = {"blocks_per_s": round(blocks / dt, 3), "s_per_launch": round(dt, 4),
                             "circuit_bootstraps_per_s": round(blocks * 1280 / dt, 1), "blocks_wrong": int(bad)}
+        print(f"aes128 ctr ({'one-product' if pbs_approx else 'exact'} PBS): {blocks} blocks in {dt:.3f} s, "
+              f"{int(bad)} wrong", file=sys.stderr, flush=True)
         if blocks <= 32:  # the same keystream replayed from its captured hipGraph
             eager = d_ks[:blocks].clone()
             ctx.aes_ctr_keystream(ctrs, d_ks, graph=True)  # capture + first replay
@@ -120,7 +122,7 @@ def main():
                     help="gate bootstraps in the decryption-failure campaign (0 to skip)")
     ap.add_argument("--valu-rates", action="store_true",
                     help="also measure the FP64 and (unpacked) FP32 FMA throughput of the library's probe kernel")
-    ap.add_argument("--aes-blocks", default="1,17,256",
+    ap.add_argument("--aes-blocks", default="1,17,256,1024",
                     help="comma-separated AES-128 CTR batch sizes to time ('' to skip)")
     ap.add_argument("--aes-pbs", choices=("both", "exact", "approx"), default="both",
                     help="PBS precision forms of the AES runs: exact two-limb, one-product FP64, or both")

@@ -23,6 +23,9 @@ bool cbs_params_supported(const tfhe_cbs_params* p) {
 }
 
 int pbs_limbs(const tfhe_cbs_params& p) { return p.pbs_approx ? 1 : 2; }
+// capacity bound: keeps every 32-bit count (2 x inputs for the PFKS) and every grid y
+// dimension (padded inputs / 64) of the path in range
+constexpr uint32_t kMaxBlocks = 4096;
 size_t bsk2_cplx(const tfhe_cbs_params& p) { return (size_t)p.n * 8 * 2 * pbs_limbs(p) * 1024; }
 size_t pfks_words_per_u() { return (size_t)(kN + 1) * 8 * 15 * 2 * kN; }
 size_t ksk2_words(const tfhe_cbs_params& p) { return (size_t)kN * 8 * 4 * (p.n + 1); }
@@ -151,6 +154,8 @@ extern "C" int tfhe_cbs_ctx_create(const tfhe_cbs_params* params, uint32_t max_b
     if (!cbs_params_supported(params))
         return fail(TFHE_E_PARAM, "tfhe_cbs_ctx_create: only N=2048, PBS 2^9x4, CB 2^8x2, PFKS 2^4x8, KS 2^2x8 is built");
     if (max_blocks == 0) max_blocks = 1;
+    if (max_blocks > kMaxBlocks)
+        return fail(TFHE_E_PARAM, "tfhe_cbs_ctx_create: at most 4096 AES blocks (524288 circuit bootstraps) per context");
     auto* c = new tfhe_cbs_ctx();
     c->p = *params;
     c->max_blocks = max_blocks;
@@ -335,7 +340,7 @@ extern "C" int tfhe_cbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t b
     // both functions in one launch over the rounded, transposed inputs
     const uint32_t count = 2 * batch;
     launch_pfks_digits(c->d_lwe64, count, c->d_at, st);
-    launch_zero_rows32(d_rgsw, 1, 0, (uint32_t)((size_t)count * 4 * kN), st);
+    launch_zero_rows32(d_rgsw, count, 4 * (size_t)kN, 4 * kN, st);
     launch_pfks32(c->d_at, count, c->d_pfks, pfks_words_per_u(), 0, 2, d_rgsw, 4 * (size_t)kN, 2 * (size_t)kN, st);
     return check_launch("cbs pfks");
 }

@@ -154,7 +154,8 @@ typedef struct tfhe_cbs_params {
 typedef struct tfhe_cbs_ctx tfhe_cbs_ctx;
 
 /* GPUCirBTSContext(cc, maxBatchSize) (cirbts.cuh:420-426): max_blocks AES blocks,
- * i.e. 128 * max_blocks circuit bootstraps per call. */
+ * i.e. 128 * max_blocks circuit bootstraps per call; at most 4096 blocks
+ * (TFHE_E_PARAM beyond). */
 int tfhe_cbs_ctx_create(const tfhe_cbs_params* params, uint32_t max_blocks, tfhe_cbs_ctx** out);
 int tfhe_cbs_ctx_destroy(tfhe_cbs_ctx* ctx);
 

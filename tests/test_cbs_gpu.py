@@ -380,6 +380,9 @@ def test_cbs_error_paths(gpu):
     with pytest.raises(tfhe_amd.TfheError) as e:  # no such PBS precision form
         tfhe_amd.CbsContext(max_blocks=1, params=tfhe_amd.cbs_params(pbs_approx=2))
     assert e.value.code == tfhe_amd.TFHE_E_PARAM
+    with pytest.raises(tfhe_amd.TfheError) as e:  # above the 4096-block capacity bound
+        tfhe_amd.CbsContext(max_blocks=4097)
+    assert e.value.code == tfhe_amd.TFHE_E_PARAM
     c = tfhe_amd.CbsContext(max_blocks=1)
     try:
         d = torch.zeros((4, 637), dtype=torch.int32, device="cuda")
