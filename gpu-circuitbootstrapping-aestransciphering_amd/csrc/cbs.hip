@@ -44,10 +44,10 @@ __device__ __forceinline__ void split_limbs(uint64_t v, double& lo, double& hi) 
 // (BF = true: fewer instructions and a longer straight-line schedule, but more
 // registers; faster where there is one wave per SIMD anyway).  Both round
 // at the same points, so the outputs are identical.
-template <bool BF>
+template <bool BF, bool RT = false>
 __device__ __forceinline__ void f1024_fwd(cplx (&x)[4], cplx* lds, cplx* xbuf, const FFT1024Tables& T,
-                                          const cplx (&trw)[4], int w, int lane) {
-    fft256_fwd(x, lds, T, lane);
+                                          const cplx (&trw)[4], int w, int lane, const Tw256Regs& tw = Tw256Regs{}) {
+    fft256_fwd_t<RT>(x, lds, T, tw, lane);
 #pragma unroll
     for (int s = 0; s < 4; ++s) xbuf[(w * 4 + s) * 64 + lane] = cmul(x[s], trw[s]);
     __syncthreads();
@@ -64,9 +64,9 @@ __device__ __forceinline__ void f1024_fwd(cplx (&x)[4], cplx* lds, cplx* xbuf, c
     }
 }
 
-template <bool BF>
+template <bool BF, bool RT = false>
 __device__ __forceinline__ void f1024_inv(cplx (&x)[4], cplx* lds, cplx* xbuf, const FFT1024Tables& T,
-                                          const cplx (&trw)[4], int w, int lane) {
+                                          const cplx (&trw)[4], int w, int lane, const Tw256Regs& tw = Tw256Regs{}) {
 #pragma unroll
     for (int s = 0; s < 4; ++s) xbuf[(w * 4 + s) * 64 + lane] = x[s];
     __syncthreads();
@@ -83,7 +83,7 @@ __device__ __forceinline__ void f1024_inv(cplx (&x)[4], cplx* lds, cplx* xbuf, c
         }
         x[s] = cmulc(y, trw[s]);
     }
-    fft256_inv(x, lds, T, lane);
+    fft256_inv_t<RT>(x, lds, T, tw, lane);
 }
 
 __device__ __forceinline__ void load_trw(const FFT1024Tables& Tg, int w, int lane, cplx (&trw)[4]) {
@@ -426,6 +426,8 @@ __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     cplx trw[4];
     load_trw(Tg, w, lane, trw);
+    constexpr bool RT = LIMBS == 1;  // the one-product form has registers for the p2 / p3 twiddles
+    const Tw256Regs tw = RT ? load_tw256(Tg, lane) : Tw256Regs{};
     cplx* lds = lds_all[w];
     uint64_t* x64 = reinterpret_cast<uint64_t*>(&xbuf[0][0]);  // [2][2048] u64 scratch for the monic exchange
     const uint64_t OFF = pbs_digit_offset();
@@ -476,7 +478,7 @@ __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict
                     cplx x[4];
 #pragma unroll
                     for (int a = 0; a < 4; ++a) x[a] = {pbs_digit(tp[a], j), pbs_digit(tp[a + 4], j)};
-                    f1024_fwd<BF>(x, lds, xbuf[buf], T, trw, w, lane);
+                    f1024_fwd<BF, RT>(x, lds, xbuf[buf], T, trw, w, lane, tw);
                     buf ^= 1;
                     const cplx* kr = key + (size_t)r * 2 * LIMBS * CB_M + (w * 4) * 64 + lane;
 #pragma unroll
@@ -503,7 +505,7 @@ __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict
                     cplx y0[4];
 #pragma unroll
                     for (int s = 0; s < 4; ++s) y0[s] = cmul(macc[o][0][s], rf[s]);
-                    f1024_inv<BF>(y0, lds, xbuf[buf], T, trw, w, lane);
+                    f1024_inv<BF, RT>(y0, lds, xbuf[buf], T, trw, w, lane, tw);
                     buf ^= 1;
 #pragma unroll
                     for (int a = 0; a < 4; ++a) {

@@ -51,7 +51,26 @@ __device__ __forceinline__ int f256_a2(int k0, int e, int d) { return k0 * 80 + 
 __device__ __forceinline__ int f256_a3(int k0, int e, int p, int h) { return k0 * 80 + e * 20 + p * 5 + h; }
 
 // forward 256-point: x = untwisted folded points (lane b, slot a -> point 64a + b)
-__device__ __forceinline__ void fft256_fwd(cplx (&x)[4], cplx* lds, const FFT1024Tables& T, int lane) {
+// p2 and p3 twiddles of one lane held in registers (RT = true below): p2[e - 1] =
+// T.p2[64 e + lane], p3[p - 1] = T.p3[64 p + lane]; they are the same for every
+// transform a lane runs, so a kernel with registers to spare loads them once.
+struct Tw256Regs {
+    cplx p2[3], p3[3];
+};
+
+__device__ __forceinline__ Tw256Regs load_tw256(const FFT1024Tables& T, int lane) {
+    Tw256Regs r;
+#pragma unroll
+    for (int e = 1; e < 4; ++e) {
+        r.p2[e - 1] = T.p2[e * 64 + lane];
+        r.p3[e - 1] = T.p3[e * 64 + lane];
+    }
+    return r;
+}
+
+template <bool RT>
+__device__ __forceinline__ void fft256_fwd_t(cplx (&x)[4], cplx* lds, const FFT1024Tables& T, const Tw256Regs& tw,
+                                             int lane) {
 #pragma unroll
     for (int a = 1; a < 4; ++a) x[a] = cmul(x[a], twist4_const(a));
     dft4<false>(x);
@@ -66,7 +85,7 @@ __device__ __forceinline__ void fft256_fwd(cplx (&x)[4], cplx* lds, const FFT102
     wave_sync();
     dft4<false>(x);
 #pragma unroll
-    for (int e = 1; e < 4; ++e) x[e] = cmul(x[e], T.p2[e * 64 + lane]);
+    for (int e = 1; e < 4; ++e) x[e] = cmul(x[e], RT ? tw.p2[e - 1] : T.p2[e * 64 + lane]);
 #pragma unroll
     for (int e = 0; e < 4; ++e) lds[f256_a2(k0, e, d)] = x[e];
     wave_sync();
@@ -76,7 +95,7 @@ __device__ __forceinline__ void fft256_fwd(cplx (&x)[4], cplx* lds, const FFT102
     wave_sync();
     dft4<false>(x);
 #pragma unroll
-    for (int p = 1; p < 4; ++p) x[p] = cmul(x[p], T.p3[p * 64 + lane]);
+    for (int p = 1; p < 4; ++p) x[p] = cmul(x[p], RT ? tw.p3[p - 1] : T.p3[p * 64 + lane]);
 #pragma unroll
     for (int p = 0; p < 4; ++p) lds[f256_a3(k0, e, p, h)] = x[p];
     wave_sync();
@@ -87,8 +106,14 @@ __device__ __forceinline__ void fft256_fwd(cplx (&x)[4], cplx* lds, const FFT102
     dft4<false>(x);
 }
 
+__device__ __forceinline__ void fft256_fwd(cplx (&x)[4], cplx* lds, const FFT1024Tables& T, int lane) {
+    fft256_fwd_t<false>(x, lds, T, Tw256Regs{}, lane);
+}
+
 // inverse 256-point: frequency layout in, 256 x folded points out
-__device__ __forceinline__ void fft256_inv(cplx (&x)[4], cplx* lds, const FFT1024Tables& T, int lane) {
+template <bool RT>
+__device__ __forceinline__ void fft256_inv_t(cplx (&x)[4], cplx* lds, const FFT1024Tables& T, const Tw256Regs& tw,
+                                             int lane) {
     dft4<true>(x);
     const int k0 = lane >> 4, e = (lane >> 2) & 3, low = lane & 3;
 #pragma unroll
@@ -98,7 +123,7 @@ __device__ __forceinline__ void fft256_inv(cplx (&x)[4], cplx* lds, const FFT102
     for (int p = 0; p < 4; ++p) x[p] = lds[f256_a3(k0, e, p, low)];
     wave_sync();
 #pragma unroll
-    for (int p = 1; p < 4; ++p) x[p] = cmulc(x[p], T.p3[p * 64 + lane]);
+    for (int p = 1; p < 4; ++p) x[p] = cmulc(x[p], RT ? tw.p3[p - 1] : T.p3[p * 64 + lane]);
     dft4<true>(x);
 #pragma unroll
     for (int g = 0; g < 4; ++g) lds[f256_a2(k0, e, 4 * g + low)] = x[g];
@@ -108,7 +133,7 @@ __device__ __forceinline__ void fft256_inv(cplx (&x)[4], cplx* lds, const FFT102
     for (int ee = 0; ee < 4; ++ee) x[ee] = lds[f256_a2(k0, ee, d)];
     wave_sync();
 #pragma unroll
-    for (int ee = 1; ee < 4; ++ee) x[ee] = cmulc(x[ee], T.p2[ee * 64 + lane]);
+    for (int ee = 1; ee < 4; ++ee) x[ee] = cmulc(x[ee], RT ? tw.p2[ee - 1] : T.p2[ee * 64 + lane]);
     dft4<true>(x);
 #pragma unroll
     for (int c = 0; c < 4; ++c) lds[f256_a1(k0, 16 * c + d)] = x[c];
@@ -121,6 +146,10 @@ __device__ __forceinline__ void fft256_inv(cplx (&x)[4], cplx* lds, const FFT102
     dft4<true>(x);
 #pragma unroll
     for (int a = 1; a < 4; ++a) x[a] = cmulc(x[a], twist4_const(a));
+}
+
+__device__ __forceinline__ void fft256_inv(cplx (&x)[4], cplx* lds, const FFT1024Tables& T, int lane) {
+    fft256_inv_t<false>(x, lds, T, Tw256Regs{}, lane);
 }
 
 // Output w of the 4-point DFT across the four waves of an FFT1024,
