@@ -49,15 +49,16 @@ def cpu_baseline(sample=24):
                       f"oracle C restatement, FP64 radix-2 FFT, one thread"}
 
 
-def aes_ctr_bench(batches, reps=1, pbs_approx=0):
+def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1):
     """AES-128 CTR keystream: circuit bootstrap + S-box LUT per round,
     linear layers as LWE additions; every block decrypted and compared with plain AES.
-    pbs_approx=1 runs the one-product FP64 PBS instead of the exact two-limb one."""
+    pbs_approx=1 runs the one-product FP64 PBS instead of the exact two-limb one;
+    pbs_multibit=2 blind-rotates two key bits per step (one-product form only)."""
     import numpy as np
     import torch
     import tfhe_amd
     maxb = max(batches)
-    ctx = tfhe_amd.CbsContext(max_blocks=maxb, params=tfhe_amd.cbs_params(pbs_approx))
+    ctx = tfhe_amd.CbsContext(max_blocks=maxb, params=tfhe_amd.cbs_params(pbs_approx, pbs_multibit))
     sk = tfhe_amd.lwe_keygen(N_LWE, 3)
     t0 = time.perf_counter()
     ctx.keygen(sk, 3)
@@ -90,7 +91,8 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0):
             bad += v != want
         out[str(blocks)] = {"blocks_per_s": round(blocks / dt, 3), "s_per_launch": round(dt, 4),
                             "circuit_bootstraps_per_s": round(blocks * 1280 / dt, 1), "blocks_wrong": int(bad)}
-        print(f"aes128 ctr ({'one-product' if pbs_approx else 'exact'} PBS): {blocks} blocks in {dt:.3f} s, "
+        print(f"aes128 ctr ({'one-product' if pbs_approx else 'exact'} PBS, {pbs_multibit} key bit(s) per step): "
+              f"{blocks} blocks in {dt:.3f} s, "
               f"{int(bad)} wrong", file=sys.stderr, flush=True)
         if blocks <= 32:  # the same keystream replayed from its captured hipGraph
             eager = d_ks[:blocks].clone()
@@ -104,7 +106,7 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0):
             out[str(blocks)]["graph_equals_eager"] = bool(torch.equal(d_ks[:blocks], eager))
     ctx.close()
     pbs = ("one FP64 product per digit polynomial, rounding error ~2^-33" if pbs_approx
-           else "2-limb exact")
+           else "2-limb exact") + (", two key bits per blind-rotation step" if pbs_multibit == 2 else "")
     return {"form": f"fp64 FFT: PBS on the 64-bit torus ({pbs}), PFKS / CMux LUT on the 32-bit torus "
                     "(1-limb exact)", "keygen_s": round(keygen_s, 2), "batches": out}
 
@@ -124,8 +126,9 @@ def main():
                     help="also measure the FP64 and (unpacked) FP32 FMA throughput of the library's probe kernel")
     ap.add_argument("--aes-blocks", default="1,17,256,1024",
                     help="comma-separated AES-128 CTR batch sizes to time ('' to skip)")
-    ap.add_argument("--aes-pbs", choices=("both", "exact", "approx"), default="both",
-                    help="PBS precision forms of the AES runs: exact two-limb, one-product FP64, or both")
+    ap.add_argument("--aes-pbs", default="exact,approx",
+                    help="comma-separated PBS forms of the AES runs: exact (two-limb), approx (one FP64 "
+                         "product), approx-mb2 (one product, two key bits per step)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -291,10 +294,13 @@ def main():
                                    "fp32_unpacked": round(tfhe_amd.probe_fma_tflops(0), 2)}
     if args.aes_blocks:
         sizes = [int(b) for b in args.aes_blocks.split(",") if b]
-        if args.aes_pbs in ("both", "exact"):
-            line["aes128_ctr"] = aes_ctr_bench(sizes)
-        if args.aes_pbs in ("both", "approx"):
-            line["aes128_ctr_one_product_pbs"] = aes_ctr_bench(sizes, pbs_approx=1)
+        forms = {"exact": ("aes128_ctr", 0, 1), "approx": ("aes128_ctr_one_product_pbs", 1, 1),
+                 "approx-mb2": ("aes128_ctr_one_product_pbs_2bit", 1, 2)}
+        for f in [f for f in args.aes_pbs.split(",") if f]:
+            if f not in forms:
+                sys.exit(f"--aes-pbs: unknown form {f!r} (exact, approx, approx-mb2)")
+            key, approx, mb = forms[f]
+            line[key] = aes_ctr_bench(sizes, pbs_approx=approx, pbs_multibit=mb)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

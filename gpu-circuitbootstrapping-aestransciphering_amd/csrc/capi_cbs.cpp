@@ -19,14 +19,18 @@ bool cbs_params_supported(const tfhe_cbs_params* p) {
     return p && p->N == kN && p->bk_logB == 9 && p->bk_l == 4 && p->cb_logB == 8 && p->cb_l == 2 && p->pf_logB == 4 &&
            p->pf_l == 8 && p->ks_logB == 2 && p->ks_l == 8 && p->n > 0 && p->n <= 1024 && p->lwe_nshift >= 1 &&
            p->lwe_nshift <= 31 && p->rlwe_nshift >= -31 && p->rlwe_nshift <= 31 && p->pf_nshift >= 1 &&
-           p->pf_nshift <= 31 && (p->pbs_approx == 0 || p->pbs_approx == 1);
+           p->pf_nshift <= 31 && (p->pbs_approx == 0 || p->pbs_approx == 1) && p->pbs_multibit >= 0 &&
+           p->pbs_multibit <= 2 && (p->pbs_multibit < 2 || (p->pbs_approx == 1 && p->n % 2 == 0));
 }
 
 int pbs_limbs(const tfhe_cbs_params& p) { return p.pbs_approx ? 1 : 2; }
+int pbs_mb(const tfhe_cbs_params& p) { return p.pbs_multibit == 2 ? 2 : 1; }
+// RGSWs of the BSK2: one per key bit, or three pattern keys per key-bit pair
+size_t bsk2_rgsw(const tfhe_cbs_params& p) { return pbs_mb(p) == 2 ? (size_t)p.n / 2 * 3 : (size_t)p.n; }
 // capacity bound: keeps every 32-bit count (2 x inputs for the PFKS) and every grid y
 // dimension (padded inputs / 64) of the path in range
 constexpr uint32_t kMaxBlocks = 4096;
-size_t bsk2_cplx(const tfhe_cbs_params& p) { return (size_t)p.n * 8 * 2 * pbs_limbs(p) * 1024; }
+size_t bsk2_cplx(const tfhe_cbs_params& p) { return bsk2_rgsw(p) * 8 * 2 * pbs_limbs(p) * 1024; }
 size_t pfks_words_per_u() { return (size_t)(kN + 1) * 8 * 15 * 2 * kN; }
 size_t ksk2_words(const tfhe_cbs_params& p) { return (size_t)kN * 8 * 4 * (p.n + 1); }
 
@@ -152,7 +156,8 @@ extern "C" int tfhe_cbs_ctx_create(const tfhe_cbs_params* params, uint32_t max_b
     if (!out) return fail(TFHE_E_PARAM, "tfhe_cbs_ctx_create: out is NULL");
     *out = nullptr;
     if (!cbs_params_supported(params))
-        return fail(TFHE_E_PARAM, "tfhe_cbs_ctx_create: only N=2048, PBS 2^9x4, CB 2^8x2, PFKS 2^4x8, KS 2^2x8 is built");
+        return fail(TFHE_E_PARAM, "tfhe_cbs_ctx_create: only N=2048, PBS 2^9x4, CB 2^8x2, PFKS 2^4x8, KS 2^2x8 is "
+                                  "built (two key bits per step: one-product form and even n)");
     if (max_blocks == 0) max_blocks = 1;
     if (max_blocks > kMaxBlocks)
         return fail(TFHE_E_PARAM, "tfhe_cbs_ctx_create: at most 4096 AES blocks (524288 circuit bootstraps) per context");
@@ -255,7 +260,7 @@ extern "C" int tfhe_cbs_keygen(tfhe_cbs_ctx* c, const int32_t* lwe_sk, uint64_t 
     uint64_t *d_bsk = nullptr, *d_K64 = nullptr;
     cplx* d_K2f = nullptr;
     rc = rc ? rc : dmalloc((void**)&d_s, p.n * sizeof(int32_t));
-    rc = rc ? rc : dmalloc((void**)&d_bsk, (size_t)p.n * 8 * 2 * kN * sizeof(uint64_t));
+    rc = rc ? rc : dmalloc((void**)&d_bsk, bsk2_rgsw(p) * 8 * 2 * kN * sizeof(uint64_t));
     rc = rc ? rc : dmalloc((void**)&d_K64, kN * sizeof(uint64_t));
     rc = rc ? rc : dmalloc((void**)&d_K2f, 2 * 1024 * sizeof(cplx));
     std::vector<uint64_t> hK64(kN);
@@ -265,11 +270,11 @@ extern "C" int tfhe_cbs_keygen(tfhe_cbs_ctx* c, const int32_t* lwe_sk, uint64_t 
     if (!rc) rc = hipcheck(hipMemcpy(d_K64, hK64.data(), kN * sizeof(uint64_t), hipMemcpyHostToDevice), "K64");
     if (!rc) {
         launch_to_fourier2048(d_K64, 1, c->tables.T, d_K2f, nullptr);
-        launch_gen_bsk64(seed, p.n, p.rlwe_nshift, d_s, c->d_K2, d_K2f, c->tables.T, d_bsk, nullptr);
+        launch_gen_bsk64(seed, p.n, p.rlwe_nshift, d_s, c->d_K2, d_K2f, c->tables.T, d_bsk, nullptr, pbs_mb(p));
         if (pbs_limbs(p) == 2)
-            launch_to_fourier2048(d_bsk, (size_t)p.n * 8 * 2, c->tables.T, c->d_bsk2f, nullptr);
+            launch_to_fourier2048(d_bsk, bsk2_rgsw(p) * 8 * 2, c->tables.T, c->d_bsk2f, nullptr);
         else
-            launch_to_fourier2048_t64(d_bsk, (size_t)p.n * 8 * 2, c->tables.T, c->d_bsk2f, nullptr);
+            launch_to_fourier2048_t64(d_bsk, bsk2_rgsw(p) * 8 * 2, c->tables.T, c->d_bsk2f, nullptr);
         launch_gen_pfks(seed, p.pf_nshift, c->d_K2, d_K2f, c->tables.T, c->d_pfks, nullptr);
         launch_gen_ksk32(seed, p.n, kN, p.ks_logB, p.ks_l, p.lwe_nshift, d_s, c->d_K2, c->d_ksk2, nullptr, DOM_KSK2);
         rc = check_launch("cbs keygen");
@@ -310,7 +315,7 @@ extern "C" int tfhe_cbs_pbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32
     if (rc) return rc;
     if (!d_lwe || !d_out) return fail(TFHE_E_PARAM, "pbs: NULL buffer");
     if (!batch) return TFHE_OK;
-    launch_pbs2(d_lwe, batch, c->p.n, c->d_bsk2f, pbs_limbs(c->p), c->tables.T, d_out, (hipStream_t)stream);
+    launch_pbs2(d_lwe, batch, c->p.n, c->d_bsk2f, pbs_limbs(c->p), pbs_mb(c->p), c->tables.T, d_out, (hipStream_t)stream);
     return check_launch("pbs2");
 }
 

@@ -244,10 +244,20 @@ __global__ __launch_bounds__(256) void k_polymul2048(const int32_t* __restrict__
 // two limb products with the precomputed Fourier transform of K2.
 // Row id = i*8 + r; msg = s_i * 2^(28 + 9j) on component u of coefficient 0
 // (r = 2j + u).  Output row layout [2][N] (a then b).
+// With mb = 2 (two key bits per blind-rotation step) RGSW q = 3 ip + pat encrypts
+// the indicator of pattern pat of the key-bit pair (s_2ip, s_2ip+1): (1,0), (0,1),
+// (1,1) for pat = 0, 1, 2; Philox domain DOM_BSK2MB, row id q*8 + r.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t bsk_message_bit(const int32_t* s, uint32_t q, int mb) {
+    if (mb == 1) return (uint32_t)s[q];
+    const uint32_t ip = q / 3, pat = q % 3;
+    const uint32_t b0 = (uint32_t)s[2 * ip], b1 = (uint32_t)s[2 * ip + 1];
+    return pat == 0 ? (b0 & (1u - b1)) : pat == 1 ? ((1u - b0) & b1) : (b0 & b1);
+}
+
 __global__ __launch_bounds__(256) void k_gen_bsk64(uint64_t seed, uint32_t nrows, int nshift,
                                                   const int32_t* __restrict__ s, const cplx* __restrict__ K2f,
-                                                  FFT1024Tables Tg, uint64_t* __restrict__ out) {
+                                                  FFT1024Tables Tg, int mb, uint64_t* __restrict__ out) {
     __shared__ cplx tl[768];
     __shared__ cplx lds_all[4][F256_LDS_CPLX];
     __shared__ cplx xbuf[2][1024];
@@ -260,17 +270,18 @@ __global__ __launch_bounds__(256) void k_gen_bsk64(uint64_t seed, uint32_t nrows
     for (int sl = 0; sl < 4; ++sl) kf[sl] = K2f[(w * 4 + sl) * 64 + lane];
     int buf = 0;
     for (uint32_t row = blockIdx.x; row < nrows; row += gridDim.x) {
-        const int i = row / 8, r = row % 8, j = r >> 1;
+        const int r = row % 8, j = r >> 1;
         const int msg_comp = r & 1;
-        const uint64_t cval = (uint64_t)s[i] << (28 + 9 * j);
+        const uint64_t cval = (uint64_t)bsk_message_bit(s, row / 8, mb) << (28 + 9 * j);
         const uint32_t id = row;
+        const uint32_t dom = mb == 1 ? DOM_BSK2 : DOM_BSK2MB;
         uint64_t av[8];
         cplx A0[4], A1[4];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
             const int c = own_coeff(w, lane, a);
-            av[a] = rand64(seed, DOM_BSK2, id, (uint32_t)c);
-            av[a + 4] = rand64(seed, DOM_BSK2, id, (uint32_t)(c + CB_M));
+            av[a] = rand64(seed, dom, id, (uint32_t)c);
+            av[a + 4] = rand64(seed, dom, id, (uint32_t)(c + CB_M));
             double l0r, l1r, l0i, l1i;
             split_limbs(av[a], l0r, l1r);
             split_limbs(av[a + 4], l0i, l1i);
@@ -294,7 +305,7 @@ __global__ __launch_bounds__(256) void k_gen_bsk64(uint64_t seed, uint32_t nrows
             const int c = own_coeff(w, lane, a) + (h >= 4 ? CB_M : 0);
             const double re0 = h < 4 ? A0[a].re : A0[a].im, re1 = h < 4 ? A1[a].re : A1[a].im;
             uint64_t b = (uint64_t)round_i64(re0, 1.0 / CB_M) + ((uint64_t)round_i64(re1, 1.0 / CB_M) << 32);
-            b += (uint64_t)noise(seed, DOM_BSK2, id, (uint32_t)c, nshift);
+            b += (uint64_t)noise(seed, dom, id, (uint32_t)c, nshift);
             uint64_t aval = av[h];
             if (c == 0) {
                 if (msg_comp == 0) aval += cval; else b += cval;
@@ -413,8 +424,14 @@ __device__ __forceinline__ uint64_t torus64_from_double(double y) {
 // of the coefficients scaled to the torus (k_to_fourier2048_t64); the monic factor
 // is applied pointwise before the inverse transform and the product is rounded
 // once, to about 2^-36 of the torus per step.
-template <bool BF, int LIMBS>
-__global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict__ lwe, uint32_t batch, int n,
+// MB = 2 (one limb only): two key bits per step.  With m1, m2 the rotations of the
+// pair, acc += sum_p (X^(e_p) - 1) (dec(acc) (*) RGSW(ind_p)) over the patterns
+// (1,0), (0,1), (1,1) with e_p = m1, m2, m1 + m2: exactly one indicator is 1 (or
+// none), so this equals the two single-bit steps.  One decomposition and two
+// inverse transforms serve both bits; the factor of pattern p is applied to the
+// digit transforms before they meet that pattern's key row.
+template <bool BF, int LIMBS, int MB = 1>
+__global__ __launch_bounds__(256, MB == 2 ? 2 : 1) void k_pbs2_manylut(const uint32_t* __restrict__ lwe, uint32_t batch, int n,
                                                      const cplx* __restrict__ bskf, FFT1024Tables Tg,
                                                      uint64_t* __restrict__ out) {
     __shared__ cplx tl[768];
@@ -426,7 +443,8 @@ __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     cplx trw[4];
     load_trw(Tg, w, lane, trw);
-    constexpr bool RT = LIMBS == 1;  // the one-product form has registers for the p2 / p3 twiddles
+    static_assert(MB == 1 || LIMBS == 1, "two key bits per step only in the one-product form");
+    constexpr bool RT = LIMBS == 1 && MB == 1;  // registers to spare for the p2 / p3 twiddles
     const Tw256Regs tw = RT ? load_tw256(Tg, lane) : Tw256Regs{};
     cplx* lds = lds_all[w];
     uint64_t* x64 = reinterpret_cast<uint64_t*>(&xbuf[0][0]);  // [2][2048] u64 scratch for the monic exchange
@@ -456,10 +474,22 @@ __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict
             acc[1][h] = v;
         }
         int buf = 0;
-        for (int i = 0; i < n; ++i) {
+        for (int i = 0; i < n; i += MB) {
             const uint32_t mrot = (Q2 - (uint32_t)at[i]) & (Q2 - 1);
-            if (mrot == 0) continue;
-            const cplx* key = bskf + (size_t)i * PBS_ROWS * 2 * LIMBS * CB_M;
+            uint32_t mrot2 = 0;
+            if constexpr (MB == 2) mrot2 = (Q2 - (uint32_t)at[i + 1]) & (Q2 - 1);
+            if ((mrot | mrot2) == 0) continue;
+            const cplx* key = bskf + (size_t)(i / MB) * (MB == 2 ? 3 : 1) * PBS_ROWS * 2 * LIMBS * CB_M;
+            // MB = 2: z1 = w^(m1 (1 - 4k)), z2 = w^(m2 (1 - 4k)); the pattern factors are
+            // z1 - 1, z2 - 1 and z1 z2 - 1
+            cplx zp[MB == 2 ? 2 : 1][4];
+            if constexpr (MB == 2) {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    zp[0][s] = T.rot[(mrot * ek[s]) & (Q2 - 1)];
+                    zp[1][s] = T.rot[(mrot2 * ek[s]) & (Q2 - 1)];
+                }
+            }
             cplx macc[2][LIMBS][4];
 #pragma unroll
             for (int o = 0; o < 2; ++o)
@@ -480,14 +510,41 @@ __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict
                     for (int a = 0; a < 4; ++a) x[a] = {pbs_digit(tp[a], j), pbs_digit(tp[a + 4], j)};
                     f1024_fwd<BF, RT>(x, lds, xbuf[buf], T, trw, w, lane, tw);
                     buf ^= 1;
-                    const cplx* kr = key + (size_t)r * 2 * LIMBS * CB_M + (w * 4) * 64 + lane;
+                    if constexpr (MB == 2) {
+                        // patterns in the order (1,0), (1,1), (0,1): x z1 serves the first two
+                        cplx t1[4], y[4];
 #pragma unroll
-                    for (int o = 0; o < 2; ++o)
+                        for (int s = 0; s < 4; ++s) t1[s] = cmul(x[s], zp[0][s]);
 #pragma unroll
-                        for (int lb = 0; lb < LIMBS; ++lb)
+                        for (int q = 0; q < 3; ++q) {
+                            const int pt = q == 0 ? 0 : q == 1 ? 2 : 1;
 #pragma unroll
                             for (int s = 0; s < 4; ++s)
-                                cmac(macc[o][lb][s], x[s], kr[(o * LIMBS + lb) * CB_M + s * 64]);
+                                y[s] = csub(q == 0 ? t1[s] : q == 1 ? cmul(t1[s], zp[1][s]) : cmul(x[s], zp[1][s]), x[s]);
+                            const cplx* kr = key + ((size_t)pt * PBS_ROWS + r) * 2 * CB_M + (w * 4) * 64 + lane;
+                            // each output's 4 key values as one batch of loads: under this
+                            // kernel's register target the compiler otherwise waits on each
+#pragma unroll
+                            for (int o = 0; o < 2; ++o) {
+                                cplx kv[4];
+#pragma unroll
+                                for (int s = 0; s < 4; ++s) kv[s] = kr[o * CB_M + s * 64];
+#pragma unroll
+                                for (int k = 0; k < 4; ++k) asm volatile("" : "+v"(kv[k].re), "+v"(kv[k].im));
+#pragma unroll
+                                for (int s = 0; s < 4; ++s) cmac(macc[o][0][s], y[s], kv[s]);
+                            }
+                        }
+                    } else {
+                        const cplx* kr = key + (size_t)r * 2 * LIMBS * CB_M + (w * 4) * 64 + lane;
+#pragma unroll
+                        for (int o = 0; o < 2; ++o)
+#pragma unroll
+                            for (int lb = 0; lb < LIMBS; ++lb)
+#pragma unroll
+                                for (int s = 0; s < 4; ++s)
+                                    cmac(macc[o][lb][s], x[s], kr[(o * LIMBS + lb) * CB_M + s * 64]);
+                    }
                 }
             }
             if constexpr (LIMBS == 1) {
@@ -495,16 +552,18 @@ __global__ __launch_bounds__(256) void k_pbs2_manylut(const uint32_t* __restrict
                 // at w^(1 - 4k), so t's transform is scaled by w^(mrot (1 - 4k)) - 1 before
                 // the inverse (the product at most doubles; one rounding per coefficient)
                 cplx rf[4];
+                if constexpr (MB == 1) {
 #pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const cplx z = T.rot[(mrot * ek[s]) & (Q2 - 1)];
-                    rf[s] = {z.re - 1.0, z.im};
+                    for (int s = 0; s < 4; ++s) {
+                        const cplx z = T.rot[(mrot * ek[s]) & (Q2 - 1)];
+                        rf[s] = {z.re - 1.0, z.im};
+                    }
                 }
 #pragma unroll
                 for (int o = 0; o < 2; ++o) {
                     cplx y0[4];
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) y0[s] = cmul(macc[o][0][s], rf[s]);
+                    for (int s = 0; s < 4; ++s) y0[s] = MB == 1 ? cmul(macc[o][0][s], rf[s]) : macc[o][0][s];
                     f1024_inv<BF, RT>(y0, lds, xbuf[buf], T, trw, w, lane, tw);
                     buf ^= 1;
 #pragma unroll
@@ -887,8 +946,8 @@ void launch_xor_public(uint32_t* ct, const uint8_t* pub, uint32_t nbits, int n, 
     k_xor_public<<<(nbits + 255) / 256, 256, 0, st>>>(ct, pub, nbits, n);
 }
 
-void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, int limbs, const FFT1024Tables& T,
-                 uint64_t* out, hipStream_t st) {
+void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, int limbs, int mb,
+                 const FFT1024Tables& T, uint64_t* out, hipStream_t st) {
     // one workgroup per ciphertext.  Two limbs: the branch-free FFT form needs 267
     // registers (one wave per SIMD), which pays only up to 256 ciphertexts; beyond,
     // the branchy form keeps two waves per SIMD.  One limb: the branch-free form
@@ -899,8 +958,10 @@ void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, i
             k_pbs2_manylut<true, 2><<<batch, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
         else
             k_pbs2_manylut<false, 2><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
-    } else {
+    } else if (mb == 1) {
         k_pbs2_manylut<true, 1><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+    } else {
+        k_pbs2_manylut<true, 1, 2><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
     }
 }
 
@@ -946,10 +1007,10 @@ void launch_polymul2048(const int32_t* x, const uint64_t* y, uint32_t batch, con
 }
 
 void launch_gen_bsk64(uint64_t seed, int n, int nshift, const int32_t* s, const int32_t* K2, const cplx* K2f,
-                      const FFT1024Tables& T, uint64_t* bsk, hipStream_t st) {
+                      const FFT1024Tables& T, uint64_t* bsk, hipStream_t st, int mb) {
     (void)K2;
-    const uint32_t rows = (uint32_t)n * 8;
-    k_gen_bsk64<<<grid_for(rows, 8192), 256, 0, st>>>(seed, rows, nshift, s, K2f, T, bsk);
+    const uint32_t rows = (uint32_t)(mb == 1 ? n : n / 2 * 3) * 8;
+    k_gen_bsk64<<<grid_for(rows, 8192), 256, 0, st>>>(seed, rows, nshift, s, K2f, T, mb, bsk);
 }
 
 void launch_gen_pfks(uint64_t seed, int nshift, const int32_t* K2, const cplx* K2f, const FFT1024Tables& T,

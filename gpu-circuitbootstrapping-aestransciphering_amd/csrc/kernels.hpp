@@ -34,15 +34,17 @@ struct FFT1024Tables;
 void launch_to_fourier2048(const uint64_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st);
 void launch_polymul2048(const int32_t* x, const uint64_t* y, uint32_t batch, const FFT1024Tables& T, uint64_t* out,
                         hipStream_t st);
+// mb = 1: [n][8][2][N] rows of RGSW(s_i); mb = 2: [n/2][3][8][2][N] rows of RGSW(pattern indicators)
 void launch_gen_bsk64(uint64_t seed, int n, int nshift, const int32_t* s, const int32_t* K2, const cplx* K2f,
-                      const FFT1024Tables& T, uint64_t* bsk, hipStream_t st);
+                      const FFT1024Tables& T, uint64_t* bsk, hipStream_t st, int mb = 1);
 void launch_gen_pfks(uint64_t seed, int nshift, const int32_t* K2, const cplx* K2f, const FFT1024Tables& T,
                      uint32_t* key, hipStream_t st);
 void launch_to_fourier2048_32(const uint32_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st);
 void launch_to_fourier2048_t64(const uint64_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st);
 // limbs = 2: exact two-limb BSK2 rows (launch_to_fourier2048); 1: one torus-scaled row (launch_to_fourier2048_t64)
-void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, int limbs, const FFT1024Tables& T,
-                 uint64_t* out, hipStream_t st);
+// mb = 2 (limbs = 1, n even): two key bits per step with the pattern keys of launch_gen_bsk64(mb = 2)
+void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, int limbs, int mb,
+                 const FFT1024Tables& T, uint64_t* out, hipStream_t st);
 // PFKS: inputs rounded and transposed to at [N+1][pfks_bpad(B)] u32, then the key switch for
 // functions u0 .. u0+nu-1 added into zero-filled outputs (out + (u-u0)*out_u_offset, rows out_stride apart)
 uint32_t pfks_bpad(uint32_t B);

@@ -39,15 +39,17 @@ class CbsParams(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("N", ctypes.c_int), ("bk_logB", ctypes.c_int), ("bk_l", ctypes.c_int),
                 ("cb_logB", ctypes.c_int), ("cb_l", ctypes.c_int), ("pf_logB", ctypes.c_int), ("pf_l", ctypes.c_int),
                 ("ks_logB", ctypes.c_int), ("ks_l", ctypes.c_int), ("lwe_nshift", ctypes.c_int),
-                ("rlwe_nshift", ctypes.c_int), ("pf_nshift", ctypes.c_int), ("pbs_approx", ctypes.c_int)]
+                ("rlwe_nshift", ctypes.c_int), ("pf_nshift", ctypes.c_int), ("pbs_approx", ctypes.c_int),
+                ("pbs_multibit", ctypes.c_int)]
 
 
-def cbs_params(pbs_approx=0):
+def cbs_params(pbs_approx=0, pbs_multibit=1):
     """n=636, N=2048; PBS 2^9 x 4 on the 64-bit torus (sigma 2^-51), PFKS 2^4 x 8 with
     a torus-32 key (sigma 2^-30), CB 2^8 x 2 (torus-32 RGSW), KS 2^2 x 8 to the
     level-0 key (sigma 2^-15).  pbs_approx=1 selects the one-product FP64 PBS
-    (rounding error about 2^-33 of the torus) instead of the exact two-limb one."""
-    return CbsParams(636, 2048, 9, 4, 8, 2, 4, 8, 2, 8, 15, 19, 30, int(pbs_approx))
+    (rounding error about 2^-33 of the torus) instead of the exact two-limb one;
+    pbs_multibit=2 (with pbs_approx=1) blind-rotates two key bits per step."""
+    return CbsParams(636, 2048, 9, 4, 8, 2, 4, 8, 2, 8, 15, 19, 30, int(pbs_approx), int(pbs_multibit))
 
 
 def build():

@@ -149,6 +149,12 @@ typedef struct tfhe_cbs_params {
      *     as the exact form's do, but they are not the same words (a rounding
      *     that moves a gadget digit brings in another key row). */
     int pbs_approx;
+    /* Key bits per blind-rotation step (the reference's multi-bit PBS,
+     * main.cu:130, kernel.cu:4585-4771): 0 or 1 = one; 2 = two, with three
+     * pattern keys per key-bit pair (RGSW of the indicators of (1,0), (0,1),
+     * (1,1)): one decomposition and two inverse transforms per pair, 1.5x the
+     * MACs and key bytes.  2 needs pbs_approx = 1 and an even n. */
+    int pbs_multibit;
 } tfhe_cbs_params;
 
 typedef struct tfhe_cbs_ctx tfhe_cbs_ctx;

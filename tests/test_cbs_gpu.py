@@ -298,11 +298,12 @@ def test_aes_keystream_graph_replays_match_eager(ctx, okeys):
         assert _decrypt_blocks(okeys, from_dev(d_graph), 2) == [O.aes128_encrypt(key, c) for c in ctrs]
 
 
-@pytest.fixture(scope="module")
-def actx(gpu, okeys):
-    """The same keys with the one-product FP64 PBS (pbs_approx = 1)."""
+@pytest.fixture(scope="module", params=[1, 2], ids=["one_bit", "two_bits"])
+def actx(gpu, okeys, request):
+    """The same LWE key with the one-product FP64 PBS (pbs_approx = 1), blind-rotating
+    one or two key bits per step (pbs_multibit)."""
     import tfhe_amd
-    c = tfhe_amd.CbsContext(max_blocks=3, params=tfhe_amd.cbs_params(pbs_approx=1))
+    c = tfhe_amd.CbsContext(max_blocks=3, params=tfhe_amd.cbs_params(pbs_approx=1, pbs_multibit=request.param))
     c.keygen(okeys.s, SEED)
     yield c
     c.close()
@@ -379,6 +380,9 @@ def test_cbs_error_paths(gpu):
     L = tfhe_amd.lib()
     with pytest.raises(tfhe_amd.TfheError) as e:  # no such PBS precision form
         tfhe_amd.CbsContext(max_blocks=1, params=tfhe_amd.cbs_params(pbs_approx=2))
+    assert e.value.code == tfhe_amd.TFHE_E_PARAM
+    with pytest.raises(tfhe_amd.TfheError) as e:  # two key bits per step need the one-product form
+        tfhe_amd.CbsContext(max_blocks=1, params=tfhe_amd.cbs_params(pbs_approx=0, pbs_multibit=2))
     assert e.value.code == tfhe_amd.TFHE_E_PARAM
     with pytest.raises(tfhe_amd.TfheError) as e:  # above the 4096-block capacity bound
         tfhe_amd.CbsContext(max_blocks=4097)
