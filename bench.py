@@ -49,7 +49,7 @@ def cpu_baseline(sample=24):
                       f"oracle C restatement, FP64 radix-2 FFT, one thread"}
 
 
-def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1):
+def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0):
     """AES-128 CTR keystream: circuit bootstrap + S-box LUT per round,
     linear layers as LWE additions; every block decrypted and compared with plain AES.
     pbs_approx=1 runs the one-product FP64 PBS instead of the exact two-limb one;
@@ -104,6 +104,20 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1):
             gdt = time.perf_counter() - t0
             out[str(blocks)]["graph_s_per_launch"] = round(gdt, 4)
             out[str(blocks)]["graph_equals_eager"] = bool(torch.equal(d_ks[:blocks], eager))
+    if plain_blocks:  # the per-bootstrap form: every bootstrap, LUT and key switch a launch of one
+        ctrs = [nonce + int(b).to_bytes(8, "big") for b in range(plain_blocks)]
+        ctx.aes_ctr_keystream(ctrs, d_ks)
+        torch.cuda.synchronize()
+        eager = d_ks[:plain_blocks].clone()
+        t0 = time.perf_counter()
+        ctx.aes_ctr_keystream(ctrs, d_ks, plain=True)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out["plain_per_bootstrap"] = {"blocks": plain_blocks, "blocks_per_s": round(plain_blocks / dt, 4),
+                                      "s_per_launch": round(dt, 3),
+                                      "equals_batched": bool(torch.equal(d_ks[:plain_blocks], eager))}
+        print(f"aes128 ctr plain per-bootstrap form: {plain_blocks} block(s) in {dt:.2f} s", file=sys.stderr,
+              flush=True)
     ctx.close()
     pbs = ("one FP64 product per digit polynomial, rounding error ~2^-33" if pbs_approx
            else "2-limb exact") + (", two key bits per blind-rotation step" if pbs_multibit == 2 else "")
@@ -126,6 +140,8 @@ def main():
                     help="also measure the FP64 and (unpacked) FP32 FMA throughput of the library's probe kernel")
     ap.add_argument("--aes-blocks", default="1,17,256,1024",
                     help="comma-separated AES-128 CTR batch sizes to time ('' to skip)")
+    ap.add_argument("--aes-plain-blocks", type=int, default=1,
+                    help="AES blocks timed in the plain per-bootstrap form (exact PBS; 0 to skip)")
     ap.add_argument("--aes-pbs", default="exact,approx",
                     help="comma-separated PBS forms of the AES runs: exact (two-limb), approx (one FP64 "
                          "product), approx-mb2 (one product, two key bits per step)")
@@ -300,7 +316,14 @@ def main():
             if f not in forms:
                 sys.exit(f"--aes-pbs: unknown form {f!r} (exact, approx, approx-mb2)")
             key, approx, mb = forms[f]
-            line[key] = aes_ctr_bench(sizes, pbs_approx=approx, pbs_multibit=mb)
+            line[key] = aes_ctr_bench(sizes, pbs_approx=approx, pbs_multibit=mb,
+                                      plain_blocks=args.aes_plain_blocks if f == "exact" else 0)
+        # batched blocks/s over the plain FP64 per-bootstrap form's
+        plain = line.get("aes128_ctr", {}).get("batches", {}).get("plain_per_bootstrap")
+        if plain:
+            best = max((v["blocks_per_s"] for k in forms.values() if k[0] in line
+                        for b, v in line[k[0]]["batches"].items() if b != "plain_per_bootstrap"), default=None)
+            line["aes128_batched_over_plain"] = None if best is None else round(best / plain["blocks_per_s"], 1)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

@@ -457,13 +457,42 @@ extern "C" int tfhe_aes_round_batch(tfhe_cbs_ctx* c, int round, const uint32_t* 
 
 // the keystream's kernels after the counter upload: state init from round key 0
 // and the counter blocks in d_ctr, then rounds 1..10 ping-ponging the state buffers
-static int aes_keystream_kernels(tfhe_cbs_ctx* c, uint32_t blocks, uint32_t* d_out, hipStream_t st) {
+// One round with every circuit bootstrap, LUT and key switch launched on its own
+// (the per-bootstrap form): the same kernels and buffers as tfhe_aes_round_batch,
+// so the output is the same bit for bit.
+static int aes_round_plain(tfhe_cbs_ctx* c, int round, const uint32_t* d_in, uint32_t blocks, uint32_t* d_out,
+                           hipStream_t st) {
+    const uint32_t nb = 128 * blocks;
+    const size_t rs = (size_t)c->p.n + 1, rgsw_words = (size_t)4 * 2 * kN, rgsw_f = (size_t)8 * 1024;
+    int rc = TFHE_OK;
+    for (uint32_t b = 0; b < nb && !rc; ++b)
+        rc = tfhe_cbs_batch(c, d_in + b * rs, 1, c->d_rgsw + b * rgsw_words, st);
+    for (uint32_t b = 0; b < nb && !rc; ++b)
+        rc = tfhe_rgsw_to_fourier(c, c->d_rgsw + b * rgsw_words, 1,
+                                  reinterpret_cast<double*>(c->d_rgsw_f + b * rgsw_f), st);
+    for (uint32_t lut = 0; lut < 16 * blocks && !rc; ++lut)
+        rc = tfhe_lut8_eval_batch(c, reinterpret_cast<const double*>(c->d_rgsw_f + (size_t)8 * lut * rgsw_f), 1,
+                                  c->d_sbox_poly, 0, 8, c->d_lut32 + (size_t)8 * lut * (kN + 1), st);
+    uint32_t* sb = (d_out == c->d_state[0]) ? c->d_state[1] : c->d_state[0];
+    for (uint32_t b = 0; b < nb && !rc; ++b)
+        rc = tfhe_cbs_keyswitch_batch(c, c->d_lut32 + (size_t)b * (kN + 1), 1, sb + b * rs, st);
+    if (!rc) {
+        launch_aes_linear(sb, c->d_rk + (size_t)round * 128 * rs, round == 10 ? c->d_lin_last : c->d_lin_mix, kLinSrc,
+                          blocks, c->p.n, d_out, st);
+        rc = check_launch("aes_linear");
+    }
+    return rc;
+}
+
+static int aes_keystream_kernels(tfhe_cbs_ctx* c, uint32_t blocks, uint32_t* d_out, hipStream_t st,
+                                 bool plain = false) {
     launch_aes_init(c->d_rk, c->d_ctr, blocks, c->p.n, c->d_state[0], st);
     int rc = check_launch("aes_init");
     int cur = 0;
     for (int round = 1; round <= 10 && !rc; ++round) {
         uint32_t* dst = (round == 10) ? d_out : c->d_state[cur ^ 1];
-        rc = tfhe_aes_round_batch(c, round, c->d_state[cur], blocks, dst, st);
+        rc = plain ? aes_round_plain(c, round, c->d_state[cur], blocks, dst, st)
+                   : tfhe_aes_round_batch(c, round, c->d_state[cur], blocks, dst, st);
         cur ^= 1;
     }
     return rc;
@@ -484,6 +513,16 @@ extern "C" int tfhe_aes_ctr_keystream_batch(tfhe_cbs_ctx* c, const uint8_t* h_ct
     rc = hipcheck(hipMemcpyAsync(c->d_ctr, h_ctr, (size_t)blocks * 16, hipMemcpyHostToDevice, st), "ctr");
     if (rc) return rc;
     return aes_keystream_kernels(c, blocks, d_out, st);
+}
+
+extern "C" int tfhe_aes_ctr_keystream_plain(tfhe_cbs_ctx* c, const uint8_t* h_ctr, uint32_t blocks, uint32_t* d_out,
+                                            void* stream) {
+    int rc = aes_ctr_check(c, h_ctr, blocks, d_out);
+    if (rc || !blocks) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    rc = hipcheck(hipMemcpyAsync(c->d_ctr, h_ctr, (size_t)blocks * 16, hipMemcpyHostToDevice, st), "ctr");
+    if (rc) return rc;
+    return aes_keystream_kernels(c, blocks, d_out, st, true);
 }
 
 extern "C" int tfhe_aes_ctr_keystream_graph(tfhe_cbs_ctx* c, const uint8_t* h_ctr, uint32_t blocks, uint32_t* d_out,

@@ -109,6 +109,8 @@ SIGNATURES = {
     "tfhe_probe_fma_tflops": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "tfhe_aes_ctr_keystream_graph": (ctypes.c_int, [_vp, np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
                                                     ctypes.c_uint32, _vp, _vp]),
+    "tfhe_aes_ctr_keystream_plain": (ctypes.c_int, [_vp, np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
+                                                    ctypes.c_uint32, _vp, _vp]),
     "tfhe_lwe_xor_public": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_int, _vp]),
 }
 
@@ -292,11 +294,13 @@ class CbsContext:
     def aes_round(self, rnd, d_in, blocks, d_out, stream=None):
         check(lib().tfhe_aes_round_batch(self.h, rnd, _ptr(d_in), blocks, _ptr(d_out), _stream(stream)))
 
-    def aes_ctr_keystream(self, ctr_blocks, d_out, stream=None, graph=False):
+    def aes_ctr_keystream(self, ctr_blocks, d_out, stream=None, graph=False, plain=False):
         """CTR keystream bits of AES_k(ctr) for each 16-byte counter block; graph=True
-        replays a captured hipGraph (tfhe_aes_ctr_keystream_graph)."""
+        replays a captured hipGraph (tfhe_aes_ctr_keystream_graph), plain=True launches
+        every bootstrap on its own (tfhe_aes_ctr_keystream_plain)."""
         ctr = np.ascontiguousarray(np.frombuffer(b"".join(ctr_blocks), np.uint8))
-        fn = lib().tfhe_aes_ctr_keystream_graph if graph else lib().tfhe_aes_ctr_keystream_batch
+        fn = (lib().tfhe_aes_ctr_keystream_graph if graph else
+              lib().tfhe_aes_ctr_keystream_plain if plain else lib().tfhe_aes_ctr_keystream_batch)
         check(fn(self.h, ctr, len(ctr_blocks), _ptr(d_out), _stream(stream)))
 
 

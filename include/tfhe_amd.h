@@ -237,6 +237,14 @@ int tfhe_aes_round_batch(tfhe_cbs_ctx* ctx, int round, const uint32_t* d_in, uin
  * public counter blocks h_ctr [blocks][16] (host) */
 int tfhe_aes_ctr_keystream_batch(tfhe_cbs_ctx* ctx, const uint8_t* h_ctr, uint32_t blocks, uint32_t* d_out,
                                  void* stream);
+/* The same keystream in the per-bootstrap form: every circuit bootstrap, S-box
+ * LUT and key switch is its own launch of one (the reference's single-ciphertext
+ * latency path gpu_CircuitBootstrapping, cirbts-bootstrap.cu:89-596, and
+ * EvalLUT8x32_RLWE_GPU, gpu_lut.cuh:1436).  Results equal
+ * tfhe_aes_ctr_keystream_batch bit for bit; it is the plain form the batched
+ * ones are measured against. */
+int tfhe_aes_ctr_keystream_plain(tfhe_cbs_ctx* ctx, const uint8_t* h_ctr, uint32_t blocks, uint32_t* d_out,
+                                 void* stream);
 /* The same keystream replayed from a captured hipGraph (the reference captures
  * its LUT and circuit-bootstrap launch sequences as CUDA graphs with thread-local
  * capture, gpu_lut.cuh:1859-2039, cirbts-batch.cu:559-581): the first

@@ -298,6 +298,23 @@ def test_aes_keystream_graph_replays_match_eager(ctx, okeys):
         assert _decrypt_blocks(okeys, from_dev(d_graph), 2) == [O.aes128_encrypt(key, c) for c in ctrs]
 
 
+def test_aes_plain_form_equals_batched(ctx, okeys):
+    """The per-bootstrap keystream (every circuit bootstrap, LUT and key switch a
+    launch of one) equals the batched keystream bit for bit and decrypts to AES."""
+    import torch
+    import tfhe_amd
+    key = bytes(range(50, 66))
+    ctx.set_round_keys(tfhe_amd.aes_encrypt_key_bits(okeys.s, 81, key))
+    ctr = [bytes(np.random.default_rng(6).integers(0, 256, 16, dtype=np.uint8))]
+    d_batch = torch.empty((1, 128, 637), dtype=torch.int32, device="cuda")
+    d_plain = torch.empty((1, 128, 637), dtype=torch.int32, device="cuda")
+    ctx.aes_ctr_keystream(ctr, d_batch)
+    ctx.aes_ctr_keystream(ctr, d_plain, plain=True)
+    torch.cuda.synchronize()
+    assert np.array_equal(from_dev(d_plain), from_dev(d_batch))
+    assert _decrypt_blocks(okeys, from_dev(d_plain), 1) == [O.aes128_encrypt(key, ctr[0])]
+
+
 @pytest.fixture(scope="module", params=[1, 2], ids=["one_bit", "two_bits"])
 def actx(gpu, okeys, request):
     """The same LWE key with the one-product FP64 PBS (pbs_approx = 1), blind-rotating
