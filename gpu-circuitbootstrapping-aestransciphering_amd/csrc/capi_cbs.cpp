@@ -220,30 +220,97 @@ extern "C" int tfhe_cbs_ctx_create(const tfhe_cbs_params* params, uint32_t max_b
     return TFHE_OK;
 }
 
-extern "C" int tfhe_gen_bsk64(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_lwe_sk, const int32_t* d_rlwe_sk,
-                              uint64_t* d_bsk, void* stream) {
-    if (!cbs_params_supported(p) || !d_lwe_sk || !d_rlwe_sk || !d_bsk) return fail(TFHE_E_PARAM, "tfhe_gen_bsk64: bad argument");
+// Fourier transform of the RLWE secret (host, N int32) into a new device buffer
+// (2 limbs, k_to_fourier2048 layout), as the key generators take it
+static int k2_fourier(const int32_t* h_K2, const FFT1024Tables& T, cplx** d_K2f, hipStream_t st) {
+    uint64_t* d_K64 = nullptr;
+    *d_K2f = nullptr;
+    int rc = dmalloc((void**)d_K2f, 2 * 1024 * sizeof(cplx));
+    rc = rc ? rc : dmalloc((void**)&d_K64, kN * sizeof(uint64_t));
+    std::vector<uint64_t> hK64(kN);
+    for (int i = 0; i < kN; ++i) hK64[i] = (uint64_t)(int64_t)h_K2[i];
+    if (!rc) rc = hipcheck(hipMemcpy(d_K64, hK64.data(), kN * sizeof(uint64_t), hipMemcpyHostToDevice), "K64");
+    if (!rc) launch_to_fourier2048(d_K64, 1, T, *d_K2f, st);
+    if (!rc) rc = check_launch("K2 fourier");
+    if (!rc) rc = hipcheck(hipStreamSynchronize(st), "K2 fourier sync");
+    (void)hipFree(d_K64);
+    return rc;
+}
+
+static int stateless_tables(DeviceTables1024*& t) {
     static DeviceTables1024 tables;
     if (!tables.d_buf) {
         int rc = tables.init();
         if (rc) return rc;
     }
+    t = &tables;
+    return TFHE_OK;
+}
+
+extern "C" int tfhe_gen_bsk64(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_lwe_sk, const int32_t* d_rlwe_sk,
+                              uint64_t* d_bsk, void* stream) {
+    if (!cbs_params_supported(p) || !d_lwe_sk || !d_rlwe_sk || !d_bsk) return fail(TFHE_E_PARAM, "tfhe_gen_bsk64: bad argument");
+    DeviceTables1024* tables = nullptr;
+    int rc = stateless_tables(tables);
+    if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
-    cplx* d_K2f = nullptr;
-    uint64_t* d_K64 = nullptr;
-    int rc = dmalloc((void**)&d_K2f, 2 * 1024 * sizeof(cplx));
-    rc = rc ? rc : dmalloc((void**)&d_K64, kN * sizeof(uint64_t));
     std::vector<int32_t> hK(kN);
-    std::vector<uint64_t> hK64(kN);
-    if (!rc) rc = hipcheck(hipMemcpy(hK.data(), d_rlwe_sk, kN * sizeof(int32_t), hipMemcpyDeviceToHost), "K2");
-    for (int i = 0; i < kN; ++i) hK64[i] = (uint64_t)(int64_t)hK[i];
-    if (!rc) rc = hipcheck(hipMemcpy(d_K64, hK64.data(), kN * sizeof(uint64_t), hipMemcpyHostToDevice), "K64");
-    if (!rc) launch_to_fourier2048(d_K64, 1, tables.T, d_K2f, st);
-    if (!rc) launch_gen_bsk64(seed, p->n, p->rlwe_nshift, d_lwe_sk, d_rlwe_sk, d_K2f, tables.T, d_bsk, st);
+    rc = hipcheck(hipMemcpy(hK.data(), d_rlwe_sk, kN * sizeof(int32_t), hipMemcpyDeviceToHost), "K2");
+    cplx* d_K2f = nullptr;
+    if (!rc) rc = k2_fourier(hK.data(), tables->T, &d_K2f, st);
+    if (!rc) launch_gen_bsk64(seed, p->n, p->rlwe_nshift, d_lwe_sk, d_rlwe_sk, d_K2f, tables->T, d_bsk, st, pbs_mb(*p));
     if (!rc) rc = check_launch("gen_bsk64");
     if (!rc) rc = hipcheck(hipStreamSynchronize(st), "gen_bsk64 sync");
     (void)hipFree(d_K2f);
-    (void)hipFree(d_K64);
+    return rc;
+}
+
+extern "C" int tfhe_gen_pfks32(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_rlwe_sk, uint32_t* d_pfks,
+                               void* stream) {
+    if (!cbs_params_supported(p) || !d_rlwe_sk || !d_pfks) return fail(TFHE_E_PARAM, "tfhe_gen_pfks32: bad argument");
+    DeviceTables1024* tables = nullptr;
+    int rc = stateless_tables(tables);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<int32_t> hK(kN);
+    rc = hipcheck(hipMemcpy(hK.data(), d_rlwe_sk, kN * sizeof(int32_t), hipMemcpyDeviceToHost), "K2");
+    cplx* d_K2f = nullptr;
+    if (!rc) rc = k2_fourier(hK.data(), tables->T, &d_K2f, st);
+    if (!rc) launch_gen_pfks(seed, p->pf_nshift, d_rlwe_sk, d_K2f, tables->T, d_pfks, st);
+    if (!rc) rc = check_launch("gen_pfks32");
+    if (!rc) rc = hipcheck(hipStreamSynchronize(st), "gen_pfks32 sync");
+    (void)hipFree(d_K2f);
+    return rc;
+}
+
+extern "C" int tfhe_cbs_set_keys(tfhe_cbs_ctx* c, const int32_t* rlwe_sk, const uint64_t* bsk, const uint32_t* pfks,
+                                 const uint32_t* ksk2) {
+    if (!c || !rlwe_sk || !bsk || !pfks || !ksk2) return fail(TFHE_E_PARAM, "tfhe_cbs_set_keys: bad argument");
+    const auto& p = c->p;
+    int rc = TFHE_OK;
+    if (!c->d_pfks) rc = dmalloc((void**)&c->d_pfks, 2 * pfks_words_per_u() * sizeof(uint32_t));
+    uint64_t* d_bsk = nullptr;
+    const size_t bsk_words = bsk2_rgsw(p) * 8 * 2 * kN;
+    rc = rc ? rc : dmalloc((void**)&d_bsk, bsk_words * sizeof(uint64_t));
+    c->has_keys = false;
+    c->K2.assign(rlwe_sk, rlwe_sk + kN);
+    c->lwe_sk.clear();
+    // host or device sources: hipMemcpyDefault follows the pointers
+    if (!rc) rc = hipcheck(hipMemcpy(c->d_K2, c->K2.data(), kN * sizeof(int32_t), hipMemcpyHostToDevice), "K2");
+    if (!rc) rc = hipcheck(hipMemcpy(d_bsk, bsk, bsk_words * sizeof(uint64_t), hipMemcpyDefault), "bsk2");
+    if (!rc) rc = hipcheck(hipMemcpy(c->d_pfks, pfks, 2 * pfks_words_per_u() * sizeof(uint32_t), hipMemcpyDefault),
+                           "pfks");
+    if (!rc) rc = hipcheck(hipMemcpy(c->d_ksk2, ksk2, ksk2_words(p) * sizeof(uint32_t), hipMemcpyDefault), "ksk2");
+    if (!rc) {
+        if (pbs_limbs(p) == 2)
+            launch_to_fourier2048(d_bsk, bsk2_rgsw(p) * 8 * 2, c->tables.T, c->d_bsk2f, nullptr);
+        else
+            launch_to_fourier2048_t64(d_bsk, bsk2_rgsw(p) * 8 * 2, c->tables.T, c->d_bsk2f, nullptr);
+        rc = check_launch("set_keys fourier");
+    }
+    if (!rc) rc = hipcheck(hipDeviceSynchronize(), "set_keys sync");
+    (void)hipFree(d_bsk);
+    if (!rc) c->has_keys = true;
     return rc;
 }
 

@@ -91,6 +91,8 @@ SIGNATURES = {
     "tfhe_cbs_get_pfks_entry": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")]),
     "tfhe_gen_bsk64": (ctypes.c_int, [ctypes.POINTER(CbsParams), ctypes.c_uint64, _vp, _vp, _vp, _vp]),
+    "tfhe_gen_pfks32": (ctypes.c_int, [ctypes.POINTER(CbsParams), ctypes.c_uint64, _vp, _vp, _vp]),
+    "tfhe_cbs_set_keys": (ctypes.c_int, [_vp, _i32p, _vp, _vp, _vp]),
     "tfhe_cbs_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_cbs_pbs_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_cbs_pfks_batch": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_uint32, ctypes.c_size_t, _vp, _vp]),
@@ -261,6 +263,13 @@ class CbsContext:
         K = np.zeros(self.p.N, np.int32)
         check(lib().tfhe_cbs_get_rlwe_sk(self.h, K))
         return K
+
+    def set_keys(self, rlwe_sk, bsk, pfks, ksk2):
+        """Upload keys made elsewhere (tfhe_cbs_set_keys); bsk / pfks / ksk2 are numpy
+        arrays (host) or device tensors."""
+        def p_(a):
+            return a.ctypes.data if isinstance(a, np.ndarray) else _ptr(a)
+        check(lib().tfhe_cbs_set_keys(self.h, np.ascontiguousarray(rlwe_sk, np.int32), p_(bsk), p_(pfks), p_(ksk2)))
 
     def pfks_entry(self, u, i, t, v):
         out = np.zeros(2 * self.p.N, np.uint32)

@@ -173,9 +173,22 @@ int tfhe_cbs_keygen(tfhe_cbs_ctx* ctx, const int32_t* lwe_sk, uint64_t seed);
 int tfhe_cbs_get_rlwe_sk(const tfhe_cbs_ctx* ctx, int32_t* out);
 /* host copy of PFKS key entry (u, i, t, v), 2N uint32 (keygen parity) */
 int tfhe_cbs_get_pfks_entry(const tfhe_cbs_ctx* ctx, int u, int i, int t, int v, uint32_t* out);
-/* stateless BSK2 generator into d_bsk [n][8][2][N] uint64 (keygen parity) */
+/* stateless BSK2 generator into d_bsk [n][8][2][N] uint64, or [n/2][3][8][2][N]
+ * pattern keys with pbs_multibit = 2 (keygen parity) */
 int tfhe_gen_bsk64(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_lwe_sk, const int32_t* d_rlwe_sk,
                    uint64_t* d_bsk, void* stream);
+/* stateless PFKS key generator into d_pfks [2][N+1][8][15][2N] uint32 (the
+ * context's key; tfhe_cbs_get_pfks_entry reads entries of it) */
+int tfhe_gen_pfks32(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_rlwe_sk, uint32_t* d_pfks,
+                    void* stream);
+/* Keys made elsewhere into the context, as GPUCirBTSContext(cc, maxBatchSize)
+ * uploads the CPU context's keys (cirbts.cuh:420-426, gpu_init_cirbsk
+ * cirbts-init.cu:200-865): rlwe_sk host [N] int32; bsk coefficient rows as
+ * tfhe_gen_bsk64 writes them, pfks as tfhe_gen_pfks32, ksk2 [N][8][4][n+1]
+ * uint32.  The three key buffers may be host or device memory; the BSK2 is
+ * transformed to the context's PBS form. */
+int tfhe_cbs_set_keys(tfhe_cbs_ctx* ctx, const int32_t* rlwe_sk, const uint64_t* bsk, const uint32_t* pfks,
+                      const uint32_t* ksk2);
 
 /* gpu_CircuitBootstrappingBatchToDeviceFromDeviceLWE (cirbts.cuh:466-474,
  * cirbts-batch.cu:1416-1879): d_lwe [batch][n+1] -> d_rgsw [batch][4][2][N] uint32.

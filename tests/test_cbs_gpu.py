@@ -298,6 +298,41 @@ def test_aes_keystream_graph_replays_match_eager(ctx, okeys):
         assert _decrypt_blocks(okeys, from_dev(d_graph), 2) == [O.aes128_encrypt(key, c) for c in ctrs]
 
 
+def test_set_keys_from_stateless_generators(ctx, okeys):
+    """Keys made outside a context (tfhe_gen_bsk64 / tfhe_gen_pfks32 on the device,
+    the oracle's KSK2 on the host) uploaded with tfhe_cbs_set_keys give the same
+    PBS and circuit-bootstrap outputs as the keygen'd context, bit for bit."""
+    import torch
+    import tfhe_amd
+    p = tfhe_amd.cbs_params()
+    L = tfhe_amd.lib()
+    d_s = to_dev32(okeys.s.view(np.uint32))
+    d_K2 = to_dev32(okeys.K2.view(np.uint32))
+    d_bsk = torch.empty(okeys.bsk.size, dtype=torch.int64, device="cuda")
+    tfhe_amd.check(L.tfhe_gen_bsk64(p, SEED, tfhe_amd._ptr(d_s), tfhe_amd._ptr(d_K2), tfhe_amd._ptr(d_bsk), None))
+    d_pfks = torch.empty(2 * 2049 * 8 * 15 * 2 * 2048, dtype=torch.int32, device="cuda")
+    tfhe_amd.check(L.tfhe_gen_pfks32(p, SEED, tfhe_amd._ptr(d_K2), tfhe_amd._ptr(d_pfks), None))
+    c2 = tfhe_amd.CbsContext(max_blocks=1)
+    try:
+        c2.set_keys(okeys.K2, d_bsk, d_pfks, okeys.ksk2)
+        del d_pfks
+        for (u, i, t, v) in [(0, 3, 1, 2), (1, 2048, 7, 15)]:
+            assert np.array_equal(c2.pfks_entry(u, i, t, v), okeys.pfks_entry(u, i, t, v)), (u, i, t, v)
+        cts = to_dev32(_bits_cts(okeys, [1, 0, 1], 900))
+        outs = []
+        for c in (ctx, c2):
+            d_pbs = torch.empty((3, 2, 2049), dtype=torch.int64, device="cuda")
+            d_rgsw = torch.empty((3, 4, 2, 2048), dtype=torch.int32, device="cuda")
+            c.pbs_batch(cts, d_pbs)
+            c.cbs_batch(cts, d_rgsw)
+            torch.cuda.synchronize()
+            outs.append((from_dev(d_pbs, np.uint64), from_dev(d_rgsw)))
+        assert np.array_equal(outs[0][0], outs[1][0])
+        assert np.array_equal(outs[0][1], outs[1][1])
+    finally:
+        c2.close()
+
+
 def test_aes_plain_form_equals_batched(ctx, okeys):
     """The per-bootstrap keystream (every circuit bootstrap, LUT and key switch a
     launch of one) equals the batched keystream bit for bit and decrypts to AES."""
