@@ -49,6 +49,27 @@ def cpu_baseline(sample=24):
                       f"oracle C restatement, FP64 radix-2 FFT, one thread"}
 
 
+def one_product_error(batch=256):
+    """Max coefficient error of the PBS's one-product form against the exact
+    integer product (both on the GPU; the exact one is bit-exact by the tests):
+    digit polynomials in [-256, 256) times uniform torus-64 polynomials, N = 2048."""
+    import numpy as np
+    import torch
+    import tfhe_amd
+    rng = np.random.default_rng(5)
+    x = torch.from_numpy(rng.integers(-256, 256, (batch, 2048)).astype(np.int32)).to("cuda")
+    y = torch.from_numpy(rng.integers(0, 2 ** 64, (batch, 2048), dtype=np.uint64).view(np.int64)).to("cuda")
+    exact = torch.empty_like(y)
+    approx = torch.empty_like(y)
+    tfhe_amd.polymul2048_batch(x, y, exact)
+    tfhe_amd.polymul2048_batch(x, y, approx, approx=True)
+    torch.cuda.synchronize()
+    d = (approx.cpu().numpy().view(np.uint64) - exact.cpu().numpy().view(np.uint64)).view(np.int64)
+    worst = int(np.abs(d).max())
+    return {"products": batch, "max_abs_error_log2_of_torus": round(float(np.log2(max(worst, 1))) - 64, 2),
+            "rms_error_log2_of_torus": round(float(np.log2(np.sqrt(np.mean(d.astype(np.float64) ** 2)))) - 64, 2)}
+
+
 def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0):
     """AES-128 CTR keystream: circuit bootstrap + S-box LUT per round,
     linear layers as LWE additions; every block decrypted and compared with plain AES.
@@ -318,6 +339,8 @@ def main():
             key, approx, mb = forms[f]
             line[key] = aes_ctr_bench(sizes, pbs_approx=approx, pbs_multibit=mb,
                                       plain_blocks=args.aes_plain_blocks if f == "exact" else 0)
+            if approx:
+                line[key]["one_product_vs_exact_product"] = one_product_error()
         # batched blocks/s over the plain FP64 per-bootstrap form's
         plain = line.get("aes128_ctr", {}).get("batches", {}).get("plain_per_bootstrap")
         if plain:

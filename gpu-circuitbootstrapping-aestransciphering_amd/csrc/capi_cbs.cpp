@@ -447,17 +447,25 @@ extern "C" int tfhe_cbs_keyswitch_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe32
     return check_launch("keyswitch32 N=2048");
 }
 
-extern "C" int tfhe_polymul2048_batch(const int32_t* d_x, const uint64_t* d_y, uint32_t batch, uint64_t* d_out,
-                                      void* stream) {
+static int polymul2048(const int32_t* d_x, const uint64_t* d_y, uint32_t batch, uint64_t* d_out, void* stream,
+                       int limbs) {
     if (!d_x || !d_y || !d_out) return fail(TFHE_E_PARAM, "polymul2048: NULL argument");
     if (!batch) return TFHE_OK;
-    static DeviceTables1024 tables;
-    if (!tables.d_buf) {
-        int rc = tables.init();
-        if (rc) return rc;
-    }
-    launch_polymul2048(d_x, d_y, batch, tables.T, d_out, (hipStream_t)stream);
+    DeviceTables1024* tables = nullptr;
+    int rc = stateless_tables(tables);
+    if (rc) return rc;
+    launch_polymul2048(d_x, d_y, batch, tables->T, d_out, (hipStream_t)stream, limbs);
     return check_launch("polymul2048");
+}
+
+extern "C" int tfhe_polymul2048_batch(const int32_t* d_x, const uint64_t* d_y, uint32_t batch, uint64_t* d_out,
+                                      void* stream) {
+    return polymul2048(d_x, d_y, batch, d_out, stream, 2);
+}
+
+extern "C" int tfhe_polymul2048_approx_batch(const int32_t* d_x, const uint64_t* d_y, uint32_t batch,
+                                             uint64_t* d_out, void* stream) {
+    return polymul2048(d_x, d_y, batch, d_out, stream, 1);
 }
 
 // ---------------------------------------------------------------------------

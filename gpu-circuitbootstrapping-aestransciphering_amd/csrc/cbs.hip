@@ -193,7 +193,14 @@ __global__ __launch_bounds__(256) void k_to_fourier2048_t64(const uint64_t* __re
     }
 }
 
-// Exact negacyclic product of small-integer x by torus-64 y (N = 2048), one block per product.
+// Negacyclic product of small-integer x by torus-64 y (N = 2048), one block per
+// product.  LIMBS = 2: exact (two signed 32-bit limbs of y, each product rounded
+// exactly).  LIMBS = 1: the one-product form of the PBS (y scaled to the torus,
+// one FP64 product, rounded once with torus64_from_double); its difference from
+// the exact product is the per-product rounding error of that form.
+__device__ __forceinline__ uint64_t torus64_from_double(double y);
+
+template <int LIMBS>
 __global__ __launch_bounds__(256) void k_polymul2048(const int32_t* __restrict__ x, const uint64_t* __restrict__ y,
                                                     uint32_t batch, FFT1024Tables Tg, uint64_t* __restrict__ out) {
     __shared__ cplx tl[768];
@@ -202,6 +209,7 @@ __global__ __launch_bounds__(256) void k_polymul2048(const int32_t* __restrict__
     const FFT1024Tables T = stage_tables(Tg, tl);
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    constexpr double SCALE = 0x1p-74;  // 2^-64 / M
     cplx trw[4];
     load_trw(Tg, w, lane, trw);
     int buf = 0;
@@ -213,28 +221,44 @@ __global__ __launch_bounds__(256) void k_polymul2048(const int32_t* __restrict__
         for (int a = 0; a < 4; ++a) {
             const int c = own_coeff(w, lane, a);
             X[a] = {(double)xp[c], (double)xp[c + CB_M]};
-            double l0r, l1r, l0i, l1i;
-            split_limbs(yp[c], l0r, l1r);
-            split_limbs(yp[c + CB_M], l0i, l1i);
-            Y0[a] = {l0r, l0i};
-            Y1[a] = {l1r, l1i};
+            if constexpr (LIMBS == 2) {
+                double l0r, l1r, l0i, l1i;
+                split_limbs(yp[c], l0r, l1r);
+                split_limbs(yp[c + CB_M], l0i, l1i);
+                Y0[a] = {l0r, l0i};
+                Y1[a] = {l1r, l1i};
+            } else {
+                Y0[a] = {(double)(int64_t)yp[c] * SCALE, (double)(int64_t)yp[c + CB_M] * SCALE};
+            }
         }
         f1024_fwd<false>(X, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
         f1024_fwd<false>(Y0, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
-        f1024_fwd<false>(Y1, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        if constexpr (LIMBS == 2) {
+            f1024_fwd<false>(Y1, lds_all[w], xbuf[buf], T, trw, w, lane);
+            buf ^= 1;
+        }
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             Y0[s] = cmul(Y0[s], X[s]);
-            Y1[s] = cmul(Y1[s], X[s]);
+            if constexpr (LIMBS == 2) Y1[s] = cmul(Y1[s], X[s]);
         }
         f1024_inv<false>(Y0, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
-        f1024_inv<false>(Y1, lds_all[w], xbuf[buf], T, trw, w, lane); buf ^= 1;
+        if constexpr (LIMBS == 2) {
+            f1024_inv<false>(Y1, lds_all[w], xbuf[buf], T, trw, w, lane);
+            buf ^= 1;
+        }
         uint64_t* op = out + (size_t)q * CB_N;
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
             const int c = own_coeff(w, lane, a);
-            op[c] = (uint64_t)round_i64(Y0[a].re, 1.0 / CB_M) + ((uint64_t)round_i64(Y1[a].re, 1.0 / CB_M) << 32);
-            op[c + CB_M] = (uint64_t)round_i64(Y0[a].im, 1.0 / CB_M) + ((uint64_t)round_i64(Y1[a].im, 1.0 / CB_M) << 32);
+            if constexpr (LIMBS == 2) {
+                op[c] = (uint64_t)round_i64(Y0[a].re, 1.0 / CB_M) + ((uint64_t)round_i64(Y1[a].re, 1.0 / CB_M) << 32);
+                op[c + CB_M] =
+                    (uint64_t)round_i64(Y0[a].im, 1.0 / CB_M) + ((uint64_t)round_i64(Y1[a].im, 1.0 / CB_M) << 32);
+            } else {
+                op[c] = torus64_from_double(Y0[a].re);
+                op[c + CB_M] = torus64_from_double(Y0[a].im);
+            }
         }
     }
 }
@@ -1002,8 +1026,11 @@ void launch_to_fourier2048_32(const uint32_t* coef, size_t npoly, const FFT1024T
 }
 
 void launch_polymul2048(const int32_t* x, const uint64_t* y, uint32_t batch, const FFT1024Tables& T, uint64_t* out,
-                        hipStream_t st) {
-    k_polymul2048<<<grid_for(batch, 4096), 256, 0, st>>>(x, y, batch, T, out);
+                        hipStream_t st, int limbs) {
+    if (limbs == 2)
+        k_polymul2048<2><<<grid_for(batch, 4096), 256, 0, st>>>(x, y, batch, T, out);
+    else
+        k_polymul2048<1><<<grid_for(batch, 4096), 256, 0, st>>>(x, y, batch, T, out);
 }
 
 void launch_gen_bsk64(uint64_t seed, int n, int nshift, const int32_t* s, const int32_t* K2, const cplx* K2f,

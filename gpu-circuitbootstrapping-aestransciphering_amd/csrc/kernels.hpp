@@ -32,8 +32,9 @@ int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int l
 // circuit-bootstrapping level (N = 2048; PBS torus-64, PFKS / RGSW / LUT torus-32)
 struct FFT1024Tables;
 void launch_to_fourier2048(const uint64_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st);
+// limbs = 2: exact product; 1: the one-product form of the PBS (rounded once)
 void launch_polymul2048(const int32_t* x, const uint64_t* y, uint32_t batch, const FFT1024Tables& T, uint64_t* out,
-                        hipStream_t st);
+                        hipStream_t st, int limbs = 2);
 // mb = 1: [n][8][2][N] rows of RGSW(s_i); mb = 2: [n/2][3][8][2][N] rows of RGSW(pattern indicators)
 void launch_gen_bsk64(uint64_t seed, int n, int nshift, const int32_t* s, const int32_t* K2, const cplx* K2f,
                       const FFT1024Tables& T, uint64_t* bsk, hipStream_t st, int mb = 1);

@@ -100,6 +100,7 @@ SIGNATURES = {
     "tfhe_lut8_eval_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_size_t, ctypes.c_int, _vp, _vp]),
     "tfhe_cbs_keyswitch_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_polymul2048_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
+    "tfhe_polymul2048_approx_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_aes_encrypt_key_bits": (None, [ctypes.c_int, _i32p, ctypes.c_uint64,
                                          np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
                                          ctypes.c_int, _u32p]),
@@ -216,8 +217,11 @@ class GateContext:
         check(lib().tfhe_gate_keyswitch_batch(self.h, _ptr(d_lweN), d_lweN.shape[0], _ptr(d_out), _stream(stream)))
 
 
-def polymul2048_batch(d_x, d_y, d_out, stream=None):
-    check(lib().tfhe_polymul2048_batch(_ptr(d_x), _ptr(d_y), d_x.shape[0], _ptr(d_out), _stream(stream)))
+def polymul2048_batch(d_x, d_y, d_out, stream=None, approx=False):
+    """Negacyclic products x * y (N = 2048, torus-64 y): exact, or in the PBS's
+    one-product form (approx=True)."""
+    fn = lib().tfhe_polymul2048_approx_batch if approx else lib().tfhe_polymul2048_batch
+    check(fn(_ptr(d_x), _ptr(d_y), d_x.shape[0], _ptr(d_out), _stream(stream)))
 
 
 def aes_encrypt_key_bits(sk, seed, key, nshift=15):

@@ -222,6 +222,13 @@ int tfhe_cbs_keyswitch_batch(tfhe_cbs_ctx* ctx, const uint32_t* d_lwe32, uint32_
 /* Exact negacyclic product through the 4-wave FP64 FFT, N = 2048:
  * d_out = d_x * d_y mod (X^N + 1, 2^64), small-integer d_x, torus-64 d_y. */
 int tfhe_polymul2048_batch(const int32_t* d_x, const uint64_t* d_y, uint32_t batch, uint64_t* d_out, void* stream);
+/* The same product in the one-product form of the PBS (pbs_approx = 1): y scaled
+ * to the torus, one FP64 product, rounded once; its difference from
+ * tfhe_polymul2048_batch is the per-product rounding error of that form (the
+ * "max coefficient error against the exact integer product" of the reference's
+ * split-precision FFT, cirbts-fft.cu:159-226). */
+int tfhe_polymul2048_approx_batch(const int32_t* d_x, const uint64_t* d_y, uint32_t batch, uint64_t* d_out,
+                                  void* stream);
 
 /* Measured VALU FMA throughput (TFLOP/s) of a chain kernel in FP64 (fp64 = 1)
  * or FP32 (fp64 = 0), built like the rest of the library (no packed FP32).  No

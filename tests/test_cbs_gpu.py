@@ -58,6 +58,29 @@ def test_polymul2048_exact(gpu):
         assert np.array_equal(got[k], O.nc_mul_exact64(x[k], y[k])), k
 
 
+def test_polymul2048_one_product_error(gpu):
+    """The PBS's one-product form of a digit-polynomial x torus-64 product differs
+    from the exact integer product (the oracle's schoolbook restatement) by FP64
+    rounding only: every coefficient within 2^-36 of the torus (a CPU emulation
+    gives about 2^-39)."""
+    import torch
+    import tfhe_amd
+    rng = np.random.default_rng(14)
+    batch = 8
+    x = rng.integers(-256, 256, (batch, 2048)).astype(np.int32)
+    y = rng.integers(0, 2 ** 64, (batch, 2048), dtype=np.uint64)
+    d_out = torch.empty((batch, 2048), dtype=torch.int64, device="cuda")
+    tfhe_amd.polymul2048_batch(to_dev32(x.view(np.uint32)), to_dev64(y), d_out, approx=True)
+    torch.cuda.synchronize()
+    got = from_dev(d_out, np.uint64)
+    worst = 0
+    for k in range(batch):
+        d = np.abs((got[k] - O.nc_mul_exact64(x[k], y[k])).view(np.int64))
+        worst = max(worst, int(d.max()))
+    print(f"one-product polymul: max |error| = 2^{np.log2(max(worst, 1)):.2f} of 2^64")
+    assert 0 < worst < 2 ** 28
+
+
 def test_keygen_parity(ctx, okeys):
     """GPU key generation reproduces the oracle's K2, BSK2 and PFKS entries bit for bit."""
     import torch
