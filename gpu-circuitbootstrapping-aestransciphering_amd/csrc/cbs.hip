@@ -1003,11 +1003,22 @@ void launch_zero_rows32(uint32_t* out, uint32_t count, size_t stride, uint32_t w
 
 void launch_pfks32(const uint32_t* at, uint32_t B, const uint32_t* key, size_t key_u_stride, int u0, int nu,
                    uint32_t* out, size_t out_stride, size_t out_u_offset, hipStream_t st) {
-    // i-slices: enough workgroups to keep the key stream busy at small batches (about 1536),
-    // few at large ones (each slice adds one atomic pass over the outputs)
+    // i-slices (4..32): the launch runs in rounds of as many workgroups as fit at once
+    // (two per CU: 92 VGPRs and 8 waves each), and a workgroup's time is its slice
+    // length, so pick the slice count with the fewest rounds x coefficients per
+    // slice; ties go to fewer slices (each slice is one more atomic pass over the outputs)
     const unsigned tiles = 2 * CB_N / 128, groups = pfks_bpad(B) / PF_GROUP;
-    const unsigned want = (1536 + tiles * groups * nu - 1) / (tiles * groups * nu);
-    const int slices = (int)(want < 4 ? 4 : want > 32 ? 32 : want);
+    const unsigned slots = 2 * device_cus(), per_slice = tiles * groups * (unsigned)nu;
+    int slices = 4;
+    unsigned long long best = ~0ull;
+    for (unsigned sl = 4; sl <= 32; ++sl) {
+        const unsigned long long rounds = ((unsigned long long)per_slice * sl + slots - 1) / slots;
+        const unsigned long long cost = rounds * ((CB_N + 1 + sl - 1) / sl);
+        if (cost < best) {
+            best = cost;
+            slices = (int)sl;
+        }
+    }
     dim3 grid(tiles, groups, (unsigned)(slices * nu));
     k_pfks32<<<grid, 64 * PF_WAVES, 0, st>>>(at, B, pfks_bpad(B), key, key_u_stride, u0, slices, out, out_stride,
                                               out_u_offset);

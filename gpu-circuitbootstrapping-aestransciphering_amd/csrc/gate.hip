@@ -382,8 +382,9 @@ __global__ __launch_bounds__(64 * KS_WAVES) void k_keyswitch32(const uint32_t* _
     const int tile = (int)blockIdx.x % cols, slice = (int)blockIdx.x / cols;
     const int col0 = tile * 128 + 2 * lane;
     const bool ok0 = col0 <= n, ok1 = col0 + 1 <= n;
-    const int islice = N / slices;
-    const int ibeg = slice * islice, iend = ibeg + islice;
+    const int islice = (N + slices - 1) / slices;
+    const int ibeg = slice * islice, iend = ibeg + islice < N ? ibeg + islice : N;
+    if (ibeg >= iend) return;  // empty last slice (uniform over the workgroup)
     const uint32_t cbase = blockIdx.y * KS_GROUP + w * KS_CT;
     const bool active = cbase < batch;  // idle waves still take part in the barriers
     const size_t rs = (size_t)(n + 1);
@@ -530,14 +531,29 @@ void launch_external_product1024(uint32_t* acc, cplx* digf, uint32_t* t, const c
 
 uint32_t ks_bpad(uint32_t batch) { return (batch + KS_GROUP - 1) / KS_GROUP * KS_GROUP; }
 
+unsigned device_cus() {
+    static unsigned cache[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cache[dev]) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+        cache[dev] = (unsigned)v;
+    }
+    return cache[dev];
+}
+
 int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int logB, int l, const uint32_t* ksk,
                        uint32_t* at, uint32_t* out, hipStream_t st) {
     if (!(logB == 2 && l == 8) || N % 64 || n + 1 > 64 * 1024) return -1;
     const uint32_t Bpad = ks_bpad(batch);
     const unsigned cols = (unsigned)(n + 1 + 127) / 128, groups = Bpad / KS_GROUP;
-    // i-slices: enough workgroups to fill the chip (about 512), each slice >= 32 coefficients
-    unsigned slices = 1;
-    while (cols * groups * slices < 512 && (unsigned)N / (slices * 2) >= 32) slices *= 2;
+    // i-slices: as many workgroups as run at once (two per CU: 100 VGPRs and 8 waves
+    // each), so the launch is one full round; each slice >= 32 coefficients
+    const unsigned slots = 2 * device_cus();
+    unsigned slices = slots / (cols * groups);
+    if (slices > (unsigned)N / 32) slices = (unsigned)N / 32;
+    if (slices < 1) slices = 1;
     const size_t count = (size_t)batch * (n + 1);
     k_fill_zero<<<(unsigned)((count + 255) / 256 < 4096 ? (count + 255) / 256 : 4096), 256, 0, st>>>(out, count);
     k_ks_digits<16><<<dim3((unsigned)N / 64, Bpad / 64), 256, 0, st>>>(lweN, batch, N, Bpad, at);

@@ -27,6 +27,9 @@ void launch_external_product1024(uint32_t* acc, cplx* digf, uint32_t* t, const c
                                  hipStream_t st);
 // key switch N -> n; at: scratch of N * ks_bpad(batch) u32 (rounded, transposed inputs)
 uint32_t ks_bpad(uint32_t batch);
+// compute units of the current device (cached per device), for launch shapes sized
+// to the number of workgroups that run at once
+unsigned device_cus();
 int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int logB, int l, const uint32_t* ksk,
                        uint32_t* at, uint32_t* out, hipStream_t st);
 // circuit-bootstrapping level (N = 2048; PBS torus-64, PFKS / RGSW / LUT torus-32)
