@@ -2,9 +2,10 @@
 // Fourier conversions, key generation, the multi-value programmable bootstrap
 // (torus-64), private functional key switching to torus-32 RLWE, the torus-32
 // CMux vertical-packing LUT and the AES linear layers.  A product of a digit
-// polynomial with a torus-64 polynomial is two FP64 products (one per signed
-// 32-bit limb), rounded and recombined mod 2^64; with a torus-32 polynomial it
-// is one FP64 product.  Both are exact.
+// polynomial with a torus-64 polynomial is either two FP64 products (one per
+// signed 32-bit limb), rounded and recombined mod 2^64, which is exact, or (the
+// one-product PBS form) one FP64 product of the polynomial scaled to the torus,
+// rounded once.  With a torus-32 polynomial it is one FP64 product, exact.
 #include "fft1024.hpp"
 #include "kernels.hpp"
 
@@ -406,7 +407,8 @@ __global__ __launch_bounds__(256) void k_gen_pfks32(uint64_t seed, uint32_t nrow
 // Steps (reference ManyLUT path, cirbts-batch.cu:3293-4006, restated for the
 // torus-64 CGGI accumulator): b += 2^30; a~_i = special_ms(a_i, bw = 1);
 // test vector with b~ folded in; n CMux steps acc += (X^m - 1)(dec(acc) (*) BSK_i),
-// m = -a~_i mod 2N; extract coefficients 0 and 1 and add mu_j = 2^(47 + 8j).
+// m = -a~_i mod 2N (or n/2 steps over key-bit pairs, MB = 2); extract
+// coefficients 0 and 1 and add mu_j = 2^(47 + 8j).
 // Output [batch][2][N+1] u64: LWE_K2 of b * 2^(48 + 8j).
 // ===========================================================================
 constexpr int PBS_LOGB = 9, PBS_L = 4, PBS_ROWS = 8, PBS_IGN = 28;
