@@ -238,13 +238,7 @@ static int k2_fourier(const int32_t* h_K2, const FFT1024Tables& T, cplx** d_K2f,
 }
 
 static int stateless_tables(DeviceTables1024*& t) {
-    static DeviceTables1024 tables;
-    if (!tables.d_buf) {
-        int rc = tables.init();
-        if (rc) return rc;
-    }
-    t = &tables;
-    return TFHE_OK;
+    return per_device_tables(t, [](DeviceTables1024& d) { return d.init(); });
 }
 
 extern "C" int tfhe_gen_bsk64(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_lwe_sk, const int32_t* d_rlwe_sk,

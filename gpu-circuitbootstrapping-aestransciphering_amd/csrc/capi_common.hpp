@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -97,5 +98,26 @@ struct DeviceTables1024 {
         d_buf = nullptr;
     }
 };
+
+// Process-lifetime twiddle tables for the stateless entry points, one set per
+// device (tables in one device's memory are not read from another), each built
+// once under a lock.
+constexpr int kMaxDevices = 64;
+template <class Tables, class Init>
+int per_device_tables(Tables*& out, Init init) {
+    static std::mutex mu;
+    static Tables tables[kMaxDevices];
+    int dev = 0;
+    int rc = hipcheck(hipGetDevice(&dev), "get device");
+    if (rc) return rc;
+    if (dev < 0 || dev >= kMaxDevices) return fail(TFHE_E_PARAM, "device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!tables[dev].d_buf) {
+        rc = init(tables[dev]);
+        if (rc) return rc;
+    }
+    out = &tables[dev];
+    return TFHE_OK;
+}
 
 }  // namespace tfhe

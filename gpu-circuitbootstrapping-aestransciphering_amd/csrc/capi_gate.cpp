@@ -251,12 +251,10 @@ extern "C" int tfhe_polymul1024_batch(const int32_t* d_x, const uint32_t* d_y, u
                                      void* stream) {
     if (!d_x || !d_y || !d_out) return fail(TFHE_E_PARAM, "polymul: NULL argument");
     if (batch == 0) return TFHE_OK;
-    static DeviceTables tables;  // process-lifetime twiddle tables for the stateless entry point
-    if (!tables.d_buf) {
-        int rc = tables.init(1024);
-        if (rc) return rc;
-    }
-    launch_polymul1024(d_x, d_y, batch, tables.T, d_out, (hipStream_t)stream);
+    DeviceTables* tables = nullptr;
+    int rc = per_device_tables(tables, [](DeviceTables& d) { return d.init(1024); });
+    if (rc) return rc;
+    launch_polymul1024(d_x, d_y, batch, tables->T, d_out, (hipStream_t)stream);
     return check_launch("polymul1024");
 }
 

@@ -1,6 +1,8 @@
 // gate.hip -- gfx950 kernels of the TFHE gate-bootstrapping path (N = 1024, torus-32):
 // key generation, BSK Fourier conversion, the persistent batched blind rotation,
 // the plain per-step form, sample extraction and the batched LWE key switch.
+#include <atomic>
+
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -532,15 +534,17 @@ void launch_external_product1024(uint32_t* acc, cplx* digf, uint32_t* t, const c
 uint32_t ks_bpad(uint32_t batch) { return (batch + KS_GROUP - 1) / KS_GROUP * KS_GROUP; }
 
 unsigned device_cus() {
-    static unsigned cache[64] = {0};
+    static std::atomic<unsigned> cache[64];  // zero-initialised (static storage); 0 = not yet read
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-    if (!cache[dev]) {
-        int v = 0;
-        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
-        cache[dev] = (unsigned)v;
+    unsigned v = cache[dev].load(std::memory_order_relaxed);
+    if (!v) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+        v = (unsigned)cus;
+        cache[dev].store(v, std::memory_order_relaxed);
     }
-    return cache[dev];
+    return v;
 }
 
 int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int logB, int l, const uint32_t* ksk,
