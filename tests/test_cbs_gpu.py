@@ -389,8 +389,8 @@ def _pbs_noise(K2, out, bit):
     K = K2.astype(np.int64).astype(np.uint64)
     errs = []
     for j in range(2):
-        ph = out[j, -1] - np.dot(out[j, :-1], K)
-        errs.append(abs(int((ph - np.uint64(bit << (48 + 8 * j))).view(np.int64))))
+        e = (int(out[j, -1]) - int(np.dot(out[j, :-1], K)) - (bit << (48 + 8 * j))) % 2 ** 64
+        errs.append(2 ** 64 - e if e >= 2 ** 63 else e)
     return max(errs)
 
 
