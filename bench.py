@@ -27,6 +27,11 @@ F_EP = 2 * (BK_L + 1) * 5 * M_FFT * 9 + 8 * 4 * BK_L * M_FFT
 F_PBS = N_LWE * F_EP  # 148.49 MFLOP per blind rotation
 BSK_BYTES = N_LWE * 2 * BK_L * 2 * M_FFT * 16  # Fourier BSK streamed once per pass (62.5 MB)
 PEAK_FP64_TFLOPS = 78.6
+PEAK_LDS_TBS = 78.6  # 256 CUs x 128 B/clk x 2.4 GHz
+# circuit-bootstrapping level: N = 2048 (M = 1024), PBS gadget 2^9 x 4 (8 digit rows)
+N_CB, M_CB, CB_ROWS = 2048, 1024, 8
+F_EP_CB = {1: (CB_ROWS + 2) * 5 * M_CB * 10 + 8 * 2 * CB_ROWS * M_CB,      # one product: 643,072
+           2: (CB_ROWS + 4) * 5 * M_CB * 10 + 8 * 2 * 2 * CB_ROWS * M_CB}  # two limbs: 876,544
 
 
 def cpu_baseline(sample=24):
@@ -68,6 +73,41 @@ def one_product_error(batch=256):
     worst = int(np.abs(d).max())
     return {"products": batch, "max_abs_error_log2_of_torus": round(float(np.log2(max(worst, 1))) - 64, 2),
             "rms_error_log2_of_torus": round(float(np.log2(np.sqrt(np.mean(d.astype(np.float64) ** 2)))) - 64, 2)}
+
+
+def aes_kernel_rooflines(ctx, pbs_approx, nb):
+    """PBS and PFKS of one AES round's shape timed on their own (HIP events): the PBS
+    against the FP64 peak by its algorithmic FLOPs, the PFKS against the LDS
+    bandwidth by the bytes its selects read (4 per output word per digit)."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(13)
+    lwe = torch.from_numpy(rng.integers(0, 2 ** 32, (nb, N_LWE + 1), dtype=np.uint64).astype(np.uint32)
+                           .view(np.int32)).to("cuda")
+    pbs_out = torch.empty((nb, 2, N_CB + 1), dtype=torch.int64, device="cuda")
+    rows = torch.empty((2 * nb, 2 * N_CB), dtype=torch.int32, device="cuda")
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    t_pbs = timed(lambda: ctx.pbs_batch(lwe, pbs_out))
+    t_pfks = timed(lambda: [ctx.pfks_batch(u, pbs_out, 2 * nb, 2 * N_CB, rows) for u in (0, 1)])
+    limbs = 1 if pbs_approx else 2
+    tflops = nb * N_LWE * F_EP_CB[limbs] / (t_pbs * 1e-3) / 1e12
+    lds_bytes = 2 * (2 * nb) * (N_CB + 1) * 8 * (2 * N_CB) * 4  # functions x inputs x digits x words x 4 B
+    lds_tbs = lds_bytes / (t_pfks * 1e-3) / 1e12
+    return {"inputs": nb,
+            "pbs": {"ms": round(t_pbs, 3), "bound": "fp64", "achieved": round(tflops, 2), "peak": PEAK_FP64_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(tflops / PEAK_FP64_TFLOPS, 4)},
+            "pfks_both_functions": {"ms": round(t_pfks, 3), "bound": "lds", "achieved": round(lds_tbs, 2),
+                                    "peak": PEAK_LDS_TBS, "unit": "TB/s", "frac": round(lds_tbs / PEAK_LDS_TBS, 4)}}
 
 
 def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0):
@@ -125,6 +165,8 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0)
             gdt = time.perf_counter() - t0
             out[str(blocks)]["graph_s_per_launch"] = round(gdt, 4)
             out[str(blocks)]["graph_equals_eager"] = bool(torch.equal(d_ks[:blocks], eager))
+    if pbs_multibit == 1:  # the round's two big kernels on their own, at 17 blocks' shape (or less)
+        out["kernels"] = aes_kernel_rooflines(ctx, pbs_approx, min(17, maxb) * 128)
     if plain_blocks:  # the per-bootstrap form: every bootstrap, LUT and key switch a launch of one
         ctrs = [nonce + int(b).to_bytes(8, "big") for b in range(plain_blocks)]
         ctx.aes_ctr_keystream(ctrs, d_ks)
@@ -345,7 +387,7 @@ def main():
         plain = line.get("aes128_ctr", {}).get("batches", {}).get("plain_per_bootstrap")
         if plain:
             best = max((v["blocks_per_s"] for k in forms.values() if k[0] in line
-                        for b, v in line[k[0]]["batches"].items() if b != "plain_per_bootstrap"), default=None)
+                        for b, v in line[k[0]]["batches"].items() if b.isdigit()), default=None)
             line["aes128_batched_over_plain"] = None if best is None else round(best / plain["blocks_per_s"], 1)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline()
