@@ -111,6 +111,21 @@ def test_keyswitch_bit_exact(ctx, okeys):
         assert np.array_equal(got[k], O.keyswitch32(okeys, lweN[k])), k
 
 
+def test_keyswitch_uneven_slices_bit_exact(ctx, okeys):
+    """1100 ciphertexts: 9 groups x 5 column tiles, so the coefficient range splits
+    into 11 slices of 94 with a short last one (84); sampled outputs equal the oracle."""
+    import torch
+    rng = np.random.default_rng(71)
+    batch = 1100
+    lweN = rng.integers(0, 2 ** 32, (batch, okeys.p.N + 1), dtype=np.uint64).astype(np.uint32)
+    d_out = torch.empty((batch, okeys.p.n + 1), dtype=torch.int32, device="cuda")
+    ctx.keyswitch_batch(to_dev(lweN), d_out)
+    torch.cuda.synchronize()
+    got = from_dev(d_out)
+    for k in (0, 127, 128, 600, 1099):
+        assert np.array_equal(got[k], O.keyswitch32(okeys, lweN[k])), k
+
+
 @pytest.mark.parametrize("gate", [O.AND, O.OR, O.NAND, O.NOR, O.XOR, O.XNOR, O.XOR_FAST, O.XNOR_FAST])
 def test_gate_eval_batch_truth_table_and_parity(ctx, okeys, gate):
     """Whole gate (rotation + key switch) on all four input pairs: decrypts to the
