@@ -445,6 +445,11 @@ def test_aes128_approx_pbs_form(actx, okeys):
     ph = (got[:, :, -1].astype(np.uint64) - (got[:, :, :-1].astype(np.uint64) @ s64)) % 2 ** 32
     err = ((ph + 2 ** 30) % 2 ** 31).astype(np.int64) - 2 ** 30
     assert np.abs(err).max() < 2 ** 29
+    d_graph = torch.empty_like(d_ks)  # the captured keystream of this form equals its eager one
+    for _ in range(2):
+        actx.aes_ctr_keystream(blocks, d_graph, graph=True)
+        torch.cuda.synchronize()
+        assert np.array_equal(from_dev(d_graph), got)
 
 
 def test_cbs_error_paths(gpu):
