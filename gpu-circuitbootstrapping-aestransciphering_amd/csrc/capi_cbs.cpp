@@ -135,6 +135,10 @@ struct tfhe_cbs_ctx {
     std::vector<AesGraph> graphs;
     hipStream_t gstream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    // second stream of tfhe_cbs_batch: the PBS of a partial last round runs there while the
+    // PFKS of the full rounds' outputs fills the idle CUs (fork / join by events)
+    hipStream_t cstream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 extern "C" int tfhe_cbs_ctx_destroy(tfhe_cbs_ctx* c) {
@@ -147,6 +151,9 @@ extern "C" int tfhe_cbs_ctx_destroy(tfhe_cbs_ctx* c) {
     if (c->ev_in) (void)hipEventDestroy(c->ev_in);
     if (c->ev_out) (void)hipEventDestroy(c->ev_out);
     if (c->gstream) (void)hipStreamDestroy(c->gstream);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->cstream) (void)hipStreamDestroy(c->cstream);
     c->tables.release();
     delete c;
     return TFHE_OK;
@@ -172,7 +179,8 @@ extern "C" int tfhe_cbs_ctx_create(const tfhe_cbs_params* params, uint32_t max_b
     rc = rc ? rc : dmalloc((void**)&c->d_bsk2f, bsk2_cplx(p) * sizeof(cplx));
     rc = rc ? rc : dmalloc((void**)&c->d_ksk2, ksk2_words(p) * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_lwe64, B * 2 * (kN + 1) * sizeof(uint64_t));
-    rc = rc ? rc : dmalloc((void**)&c->d_at, (size_t)(kN + 1) * pfks_bpad(2 * (uint32_t)B) * sizeof(uint32_t));
+    // + 256 columns: a batch split into full rounds and a tail pads each part separately
+    rc = rc ? rc : dmalloc((void**)&c->d_at, (size_t)(kN + 1) * (pfks_bpad(2 * (uint32_t)B) + 256) * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_ks_at, (size_t)kN * ks_bpad((uint32_t)B) * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_rgsw, B * 4 * 2 * kN * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_rgsw_f, B * 8 * 1024 * sizeof(cplx));
@@ -394,20 +402,51 @@ extern "C" int tfhe_cbs_pfks_batch(tfhe_cbs_ctx* c, int u, const uint64_t* d_lwe
     return check_launch("pfks");
 }
 
+// PFKS of count PBS outputs (LWE L = 2 input + j -> RGSW row 2j + u of that input,
+// at L * 4N + u * 2N): both functions in one launch over the rounded, transposed inputs
+static void cbs_pfks(tfhe_cbs_ctx* c, const uint64_t* lwe64, uint32_t count, uint32_t* at, uint32_t* rows,
+                     hipStream_t st) {
+    launch_pfks_digits(lwe64, count, at, st);
+    launch_pfks32(at, count, c->d_pfks, pfks_words_per_u(), 0, 2, rows, 4 * (size_t)kN, 2 * (size_t)kN, st);
+}
+
 extern "C" int tfhe_cbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t batch, uint32_t* d_rgsw, void* stream) {
     int rc = check_cbs(c, batch, c ? c->max_batch : 0);
     if (rc) return rc;
     if (!d_lwe || !d_rgsw) return fail(TFHE_E_PARAM, "cbs: NULL buffer");
     if (!batch) return TFHE_OK;
     hipStream_t st = (hipStream_t)stream;
-    rc = tfhe_cbs_pbs_batch(c, d_lwe, batch, c->d_lwe64, stream);
-    if (rc) return rc;
-    // LWE L = 2 input + j -> RGSW row 2j + u of that input, at L * 4N + u * 2N:
-    // both functions in one launch over the rounded, transposed inputs
-    const uint32_t count = 2 * batch;
-    launch_pfks_digits(c->d_lwe64, count, c->d_at, st);
-    launch_zero_rows32(d_rgsw, count, 4 * (size_t)kN, 4 * kN, st);
-    launch_pfks32(c->d_at, count, c->d_pfks, pfks_words_per_u(), 0, 2, d_rgsw, 4 * (size_t)kN, 2 * (size_t)kN, st);
+    const size_t lwe64_words = 2 * (size_t)(kN + 1), rgsw_words = 4 * 2 * (size_t)kN;
+    launch_zero_rows32(d_rgsw, 2 * batch, 4 * (size_t)kN, 4 * kN, st);
+    // The PBS runs in rounds of as many workgroups as fit at once (two per CU).  A partial
+    // last round leaves CUs idle, so it runs on the second stream while the full rounds'
+    // PFKS starts on this one.
+    const uint32_t slots = 2 * device_cus();
+    const uint32_t full = batch / slots * slots, tail = batch - full;
+    if (full && tail) {
+        if (!c->cstream) {
+            rc = hipcheck(hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking), "cbs stream");
+            if (!rc) rc = hipcheck(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "cbs event");
+            if (!rc) rc = hipcheck(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "cbs event");
+            if (rc) return rc;
+        }
+        rc = tfhe_cbs_pbs_batch(c, d_lwe, full, c->d_lwe64, stream);
+        if (!rc) rc = hipcheck(hipEventRecord(c->ev_fork, st), "fork");
+        if (!rc) rc = hipcheck(hipStreamWaitEvent(c->cstream, c->ev_fork, 0), "fork wait");
+        if (!rc) rc = tfhe_cbs_pbs_batch(c, d_lwe + (size_t)full * (c->p.n + 1), tail,
+                                         c->d_lwe64 + full * lwe64_words, c->cstream);
+        if (!rc) rc = hipcheck(hipEventRecord(c->ev_join, c->cstream), "join");
+        if (rc) return rc;
+        cbs_pfks(c, c->d_lwe64, 2 * full, c->d_at, d_rgsw, st);
+        rc = hipcheck(hipStreamWaitEvent(st, c->ev_join, 0), "join wait");
+        if (rc) return rc;
+        cbs_pfks(c, c->d_lwe64 + full * lwe64_words, 2 * tail,
+                 c->d_at + (size_t)(kN + 1) * pfks_bpad(2 * full), d_rgsw + full * rgsw_words, st);
+    } else {
+        rc = tfhe_cbs_pbs_batch(c, d_lwe, batch, c->d_lwe64, stream);
+        if (rc) return rc;
+        cbs_pfks(c, c->d_lwe64, 2 * batch, c->d_at, d_rgsw, st);
+    }
     return check_launch("cbs pfks");
 }
 

@@ -37,7 +37,7 @@ def okeys():
 @pytest.fixture(scope="module")
 def ctx(gpu, okeys):
     import tfhe_amd
-    c = tfhe_amd.CbsContext(max_blocks=3)  # 384 inputs: room for the large-batch PBS form
+    c = tfhe_amd.CbsContext(max_blocks=5)  # 640 inputs: the large-batch PBS form and a split batch
     c.keygen(okeys.s, SEED)
     yield c
     c.close()
@@ -168,6 +168,28 @@ def test_pfks_bit_exact(ctx, okeys):
         for j in range(count):
             assert np.array_equal(got[j, :4096], okeys.pfks(u, lwes[j])), (u, j)
         assert (got[:, 4096:] == 7).all()
+
+
+def test_cbs_split_batch_equals_parts(ctx, okeys):
+    """600 inputs are one full round of 512 PBS workgroups plus a tail of 88, which
+    runs on the context's second stream beside the first part's PFKS: the rows equal
+    two unsplit calls of 300 bit for bit."""
+    import torch
+    import tfhe_amd
+    batch = 600
+    rng = np.random.default_rng(41)
+    bits = rng.integers(0, 2, batch)
+    cts = np.stack([tfhe_amd.lwe_encrypt(okeys.s, 35, k, int(bits[k]), p=2) for k in range(batch)])
+    d_cts = to_dev32(cts)
+    d_all = torch.empty((batch, 4, 2, 2048), dtype=torch.int32, device="cuda")
+    ctx.cbs_batch(d_cts, d_all)
+    d_part = torch.empty((batch // 2, 4, 2, 2048), dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    whole = from_dev(d_all)
+    for lo in (0, batch // 2):
+        ctx.cbs_batch(d_cts[lo:lo + batch // 2].contiguous(), d_part)
+        torch.cuda.synchronize()
+        assert np.array_equal(from_dev(d_part), whole[lo:lo + batch // 2]), lo
 
 
 def test_cbs_rgsw_rows_decrypt(ctx, okeys):
@@ -303,22 +325,24 @@ def test_aes128_fips197_and_random_blocks(ctx, okeys):
 
 def test_aes_keystream_graph_replays_match_eager(ctx, okeys):
     """The hipGraph-captured keystream equals the eager launch sequence on every
-    replay, with eager work and new counter blocks in between replays."""
+    replay, with eager work and new counter blocks in between replays.  5 blocks
+    (640 inputs) split each round's PBS across the context's two streams, so the
+    capture holds the fork and join."""
     import torch
     import tfhe_amd
     key = bytes(range(100, 116))
     ctx.set_round_keys(tfhe_amd.aes_encrypt_key_bits(okeys.s, 79, key))
     rng = np.random.default_rng(8)
-    d_eager = torch.empty((2, 128, 637), dtype=torch.int32, device="cuda")
-    d_graph = torch.empty((2, 128, 637), dtype=torch.int32, device="cuda")
+    d_eager = torch.empty((5, 128, 637), dtype=torch.int32, device="cuda")
+    d_graph = torch.empty((5, 128, 637), dtype=torch.int32, device="cuda")
     for rep in range(3):
-        ctrs = [bytes(rng.integers(0, 256, 16, dtype=np.uint8)) for _ in range(2)]
+        ctrs = [bytes(rng.integers(0, 256, 16, dtype=np.uint8)) for _ in range(5)]
         ctx.aes_ctr_keystream(ctrs, d_eager)
         torch.cuda.synchronize()
         ctx.aes_ctr_keystream(ctrs, d_graph, graph=True)
         torch.cuda.synchronize()
         assert np.array_equal(from_dev(d_graph), from_dev(d_eager)), rep
-        assert _decrypt_blocks(okeys, from_dev(d_graph), 2) == [O.aes128_encrypt(key, c) for c in ctrs]
+        assert _decrypt_blocks(okeys, from_dev(d_graph), 5) == [O.aes128_encrypt(key, c) for c in ctrs]
 
 
 def test_set_keys_from_stateless_generators(ctx, okeys):
