@@ -469,6 +469,16 @@ extern "C" int tfhe_lut8_eval_batch(tfhe_cbs_ctx* c, const double* d_rgsw_f, uin
     return check_launch("lut_vertical");
 }
 
+extern "C" int tfhe_lut8x32_eval_batch(tfhe_cbs_ctx* c, const double* d_rgsw_f, uint32_t nbytes, const uint32_t* d_tables,
+                                      uint32_t* d_out32, void* stream) {
+    if (!c || !d_rgsw_f || !d_tables || !d_out32) return fail(TFHE_E_PARAM, "lut8x32: bad argument");
+    if (nbytes > 16 * c->max_blocks) return fail(TFHE_E_CAPACITY, "lut8x32: bytes exceed the context capacity");
+    if (!nbytes) return TFHE_OK;
+    launch_lut_vertical(reinterpret_cast<const cplx*>(d_rgsw_f), 4 * nbytes, d_tables, kN, 8, 8, c->tables.T, d_out32,
+                        (hipStream_t)stream, 4);
+    return check_launch("lut8x32");
+}
+
 extern "C" int tfhe_cbs_keyswitch_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe32, uint32_t count, uint32_t* d_out,
                                         void* stream) {
     if (!c || !c->has_keys || !d_lwe32 || !d_out) return fail(TFHE_E_PARAM, "cbs keyswitch: bad argument");

@@ -813,7 +813,7 @@ __device__ __forceinline__ uint32_t cb_digit_offset() {
 
 __global__ __launch_bounds__(256) void k_lut_vertical(const cplx* __restrict__ rgsw_f, uint32_t nlut,
                                                      const uint32_t* __restrict__ table, size_t table_stride,
-                                                     int stride, int out_bits, FFT1024Tables Tg,
+                                                     int stride, int out_bits, int parts, FFT1024Tables Tg,
                                                      uint32_t* __restrict__ out) {
     __shared__ cplx tl[768];
     __shared__ cplx lds_all[4][F256_LDS_CPLX];
@@ -829,7 +829,10 @@ __global__ __launch_bounds__(256) void k_lut_vertical(const cplx* __restrict__ r
     constexpr uint32_t N = CB_N;
 
     for (uint32_t lut = blockIdx.x; lut < nlut; lut += gridDim.x) {
-        const uint32_t* tab = table + lut * table_stride;
+        // parts > 1: `parts` table polynomials share one group of 8 RGSWs (an 8 -> 8 parts
+        // bit LUT); LUT lut reads group lut / parts and table polynomial lut % parts
+        const uint32_t group = parts > 1 ? lut / (uint32_t)parts : lut;
+        const uint32_t* tab = table + (size_t)(parts > 1 ? lut % (uint32_t)parts : lut) * table_stride;
         uint32_t acc[2][8];
 #pragma unroll
         for (int h = 0; h < 8; ++h) {
@@ -857,7 +860,7 @@ __global__ __launch_bounds__(256) void k_lut_vertical(const cplx* __restrict__ r
                     tp[o][h] = v - acc[o][h] + OFF;
                 }
             __syncthreads();
-            const cplx* key = rgsw_f + ((size_t)lut * 8 + bit) * CB_ROWS * 2 * CB_M;
+            const cplx* key = rgsw_f + ((size_t)group * 8 + bit) * CB_ROWS * 2 * CB_M;
             cplx macc[2][4];
 #pragma unroll
             for (int o = 0; o < 2; ++o)
@@ -955,8 +958,9 @@ __global__ void k_xor_public(uint32_t* __restrict__ ct, const uint8_t* __restric
 static unsigned grid_for(size_t work, size_t cap) { return (unsigned)(work < cap ? work : cap); }
 
 void launch_lut_vertical(const cplx* rgsw_f, uint32_t nlut, const uint32_t* table, size_t table_stride, int stride,
-                         int out_bits, const FFT1024Tables& T, uint32_t* out, hipStream_t st) {
-    k_lut_vertical<<<grid_for(nlut, 2048), 256, 0, st>>>(rgsw_f, nlut, table, table_stride, stride, out_bits, T, out);
+                         int out_bits, const FFT1024Tables& T, uint32_t* out, hipStream_t st, int parts) {
+    k_lut_vertical<<<grid_for(nlut, 2048), 256, 0, st>>>(rgsw_f, nlut, table, table_stride, stride, out_bits, parts, T,
+                                                           out);
 }
 
 void launch_aes_init(const uint32_t* rk0, const uint8_t* ctr, uint32_t blocks, int n, uint32_t* state, hipStream_t st) {

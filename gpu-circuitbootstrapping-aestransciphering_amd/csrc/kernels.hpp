@@ -56,8 +56,10 @@ void launch_pfks_digits(const uint64_t* lwe64, uint32_t B, uint32_t* at, hipStre
 void launch_zero_rows32(uint32_t* out, uint32_t count, size_t stride, uint32_t width, hipStream_t st);
 void launch_pfks32(const uint32_t* at, uint32_t B, const uint32_t* key, size_t key_u_stride, int u0, int nu,
                    uint32_t* out, size_t out_stride, size_t out_u_offset, hipStream_t st);
+// parts = 1: LUT k reads RGSW group k and table k; parts > 1: `parts` table polynomials
+// (table_stride apart) share each RGSW group (LUT k: group k / parts, table k % parts)
 void launch_lut_vertical(const cplx* rgsw_f, uint32_t nlut, const uint32_t* table, size_t table_stride, int stride,
-                         int out_bits, const FFT1024Tables& T, uint32_t* out, hipStream_t st);
+                         int out_bits, const FFT1024Tables& T, uint32_t* out, hipStream_t st, int parts = 1);
 void launch_aes_init(const uint32_t* rk0, const uint8_t* ctr, uint32_t blocks, int n, uint32_t* state, hipStream_t st);
 void launch_aes_linear(const uint32_t* in, const uint32_t* rk, const int16_t* src, int nsrc, uint32_t blocks, int n,
                        uint32_t* out, hipStream_t st);

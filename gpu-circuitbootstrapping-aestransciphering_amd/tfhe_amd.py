@@ -98,6 +98,7 @@ SIGNATURES = {
     "tfhe_cbs_pfks_batch": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_uint32, ctypes.c_size_t, _vp, _vp]),
     "tfhe_rgsw_to_fourier": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_lut8_eval_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_size_t, ctypes.c_int, _vp, _vp]),
+    "tfhe_lut8x32_eval_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp, _vp]),
     "tfhe_cbs_keyswitch_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_polymul2048_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_polymul2048_approx_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
@@ -297,6 +298,11 @@ class CbsContext:
     def lut8(self, d_rgsw_f, nlut, d_table, table_stride, out_bits, d_out32, stream=None):
         check(lib().tfhe_lut8_eval_batch(self.h, _ptr(d_rgsw_f), nlut, _ptr(d_table), table_stride, out_bits,
                                          _ptr(d_out32), _stream(stream)))
+
+    def lut8x32(self, d_rgsw_f, nbytes, d_tables, d_out32, stream=None):
+        """8 -> 32-bit LUT (T-table lookup) of nbytes bytes; d_tables [4][N] (tfhe_lut8x32_eval_batch)."""
+        check(lib().tfhe_lut8x32_eval_batch(self.h, _ptr(d_rgsw_f), nbytes, _ptr(d_tables), _ptr(d_out32),
+                                            _stream(stream)))
 
     def keyswitch_batch(self, d_lwe32, count, d_out, stream=None):
         check(lib().tfhe_cbs_keyswitch_batch(self.h, _ptr(d_lwe32), count, _ptr(d_out), _stream(stream)))

@@ -214,6 +214,14 @@ int tfhe_rgsw_to_fourier(tfhe_cbs_ctx* ctx, const uint32_t* d_rgsw, uint32_t cou
  * uint32 = the extracted coefficients. */
 int tfhe_lut8_eval_batch(tfhe_cbs_ctx* ctx, const double* d_rgsw_f, uint32_t nlut, const uint32_t* d_table,
                          size_t table_stride, int out_bits, uint32_t* d_out32, void* stream);
+/* 8 -> 32-bit LUT (the reference's T-table lookup, EvalLUT8x32_RLWE_GPU_DeviceCtrl_Batch,
+ * gpu_lut.cuh:1783-2214): byte k's 8 RGSWs (d_rgsw_f as for tfhe_lut8_eval_batch)
+ * select entry x of a 256-entry table of 32-bit words held in 4 vertical-packing
+ * polynomials d_tables [4][N] (coefficient 8x + j of polynomial p = bit 8p + j of
+ * word x, times 2^31).  Output d_out32 [nbytes][32][N+1]: the LWE of bit b of the
+ * word, b = 8p + j.  At most 16 * max_blocks bytes per call. */
+int tfhe_lut8x32_eval_batch(tfhe_cbs_ctx* ctx, const double* d_rgsw_f, uint32_t nbytes, const uint32_t* d_tables,
+                            uint32_t* d_out32, void* stream);
 
 /* N -> n key switch of [count][N+1] uint32 (32-bit torus) LWE_K2 ciphertexts
  * (reference kernel_KeySwitchU16, aes_gpu_ks.cuh:341-370). */

@@ -247,6 +247,45 @@ def test_lut_bit_exact(ctx, okeys):
             assert np.array_equal(got[k, j], okeys.extract32(acc, j)), (x, j)
 
 
+def _xtime(b):
+    return ((b << 1) ^ (0x1B if b & 0x80 else 0)) & 0xFF
+
+
+def test_lut8x32_ttable(ctx, okeys):
+    """8 -> 32-bit LUT on the AES T-table Te0 (bytes 2 S(x), S(x), S(x), 3 S(x)): four
+    table polynomials share each byte's 8 RGSWs; every output equals the oracle's
+    vertical packing of its part, and the 32 bits decrypt to Te0[x]."""
+    import torch
+    S = [O.aes_sbox(x) for x in range(256)]
+    te0 = [_xtime(S[x]) | (S[x] << 8) | (S[x] << 16) | ((_xtime(S[x]) ^ S[x]) << 24) for x in range(256)]
+    polys = np.zeros((4, 2048), np.uint32)
+    for x in range(256):
+        for b in range(32):
+            polys[b // 8, 8 * x + b % 8] = np.uint32(((te0[x] >> b) & 1) << 31)
+    xs = [0x53, 0x00, 0xA7]
+    rgsws = [okeys.rgsw_encrypt((x >> i) & 1, 3000 + 16 * k + 2 * i) for k, x in enumerate(xs) for i in range(8)]
+    d_rgsw = to_dev32(np.concatenate(rgsws))
+    d_f = torch.empty(len(rgsws) * 8 * 1024 * 2, dtype=torch.float64, device="cuda")
+    ctx.rgsw_to_fourier(d_rgsw, len(rgsws), d_f)
+    d_out = torch.empty((len(xs), 32, 2049), dtype=torch.int32, device="cuda")
+    ctx.lut8x32(d_f, len(xs), to_dev32(polys), d_out)
+    torch.cuda.synchronize()
+    got = from_dev(d_out)
+    K = okeys.K2.astype(np.int64).astype(np.uint64)
+    for k, x in enumerate(xs):
+        fs = [okeys.rgsw_fourier(r) for r in rgsws[8 * k:8 * k + 8]]
+        word = 0
+        for p in range(4):
+            acc, err = okeys.vertical_packing(fs, polys[p])
+            assert err < 0.25
+            for j in range(8):
+                lwe = got[k, 8 * p + j]
+                assert np.array_equal(lwe, okeys.extract32(acc, j)), (hex(x), p, j)
+                ph = (int(lwe[-1]) - int(np.dot(lwe[:-1].astype(np.uint64), K))) % 2 ** 32
+                word |= (((ph + 2 ** 30) >> 31) & 1) << (8 * p + j)
+        assert word == te0[x], (hex(x), hex(word), hex(te0[x]))
+
+
 def test_ks2_bit_exact(ctx, okeys):
     import torch
     rng = np.random.default_rng(9)
