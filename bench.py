@@ -110,7 +110,7 @@ def aes_kernel_rooflines(ctx, pbs_approx, nb):
                                     "peak": PEAK_LDS_TBS, "unit": "TB/s", "frac": round(lds_tbs / PEAK_LDS_TBS, 4)}}
 
 
-def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0):
+def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0, tune=True):
     """AES-128 CTR keystream: circuit bootstrap + S-box LUT per round,
     linear layers as LWE additions; every block decrypted and compared with plain AES.
     pbs_approx=1 runs the one-product FP64 PBS instead of the exact two-limb one;
@@ -136,6 +136,12 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0)
     torch.cuda.synchronize()
     for blocks in batches:
         ctrs = [nonce + int(b).to_bytes(8, "big") for b in range(blocks)]
+        tuned = None
+        if tune:  # launch shapes for this batch, timed before the timed region
+            t0 = time.perf_counter()
+            tuned = ctx.autotune(blocks)
+            tuned["tune_s"] = round(time.perf_counter() - t0, 3)
+            ctx.aes_ctr_keystream(ctrs, d_ks)  # first launch with the tuned shapes
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
@@ -152,6 +158,8 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0)
             bad += v != want
         out[str(blocks)] = {"blocks_per_s": round(blocks / dt, 3), "s_per_launch": round(dt, 4),
                             "circuit_bootstraps_per_s": round(blocks * 1280 / dt, 1), "blocks_wrong": int(bad)}
+        if tuned is not None:
+            out[str(blocks)]["tuned_shapes"] = tuned
         print(f"aes128 ctr ({'one-product' if pbs_approx else 'exact'} PBS, {pbs_multibit} key bit(s) per step): "
               f"{blocks} blocks in {dt:.3f} s, "
               f"{int(bad)} wrong", file=sys.stderr, flush=True)
@@ -208,6 +216,8 @@ def main():
     ap.add_argument("--aes-pbs", default="exact,approx",
                     help="comma-separated PBS forms of the AES runs: exact (two-limb), approx (one FP64 "
                          "product), approx-mb2 (one product, two key bits per step)")
+    ap.add_argument("--no-aes-tune", action="store_true",
+                    help="keep the kernels' model launch shapes (no tfhe_cbs_autotune before each AES batch)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -380,7 +390,8 @@ def main():
                 sys.exit(f"--aes-pbs: unknown form {f!r} (exact, approx, approx-mb2)")
             key, approx, mb = forms[f]
             line[key] = aes_ctr_bench(sizes, pbs_approx=approx, pbs_multibit=mb,
-                                      plain_blocks=args.aes_plain_blocks if f == "exact" else 0)
+                                      plain_blocks=args.aes_plain_blocks if f == "exact" else 0,
+                                      tune=not args.no_aes_tune)
             if approx:
                 line[key]["one_product_vs_exact_product"] = one_product_error()
         # batched blocks/s over the plain FP64 per-bootstrap form's

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -139,6 +140,9 @@ struct tfhe_cbs_ctx {
     // PFKS of the full rounds' outputs fills the idle CUs (fork / join by events)
     hipStream_t cstream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // launch shapes chosen by tfhe_cbs_autotune, keyed by the launch's size (absent: the
+    // kernels' own models): PBS inputs -> FFT form, PFKS LWEs -> i-slices, key switches -> i-slices
+    std::map<uint32_t, int> tune_pbs, tune_pfks, tune_ks;
 };
 
 extern "C" int tfhe_cbs_ctx_destroy(tfhe_cbs_ctx* c) {
@@ -372,6 +376,11 @@ extern "C" int tfhe_cbs_get_pfks_entry(const tfhe_cbs_ctx* c, int u, int i, int 
                     "pfks entry");
 }
 
+static int tuned(const std::map<uint32_t, int>& m, uint32_t key, int none) {
+    const auto it = m.find(key);
+    return it == m.end() ? none : it->second;
+}
+
 static int check_cbs(const tfhe_cbs_ctx* c, uint32_t batch, uint32_t cap) {
     if (!c) return fail(TFHE_E_PARAM, "cbs call: ctx is NULL");
     if (!c->has_keys) return fail(TFHE_E_STATE, "cbs call: no keys (call tfhe_cbs_keygen)");
@@ -384,7 +393,8 @@ extern "C" int tfhe_cbs_pbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32
     if (rc) return rc;
     if (!d_lwe || !d_out) return fail(TFHE_E_PARAM, "pbs: NULL buffer");
     if (!batch) return TFHE_OK;
-    launch_pbs2(d_lwe, batch, c->p.n, c->d_bsk2f, pbs_limbs(c->p), pbs_mb(c->p), c->tables.T, d_out, (hipStream_t)stream);
+    launch_pbs2(d_lwe, batch, c->p.n, c->d_bsk2f, pbs_limbs(c->p), pbs_mb(c->p), c->tables.T, d_out, (hipStream_t)stream,
+                tuned(c->tune_pbs, batch, -1));
     return check_launch("pbs2");
 }
 
@@ -402,12 +412,22 @@ extern "C" int tfhe_cbs_pfks_batch(tfhe_cbs_ctx* c, int u, const uint64_t* d_lwe
     return check_launch("pfks");
 }
 
+// The PBS runs in rounds of 2 x CUs resident workgroups: a batch with full rounds and a
+// partial last one is split there (full, tail); otherwise full = 0.
+static void cbs_split(uint32_t batch, uint32_t& full, uint32_t& tail) {
+    const uint32_t slots = 2 * device_cus();
+    full = batch / slots * slots;
+    tail = batch - full;
+    if (!full || !tail) full = 0, tail = batch;
+}
+
 // PFKS of count PBS outputs (LWE L = 2 input + j -> RGSW row 2j + u of that input,
 // at L * 4N + u * 2N): both functions in one launch over the rounded, transposed inputs
 static void cbs_pfks(tfhe_cbs_ctx* c, const uint64_t* lwe64, uint32_t count, uint32_t* at, uint32_t* rows,
                      hipStream_t st) {
     launch_pfks_digits(lwe64, count, at, st);
-    launch_pfks32(at, count, c->d_pfks, pfks_words_per_u(), 0, 2, rows, 4 * (size_t)kN, 2 * (size_t)kN, st);
+    launch_pfks32(at, count, c->d_pfks, pfks_words_per_u(), 0, 2, rows, 4 * (size_t)kN, 2 * (size_t)kN, st,
+                  tuned(c->tune_pfks, count, 0));
 }
 
 extern "C" int tfhe_cbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t batch, uint32_t* d_rgsw, void* stream) {
@@ -421,8 +441,8 @@ extern "C" int tfhe_cbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t b
     // The PBS runs in rounds of as many workgroups as fit at once (two per CU).  A partial
     // last round leaves CUs idle, so it runs on the second stream while the full rounds'
     // PFKS starts on this one.
-    const uint32_t slots = 2 * device_cus();
-    const uint32_t full = batch / slots * slots, tail = batch - full;
+    uint32_t full = 0, tail = 0;
+    cbs_split(batch, full, tail);
     if (full && tail) {
         if (!c->cstream) {
             rc = hipcheck(hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking), "cbs stream");
@@ -485,9 +505,146 @@ extern "C" int tfhe_cbs_keyswitch_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe32
     if (count > c->max_batch) return fail(TFHE_E_CAPACITY, "cbs keyswitch: count exceeds the context capacity");
     if (!count) return TFHE_OK;
     if (launch_keyswitch32(d_lwe32, count, kN, c->p.n, c->p.ks_logB, c->p.ks_l, c->d_ksk2, c->d_ks_at, d_out,
-                           (hipStream_t)stream))
+                           (hipStream_t)stream, tuned(c->tune_ks, count, 0)))
         return fail(TFHE_E_PARAM, "cbs keyswitch: unsupported shape");
     return check_launch("keyswitch32 N=2048");
+}
+
+namespace {
+// Times launch(choice) for each candidate with events on st (the first candidate is the
+// model's own choice, run once untimed to warm up) and returns the fastest.  The model's
+// choice is kept unless another is at least 2% faster, and alone when one run of it
+// takes longer than budget_ms (a large launch: its shape is whole rounds anyway).
+template <class Launch>
+int tune_one(hipStream_t st, hipEvent_t e0, hipEvent_t e1, const std::vector<int>& cand, float budget_ms,
+             Launch&& launch, int& best) {
+    auto timed = [&](int choice, int reps, float& ms) -> int {
+        ms = 1e30f;
+        for (int r = 0; r < reps; ++r) {
+            int rc = hipcheck(hipEventRecord(e0, st), "autotune event");
+            if (!rc) launch(choice);
+            if (!rc) rc = check_launch("autotune");
+            if (!rc) rc = hipcheck(hipEventRecord(e1, st), "autotune event");
+            if (!rc) rc = hipcheck(hipEventSynchronize(e1), "autotune sync");
+            float t = 0.f;
+            if (!rc) rc = hipcheck(hipEventElapsedTime(&t, e0, e1), "autotune time");
+            if (rc) return rc;
+            if (t < ms) ms = t;
+        }
+        return TFHE_OK;
+    };
+    best = cand[0];
+    launch(cand[0]);
+    float t0 = 0.f;
+    int rc = timed(cand[0], 1, t0);
+    if (rc || t0 > budget_ms) return rc;
+    const int reps = t0 < 20.f ? 3 : 1;
+    if (reps > 1 && (rc = timed(cand[0], reps, t0))) return rc;
+    float tbest = t0;
+    for (size_t k = 1; k < cand.size(); ++k) {
+        if (cand[k] == cand[0]) continue;
+        float t = 0.f;
+        if ((rc = timed(cand[k], reps, t))) return rc;
+        if (t < tbest && t < 0.98f * t0) tbest = t, best = cand[k];
+    }
+    return TFHE_OK;
+}
+}  // namespace
+
+// Times the launch shapes of one AES round over `blocks` blocks (128 circuit bootstraps
+// per block) and keeps the fastest in the context for later calls of that size: the PBS
+// FFT form of each part of tfhe_cbs_batch's split, the PFKS i-slices of each part and
+// the N=2048 key switch's i-slices.  Inputs are hash-filled context buffers; the round
+// state, RGSW and LUT buffers of the context are overwritten.  The choices change
+// launch shapes only: the exact PBS, the PFKS and the key switch give the same words.
+extern "C" int tfhe_cbs_autotune(tfhe_cbs_ctx* c, uint32_t blocks, int* choices, void* stream) {
+    if (!c || !choices) return fail(TFHE_E_PARAM, "autotune: bad argument");
+    if (!c->has_keys) return fail(TFHE_E_STATE, "autotune: no keys");
+    if (blocks > c->max_blocks) return fail(TFHE_E_CAPACITY, "autotune: blocks exceed the context capacity");
+    for (int k = 0; k < 5; ++k) choices[k] = -1;
+    if (!blocks) return TFHE_OK;
+    hipStream_t st = (hipStream_t)stream;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    int rc = hipcheck(hipStreamIsCapturing(st, &cap), "autotune capture status");
+    if (rc) return rc;
+    if (cap != hipStreamCaptureStatusNone) return fail(TFHE_E_STATE, "autotune: the stream is capturing");
+    // captured keystreams keep the shapes they were captured with: drop them so that the
+    // next graph call captures the tuned shapes
+    if (c->gstream) rc = hipcheck(hipStreamSynchronize(c->gstream), "autotune sync");
+    if (!rc && c->cstream) rc = hipcheck(hipStreamSynchronize(c->cstream), "autotune sync");
+    if (rc) return rc;
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    rc = hipcheck(hipEventCreate(&e0), "autotune event");
+    if (!rc) rc = hipcheck(hipEventCreate(&e1), "autotune event");
+    const uint32_t nb = 128 * blocks;
+    const int n = c->p.n, limbs = pbs_limbs(c->p), mb = pbs_mb(c->p);
+    const float budget_ms = 250.f;
+    if (!rc) {
+        launch_fill_hash32(c->d_state[0], (size_t)nb * (n + 1), 0x9e3779b9u, st);
+        launch_fill_hash32(c->d_lut32, (size_t)nb * (kN + 1), 0x7f4a7c15u, st);
+        rc = check_launch("autotune inputs");
+    }
+    uint32_t full = 0, tail = 0;
+    cbs_split(nb, full, tail);
+    const uint32_t parts[2] = {full, tail};
+    // 16 or more rounds of resident workgroups: the shape is whole rounds whatever the
+    // choice, and one timed launch takes seconds; the models' choices stand
+    const uint32_t many = 16 * 2 * device_cus();
+    for (int part = 0; part < 2 && !rc; ++part) {
+        const uint32_t batch = parts[part];
+        if (!batch) continue;
+        if (batch >= many) {
+            choices[part] = pbs_model_form(batch, limbs, mb);
+            choices[2 + part] = pfks_model_slices(2 * batch, 2);
+            continue;
+        }
+        const int model_form = pbs_model_form(batch, limbs, mb);
+        int form = model_form;
+        if (model_form >= 0) {
+            rc = tune_one(st, e0, e1, {model_form, 1 - model_form}, budget_ms,
+                          [&](int f) {
+                              launch_pbs2(c->d_state[0], batch, n, c->d_bsk2f, limbs, mb, c->tables.T, c->d_lwe64,
+                                          st, f);
+                          },
+                          form);
+            if (!rc) c->tune_pbs[batch] = form;
+        } else {
+            // one form: its output still feeds the PFKS timing below
+            launch_pbs2(c->d_state[0], batch, n, c->d_bsk2f, limbs, mb, c->tables.T, c->d_lwe64, st);
+            rc = check_launch("autotune pbs");
+        }
+        choices[part] = form;
+        if (rc) break;
+        const uint32_t count = 2 * batch;
+        launch_pfks_digits(c->d_lwe64, count, c->d_at, st);
+        int sl = 0;
+        rc = tune_one(st, e0, e1, {pfks_model_slices(count, 2), 4, 6, 8, 12, 16, 24, 32}, budget_ms,
+                      [&](int s) {
+                          launch_pfks32(c->d_at, count, c->d_pfks, pfks_words_per_u(), 0, 2, c->d_rgsw,
+                                        4 * (size_t)kN, 2 * (size_t)kN, st, s);
+                      },
+                      sl);
+        if (rc) break;
+        c->tune_pfks[count] = sl;
+        choices[2 + part] = sl;
+    }
+    if (!rc && nb >= many) {
+        choices[4] = ks_model_slices(nb, kN, n);
+    } else if (!rc) {
+        int sl = 0;
+        rc = tune_one(st, e0, e1, {ks_model_slices(nb, kN, n), 1, 2, 4, 6, 8, 12, 16, 24, 32, 48, 64}, budget_ms,
+                      [&](int s) {
+                          launch_keyswitch32(c->d_lut32, nb, kN, n, c->p.ks_logB, c->p.ks_l, c->d_ksk2, c->d_ks_at,
+                                             c->d_state[1], st, s);
+                      },
+                      sl);
+        if (!rc) c->tune_ks[nb] = sl, choices[4] = sl;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return rc;
 }
 
 static int polymul2048(const int32_t* d_x, const uint64_t* d_y, uint32_t batch, uint64_t* d_out, void* stream,

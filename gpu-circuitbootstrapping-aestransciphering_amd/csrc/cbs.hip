@@ -787,6 +787,20 @@ __global__ __launch_bounds__(64 * PF_WAVES) void k_pfks32(const uint32_t* __rest
 }
 
 // zero `count` rows of `width` u32 spaced `stride` apart
+// Pseudo-random words (a 32-bit integer hash of the index) for timing inputs: a blind
+// rotation over zero data would skip every step.
+__global__ void k_fill_hash32(uint32_t* __restrict__ p, size_t count, uint32_t seed) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (size_t)gridDim.x * blockDim.x) {
+        uint32_t x = (uint32_t)e * 0x9E3779B9u ^ seed ^ (uint32_t)(e >> 32);
+        x ^= x >> 16;
+        x *= 0x7FEB352Du;
+        x ^= x >> 15;
+        x *= 0x846CA68Bu;
+        x ^= x >> 16;
+        p[e] = x;
+    }
+}
+
 __global__ void k_zero_rows32(uint32_t* __restrict__ out, uint32_t count, size_t stride, uint32_t width) {
     const size_t total = (size_t)count * width;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x)
@@ -976,20 +990,32 @@ void launch_xor_public(uint32_t* ct, const uint8_t* pub, uint32_t nbits, int n, 
     k_xor_public<<<(nbits + 255) / 256, 256, 0, st>>>(ct, pub, nbits, n);
 }
 
+int pbs_model_form(uint32_t batch, int limbs, int mb) {
+    // Two limbs: the branch-free FFT form needs 267 registers (one wave per SIMD),
+    // which pays only up to 256 ciphertexts; beyond, the branchy form keeps two waves
+    // per SIMD.  One limb: the branch-free form fits two waves per SIMD (220
+    // registers) and was the faster one at every size measured.  Two key bits per
+    // step has the branch-free form only (-1: no choice).
+    if (limbs == 2) return batch <= 256 ? 1 : 0;
+    return mb == 1 ? 1 : -1;
+}
+
 void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, int limbs, int mb,
-                 const FFT1024Tables& T, uint64_t* out, hipStream_t st) {
-    // one workgroup per ciphertext.  Two limbs: the branch-free FFT form needs 267
-    // registers (one wave per SIMD), which pays only up to 256 ciphertexts; beyond,
-    // the branchy form keeps two waves per SIMD.  One limb: the branch-free form
-    // fits two waves per SIMD (220 registers) and is the faster one at every size.
+                 const FFT1024Tables& T, uint64_t* out, hipStream_t st, int form) {
+    // one workgroup per ciphertext; form: 1 = branch-free cross-wave FFT step,
+    // 0 = branchy, < 0 = pbs_model_form's choice
     const unsigned big = grid_for(batch, 2048);
+    if (form < 0) form = pbs_model_form(batch, limbs, mb);
     if (limbs == 2) {
-        if (batch <= 256)
-            k_pbs2_manylut<true, 2><<<batch, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+        if (form == 1)
+            k_pbs2_manylut<true, 2><<<batch <= 256 ? batch : big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
         else
             k_pbs2_manylut<false, 2><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
     } else if (mb == 1) {
-        k_pbs2_manylut<true, 1><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+        if (form == 0)
+            k_pbs2_manylut<false, 1><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+        else
+            k_pbs2_manylut<true, 1><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
     } else {
         k_pbs2_manylut<true, 1, 2><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
     }
@@ -1002,13 +1028,16 @@ void launch_pfks_digits(const uint64_t* lwe64, uint32_t B, uint32_t* at, hipStre
     k_pfks_digits<<<dim3((CB_N + 1 + 63) / 64, Bpad / 64), 256, 0, st>>>(lwe64, B, Bpad, at);
 }
 
+void launch_fill_hash32(uint32_t* p, size_t count, uint32_t seed, hipStream_t st) {
+    k_fill_hash32<<<grid_for((count + 255) / 256, 8192), 256, 0, st>>>(p, count, seed);
+}
+
 void launch_zero_rows32(uint32_t* out, uint32_t count, size_t stride, uint32_t width, hipStream_t st) {
     const size_t total = (size_t)count * width;
     k_zero_rows32<<<grid_for((total + 255) / 256, 8192), 256, 0, st>>>(out, count, stride, width);
 }
 
-void launch_pfks32(const uint32_t* at, uint32_t B, const uint32_t* key, size_t key_u_stride, int u0, int nu,
-                   uint32_t* out, size_t out_stride, size_t out_u_offset, hipStream_t st) {
+int pfks_model_slices(uint32_t B, int nu) {
     // i-slices (4..32): the launch runs in rounds of as many workgroups as fit at once
     // (two per CU: 92 VGPRs and 8 waves each), and a workgroup's time is its slice
     // length, so pick the slice count with the fewest rounds x coefficients per
@@ -1025,6 +1054,14 @@ void launch_pfks32(const uint32_t* at, uint32_t B, const uint32_t* key, size_t k
             slices = (int)sl;
         }
     }
+    return slices;
+}
+
+void launch_pfks32(const uint32_t* at, uint32_t B, const uint32_t* key, size_t key_u_stride, int u0, int nu,
+                   uint32_t* out, size_t out_stride, size_t out_u_offset, hipStream_t st, int slices) {
+    if (slices <= 0) slices = pfks_model_slices(B, nu);
+    if (slices > CB_N + 1) slices = CB_N + 1;
+    const unsigned tiles = 2 * CB_N / 128, groups = pfks_bpad(B) / PF_GROUP;
     dim3 grid(tiles, groups, (unsigned)(slices * nu));
     k_pfks32<<<grid, 64 * PF_WAVES, 0, st>>>(at, B, pfks_bpad(B), key, key_u_stride, u0, slices, out, out_stride,
                                               out_u_offset);

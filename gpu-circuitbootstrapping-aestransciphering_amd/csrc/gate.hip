@@ -547,17 +547,24 @@ unsigned device_cus() {
     return v;
 }
 
-int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int logB, int l, const uint32_t* ksk,
-                       uint32_t* at, uint32_t* out, hipStream_t st) {
-    if (!(logB == 2 && l == 8) || N % 64 || n + 1 > 64 * 1024) return -1;
-    const uint32_t Bpad = ks_bpad(batch);
-    const unsigned cols = (unsigned)(n + 1 + 127) / 128, groups = Bpad / KS_GROUP;
+int ks_model_slices(uint32_t batch, int N, int n) {
     // i-slices: as many workgroups as run at once (two per CU: 100 VGPRs and 8 waves
     // each), so the launch is one full round; each slice >= 32 coefficients
+    const unsigned cols = (unsigned)(n + 1 + 127) / 128, groups = ks_bpad(batch) / KS_GROUP;
     const unsigned slots = 2 * device_cus();
     unsigned slices = slots / (cols * groups);
     if (slices > (unsigned)N / 32) slices = (unsigned)N / 32;
     if (slices < 1) slices = 1;
+    return (int)slices;
+}
+
+int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int logB, int l, const uint32_t* ksk,
+                       uint32_t* at, uint32_t* out, hipStream_t st, int slices_in) {
+    if (!(logB == 2 && l == 8) || N % 64 || n + 1 > 64 * 1024) return -1;
+    const uint32_t Bpad = ks_bpad(batch);
+    const unsigned cols = (unsigned)(n + 1 + 127) / 128, groups = Bpad / KS_GROUP;
+    unsigned slices = slices_in > 0 ? (unsigned)slices_in : (unsigned)ks_model_slices(batch, N, n);
+    if (slices > (unsigned)N) slices = (unsigned)N;
     const size_t count = (size_t)batch * (n + 1);
     k_fill_zero<<<(unsigned)((count + 255) / 256 < 4096 ? (count + 255) / 256 : 4096), 256, 0, st>>>(out, count);
     k_ks_digits<16><<<dim3((unsigned)N / 64, Bpad / 64), 256, 0, st>>>(lweN, batch, N, Bpad, at);

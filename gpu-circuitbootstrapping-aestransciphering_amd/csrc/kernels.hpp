@@ -30,8 +30,10 @@ uint32_t ks_bpad(uint32_t batch);
 // compute units of the current device (cached per device), for launch shapes sized
 // to the number of workgroups that run at once
 unsigned device_cus();
+// slices <= 0: the resident-workgroup model (ks_model_slices); > 0: that many i-slices
+int ks_model_slices(uint32_t batch, int N, int n);
 int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int logB, int l, const uint32_t* ksk,
-                       uint32_t* at, uint32_t* out, hipStream_t st);
+                       uint32_t* at, uint32_t* out, hipStream_t st, int slices = 0);
 // circuit-bootstrapping level (N = 2048; PBS torus-64, PFKS / RGSW / LUT torus-32)
 struct FFT1024Tables;
 void launch_to_fourier2048(const uint64_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st);
@@ -47,15 +49,21 @@ void launch_to_fourier2048_32(const uint32_t* coef, size_t npoly, const FFT1024T
 void launch_to_fourier2048_t64(const uint64_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st);
 // limbs = 2: exact two-limb BSK2 rows (launch_to_fourier2048); 1: one torus-scaled row (launch_to_fourier2048_t64)
 // mb = 2 (limbs = 1, n even): two key bits per step with the pattern keys of launch_gen_bsk64(mb = 2)
+// form: 1 = branch-free cross-wave FFT step, 0 = branchy, < 0 = pbs_model_form's choice
+int pbs_model_form(uint32_t batch, int limbs, int mb);
 void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, int limbs, int mb,
-                 const FFT1024Tables& T, uint64_t* out, hipStream_t st);
+                 const FFT1024Tables& T, uint64_t* out, hipStream_t st, int form = -1);
 // PFKS: inputs rounded and transposed to at [N+1][pfks_bpad(B)] u32, then the key switch for
 // functions u0 .. u0+nu-1 added into zero-filled outputs (out + (u-u0)*out_u_offset, rows out_stride apart)
 uint32_t pfks_bpad(uint32_t B);
 void launch_pfks_digits(const uint64_t* lwe64, uint32_t B, uint32_t* at, hipStream_t st);
 void launch_zero_rows32(uint32_t* out, uint32_t count, size_t stride, uint32_t width, hipStream_t st);
+// pseudo-random words (an integer hash of the index) for timing inputs
+void launch_fill_hash32(uint32_t* p, size_t count, uint32_t seed, hipStream_t st);
+// slices <= 0: the resident-workgroup cost model (pfks_model_slices); > 0: that many i-slices
+int pfks_model_slices(uint32_t B, int nu);
 void launch_pfks32(const uint32_t* at, uint32_t B, const uint32_t* key, size_t key_u_stride, int u0, int nu,
-                   uint32_t* out, size_t out_stride, size_t out_u_offset, hipStream_t st);
+                   uint32_t* out, size_t out_stride, size_t out_u_offset, hipStream_t st, int slices = 0);
 // parts = 1: LUT k reads RGSW group k and table k; parts > 1: `parts` table polynomials
 // (table_stride apart) share each RGSW group (LUT k: group k / parts, table k % parts)
 void launch_lut_vertical(const cplx* rgsw_f, uint32_t nlut, const uint32_t* table, size_t table_stride, int stride,

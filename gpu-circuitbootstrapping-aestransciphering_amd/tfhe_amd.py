@@ -100,6 +100,7 @@ SIGNATURES = {
     "tfhe_lut8_eval_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_size_t, ctypes.c_int, _vp, _vp]),
     "tfhe_lut8x32_eval_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp, _vp]),
     "tfhe_cbs_keyswitch_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
+    "tfhe_cbs_autotune": (ctypes.c_int, [_vp, ctypes.c_uint32, _i32p, _vp]),
     "tfhe_polymul2048_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_polymul2048_approx_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_aes_encrypt_key_bits": (None, [ctypes.c_int, _i32p, ctypes.c_uint64,
@@ -306,6 +307,14 @@ class CbsContext:
 
     def keyswitch_batch(self, d_lwe32, count, d_out, stream=None):
         check(lib().tfhe_cbs_keyswitch_batch(self.h, _ptr(d_lwe32), count, _ptr(d_out), _stream(stream)))
+
+    def autotune(self, blocks, stream=None):
+        """Time and keep the fastest launch shapes for AES rounds of `blocks` blocks
+        (tfhe_cbs_autotune); returns the choices as a dict."""
+        ch = np.full(5, -1, np.int32)
+        check(lib().tfhe_cbs_autotune(self.h, blocks, ch, _stream(stream)))
+        return {"pbs_form": [int(ch[0]), int(ch[1])], "pfks_slices": [int(ch[2]), int(ch[3])],
+                "ks_slices": int(ch[4])}
 
     def set_round_keys(self, rk_bits):
         check(lib().tfhe_aes_set_round_keys(self.h, np.ascontiguousarray(rk_bits, np.uint32).reshape(-1)))

@@ -227,6 +227,20 @@ int tfhe_lut8x32_eval_batch(tfhe_cbs_ctx* ctx, const double* d_rgsw_f, uint32_t 
  * (reference kernel_KeySwitchU16, aes_gpu_ks.cuh:341-370). */
 int tfhe_cbs_keyswitch_batch(tfhe_cbs_ctx* ctx, const uint32_t* d_lwe32, uint32_t count, uint32_t* d_out, void* stream);
 
+/* Launch-shape autotuner (no counterpart in the reference, whose shapes are fixed at
+ * compile time): times, with events on `stream`, the candidate shapes of one AES
+ * round over `blocks` blocks and keeps the fastest in the context for later calls
+ * of those sizes.  choices[5] receives the PBS FFT form (0 branchy, 1 branch-free,
+ * -1 single form) of the full-round part and of the rest of tfhe_cbs_batch's split,
+ * the PFKS i-slices of the same two parts, and the N=2048 key switch's i-slices;
+ * -1 where a part does not exist.  A part of 16 or more rounds of resident
+ * workgroups (8192 inputs on 256 CUs), or whose model shape takes over 250 ms per
+ * launch, keeps the model shape without timing others.  Overwrites the context's AES state, RGSW and LUT
+ * buffers; drops captured keystream graphs (the next graph call recaptures).  The
+ * shapes change no output word of the exact PBS, the PFKS or the key switch.  Not
+ * during stream capture. */
+int tfhe_cbs_autotune(tfhe_cbs_ctx* ctx, uint32_t blocks, int* choices, void* stream);
+
 /* Exact negacyclic product through the 4-wave FP64 FFT, N = 2048:
  * d_out = d_x * d_y mod (X^N + 1, 2^64), small-integer d_x, torus-64 d_y. */
 int tfhe_polymul2048_batch(const int32_t* d_x, const uint64_t* d_y, uint32_t batch, uint64_t* d_out, void* stream);

@@ -436,6 +436,43 @@ def test_aes_plain_form_equals_batched(ctx, okeys):
     assert _decrypt_blocks(okeys, from_dev(d_plain), 1) == [O.aes128_encrypt(key, ctr[0])]
 
 
+def _check_choices(ch):
+    for k in range(2):
+        assert ch["pbs_form"][k] in (-1, 0, 1)
+        assert ch["pfks_slices"][k] == -1 or 4 <= ch["pfks_slices"][k] <= 32
+    assert ch["pfks_slices"][1] >= 4  # the whole batch or its tail always exists
+    assert 1 <= ch["ks_slices"] <= 64
+
+
+def test_autotune_keeps_keystream_words(ctx, okeys):
+    """tfhe_cbs_autotune changes launch shapes only: after tuning 5 blocks (a batch
+    split into a full round and a tail) and 1 block, the eager and the recaptured
+    graph keystreams equal the untuned keystream bit for bit."""
+    import torch
+    import tfhe_amd
+    key = bytes(range(30, 46))
+    ctx.set_round_keys(tfhe_amd.aes_encrypt_key_bits(okeys.s, 83, key))
+    rng = np.random.default_rng(12)
+    ctrs = [bytes(rng.integers(0, 256, 16, dtype=np.uint8)) for _ in range(5)]
+    d_ref = torch.empty((5, 128, 637), dtype=torch.int32, device="cuda")
+    ctx.aes_ctr_keystream(ctrs, d_ref, graph=True)  # captured before tuning: dropped by it
+    torch.cuda.synchronize()
+    ref = from_dev(d_ref)
+    assert _decrypt_blocks(okeys, ref, 5) == [O.aes128_encrypt(key, c) for c in ctrs]
+    for blocks in (5, 1):
+        _check_choices(ctx.autotune(blocks))
+    torch.cuda.synchronize()
+    for graph in (False, True, True):
+        d = torch.empty_like(d_ref)
+        ctx.aes_ctr_keystream(ctrs, d, graph=graph)
+        torch.cuda.synchronize()
+        assert np.array_equal(from_dev(d), ref), graph
+    d1 = torch.empty((1, 128, 637), dtype=torch.int32, device="cuda")
+    ctx.aes_ctr_keystream(ctrs[:1], d1)
+    torch.cuda.synchronize()
+    assert np.array_equal(from_dev(d1), ref[:1])
+
+
 @pytest.fixture(scope="module", params=[1, 2], ids=["one_bit", "two_bits"])
 def actx(gpu, okeys, request):
     """The same LWE key with the one-product FP64 PBS (pbs_approx = 1), blind-rotating
@@ -513,6 +550,13 @@ def test_aes128_approx_pbs_form(actx, okeys):
         actx.aes_ctr_keystream(blocks, d_graph, graph=True)
         torch.cuda.synchronize()
         assert np.array_equal(from_dev(d_graph), got)
+    # tuned shapes (the one-product PBS may change FFT form, which moves rounding
+    # but not decryption): the same blocks again decrypt to the same AES output
+    _check_choices(actx.autotune(2))
+    d_tuned = torch.empty_like(d_ks)
+    actx.aes_ctr_keystream(blocks, d_tuned)
+    torch.cuda.synchronize()
+    assert _decrypt_blocks(okeys, from_dev(d_tuned), 2) == dec
 
 
 def test_cbs_error_paths(gpu):
@@ -540,7 +584,13 @@ def test_cbs_error_paths(gpu):
         with pytest.raises(tfhe_amd.TfheError) as e:  # no round keys
             c.aes_ctr_keystream([bytes(16)], torch.zeros((1, 128, 637), dtype=torch.int32, device="cuda"))
         assert e.value.code == tfhe_amd.TFHE_E_STATE
+        with pytest.raises(tfhe_amd.TfheError) as e:  # nothing to time without keys
+            c.autotune(1)
+        assert e.value.code == tfhe_amd.TFHE_E_STATE
         c.keygen(tfhe_amd.lwe_keygen(636, 4), 4)
+        with pytest.raises(tfhe_amd.TfheError) as e:
+            c.autotune(2)
+        assert e.value.code == tfhe_amd.TFHE_E_CAPACITY
         with pytest.raises(tfhe_amd.TfheError) as e:  # 129 inputs in a 1-block (128) context
             tfhe_amd.check(L.tfhe_cbs_batch(c.h, tfhe_amd._ptr(d), 129, tfhe_amd._ptr(out), None))
         assert e.value.code == tfhe_amd.TFHE_E_CAPACITY
