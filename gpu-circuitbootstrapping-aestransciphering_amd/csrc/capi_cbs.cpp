@@ -470,6 +470,29 @@ extern "C" int tfhe_cbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t b
     return check_launch("cbs pfks");
 }
 
+extern "C" int tfhe_cbs_batch_from_device_lwe(tfhe_cbs_ctx* c, const uint64_t* d_a, const uint64_t* d_b,
+                                              uint32_t batch, uint32_t a_stride, tfhe_rgsw_batch_view* view,
+                                              void* stream) {
+    if (!c || !view) return fail(TFHE_E_PARAM, "cbs_from_device_lwe: bad argument");
+    *view = tfhe_rgsw_batch_view{};
+    if (!d_a || !d_b || batch == 0) return TFHE_OK;  // the reference returns an empty view
+    if (a_stride < (uint32_t)c->p.n) return fail(TFHE_E_PARAM, "cbs_from_device_lwe: a_stride < n");
+    int rc = check_cbs(c, batch, c->max_batch);
+    if (rc) return rc;
+    // packed inputs in the spare AES state buffer (context scratch)
+    launch_pack_lwe64(d_a, d_b, batch, c->p.n, a_stride, c->d_state[1], (hipStream_t)stream);
+    rc = check_launch("pack_lwe64");
+    if (!rc) rc = tfhe_cbs_batch(c, c->d_state[1], batch, c->d_rgsw, stream);
+    if (rc) return rc;
+    view->d_rgsw = c->d_rgsw;
+    view->batch = batch;
+    view->digits = 2;
+    view->N = kN;
+    view->per_ciphertext_elems = (size_t)2 * 4 * kN;
+    view->stream = stream;
+    return TFHE_OK;
+}
+
 extern "C" int tfhe_rgsw_to_fourier(tfhe_cbs_ctx* c, const uint32_t* d_rgsw, uint32_t count, double* d_rgsw_f,
                                     void* stream) {
     if (!c || !d_rgsw || !d_rgsw_f) return fail(TFHE_E_PARAM, "rgsw_to_fourier: bad argument");

@@ -786,7 +786,6 @@ __global__ __launch_bounds__(64 * PF_WAVES) void k_pfks32(const uint32_t* __rest
     }
 }
 
-// zero `count` rows of `width` u32 spaced `stride` apart
 // Pseudo-random words (a 32-bit integer hash of the index) for timing inputs: a blind
 // rotation over zero data would skip every step.
 __global__ void k_fill_hash32(uint32_t* __restrict__ p, size_t count, uint32_t seed) {
@@ -801,6 +800,19 @@ __global__ void k_fill_hash32(uint32_t* __restrict__ p, size_t count, uint32_t s
     }
 }
 
+// gathers LWEs given as a mask array (rows a_stride words apart) and a body array,
+// both NativeInt-wide (u64) with values on the 32-bit torus, into [batch][n+1] u32
+__global__ void k_pack_lwe64(const uint64_t* __restrict__ a, const uint64_t* __restrict__ b, uint32_t batch, int n,
+                             uint32_t a_stride, uint32_t* __restrict__ out) {
+    const size_t total = (size_t)batch * (n + 1);
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t k = e / (size_t)(n + 1);
+        const int i = (int)(e - k * (size_t)(n + 1));
+        out[e] = (uint32_t)(i < n ? a[k * a_stride + i] : b[k]);
+    }
+}
+
+// zero `count` rows of `width` u32 spaced `stride` apart
 __global__ void k_zero_rows32(uint32_t* __restrict__ out, uint32_t count, size_t stride, uint32_t width) {
     const size_t total = (size_t)count * width;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x)
@@ -1030,6 +1042,12 @@ void launch_pfks_digits(const uint64_t* lwe64, uint32_t B, uint32_t* at, hipStre
 
 void launch_fill_hash32(uint32_t* p, size_t count, uint32_t seed, hipStream_t st) {
     k_fill_hash32<<<grid_for((count + 255) / 256, 8192), 256, 0, st>>>(p, count, seed);
+}
+
+void launch_pack_lwe64(const uint64_t* a, const uint64_t* b, uint32_t batch, int n, uint32_t a_stride, uint32_t* out,
+                       hipStream_t st) {
+    const size_t total = (size_t)batch * (n + 1);
+    k_pack_lwe64<<<grid_for((total + 255) / 256, 8192), 256, 0, st>>>(a, b, batch, n, a_stride, out);
 }
 
 void launch_zero_rows32(uint32_t* out, uint32_t count, size_t stride, uint32_t width, hipStream_t st) {

@@ -60,6 +60,9 @@ void launch_pfks_digits(const uint64_t* lwe64, uint32_t B, uint32_t* at, hipStre
 void launch_zero_rows32(uint32_t* out, uint32_t count, size_t stride, uint32_t width, hipStream_t st);
 // pseudo-random words (an integer hash of the index) for timing inputs
 void launch_fill_hash32(uint32_t* p, size_t count, uint32_t seed, hipStream_t st);
+// LWEs as a u64 mask array (rows a_stride apart) and a u64 body array -> [batch][n+1] u32
+void launch_pack_lwe64(const uint64_t* a, const uint64_t* b, uint32_t batch, int n, uint32_t a_stride, uint32_t* out,
+                       hipStream_t st);
 // slices <= 0: the resident-workgroup cost model (pfks_model_slices); > 0: that many i-slices
 int pfks_model_slices(uint32_t B, int nu);
 void launch_pfks32(const uint32_t* at, uint32_t B, const uint32_t* key, size_t key_u_stride, int u0, int nu,

@@ -35,6 +35,12 @@ def gate_params():
     return GateParams(636, 1024, 7, 3, 2, 8, 15, 25)
 
 
+class RgswBatchView(ctypes.Structure):
+    """tfhe_rgsw_batch_view: borrowed circuit-bootstrap outputs held in the context."""
+    _fields_ = [("d_rgsw", ctypes.c_void_p), ("batch", ctypes.c_uint32), ("digits", ctypes.c_uint32),
+                ("N", ctypes.c_uint32), ("per_ciphertext_elems", ctypes.c_size_t), ("stream", ctypes.c_void_p)]
+
+
 class CbsParams(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("N", ctypes.c_int), ("bk_logB", ctypes.c_int), ("bk_l", ctypes.c_int),
                 ("cb_logB", ctypes.c_int), ("cb_l", ctypes.c_int), ("pf_logB", ctypes.c_int), ("pf_l", ctypes.c_int),
@@ -96,6 +102,8 @@ SIGNATURES = {
     "tfhe_cbs_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_cbs_pbs_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_cbs_pfks_batch": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_uint32, ctypes.c_size_t, _vp, _vp]),
+    "tfhe_cbs_batch_from_device_lwe": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32,
+                                                      ctypes.POINTER(RgswBatchView), _vp]),
     "tfhe_rgsw_to_fourier": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_lut8_eval_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_size_t, ctypes.c_int, _vp, _vp]),
     "tfhe_lut8x32_eval_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp, _vp]),
@@ -292,6 +300,15 @@ class CbsContext:
     def cbs_batch(self, d_lwe, d_rgsw, batch=None, stream=None):
         b = d_lwe.shape[0] if batch is None else batch
         check(lib().tfhe_cbs_batch(self.h, _ptr(d_lwe), b, _ptr(d_rgsw), _stream(stream)))
+
+    def cbs_batch_from_device_lwe(self, d_a, d_b, batch, a_stride, stream=None):
+        """Circuit bootstraps of LWEs given as u64 masks (rows a_stride apart) and u64
+        bodies; returns the RgswBatchView of the outputs held in the context."""
+        v = RgswBatchView()
+        check(lib().tfhe_cbs_batch_from_device_lwe(self.h, _ptr(d_a) if d_a is not None else None,
+                                                   _ptr(d_b) if d_b is not None else None, batch, a_stride,
+                                                   ctypes.byref(v), _stream(stream)))
+        return v
 
     def rgsw_to_fourier(self, d_rgsw, count, d_rgsw_f, stream=None):
         check(lib().tfhe_rgsw_to_fourier(self.h, _ptr(d_rgsw), count, _ptr(d_rgsw_f), _stream(stream)))

@@ -202,6 +202,31 @@ int tfhe_cbs_pbs_batch(tfhe_cbs_ctx* ctx, const uint32_t* d_lwe, uint32_t batch,
 int tfhe_cbs_pfks_batch(tfhe_cbs_ctx* ctx, int u, const uint64_t* d_lwe64, uint32_t count, size_t out_stride,
                         uint32_t* d_out, void* stream);
 
+/* Borrowed view of circuit-bootstrap outputs held in the context (the reference's
+ * DeviceRGSWBatchView, cirbts.cuh:406-418): per ciphertext digits * 2 rows of 2
+ * polynomials of N coefficients, element ((row * 2 + poly) * N + coeff).  Here the
+ * elements are torus-32 coefficients (uint32), not NTT-domain words.  Valid until
+ * the next call on the context that reuses its RGSW buffer (another circuit
+ * bootstrap into the context, an AES round or keystream, tfhe_cbs_autotune). */
+typedef struct tfhe_rgsw_batch_view {
+    const uint32_t* d_rgsw;
+    uint32_t batch;
+    uint32_t digits;              /* CB levels (2) */
+    uint32_t N;
+    size_t per_ciphertext_elems;  /* digits * 4 * N */
+    void* stream;
+} tfhe_rgsw_batch_view;
+
+/* gpu_CircuitBootstrappingBatchToDeviceFromDeviceLWE with the reference's argument
+ * layout (cirbts.cuh:466-474, cirbts-batch.cu:1416-1440): masks d_a (u64 words,
+ * ciphertext k at d_a + k * a_stride) and bodies d_b [batch] (u64), values on the
+ * 32-bit torus (the low 32 bits are used).  The outputs stay in the context; *view
+ * describes them.  As in the reference, NULL inputs or batch 0 give an empty view
+ * and no error; a_stride < n is TFHE_E_PARAM and batch above the context's
+ * capacity TFHE_E_CAPACITY. */
+int tfhe_cbs_batch_from_device_lwe(tfhe_cbs_ctx* ctx, const uint64_t* d_a, const uint64_t* d_b, uint32_t batch,
+                                   uint32_t a_stride, tfhe_rgsw_batch_view* view, void* stream);
+
 /* RGSW (torus-32 coefficients, [count][4][2][N] uint32) -> Fourier, one
  * exact FP64 transform per polynomial: d_rgsw_f [count][8][1024] complex doubles. */
 int tfhe_rgsw_to_fourier(tfhe_cbs_ctx* ctx, const uint32_t* d_rgsw, uint32_t count, double* d_rgsw_f, void* stream);

@@ -1,0 +1,159 @@
+// Drives libtfhe_amd.so through its C++ interface (include/tfhe_amd.hpp) the way the
+// reference's AES example drives its GPU contexts (bitwise/examples/aes_transcipher/main.cu):
+// key generation, single and batched gates against their truth tables, circuit
+// bootstraps from device LWEs in the reference's argument layout, and the AES-128 CTR
+// keystream against FIPS-197 C.1 (the example's self-check, main.cu:1627-1630).
+// Exit status 0 and a last line "facade ok" when every check passes.
+#include <cstdio>
+#include <cstring>
+#include <random>
+
+#include "tfhe_amd.hpp"
+
+using namespace tfhe_amd;
+
+#define EXPECT(cond, ...)                          \
+    do {                                           \
+        if (!(cond)) {                             \
+            std::fprintf(stderr, "FAIL: ");        \
+            std::fprintf(stderr, __VA_ARGS__);     \
+            std::fprintf(stderr, "\n");            \
+            return 1;                              \
+        }                                          \
+    } while (0)
+
+template <class F>
+static int expect_throw(F&& f, int code, const char* what) {
+    try {
+        f();
+    } catch (const Error& e) {
+        if (e.code != code) {
+            std::fprintf(stderr, "FAIL: %s threw code %d (%s), expected %d\n", what, e.code, e.what(), code);
+            return 1;
+        }
+        return 0;
+    }
+    std::fprintf(stderr, "FAIL: %s did not throw\n", what);
+    return 1;
+}
+
+static int gates(const std::vector<int32_t>& sk) {
+    GPUBinFHEContext cc(GateParams(), 64);
+    cc.GPUBTKeyGen(sk, 7);
+    struct G {
+        int gate;
+        int f[4];
+        const char* name;
+    } tabs[] = {{TFHE_AND, {0, 0, 0, 1}, "AND"},   {TFHE_OR, {0, 1, 1, 1}, "OR"},
+                {TFHE_NAND, {1, 1, 1, 0}, "NAND"}, {TFHE_NOR, {1, 0, 0, 0}, "NOR"},
+                {TFHE_XOR, {0, 1, 1, 0}, "XOR"},   {TFHE_XNOR, {1, 0, 0, 1}, "XNOR"}};
+    uint32_t id = 0;
+    for (const G& g : tabs)
+        for (int x = 0; x < 4; ++x) {
+            const LWECiphertext c1 = LWEEncrypt(sk, 9, id++, (uint32_t)(x >> 1));
+            const LWECiphertext c2 = LWEEncrypt(sk, 9, id++, (uint32_t)(x & 1));
+            const LWECiphertext out = cc.GPUEvalBinGate(g.gate, c1, c2);
+            EXPECT(LWEDecrypt(sk, out) == (uint32_t)g.f[x], "GPUEvalBinGate %s(%d, %d)", g.name, x >> 1, x & 1);
+        }
+    std::printf("GPUEvalBinGate: 6 gates x 4 inputs ok\n");
+
+    std::mt19937 rng(5);
+    std::vector<LWECiphertext> v1, v2;
+    std::vector<int> a, b;
+    for (int k = 0; k < 64; ++k) {
+        a.push_back((int)(rng() & 1));
+        b.push_back((int)(rng() & 1));
+        v1.push_back(LWEEncrypt(sk, 11, 2 * k, (uint32_t)a.back()));
+        v2.push_back(LWEEncrypt(sk, 11, 2 * k + 1, (uint32_t)b.back()));
+    }
+    for (int gate : {TFHE_NAND, TFHE_XOR}) {
+        const std::vector<LWECiphertext> out = cc.BatchGPUEvalBinGate(gate, v1, v2);
+        EXPECT(out.size() == 64, "BatchGPUEvalBinGate size");
+        for (int k = 0; k < 64; ++k) {
+            const int want = gate == TFHE_NAND ? 1 - (a[k] & b[k]) : a[k] ^ b[k];
+            EXPECT(LWEDecrypt(sk, out[k]) == (uint32_t)want, "BatchGPUEvalBinGate gate %d index %d", gate, k);
+        }
+    }
+    std::printf("BatchGPUEvalBinGate: 2 x 64 ok\n");
+
+    if (expect_throw([&] { (void)cc.GPUEvalBinGate(TFHE_AND, v1[0], v1[0]); }, TFHE_E_PARAM, "same ciphertext twice"))
+        return 1;
+    std::vector<LWECiphertext> big(65, v1[0]);
+    if (expect_throw([&] { (void)cc.BatchGPUEvalBinGate(TFHE_AND, big, big); }, TFHE_E_CAPACITY, "65 > 64"))
+        return 1;
+    if (expect_throw([&] { (void)cc.GPUEvalBinGate(99, v1[0], v2[0]); }, TFHE_E_PARAM, "unknown gate")) return 1;
+    std::printf("gate errors ok\n");
+    return 0;
+}
+
+static int circuit_bootstrap_and_aes(const std::vector<int32_t>& sk) {
+    GPUCirBTSContext cbs(CirBTSParams(), 128);
+    cbs.KeyGen(sk, 7);
+    const int n = cbs.params().n, N = cbs.params().N;
+
+    // the reference's device-LWE layout: u64 masks a_stride apart, u64 bodies
+    const uint32_t batch = 3, stride = (uint32_t)n + 3;
+    std::vector<LWECiphertext> cts;
+    for (uint32_t k = 0; k < batch; ++k) cts.push_back(LWEEncrypt(sk, 13, k, k & 1, 2));
+    std::vector<uint64_t> ha((size_t)batch * stride, 0), hb(batch);
+    for (uint32_t k = 0; k < batch; ++k) {
+        for (int i = 0; i < n; ++i) ha[(size_t)k * stride + i] = cts[k].a[i];
+        hb[k] = cts[k].b;
+    }
+    DeviceBuffer<uint64_t> da(ha.size()), db(hb.size());
+    da.upload(ha.data(), ha.size(), hipStreamPerThread);
+    db.upload(hb.data(), hb.size(), hipStreamPerThread);
+    const auto view =
+        cbs.gpu_CircuitBootstrappingBatchToDeviceFromDeviceLWE(da.get(), db.get(), batch, stride, hipStreamPerThread);
+    EXPECT(view.batch == batch && view.digitsCC == 2 && view.N == (uint32_t)N &&
+               view.per_ciphertext_elems == (size_t)2 * 4 * N,
+           "view fields");
+    std::vector<uint32_t> rows(batch * view.per_ciphertext_elems);
+    hip_check(hipMemcpyAsync(rows.data(), view.d_rgsw, rows.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             view.stream),
+              "view download");
+    hip_check(hipStreamSynchronize(view.stream), "view sync");
+    const auto host = cbs.gpu_CircuitBootstrappingBatch(cts);
+    for (uint32_t k = 0; k < batch; ++k)
+        EXPECT(std::memcmp(host[k].data(), rows.data() + k * view.per_ciphertext_elems,
+                           view.per_ciphertext_elems * sizeof(uint32_t)) == 0,
+               "FromDeviceLWE rows differ from gpu_CircuitBootstrappingBatch, ciphertext %u", k);
+    EXPECT(cbs.gpu_CircuitBootstrapping(cts[1]) == host[1], "gpu_CircuitBootstrapping vs batch");
+    const auto empty = cbs.gpu_CircuitBootstrappingBatchToDeviceFromDeviceLWE(nullptr, db.get(), batch, stride);
+    EXPECT(empty.batch == 0 && empty.d_rgsw == nullptr, "empty view for NULL masks");
+    if (expect_throw([&] { (void)cbs.gpu_CircuitBootstrappingBatchToDeviceFromDeviceLWE(da.get(), db.get(), batch, 5); },
+                     TFHE_E_PARAM, "a_stride < n"))
+        return 1;
+    std::vector<LWECiphertext> many(129, cts[0]);
+    if (expect_throw([&] { (void)cbs.gpu_CircuitBootstrappingBatch(many); }, TFHE_E_CAPACITY, "129 > 128")) return 1;
+    std::printf("circuit bootstrap: FromDeviceLWE view == host batch == single, errors ok\n");
+
+    // AES-128 CTR keystream of the FIPS-197 C.1 block
+    std::array<uint8_t, 16> key{}, pt{};
+    for (int i = 0; i < 16; ++i) key[i] = (uint8_t)i, pt[i] = (uint8_t)(0x11 * i);
+    cbs.SetRoundKeys(sk, 77, key);
+    const uint8_t want[16] = {0x69, 0xc4, 0xe0, 0xd8, 0x6a, 0x7b, 0x04, 0x30,
+                              0xd8, 0xcd, 0xb7, 0x80, 0x70, 0xb4, 0xc5, 0x5a};
+    for (bool graph : {false, true}) {
+        const auto ks = cbs.AesCtrKeystream({pt}, graph);
+        uint8_t got[16] = {0};
+        for (int bit = 0; bit < 128; ++bit) got[bit / 8] |= (uint8_t)(LWEDecrypt(sk, ks[0][bit], 2) << (bit % 8));
+        EXPECT(std::memcmp(got, want, 16) == 0, "AES-128 FIPS-197 C.1 through the homomorphic keystream (graph=%d)",
+               (int)graph);
+    }
+    std::printf("AES-128 CTR keystream: FIPS-197 C.1 ok (eager and graph)\n");
+    return 0;
+}
+
+int main() {
+    try {
+        const std::vector<int32_t> sk = LWEKeyGen(636, 3);
+        if (gates(sk)) return 1;
+        if (circuit_bootstrap_and_aes(sk)) return 1;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "FAIL: unexpected exception: %s\n", e.what());
+        return 1;
+    }
+    std::printf("facade ok\n");
+    return 0;
+}

@@ -192,6 +192,40 @@ def test_cbs_split_batch_equals_parts(ctx, okeys):
         assert np.array_equal(from_dev(d_part), whole[lo:lo + batch // 2]), lo
 
 
+def test_cbs_from_device_lwe_view(ctx, okeys):
+    """The reference's argument layout (u64 masks a_stride apart, u64 bodies, a
+    borrowed view of outputs held in the context) gives the same RGSW rows as
+    tfhe_cbs_batch on the packed LWEs; only the low 32 bits of each word count.
+    NULL inputs give an empty view, a_stride < n an error (cirbts-batch.cu:1423-1440)."""
+    import ctypes
+    import torch
+    import tfhe_amd
+    L = tfhe_amd.lib()
+    cts = _bits_cts(okeys, [1, 0, 1, 1, 0], 950)
+    n = cts.shape[1] - 1
+    stride = n + 7
+    a = np.full((5, stride), 0xDEADBEEF12345678, np.uint64)  # row padding the entry must skip
+    a[:, :n] = cts[:, :n].astype(np.uint64) | (np.uint64(5) << np.uint64(40))
+    b = cts[:, n].astype(np.uint64)
+    d_a, d_b = to_dev64(a), to_dev64(b)
+    v = ctx.cbs_batch_from_device_lwe(d_a, d_b, 5, stride)
+    assert (v.batch, v.digits, v.N, v.per_ciphertext_elems) == (5, 2, 2048, 2 * 4 * 2048)
+    d_f_view = torch.empty((5, 8, 1024, 2), dtype=torch.float64, device="cuda")
+    tfhe_amd.check(L.tfhe_rgsw_to_fourier(ctx.h, ctypes.c_void_p(v.d_rgsw), 5, tfhe_amd._ptr(d_f_view), None))
+    d_rgsw = torch.empty((5, 4, 2, 2048), dtype=torch.int32, device="cuda")
+    ctx.cbs_batch(to_dev32(cts), d_rgsw)
+    d_f = torch.empty_like(d_f_view)
+    ctx.rgsw_to_fourier(d_rgsw, 5, d_f)
+    torch.cuda.synchronize()
+    assert torch.equal(d_f_view, d_f)  # the exact transforms of equal rows
+    for args in ((None, d_b, 5), (d_a, None, 5), (d_a, d_b, 0)):
+        e = ctx.cbs_batch_from_device_lwe(*args, stride)
+        assert e.batch == 0 and not e.d_rgsw
+    with pytest.raises(tfhe_amd.TfheError) as err:
+        ctx.cbs_batch_from_device_lwe(d_a, d_b, 5, n - 1)
+    assert err.value.code == tfhe_amd.TFHE_E_PARAM
+
+
 def test_cbs_rgsw_rows_decrypt(ctx, okeys):
     """Circuit bootstrap of 6 bits: row 2j+u decrypts to f_u(bit * 2^(16+8j)) on
     the 32-bit torus; the first input is also checked bit for bit against
