@@ -111,8 +111,10 @@ def aes_kernel_rooflines(ctx, pbs_approx, nb):
 
 
 def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0, tune=True):
-    """AES-128 CTR keystream: circuit bootstrap + S-box LUT per round,
-    linear layers as LWE additions; every block decrypted and compared with plain AES.
+    """AES-128 CTR transciphering: the keystream (circuit bootstrap + S-box LUT per
+    round, linear layers as LWE additions) plus the public AES-CTR ciphertext bits, so
+    the outputs encrypt the plaintext bits; every block decrypted and compared with the
+    seeded random plaintext.
     pbs_approx=1 runs the one-product FP64 PBS instead of the exact two-limb one;
     pbs_multibit=2 blind-rotates two key bits per step (one-product form only)."""
     import numpy as np
@@ -131,21 +133,28 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0,
     out = {}
     d_ks = torch.empty((maxb, 128, N_LWE + 1), dtype=torch.int32, device="cuda")
     s64 = sk.astype(np.uint64)
-    warm = [nonce + int(b).to_bytes(8, "big") for b in range(min(batches))]
-    ctx.aes_ctr_keystream(warm, d_ks)  # warm-up (module load, first launches)
+    def workload(blocks):  # counters, seeded random plaintexts and their public AES-CTR ciphertexts
+        ctrs = [nonce + int(b).to_bytes(8, "big") for b in range(blocks)]
+        plain = [bytes(rng.integers(0, 256, 16, dtype=np.uint8)) for _ in range(blocks)]
+        cipher = [bytes(p ^ k for p, k in zip(pt, tfhe_amd.aes128_encrypt_plain(key, c)))
+                  for pt, c in zip(plain, ctrs)]
+        return ctrs, plain, cipher
+
+    warm = workload(min(batches))
+    ctx.aes_ctr_transcipher(warm[0], warm[2], d_ks)  # warm-up (module load, first launches)
     torch.cuda.synchronize()
     for blocks in batches:
-        ctrs = [nonce + int(b).to_bytes(8, "big") for b in range(blocks)]
+        ctrs, plain, cipher = workload(blocks)
         tuned = None
         if tune:  # launch shapes for this batch, timed before the timed region
             t0 = time.perf_counter()
             tuned = ctx.autotune(blocks)
             tuned["tune_s"] = round(time.perf_counter() - t0, 3)
-            ctx.aes_ctr_keystream(ctrs, d_ks)  # first launch with the tuned shapes
+            ctx.aes_ctr_transcipher(ctrs, cipher, d_ks)  # first launch with the tuned shapes
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
-            ctx.aes_ctr_keystream(ctrs, d_ks)
+            ctx.aes_ctr_transcipher(ctrs, cipher, d_ks)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / reps
         got = d_ks[:blocks].cpu().numpy().view(np.uint32)
@@ -153,22 +162,22 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0,
         bits = ((ph + (1 << 30)) >> 31) & 1
         bad = 0
         for b in range(blocks):
-            want = tfhe_amd.aes128_encrypt_plain(key, ctrs[b])
             v = bytes(int(sum(int(bits[b, 8 * k + j]) << j for j in range(8))) for k in range(16))
-            bad += v != want
+            bad += v != plain[b]
         out[str(blocks)] = {"blocks_per_s": round(blocks / dt, 3), "s_per_launch": round(dt, 4),
                             "circuit_bootstraps_per_s": round(blocks * 1280 / dt, 1), "blocks_wrong": int(bad)}
         if tuned is not None:
             out[str(blocks)]["tuned_shapes"] = tuned
-        print(f"aes128 ctr ({'one-product' if pbs_approx else 'exact'} PBS, {pbs_multibit} key bit(s) per step): "
+        print(f"aes128 ctr transcipher ({'one-product' if pbs_approx else 'exact'} PBS, {pbs_multibit} key bit(s) "
+              f"per step): "
               f"{blocks} blocks in {dt:.3f} s, "
               f"{int(bad)} wrong", file=sys.stderr, flush=True)
-        if blocks <= 32:  # the same keystream replayed from its captured hipGraph
+        if blocks <= 32:  # the same transcipher with the keystream replayed from its captured hipGraph
             eager = d_ks[:blocks].clone()
-            ctx.aes_ctr_keystream(ctrs, d_ks, graph=True)  # capture + first replay
+            ctx.aes_ctr_transcipher(ctrs, cipher, d_ks, mode="graph")  # capture + first replay
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            ctx.aes_ctr_keystream(ctrs, d_ks, graph=True)
+            ctx.aes_ctr_transcipher(ctrs, cipher, d_ks, mode="graph")
             torch.cuda.synchronize()
             gdt = time.perf_counter() - t0
             out[str(blocks)]["graph_s_per_launch"] = round(gdt, 4)
@@ -176,12 +185,12 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0,
     if pbs_multibit == 1:  # the round's two big kernels on their own, at 17 blocks' shape (or less)
         out["kernels"] = aes_kernel_rooflines(ctx, pbs_approx, min(17, maxb) * 128)
     if plain_blocks:  # the per-bootstrap form: every bootstrap, LUT and key switch a launch of one
-        ctrs = [nonce + int(b).to_bytes(8, "big") for b in range(plain_blocks)]
-        ctx.aes_ctr_keystream(ctrs, d_ks)
+        ctrs, plain, cipher = workload(plain_blocks)
+        ctx.aes_ctr_transcipher(ctrs, cipher, d_ks)
         torch.cuda.synchronize()
         eager = d_ks[:plain_blocks].clone()
         t0 = time.perf_counter()
-        ctx.aes_ctr_keystream(ctrs, d_ks, plain=True)
+        ctx.aes_ctr_transcipher(ctrs, cipher, d_ks, mode="plain")
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         out["plain_per_bootstrap"] = {"blocks": plain_blocks, "blocks_per_s": round(plain_blocks / dt, 4),

@@ -125,6 +125,7 @@ struct tfhe_cbs_ctx {
     int16_t* d_lin_last = nullptr;
     uint32_t* d_rk = nullptr;                   // [11][128][n+1] encrypted round-key bits
     uint8_t* d_ctr = nullptr;                   // [max_blocks][16]
+    uint8_t* d_pub = nullptr;                   // [max_blocks][16] public AES-CTR ciphertext bytes
     bool has_round_keys = false;
     // captured CTR keystreams (aes_init + 10 rounds), one per (blocks, output buffer),
     // replayed on the context's own stream, ordered after / before the caller's stream by events
@@ -148,7 +149,7 @@ struct tfhe_cbs_ctx {
 extern "C" int tfhe_cbs_ctx_destroy(tfhe_cbs_ctx* c) {
     if (!c) return TFHE_OK;
     void* bufs[] = {c->d_K2, c->d_bsk2f, c->d_pfks, c->d_ksk2, c->d_lwe64, c->d_at, c->d_ks_at, c->d_rgsw, c->d_rgsw_f, c->d_lut32,
-                    c->d_state[0], c->d_state[1], c->d_sbox_poly, c->d_lin_mix, c->d_lin_last, c->d_rk, c->d_ctr};
+                    c->d_state[0], c->d_state[1], c->d_sbox_poly, c->d_lin_mix, c->d_lin_last, c->d_rk, c->d_ctr, c->d_pub};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
@@ -196,6 +197,7 @@ extern "C" int tfhe_cbs_ctx_create(const tfhe_cbs_params* params, uint32_t max_b
     rc = rc ? rc : dmalloc((void**)&c->d_lin_last, 128 * kLinSrc * sizeof(int16_t));
     rc = rc ? rc : dmalloc((void**)&c->d_rk, (size_t)11 * 128 * (p.n + 1) * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_ctr, (size_t)max_blocks * 16);
+    rc = rc ? rc : dmalloc((void**)&c->d_pub, (size_t)max_blocks * 16);
     rc = rc ? rc : c->tables.init();
     if (!rc) {
         // S-box table packed for vertical packing: coefficient 8x + j = bit j of S(x), times 2^31
@@ -860,8 +862,24 @@ extern "C" int tfhe_aes_ctr_keystream_graph(tfhe_cbs_ctx* c, const uint8_t* h_ct
     return rc;
 }
 
+extern "C" int tfhe_aes_ctr_transcipher_batch(tfhe_cbs_ctx* c, int mode, const uint8_t* h_ctr, const uint8_t* h_cipher,
+                                              uint32_t blocks, uint32_t* d_out, void* stream) {
+    int rc = aes_ctr_check(c, h_ctr, blocks, d_out);
+    if (rc) return rc;
+    if (!h_cipher || mode < 0 || mode > 2) return fail(TFHE_E_PARAM, "aes_transcipher: bad argument");
+    if (!blocks) return TFHE_OK;
+    rc = hipcheck(hipMemcpyAsync(c->d_pub, h_cipher, (size_t)blocks * 16, hipMemcpyHostToDevice, (hipStream_t)stream),
+                  "public ciphertext");
+    if (rc) return rc;
+    rc = mode == 1   ? tfhe_aes_ctr_keystream_graph(c, h_ctr, blocks, d_out, stream)
+         : mode == 2 ? tfhe_aes_ctr_keystream_plain(c, h_ctr, blocks, d_out, stream)
+                     : tfhe_aes_ctr_keystream_batch(c, h_ctr, blocks, d_out, stream);
+    if (rc) return rc;
+    return tfhe_lwe_xor_public(d_out, c->d_pub, 128 * blocks, c->p.n, stream);
+}
+
 extern "C" int tfhe_lwe_xor_public(uint32_t* d_ct, const uint8_t* d_pub, uint32_t nbits, int n, void* stream) {
-    if (!d_ct || !d_pub) return fail(TFHE_E_PARAM, "xor_public: bad argument");
+    if (!d_ct || !d_pub || n <= 0) return fail(TFHE_E_PARAM, "xor_public: bad argument");
     if (!nbits) return TFHE_OK;
     launch_xor_public(d_ct, d_pub, nbits, n, (hipStream_t)stream);
     return check_launch("xor_public");

@@ -125,6 +125,10 @@ SIGNATURES = {
     "tfhe_aes_ctr_keystream_plain": (ctypes.c_int, [_vp, np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
                                                     ctypes.c_uint32, _vp, _vp]),
     "tfhe_lwe_xor_public": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_int, _vp]),
+    "tfhe_aes_ctr_transcipher_batch": (ctypes.c_int, [_vp, ctypes.c_int,
+                                                      np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
+                                                      np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
+                                                      ctypes.c_uint32, _vp, _vp]),
 }
 
 
@@ -347,6 +351,20 @@ class CbsContext:
         fn = (lib().tfhe_aes_ctr_keystream_graph if graph else
               lib().tfhe_aes_ctr_keystream_plain if plain else lib().tfhe_aes_ctr_keystream_batch)
         check(fn(self.h, ctr, len(ctr_blocks), _ptr(d_out), _stream(stream)))
+
+    def aes_ctr_transcipher(self, ctr_blocks, cipher_blocks, d_out, stream=None, mode="batch"):
+        """LWE bits of the plaintext of public AES-CTR ciphertext blocks: the keystream of
+        the counter blocks plus the ciphertext bits (tfhe_aes_ctr_transcipher_batch);
+        mode "batch", "graph" or "plain"."""
+        if len(cipher_blocks) != len(ctr_blocks):
+            raise ValueError("one ciphertext block per counter block")
+        ctr = np.ascontiguousarray(np.frombuffer(b"".join(ctr_blocks), np.uint8))
+        ct = np.ascontiguousarray(np.frombuffer(b"".join(cipher_blocks), np.uint8))
+        check(lib().tfhe_aes_ctr_transcipher_batch(self.h, AES_MODES[mode], ctr, ct, len(ctr_blocks), _ptr(d_out),
+                                                   _stream(stream)))
+
+
+AES_MODES = {"batch": 0, "graph": 1, "plain": 2}
 
 
 def probe_fma_tflops(fp64):

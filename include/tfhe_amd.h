@@ -322,6 +322,13 @@ int tfhe_aes_ctr_keystream_plain(tfhe_cbs_ctx* ctx, const uint8_t* h_ctr, uint32
  * the graphs). */
 int tfhe_aes_ctr_keystream_graph(tfhe_cbs_ctx* ctx, const uint8_t* h_ctr, uint32_t blocks, uint32_t* d_out,
                                  void* stream);
+/* AES-128 CTR transciphering (the CTR path of RunTranscipherBatch, main.cu:1555-1580):
+ * the keystream of the counter blocks h_ctr [blocks][16], plus the public AES-CTR
+ * ciphertext bytes h_cipher [blocks][16] (host) added as public bits, so that d_out
+ * [blocks][128][n+1] holds LWE encryptions of the plaintext bits (bit 8 byte + j).
+ * mode 0 = batched keystream, 1 = hipGraph replay, 2 = per-bootstrap form. */
+int tfhe_aes_ctr_transcipher_batch(tfhe_cbs_ctx* ctx, int mode, const uint8_t* h_ctr, const uint8_t* h_cipher,
+                                   uint32_t blocks, uint32_t* d_out, void* stream);
 /* XOR of public bits into LWE bits: d_ct[i] += (0, bit_i * 2^31), d_pub packed bits */
 int tfhe_lwe_xor_public(uint32_t* d_ct, const uint8_t* d_pub, uint32_t nbits, int n, void* stream);
 

@@ -293,6 +293,26 @@ public:
         check((graph ? tfhe_aes_ctr_keystream_graph : tfhe_aes_ctr_keystream_batch)(h_, h.data(), (uint32_t)ctr.size(),
                                                                                    d_out, s));
     }
+    /* Transciphering (the CTR path of RunTranscipherBatch, main.cu:1555-1580): LWE bits
+     * [blocks][128][n+1] of the plaintext of public AES-CTR ciphertext blocks. */
+    void TranscipherDevice(const std::vector<std::array<uint8_t, 16>>& ctr,
+                           const std::vector<std::array<uint8_t, 16>>& cipher, uint32_t* d_out, bool graph = false,
+                           hipStream_t s = hipStreamPerThread) const {
+        if (ctr.size() != cipher.size()) throw Error(TFHE_E_SHAPE, "Transcipher: one ciphertext block per counter");
+        std::vector<uint8_t> hc(ctr.size() * 16), hp(cipher.size() * 16);
+        for (size_t k = 0; k < ctr.size(); ++k)
+            for (int j = 0; j < 16; ++j) hc[16 * k + j] = ctr[k][j], hp[16 * k + j] = cipher[k][j];
+        check(tfhe_aes_ctr_transcipher_batch(h_, graph ? 1 : 0, hc.data(), hp.data(), (uint32_t)ctr.size(), d_out, s));
+    }
+    std::vector<std::vector<LWECiphertext>> Transcipher(const std::vector<std::array<uint8_t, 16>>& ctr,
+                                                        const std::vector<std::array<uint8_t, 16>>& cipher,
+                                                        bool graph = false, hipStream_t s = hipStreamPerThread) const {
+        const size_t w = (size_t)p_.n + 1, B = ctr.size();
+        DeviceBuffer<uint32_t> d(B * 128 * w);
+        TranscipherDevice(ctr, cipher, d.get(), graph, s);
+        return download_bits(d, B, s);
+    }
+
     /* The same keystream copied to the host as 128 LWE bits per block. */
     std::vector<std::vector<LWECiphertext>> AesCtrKeystream(const std::vector<std::array<uint8_t, 16>>& ctr,
                                                             bool graph = false,
@@ -300,6 +320,13 @@ public:
         const size_t w = (size_t)p_.n + 1, B = ctr.size();
         DeviceBuffer<uint32_t> d(B * 128 * w);
         AesCtrKeystreamDevice(ctr, d.get(), graph, s);
+        return download_bits(d, B, s);
+    }
+
+private:
+    std::vector<std::vector<LWECiphertext>> download_bits(const DeviceBuffer<uint32_t>& d, size_t B,
+                                                          hipStream_t s) const {
+        const size_t w = (size_t)p_.n + 1;
         std::vector<uint32_t> all(B * 128 * w);
         d.download(all.data(), all.size(), s);
         std::vector<std::vector<LWECiphertext>> out(B);

@@ -2,7 +2,8 @@
 // reference's AES example drives its GPU contexts (bitwise/examples/aes_transcipher/main.cu):
 // key generation, single and batched gates against their truth tables, circuit
 // bootstraps from device LWEs in the reference's argument layout, and the AES-128 CTR
-// keystream against FIPS-197 C.1 (the example's self-check, main.cu:1627-1630).
+// keystream against FIPS-197 C.1 (the example's self-check, main.cu:1627-1630), and
+// transciphering of public AES-CTR ciphertext back to LWE plaintext bits.
 // Exit status 0 and a last line "facade ok" when every check passes.
 #include <cstdio>
 #include <cstring>
@@ -87,7 +88,7 @@ static int gates(const std::vector<int32_t>& sk) {
 }
 
 static int circuit_bootstrap_and_aes(const std::vector<int32_t>& sk) {
-    GPUCirBTSContext cbs(CirBTSParams(), 128);
+    GPUCirBTSContext cbs(CirBTSParams(), 256);  // two AES blocks
     cbs.KeyGen(sk, 7);
     const int n = cbs.params().n, N = cbs.params().N;
 
@@ -124,8 +125,8 @@ static int circuit_bootstrap_and_aes(const std::vector<int32_t>& sk) {
     if (expect_throw([&] { (void)cbs.gpu_CircuitBootstrappingBatchToDeviceFromDeviceLWE(da.get(), db.get(), batch, 5); },
                      TFHE_E_PARAM, "a_stride < n"))
         return 1;
-    std::vector<LWECiphertext> many(129, cts[0]);
-    if (expect_throw([&] { (void)cbs.gpu_CircuitBootstrappingBatch(many); }, TFHE_E_CAPACITY, "129 > 128")) return 1;
+    std::vector<LWECiphertext> many(257, cts[0]);
+    if (expect_throw([&] { (void)cbs.gpu_CircuitBootstrappingBatch(many); }, TFHE_E_CAPACITY, "257 > 256")) return 1;
     std::printf("circuit bootstrap: FromDeviceLWE view == host batch == single, errors ok\n");
 
     // AES-128 CTR keystream of the FIPS-197 C.1 block
@@ -142,6 +143,26 @@ static int circuit_bootstrap_and_aes(const std::vector<int32_t>& sk) {
                (int)graph);
     }
     std::printf("AES-128 CTR keystream: FIPS-197 C.1 ok (eager and graph)\n");
+
+    // transciphering: public AES-CTR ciphertext of a random plaintext back to LWE plaintext bits
+    std::mt19937 rng(17);
+    std::vector<std::array<uint8_t, 16>> ctr(2), plain(2), cipher(2);
+    for (int k = 0; k < 2; ++k) {
+        for (int j = 0; j < 16; ++j) {
+            ctr[k][j] = (uint8_t)(j < 8 ? 0xA5 ^ j : (j == 15 ? k : 0));
+            plain[k][j] = (uint8_t)(rng() & 0xFF);
+        }
+        uint8_t ks[16];
+        tfhe_aes128_encrypt_plain(key.data(), ctr[k].data(), ks);
+        for (int j = 0; j < 16; ++j) cipher[k][j] = (uint8_t)(plain[k][j] ^ ks[j]);
+    }
+    const auto bits = cbs.Transcipher(ctr, cipher);
+    for (int k = 0; k < 2; ++k) {
+        uint8_t got[16] = {0};
+        for (int bit = 0; bit < 128; ++bit) got[bit / 8] |= (uint8_t)(LWEDecrypt(sk, bits[k][bit], 2) << (bit % 8));
+        EXPECT(std::memcmp(got, plain[k].data(), 16) == 0, "Transcipher block %d", k);
+    }
+    std::printf("AES-128 CTR transcipher: 2 random blocks ok\n");
     return 0;
 }
 

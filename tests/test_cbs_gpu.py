@@ -418,6 +418,34 @@ def test_aes_keystream_graph_replays_match_eager(ctx, okeys):
         assert _decrypt_blocks(okeys, from_dev(d_graph), 5) == [O.aes128_encrypt(key, c) for c in ctrs]
 
 
+def test_aes_ctr_transcipher_recovers_plaintext(ctx, okeys):
+    """Transciphering (main.cu:1555-1580): the public AES-CTR ciphertext of seeded
+    random plaintext blocks, added as public bits to the keystream, decrypts to the
+    plaintext, in the batched and graph modes; the batched mode equals the keystream
+    with the ciphertext bits added on the host."""
+    import torch
+    import tfhe_amd
+    key = bytes(range(200, 216))
+    ctx.set_round_keys(tfhe_amd.aes_encrypt_key_bits(okeys.s, 85, key))
+    rng = np.random.default_rng(14)
+    ctrs = [bytes(8) + int(b).to_bytes(8, "big") for b in range(3)]
+    plain = [bytes(rng.integers(0, 256, 16, dtype=np.uint8)) for _ in range(3)]
+    cipher = [bytes(p ^ k for p, k in zip(pt, O.aes128_encrypt(key, c))) for pt, c in zip(plain, ctrs)]
+    d_ks = torch.empty((3, 128, 637), dtype=torch.int32, device="cuda")
+    ctx.aes_ctr_keystream(ctrs, d_ks)
+    torch.cuda.synchronize()
+    ks = from_dev(d_ks).copy()
+    pub = np.array([[(cipher[b][bit // 8] >> (bit % 8)) & 1 for bit in range(128)] for b in range(3)], np.uint64)
+    ks[:, :, -1] = ((ks[:, :, -1].astype(np.uint64) + (pub << np.uint64(31))) % 2 ** 32).astype(np.uint32)
+    for mode in ("batch", "graph", "graph"):
+        d = torch.empty_like(d_ks)
+        ctx.aes_ctr_transcipher(ctrs, cipher, d, mode=mode)
+        torch.cuda.synchronize()
+        got = from_dev(d)
+        assert _decrypt_blocks(okeys, got, 3) == plain, mode
+        assert np.array_equal(got, ks), mode
+
+
 def test_set_keys_from_stateless_generators(ctx, okeys):
     """Keys made outside a context (tfhe_gen_bsk64 / tfhe_gen_pfks32 on the device,
     the oracle's KSK2 on the host) uploaded with tfhe_cbs_set_keys give the same
@@ -617,6 +645,10 @@ def test_cbs_error_paths(gpu):
         assert e.value.code == tfhe_amd.TFHE_E_STATE
         with pytest.raises(tfhe_amd.TfheError) as e:  # no round keys
             c.aes_ctr_keystream([bytes(16)], torch.zeros((1, 128, 637), dtype=torch.int32, device="cuda"))
+        assert e.value.code == tfhe_amd.TFHE_E_STATE
+        with pytest.raises(tfhe_amd.TfheError) as e:  # no round keys: nothing to transcipher
+            c.aes_ctr_transcipher([bytes(16)], [bytes(16)], torch.zeros((1, 128, 637), dtype=torch.int32,
+                                                                         device="cuda"))
         assert e.value.code == tfhe_amd.TFHE_E_STATE
         with pytest.raises(tfhe_amd.TfheError) as e:  # nothing to time without keys
             c.autotune(1)
