@@ -357,6 +357,132 @@ double orc_cbs(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, const 
     return err;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Linear-key private functional key switching on Z/2^40 (the textbook PrivKS */
+/* form, CGGI 2020 Alg. 2, with one key row per (i, t) instead of the lookup  */
+/* table of all digit multiples; the reference's configs name PFKS 2^4 x 8):  */
+/*   KL[u][i][t] = RLWE_K2(f_u(kappa_i * 2^(32 - kb + t logB)) * 2^8) on      */
+/*   Z/2^40: a = 40 low bits of Philox words, b = a * K2 + e + msg, e with    */
+/*   sigma 2^(32 - ORC_PFKL_NSHIFT) units of 2^-40 (2^-33 of the torus);      */
+/*   digits signed, d_t in [-B/2, B/2), a~_i = sum_t d_t B^t mod 2^kb;         */
+/*   out40 = - sum_{i<=N} sum_t d_t(a~_i) KL[u][i][t]  (mod 2^40), and the    */
+/*   torus-32 output is out32 = round(out40 / 2^8).                            */
+/* Noise: the sum has variance ~16392 * E[d^2] * sigma^2 = 16392 * 21.5 *     */
+/* 2^-66, i.e. sigma 2^-23.8 of the torus, below the lookup form's 2^-23.0.   */
+/* ------------------------------------------------------------------------- */
+#define PFKL_MASK ((((uint64_t)1) << ORC_PFKL_BITS) - 1)
+
+static uint32_t pfkl_id(const orc_cbs_params* p, int u, int i, int t) {
+    return (uint32_t)(((size_t)u * (p->N + 1) + i) * p->pf_l + t);
+}
+
+/* 40-bit message scalar of entry (i, t): kappa_i * 2^(32 - kb + t logB) on the
+ * 32-bit torus, times 2^8 */
+static uint64_t pfkl_cval(const orc_cbs_params* p, const int32_t* K2, int i, int t) {
+    int kb = p->pf_l * p->pf_logB;
+    uint32_t kappa = i < p->N ? (uint32_t)K2[i] : 0xFFFFFFFFu;
+    uint32_t c32 = kappa * ((uint32_t)1 << (32 - kb + t * p->pf_logB));
+    return ((uint64_t)c32 << 8) & PFKL_MASK;
+}
+
+/* exact negacyclic a * K mod 2^40 for binary K (schoolbook) */
+static void nc_mul_binary40(const int32_t* K, const uint64_t* a, uint64_t* out, int N) {
+    for (int c = 0; c < N; ++c) {
+        uint64_t acc = 0;
+        for (int j = 0; j < N; ++j) {
+            if (!K[j]) continue;
+            acc += j <= c ? a[c - j] : (0 - a[N + c - j]);
+        }
+        out[c] = acc & PFKL_MASK;
+    }
+}
+
+void orc_gen_pfkl_entry(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, int u, int i, int t,
+                        uint64_t* ct40) {
+    int N = p->N;
+    uint32_t id = pfkl_id(p, u, i, t);
+    uint64_t cval = pfkl_cval(p, K2, i, t);
+    uint64_t* a = ct40;
+    uint64_t* b = ct40 + N;
+    for (int j = 0; j < N; ++j) a[j] = orc_rand64(seed, ORC_DOM_PFKL, id, (uint32_t)j) & PFKL_MASK;
+    nc_mul_binary40(K2, a, b, N);
+    for (int j = 0; j < N; ++j) {
+        uint64_t m = u == 1 ? (j == 0 ? cval : 0) : (0 - (uint64_t)K2[j] * cval);
+        b[j] = (b[j] + (uint64_t)orc_noise(seed, ORC_DOM_PFKL, id, (uint32_t)j, ORC_PFKL_NSHIFT) + m) & PFKL_MASK;
+    }
+}
+
+/* The sum restated by linearity, as orc_pfks does: (A, A * K2 + E + msg(C)) with
+ * A = sum d a_e, E = sum d e_e, C = sum d cval_e, all mod 2^40. */
+void orc_pfks_linear(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, int u, const uint64_t* lwe,
+                     uint32_t* out) {
+    int N = p->N, B = 1 << p->pf_logB, kb = p->pf_l * p->pf_logB;
+    uint64_t kmask = kb == 64 ? ~(uint64_t)0 : (((uint64_t)1 << kb) - 1);
+    uint64_t off = 0;
+    for (int t = 0; t < p->pf_l; ++t) off += (uint64_t)(B / 2) << (t * p->pf_logB);
+    uint64_t* A = (uint64_t*)calloc((size_t)N, sizeof(uint64_t));
+    uint64_t* E = (uint64_t*)calloc((size_t)N, sizeof(uint64_t));
+    uint64_t C = 0;
+#pragma omp parallel
+    {
+        uint64_t* al = (uint64_t*)calloc((size_t)N, sizeof(uint64_t));
+        uint64_t* el = (uint64_t*)calloc((size_t)N, sizeof(uint64_t));
+        uint64_t cl = 0;
+#pragma omp for schedule(static)
+        for (int i = 0; i <= N; ++i) {
+            uint64_t at = (lwe[i] + ((uint64_t)1 << (63 - kb))) >> (64 - kb);
+            uint64_t w = (at + off) & kmask;
+            for (int t = 0; t < p->pf_l; ++t) {
+                int64_t d = (int64_t)((w >> (t * p->pf_logB)) & (uint64_t)(B - 1)) - B / 2;
+                if (!d) continue;
+                uint32_t id = pfkl_id(p, u, i, t);
+                for (int j = 0; j < N; ++j) {
+                    al[j] += (uint64_t)d * (orc_rand64(seed, ORC_DOM_PFKL, id, (uint32_t)j) & PFKL_MASK);
+                    el[j] += (uint64_t)d * (uint64_t)orc_noise(seed, ORC_DOM_PFKL, id, (uint32_t)j, ORC_PFKL_NSHIFT);
+                }
+                cl += (uint64_t)d * pfkl_cval(p, K2, i, t);
+            }
+        }
+#pragma omp critical
+        {
+            for (int j = 0; j < N; ++j) {
+                A[j] += al[j];
+                E[j] += el[j];
+            }
+            C += cl;
+        }
+        free(al);
+        free(el);
+    }
+    for (int j = 0; j < N; ++j) A[j] &= PFKL_MASK;
+    C &= PFKL_MASK;
+    uint64_t* b = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    nc_mul_binary40(K2, A, b, N);
+    for (int j = 0; j < N; ++j) {
+        uint64_t m = u == 1 ? (j == 0 ? C : 0) : (0 - (uint64_t)K2[j] * C);
+        uint64_t oa = (0 - A[j]) & PFKL_MASK;
+        uint64_t ob = (0 - (b[j] + E[j] + m)) & PFKL_MASK;
+        out[j] = (uint32_t)((oa + 128) >> 8);
+        out[N + j] = (uint32_t)((ob + 128) >> 8);
+    }
+    free(b);
+    free(A);
+    free(E);
+}
+
+/* orc_cbs with the linear-key PFKS */
+double orc_cbs_linear(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, const double* bsk_f,
+                      const uint32_t* ct32, uint32_t* rgsw) {
+    int N = p->N;
+    uint64_t* lwes = (uint64_t*)malloc(sizeof(uint64_t) * 2 * (N + 1));
+    double err = orc_pbs_manylut(p, bsk_f, ct32, lwes);
+    for (int j = 0; j < p->cb_l; ++j)
+        for (int u = 0; u < 2; ++u)
+            orc_pfks_linear(seed, p, K2, u, lwes + (size_t)j * (N + 1), rgsw + (size_t)(2 * j + u) * 2 * N);
+    free(lwes);
+    return err;
+}
+
 /* Fourier RGSW (torus-32, one limb): [2 cb_l rows][2 out][re M | im M]. */
 void orc_rgsw_to_fourier32(int N, int rows, const uint32_t* rgsw, double* rgsw_f) {
     int M = N / 2;

@@ -267,6 +267,10 @@ def _cbs_lib():
                                           ctypes.c_int, _u32p]),
             "orc_pfks": (None, [ctypes.c_uint64, P, _i32p, ctypes.c_int, _u64p, _u32p]),
             "orc_cbs": (ctypes.c_double, [ctypes.c_uint64, P, _i32p, _f64p, _u32p, _u32p]),
+            "orc_gen_pfkl_entry": (None, [ctypes.c_uint64, P, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          _u64p]),
+            "orc_pfks_linear": (None, [ctypes.c_uint64, P, _i32p, ctypes.c_int, _u64p, _u32p]),
+            "orc_cbs_linear": (ctypes.c_double, [ctypes.c_uint64, P, _i32p, _f64p, _u32p, _u32p]),
             "orc_rgsw_to_fourier32": (None, [ctypes.c_int, ctypes.c_int, _u32p, _f64p]),
             "orc_cmux32": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p, _u32p, _u32p, _u32p]),
             "orc_vertical_packing": (ctypes.c_double, [P, ctypes.c_int, _f64p, _u32p, _u32p]),
@@ -386,11 +390,26 @@ class CbsKeys:
         _cbs_lib().orc_gen_pfks_entry(self.seed, ctypes.byref(p), self.K2, u, i, t, v, out)
         return out
 
-    def cbs(self, ct32):
+    def pfks_linear(self, u, lwe64):
+        """Torus-32 RLWE_K2 of f_u(phase of lwe64) through the linear 40-bit key."""
+        p = self.p
+        out = np.zeros(2 * p.N, np.uint32)
+        _cbs_lib().orc_pfks_linear(self.seed, ctypes.byref(p), self.K2, u, np.ascontiguousarray(lwe64, np.uint64),
+                                   out)
+        return out
+
+    def pfkl_entry(self, u, i, t):
+        """Linear-key entry (u, i, t): [2][N] values mod 2^40."""
+        p = self.p
+        out = np.zeros(2 * p.N, np.uint64)
+        _cbs_lib().orc_gen_pfkl_entry(self.seed, ctypes.byref(p), self.K2, u, i, t, out)
+        return out
+
+    def cbs(self, ct32, linear=False):
         p = self.p
         rgsw = np.zeros(2 * p.cb_l * 2 * p.N, np.uint32)
-        err = _cbs_lib().orc_cbs(self.seed, ctypes.byref(p), self.K2, self.bsk_f,
-                                 np.ascontiguousarray(ct32, np.uint32), rgsw)
+        fn = _cbs_lib().orc_cbs_linear if linear else _cbs_lib().orc_cbs
+        err = fn(self.seed, ctypes.byref(p), self.K2, self.bsk_f, np.ascontiguousarray(ct32, np.uint32), rgsw)
         return rgsw, err
 
     def rgsw_encrypt(self, m, id0, logB=None, l=None):

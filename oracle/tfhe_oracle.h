@@ -32,7 +32,7 @@ int32_t orc_secret_bit(uint64_t seed, uint32_t dom, uint32_t i);
 enum {
     ORC_DOM_LWE_SK = 1, ORC_DOM_RLWE1_SK = 2, ORC_DOM_RLWE2_SK = 3,
     ORC_DOM_BSK1 = 4, ORC_DOM_KSK1 = 5, ORC_DOM_BSK2 = 6, ORC_DOM_PFKS = 7,
-    ORC_DOM_KSK2 = 8, ORC_DOM_ENC = 16
+    ORC_DOM_KSK2 = 8, ORC_DOM_PFKL = 10, ORC_DOM_ENC = 16
 };
 
 /* ---- parameter sets ---- */
@@ -119,6 +119,14 @@ void orc_gen_pfks_entry(uint64_t seed, const orc_cbs_params* p, const int32_t* K
 void orc_pfks(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, int u, const uint64_t* lwe, uint32_t* out);
 double orc_cbs(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, const double* bsk_f,
                const uint32_t* ct32, uint32_t* rgsw);
+/* linear-key PFKS on Z/2^40 (one key row per (i, t), signed digits), see cbs_oracle.c */
+enum { ORC_PFKL_BITS = 40, ORC_PFKL_NSHIFT = 25 };
+void orc_gen_pfkl_entry(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, int u, int i, int t,
+                        uint64_t* ct40);
+void orc_pfks_linear(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, int u, const uint64_t* lwe,
+                     uint32_t* out);
+double orc_cbs_linear(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, const double* bsk_f,
+                      const uint32_t* ct32, uint32_t* rgsw);
 void orc_rgsw_to_fourier32(int N, int rows, const uint32_t* rgsw, double* rgsw_f);
 double orc_cmux32(int N, int logB, int l, const double* rgsw_f, const uint32_t* d0, const uint32_t* d1,
                   uint32_t* out);

@@ -163,6 +163,82 @@ def test_pfks_sum_equals_entry_by_entry(keys):
         assert np.array_equal(keys.pfks(u, l2), want), u
 
 
+M40 = (1 << 40) - 1
+
+
+def _nc_mul_binary_u64(K, a):
+    """Exact negacyclic a * K mod 2^64 for binary K (wrapping uint64 adds)."""
+    N = len(K)
+    out = np.zeros(N, np.uint64)
+    for j in np.nonzero(K)[0]:
+        r = np.roll(a, j)
+        r[:j] = (np.uint64(0) - r[:j])
+        out += r
+    return out
+
+
+def test_pfks_linear_entries_and_functions(keys):
+    """Linear-key entries (Z/2^40) decrypt to f_u(kappa_i 2^(4t)) 2^8 with sigma 2^7
+    units (2^-33 of the torus); the switched RLWE decrypts to f_u(phase) of the
+    torus-64 input on the 32-bit torus."""
+    p = keys.p
+    N = p.N
+    for (u, i, t) in [(1, 0, 0), (1, N, 7), (0, 5, 3), (0, N, 0)]:
+        ct = keys.pfkl_entry(u, i, t)
+        a, b = ct[:N], ct[N:]
+        assert int(a.max()) <= M40 and int(b.max()) <= M40
+        kappa = int(keys.K2[i]) if i < N else -1
+        c = (((kappa * 2 ** (4 * t)) % 2 ** 32) << 8) & M40
+        m = np.zeros(N, np.uint64)
+        m[0] = c
+        want = m if u == 1 else (np.uint64(0) - _nc_mul_binary_u64(keys.K2, m)) & np.uint64(M40)
+        ph = (b - _nc_mul_binary_u64(keys.K2, a)) & np.uint64(M40)
+        err = ((ph - want) & np.uint64(M40)).astype(np.int64)
+        err = np.where(err >= 2 ** 39, err - 2 ** 40, err)
+        assert np.abs(err).max() <= 7 * 2 ** 7  # Irwin-Hall tail is +-6 sigma
+        assert np.abs(err).max() > 2 ** 5
+    rng = np.random.default_rng(4)
+    a = rng.integers(0, 2 ** 64, N, dtype=np.uint64)
+    mu = 2 ** 56
+    b = (int(np.dot(a, keys.K2.astype(np.uint64))) + mu + 999) % 2 ** 64
+    lwe = np.concatenate([a, np.array([b], np.uint64)])
+    for u in (0, 1):
+        r = keys.pfks_linear(u, lwe)
+        ph = O.rlwe_phase32(keys.K2, r)
+        m = np.zeros(N, np.uint32)
+        m[0] = mu >> 32
+        want = m if u == 1 else (-O.nc_mul_exact32(keys.K2, m)).astype(np.uint32)
+        err = ((ph - want).astype(np.uint32)).view(np.int32)
+        # key noise sqrt(16392 * 21.5) * 2^-33 ~ 2^8.2 units plus the input rounding
+        # (half a unit per coefficient, ~2^9 over 2049 coefficients)
+        assert np.abs(err).max() < 2 ** 13
+
+
+def test_pfks_linear_sum_equals_entry_by_entry(keys):
+    """orc_pfks_linear sums d * entry by linearity (one mask product mod 2^40); on a
+    sparse input (zero coefficients give all-zero signed digits) it equals
+    -sum_t d_t KL[i][t] taken entry by entry, rounded to 32 bits, for both functions,
+    including the b row (i = N) and words whose top digit carries out."""
+    N = keys.p.N
+    l2 = np.zeros(N + 1, np.uint64)
+    l2[3] = np.uint64(0x5 << 32)
+    l2[N] = np.uint64(0xA3 << 32)
+    l2[700] = np.uint64((0xF8000009 << 32) - (1 << 31))  # rounds up to 0xF8000009
+    for u in (0, 1):
+        acc = np.zeros(2 * N, np.uint64)
+        for i in (3, N, 700):
+            at = ((int(l2[i]) + (1 << 31)) >> 32) & 0xFFFFFFFF
+            w = (at + 0x88888888) & 0xFFFFFFFF
+            digits = [((w >> (4 * t)) & 15) - 8 for t in range(8)]
+            assert sum(d * 16 ** t for t, d in enumerate(digits)) % 2 ** 32 == at
+            for t, d in enumerate(digits):
+                if d:
+                    acc = acc - np.uint64(d % 2 ** 64) * keys.pfkl_entry(u, i, t)
+        out40 = acc & np.uint64(M40)
+        want = (((out40 + np.uint64(128)) >> np.uint64(8)) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+        assert np.array_equal(keys.pfks_linear(u, l2), want), u
+
+
 @pytest.mark.parametrize("m", [0, 1])
 def test_cmux_selects(keys, m):
     p = keys.p
