@@ -28,6 +28,7 @@ F_PBS = N_LWE * F_EP  # 148.49 MFLOP per blind rotation
 BSK_BYTES = N_LWE * 2 * BK_L * 2 * M_FFT * 16  # Fourier BSK streamed once per pass (62.5 MB)
 PEAK_FP64_TFLOPS = 78.6
 PEAK_LDS_TBS = 78.6  # 256 CUs x 128 B/clk x 2.4 GHz
+PEAK_INT8_TOPS = 5000.0  # dense int8 matrix-core peak
 # circuit-bootstrapping level: N = 2048 (M = 1024), PBS gadget 2^9 x 4 (8 digit rows)
 N_CB, M_CB, CB_ROWS = 2048, 1024, 8
 F_EP_CB = {1: (CB_ROWS + 2) * 5 * M_CB * 10 + 8 * 2 * CB_ROWS * M_CB,      # one product: 643,072
@@ -75,10 +76,12 @@ def one_product_error(batch=256):
             "rms_error_log2_of_torus": round(float(np.log2(np.sqrt(np.mean(d.astype(np.float64) ** 2)))) - 64, 2)}
 
 
-def aes_kernel_rooflines(ctx, pbs_approx, nb):
+def aes_kernel_rooflines(ctx, pbs_approx, nb, pfks_linear=0):
     """PBS and PFKS of one AES round's shape timed on their own (HIP events): the PBS
-    against the FP64 peak by its algorithmic FLOPs, the PFKS against the LDS
-    bandwidth by the bytes its selects read (4 per output word per digit)."""
+    against the FP64 peak by its algorithmic FLOPs; the lookup PFKS against the LDS
+    bandwidth by the bytes its selects read (4 per output word per digit), the linear
+    PFKS against the dense int8 matrix-core peak by its products (inputs x 16392
+    digits x 2N columns x 5 key bytes per function, two operations each)."""
     import numpy as np
     import torch
     rng = np.random.default_rng(13)
@@ -101,16 +104,25 @@ def aes_kernel_rooflines(ctx, pbs_approx, nb):
     t_pfks = timed(lambda: [ctx.pfks_batch(u, pbs_out, 2 * nb, 2 * N_CB, rows) for u in (0, 1)])
     limbs = 1 if pbs_approx else 2
     tflops = nb * N_LWE * F_EP_CB[limbs] / (t_pbs * 1e-3) / 1e12
-    lds_bytes = 2 * (2 * nb) * (N_CB + 1) * 8 * (2 * N_CB) * 4  # functions x inputs x digits x words x 4 B
-    lds_tbs = lds_bytes / (t_pfks * 1e-3) / 1e12
-    return {"inputs": nb,
-            "pbs": {"ms": round(t_pbs, 3), "bound": "fp64", "achieved": round(tflops, 2), "peak": PEAK_FP64_TFLOPS,
-                    "unit": "TFLOP/s", "frac": round(tflops / PEAK_FP64_TFLOPS, 4)},
-            "pfks_both_functions": {"ms": round(t_pfks, 3), "bound": "lds", "achieved": round(lds_tbs, 2),
-                                    "peak": PEAK_LDS_TBS, "unit": "TB/s", "frac": round(lds_tbs / PEAK_LDS_TBS, 4)}}
+    out = {"inputs": nb,
+           "pbs": {"ms": round(t_pbs, 3), "bound": "fp64", "achieved": round(tflops, 2), "peak": PEAK_FP64_TFLOPS,
+                   "unit": "TFLOP/s", "frac": round(tflops / PEAK_FP64_TFLOPS, 4)}}
+    if pfks_linear:
+        ops = 2 * 2 * (2 * nb) * (N_CB + 1) * 8 * (2 * N_CB) * 5  # 2 ops x functions x inputs x digits x cols x bytes
+        tops = ops / (t_pfks * 1e-3) / 1e12
+        out["pfks_both_functions"] = {"ms": round(t_pfks, 3), "form": "linear key, int8 GEMM", "bound": "mfma",
+                                      "achieved": round(tops, 1), "peak": PEAK_INT8_TOPS, "unit": "TOP/s",
+                                      "frac": round(tops / PEAK_INT8_TOPS, 4)}
+    else:
+        lds_bytes = 2 * (2 * nb) * (N_CB + 1) * 8 * (2 * N_CB) * 4  # functions x inputs x digits x words x 4 B
+        lds_tbs = lds_bytes / (t_pfks * 1e-3) / 1e12
+        out["pfks_both_functions"] = {"ms": round(t_pfks, 3), "form": "lookup key", "bound": "lds",
+                                      "achieved": round(lds_tbs, 2), "peak": PEAK_LDS_TBS, "unit": "TB/s",
+                                      "frac": round(lds_tbs / PEAK_LDS_TBS, 4)}
+    return out
 
 
-def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0, tune=True):
+def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0, tune=True, pfks_linear=0):
     """AES-128 CTR transciphering: the keystream (circuit bootstrap + S-box LUT per
     round, linear layers as LWE additions) plus the public AES-CTR ciphertext bits, so
     the outputs encrypt the plaintext bits; every block decrypted and compared with the
@@ -121,7 +133,7 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0,
     import torch
     import tfhe_amd
     maxb = max(batches)
-    ctx = tfhe_amd.CbsContext(max_blocks=maxb, params=tfhe_amd.cbs_params(pbs_approx, pbs_multibit))
+    ctx = tfhe_amd.CbsContext(max_blocks=maxb, params=tfhe_amd.cbs_params(pbs_approx, pbs_multibit, pfks_linear))
     sk = tfhe_amd.lwe_keygen(N_LWE, 3)
     t0 = time.perf_counter()
     ctx.keygen(sk, 3)
@@ -183,7 +195,7 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0,
             out[str(blocks)]["graph_s_per_launch"] = round(gdt, 4)
             out[str(blocks)]["graph_equals_eager"] = bool(torch.equal(d_ks[:blocks], eager))
     if pbs_multibit == 1:  # the round's two big kernels on their own, at 17 blocks' shape (or less)
-        out["kernels"] = aes_kernel_rooflines(ctx, pbs_approx, min(17, maxb) * 128)
+        out["kernels"] = aes_kernel_rooflines(ctx, pbs_approx, min(17, maxb) * 128, pfks_linear)
     if plain_blocks:  # the per-bootstrap form: every bootstrap, LUT and key switch a launch of one
         ctrs, plain, cipher = workload(plain_blocks)
         ctx.aes_ctr_transcipher(ctrs, cipher, d_ks)
@@ -201,7 +213,9 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0,
     ctx.close()
     pbs = ("one FP64 product per digit polynomial, rounding error ~2^-33" if pbs_approx
            else "2-limb exact") + (", two key bits per blind-rotation step" if pbs_multibit == 2 else "")
-    return {"form": f"fp64 FFT: PBS on the 64-bit torus ({pbs}), PFKS / CMux LUT on the 32-bit torus "
+    pf = ("linear 40-bit key as an int8 GEMM on the matrix cores" if pfks_linear
+          else "lookup key, LDS selects")
+    return {"form": f"fp64 FFT: PBS on the 64-bit torus ({pbs}), PFKS ({pf}) and CMux LUT on the 32-bit torus "
                     "(1-limb exact)", "keygen_s": round(keygen_s, 2), "batches": out}
 
 
@@ -224,7 +238,8 @@ def main():
                     help="AES blocks timed in the plain per-bootstrap form (exact PBS; 0 to skip)")
     ap.add_argument("--aes-pbs", default="exact,approx",
                     help="comma-separated PBS forms of the AES runs: exact (two-limb), approx (one FP64 "
-                         "product), approx-mb2 (one product, two key bits per step)")
+                         "product), approx-mb2 (one product, two key bits per step); exact-lin / approx-lin "
+                         "with the linear-key PFKS on the matrix cores")
     ap.add_argument("--no-aes-tune", action="store_true",
                     help="keep the kernels' model launch shapes (no tfhe_cbs_autotune before each AES batch)")
     args = ap.parse_args()
@@ -392,15 +407,17 @@ def main():
                                    "fp32_unpacked": round(tfhe_amd.probe_fma_tflops(0), 2)}
     if args.aes_blocks:
         sizes = [int(b) for b in args.aes_blocks.split(",") if b]
-        forms = {"exact": ("aes128_ctr", 0, 1), "approx": ("aes128_ctr_one_product_pbs", 1, 1),
-                 "approx-mb2": ("aes128_ctr_one_product_pbs_2bit", 1, 2)}
+        forms = {"exact": ("aes128_ctr", 0, 1, 0), "approx": ("aes128_ctr_one_product_pbs", 1, 1, 0),
+                 "approx-mb2": ("aes128_ctr_one_product_pbs_2bit", 1, 2, 0),
+                 "exact-lin": ("aes128_ctr_linear_pfks", 0, 1, 1),
+                 "approx-lin": ("aes128_ctr_one_product_pbs_linear_pfks", 1, 1, 1)}
         for f in [f for f in args.aes_pbs.split(",") if f]:
             if f not in forms:
-                sys.exit(f"--aes-pbs: unknown form {f!r} (exact, approx, approx-mb2)")
-            key, approx, mb = forms[f]
+                sys.exit(f"--aes-pbs: unknown form {f!r} (exact, approx, approx-mb2, exact-lin, approx-lin)")
+            key, approx, mb, lin = forms[f]
             line[key] = aes_ctr_bench(sizes, pbs_approx=approx, pbs_multibit=mb,
                                       plain_blocks=args.aes_plain_blocks if f == "exact" else 0,
-                                      tune=not args.no_aes_tune)
+                                      tune=not args.no_aes_tune, pfks_linear=lin)
             if approx:
                 line[key]["one_product_vs_exact_product"] = one_product_error()
         # batched blocks/s over the plain FP64 per-bootstrap form's

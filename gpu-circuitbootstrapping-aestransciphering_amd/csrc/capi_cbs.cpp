@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -21,7 +22,8 @@ bool cbs_params_supported(const tfhe_cbs_params* p) {
            p->pf_l == 8 && p->ks_logB == 2 && p->ks_l == 8 && p->n > 0 && p->n <= 1024 && p->lwe_nshift >= 1 &&
            p->lwe_nshift <= 31 && p->rlwe_nshift >= -31 && p->rlwe_nshift <= 31 && p->pf_nshift >= 1 &&
            p->pf_nshift <= 31 && (p->pbs_approx == 0 || p->pbs_approx == 1) && p->pbs_multibit >= 0 &&
-           p->pbs_multibit <= 2 && (p->pbs_multibit < 2 || (p->pbs_approx == 1 && p->n % 2 == 0));
+           p->pbs_multibit <= 2 && (p->pbs_multibit < 2 || (p->pbs_approx == 1 && p->n % 2 == 0)) &&
+           (p->pfks_linear == 0 || p->pfks_linear == 1);
 }
 
 int pbs_limbs(const tfhe_cbs_params& p) { return p.pbs_approx ? 1 : 2; }
@@ -31,6 +33,8 @@ size_t bsk2_rgsw(const tfhe_cbs_params& p) { return pbs_mb(p) == 2 ? (size_t)p.n
 // capacity bound: keeps every 32-bit count (2 x inputs for the PFKS) and every grid y
 // dimension (padded inputs / 64) of the path in range
 constexpr uint32_t kMaxBlocks = 4096;
+// inputs per linear-form PFKS launch (digit scratch pfkl_digit_bytes(kPfklChunk) = 134 MB)
+constexpr uint32_t kPfklChunk = 8192;
 size_t bsk2_cplx(const tfhe_cbs_params& p) { return bsk2_rgsw(p) * 8 * 2 * pbs_limbs(p) * 1024; }
 size_t pfks_words_per_u() { return (size_t)(kN + 1) * 8 * 15 * 2 * kN; }
 size_t ksk2_words(const tfhe_cbs_params& p) { return (size_t)kN * 8 * 4 * (p.n + 1); }
@@ -108,7 +112,9 @@ struct tfhe_cbs_ctx {
     uint64_t seed = 0;
     int32_t* d_K2 = nullptr;
     cplx* d_bsk2f = nullptr;
-    uint32_t* d_pfks = nullptr;  // [2][N+1][8][15][2N] torus-32
+    uint32_t* d_pfks = nullptr;  // [2][N+1][8][15][2N] torus-32 (lookup form)
+    int8_t* d_pfkl = nullptr;    // [2][5][2N][pfkl_kp()] signed key bytes (linear form)
+    int8_t* d_pfkl_D = nullptr;  // digit scratch of the linear form, kPfklChunk inputs
     uint32_t* d_ksk2 = nullptr;
     uint64_t* d_lwe64 = nullptr;  // [max_batch][2][N+1]
     uint32_t* d_at = nullptr;     // [N+1][pfks_bpad(2 max_batch)] rounded, transposed PFKS inputs
@@ -149,7 +155,7 @@ struct tfhe_cbs_ctx {
 extern "C" int tfhe_cbs_ctx_destroy(tfhe_cbs_ctx* c) {
     if (!c) return TFHE_OK;
     void* bufs[] = {c->d_K2, c->d_bsk2f, c->d_pfks, c->d_ksk2, c->d_lwe64, c->d_at, c->d_ks_at, c->d_rgsw, c->d_rgsw_f, c->d_lut32,
-                    c->d_state[0], c->d_state[1], c->d_sbox_poly, c->d_lin_mix, c->d_lin_last, c->d_rk, c->d_ctr, c->d_pub};
+                    c->d_state[0], c->d_state[1], c->d_sbox_poly, c->d_lin_mix, c->d_lin_last, c->d_rk, c->d_ctr, c->d_pub, c->d_pfkl, c->d_pfkl_D};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
@@ -198,6 +204,8 @@ extern "C" int tfhe_cbs_ctx_create(const tfhe_cbs_params* params, uint32_t max_b
     rc = rc ? rc : dmalloc((void**)&c->d_rk, (size_t)11 * 128 * (p.n + 1) * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_ctr, (size_t)max_blocks * 16);
     rc = rc ? rc : dmalloc((void**)&c->d_pub, (size_t)max_blocks * 16);
+    if (p.pfks_linear)
+        rc = rc ? rc : dmalloc((void**)&c->d_pfkl_D, pfkl_digit_bytes(std::min<uint32_t>(2 * (uint32_t)B, kPfklChunk)));
     rc = rc ? rc : c->tables.init();
     if (!rc) {
         // S-box table packed for vertical packing: coefficient 8x + j = bit j of S(x), times 2^31
@@ -291,12 +299,30 @@ extern "C" int tfhe_gen_pfks32(const tfhe_cbs_params* p, uint64_t seed, const in
     return rc;
 }
 
+// the PFKS key of the context's form (allocated once)
+static int alloc_pfks_key(tfhe_cbs_ctx* c) {
+    if (c->p.pfks_linear) return c->d_pfkl ? TFHE_OK : dmalloc((void**)&c->d_pfkl, pfkl_key_bytes());
+    return c->d_pfks ? TFHE_OK : dmalloc((void**)&c->d_pfks, 2 * pfks_words_per_u() * sizeof(uint32_t));
+}
+
+extern "C" int tfhe_gen_pfkl40(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_rlwe_sk, int8_t* d_pfkl,
+                               void* stream) {
+    if (!cbs_params_supported(p) || !d_rlwe_sk || !d_pfkl) return fail(TFHE_E_PARAM, "tfhe_gen_pfkl40: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    int rc = hipcheck(hipMemsetAsync(d_pfkl, 0, pfkl_key_bytes(), st), "pfkl zero");
+    if (!rc) {
+        launch_gen_pfkl40(seed, d_rlwe_sk, d_pfkl, st);
+        rc = check_launch("gen_pfkl40");
+    }
+    if (!rc) rc = hipcheck(hipStreamSynchronize(st), "gen_pfkl40 sync");
+    return rc;
+}
+
 extern "C" int tfhe_cbs_set_keys(tfhe_cbs_ctx* c, const int32_t* rlwe_sk, const uint64_t* bsk, const uint32_t* pfks,
                                  const uint32_t* ksk2) {
     if (!c || !rlwe_sk || !bsk || !pfks || !ksk2) return fail(TFHE_E_PARAM, "tfhe_cbs_set_keys: bad argument");
     const auto& p = c->p;
-    int rc = TFHE_OK;
-    if (!c->d_pfks) rc = dmalloc((void**)&c->d_pfks, 2 * pfks_words_per_u() * sizeof(uint32_t));
+    int rc = alloc_pfks_key(c);
     uint64_t* d_bsk = nullptr;
     const size_t bsk_words = bsk2_rgsw(p) * 8 * 2 * kN;
     rc = rc ? rc : dmalloc((void**)&d_bsk, bsk_words * sizeof(uint64_t));
@@ -306,8 +332,10 @@ extern "C" int tfhe_cbs_set_keys(tfhe_cbs_ctx* c, const int32_t* rlwe_sk, const 
     // host or device sources: hipMemcpyDefault follows the pointers
     if (!rc) rc = hipcheck(hipMemcpy(c->d_K2, c->K2.data(), kN * sizeof(int32_t), hipMemcpyHostToDevice), "K2");
     if (!rc) rc = hipcheck(hipMemcpy(d_bsk, bsk, bsk_words * sizeof(uint64_t), hipMemcpyDefault), "bsk2");
-    if (!rc) rc = hipcheck(hipMemcpy(c->d_pfks, pfks, 2 * pfks_words_per_u() * sizeof(uint32_t), hipMemcpyDefault),
-                           "pfks");
+    if (!rc)
+        rc = p.pfks_linear ? hipcheck(hipMemcpy(c->d_pfkl, pfks, pfkl_key_bytes(), hipMemcpyDefault), "pfkl")
+                           : hipcheck(hipMemcpy(c->d_pfks, pfks, 2 * pfks_words_per_u() * sizeof(uint32_t),
+                                                hipMemcpyDefault), "pfks");
     if (!rc) rc = hipcheck(hipMemcpy(c->d_ksk2, ksk2, ksk2_words(p) * sizeof(uint32_t), hipMemcpyDefault), "ksk2");
     if (!rc) {
         if (pbs_limbs(p) == 2)
@@ -329,8 +357,7 @@ extern "C" int tfhe_cbs_keygen(tfhe_cbs_ctx* c, const int32_t* lwe_sk, uint64_t 
     c->lwe_sk.assign(lwe_sk, lwe_sk + p.n);
     c->K2.resize(kN);
     tfhe_rlwe_keygen(kN, seed, DOM_RLWE2_SK, c->K2.data());
-    int rc = TFHE_OK;
-    if (!c->d_pfks) rc = dmalloc((void**)&c->d_pfks, 2 * pfks_words_per_u() * sizeof(uint32_t));
+    int rc = alloc_pfks_key(c);
     int32_t* d_s = nullptr;
     uint64_t *d_bsk = nullptr, *d_K64 = nullptr;
     cplx* d_K2f = nullptr;
@@ -350,9 +377,14 @@ extern "C" int tfhe_cbs_keygen(tfhe_cbs_ctx* c, const int32_t* lwe_sk, uint64_t 
             launch_to_fourier2048(d_bsk, bsk2_rgsw(p) * 8 * 2, c->tables.T, c->d_bsk2f, nullptr);
         else
             launch_to_fourier2048_t64(d_bsk, bsk2_rgsw(p) * 8 * 2, c->tables.T, c->d_bsk2f, nullptr);
-        launch_gen_pfks(seed, p.pf_nshift, c->d_K2, d_K2f, c->tables.T, c->d_pfks, nullptr);
+        if (p.pfks_linear) {
+            rc = hipcheck(hipMemset(c->d_pfkl, 0, pfkl_key_bytes()), "pfkl zero");
+            launch_gen_pfkl40(seed, c->d_K2, c->d_pfkl, nullptr);
+        } else {
+            launch_gen_pfks(seed, p.pf_nshift, c->d_K2, d_K2f, c->tables.T, c->d_pfks, nullptr);
+        }
         launch_gen_ksk32(seed, p.n, kN, p.ks_logB, p.ks_l, p.lwe_nshift, d_s, c->d_K2, c->d_ksk2, nullptr, DOM_KSK2);
-        rc = check_launch("cbs keygen");
+        if (!rc) rc = check_launch("cbs keygen");
     }
     if (!rc) rc = hipcheck(hipDeviceSynchronize(), "cbs keygen sync");
     (void)hipFree(d_s);
@@ -369,8 +401,28 @@ extern "C" int tfhe_cbs_get_rlwe_sk(const tfhe_cbs_ctx* c, int32_t* out) {
     return TFHE_OK;
 }
 
+extern "C" int tfhe_cbs_get_pfkl_entry(const tfhe_cbs_ctx* c, int u, int i, int t, uint64_t* out) {
+    if (!c || !out || !c->has_keys || !c->d_pfkl) return fail(TFHE_E_STATE, "tfhe_cbs_get_pfkl_entry: no linear key");
+    if (u < 0 || u > 1 || i < 0 || i > kN || t < 0 || t > 7)
+        return fail(TFHE_E_PARAM, "tfhe_cbs_get_pfkl_entry: index out of range");
+    // the entry's 2N columns of each signed byte slice, then the words mod 2^40
+    const size_t kp = pfkl_kp(), cols = 2 * kN;
+    std::vector<int8_t> sl((size_t)5 * cols);
+    int rc = TFHE_OK;
+    for (int b = 0; b < 5 && !rc; ++b)
+        rc = hipcheck(hipMemcpy2D(sl.data() + (size_t)b * cols, 1,
+                                  c->d_pfkl + (((size_t)u * 5 + b) * cols) * kp + (size_t)8 * i + t, kp, 1, cols,
+                                  hipMemcpyDeviceToHost), "pfkl entry");
+    for (size_t col = 0; col < cols && !rc; ++col) {
+        int64_t v = 0;
+        for (int b = 0; b < 5; ++b) v += (int64_t)sl[(size_t)b * cols + col] * ((int64_t)1 << (8 * b));
+        out[col] = (uint64_t)v & ((1ull << 40) - 1);
+    }
+    return rc;
+}
+
 extern "C" int tfhe_cbs_get_pfks_entry(const tfhe_cbs_ctx* c, int u, int i, int t, int v, uint32_t* out) {
-    if (!c || !out || !c->has_keys) return fail(TFHE_E_STATE, "tfhe_cbs_get_pfks_entry: no keys");
+    if (!c || !out || !c->has_keys || !c->d_pfks) return fail(TFHE_E_STATE, "tfhe_cbs_get_pfks_entry: no keys");
     if (u < 0 || u > 1 || i < 0 || i > kN || t < 0 || t > 7 || v < 1 || v > 15)
         return fail(TFHE_E_PARAM, "tfhe_cbs_get_pfks_entry: index out of range");
     const size_t row = (((size_t)u * (kN + 1) + i) * 8 + t) * 15 + (v - 1);
@@ -400,6 +452,16 @@ extern "C" int tfhe_cbs_pbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32
     return check_launch("pbs2");
 }
 
+// linear-form PFKS in launches of at most kPfklChunk inputs (the digit scratch)
+static void pfkl_chunks(tfhe_cbs_ctx* c, const uint64_t* lwe64, uint32_t count, int u0, int nu, uint32_t* out,
+                        size_t out_stride, size_t out_u_offset, hipStream_t st) {
+    for (uint32_t c0 = 0; c0 < count; c0 += kPfklChunk) {
+        const uint32_t cnt = std::min(kPfklChunk, count - c0);
+        launch_pfkl(lwe64 + (size_t)c0 * (kN + 1), cnt, c->d_pfkl_D, c->d_pfkl, u0, nu, out + (size_t)c0 * out_stride,
+                    out_stride, out_u_offset, st);
+    }
+}
+
 extern "C" int tfhe_cbs_pfks_batch(tfhe_cbs_ctx* c, int u, const uint64_t* d_lwe64, uint32_t count, size_t out_stride,
                                    uint32_t* d_out, void* stream) {
     int rc = check_cbs(c, count, c ? 2 * c->max_batch : 0);
@@ -408,6 +470,10 @@ extern "C" int tfhe_cbs_pfks_batch(tfhe_cbs_ctx* c, int u, const uint64_t* d_lwe
         return fail(TFHE_E_PARAM, "pfks: bad argument");
     if (!count) return TFHE_OK;
     hipStream_t st = (hipStream_t)stream;
+    if (c->p.pfks_linear) {
+        pfkl_chunks(c, d_lwe64, count, u, 1, d_out, out_stride, 0, st);
+        return check_launch("pfks linear");
+    }
     launch_pfks_digits(d_lwe64, count, c->d_at, st);
     launch_zero_rows32(d_out, count, out_stride, 2 * kN, st);
     launch_pfks32(c->d_at, count, c->d_pfks, pfks_words_per_u(), u, 1, d_out, out_stride, 0, st);
@@ -427,6 +493,10 @@ static void cbs_split(uint32_t batch, uint32_t& full, uint32_t& tail) {
 // at L * 4N + u * 2N): both functions in one launch over the rounded, transposed inputs
 static void cbs_pfks(tfhe_cbs_ctx* c, const uint64_t* lwe64, uint32_t count, uint32_t* at, uint32_t* rows,
                      hipStream_t st) {
+    if (c->p.pfks_linear) {
+        pfkl_chunks(c, lwe64, count, 0, 2, rows, 4 * (size_t)kN, 2 * (size_t)kN, st);
+        return;
+    }
     launch_pfks_digits(lwe64, count, at, st);
     launch_pfks32(at, count, c->d_pfks, pfks_words_per_u(), 0, 2, rows, 4 * (size_t)kN, 2 * (size_t)kN, st,
                   tuned(c->tune_pfks, count, 0));
@@ -439,7 +509,7 @@ extern "C" int tfhe_cbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t b
     if (!batch) return TFHE_OK;
     hipStream_t st = (hipStream_t)stream;
     const size_t lwe64_words = 2 * (size_t)(kN + 1), rgsw_words = 4 * 2 * (size_t)kN;
-    launch_zero_rows32(d_rgsw, 2 * batch, 4 * (size_t)kN, 4 * kN, st);
+    if (!c->p.pfks_linear) launch_zero_rows32(d_rgsw, 2 * batch, 4 * (size_t)kN, 4 * kN, st);  // lookup form adds
     // The PBS runs in rounds of as many workgroups as fit at once (two per CU).  A partial
     // last round leaves CUs idle, so it runs on the second stream while the full rounds'
     // PFKS starts on this one.
@@ -643,6 +713,7 @@ extern "C" int tfhe_cbs_autotune(tfhe_cbs_ctx* c, uint32_t blocks, int* choices,
         choices[part] = form;
         if (rc) break;
         const uint32_t count = 2 * batch;
+        if (c->p.pfks_linear) continue;  // one GEMM shape: nothing to choose
         launch_pfks_digits(c->d_lwe64, count, c->d_at, st);
         int sl = 0;
         rc = tune_one(st, e0, e1, {pfks_model_slices(count, 2), 4, 6, 8, 12, 16, 24, 32}, budget_ms,

@@ -820,6 +820,179 @@ __global__ void k_zero_rows32(uint32_t* __restrict__ out, uint32_t count, size_t
 }
 
 // ===========================================================================
+// Linear-key private functional key switch as an int8 GEMM on the matrix cores.
+// Key KL[u][i][t] = RLWE_K2(f_u(kappa_i 2^(4t)) 2^8) on Z/2^40 (sigma 2^7 units,
+// 2^-33 of the torus), one row per (i, t); input digits signed, d_t in [-8, 8):
+//   out40[L][col] = - sum_{k = 8i + t} d[L][k] KL[u][k][col]   (mod 2^40)
+//   out32 = round(out40 / 2^8)
+// K = 8 (N + 1) = 16392, padded with zero digits to PFKL_KP = 513 x 32.  Each
+// 40-bit key word is held as 5 signed bytes s_0..s_4 (sum s_b 2^(8b) = word mod
+// 2^40), so the sum is 5 int8 products of the same digit matrix: |sum| <= 16392
+// x 8 x 128 < 2^31 in the int32 accumulators, recombined as sum_b acc_b 2^(8b).
+// Layouts (k contiguous, one 16-byte run per lane and MFMA operand):
+//   digits D [Mpad][PFKL_KP] int8, key KT [2][5][2N][PFKL_KP] int8.
+// ===========================================================================
+constexpr int PFKL_SLICES = 5, PFKL_KP = 16416, PFKL_NSHIFT = 25;
+constexpr uint64_t PFKL_MASK = (1ull << 40) - 1;
+static_assert(PFKL_KP % 32 == 0 && PFKL_KP >= 8 * (CB_N + 1), "padded digit count");
+
+// one workgroup per key row (u, i, t), Philox id (u (N + 1) + i) 8 + t: a = 40 low
+// bits of Philox words, b = a * K2 (schoolbook over the support of the binary K2) + e
+// + msg, both polynomials written as signed byte slices at k = 8 i + t (columns c for
+// a, N + c for b).  The padded k of the key buffer are zeroed by the caller.
+__global__ __launch_bounds__(256) void k_gen_pfkl40(uint64_t seed, const int32_t* __restrict__ K2,
+                                                   int8_t* __restrict__ KT) {
+    __shared__ uint64_t as[CB_N];
+    __shared__ uint16_t sup[CB_N];
+    __shared__ uint32_t nsup;
+    constexpr int N = CB_N;
+    const uint32_t row = blockIdx.x;
+    const int t = (int)(row % 8);
+    const uint32_t ui = row / 8;
+    const int i = (int)(ui % (N + 1)), u = (int)(ui / (N + 1));
+    if (threadIdx.x == 0) nsup = 0;
+    __syncthreads();
+    for (int c = threadIdx.x; c < N; c += blockDim.x) {
+        as[c] = rand64(seed, DOM_PFKL, row, (uint32_t)c) & PFKL_MASK;
+        if (K2[c]) sup[atomicAdd(&nsup, 1u)] = (uint16_t)c;  // LDS atomic: order-free sum below
+    }
+    __syncthreads();
+    const uint32_t ns = nsup;
+    const uint32_t kappa = i < N ? (uint32_t)K2[i] : 0xFFFFFFFFu;
+    const uint64_t cval = ((uint64_t)(kappa << (4 * t)) << 8) & PFKL_MASK;
+    const size_t kpos = (size_t)8 * i + t;
+    int8_t* ka = KT + (size_t)u * PFKL_SLICES * 2 * N * PFKL_KP + kpos;
+    const size_t sstride = (size_t)2 * N * PFKL_KP;
+    for (int c = threadIdx.x; c < N; c += blockDim.x) {
+        uint64_t b = 0;
+        for (uint32_t q = 0; q < ns; ++q) {
+            const int j = sup[q];
+            b += j <= c ? as[c - j] : (0 - as[N + c - j]);
+        }
+        b += (uint64_t)noise(seed, DOM_PFKL, row, (uint32_t)c, PFKL_NSHIFT);
+        if (u == 1) {
+            if (c == 0) b += cval;
+        } else {
+            b -= (uint64_t)K2[c] * cval;
+        }
+        uint64_t v[2] = {as[c], b & PFKL_MASK};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            int carry = 0;
+            int8_t* o = ka + (size_t)(h * N + c) * PFKL_KP;
+#pragma unroll
+            for (int sl = 0; sl < PFKL_SLICES; ++sl) {
+                int x = (int)((v[h] >> (8 * sl)) & 255u) + carry;
+                carry = x >= 128;
+                o[sl * sstride] = (int8_t)(x - (carry << 8));
+            }
+        }
+    }
+}
+
+// signed base-16 digits of the rounded inputs: D[m][8 q + t] = d_t(round(lwe64[m][q] / 2^32))
+// for q <= N and m < B, zero elsewhere in [Mpad][PFKL_KP]
+__global__ __launch_bounds__(256) void k_pfkl_digits(const uint64_t* __restrict__ lwe64, uint32_t B, uint32_t Mpad,
+                                                    int8_t* __restrict__ D) {
+    constexpr int N = CB_N, Q = PFKL_KP / 8;
+    const size_t total = (size_t)Mpad * Q;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t m = e / Q;
+        const int q = (int)(e - m * Q);
+        uint64_t word = 0;
+        if (m < B && q <= N) {
+            const uint32_t at = (uint32_t)((lwe64[m * (N + 1) + q] + (1ull << 31)) >> 32);
+            const uint32_t w = at + 0x88888888u;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int d = (int)((w >> (4 * t)) & 15u) - 8;
+                word |= (uint64_t)(uint8_t)(int8_t)d << (8 * t);
+            }
+        }
+        *reinterpret_cast<uint64_t*>(D + m * PFKL_KP + 8 * q) = word;
+    }
+}
+
+typedef int pfkl_v4i __attribute__((ext_vector_type(4)));
+typedef int pfkl_v16i __attribute__((ext_vector_type(16)));
+
+// v_mfma_i32_32x32x32_i8: lane l holds 16 bytes of A row (l % 32) and of B column
+// (l % 32) at k = 16 (l / 32) + 0..15; accumulator register j of lane l is
+// C[8 (j / 4) + 4 (l / 32) + j % 4][l % 32].
+// Grid x = Mpad / 128 input tiles, y = nu * 2N / 64 column tiles.  Wave w computes
+// 64 inputs (m0 + 0..63, two 32-row blocks) x 32 columns x 5 slices: ten 16-register
+// accumulators, one 16-byte digit load per row block and one key load per slice
+// per 32 k.
+__global__ __launch_bounds__(256, 2) void k_pfkl_gemm(const int8_t* __restrict__ D, uint32_t B,
+                                                  const int8_t* __restrict__ KT, int u0,
+                                                  uint32_t* __restrict__ out, size_t out_stride,
+                                                  size_t out_u_offset) {
+    constexpr int COLS = 2 * CB_N;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t m0 = blockIdx.x * 128 + (w & 1) * 64;
+    const uint32_t cg = blockIdx.y * 64 + (w >> 1) * 32;
+    const int uu = (int)(cg / COLS), col0 = (int)(cg % COLS);
+    const int8_t* a0 = D + (size_t)(m0 + (lane & 31)) * PFKL_KP + 16 * (lane >> 5);
+    const int8_t* a1 = a0 + (size_t)32 * PFKL_KP;
+    const size_t sstride = (size_t)COLS * PFKL_KP;
+    const int8_t* bp = KT + (size_t)(u0 + uu) * PFKL_SLICES * sstride + (size_t)(col0 + (lane & 31)) * PFKL_KP +
+                       16 * (lane >> 5);
+    pfkl_v16i acc[2][PFKL_SLICES];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int sl = 0; sl < PFKL_SLICES; ++sl) acc[r][sl] = pfkl_v16i{};
+    // two operand sets: X holds even steps, Y odd ones; each set is loaded one step
+    // ahead of its products (the scheduling barriers keep the loads in place), so a
+    // load has the other set's ten products (~320 cycles) to arrive.  513 steps: 256
+    // pairs and a last even step.
+    struct Ops { pfkl_v4i a[2], b[PFKL_SLICES]; };
+    auto load = [&](Ops& x, int k) {
+        x.a[0] = *reinterpret_cast<const pfkl_v4i*>(a0 + k);
+        x.a[1] = *reinterpret_cast<const pfkl_v4i*>(a1 + k);
+#pragma unroll
+        for (int sl = 0; sl < PFKL_SLICES; ++sl) x.b[sl] = *reinterpret_cast<const pfkl_v4i*>(bp + sl * sstride + k);
+    };
+    auto mma = [&](const Ops& x) {
+#pragma unroll
+        for (int sl = 0; sl < PFKL_SLICES; ++sl) {
+            acc[0][sl] = __builtin_amdgcn_mfma_i32_32x32x32_i8(x.a[0], x.b[sl], acc[0][sl], 0, 0, 0);
+            acc[1][sl] = __builtin_amdgcn_mfma_i32_32x32x32_i8(x.a[1], x.b[sl], acc[1][sl], 0, 0, 0);
+        }
+    };
+    static_assert(PFKL_KP / 32 % 2 == 1, "an odd step count: pairs plus one");
+    Ops X, Y;
+    load(X, 0);
+#pragma unroll 1
+    for (int k = 0; k + 64 <= PFKL_KP - 32; k += 64) {
+        load(Y, k + 32);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(X);
+        __builtin_amdgcn_sched_barrier(0);
+        load(X, k + 64);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(Y);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    mma(X);
+    uint32_t* o = out + (size_t)uu * out_u_offset + col0 + (lane & 31);
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const uint32_t m = m0 + 32 * r + 8 * (j / 4) + 4 * (lane >> 5) + (j & 3);
+            if (m < B) {
+                int64_t tsum = 0;
+#pragma unroll
+                for (int sl = 0; sl < PFKL_SLICES; ++sl) tsum += (int64_t)acc[r][sl][j] * ((int64_t)1 << (8 * sl));
+                const uint64_t o40 = (0 - (uint64_t)tsum) & PFKL_MASK;
+                o[(size_t)m * out_stride] = (uint32_t)((o40 + 128) >> 8);
+            }
+        }
+}
+
+// ===========================================================================
 // Vertical-packing LUT on the 32-bit torus (reference CMux semantics,
 // rlwe_tables.cuh:79-108): acc = (0, table); for bit i = 0..7:
 // acc = CMux(C_i, acc, X^{-stride 2^i} acc) = acc + C_i (*) (X^{-stride 2^i} acc - acc),
@@ -1083,6 +1256,22 @@ void launch_pfks32(const uint32_t* at, uint32_t B, const uint32_t* key, size_t k
     dim3 grid(tiles, groups, (unsigned)(slices * nu));
     k_pfks32<<<grid, 64 * PF_WAVES, 0, st>>>(at, B, pfks_bpad(B), key, key_u_stride, u0, slices, out, out_stride,
                                               out_u_offset);
+}
+
+size_t pfkl_key_bytes() { return (size_t)2 * PFKL_SLICES * 2 * CB_N * PFKL_KP; }
+size_t pfkl_digit_bytes(uint32_t rows) { return (size_t)((rows + 127) / 128 * 128) * PFKL_KP; }
+uint32_t pfkl_kp() { return PFKL_KP; }
+
+void launch_gen_pfkl40(uint64_t seed, const int32_t* K2, int8_t* KT, hipStream_t st) {
+    k_gen_pfkl40<<<2 * (CB_N + 1) * 8, 256, 0, st>>>(seed, K2, KT);
+}
+
+void launch_pfkl(const uint64_t* lwe64, uint32_t B, int8_t* D, const int8_t* KT, int u0, int nu, uint32_t* out,
+                 size_t out_stride, size_t out_u_offset, hipStream_t st) {
+    const uint32_t Mpad = (B + 127) / 128 * 128;
+    k_pfkl_digits<<<grid_for((size_t)Mpad * (PFKL_KP / 8) / 256 + 1, 8192), 256, 0, st>>>(lwe64, B, Mpad, D);
+    k_pfkl_gemm<<<dim3(Mpad / 128, (unsigned)nu * 2 * CB_N / 64), 256, 0, st>>>(D, B, KT, u0, out, out_stride,
+                                                                              out_u_offset);
 }
 
 void launch_to_fourier2048(const uint64_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st) {

@@ -46,16 +46,18 @@ class CbsParams(ctypes.Structure):
                 ("cb_logB", ctypes.c_int), ("cb_l", ctypes.c_int), ("pf_logB", ctypes.c_int), ("pf_l", ctypes.c_int),
                 ("ks_logB", ctypes.c_int), ("ks_l", ctypes.c_int), ("lwe_nshift", ctypes.c_int),
                 ("rlwe_nshift", ctypes.c_int), ("pf_nshift", ctypes.c_int), ("pbs_approx", ctypes.c_int),
-                ("pbs_multibit", ctypes.c_int)]
+                ("pbs_multibit", ctypes.c_int), ("pfks_linear", ctypes.c_int)]
 
 
-def cbs_params(pbs_approx=0, pbs_multibit=1):
+def cbs_params(pbs_approx=0, pbs_multibit=1, pfks_linear=0):
     """n=636, N=2048; PBS 2^9 x 4 on the 64-bit torus (sigma 2^-51), PFKS 2^4 x 8 with
     a torus-32 key (sigma 2^-30), CB 2^8 x 2 (torus-32 RGSW), KS 2^2 x 8 to the
     level-0 key (sigma 2^-15).  pbs_approx=1 selects the one-product FP64 PBS
     (rounding error about 2^-33 of the torus) instead of the exact two-limb one;
-    pbs_multibit=2 (with pbs_approx=1) blind-rotates two key bits per step."""
-    return CbsParams(636, 2048, 9, 4, 8, 2, 4, 8, 2, 8, 15, 19, 30, int(pbs_approx), int(pbs_multibit))
+    pbs_multibit=2 (with pbs_approx=1) blind-rotates two key bits per step; pfks_linear=1
+    switches keys through the linear 40-bit PFKS key on the matrix cores."""
+    return CbsParams(636, 2048, 9, 4, 8, 2, 4, 8, 2, 8, 15, 19, 30, int(pbs_approx), int(pbs_multibit),
+                     int(pfks_linear))
 
 
 def build():
@@ -97,6 +99,9 @@ SIGNATURES = {
     "tfhe_cbs_get_pfks_entry": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")]),
     "tfhe_gen_bsk64": (ctypes.c_int, [ctypes.POINTER(CbsParams), ctypes.c_uint64, _vp, _vp, _vp, _vp]),
+    "tfhe_gen_pfkl40": (ctypes.c_int, [ctypes.POINTER(CbsParams), ctypes.c_uint64, _vp, _vp, _vp]),
+    "tfhe_cbs_get_pfkl_entry": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")]),
     "tfhe_gen_pfks32": (ctypes.c_int, [ctypes.POINTER(CbsParams), ctypes.c_uint64, _vp, _vp, _vp]),
     "tfhe_cbs_set_keys": (ctypes.c_int, [_vp, _i32p, _vp, _vp, _vp]),
     "tfhe_cbs_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
@@ -288,6 +293,12 @@ class CbsContext:
         def p_(a):
             return a.ctypes.data if isinstance(a, np.ndarray) else _ptr(a)
         check(lib().tfhe_cbs_set_keys(self.h, np.ascontiguousarray(rlwe_sk, np.int32), p_(bsk), p_(pfks), p_(ksk2)))
+
+    def pfkl_entry(self, u, i, t):
+        """Linear-key entry (u, i, t): 2N words mod 2^40."""
+        out = np.zeros(2 * self.p.N, np.uint64)
+        check(lib().tfhe_cbs_get_pfkl_entry(self.h, u, i, t, out))
+        return out
 
     def pfks_entry(self, u, i, t, v):
         out = np.zeros(2 * self.p.N, np.uint32)

@@ -155,6 +155,14 @@ typedef struct tfhe_cbs_params {
      * (1,1)): one decomposition and two inverse transforms per pair, 1.5x the
      * MACs and key bytes.  2 needs pbs_approx = 1 and an even n. */
     int pbs_multibit;
+    /* Private functional key switch form (configs: PFKS base 2^4, 8 digits):
+     * 0 = lookup key, every digit multiple v = 1..15 its own torus-32 RLWE row
+     *     (2 x 4.03 GB), one LDS read and subtraction per (input, row, column);
+     * 1 = linear key, one row per (i, t) on Z/2^40 (sigma 2^-33, 2 x 336 MB as
+     *     signed bytes), signed digits, the sum an int8 GEMM on the matrix cores,
+     *     rounded to the 32-bit torus.  Output noise 2^-23.8 of the torus against
+     *     2^-23.0 for the lookup key; the two forms give different words. */
+    int pfks_linear;
 } tfhe_cbs_params;
 
 typedef struct tfhe_cbs_ctx tfhe_cbs_ctx;
@@ -173,12 +181,18 @@ int tfhe_cbs_keygen(tfhe_cbs_ctx* ctx, const int32_t* lwe_sk, uint64_t seed);
 int tfhe_cbs_get_rlwe_sk(const tfhe_cbs_ctx* ctx, int32_t* out);
 /* host copy of PFKS key entry (u, i, t, v), 2N uint32 (keygen parity) */
 int tfhe_cbs_get_pfks_entry(const tfhe_cbs_ctx* ctx, int u, int i, int t, int v, uint32_t* out);
+/* host copy of linear-key entry (u, i, t) (pfks_linear = 1): 2N words mod 2^40 (a then b) */
+int tfhe_cbs_get_pfkl_entry(const tfhe_cbs_ctx* ctx, int u, int i, int t, uint64_t* out);
 /* stateless BSK2 generator into d_bsk [n][8][2][N] uint64, or [n/2][3][8][2][N]
  * pattern keys with pbs_multibit = 2 (keygen parity) */
 int tfhe_gen_bsk64(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_lwe_sk, const int32_t* d_rlwe_sk,
                    uint64_t* d_bsk, void* stream);
 /* stateless PFKS key generator into d_pfks [2][N+1][8][15][2N] uint32 (the
  * context's key; tfhe_cbs_get_pfks_entry reads entries of it) */
+/* stateless linear PFKS key generator (pfks_linear = 1) into d_pfkl, [2][5][2N][16416]
+ * int8 (2 x 336 MB): entry (u, i, t) at k = 8 i + t of every column, as 5 signed bytes
+ * of its 40-bit words; synchronises the stream */
+int tfhe_gen_pfkl40(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_rlwe_sk, int8_t* d_pfkl, void* stream);
 int tfhe_gen_pfks32(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_rlwe_sk, uint32_t* d_pfks,
                     void* stream);
 /* Keys made elsewhere into the context, as GPUCirBTSContext(cc, maxBatchSize)
@@ -186,7 +200,8 @@ int tfhe_gen_pfks32(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_rl
  * cirbts-init.cu:200-865): rlwe_sk host [N] int32; bsk coefficient rows as
  * tfhe_gen_bsk64 writes them, pfks as tfhe_gen_pfks32, ksk2 [N][8][4][n+1]
  * uint32.  The three key buffers may be host or device memory; the BSK2 is
- * transformed to the context's PBS form. */
+ * transformed to the context's PBS form.  With pfks_linear = 1, pfks points at the
+ * linear key as tfhe_gen_pfkl40 writes it. */
 int tfhe_cbs_set_keys(tfhe_cbs_ctx* ctx, const int32_t* rlwe_sk, const uint64_t* bsk, const uint32_t* pfks,
                       const uint32_t* ksk2);
 

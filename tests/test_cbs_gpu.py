@@ -621,6 +621,112 @@ def test_aes128_approx_pbs_form(actx, okeys):
     assert _decrypt_blocks(okeys, from_dev(d_tuned), 2) == dec
 
 
+@pytest.fixture(scope="module")
+def lctx(gpu, okeys):
+    """The same keys with the linear-key PFKS (pfks_linear = 1, int8 GEMM on the matrix cores)."""
+    import tfhe_amd
+    c = tfhe_amd.CbsContext(max_blocks=33, params=tfhe_amd.cbs_params(pfks_linear=1))
+    c.keygen(okeys.s, SEED)
+    yield c
+    c.close()
+
+
+def test_pfkl_key_entries_match_oracle(lctx, okeys):
+    """GPU linear-key entries (5 signed bytes per 40-bit word, k = 8 i + t) equal the
+    oracle's, including the b row (i = N) and both functions."""
+    N = okeys.p.N
+    for (u, i, t) in [(0, 0, 0), (1, 1, 7), (0, 777, 3), (1, N, 5), (0, N, 0)]:
+        assert np.array_equal(lctx.pfkl_entry(u, i, t), okeys.pfkl_entry(u, i, t)), (u, i, t)
+
+
+def test_pfkl_switch_bit_exact(lctx, okeys):
+    """The GEMM-form PFKS of random torus-64 LWE_K2 inputs equals the oracle's
+    linear-key switch bit for bit, for each function, at out_stride > 2N."""
+    import torch
+    N = okeys.p.N
+    rng = np.random.default_rng(23)
+    lwe = rng.integers(0, 2 ** 64, (3, N + 1), dtype=np.uint64)
+    d_lwe = to_dev64(lwe)
+    stride = 2 * N + 64
+    for u in (0, 1):
+        d_out = torch.zeros((3, stride), dtype=torch.int32, device="cuda")
+        lctx.pfks_batch(u, d_lwe, 3, stride, d_out)
+        torch.cuda.synchronize()
+        got = from_dev(d_out)
+        for k in range(3):
+            assert np.array_equal(got[k, :2 * N], okeys.pfks_linear(u, lwe[k])), (u, k)
+        assert not got[:, 2 * N:].any()  # nothing written past the row
+
+
+def test_cbs_linear_bit_exact(lctx, okeys):
+    """A whole circuit bootstrap with the linear PFKS equals the oracle's (the PBS
+    and all four RGSW rows)."""
+    import torch
+    cts = _bits_cts(okeys, [1], 910)
+    d_rgsw = torch.empty((1, 4, 2, 2048), dtype=torch.int32, device="cuda")
+    lctx.cbs_batch(to_dev32(cts), d_rgsw)
+    torch.cuda.synchronize()
+    want, err = okeys.cbs(cts[0], linear=True)
+    assert err < 0.25
+    assert np.array_equal(from_dev(d_rgsw).reshape(-1), want)
+
+
+def test_cbs_linear_chunked_batch_equals_parts(lctx, okeys):
+    """4100 inputs give 8200 PFKS inputs, over the 8192-input launch chunk, and a PBS
+    split at 4096: the rows equal two calls of 2050 bit for bit."""
+    import torch
+    import tfhe_amd
+    batch = 4100
+    rng = np.random.default_rng(43)
+    bits = rng.integers(0, 2, batch)
+    cts = np.stack([tfhe_amd.lwe_encrypt(okeys.s, 37, k, int(bits[k]), p=2) for k in range(batch)])
+    d_cts = to_dev32(cts)
+    d_all = torch.empty((batch, 4, 2, 2048), dtype=torch.int32, device="cuda")
+    lctx.cbs_batch(d_cts, d_all)
+    torch.cuda.synchronize()
+    whole = from_dev(d_all)
+    d_part = torch.empty((batch // 2, 4, 2, 2048), dtype=torch.int32, device="cuda")
+    for lo in (0, batch // 2):
+        lctx.cbs_batch(d_cts[lo:lo + batch // 2].contiguous(), d_part)
+        torch.cuda.synchronize()
+        assert np.array_equal(from_dev(d_part), whole[lo:lo + batch // 2]), lo
+    del d_all, d_part
+
+
+def test_aes128_linear_pfks(lctx, okeys):
+    """Homomorphic AES-128 with the linear PFKS: FIPS-197 C.1 and random blocks
+    decrypt exactly with phases inside half the margin; the graph replay equals the
+    eager keystream; transciphered plaintexts come back."""
+    import torch
+    import tfhe_amd
+    key = bytes(range(16))
+    lctx.set_round_keys(tfhe_amd.aes_encrypt_key_bits(okeys.s, 77, key))
+    rng = np.random.default_rng(9)
+    blocks = [bytes.fromhex("00112233445566778899aabbccddeeff")] + \
+        [bytes(rng.integers(0, 256, 16, dtype=np.uint8)) for _ in range(4)]
+    d_ks = torch.empty((5, 128, 637), dtype=torch.int32, device="cuda")
+    lctx.aes_ctr_keystream(blocks, d_ks)
+    torch.cuda.synchronize()
+    got = from_dev(d_ks)
+    dec = _decrypt_blocks(okeys, got, 5)
+    assert dec[0].hex() == "69c4e0d86a7b0430d8cdb78070b4c55a"
+    assert dec[1:] == [O.aes128_encrypt(key, b) for b in blocks[1:]]
+    s64 = okeys.s.astype(np.uint64)
+    ph = (got[:, :, -1].astype(np.uint64) - (got[:, :, :-1].astype(np.uint64) @ s64)) % 2 ** 32
+    err = ((ph + 2 ** 30) % 2 ** 31).astype(np.int64) - 2 ** 30
+    assert np.abs(err).max() < 2 ** 29
+    d_graph = torch.empty_like(d_ks)
+    for _ in range(2):
+        lctx.aes_ctr_keystream(blocks, d_graph, graph=True)
+        torch.cuda.synchronize()
+        assert np.array_equal(from_dev(d_graph), got)
+    plain = [bytes(rng.integers(0, 256, 16, dtype=np.uint8)) for _ in range(5)]
+    cipher = [bytes(p ^ k for p, k in zip(pt, O.aes128_encrypt(key, c))) for pt, c in zip(plain, blocks)]
+    lctx.aes_ctr_transcipher(blocks, cipher, d_ks)
+    torch.cuda.synchronize()
+    assert _decrypt_blocks(okeys, from_dev(d_ks), 5) == plain
+
+
 def test_cbs_error_paths(gpu):
     """Capacity, argument and state errors come back as codes (the reference throws,
     cirbts-batch.cu:97-100); nothing is launched."""
