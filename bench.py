@@ -236,7 +236,7 @@ def main():
                     help="comma-separated AES-128 CTR batch sizes to time ('' to skip)")
     ap.add_argument("--aes-plain-blocks", type=int, default=1,
                     help="AES blocks timed in the plain per-bootstrap form (exact PBS; 0 to skip)")
-    ap.add_argument("--aes-pbs", default="exact,approx",
+    ap.add_argument("--aes-pbs", default="exact,exact-lin,approx-lin",
                     help="comma-separated PBS forms of the AES runs: exact (two-limb), approx (one FP64 "
                          "product), approx-mb2 (one product, two key bits per step); exact-lin / approx-lin "
                          "with the linear-key PFKS on the matrix cores")

@@ -405,14 +405,17 @@ extern "C" int tfhe_cbs_get_pfkl_entry(const tfhe_cbs_ctx* c, int u, int i, int 
     if (!c || !out || !c->has_keys || !c->d_pfkl) return fail(TFHE_E_STATE, "tfhe_cbs_get_pfkl_entry: no linear key");
     if (u < 0 || u > 1 || i < 0 || i > kN || t < 0 || t > 7)
         return fail(TFHE_E_PARAM, "tfhe_cbs_get_pfkl_entry: index out of range");
-    // the entry's 2N columns of each signed byte slice, then the words mod 2^40
-    const size_t kp = pfkl_kp(), cols = 2 * kN;
+    // the entry's 2N columns of each signed byte slice (one byte per 32-column fragment
+    // row: columns 32 apart are one fragment block, pfkl_kp() x 32 bytes, apart), then the
+    // words mod 2^40
+    const size_t cols = 2 * kN, kp = pfkl_kp();
+    const uint32_t k = 8u * (uint32_t)i + (uint32_t)t;
     std::vector<int8_t> sl((size_t)5 * cols);
     int rc = TFHE_OK;
     for (int b = 0; b < 5 && !rc; ++b)
-        rc = hipcheck(hipMemcpy2D(sl.data() + (size_t)b * cols, 1,
-                                  c->d_pfkl + (((size_t)u * 5 + b) * cols) * kp + (size_t)8 * i + t, kp, 1, cols,
-                                  hipMemcpyDeviceToHost), "pfkl entry");
+        for (int cl = 0; cl < 32 && !rc; ++cl)
+            rc = hipcheck(hipMemcpy2D(sl.data() + (size_t)b * cols + cl, 32, c->d_pfkl + pfkl_key_offset(u, b, cl, k),
+                                      kp * 32, 1, cols / 32, hipMemcpyDeviceToHost), "pfkl entry");
     for (size_t col = 0; col < cols && !rc; ++col) {
         int64_t v = 0;
         for (int b = 0; b < 5; ++b) v += (int64_t)sl[(size_t)b * cols + col] * ((int64_t)1 << (8 * b));

@@ -829,12 +829,21 @@ __global__ void k_zero_rows32(uint32_t* __restrict__ out, uint32_t count, size_t
 // 40-bit key word is held as 5 signed bytes s_0..s_4 (sum s_b 2^(8b) = word mod
 // 2^40), so the sum is 5 int8 products of the same digit matrix: |sum| <= 16392
 // x 8 x 128 < 2^31 in the int32 accumulators, recombined as sum_b acc_b 2^(8b).
-// Layouts (k contiguous, one 16-byte run per lane and MFMA operand):
-//   digits D [Mpad][PFKL_KP] int8, key KT [2][5][2N][PFKL_KP] int8.
+// Both operands are stored in the MFMA fragment order, so that one operand load of
+// a wave is one contiguous 1 KB run: element (r, k) of a 32-row block (digit rows =
+// inputs, key rows = columns) sits in fragment (block, k / 32) at byte
+// 16 (r + 32 ((k % 32) / 16)) + k % 16 (pfkl_frag_off):
+//   digits D [Mpad / 32][PFKL_KP / 32][1 KB], key KT [2][5][2N / 32][PFKL_KP / 32][1 KB].
 // ===========================================================================
 constexpr int PFKL_SLICES = 5, PFKL_KP = 16416, PFKL_NSHIFT = 25;
 constexpr uint64_t PFKL_MASK = (1ull << 40) - 1;
 static_assert(PFKL_KP % 32 == 0 && PFKL_KP >= 8 * (CB_N + 1), "padded digit count");
+constexpr int PFKL_KS = PFKL_KP / 32;  // k steps of 32
+
+// byte offset of element (r, k) of a run of 32-row blocks in fragment order
+__host__ __device__ inline size_t pfkl_frag_off(size_t r, uint32_t k) {
+    return (((r / 32) * PFKL_KS + k / 32) * 64 + (r % 32) + 32 * ((k % 32) / 16)) * 16 + k % 16;
+}
 
 // one workgroup per key row (u, i, t), Philox id (u (N + 1) + i) 8 + t: a = 40 low
 // bits of Philox words, b = a * K2 (schoolbook over the support of the binary K2) + e
@@ -860,8 +869,8 @@ __global__ __launch_bounds__(256) void k_gen_pfkl40(uint64_t seed, const int32_t
     const uint32_t ns = nsup;
     const uint32_t kappa = i < N ? (uint32_t)K2[i] : 0xFFFFFFFFu;
     const uint64_t cval = ((uint64_t)(kappa << (4 * t)) << 8) & PFKL_MASK;
-    const size_t kpos = (size_t)8 * i + t;
-    int8_t* ka = KT + (size_t)u * PFKL_SLICES * 2 * N * PFKL_KP + kpos;
+    const uint32_t kpos = 8u * i + t;
+    int8_t* ka = KT + (size_t)u * PFKL_SLICES * 2 * N * PFKL_KP;
     const size_t sstride = (size_t)2 * N * PFKL_KP;
     for (int c = threadIdx.x; c < N; c += blockDim.x) {
         uint64_t b = 0;
@@ -879,7 +888,7 @@ __global__ __launch_bounds__(256) void k_gen_pfkl40(uint64_t seed, const int32_t
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             int carry = 0;
-            int8_t* o = ka + (size_t)(h * N + c) * PFKL_KP;
+            int8_t* o = ka + pfkl_frag_off((size_t)h * N + c, kpos);
 #pragma unroll
             for (int sl = 0; sl < PFKL_SLICES; ++sl) {
                 int x = (int)((v[h] >> (8 * sl)) & 255u) + carry;
@@ -909,7 +918,7 @@ __global__ __launch_bounds__(256) void k_pfkl_digits(const uint64_t* __restrict_
                 word |= (uint64_t)(uint8_t)(int8_t)d << (8 * t);
             }
         }
-        *reinterpret_cast<uint64_t*>(D + m * PFKL_KP + 8 * q) = word;
+        *reinterpret_cast<uint64_t*>(D + pfkl_frag_off(m, 8u * q)) = word;  // 8 bytes of one 16-byte run
     }
 }
 
@@ -933,11 +942,12 @@ __global__ __launch_bounds__(256, 2) void k_pfkl_gemm(const int8_t* __restrict__
     const uint32_t m0 = blockIdx.x * 128 + (w & 1) * 64;
     const uint32_t cg = blockIdx.y * 64 + (w >> 1) * 32;
     const int uu = (int)(cg / COLS), col0 = (int)(cg % COLS);
-    const int8_t* a0 = D + (size_t)(m0 + (lane & 31)) * PFKL_KP + 16 * (lane >> 5);
-    const int8_t* a1 = a0 + (size_t)32 * PFKL_KP;
+    // fragment (block, step) of lane l at ((block * PFKL_KS + step) * 64 + l) * 16: step k / 32
+    // advances a pointer by 1 KB
+    const int8_t* a0 = D + ((size_t)(m0 / 32) * PFKL_KS * 64 + lane) * 16;
+    const int8_t* a1 = a0 + (size_t)PFKL_KS * 1024;
     const size_t sstride = (size_t)COLS * PFKL_KP;
-    const int8_t* bp = KT + (size_t)(u0 + uu) * PFKL_SLICES * sstride + (size_t)(col0 + (lane & 31)) * PFKL_KP +
-                       16 * (lane >> 5);
+    const int8_t* bp = KT + (size_t)(u0 + uu) * PFKL_SLICES * sstride + ((size_t)(col0 / 32) * PFKL_KS * 64 + lane) * 16;
     pfkl_v16i acc[2][PFKL_SLICES];
 #pragma unroll
     for (int r = 0; r < 2; ++r)
@@ -948,11 +958,12 @@ __global__ __launch_bounds__(256, 2) void k_pfkl_gemm(const int8_t* __restrict__
     // load has the other set's ten products (~320 cycles) to arrive.  513 steps: 256
     // pairs and a last even step.
     struct Ops { pfkl_v4i a[2], b[PFKL_SLICES]; };
-    auto load = [&](Ops& x, int k) {
-        x.a[0] = *reinterpret_cast<const pfkl_v4i*>(a0 + k);
-        x.a[1] = *reinterpret_cast<const pfkl_v4i*>(a1 + k);
+    auto load = [&](Ops& x, int k) {  // k multiple of 32: fragment step k / 32 at byte 32 k
+        const size_t fo = (size_t)k * 32;
+        x.a[0] = *reinterpret_cast<const pfkl_v4i*>(a0 + fo);
+        x.a[1] = *reinterpret_cast<const pfkl_v4i*>(a1 + fo);
 #pragma unroll
-        for (int sl = 0; sl < PFKL_SLICES; ++sl) x.b[sl] = *reinterpret_cast<const pfkl_v4i*>(bp + sl * sstride + k);
+        for (int sl = 0; sl < PFKL_SLICES; ++sl) x.b[sl] = *reinterpret_cast<const pfkl_v4i*>(bp + sl * sstride + fo);
     };
     auto mma = [&](const Ops& x) {
 #pragma unroll
@@ -1259,6 +1270,9 @@ void launch_pfks32(const uint32_t* at, uint32_t B, const uint32_t* key, size_t k
 }
 
 size_t pfkl_key_bytes() { return (size_t)2 * PFKL_SLICES * 2 * CB_N * PFKL_KP; }
+size_t pfkl_key_offset(int u, int slice, int col, uint32_t k) {
+    return ((size_t)u * PFKL_SLICES + slice) * 2 * CB_N * PFKL_KP + pfkl_frag_off((size_t)col, k);
+}
 size_t pfkl_digit_bytes(uint32_t rows) { return (size_t)((rows + 127) / 128 * 128) * PFKL_KP; }
 uint32_t pfkl_kp() { return PFKL_KP; }
 

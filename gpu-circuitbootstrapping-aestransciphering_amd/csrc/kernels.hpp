@@ -60,11 +60,13 @@ void launch_pfks_digits(const uint64_t* lwe64, uint32_t B, uint32_t* at, hipStre
 void launch_zero_rows32(uint32_t* out, uint32_t count, size_t stride, uint32_t width, hipStream_t st);
 // pseudo-random words (an integer hash of the index) for timing inputs
 void launch_fill_hash32(uint32_t* p, size_t count, uint32_t seed, hipStream_t st);
-// Linear-key PFKS (int8 GEMM on the matrix cores): key KT [2][5][2N][pfkl_kp()] int8
+// Linear-key PFKS (int8 GEMM on the matrix cores): key KT [2][5] x (2N x pfkl_kp()) int8
 // (zero the buffer, then launch_gen_pfkl40), digit scratch pfkl_digit_bytes(B) bytes;
 // launch_pfkl writes count B torus-32 RLWE outputs (2N words) for functions u0 .. u0+nu-1
 // at out + L * out_stride + (u - u0) * out_u_offset (no zero fill needed).
 size_t pfkl_key_bytes();
+// byte offset of key byte (function u, slice, column, k = 8 i + t) in the fragment-ordered key
+size_t pfkl_key_offset(int u, int slice, int col, uint32_t k);
 size_t pfkl_digit_bytes(uint32_t rows);
 uint32_t pfkl_kp();
 void launch_gen_pfkl40(uint64_t seed, const int32_t* K2, int8_t* KT, hipStream_t st);
