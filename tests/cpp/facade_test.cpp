@@ -163,6 +163,16 @@ static int circuit_bootstrap_and_aes(const std::vector<int32_t>& sk) {
         EXPECT(std::memcmp(got, plain[k].data(), 16) == 0, "Transcipher block %d", k);
     }
     std::printf("AES-128 CTR transcipher: 2 random blocks ok\n");
+
+    // the same FIPS-197 block with the one-product PBS and the linear-key PFKS
+    GPUCirBTSContext fast(CirBTSParams(1, 1, 1), 128);
+    fast.KeyGen(sk, 7);
+    fast.SetRoundKeys(sk, 77, key);
+    const auto ks = fast.AesCtrKeystream({pt});
+    uint8_t got[16] = {0};
+    for (int bit = 0; bit < 128; ++bit) got[bit / 8] |= (uint8_t)(LWEDecrypt(sk, ks[0][bit], 2) << (bit % 8));
+    EXPECT(std::memcmp(got, want, 16) == 0, "AES-128 FIPS-197 C.1, one-product PBS + linear PFKS");
+    std::printf("AES-128 CTR keystream, one-product PBS + linear PFKS: FIPS-197 C.1 ok\n");
     return 0;
 }
 

@@ -5,7 +5,8 @@ bootstrapping path; keys are not stored (they are regenerated from the seed by
 the oracle on the CPU and by the library's key generators on the GPU, which
 are bit-identical).  Run from the repository root:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            (all fixtures)
+    python tests/golden/make_golden.py cbs_linear (one of them)
 
 The fixtures are then frozen: tests/test_golden.py checks that the oracle still
 reproduces them (CPU) and that the C ABI reproduces them (GPU).
@@ -73,6 +74,22 @@ def make_cbs():
              lut_x=np.uint8(LUT_X), lut_rgsw_id0=np.uint32(LUT_RGSW_ID0), lut=lut)
 
 
+def make_cbs_linear():
+    """The linear-key PFKS (40-bit key, signed digits) on the inputs of make_cbs:
+    the switched RLWEs of both functions, a whole circuit bootstrap, key entries."""
+    k = O.CbsKeys(CBS_SEED)
+    bits = np.array([1, 0], np.uint8)
+    cts = np.stack([k.encrypt_bit(int(bits[q]), 900 + q) for q in range(2)])
+    rng = np.random.default_rng(2024)
+    lwe64 = rng.integers(0, 2 ** 64, (2, 2049), dtype=np.uint64)
+    pfkl = np.stack([np.stack([k.pfks_linear(u, lwe64[q]) for u in (0, 1)]) for q in range(2)])
+    rgsw0, _ = k.cbs(cts[0], linear=True)
+    entries = np.array([[0, 0, 0], [1, 2048, 7], [0, 1000, 4]], np.int32)
+    ent = np.stack([k.pfkl_entry(int(u), int(i), int(t)) for u, i, t in entries])
+    np.savez(os.path.join(HERE, "cbs_linear_seed21.npz"), seed=np.uint64(CBS_SEED), ct=cts, lwe64=lwe64, pfkl=pfkl,
+             rgsw0=rgsw0.reshape(4, 2, 2048), entries=entries, entry_words=ent)
+
+
 def make_aes():
     rng = np.random.default_rng(77)
     keys = rng.integers(0, 256, (3, 16), dtype=np.uint8)
@@ -82,9 +99,9 @@ def make_aes():
 
 
 if __name__ == "__main__":
-    make_gate()
-    make_cbs()
-    make_aes()
+    makers = {"gate": make_gate, "cbs": make_cbs, "cbs_linear": make_cbs_linear, "aes": make_aes}
+    for name in (sys.argv[1:] or list(makers)):
+        makers[name]()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

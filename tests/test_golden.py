@@ -58,6 +58,27 @@ def test_oracle_cbs_fixture():
         assert ((ph + 2 ** 30) >> 31) & 1 == (int(g["table"][x]) >> j) & 1
 
 
+def test_oracle_cbs_linear_fixture():
+    """Linear-key PFKS fixture: the oracle reproduces the switched RLWEs, the circuit
+    bootstrap and the key entries; each switched RLWE decrypts to f_u(phase)."""
+    g = load("cbs_linear_seed21.npz")
+    k = O.CbsKeys(int(g["seed"]))
+    for q in range(2):
+        for u in (0, 1):
+            assert np.array_equal(k.pfks_linear(u, g["lwe64"][q]), g["pfkl"][q, u]), (q, u)
+    for (u, i, t), words in zip(g["entries"], g["entry_words"]):
+        assert np.array_equal(k.pfkl_entry(int(u), int(i), int(t)), words), (u, i, t)
+    rgsw0, _ = k.cbs(g["ct"][0], linear=True)
+    assert np.array_equal(rgsw0, g["rgsw0"].reshape(-1))
+    N = k.p.N
+    for q in range(2):
+        ph = k.lwe64_phase(g["lwe64"][q]) >> 32
+        m = np.zeros(N, np.uint32)
+        m[0] = ph
+        err = ((O.rlwe_phase32(k.K2, g["pfkl"][q, 1]) - m).astype(np.uint32)).view(np.int32)
+        assert np.abs(err).max() < 2 ** 13, q
+
+
 def test_aes_plain_fixture_oracle_and_library():
     import tfhe_amd
     g = load("aes128_plain.npz")
@@ -126,5 +147,33 @@ def test_gpu_cbs_fixture(gpu):
         assert np.array_equal(from_dev(d_pf), g["pfks"])
         assert np.array_equal(from_dev(d_ks), g["ks2"])
         assert np.array_equal(from_dev(d_lut)[0], g["lut"])
+    finally:
+        c.close()
+
+
+@pytest.mark.gpu
+def test_gpu_cbs_linear_fixture(gpu):
+    """tfhe_cbs_keygen with pfks_linear = 1 from the fixture's seed: the key entries,
+    both switched RLWEs of each input and a whole circuit bootstrap reproduce the
+    fixture bit for bit."""
+    import torch
+    import tfhe_amd
+    g = load("cbs_linear_seed21.npz")
+    seed = int(g["seed"])
+    okeys = O.CbsKeys(seed, with_bsk=False)
+    c = tfhe_amd.CbsContext(max_blocks=1, params=tfhe_amd.cbs_params(pfks_linear=1))
+    try:
+        c.keygen(okeys.s, seed)
+        for (u, i, t), words in zip(g["entries"], g["entry_words"]):
+            assert np.array_equal(c.pfkl_entry(int(u), int(i), int(t)), words), (u, i, t)
+        d_l64 = torch.from_numpy(g["lwe64"].view(np.int64).copy()).to("cuda")
+        d_pf = torch.empty((2, 2, 4096), dtype=torch.int32, device="cuda")
+        for u in (0, 1):
+            c.pfks_batch(u, d_l64, 2, 2 * 4096, d_pf[:, u])
+        d_rgsw = torch.empty((1, 4, 2, 2048), dtype=torch.int32, device="cuda")
+        c.cbs_batch(to_dev(g["ct"][:1]), d_rgsw)
+        torch.cuda.synchronize()
+        assert np.array_equal(from_dev(d_pf), g["pfkl"])
+        assert np.array_equal(from_dev(d_rgsw)[0], g["rgsw0"])
     finally:
         c.close()
