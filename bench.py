@@ -410,10 +410,11 @@ def main():
         forms = {"exact": ("aes128_ctr", 0, 1, 0), "approx": ("aes128_ctr_one_product_pbs", 1, 1, 0),
                  "approx-mb2": ("aes128_ctr_one_product_pbs_2bit", 1, 2, 0),
                  "exact-lin": ("aes128_ctr_linear_pfks", 0, 1, 1),
-                 "approx-lin": ("aes128_ctr_one_product_pbs_linear_pfks", 1, 1, 1)}
+                 "approx-lin": ("aes128_ctr_one_product_pbs_linear_pfks", 1, 1, 1),
+                 "approx-mb2-lin": ("aes128_ctr_one_product_pbs_2bit_linear_pfks", 1, 2, 1)}
         for f in [f for f in args.aes_pbs.split(",") if f]:
             if f not in forms:
-                sys.exit(f"--aes-pbs: unknown form {f!r} (exact, approx, approx-mb2, exact-lin, approx-lin)")
+                sys.exit(f"--aes-pbs: unknown form {f!r} ({', '.join(forms)})")
             key, approx, mb, lin = forms[f]
             line[key] = aes_ctr_bench(sizes, pbs_approx=approx, pbs_multibit=mb,
                                       plain_blocks=args.aes_plain_blocks if f == "exact" else 0,

@@ -102,6 +102,44 @@ void shift_mix(const uint8_t in[16], uint8_t out[16], bool mix) {
     }
 }
 constexpr int kLinSrc = 8;  // max sources per output bit of ShiftRows+MixColumns
+
+// Per-call scratch of one group of AES blocks: the context's main set, or the small
+// set that the remainder group of an uneven batch uses on the second stream while the
+// full-round group runs on the caller's stream.
+struct CbsScratch {
+    uint64_t* lwe64 = nullptr;  // [cap][2][N+1] PBS outputs
+    uint32_t* at = nullptr;     // [N+1][pfks_bpad(2 cap) + 256] lookup-PFKS inputs, rounded and transposed
+    int8_t* pfkl_D = nullptr;   // linear-PFKS digits, min(2 cap, kPfklChunk) inputs
+    uint32_t* ks_at = nullptr;  // [N][ks_bpad(cap)] key-switch digits
+    uint32_t* rgsw = nullptr;   // [cap][4][2][N]
+    cplx* rgsw_f = nullptr;     // [cap][8][1024]
+    uint32_t* lut32 = nullptr;  // [cap][N+1]
+    uint32_t* state[2] = {nullptr, nullptr};  // [cap][n+1]
+};
+
+// allocates a scratch set for cap circuit bootstraps
+int alloc_scratch(CbsScratch& s, size_t cap, int n, bool linear) {
+    int rc = TFHE_OK;
+    rc = rc ? rc : dmalloc((void**)&s.lwe64, cap * 2 * (kN + 1) * sizeof(uint64_t));
+    // + 256 columns: a batch split into full rounds and a tail pads each part separately
+    rc = rc ? rc : dmalloc((void**)&s.at, (size_t)(kN + 1) * (pfks_bpad(2 * (uint32_t)cap) + 256) * sizeof(uint32_t));
+    rc = rc ? rc : dmalloc((void**)&s.ks_at, (size_t)kN * ks_bpad((uint32_t)cap) * sizeof(uint32_t));
+    rc = rc ? rc : dmalloc((void**)&s.rgsw, cap * 4 * 2 * kN * sizeof(uint32_t));
+    rc = rc ? rc : dmalloc((void**)&s.rgsw_f, cap * 8 * 1024 * sizeof(cplx));
+    rc = rc ? rc : dmalloc((void**)&s.lut32, cap * (kN + 1) * sizeof(uint32_t));
+    rc = rc ? rc : dmalloc((void**)&s.state[0], cap * (n + 1) * sizeof(uint32_t));
+    rc = rc ? rc : dmalloc((void**)&s.state[1], cap * (n + 1) * sizeof(uint32_t));
+    if (linear)
+        rc = rc ? rc : dmalloc((void**)&s.pfkl_D, pfkl_digit_bytes(std::min<uint32_t>(2 * (uint32_t)cap, kPfklChunk)));
+    return rc;
+}
+
+void free_scratch(CbsScratch& s) {
+    void* bufs[] = {s.lwe64, s.at, s.pfkl_D, s.ks_at, s.rgsw, s.rgsw_f, s.lut32, s.state[0], s.state[1]};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    s = CbsScratch{};
+}
 }  // namespace
 
 struct tfhe_cbs_ctx {
@@ -114,18 +152,16 @@ struct tfhe_cbs_ctx {
     cplx* d_bsk2f = nullptr;
     uint32_t* d_pfks = nullptr;  // [2][N+1][8][15][2N] torus-32 (lookup form)
     int8_t* d_pfkl = nullptr;    // [2][5][2N][pfkl_kp()] signed key bytes (linear form)
-    int8_t* d_pfkl_D = nullptr;  // digit scratch of the linear form, kPfklChunk inputs
     uint32_t* d_ksk2 = nullptr;
-    uint64_t* d_lwe64 = nullptr;  // [max_batch][2][N+1]
-    uint32_t* d_at = nullptr;     // [N+1][pfks_bpad(2 max_batch)] rounded, transposed PFKS inputs
-    uint32_t* d_ks_at = nullptr;  // [N][ks_bpad(max_batch)] key-switch digits scratch
+    CbsScratch sc;                // per-call scratch for max_batch circuit bootstraps
+    // scratch of the remainder group of an uneven AES batch (tail_blocks blocks at most),
+    // run on cstream beside the full-round group
+    CbsScratch tail;
+    uint32_t tail_blocks = 0;
+    hipEvent_t ev_gfork = nullptr, ev_gjoin = nullptr;
     DeviceTables1024 tables;
     // AES state
     uint32_t max_blocks = 0;
-    uint32_t* d_rgsw = nullptr;   // [128 max_blocks][4][2][N] torus-32
-    cplx* d_rgsw_f = nullptr;     // [128 max_blocks][8][1024]
-    uint32_t* d_lut32 = nullptr;  // [128 max_blocks][N+1]
-    uint32_t* d_state[2] = {nullptr, nullptr};  // [max_blocks][128][n+1]
     uint32_t* d_sbox_poly = nullptr;            // [N]
     int16_t* d_lin_mix = nullptr;               // [128][kLinSrc]
     int16_t* d_lin_last = nullptr;
@@ -154,10 +190,14 @@ struct tfhe_cbs_ctx {
 
 extern "C" int tfhe_cbs_ctx_destroy(tfhe_cbs_ctx* c) {
     if (!c) return TFHE_OK;
-    void* bufs[] = {c->d_K2, c->d_bsk2f, c->d_pfks, c->d_ksk2, c->d_lwe64, c->d_at, c->d_ks_at, c->d_rgsw, c->d_rgsw_f, c->d_lut32,
-                    c->d_state[0], c->d_state[1], c->d_sbox_poly, c->d_lin_mix, c->d_lin_last, c->d_rk, c->d_ctr, c->d_pub, c->d_pfkl, c->d_pfkl_D};
+    void* bufs[] = {c->d_K2, c->d_bsk2f, c->d_pfks, c->d_ksk2, c->d_sbox_poly, c->d_lin_mix, c->d_lin_last, c->d_rk,
+                    c->d_ctr, c->d_pub, c->d_pfkl};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    free_scratch(c->sc);
+    free_scratch(c->tail);
+    if (c->ev_gfork) (void)hipEventDestroy(c->ev_gfork);
+    if (c->ev_gjoin) (void)hipEventDestroy(c->ev_gjoin);
     for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
     if (c->ev_in) (void)hipEventDestroy(c->ev_in);
     if (c->ev_out) (void)hipEventDestroy(c->ev_out);
@@ -189,23 +229,27 @@ extern "C" int tfhe_cbs_ctx_create(const tfhe_cbs_params* params, uint32_t max_b
     rc = rc ? rc : dmalloc((void**)&c->d_K2, kN * sizeof(int32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_bsk2f, bsk2_cplx(p) * sizeof(cplx));
     rc = rc ? rc : dmalloc((void**)&c->d_ksk2, ksk2_words(p) * sizeof(uint32_t));
-    rc = rc ? rc : dmalloc((void**)&c->d_lwe64, B * 2 * (kN + 1) * sizeof(uint64_t));
-    // + 256 columns: a batch split into full rounds and a tail pads each part separately
-    rc = rc ? rc : dmalloc((void**)&c->d_at, (size_t)(kN + 1) * (pfks_bpad(2 * (uint32_t)B) + 256) * sizeof(uint32_t));
-    rc = rc ? rc : dmalloc((void**)&c->d_ks_at, (size_t)kN * ks_bpad((uint32_t)B) * sizeof(uint32_t));
-    rc = rc ? rc : dmalloc((void**)&c->d_rgsw, B * 4 * 2 * kN * sizeof(uint32_t));
-    rc = rc ? rc : dmalloc((void**)&c->d_rgsw_f, B * 8 * 1024 * sizeof(cplx));
-    rc = rc ? rc : dmalloc((void**)&c->d_lut32, B * (kN + 1) * sizeof(uint32_t));
-    rc = rc ? rc : dmalloc((void**)&c->d_state[0], B * (p.n + 1) * sizeof(uint32_t));
-    rc = rc ? rc : dmalloc((void**)&c->d_state[1], B * (p.n + 1) * sizeof(uint32_t));
+    rc = rc ? rc : alloc_scratch(c->sc, B, p.n, p.pfks_linear != 0);
+    {
+        // uneven AES batches: blocks beyond the last full round of resident PBS workgroups
+        // (2 x CUs inputs, rb blocks) run as their own group on the second stream
+        const uint32_t slots = 2 * device_cus(), rb = slots / 128;
+        if (!rc && slots % 128 == 0 && rb >= 2 && max_blocks > rb) {
+            c->tail_blocks = rb - 1;
+            rc = alloc_scratch(c->tail, 128 * (size_t)c->tail_blocks, p.n, p.pfks_linear != 0);
+            if (!rc) rc = hipcheck(hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking), "cbs stream");
+            if (!rc) rc = hipcheck(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "cbs event");
+            if (!rc) rc = hipcheck(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "cbs event");
+            if (!rc) rc = hipcheck(hipEventCreateWithFlags(&c->ev_gfork, hipEventDisableTiming), "cbs event");
+            if (!rc) rc = hipcheck(hipEventCreateWithFlags(&c->ev_gjoin, hipEventDisableTiming), "cbs event");
+        }
+    }
     rc = rc ? rc : dmalloc((void**)&c->d_sbox_poly, kN * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_lin_mix, 128 * kLinSrc * sizeof(int16_t));
     rc = rc ? rc : dmalloc((void**)&c->d_lin_last, 128 * kLinSrc * sizeof(int16_t));
     rc = rc ? rc : dmalloc((void**)&c->d_rk, (size_t)11 * 128 * (p.n + 1) * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_ctr, (size_t)max_blocks * 16);
     rc = rc ? rc : dmalloc((void**)&c->d_pub, (size_t)max_blocks * 16);
-    if (p.pfks_linear)
-        rc = rc ? rc : dmalloc((void**)&c->d_pfkl_D, pfkl_digit_bytes(std::min<uint32_t>(2 * (uint32_t)B, kPfklChunk)));
     rc = rc ? rc : c->tables.init();
     if (!rc) {
         // S-box table packed for vertical packing: coefficient 8x + j = bit j of S(x), times 2^31
@@ -456,12 +500,12 @@ extern "C" int tfhe_cbs_pbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32
 }
 
 // linear-form PFKS in launches of at most kPfklChunk inputs (the digit scratch)
-static void pfkl_chunks(tfhe_cbs_ctx* c, const uint64_t* lwe64, uint32_t count, int u0, int nu, uint32_t* out,
-                        size_t out_stride, size_t out_u_offset, hipStream_t st) {
+static void pfkl_chunks(tfhe_cbs_ctx* c, int8_t* D, const uint64_t* lwe64, uint32_t count, int u0, int nu,
+                        uint32_t* out, size_t out_stride, size_t out_u_offset, hipStream_t st) {
     for (uint32_t c0 = 0; c0 < count; c0 += kPfklChunk) {
         const uint32_t cnt = std::min(kPfklChunk, count - c0);
-        launch_pfkl(lwe64 + (size_t)c0 * (kN + 1), cnt, c->d_pfkl_D, c->d_pfkl, u0, nu, out + (size_t)c0 * out_stride,
-                    out_stride, out_u_offset, st);
+        launch_pfkl(lwe64 + (size_t)c0 * (kN + 1), cnt, D, c->d_pfkl, u0, nu, out + (size_t)c0 * out_stride, out_stride,
+                    out_u_offset, st);
     }
 }
 
@@ -474,12 +518,12 @@ extern "C" int tfhe_cbs_pfks_batch(tfhe_cbs_ctx* c, int u, const uint64_t* d_lwe
     if (!count) return TFHE_OK;
     hipStream_t st = (hipStream_t)stream;
     if (c->p.pfks_linear) {
-        pfkl_chunks(c, d_lwe64, count, u, 1, d_out, out_stride, 0, st);
+        pfkl_chunks(c, c->sc.pfkl_D, d_lwe64, count, u, 1, d_out, out_stride, 0, st);
         return check_launch("pfks linear");
     }
-    launch_pfks_digits(d_lwe64, count, c->d_at, st);
+    launch_pfks_digits(d_lwe64, count, c->sc.at, st);
     launch_zero_rows32(d_out, count, out_stride, 2 * kN, st);
-    launch_pfks32(c->d_at, count, c->d_pfks, pfks_words_per_u(), u, 1, d_out, out_stride, 0, st);
+    launch_pfks32(c->sc.at, count, c->d_pfks, pfks_words_per_u(), u, 1, d_out, out_stride, 0, st);
     return check_launch("pfks");
 }
 
@@ -494,10 +538,10 @@ static void cbs_split(uint32_t batch, uint32_t& full, uint32_t& tail) {
 
 // PFKS of count PBS outputs (LWE L = 2 input + j -> RGSW row 2j + u of that input,
 // at L * 4N + u * 2N): both functions in one launch over the rounded, transposed inputs
-static void cbs_pfks(tfhe_cbs_ctx* c, const uint64_t* lwe64, uint32_t count, uint32_t* at, uint32_t* rows,
+static void cbs_pfks(tfhe_cbs_ctx* c, const uint64_t* lwe64, uint32_t count, uint32_t* at, int8_t* D, uint32_t* rows,
                      hipStream_t st) {
     if (c->p.pfks_linear) {
-        pfkl_chunks(c, lwe64, count, 0, 2, rows, 4 * (size_t)kN, 2 * (size_t)kN, st);
+        pfkl_chunks(c, D, lwe64, count, 0, 2, rows, 4 * (size_t)kN, 2 * (size_t)kN, st);
         return;
     }
     launch_pfks_digits(lwe64, count, at, st);
@@ -505,12 +549,9 @@ static void cbs_pfks(tfhe_cbs_ctx* c, const uint64_t* lwe64, uint32_t count, uin
                   tuned(c->tune_pfks, count, 0));
 }
 
-extern "C" int tfhe_cbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t batch, uint32_t* d_rgsw, void* stream) {
-    int rc = check_cbs(c, batch, c ? c->max_batch : 0);
-    if (rc) return rc;
-    if (!d_lwe || !d_rgsw) return fail(TFHE_E_PARAM, "cbs: NULL buffer");
-    if (!batch) return TFHE_OK;
-    hipStream_t st = (hipStream_t)stream;
+// circuit bootstraps with scratch set sc (checked arguments)
+static int cbs_batch_impl(tfhe_cbs_ctx* c, const CbsScratch& sc, const uint32_t* d_lwe, uint32_t batch,
+                          uint32_t* d_rgsw, hipStream_t st) {
     const size_t lwe64_words = 2 * (size_t)(kN + 1), rgsw_words = 4 * 2 * (size_t)kN;
     if (!c->p.pfks_linear) launch_zero_rows32(d_rgsw, 2 * batch, 4 * (size_t)kN, 4 * kN, st);  // lookup form adds
     // The PBS runs in rounds of as many workgroups as fit at once (two per CU).  A partial
@@ -518,6 +559,7 @@ extern "C" int tfhe_cbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t b
     // PFKS starts on this one.
     uint32_t full = 0, tail = 0;
     cbs_split(batch, full, tail);
+    int rc = TFHE_OK;
     if (full && tail) {
         if (!c->cstream) {
             rc = hipcheck(hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking), "cbs stream");
@@ -525,24 +567,32 @@ extern "C" int tfhe_cbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t b
             if (!rc) rc = hipcheck(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "cbs event");
             if (rc) return rc;
         }
-        rc = tfhe_cbs_pbs_batch(c, d_lwe, full, c->d_lwe64, stream);
+        rc = tfhe_cbs_pbs_batch(c, d_lwe, full, sc.lwe64, st);
         if (!rc) rc = hipcheck(hipEventRecord(c->ev_fork, st), "fork");
         if (!rc) rc = hipcheck(hipStreamWaitEvent(c->cstream, c->ev_fork, 0), "fork wait");
-        if (!rc) rc = tfhe_cbs_pbs_batch(c, d_lwe + (size_t)full * (c->p.n + 1), tail,
-                                         c->d_lwe64 + full * lwe64_words, c->cstream);
+        if (!rc) rc = tfhe_cbs_pbs_batch(c, d_lwe + (size_t)full * (c->p.n + 1), tail, sc.lwe64 + full * lwe64_words,
+                                         c->cstream);
         if (!rc) rc = hipcheck(hipEventRecord(c->ev_join, c->cstream), "join");
         if (rc) return rc;
-        cbs_pfks(c, c->d_lwe64, 2 * full, c->d_at, d_rgsw, st);
+        cbs_pfks(c, sc.lwe64, 2 * full, sc.at, sc.pfkl_D, d_rgsw, st);
         rc = hipcheck(hipStreamWaitEvent(st, c->ev_join, 0), "join wait");
         if (rc) return rc;
-        cbs_pfks(c, c->d_lwe64 + full * lwe64_words, 2 * tail,
-                 c->d_at + (size_t)(kN + 1) * pfks_bpad(2 * full), d_rgsw + full * rgsw_words, st);
+        cbs_pfks(c, sc.lwe64 + full * lwe64_words, 2 * tail, sc.at + (size_t)(kN + 1) * pfks_bpad(2 * full),
+                 sc.pfkl_D, d_rgsw + full * rgsw_words, st);
     } else {
-        rc = tfhe_cbs_pbs_batch(c, d_lwe, batch, c->d_lwe64, stream);
+        rc = tfhe_cbs_pbs_batch(c, d_lwe, batch, sc.lwe64, st);
         if (rc) return rc;
-        cbs_pfks(c, c->d_lwe64, 2 * batch, c->d_at, d_rgsw, st);
+        cbs_pfks(c, sc.lwe64, 2 * batch, sc.at, sc.pfkl_D, d_rgsw, st);
     }
     return check_launch("cbs pfks");
+}
+
+extern "C" int tfhe_cbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t batch, uint32_t* d_rgsw, void* stream) {
+    int rc = check_cbs(c, batch, c ? c->max_batch : 0);
+    if (rc) return rc;
+    if (!d_lwe || !d_rgsw) return fail(TFHE_E_PARAM, "cbs: NULL buffer");
+    if (!batch) return TFHE_OK;
+    return cbs_batch_impl(c, c->sc, d_lwe, batch, d_rgsw, (hipStream_t)stream);
 }
 
 extern "C" int tfhe_cbs_batch_from_device_lwe(tfhe_cbs_ctx* c, const uint64_t* d_a, const uint64_t* d_b,
@@ -555,11 +605,11 @@ extern "C" int tfhe_cbs_batch_from_device_lwe(tfhe_cbs_ctx* c, const uint64_t* d
     int rc = check_cbs(c, batch, c->max_batch);
     if (rc) return rc;
     // packed inputs in the spare AES state buffer (context scratch)
-    launch_pack_lwe64(d_a, d_b, batch, c->p.n, a_stride, c->d_state[1], (hipStream_t)stream);
+    launch_pack_lwe64(d_a, d_b, batch, c->p.n, a_stride, c->sc.state[1], (hipStream_t)stream);
     rc = check_launch("pack_lwe64");
-    if (!rc) rc = tfhe_cbs_batch(c, c->d_state[1], batch, c->d_rgsw, stream);
+    if (!rc) rc = tfhe_cbs_batch(c, c->sc.state[1], batch, c->sc.rgsw, stream);
     if (rc) return rc;
-    view->d_rgsw = c->d_rgsw;
+    view->d_rgsw = c->sc.rgsw;
     view->batch = batch;
     view->digits = 2;
     view->N = kN;
@@ -597,15 +647,20 @@ extern "C" int tfhe_lut8x32_eval_batch(tfhe_cbs_ctx* c, const double* d_rgsw_f, 
     return check_launch("lut8x32");
 }
 
+static int cbs_keyswitch(tfhe_cbs_ctx* c, uint32_t* ks_at, const uint32_t* d_lwe32, uint32_t count, uint32_t* d_out,
+                         hipStream_t st) {
+    if (launch_keyswitch32(d_lwe32, count, kN, c->p.n, c->p.ks_logB, c->p.ks_l, c->d_ksk2, ks_at, d_out, st,
+                           tuned(c->tune_ks, count, 0)))
+        return fail(TFHE_E_PARAM, "cbs keyswitch: unsupported shape");
+    return check_launch("keyswitch32 N=2048");
+}
+
 extern "C" int tfhe_cbs_keyswitch_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe32, uint32_t count, uint32_t* d_out,
                                         void* stream) {
     if (!c || !c->has_keys || !d_lwe32 || !d_out) return fail(TFHE_E_PARAM, "cbs keyswitch: bad argument");
     if (count > c->max_batch) return fail(TFHE_E_CAPACITY, "cbs keyswitch: count exceeds the context capacity");
     if (!count) return TFHE_OK;
-    if (launch_keyswitch32(d_lwe32, count, kN, c->p.n, c->p.ks_logB, c->p.ks_l, c->d_ksk2, c->d_ks_at, d_out,
-                           (hipStream_t)stream, tuned(c->tune_ks, count, 0)))
-        return fail(TFHE_E_PARAM, "cbs keyswitch: unsupported shape");
-    return check_launch("keyswitch32 N=2048");
+    return cbs_keyswitch(c, c->sc.ks_at, d_lwe32, count, d_out, (hipStream_t)stream);
 }
 
 namespace {
@@ -680,8 +735,8 @@ extern "C" int tfhe_cbs_autotune(tfhe_cbs_ctx* c, uint32_t blocks, int* choices,
     const int n = c->p.n, limbs = pbs_limbs(c->p), mb = pbs_mb(c->p);
     const float budget_ms = 250.f;
     if (!rc) {
-        launch_fill_hash32(c->d_state[0], (size_t)nb * (n + 1), 0x9e3779b9u, st);
-        launch_fill_hash32(c->d_lut32, (size_t)nb * (kN + 1), 0x7f4a7c15u, st);
+        launch_fill_hash32(c->sc.state[0], (size_t)nb * (n + 1), 0x9e3779b9u, st);
+        launch_fill_hash32(c->sc.lut32, (size_t)nb * (kN + 1), 0x7f4a7c15u, st);
         rc = check_launch("autotune inputs");
     }
     uint32_t full = 0, tail = 0;
@@ -703,25 +758,25 @@ extern "C" int tfhe_cbs_autotune(tfhe_cbs_ctx* c, uint32_t blocks, int* choices,
         if (model_form >= 0) {
             rc = tune_one(st, e0, e1, {model_form, 1 - model_form}, budget_ms,
                           [&](int f) {
-                              launch_pbs2(c->d_state[0], batch, n, c->d_bsk2f, limbs, mb, c->tables.T, c->d_lwe64,
+                              launch_pbs2(c->sc.state[0], batch, n, c->d_bsk2f, limbs, mb, c->tables.T, c->sc.lwe64,
                                           st, f);
                           },
                           form);
             if (!rc) c->tune_pbs[batch] = form;
         } else {
             // one form: its output still feeds the PFKS timing below
-            launch_pbs2(c->d_state[0], batch, n, c->d_bsk2f, limbs, mb, c->tables.T, c->d_lwe64, st);
+            launch_pbs2(c->sc.state[0], batch, n, c->d_bsk2f, limbs, mb, c->tables.T, c->sc.lwe64, st);
             rc = check_launch("autotune pbs");
         }
         choices[part] = form;
         if (rc) break;
         const uint32_t count = 2 * batch;
         if (c->p.pfks_linear) continue;  // one GEMM shape: nothing to choose
-        launch_pfks_digits(c->d_lwe64, count, c->d_at, st);
+        launch_pfks_digits(c->sc.lwe64, count, c->sc.at, st);
         int sl = 0;
         rc = tune_one(st, e0, e1, {pfks_model_slices(count, 2), 4, 6, 8, 12, 16, 24, 32}, budget_ms,
                       [&](int s) {
-                          launch_pfks32(c->d_at, count, c->d_pfks, pfks_words_per_u(), 0, 2, c->d_rgsw,
+                          launch_pfks32(c->sc.at, count, c->d_pfks, pfks_words_per_u(), 0, 2, c->sc.rgsw,
                                         4 * (size_t)kN, 2 * (size_t)kN, st, s);
                       },
                       sl);
@@ -735,8 +790,8 @@ extern "C" int tfhe_cbs_autotune(tfhe_cbs_ctx* c, uint32_t blocks, int* choices,
         int sl = 0;
         rc = tune_one(st, e0, e1, {ks_model_slices(nb, kN, n), 1, 2, 4, 6, 8, 12, 16, 24, 32, 48, 64}, budget_ms,
                       [&](int s) {
-                          launch_keyswitch32(c->d_lut32, nb, kN, n, c->p.ks_logB, c->p.ks_l, c->d_ksk2, c->d_ks_at,
-                                             c->d_state[1], st, s);
+                          launch_keyswitch32(c->sc.lut32, nb, kN, n, c->p.ks_logB, c->p.ks_l, c->d_ksk2, c->sc.ks_at,
+                                             c->sc.state[1], st, s);
                       },
                       sl);
         if (!rc) c->tune_ks[nb] = sl, choices[4] = sl;
@@ -802,6 +857,27 @@ extern "C" int tfhe_aes_set_round_keys(tfhe_cbs_ctx* c, const uint32_t* rk_bits)
     return rc;
 }
 
+// one AES round of `blocks` blocks with scratch set sc (sc.state[0 or 1] may be d_in or
+// d_out; the other is the key switch's output)
+static int aes_round_impl(tfhe_cbs_ctx* c, const CbsScratch& sc, int round, const uint32_t* d_in, uint32_t blocks,
+                          uint32_t* d_out, hipStream_t st) {
+    const uint32_t nb = 128 * blocks;
+    const int n = c->p.n;
+    int rc = cbs_batch_impl(c, sc, d_in, nb, sc.rgsw, st);
+    if (!rc) rc = tfhe_rgsw_to_fourier(c, sc.rgsw, nb, reinterpret_cast<double*>(sc.rgsw_f), st);
+    if (!rc) rc = tfhe_lut8_eval_batch(c, reinterpret_cast<const double*>(sc.rgsw_f), 16 * blocks, c->d_sbox_poly, 0, 8,
+                                       sc.lut32, st);
+    // SubBytes output into the spare state buffer, then the linear layer into d_out
+    uint32_t* sb = (d_out == sc.state[0]) ? sc.state[1] : sc.state[0];
+    if (!rc) rc = cbs_keyswitch(c, sc.ks_at, sc.lut32, nb, sb, st);
+    if (!rc) {
+        launch_aes_linear(sb, c->d_rk + (size_t)round * 128 * (n + 1), round == 10 ? c->d_lin_last : c->d_lin_mix,
+                          kLinSrc, blocks, n, d_out, st);
+        rc = check_launch("aes_linear");
+    }
+    return rc;
+}
+
 // One AES round (1..10) on device LWE states [blocks][128][n+1]:
 // SubBytes (circuit bootstrap of every bit, S-box vertical packing, key switch),
 // then ShiftRows, MixColumns (rounds 1..9) and AddRoundKey as LWE additions.
@@ -811,22 +887,7 @@ extern "C" int tfhe_aes_round_batch(tfhe_cbs_ctx* c, int round, const uint32_t* 
     if (!c->has_round_keys) return fail(TFHE_E_STATE, "aes_round: no round keys");
     if (blocks > c->max_blocks) return fail(TFHE_E_CAPACITY, "aes_round: blocks exceed the context capacity");
     if (!blocks) return TFHE_OK;
-    hipStream_t st = (hipStream_t)stream;
-    const uint32_t nb = 128 * blocks;
-    const int n = c->p.n;
-    int rc = tfhe_cbs_batch(c, d_in, nb, c->d_rgsw, stream);
-    if (!rc) rc = tfhe_rgsw_to_fourier(c, c->d_rgsw, nb, reinterpret_cast<double*>(c->d_rgsw_f), stream);
-    if (!rc) rc = tfhe_lut8_eval_batch(c, reinterpret_cast<const double*>(c->d_rgsw_f), 16 * blocks, c->d_sbox_poly, 0, 8,
-                                       c->d_lut32, stream);
-    // SubBytes output into d_out as scratch, then the linear layer into d_out via the spare state buffer
-    uint32_t* sb = (d_out == c->d_state[0]) ? c->d_state[1] : c->d_state[0];
-    if (!rc) rc = tfhe_cbs_keyswitch_batch(c, c->d_lut32, nb, sb, stream);
-    if (!rc) {
-        launch_aes_linear(sb, c->d_rk + (size_t)round * 128 * (n + 1), round == 10 ? c->d_lin_last : c->d_lin_mix,
-                          kLinSrc, blocks, n, d_out, st);
-        rc = check_launch("aes_linear");
-    }
-    return rc;
+    return aes_round_impl(c, c->sc, round, d_in, blocks, d_out, (hipStream_t)stream);
 }
 
 // the keystream's kernels after the counter upload: state init from round key 0
@@ -840,16 +901,16 @@ static int aes_round_plain(tfhe_cbs_ctx* c, int round, const uint32_t* d_in, uin
     const size_t rs = (size_t)c->p.n + 1, rgsw_words = (size_t)4 * 2 * kN, rgsw_f = (size_t)8 * 1024;
     int rc = TFHE_OK;
     for (uint32_t b = 0; b < nb && !rc; ++b)
-        rc = tfhe_cbs_batch(c, d_in + b * rs, 1, c->d_rgsw + b * rgsw_words, st);
+        rc = tfhe_cbs_batch(c, d_in + b * rs, 1, c->sc.rgsw + b * rgsw_words, st);
     for (uint32_t b = 0; b < nb && !rc; ++b)
-        rc = tfhe_rgsw_to_fourier(c, c->d_rgsw + b * rgsw_words, 1,
-                                  reinterpret_cast<double*>(c->d_rgsw_f + b * rgsw_f), st);
+        rc = tfhe_rgsw_to_fourier(c, c->sc.rgsw + b * rgsw_words, 1,
+                                  reinterpret_cast<double*>(c->sc.rgsw_f + b * rgsw_f), st);
     for (uint32_t lut = 0; lut < 16 * blocks && !rc; ++lut)
-        rc = tfhe_lut8_eval_batch(c, reinterpret_cast<const double*>(c->d_rgsw_f + (size_t)8 * lut * rgsw_f), 1,
-                                  c->d_sbox_poly, 0, 8, c->d_lut32 + (size_t)8 * lut * (kN + 1), st);
-    uint32_t* sb = (d_out == c->d_state[0]) ? c->d_state[1] : c->d_state[0];
+        rc = tfhe_lut8_eval_batch(c, reinterpret_cast<const double*>(c->sc.rgsw_f + (size_t)8 * lut * rgsw_f), 1,
+                                  c->d_sbox_poly, 0, 8, c->sc.lut32 + (size_t)8 * lut * (kN + 1), st);
+    uint32_t* sb = (d_out == c->sc.state[0]) ? c->sc.state[1] : c->sc.state[0];
     for (uint32_t b = 0; b < nb && !rc; ++b)
-        rc = tfhe_cbs_keyswitch_batch(c, c->d_lut32 + (size_t)b * (kN + 1), 1, sb + b * rs, st);
+        rc = tfhe_cbs_keyswitch_batch(c, c->sc.lut32 + (size_t)b * (kN + 1), 1, sb + b * rs, st);
     if (!rc) {
         launch_aes_linear(sb, c->d_rk + (size_t)round * 128 * rs, round == 10 ? c->d_lin_last : c->d_lin_mix, kLinSrc,
                           blocks, c->p.n, d_out, st);
@@ -858,17 +919,45 @@ static int aes_round_plain(tfhe_cbs_ctx* c, int round, const uint32_t* d_in, uin
     return rc;
 }
 
-static int aes_keystream_kernels(tfhe_cbs_ctx* c, uint32_t blocks, uint32_t* d_out, hipStream_t st,
-                                 bool plain = false) {
-    launch_aes_init(c->d_rk, c->d_ctr, blocks, c->p.n, c->d_state[0], st);
+// state init from round key 0 and the counter blocks d_ctr, then rounds 1..10 ping-ponging
+// the scratch set's state buffers
+static int aes_group(tfhe_cbs_ctx* c, const CbsScratch& sc, const uint8_t* d_ctr, uint32_t blocks, uint32_t* d_out,
+                     hipStream_t st, bool plain) {
+    launch_aes_init(c->d_rk, d_ctr, blocks, c->p.n, sc.state[0], st);
     int rc = check_launch("aes_init");
     int cur = 0;
     for (int round = 1; round <= 10 && !rc; ++round) {
-        uint32_t* dst = (round == 10) ? d_out : c->d_state[cur ^ 1];
-        rc = plain ? aes_round_plain(c, round, c->d_state[cur], blocks, dst, st)
-                   : tfhe_aes_round_batch(c, round, c->d_state[cur], blocks, dst, st);
+        uint32_t* dst = (round == 10) ? d_out : sc.state[cur ^ 1];
+        rc = plain ? aes_round_plain(c, round, sc.state[cur], blocks, dst, st)
+                   : aes_round_impl(c, sc, round, sc.state[cur], blocks, dst, st);
         cur ^= 1;
     }
+    return rc;
+}
+
+// The keystream's kernels after the counter upload.  An uneven batch (full rounds of
+// resident PBS workgroups plus a remainder of fewer blocks than one round holds) runs as
+// two groups: the full-round blocks on st, the remainder on the second stream with its own
+// scratch, so that the remainder's bootstraps fill the full group's idle slots instead of
+// a partial round of their own every AES round.  Blocks are independent, so both groups
+// give the words of the single-group keystream.
+static int aes_keystream_kernels(tfhe_cbs_ctx* c, uint32_t blocks, uint32_t* d_out, hipStream_t st,
+                                 bool plain = false) {
+    uint32_t b1 = 0;
+    if (!plain && c->tail_blocks) {
+        const uint32_t rb = 2 * device_cus() / 128;
+        b1 = blocks / rb * rb;
+        if (b1 == blocks || blocks - b1 > c->tail_blocks) b1 = 0;
+    }
+    if (!b1) return aes_group(c, c->sc, c->d_ctr, blocks, d_out, st, plain);
+    int rc = hipcheck(hipEventRecord(c->ev_gfork, st), "group fork");
+    if (!rc) rc = hipcheck(hipStreamWaitEvent(c->cstream, c->ev_gfork, 0), "group fork wait");
+    if (!rc) rc = aes_group(c, c->sc, c->d_ctr, b1, d_out, st, false);
+    if (!rc)
+        rc = aes_group(c, c->tail, c->d_ctr + (size_t)16 * b1, blocks - b1, d_out + (size_t)b1 * 128 * (c->p.n + 1),
+                       c->cstream, false);
+    if (!rc) rc = hipcheck(hipEventRecord(c->ev_gjoin, c->cstream), "group join");
+    if (!rc) rc = hipcheck(hipStreamWaitEvent(st, c->ev_gjoin, 0), "group join wait");
     return rc;
 }
 

@@ -446,6 +446,29 @@ def test_aes_ctr_transcipher_recovers_plaintext(ctx, okeys):
         assert np.array_equal(got, ks), mode
 
 
+def test_uneven_keystream_groups_equal_separate_calls(ctx, okeys):
+    """5 blocks are one full round of resident PBS workgroups (4 blocks) plus one: the
+    remainder runs as its own group on the second stream; the keystream equals the
+    keystreams of the 4 and the 1 block computed in separate calls, bit for bit."""
+    import torch
+    import tfhe_amd
+    key = bytes(range(7, 23))
+    ctx.set_round_keys(tfhe_amd.aes_encrypt_key_bits(okeys.s, 87, key))
+    rng = np.random.default_rng(15)
+    ctrs = [bytes(rng.integers(0, 256, 16, dtype=np.uint8)) for _ in range(5)]
+    d_all = torch.empty((5, 128, 637), dtype=torch.int32, device="cuda")
+    ctx.aes_ctr_keystream(ctrs, d_all)
+    d4 = torch.empty((4, 128, 637), dtype=torch.int32, device="cuda")
+    d1 = torch.empty((1, 128, 637), dtype=torch.int32, device="cuda")
+    ctx.aes_ctr_keystream(ctrs[:4], d4)
+    ctx.aes_ctr_keystream(ctrs[4:], d1)
+    torch.cuda.synchronize()
+    whole = from_dev(d_all)
+    assert np.array_equal(whole[:4], from_dev(d4))
+    assert np.array_equal(whole[4:], from_dev(d1))
+    assert _decrypt_blocks(okeys, whole, 5) == [O.aes128_encrypt(key, c) for c in ctrs]
+
+
 def test_set_keys_from_stateless_generators(ctx, okeys):
     """Keys made outside a context (tfhe_gen_bsk64 / tfhe_gen_pfks32 on the device,
     the oracle's KSK2 on the host) uploaded with tfhe_cbs_set_keys give the same
