@@ -108,8 +108,8 @@ constexpr int kLinSrc = 8;  // max sources per output bit of ShiftRows+MixColumn
 // full-round group runs on the caller's stream.
 struct CbsScratch {
     uint64_t* lwe64 = nullptr;  // [cap][2][N+1] PBS outputs
-    uint32_t* at = nullptr;     // [N+1][pfks_bpad(2 cap) + 256] lookup-PFKS inputs, rounded and transposed
-    int8_t* pfkl_D = nullptr;   // linear-PFKS digits, min(2 cap, kPfklChunk) inputs
+    uint32_t* at = nullptr;     // lookup form: [N+1][pfks_bpad(2 cap) + 256] PFKS inputs, rounded and transposed
+    int8_t* pfkl_D = nullptr;   // linear form: digits of min(2 cap, kPfklChunk) PFKS inputs
     uint32_t* ks_at = nullptr;  // [N][ks_bpad(cap)] key-switch digits
     uint32_t* rgsw = nullptr;   // [cap][4][2][N]
     cplx* rgsw_f = nullptr;     // [cap][8][1024]
@@ -122,7 +122,9 @@ int alloc_scratch(CbsScratch& s, size_t cap, int n, bool linear) {
     int rc = TFHE_OK;
     rc = rc ? rc : dmalloc((void**)&s.lwe64, cap * 2 * (kN + 1) * sizeof(uint64_t));
     // + 256 columns: a batch split into full rounds and a tail pads each part separately
-    rc = rc ? rc : dmalloc((void**)&s.at, (size_t)(kN + 1) * (pfks_bpad(2 * (uint32_t)cap) + 256) * sizeof(uint32_t));
+    if (!linear)
+        rc = rc ? rc
+                : dmalloc((void**)&s.at, (size_t)(kN + 1) * (pfks_bpad(2 * (uint32_t)cap) + 256) * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&s.ks_at, (size_t)kN * ks_bpad((uint32_t)cap) * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&s.rgsw, cap * 4 * 2 * kN * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&s.rgsw_f, cap * 8 * 1024 * sizeof(cplx));
@@ -577,8 +579,8 @@ static int cbs_batch_impl(tfhe_cbs_ctx* c, const CbsScratch& sc, const uint32_t*
         cbs_pfks(c, sc.lwe64, 2 * full, sc.at, sc.pfkl_D, d_rgsw, st);
         rc = hipcheck(hipStreamWaitEvent(st, c->ev_join, 0), "join wait");
         if (rc) return rc;
-        cbs_pfks(c, sc.lwe64 + full * lwe64_words, 2 * tail, sc.at + (size_t)(kN + 1) * pfks_bpad(2 * full),
-                 sc.pfkl_D, d_rgsw + full * rgsw_words, st);
+        uint32_t* at_tail = sc.at ? sc.at + (size_t)(kN + 1) * pfks_bpad(2 * full) : nullptr;  // lookup form only
+        cbs_pfks(c, sc.lwe64 + full * lwe64_words, 2 * tail, at_tail, sc.pfkl_D, d_rgsw + full * rgsw_words, st);
     } else {
         rc = tfhe_cbs_pbs_batch(c, d_lwe, batch, sc.lwe64, st);
         if (rc) return rc;
