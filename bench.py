@@ -194,7 +194,7 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0,
             gdt = time.perf_counter() - t0
             out[str(blocks)]["graph_s_per_launch"] = round(gdt, 4)
             out[str(blocks)]["graph_equals_eager"] = bool(torch.equal(d_ks[:blocks], eager))
-    if pbs_multibit == 1:  # the round's two big kernels on their own, at 17 blocks' shape (or less)
+    if pbs_multibit in (1, 3):  # the round's two big kernels on their own, at 17 blocks' shape (or less)
         out["kernels"] = aes_kernel_rooflines(ctx, pbs_approx, min(17, maxb) * 128, pfks_linear)
     if plain_blocks:  # the per-bootstrap form: every bootstrap, LUT and key switch a launch of one
         ctrs, plain, cipher = workload(plain_blocks)
@@ -212,7 +212,8 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0,
               flush=True)
     ctx.close()
     pbs = ("one FP64 product per digit polynomial, rounding error ~2^-33" if pbs_approx
-           else "2-limb exact") + (", two key bits per blind-rotation step" if pbs_multibit == 2 else "")
+           else "2-limb exact") + (", two key bits per blind-rotation step" if pbs_multibit == 2 else
+                                   ", one or two key bits per step by launch size" if pbs_multibit == 3 else "")
     pf = ("linear 40-bit key as an int8 GEMM on the matrix cores" if pfks_linear
           else "lookup key, LDS selects")
     return {"form": f"fp64 FFT: PBS on the 64-bit torus ({pbs}), PFKS ({pf}) and CMux LUT on the 32-bit torus "
@@ -236,10 +237,11 @@ def main():
                     help="comma-separated AES-128 CTR batch sizes to time ('' to skip)")
     ap.add_argument("--aes-plain-blocks", type=int, default=1,
                     help="AES blocks timed in the plain per-bootstrap form (exact PBS; 0 to skip)")
-    ap.add_argument("--aes-pbs", default="exact,exact-lin,approx-lin",
+    ap.add_argument("--aes-pbs", default="exact,exact-lin,approx-auto-lin",
                     help="comma-separated PBS forms of the AES runs: exact (two-limb), approx (one FP64 "
                          "product), approx-mb2 (one product, two key bits per step); exact-lin / approx-lin "
-                         "with the linear-key PFKS on the matrix cores")
+                         "with the linear-key PFKS on the matrix cores; approx-mb2-lin two key bits per step; "
+                         "approx-auto-lin one or two bits by launch size")
     ap.add_argument("--no-aes-tune", action="store_true",
                     help="keep the kernels' model launch shapes (no tfhe_cbs_autotune before each AES batch)")
     args = ap.parse_args()
@@ -411,7 +413,8 @@ def main():
                  "approx-mb2": ("aes128_ctr_one_product_pbs_2bit", 1, 2, 0),
                  "exact-lin": ("aes128_ctr_linear_pfks", 0, 1, 1),
                  "approx-lin": ("aes128_ctr_one_product_pbs_linear_pfks", 1, 1, 1),
-                 "approx-mb2-lin": ("aes128_ctr_one_product_pbs_2bit_linear_pfks", 1, 2, 1)}
+                 "approx-mb2-lin": ("aes128_ctr_one_product_pbs_2bit_linear_pfks", 1, 2, 1),
+                 "approx-auto-lin": ("aes128_ctr_one_product_pbs_auto_bits_linear_pfks", 1, 3, 1)}
         for f in [f for f in args.aes_pbs.split(",") if f]:
             if f not in forms:
                 sys.exit(f"--aes-pbs: unknown form {f!r} ({', '.join(forms)})")

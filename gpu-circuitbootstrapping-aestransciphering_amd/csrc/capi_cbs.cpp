@@ -22,14 +22,20 @@ bool cbs_params_supported(const tfhe_cbs_params* p) {
            p->pf_l == 8 && p->ks_logB == 2 && p->ks_l == 8 && p->n > 0 && p->n <= 1024 && p->lwe_nshift >= 1 &&
            p->lwe_nshift <= 31 && p->rlwe_nshift >= -31 && p->rlwe_nshift <= 31 && p->pf_nshift >= 1 &&
            p->pf_nshift <= 31 && (p->pbs_approx == 0 || p->pbs_approx == 1) && p->pbs_multibit >= 0 &&
-           p->pbs_multibit <= 2 && (p->pbs_multibit < 2 || (p->pbs_approx == 1 && p->n % 2 == 0)) &&
+           p->pbs_multibit <= 3 && (p->pbs_multibit < 2 || (p->pbs_approx == 1 && p->n % 2 == 0)) &&
            (p->pfks_linear == 0 || p->pfks_linear == 1);
 }
 
 int pbs_limbs(const tfhe_cbs_params& p) { return p.pbs_approx ? 1 : 2; }
 int pbs_mb(const tfhe_cbs_params& p) { return p.pbs_multibit == 2 ? 2 : 1; }
-// RGSWs of the BSK2: one per key bit, or three pattern keys per key-bit pair
-size_t bsk2_rgsw(const tfhe_cbs_params& p) { return pbs_mb(p) == 2 ? (size_t)p.n / 2 * 3 : (size_t)p.n; }
+// pbs_multibit = 3: both key forms, one or two key bits per step chosen per launch size
+bool pbs_auto(const tfhe_cbs_params& p) { return p.pbs_multibit == 3; }
+// RGSWs of the BSK2: one per key bit, three pattern keys per key-bit pair, or both
+size_t bsk2_rgsw(const tfhe_cbs_params& p) {
+    return pbs_auto(p) ? (size_t)p.n + (size_t)p.n / 2 * 3 : pbs_mb(p) == 2 ? (size_t)p.n / 2 * 3 : (size_t)p.n;
+}
+// first two-bit pattern RGSW in the BSK2 of an auto context (after the n one-bit RGSWs)
+size_t bsk2_mb2_first(const tfhe_cbs_params& p) { return pbs_auto(p) ? (size_t)p.n : 0; }
 // capacity bound: keeps every 32-bit count (2 x inputs for the PFKS) and every grid y
 // dimension (padded inputs / 64) of the path in range
 constexpr uint32_t kMaxBlocks = 4096;
@@ -320,7 +326,15 @@ extern "C" int tfhe_gen_bsk64(const tfhe_cbs_params* p, uint64_t seed, const int
     rc = hipcheck(hipMemcpy(hK.data(), d_rlwe_sk, kN * sizeof(int32_t), hipMemcpyDeviceToHost), "K2");
     cplx* d_K2f = nullptr;
     if (!rc) rc = k2_fourier(hK.data(), tables->T, &d_K2f, st);
-    if (!rc) launch_gen_bsk64(seed, p->n, p->rlwe_nshift, d_lwe_sk, d_rlwe_sk, d_K2f, tables->T, d_bsk, st, pbs_mb(*p));
+    if (!rc) {
+        if (pbs_auto(*p)) {  // one-bit RGSWs, then the two-bit pattern RGSWs
+            launch_gen_bsk64(seed, p->n, p->rlwe_nshift, d_lwe_sk, d_rlwe_sk, d_K2f, tables->T, d_bsk, st, 1);
+            launch_gen_bsk64(seed, p->n, p->rlwe_nshift, d_lwe_sk, d_rlwe_sk, d_K2f, tables->T,
+                             d_bsk + bsk2_mb2_first(*p) * 8 * 2 * kN, st, 2);
+        } else {
+            launch_gen_bsk64(seed, p->n, p->rlwe_nshift, d_lwe_sk, d_rlwe_sk, d_K2f, tables->T, d_bsk, st, pbs_mb(*p));
+        }
+    }
     if (!rc) rc = check_launch("gen_bsk64");
     if (!rc) rc = hipcheck(hipStreamSynchronize(st), "gen_bsk64 sync");
     (void)hipFree(d_K2f);
@@ -418,7 +432,13 @@ extern "C" int tfhe_cbs_keygen(tfhe_cbs_ctx* c, const int32_t* lwe_sk, uint64_t 
     if (!rc) rc = hipcheck(hipMemcpy(d_K64, hK64.data(), kN * sizeof(uint64_t), hipMemcpyHostToDevice), "K64");
     if (!rc) {
         launch_to_fourier2048(d_K64, 1, c->tables.T, d_K2f, nullptr);
-        launch_gen_bsk64(seed, p.n, p.rlwe_nshift, d_s, c->d_K2, d_K2f, c->tables.T, d_bsk, nullptr, pbs_mb(p));
+        if (pbs_auto(p)) {
+            launch_gen_bsk64(seed, p.n, p.rlwe_nshift, d_s, c->d_K2, d_K2f, c->tables.T, d_bsk, nullptr, 1);
+            launch_gen_bsk64(seed, p.n, p.rlwe_nshift, d_s, c->d_K2, d_K2f, c->tables.T,
+                             d_bsk + bsk2_mb2_first(p) * 8 * 2 * kN, nullptr, 2);
+        } else {
+            launch_gen_bsk64(seed, p.n, p.rlwe_nshift, d_s, c->d_K2, d_K2f, c->tables.T, d_bsk, nullptr, pbs_mb(p));
+        }
         if (pbs_limbs(p) == 2)
             launch_to_fourier2048(d_bsk, bsk2_rgsw(p) * 8 * 2, c->tables.T, c->d_bsk2f, nullptr);
         else
@@ -491,13 +511,24 @@ static int check_cbs(const tfhe_cbs_ctx* c, uint32_t batch, uint32_t cap) {
     return TFHE_OK;
 }
 
+// pbs_multibit = 3 without a tuned choice: two key bits per step from 16 rounds of resident
+// workgroups on (measured 6% faster at 64 rounds, 2% slower at 4 and 26% slower at a quarter
+// round), else one bit in the branch-free form
+static int auto_pbs_model(uint32_t batch) { return batch >= 16 * 2 * device_cus() ? 2 : 1; }
+
 extern "C" int tfhe_cbs_pbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t batch, uint64_t* d_out, void* stream) {
     int rc = check_cbs(c, batch, c ? c->max_batch : 0);
     if (rc) return rc;
     if (!d_lwe || !d_out) return fail(TFHE_E_PARAM, "pbs: NULL buffer");
     if (!batch) return TFHE_OK;
-    launch_pbs2(d_lwe, batch, c->p.n, c->d_bsk2f, pbs_limbs(c->p), pbs_mb(c->p), c->tables.T, d_out, (hipStream_t)stream,
-                tuned(c->tune_pbs, batch, -1));
+    int mb = pbs_mb(c->p), form = tuned(c->tune_pbs, batch, -1);
+    if (pbs_auto(c->p)) {  // form 2 = two key bits per step, else one bit with that FFT form
+        if (form < 0) form = auto_pbs_model(batch);
+        mb = form == 2 ? 2 : 1;
+        if (mb == 2) form = -1;
+    }
+    const cplx* bsk = c->d_bsk2f + (mb == 2 ? bsk2_mb2_first(c->p) * 8 * 2 * 1024 : 0);
+    launch_pbs2(d_lwe, batch, c->p.n, bsk, pbs_limbs(c->p), mb, c->tables.T, d_out, (hipStream_t)stream, form);
     return check_launch("pbs2");
 }
 
@@ -751,17 +782,24 @@ extern "C" int tfhe_cbs_autotune(tfhe_cbs_ctx* c, uint32_t blocks, int* choices,
         const uint32_t batch = parts[part];
         if (!batch) continue;
         if (batch >= many) {
-            choices[part] = pbs_model_form(batch, limbs, mb);
+            choices[part] = pbs_auto(c->p) ? auto_pbs_model(batch) : pbs_model_form(batch, limbs, mb);
             choices[2 + part] = pfks_model_slices(2 * batch, 2);
             continue;
         }
-        const int model_form = pbs_model_form(batch, limbs, mb);
+        const int model_form = pbs_auto(c->p) ? auto_pbs_model(batch) : pbs_model_form(batch, limbs, mb);
         int form = model_form;
         if (model_form >= 0) {
-            rc = tune_one(st, e0, e1, {model_form, 1 - model_form}, budget_ms,
+            // auto contexts: 2 = two key bits per step, 0 / 1 = one bit with that FFT form
+            const std::vector<int> cand = pbs_auto(c->p) ? std::vector<int>{model_form, 1, 0, 2}
+                                                         : std::vector<int>{model_form, 1 - model_form};
+            const cplx* bsk2 = c->d_bsk2f + bsk2_mb2_first(c->p) * 8 * 2 * 1024;
+            rc = tune_one(st, e0, e1, cand, budget_ms,
                           [&](int f) {
-                              launch_pbs2(c->sc.state[0], batch, n, c->d_bsk2f, limbs, mb, c->tables.T, c->sc.lwe64,
-                                          st, f);
+                              if (f == 2 && pbs_auto(c->p))
+                                  launch_pbs2(c->sc.state[0], batch, n, bsk2, limbs, 2, c->tables.T, c->sc.lwe64, st);
+                              else
+                                  launch_pbs2(c->sc.state[0], batch, n, c->d_bsk2f, limbs, mb, c->tables.T,
+                                              c->sc.lwe64, st, f);
                           },
                           form);
             if (!rc) c->tune_pbs[batch] = form;

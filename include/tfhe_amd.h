@@ -153,7 +153,10 @@ typedef struct tfhe_cbs_params {
      * main.cu:130, kernel.cu:4585-4771): 0 or 1 = one; 2 = two, with three
      * pattern keys per key-bit pair (RGSW of the indicators of (1,0), (0,1),
      * (1,1)): one decomposition and two inverse transforms per pair, 1.5x the
-     * MACs and key bytes.  2 needs pbs_approx = 1 and an even n. */
+     * MACs and key bytes.  2 needs pbs_approx = 1 and an even n.  3 = both key
+     * forms (417 MB), one or two bits per step chosen per launch size: two from 16
+     * rounds of resident workgroups on, where they are faster, or as
+     * tfhe_cbs_autotune picks (same conditions as 2). */
     int pbs_multibit;
     /* Private functional key switch form (configs: PFKS base 2^4, 8 digits):
      * 0 = lookup key, every digit multiple v = 1..15 its own torus-32 RLWE row
@@ -271,7 +274,8 @@ int tfhe_cbs_keyswitch_batch(tfhe_cbs_ctx* ctx, const uint32_t* d_lwe32, uint32_
  * compile time): times, with events on `stream`, the candidate shapes of one AES
  * round over `blocks` blocks and keeps the fastest in the context for later calls
  * of those sizes.  choices[5] receives the PBS FFT form (0 branchy, 1 branch-free,
- * -1 single form) of the full-round part and of the rest of tfhe_cbs_batch's split,
+ * -1 single form; 2 = two key bits per step in a pbs_multibit = 3 context) of the
+ * full-round part and of the rest of tfhe_cbs_batch's split,
  * the PFKS i-slices of the same two parts, and the N=2048 key switch's i-slices;
  * -1 where a part does not exist.  A part of 16 or more rounds of resident
  * workgroups (8192 inputs on 256 CUs), or whose model shape takes over 250 ms per

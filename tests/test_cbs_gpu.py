@@ -750,6 +750,43 @@ def test_aes128_linear_pfks(lctx, okeys):
     assert _decrypt_blocks(okeys, from_dev(d_ks), 5) == plain
 
 
+def test_pbs_auto_multibit_matches_fixed_forms(gpu, okeys):
+    """pbs_multibit = 3 holds both key forms and picks per launch size: a 128-input
+    launch equals the one-bit context's PBS and an 8192-input launch (16 rounds of
+    resident workgroups) the two-bit context's, bit for bit; AES through it decrypts."""
+    import torch
+    import tfhe_amd
+    outs = {}
+    rng = np.random.default_rng(29)
+    bits = rng.integers(0, 2, 64)
+    base = _bits_cts(okeys, bits, 1200)
+    cts = np.tile(base, (128, 1))  # 8192 inputs from 64 encryptions
+    d_cts = to_dev32(cts)
+    for mb in (1, 2, 3):
+        c = tfhe_amd.CbsContext(max_blocks=64, params=tfhe_amd.cbs_params(pbs_approx=1, pbs_multibit=mb,
+                                                                           pfks_linear=1))
+        try:
+            c.keygen(okeys.s, SEED)
+            for nb in (128, 8192):
+                d_out = torch.empty((nb, 2, 2049), dtype=torch.int64, device="cuda")
+                c.pbs_batch(d_cts[:nb].contiguous(), d_out)
+                torch.cuda.synchronize()
+                outs[(mb, nb)] = from_dev(d_out, np.uint64)
+            if mb == 3:
+                key = bytes(range(16))
+                c.set_round_keys(tfhe_amd.aes_encrypt_key_bits(okeys.s, 77, key))
+                ctr = [bytes.fromhex("00112233445566778899aabbccddeeff")]
+                d_ks = torch.empty((1, 128, 637), dtype=torch.int32, device="cuda")
+                c.aes_ctr_keystream(ctr, d_ks)
+                torch.cuda.synchronize()
+                assert _decrypt_blocks(okeys, from_dev(d_ks), 1)[0].hex() == "69c4e0d86a7b0430d8cdb78070b4c55a"
+        finally:
+            c.close()
+    assert np.array_equal(outs[(3, 128)], outs[(1, 128)])
+    assert np.array_equal(outs[(3, 8192)], outs[(2, 8192)])
+    assert not np.array_equal(outs[(1, 128)], outs[(2, 8192)][:128])  # the two forms differ in their words
+
+
 def test_cbs_error_paths(gpu):
     """Capacity, argument and state errors come back as codes (the reference throws,
     cirbts-batch.cu:97-100); nothing is launched."""
