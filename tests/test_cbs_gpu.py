@@ -446,12 +446,14 @@ def test_aes_ctr_transcipher_recovers_plaintext(ctx, okeys):
         assert np.array_equal(got, ks), mode
 
 
-def test_uneven_keystream_groups_equal_separate_calls(ctx, okeys):
-    """5 blocks are one full round of resident PBS workgroups (4 blocks) plus one: the
-    remainder runs as its own group on the second stream; the keystream equals the
-    keystreams of the 4 and the 1 block computed in separate calls, bit for bit."""
+def test_uneven_keystream_groups_equal_separate_calls(lctx, okeys):
+    """5 blocks are one full round of resident PBS workgroups (4 blocks) plus one: in a
+    linear-PFKS context the remainder runs as its own group on the second stream; the
+    keystream equals the keystreams of the 4 and the 1 block computed in separate calls,
+    bit for bit."""
     import torch
     import tfhe_amd
+    ctx = lctx
     key = bytes(range(7, 23))
     ctx.set_round_keys(tfhe_amd.aes_encrypt_key_bits(okeys.s, 87, key))
     rng = np.random.default_rng(15)
