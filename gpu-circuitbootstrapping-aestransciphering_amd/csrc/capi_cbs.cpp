@@ -239,18 +239,21 @@ extern "C" int tfhe_cbs_ctx_create(const tfhe_cbs_params* params, uint32_t max_b
     rc = rc ? rc : dmalloc((void**)&c->d_ksk2, ksk2_words(p) * sizeof(uint32_t));
     rc = rc ? rc : alloc_scratch(c->sc, B, p.n, p.pfks_linear != 0);
     {
-        // uneven AES batches: blocks beyond the last full round of resident PBS workgroups
-        // (2 x CUs inputs, rb blocks) run as their own group on the second stream
+        // uneven AES batches of a linear-PFKS context: blocks beyond the last full round of
+        // resident PBS workgroups (2 x CUs inputs, rb blocks) run as their own group on the
+        // second stream
         const uint32_t slots = 2 * device_cus(), rb = slots / 128;
-        if (!rc && slots % 128 == 0 && rb >= 2 && max_blocks > rb) {
+        if (!rc && p.pfks_linear && slots % 128 == 0 && rb >= 2 && max_blocks > rb) {
             c->tail_blocks = rb - 1;
             rc = alloc_scratch(c->tail, 128 * (size_t)c->tail_blocks, p.n, p.pfks_linear != 0);
-            if (!rc) rc = hipcheck(hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking), "cbs stream");
-            if (!rc) rc = hipcheck(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "cbs event");
-            if (!rc) rc = hipcheck(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "cbs event");
-            if (!rc) rc = hipcheck(hipEventCreateWithFlags(&c->ev_gfork, hipEventDisableTiming), "cbs event");
-            if (!rc) rc = hipcheck(hipEventCreateWithFlags(&c->ev_gjoin, hipEventDisableTiming), "cbs event");
         }
+        // the second stream and its events exist before any call, so that no call made
+        // under stream capture creates them
+        if (!rc) rc = hipcheck(hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking), "cbs stream");
+        if (!rc) rc = hipcheck(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "cbs event");
+        if (!rc) rc = hipcheck(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "cbs event");
+        if (!rc) rc = hipcheck(hipEventCreateWithFlags(&c->ev_gfork, hipEventDisableTiming), "cbs event");
+        if (!rc) rc = hipcheck(hipEventCreateWithFlags(&c->ev_gjoin, hipEventDisableTiming), "cbs event");
     }
     rc = rc ? rc : dmalloc((void**)&c->d_sbox_poly, kN * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_lin_mix, 128 * kLinSrc * sizeof(int16_t));
@@ -594,12 +597,6 @@ static int cbs_batch_impl(tfhe_cbs_ctx* c, const CbsScratch& sc, const uint32_t*
     cbs_split(batch, full, tail);
     int rc = TFHE_OK;
     if (full && tail) {
-        if (!c->cstream) {
-            rc = hipcheck(hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking), "cbs stream");
-            if (!rc) rc = hipcheck(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "cbs event");
-            if (!rc) rc = hipcheck(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "cbs event");
-            if (rc) return rc;
-        }
         rc = tfhe_cbs_pbs_batch(c, d_lwe, full, sc.lwe64, st);
         if (!rc) rc = hipcheck(hipEventRecord(c->ev_fork, st), "fork");
         if (!rc) rc = hipcheck(hipStreamWaitEvent(c->cstream, c->ev_fork, 0), "fork wait");
@@ -986,7 +983,7 @@ static int aes_keystream_kernels(tfhe_cbs_ctx* c, uint32_t blocks, uint32_t* d_o
     // (the lookup-form PFKS is long enough to hide the partial PBS round behind it inside
     // tfhe_cbs_batch; grouping measured slower there: 17 blocks 15.3 against 16.1 blocks/s)
     uint32_t b1 = 0;
-    if (!plain && c->tail_blocks && c->p.pfks_linear) {
+    if (!plain && c->tail_blocks) {  // only linear-PFKS contexts hold the second scratch set
         const uint32_t rb = 2 * device_cus() / 128;
         b1 = blocks / rb * rb;
         if (b1 == blocks || blocks - b1 > c->tail_blocks) b1 = 0;
