@@ -930,9 +930,11 @@ typedef int pfkl_v16i __attribute__((ext_vector_type(16)));
 // C[8 (j / 4) + 4 (l / 32) + j % 4][l % 32].
 // Grid x = Mpad / 128 input tiles, y = nu * 2N / 64 column tiles.  Wave w computes
 // 64 inputs (m0 + 0..63, two 32-row blocks) x 32 columns x 5 slices: ten 16-register
-// accumulators, one 16-byte digit load per row block and one key load per slice
-// per 32 k.
-__global__ __launch_bounds__(256, 2) void k_pfkl_gemm(const int8_t* __restrict__ D, uint32_t B,
+// accumulators (160 registers, held in AGPRs), one 16-byte digit load per row block and
+// one key load per slice per 32 k.  One wave per SIMD leaves the registers for three
+// operand sets in flight: the loads are latency-bound, and two steps of prefetch beat
+// two waves per SIMD with one (3.34 against 3.78 ms per 17-block round).
+__global__ __launch_bounds__(256, 1) void k_pfkl_gemm(const int8_t* __restrict__ D, uint32_t B,
                                                   const int8_t* __restrict__ KT, int u0,
                                                   uint32_t* __restrict__ out, size_t out_stride,
                                                   size_t out_u_offset) {
@@ -953,13 +955,13 @@ __global__ __launch_bounds__(256, 2) void k_pfkl_gemm(const int8_t* __restrict__
     for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int sl = 0; sl < PFKL_SLICES; ++sl) acc[r][sl] = pfkl_v16i{};
-    // two operand sets: X holds even steps, Y odd ones; each set is loaded one step
-    // ahead of its products (the scheduling barriers keep the loads in place), so a
-    // load has the other set's ten products (~320 cycles) to arrive.  513 steps: 256
-    // pairs and a last even step.
+    // three operand sets rotate over the steps (513 = 3 x 171), each loaded two steps
+    // ahead of its products (the scheduling barriers keep the loads in place), so a load
+    // has twenty products (~640 cycles) to arrive.  Loads past the last step reread step
+    // 0 (in bounds, unused).
     struct Ops { pfkl_v4i a[2], b[PFKL_SLICES]; };
-    auto load = [&](Ops& x, int k) {  // k multiple of 32: fragment step k / 32 at byte 32 k
-        const size_t fo = (size_t)k * 32;
+    auto load = [&](Ops& x, int ks) {  // fragment step ks at byte 1024 ks
+        const size_t fo = (size_t)(ks < PFKL_KS ? ks : 0) * 1024;
         x.a[0] = *reinterpret_cast<const pfkl_v4i*>(a0 + fo);
         x.a[1] = *reinterpret_cast<const pfkl_v4i*>(a1 + fo);
 #pragma unroll
@@ -972,21 +974,25 @@ __global__ __launch_bounds__(256, 2) void k_pfkl_gemm(const int8_t* __restrict__
             acc[1][sl] = __builtin_amdgcn_mfma_i32_32x32x32_i8(x.a[1], x.b[sl], acc[1][sl], 0, 0, 0);
         }
     };
-    static_assert(PFKL_KP / 32 % 2 == 1, "an odd step count: pairs plus one");
-    Ops X, Y;
+    static_assert(PFKL_KS % 3 == 0, "the step count splits into triples");
+    Ops X, Y, Z;
     load(X, 0);
+    load(Y, 1);
 #pragma unroll 1
-    for (int k = 0; k + 64 <= PFKL_KP - 32; k += 64) {
-        load(Y, k + 32);
+    for (int ks = 0; ks < PFKL_KS; ks += 3) {
+        load(Z, ks + 2);
         __builtin_amdgcn_sched_barrier(0);
         mma(X);
         __builtin_amdgcn_sched_barrier(0);
-        load(X, k + 64);
+        load(X, ks + 3);
         __builtin_amdgcn_sched_barrier(0);
         mma(Y);
         __builtin_amdgcn_sched_barrier(0);
+        load(Y, ks + 4);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(Z);
+        __builtin_amdgcn_sched_barrier(0);
     }
-    mma(X);
     uint32_t* o = out + (size_t)uu * out_u_offset + col0 + (lane & 31);
 #pragma unroll
     for (int r = 0; r < 2; ++r)
