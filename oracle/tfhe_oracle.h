@@ -11,6 +11,15 @@
  * level-0 LWE, Z/2^64 for the N=2048 circuit-bootstrapping level); the
  * reference's formulas (decomposition, CGGI accumulator update, sample
  * extraction, key switching, CMux) are followed in that domain.
+ *
+ * Each function is pinned by a check that does not retype it (tests/test_oracle_*.py):
+ * published vectors (Philox, FIPS-197), schoolbook products, closed forms
+ * (modswitch, monomials, blind-rotation phase), decryption of every probed key
+ * row to its message, and explicit sums over key tables.  What no check pins is
+ * parity with the reference's own ciphertext words: its key generator, noise
+ * sampler and PRNG stream live in the absent OpenFHE fork, so the samplers here
+ * (Philox domains, Irwin-Hall noise) are this library's own and the ciphertext
+ * words are "parity unpinned" against the reference.
  */
 #ifndef TFHE_ORACLE_H
 #define TFHE_ORACLE_H
