@@ -164,15 +164,26 @@ static int circuit_bootstrap_and_aes(const std::vector<int32_t>& sk) {
     }
     std::printf("AES-128 CTR transcipher: 2 random blocks ok\n");
 
-    // the same FIPS-197 block with the one-product PBS and the linear-key PFKS
-    GPUCirBTSContext fast(CirBTSParams(1, 1, 1), 128);
+    // the bench's fastest form: one-product PBS with one or two key bits per step chosen by
+    // launch size, linear-key PFKS; 5 blocks (an uneven batch: a group of 4 and one of 1),
+    // the FIPS-197 block first, the others against the library's plain AES
+    GPUCirBTSContext fast(CirBTSParams(1, 3, 1), 5 * 128);
     fast.KeyGen(sk, 7);
     fast.SetRoundKeys(sk, 77, key);
-    const auto ks = fast.AesCtrKeystream({pt});
-    uint8_t got[16] = {0};
-    for (int bit = 0; bit < 128; ++bit) got[bit / 8] |= (uint8_t)(LWEDecrypt(sk, ks[0][bit], 2) << (bit % 8));
-    EXPECT(std::memcmp(got, want, 16) == 0, "AES-128 FIPS-197 C.1, one-product PBS + linear PFKS");
-    std::printf("AES-128 CTR keystream, one-product PBS + linear PFKS: FIPS-197 C.1 ok\n");
+    std::vector<std::array<uint8_t, 16>> ctr5(5);
+    ctr5[0] = pt;
+    for (int k = 1; k < 5; ++k)
+        for (int j = 0; j < 16; ++j) ctr5[k][j] = (uint8_t)(rng() & 0xFF);
+    const auto ks5 = fast.AesCtrKeystream(ctr5);
+    for (int k = 0; k < 5; ++k) {
+        uint8_t got[16] = {0}, plain_ks[16];
+        for (int bit = 0; bit < 128; ++bit) got[bit / 8] |= (uint8_t)(LWEDecrypt(sk, ks5[k][bit], 2) << (bit % 8));
+        tfhe_aes128_encrypt_plain(key.data(), ctr5[k].data(), plain_ks);
+        EXPECT(std::memcmp(got, plain_ks, 16) == 0, "AES-128 keystream block %d, one-product PBS (auto key bits) + "
+               "linear PFKS", k);
+        if (k == 0) EXPECT(std::memcmp(got, want, 16) == 0, "AES-128 FIPS-197 C.1, one-product PBS + linear PFKS");
+    }
+    std::printf("AES-128 CTR keystream, one-product PBS (auto key bits) + linear PFKS: 5 blocks ok (FIPS-197 C.1)\n");
     return 0;
 }
 

@@ -789,6 +789,46 @@ def test_pbs_auto_multibit_matches_fixed_forms(gpu, okeys):
     assert not np.array_equal(outs[(1, 128)], outs[(2, 8192)][:128])  # the two forms differ in their words
 
 
+def test_empty_batches_leave_outputs_untouched(ctx, okeys):
+    """Every batched entry point takes a count of 0 as a no-op that returns OK and
+    launches nothing: outputs filled with a sentinel keep it."""
+    import torch
+    import tfhe_amd
+    L, P, C = tfhe_amd.lib(), tfhe_amd._ptr, tfhe_amd.check
+    ctx.set_round_keys(tfhe_amd.aes_encrypt_key_bits(okeys.s, 77, bytes(range(16))))
+    SENT = 0x5A5A5A5A
+    lwe = torch.full((2, 637), SENT, dtype=torch.int32, device="cuda")
+    pbs = torch.full((2, 2, 2049), SENT, dtype=torch.int64, device="cuda")
+    rgsw = torch.full((2, 4, 2, 2048), SENT, dtype=torch.int32, device="cuda")
+    rgsw_f = torch.full((2, 8, 1024, 2), 0.5, dtype=torch.float64, device="cuda")
+    lut = torch.full((2, 32, 2049), SENT, dtype=torch.int32, device="cuda")
+    poly64 = torch.full((2, 2048), SENT, dtype=torch.int64, device="cuda")
+    poly32 = torch.full((4, 2048), SENT, dtype=torch.int32, device="cuda")
+    state = torch.full((1, 128, 637), SENT, dtype=torch.int32, device="cuda")
+    outs = [lwe, pbs, rgsw, rgsw_f, lut, poly64, poly32, state]
+    before = [o.clone() for o in outs]
+    h = ctx.h
+    C(L.tfhe_cbs_batch(h, P(lwe), 0, P(rgsw), None))
+    C(L.tfhe_cbs_pbs_batch(h, P(lwe), 0, P(pbs), None))
+    C(L.tfhe_cbs_pfks_batch(h, 0, P(pbs), 0, 4096, P(rgsw), None))
+    C(L.tfhe_rgsw_to_fourier(h, P(rgsw), 0, P(rgsw_f), None))
+    C(L.tfhe_lut8_eval_batch(h, P(rgsw_f), 0, P(poly32), 0, 8, P(lut), None))
+    C(L.tfhe_lut8x32_eval_batch(h, P(rgsw_f), 0, P(poly32), P(lut), None))
+    C(L.tfhe_cbs_keyswitch_batch(h, P(lut), 0, P(lwe), None))
+    C(L.tfhe_polymul2048_batch(P(poly32), P(poly64), 0, P(poly64), None))
+    C(L.tfhe_polymul2048_approx_batch(P(poly32), P(poly64), 0, P(poly64), None))
+    C(L.tfhe_aes_round_batch(h, 1, P(state), 0, P(state), None))
+    C(L.tfhe_lwe_xor_public(P(state), P(state), 0, 636, None))
+    empty = np.zeros(1, np.uint8)
+    for fn in (L.tfhe_aes_ctr_keystream_batch, L.tfhe_aes_ctr_keystream_graph, L.tfhe_aes_ctr_keystream_plain):
+        C(fn(h, empty, 0, P(state), None))
+    for mode in (0, 1, 2):
+        C(L.tfhe_aes_ctr_transcipher_batch(h, mode, empty, empty, 0, P(state), None))
+    torch.cuda.synchronize()
+    for k, (o, b) in enumerate(zip(outs, before)):
+        assert torch.equal(o, b), k
+
+
 def test_cbs_error_paths(gpu):
     """Capacity, argument and state errors come back as codes (the reference throws,
     cirbts-batch.cu:97-100); nothing is launched."""

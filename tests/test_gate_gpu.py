@@ -204,6 +204,23 @@ def test_gpu_keygen_context_gates(gpu):
     c.close()
 
 
+def test_empty_batches_leave_outputs_untouched(ctx):
+    """A count of 0 is a no-op for the batched gate entry points: OK, nothing launched,
+    the sentinel in the outputs kept."""
+    import torch
+    import tfhe_amd
+    L, P, C = tfhe_amd.lib(), tfhe_amd._ptr, tfhe_amd.check
+    ct1 = torch.full((2, 637), 0x5A5A5A5A, dtype=torch.int32, device="cuda")
+    ct2, out = ct1.clone(), ct1.clone()
+    lweN = torch.full((2, 1025), 0x5A5A5A5A, dtype=torch.int32, device="cuda")
+    before = (ct1.clone(), lweN.clone())
+    C(L.tfhe_gate_eval_batch(ctx.h, tfhe_amd.NAND, P(ct1), P(ct2), 0, P(out), None))
+    C(L.tfhe_gate_blind_rotate_batch(ctx.h, tfhe_amd.XOR, P(ct1), P(ct2), 0, P(lweN), None))
+    C(L.tfhe_gate_keyswitch_batch(ctx.h, P(lweN), 0, P(out), None))
+    torch.cuda.synchronize()
+    assert torch.equal(out, before[0]) and torch.equal(lweN, before[1])
+
+
 def test_error_paths(ctx, okeys):
     import torch
     import tfhe_amd
