@@ -228,7 +228,7 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0,
     ctx.close()
     pbs = ("one FP64 product per digit polynomial, rounding error ~2^-33" if pbs_approx
            else "2-limb exact") + (", two key bits per blind-rotation step" if pbs_multibit == 2 else
-                                   ", one or two key bits per step by launch size" if pbs_multibit == 3 else "")
+                                   ", two key bits per step unless the autotuner measured one bit faster" if pbs_multibit == 3 else "")
     pf = ("linear 40-bit key as an int8 GEMM on the matrix cores" if pfks_linear
           else "lookup key, LDS selects")
     return {"form": f"fp64 FFT: PBS on the 64-bit torus ({pbs}), PFKS ({pf}) and CMux LUT on the 32-bit torus "
@@ -256,7 +256,7 @@ def main():
                     help="comma-separated PBS forms of the AES runs: exact (two-limb), approx (one FP64 "
                          "product), approx-mb2 (one product, two key bits per step); exact-lin / approx-lin "
                          "with the linear-key PFKS on the matrix cores; approx-mb2-lin two key bits per step; "
-                         "approx-auto-lin one or two bits by launch size")
+                         "approx-auto-lin both key forms, two bits per step unless tuned to one")
     ap.add_argument("--no-aes-tune", action="store_true",
                     help="keep the kernels' model launch shapes (no tfhe_cbs_autotune before each AES batch)")
     args = ap.parse_args()

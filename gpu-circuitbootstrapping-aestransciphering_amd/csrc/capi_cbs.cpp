@@ -514,10 +514,12 @@ static int check_cbs(const tfhe_cbs_ctx* c, uint32_t batch, uint32_t cap) {
     return TFHE_OK;
 }
 
-// pbs_multibit = 3 without a tuned choice: two key bits per step from 16 rounds of resident
-// workgroups on (measured 6% faster at 64 rounds, 2% slower at 4 and 26% slower at a quarter
-// round), else one bit in the branch-free form
-static int auto_pbs_model(uint32_t batch) { return batch >= 16 * 2 * device_cus() ? 2 : 1; }
+// pbs_multibit = 3 without a tuned choice: two key bits per step at every size (with the key
+// loads batched it measured faster than one bit at 128, 2048 and 8192 inputs: 5.9 / 34.4 /
+// 136 ms against 8.1 / 44.4 / 178), in the wide form (3) for launches of at most one
+// workgroup per CU, else at two waves per SIMD (2).  Forms 0 and 1 are one bit per step
+// with that FFT form.
+static int auto_pbs_model(uint32_t batch) { return pbs_model_form(batch, 1, 2) == 1 ? 3 : 2; }
 
 extern "C" int tfhe_cbs_pbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t batch, uint64_t* d_out, void* stream) {
     int rc = check_cbs(c, batch, c ? c->max_batch : 0);
@@ -525,10 +527,10 @@ extern "C" int tfhe_cbs_pbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32
     if (!d_lwe || !d_out) return fail(TFHE_E_PARAM, "pbs: NULL buffer");
     if (!batch) return TFHE_OK;
     int mb = pbs_mb(c->p), form = tuned(c->tune_pbs, batch, -1);
-    if (pbs_auto(c->p)) {  // form 2 = two key bits per step, else one bit with that FFT form
+    if (pbs_auto(c->p)) {  // forms 2 and 3 = two key bits per step, else one bit with that FFT form
         if (form < 0) form = auto_pbs_model(batch);
-        mb = form == 2 ? 2 : 1;
-        if (mb == 2) form = -1;
+        mb = form >= 2 ? 2 : 1;
+        if (mb == 2) form -= 2;
     }
     const cplx* bsk = c->d_bsk2f + (mb == 2 ? bsk2_mb2_first(c->p) * 8 * 2 * 1024 : 0);
     launch_pbs2(d_lwe, batch, c->p.n, bsk, pbs_limbs(c->p), mb, c->tables.T, d_out, (hipStream_t)stream, form);
@@ -786,14 +788,17 @@ extern "C" int tfhe_cbs_autotune(tfhe_cbs_ctx* c, uint32_t blocks, int* choices,
         const int model_form = pbs_auto(c->p) ? auto_pbs_model(batch) : pbs_model_form(batch, limbs, mb);
         int form = model_form;
         if (model_form >= 0) {
-            // auto contexts: 2 = two key bits per step, 0 / 1 = one bit with that FFT form
-            const std::vector<int> cand = pbs_auto(c->p) ? std::vector<int>{model_form, 1, 0, 2}
-                                                         : std::vector<int>{model_form, 1 - model_form};
+            // auto contexts: 2 / 3 = two key bits per step (two waves per SIMD / wide),
+            // 0 / 1 = one bit with that FFT form
+            std::vector<int> cand{model_form};
+            for (int f : pbs_auto(c->p) ? std::vector<int>{3, 2, 1, 0} : std::vector<int>{1 - model_form})
+                if (f != model_form) cand.push_back(f);
             const cplx* bsk2 = c->d_bsk2f + bsk2_mb2_first(c->p) * 8 * 2 * 1024;
             rc = tune_one(st, e0, e1, cand, budget_ms,
                           [&](int f) {
-                              if (f == 2 && pbs_auto(c->p))
-                                  launch_pbs2(c->sc.state[0], batch, n, bsk2, limbs, 2, c->tables.T, c->sc.lwe64, st);
+                              if (f >= 2 && pbs_auto(c->p))
+                                  launch_pbs2(c->sc.state[0], batch, n, bsk2, limbs, 2, c->tables.T, c->sc.lwe64, st,
+                                              f - 2);
                               else
                                   launch_pbs2(c->sc.state[0], batch, n, c->d_bsk2f, limbs, mb, c->tables.T,
                                               c->sc.lwe64, st, f);

@@ -455,9 +455,13 @@ __device__ __forceinline__ uint64_t torus64_from_double(double y) {
 // (1,0), (0,1), (1,1) with e_p = m1, m2, m1 + m2: exactly one indicator is 1 (or
 // none), so this equals the two single-bit steps.  One decomposition and two
 // inverse transforms serve both bits; the factor of pattern p is applied to the
-// digit transforms before they meet that pattern's key row.
-template <bool BF, int LIMBS, int MB = 1>
-__global__ __launch_bounds__(256, MB == 2 ? 2 : 1) void k_pbs2_manylut(const uint32_t* __restrict__ lwe, uint32_t batch, int n,
+// digit transforms before they meet that pattern's key row.  The pattern keys are
+// the step's largest load stream (24 complex values per lane and digit row), and the
+// step waits on it unless many loads are in flight: WIDE = false keeps two waves per
+// SIMD and issues each pattern's 8 values as one batch; WIDE = true (launches of at
+// most one workgroup per CU) issues all 24 at once with one wave per SIMD.
+template <bool BF, int LIMBS, int MB = 1, bool WIDE = false>
+__global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylut(const uint32_t* __restrict__ lwe, uint32_t batch, int n,
                                                      const cplx* __restrict__ bskf, FFT1024Tables Tg,
                                                      uint64_t* __restrict__ out) {
     __shared__ cplx tl[768];
@@ -470,6 +474,7 @@ __global__ __launch_bounds__(256, MB == 2 ? 2 : 1) void k_pbs2_manylut(const uin
     cplx trw[4];
     load_trw(Tg, w, lane, trw);
     static_assert(MB == 1 || LIMBS == 1, "two key bits per step only in the one-product form");
+    static_assert(!WIDE || MB == 2, "the wide key batch is a form of the two-bit step");
     constexpr bool RT = LIMBS == 1 && MB == 1;  // registers to spare for the p2 / p3 twiddles
     const Tw256Regs tw = RT ? load_tw256(Tg, lane) : Tw256Regs{};
     cplx* lds = lds_all[w];
@@ -541,24 +546,39 @@ __global__ __launch_bounds__(256, MB == 2 ? 2 : 1) void k_pbs2_manylut(const uin
                         cplx t1[4], y[4];
 #pragma unroll
                         for (int s = 0; s < 4; ++s) t1[s] = cmul(x[s], zp[0][s]);
+                        // key values of pattern q (rows in the order (1,0), (0,1), (1,1)) for
+                        // both outputs; the empty asm statements close a batch of loads, which
+                        // the compiler would otherwise split, waiting on each load in turn
+                        constexpr int KB = WIDE ? 3 : 1;  // patterns per batch of loads
+                        cplx kv[KB][2][4];
 #pragma unroll
-                        for (int q = 0; q < 3; ++q) {
-                            const int pt = q == 0 ? 0 : q == 1 ? 2 : 1;
+                        for (int q0 = 0; q0 < 3; q0 += KB) {
 #pragma unroll
-                            for (int s = 0; s < 4; ++s)
-                                y[s] = csub(q == 0 ? t1[s] : q == 1 ? cmul(t1[s], zp[1][s]) : cmul(x[s], zp[1][s]), x[s]);
-                            const cplx* kr = key + ((size_t)pt * PBS_ROWS + r) * 2 * CB_M + (w * 4) * 64 + lane;
-                            // each output's 4 key values as one batch of loads: under this
-                            // kernel's register target the compiler otherwise waits on each
+                            for (int b = 0; b < KB; ++b) {
+                                const int q = q0 + b, pt = q == 0 ? 0 : q == 1 ? 2 : 1;
+                                const cplx* kr = key + ((size_t)pt * PBS_ROWS + r) * 2 * CB_M + (w * 4) * 64 + lane;
 #pragma unroll
-                            for (int o = 0; o < 2; ++o) {
-                                cplx kv[4];
+                                for (int o = 0; o < 2; ++o)
 #pragma unroll
-                                for (int s = 0; s < 4; ++s) kv[s] = kr[o * CB_M + s * 64];
+                                    for (int s = 0; s < 4; ++s) kv[b][o][s] = kr[o * CB_M + s * 64];
+                            }
 #pragma unroll
-                                for (int k = 0; k < 4; ++k) asm volatile("" : "+v"(kv[k].re), "+v"(kv[k].im));
+                            for (int b = 0; b < KB; ++b)
 #pragma unroll
-                                for (int s = 0; s < 4; ++s) cmac(macc[o][0][s], y[s], kv[s]);
+                                for (int o = 0; o < 2; ++o)
+#pragma unroll
+                                    for (int k = 0; k < 4; ++k) asm volatile("" : "+v"(kv[b][o][k].re), "+v"(kv[b][o][k].im));
+#pragma unroll
+                            for (int b = 0; b < KB; ++b) {
+                                const int q = q0 + b;
+#pragma unroll
+                                for (int s = 0; s < 4; ++s)
+                                    y[s] = csub(q == 0 ? t1[s] : q == 1 ? cmul(t1[s], zp[1][s]) : cmul(x[s], zp[1][s]),
+                                                x[s]);
+#pragma unroll
+                                for (int o = 0; o < 2; ++o)
+#pragma unroll
+                                    for (int s = 0; s < 4; ++s) cmac(macc[o][0][s], y[s], kv[b][o][s]);
                             }
                         }
                     } else {
@@ -1197,15 +1217,18 @@ int pbs_model_form(uint32_t batch, int limbs, int mb) {
     // which pays only up to 256 ciphertexts; beyond, the branchy form keeps two waves
     // per SIMD.  One limb: the branch-free form fits two waves per SIMD (220
     // registers) and was the faster one at every size measured.  Two key bits per
-    // step has the branch-free form only (-1: no choice).
-    if (limbs == 2) return batch <= 256 ? 1 : 0;
-    return mb == 1 ? 1 : -1;
+    // step (branch-free FFT): 1 = the wide key batch at one wave per SIMD, for
+    // launches of at most one workgroup per CU (128 inputs 5.9 against 6.6 ms), else
+    // 0 = two waves per SIMD (2048 inputs 34.4 against 48.0 ms).
+    if (limbs == 2 || mb == 2) return batch <= 256 ? 1 : 0;
+    return 1;
 }
 
 void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, int limbs, int mb,
                  const FFT1024Tables& T, uint64_t* out, hipStream_t st, int form) {
     // one workgroup per ciphertext; form: 1 = branch-free cross-wave FFT step,
-    // 0 = branchy, < 0 = pbs_model_form's choice
+    // 0 = branchy (two key bits per step: 1 = wide key batch, 0 = two waves per
+    // SIMD), < 0 = pbs_model_form's choice
     const unsigned big = grid_for(batch, 2048);
     if (form < 0) form = pbs_model_form(batch, limbs, mb);
     if (limbs == 2) {
@@ -1218,6 +1241,8 @@ void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, i
             k_pbs2_manylut<false, 1><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
         else
             k_pbs2_manylut<true, 1><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+    } else if (form == 1) {
+        k_pbs2_manylut<true, 1, 2, true><<<batch <= 256 ? batch : big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
     } else {
         k_pbs2_manylut<true, 1, 2><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
     }

@@ -49,7 +49,8 @@ void launch_to_fourier2048_32(const uint32_t* coef, size_t npoly, const FFT1024T
 void launch_to_fourier2048_t64(const uint64_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st);
 // limbs = 2: exact two-limb BSK2 rows (launch_to_fourier2048); 1: one torus-scaled row (launch_to_fourier2048_t64)
 // mb = 2 (limbs = 1, n even): two key bits per step with the pattern keys of launch_gen_bsk64(mb = 2)
-// form: 1 = branch-free cross-wave FFT step, 0 = branchy, < 0 = pbs_model_form's choice
+// form: 1 = branch-free cross-wave FFT step, 0 = branchy (two key bits per step: 1 = wide key
+// batch at one wave per SIMD, 0 = two waves per SIMD), < 0 = pbs_model_form's choice
 int pbs_model_form(uint32_t batch, int limbs, int mb);
 void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, int limbs, int mb,
                  const FFT1024Tables& T, uint64_t* out, hipStream_t st, int form = -1);

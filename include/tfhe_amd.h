@@ -154,9 +154,9 @@ typedef struct tfhe_cbs_params {
      * pattern keys per key-bit pair (RGSW of the indicators of (1,0), (0,1),
      * (1,1)): one decomposition and two inverse transforms per pair, 1.5x the
      * MACs and key bytes.  2 needs pbs_approx = 1 and an even n.  3 = both key
-     * forms (417 MB), one or two bits per step chosen per launch size: two from 16
-     * rounds of resident workgroups on, where they are faster, or as
-     * tfhe_cbs_autotune picks (same conditions as 2). */
+     * forms (417 MB), one or two bits per step chosen per launch: two unless
+     * tfhe_cbs_autotune measured one bit faster for that launch size (same
+     * conditions as 2). */
     int pbs_multibit;
     /* Private functional key switch form (configs: PFKS base 2^4, 8 digits):
      * 0 = lookup key, every digit multiple v = 1..15 its own torus-32 RLWE row
@@ -273,8 +273,10 @@ int tfhe_cbs_keyswitch_batch(tfhe_cbs_ctx* ctx, const uint32_t* d_lwe32, uint32_
 /* Launch-shape autotuner (no counterpart in the reference, whose shapes are fixed at
  * compile time): times, with events on `stream`, the candidate shapes of one AES
  * round over `blocks` blocks and keeps the fastest in the context for later calls
- * of those sizes.  choices[5] receives the PBS FFT form (0 branchy, 1 branch-free,
- * -1 single form; 2 = two key bits per step in a pbs_multibit = 3 context) of the
+ * of those sizes.  choices[5] receives the PBS form (0 branchy, 1 branch-free FFT step;
+ * with two key bits per step 1 = the wide key batch at one wave per SIMD, 0 = two
+ * waves per SIMD; in a pbs_multibit = 3 context 0 / 1 are one bit per step and
+ * 2 / 3 two bits at two waves per SIMD / wide) of the
  * full-round part and of the rest of tfhe_cbs_batch's split,
  * the PFKS i-slices of the same two parts, and the N=2048 key switch's i-slices;
  * -1 where a part does not exist.  A part of 16 or more rounds of resident

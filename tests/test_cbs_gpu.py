@@ -525,7 +525,7 @@ def test_aes_plain_form_equals_batched(ctx, okeys):
 
 def _check_choices(ch):
     for k in range(2):
-        assert ch["pbs_form"][k] in (-1, 0, 1)
+        assert ch["pbs_form"][k] in (-1, 0, 1)  # an exact-PBS context: no two-bit forms
         assert ch["pfks_slices"][k] == -1 or 4 <= ch["pfks_slices"][k] <= 32
     assert ch["pfks_slices"][1] >= 4  # the whole batch or its tail always exists
     assert 1 <= ch["ks_slices"] <= 64
@@ -753,9 +753,12 @@ def test_aes128_linear_pfks(lctx, okeys):
 
 
 def test_pbs_auto_multibit_matches_fixed_forms(gpu, okeys):
-    """pbs_multibit = 3 holds both key forms and picks per launch size: a 128-input
-    launch equals the one-bit context's PBS and an 8192-input launch (16 rounds of
-    resident workgroups) the two-bit context's, bit for bit; AES through it decrypts."""
+    """Two key bits per step: the wide form (a 128-input launch, one wave per SIMD and
+    all pattern keys in one batch) and the two-waves-per-SIMD form (8192 inputs) give
+    the same words on the same inputs.  pbs_multibit = 3 holds both key forms and
+    picks two bits untuned (its launches equal the two-bit context's bit for bit);
+    tuned for 1 block it equals the fixed context of the form it picked; AES through
+    it decrypts."""
     import torch
     import tfhe_amd
     outs = {}
@@ -775,6 +778,12 @@ def test_pbs_auto_multibit_matches_fixed_forms(gpu, okeys):
                 torch.cuda.synchronize()
                 outs[(mb, nb)] = from_dev(d_out, np.uint64)
             if mb == 3:
+                tuned = c.autotune(1)["pbs_form"][1]  # 1 block: no full round, all 128 in the rest
+                assert tuned in (0, 1, 2, 3)
+                d_out = torch.empty((128, 2, 2049), dtype=torch.int64, device="cuda")
+                c.pbs_batch(d_cts[:128].contiguous(), d_out)
+                torch.cuda.synchronize()
+                outs[("tuned", tuned >= 2)] = from_dev(d_out, np.uint64)
                 key = bytes(range(16))
                 c.set_round_keys(tfhe_amd.aes_encrypt_key_bits(okeys.s, 77, key))
                 ctr = [bytes.fromhex("00112233445566778899aabbccddeeff")]
@@ -784,9 +793,13 @@ def test_pbs_auto_multibit_matches_fixed_forms(gpu, okeys):
                 assert _decrypt_blocks(okeys, from_dev(d_ks), 1)[0].hex() == "69c4e0d86a7b0430d8cdb78070b4c55a"
         finally:
             c.close()
-    assert np.array_equal(outs[(3, 128)], outs[(1, 128)])
+    assert np.array_equal(outs[(2, 128)], outs[(2, 8192)][:128])  # wide form = two waves per SIMD
+    assert np.array_equal(outs[(3, 128)], outs[(2, 128)])
     assert np.array_equal(outs[(3, 8192)], outs[(2, 8192)])
-    assert not np.array_equal(outs[(1, 128)], outs[(2, 8192)][:128])  # the two forms differ in their words
+    assert np.array_equal(outs[(1, 128)], outs[(1, 8192)][:128])
+    assert not np.array_equal(outs[(1, 128)], outs[(2, 128)])  # one and two bits per step differ in their words
+    (two_bits, got), = [(k[1], v) for k, v in outs.items() if k[0] == "tuned"]
+    assert np.array_equal(got, outs[(2, 128)] if two_bits else outs[(1, 128)])
 
 
 def test_empty_batches_leave_outputs_untouched(ctx, okeys):
