@@ -458,8 +458,9 @@ __device__ __forceinline__ uint64_t torus64_from_double(double y) {
 // digit transforms before they meet that pattern's key row.  The pattern keys are
 // the step's largest load stream (24 complex values per lane and digit row), and the
 // step waits on it unless many loads are in flight: WIDE = false keeps two waves per
-// SIMD and issues each pattern's 8 values as one batch; WIDE = true (launches of at
-// most one workgroup per CU) issues all 24 at once with one wave per SIMD.
+// SIMD and issues each pattern's 8 values as one batch, the next pattern's before the
+// current pattern's MACs; WIDE = true (launches of at most one workgroup per CU)
+// issues all 24 at once with one wave per SIMD.
 template <bool BF, int LIMBS, int MB = 1, bool WIDE = false>
 __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylut(const uint32_t* __restrict__ lwe, uint32_t batch, int n,
                                                      const cplx* __restrict__ bskf, FFT1024Tables Tg,
@@ -546,39 +547,55 @@ __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylu
                         cplx t1[4], y[4];
 #pragma unroll
                         for (int s = 0; s < 4; ++s) t1[s] = cmul(x[s], zp[0][s]);
-                        // key values of pattern q (rows in the order (1,0), (0,1), (1,1)) for
-                        // both outputs; the empty asm statements close a batch of loads, which
-                        // the compiler would otherwise split, waiting on each load in turn
-                        constexpr int KB = WIDE ? 3 : 1;  // patterns per batch of loads
-                        cplx kv[KB][2][4];
+                        // key values of pattern q (rows in the order (1,0), (1,1), (0,1)) for
+                        // both outputs.  The empty asm statements close a batch of loads, which
+                        // the compiler would otherwise split, waiting on each load in turn.
+                        auto load_keys = [&](cplx(&kv)[2][4], int q) {
+                            const int pt = q == 0 ? 0 : q == 1 ? 2 : 1;
+                            const cplx* kr = key + ((size_t)pt * PBS_ROWS + r) * 2 * CB_M + (w * 4) * 64 + lane;
 #pragma unroll
-                        for (int q0 = 0; q0 < 3; q0 += KB) {
+                            for (int o = 0; o < 2; ++o)
 #pragma unroll
-                            for (int b = 0; b < KB; ++b) {
-                                const int q = q0 + b, pt = q == 0 ? 0 : q == 1 ? 2 : 1;
-                                const cplx* kr = key + ((size_t)pt * PBS_ROWS + r) * 2 * CB_M + (w * 4) * 64 + lane;
+                                for (int s = 0; s < 4; ++s) kv[o][s] = kr[o * CB_M + s * 64];
+                        };
+                        auto close_batch = [](cplx(&kv)[2][4]) {
+#pragma unroll
+                            for (int o = 0; o < 2; ++o)
+#pragma unroll
+                                for (int k = 0; k < 4; ++k) asm volatile("" : "+v"(kv[o][k].re), "+v"(kv[o][k].im));
+                        };
+                        auto mac_pattern = [&](const cplx(&kv)[2][4], int q) {
+#pragma unroll
+                            for (int s = 0; s < 4; ++s)
+                                y[s] = csub(q == 0 ? t1[s] : q == 1 ? cmul(t1[s], zp[1][s]) : cmul(x[s], zp[1][s]), x[s]);
+#pragma unroll
+                            for (int o = 0; o < 2; ++o)
+#pragma unroll
+                                for (int s = 0; s < 4; ++s) cmac(macc[o][0][s], y[s], kv[o][s]);
+                        };
+                        if constexpr (WIDE) {  // all 24 values as one batch
+                            cplx kv[3][2][4];
+#pragma unroll
+                            for (int q = 0; q < 3; ++q) load_keys(kv[q], q);
+#pragma unroll
+                            for (int q = 0; q < 3; ++q) close_batch(kv[q]);
+#pragma unroll
+                            for (int q = 0; q < 3; ++q) mac_pattern(kv[q], q);
+                        } else {  // 8 per batch, pattern q + 1 loaded before pattern q's MACs
+                            cplx kc[2][4], kn[2][4];
+                            load_keys(kc, 0);
+                            close_batch(kc);
+#pragma unroll
+                            for (int q = 0; q < 3; ++q) {
+                                if (q < 2) {
+                                    load_keys(kn, q + 1);
+                                    close_batch(kn);
+                                }
+                                mac_pattern(kc, q);
 #pragma unroll
                                 for (int o = 0; o < 2; ++o)
 #pragma unroll
-                                    for (int s = 0; s < 4; ++s) kv[b][o][s] = kr[o * CB_M + s * 64];
-                            }
-#pragma unroll
-                            for (int b = 0; b < KB; ++b)
-#pragma unroll
-                                for (int o = 0; o < 2; ++o)
-#pragma unroll
-                                    for (int k = 0; k < 4; ++k) asm volatile("" : "+v"(kv[b][o][k].re), "+v"(kv[b][o][k].im));
-#pragma unroll
-                            for (int b = 0; b < KB; ++b) {
-                                const int q = q0 + b;
-#pragma unroll
-                                for (int s = 0; s < 4; ++s)
-                                    y[s] = csub(q == 0 ? t1[s] : q == 1 ? cmul(t1[s], zp[1][s]) : cmul(x[s], zp[1][s]),
-                                                x[s]);
-#pragma unroll
-                                for (int o = 0; o < 2; ++o)
-#pragma unroll
-                                    for (int s = 0; s < 4; ++s) cmac(macc[o][0][s], y[s], kv[b][o][s]);
+                                    for (int s = 0; s < 4; ++s) kc[o][s] = kn[o][s];
                             }
                         }
                     } else {
