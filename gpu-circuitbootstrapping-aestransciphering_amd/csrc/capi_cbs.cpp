@@ -39,6 +39,8 @@ size_t bsk2_mb2_first(const tfhe_cbs_params& p) { return pbs_auto(p) ? (size_t)p
 // capacity bound: keeps every 32-bit count (2 x inputs for the PFKS) and every grid y
 // dimension (padded inputs / 64) of the path in range
 constexpr uint32_t kMaxBlocks = 4096;
+// captured keystream graphs kept per context
+constexpr size_t kMaxGraphs = 8;
 // inputs per linear-form PFKS launch (digit scratch pfkl_digit_bytes(kPfklChunk) = 134 MB)
 constexpr uint32_t kPfklChunk = 8192;
 size_t bsk2_cplx(const tfhe_cbs_params& p) { return bsk2_rgsw(p) * 8 * 2 * pbs_limbs(p) * 1024; }
@@ -177,11 +179,11 @@ struct tfhe_cbs_ctx {
     uint8_t* d_ctr = nullptr;                   // [max_blocks][16]
     uint8_t* d_pub = nullptr;                   // [max_blocks][16] public AES-CTR ciphertext bytes
     bool has_round_keys = false;
-    // captured CTR keystreams (aes_init + 10 rounds), one per (blocks, output buffer),
+    // captured CTR keystreams (aes_init + 10 rounds), one per block count (at most kMaxGraphs,
+    // least recently used evicted), writing the context's state buffers,
     // replayed on the context's own stream, ordered after / before the caller's stream by events
     struct AesGraph {
         uint32_t blocks;
-        uint32_t* d_out;
         hipGraphExec_t exec;
     };
     std::vector<AesGraph> graphs;
@@ -782,7 +784,8 @@ extern "C" int tfhe_cbs_autotune(tfhe_cbs_ctx* c, uint32_t blocks, int* choices,
         if (!batch) continue;
         if (batch >= many) {
             choices[part] = pbs_auto(c->p) ? auto_pbs_model(batch) : pbs_model_form(batch, limbs, mb);
-            choices[2 + part] = pfks_model_slices(2 * batch, 2);
+            // the linear-form PFKS is one GEMM shape: no i-slices to report
+            choices[2 + part] = c->p.pfks_linear ? -1 : pfks_model_slices(2 * batch, 2);
             continue;
         }
         const int model_form = pbs_auto(c->p) ? auto_pbs_model(batch) : pbs_model_form(batch, limbs, mb);
@@ -969,7 +972,10 @@ static int aes_group(tfhe_cbs_ctx* c, const CbsScratch& sc, const uint8_t* d_ctr
     int rc = check_launch("aes_init");
     int cur = 0;
     for (int round = 1; round <= 10 && !rc; ++round) {
-        uint32_t* dst = (round == 10) ? d_out : sc.state[cur ^ 1];
+        // d_out == NULL: the last round writes the scratch set's state[0] (where the
+        // ping-pong of ten rounds lands anyway), for a captured graph that must not name
+        // the caller's buffer
+        uint32_t* dst = (round == 10 && d_out) ? d_out : sc.state[cur ^ 1];
         rc = plain ? aes_round_plain(c, round, sc.state[cur], blocks, dst, st)
                    : aes_round_impl(c, sc, round, sc.state[cur], blocks, dst, st);
         cur ^= 1;
@@ -983,23 +989,30 @@ static int aes_group(tfhe_cbs_ctx* c, const CbsScratch& sc, const uint8_t* d_ctr
 // scratch, so that the remainder's bootstraps fill the full group's idle slots instead of
 // a partial round of their own every AES round.  Blocks are independent, so both groups
 // give the words of the single-group keystream.
-static int aes_keystream_kernels(tfhe_cbs_ctx* c, uint32_t blocks, uint32_t* d_out, hipStream_t st,
-                                 bool plain = false) {
-    // (the lookup-form PFKS is long enough to hide the partial PBS round behind it inside
-    // tfhe_cbs_batch; grouping measured slower there: 17 blocks 15.3 against 16.1 blocks/s)
+// Blocks of the full-round group of an uneven batch (0: one group).
+// (the lookup-form PFKS is long enough to hide the partial PBS round behind it inside
+// tfhe_cbs_batch; grouping measured slower there: 17 blocks 15.3 against 16.1 blocks/s)
+static uint32_t aes_full_group_blocks(const tfhe_cbs_ctx* c, uint32_t blocks, bool plain) {
     uint32_t b1 = 0;
     if (!plain && c->tail_blocks) {  // only linear-PFKS contexts hold the second scratch set
         const uint32_t rb = 2 * device_cus() / 128;
         b1 = blocks / rb * rb;
         if (b1 == blocks || blocks - b1 > c->tail_blocks) b1 = 0;
     }
+    return b1;
+}
+
+// d_out == NULL: each group's keystream stays in its scratch set's state[0]
+static int aes_keystream_kernels(tfhe_cbs_ctx* c, uint32_t blocks, uint32_t* d_out, hipStream_t st,
+                                 bool plain = false) {
+    const uint32_t b1 = aes_full_group_blocks(c, blocks, plain);
     if (!b1) return aes_group(c, c->sc, c->d_ctr, blocks, d_out, st, plain);
     int rc = hipcheck(hipEventRecord(c->ev_gfork, st), "group fork");
     if (!rc) rc = hipcheck(hipStreamWaitEvent(c->cstream, c->ev_gfork, 0), "group fork wait");
     if (!rc) rc = aes_group(c, c->sc, c->d_ctr, b1, d_out, st, false);
     if (!rc)
-        rc = aes_group(c, c->tail, c->d_ctr + (size_t)16 * b1, blocks - b1, d_out + (size_t)b1 * 128 * (c->p.n + 1),
-                       c->cstream, false);
+        rc = aes_group(c, c->tail, c->d_ctr + (size_t)16 * b1, blocks - b1,
+                       d_out ? d_out + (size_t)b1 * 128 * (c->p.n + 1) : nullptr, c->cstream, false);
     if (!rc) rc = hipcheck(hipEventRecord(c->ev_gjoin, c->cstream), "group join");
     if (!rc) rc = hipcheck(hipStreamWaitEvent(st, c->ev_gjoin, 0), "group join wait");
     return rc;
@@ -1048,22 +1061,41 @@ extern "C" int tfhe_aes_ctr_keystream_graph(tfhe_cbs_ctx* c, const uint8_t* h_ct
     if (!rc) rc = hipcheck(hipEventRecord(c->ev_in, st), "event");
     if (!rc) rc = hipcheck(hipStreamWaitEvent(c->gstream, c->ev_in, 0), "event wait");
     if (rc) return rc;
+    // the graph writes the keystream into the context's scratch, so one graph per block
+    // count serves every output buffer; the copy to d_out runs after the replay, outside it
     hipGraphExec_t exec = nullptr;
-    for (const auto& g : c->graphs)
-        if (g.blocks == blocks && g.d_out == d_out) exec = g.exec;
+    for (size_t k = 0; k < c->graphs.size(); ++k)
+        if (c->graphs[k].blocks == blocks) {
+            exec = c->graphs[k].exec;
+            std::rotate(c->graphs.begin() + k, c->graphs.begin() + k + 1, c->graphs.end());  // most recent last
+            break;
+        }
     if (!exec) {
         rc = hipcheck(hipStreamBeginCapture(c->gstream, hipStreamCaptureModeThreadLocal), "begin capture");
         if (rc) return rc;
-        rc = aes_keystream_kernels(c, blocks, d_out, c->gstream);
+        rc = aes_keystream_kernels(c, blocks, nullptr, c->gstream);
         hipGraph_t graph = nullptr;
         const int rc_end = hipcheck(hipStreamEndCapture(c->gstream, &graph), "end capture");
         if (!rc) rc = rc_end;
         if (!rc) rc = hipcheck(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
         if (graph) (void)hipGraphDestroy(graph);
         if (rc) return rc;
-        c->graphs.push_back({blocks, d_out, exec});
+        if (c->graphs.size() >= kMaxGraphs) {  // evict the least recently used
+            (void)hipGraphExecDestroy(c->graphs.front().exec);
+            c->graphs.erase(c->graphs.begin());
+        }
+        c->graphs.push_back({blocks, exec});
     }
     rc = hipcheck(hipGraphLaunch(exec, c->gstream), "graph launch");
+    const size_t row_bytes = (size_t)128 * (c->p.n + 1) * sizeof(uint32_t);
+    const uint32_t b1 = aes_full_group_blocks(c, blocks, false);
+    if (!rc)
+        rc = hipcheck(hipMemcpyAsync(d_out, c->sc.state[0], (b1 ? b1 : blocks) * row_bytes, hipMemcpyDeviceToDevice,
+                                     c->gstream), "keystream copy");
+    if (!rc && b1)
+        rc = hipcheck(hipMemcpyAsync(reinterpret_cast<uint8_t*>(d_out) + b1 * row_bytes, c->tail.state[0],
+                                     (blocks - b1) * row_bytes, hipMemcpyDeviceToDevice, c->gstream),
+                      "keystream copy");
     if (!rc) rc = hipcheck(hipEventRecord(c->ev_out, c->gstream), "event");
     if (!rc) rc = hipcheck(hipStreamWaitEvent(st, c->ev_out, 0), "event wait");
     return rc;

@@ -299,9 +299,3 @@ extern "C" uint32_t tfhe_lwe_decrypt(int n, const int32_t* sk, const uint32_t* c
     const uint64_t v = ((uint64_t)ph * p + (1ull << 31)) >> 32;
     return (uint32_t)(v % p);
 }
-
-extern "C" int tfhe_probe_fma_tflops(int fp64, double* tflops) {
-    if (!tflops) return fail(TFHE_E_PARAM, "probe_fma: NULL output");
-    if (probe_fma_tflops(fp64, tflops)) return fail(TFHE_E_HIP, "probe_fma: launch failed");
-    return TFHE_OK;
-}

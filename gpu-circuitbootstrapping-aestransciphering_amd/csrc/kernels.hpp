@@ -88,6 +88,4 @@ void launch_aes_init(const uint32_t* rk0, const uint8_t* ctr, uint32_t blocks, i
 void launch_aes_linear(const uint32_t* in, const uint32_t* rk, const int16_t* src, int nsrc, uint32_t blocks, int n,
                        uint32_t* out, hipStream_t st);
 void launch_xor_public(uint32_t* ct, const uint8_t* pub, uint32_t nbits, int n, hipStream_t st);
-// VALU FMA throughput probe (FP32 / FP64 chains), TFLOP/s
-int probe_fma_tflops(int fp64, double* tflops);
 }  // namespace tfhe

@@ -124,7 +124,6 @@ SIGNATURES = {
     "tfhe_aes_round_batch": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_aes_ctr_keystream_batch": (ctypes.c_int, [_vp, np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
                                                     ctypes.c_uint32, _vp, _vp]),
-    "tfhe_probe_fma_tflops": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "tfhe_aes_ctr_keystream_graph": (ctypes.c_int, [_vp, np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
                                                     ctypes.c_uint32, _vp, _vp]),
     "tfhe_aes_ctr_keystream_plain": (ctypes.c_int, [_vp, np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
@@ -160,12 +159,35 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _need(t, nbytes, what):
+    """Raise unless t is a contiguous device tensor holding at least nbytes: the C ABI
+    takes raw pointers, so an undersized buffer would be read or written past its end."""
+    if t is None:
+        raise TfheError(TFHE_E_PARAM, f"{what}: buffer is None")
+    if not getattr(t, "is_cuda", False) or not t.is_contiguous():
+        raise TfheError(TFHE_E_PARAM, f"{what}: expected a contiguous device tensor")
+    have = t.numel() * t.element_size()
+    if have < nbytes:
+        raise TfheError(TFHE_E_SHAPE, f"{what}: {have} bytes, {nbytes} needed")
+
+
+def _blocks16(blocks, what):
+    """Join 16-byte AES blocks into one host buffer, rejecting any other length."""
+    for k, b in enumerate(blocks):
+        if len(b) != 16:
+            raise TfheError(TFHE_E_SHAPE, f"{what}: block {k} has {len(b)} bytes, not 16")
+    return np.ascontiguousarray(np.frombuffer(b"".join(bytes(b) for b in blocks), np.uint8))
+
+
 def _stream(s):
     return ctypes.c_void_p(0 if s is None else s)
 
 
 def polymul1024_batch(d_x, d_y, d_out, stream=None):
-    check(lib().tfhe_polymul1024_batch(_ptr(d_x), _ptr(d_y), d_x.shape[0], _ptr(d_out), _stream(stream)))
+    b = d_x.shape[0]
+    for t, w in ((d_x, "x"), (d_y, "y"), (d_out, "out")):
+        _need(t, b * 1024 * 4, f"polymul1024 {w}")
+    check(lib().tfhe_polymul1024_batch(_ptr(d_x), _ptr(d_y), b, _ptr(d_out), _stream(stream)))
 
 
 # ------------------------------------------------------------------ host helpers
@@ -220,27 +242,43 @@ class GateContext:
 
     def eval_batch(self, gate, d_ct1, d_ct2, d_out, batch=None, stream=None):
         b = d_ct1.shape[0] if batch is None else batch
+        row = 4 * (self.p.n + 1)
+        for t, w in ((d_ct1, "ct1"), (d_ct2, "ct2"), (d_out, "out")):
+            _need(t, b * row, f"eval_batch {w}")
         check(lib().tfhe_gate_eval_batch(self.h, gate, _ptr(d_ct1), _ptr(d_ct2), b, _ptr(d_out), _stream(stream)))
 
     def eval_plain(self, gate, d_ct1, d_ct2, d_out, stream=None):
+        row = 4 * (self.p.n + 1)
+        for t, w in ((d_ct1, "ct1"), (d_ct2, "ct2"), (d_out, "out")):
+            _need(t, row, f"eval_plain {w}")
         check(lib().tfhe_gate_eval_plain(self.h, gate, _ptr(d_ct1), _ptr(d_ct2), _ptr(d_out), _stream(stream)))
 
     def blind_rotate_batch(self, gate, d_ct1, d_ct2, d_lweN, stream=None):
-        check(lib().tfhe_gate_blind_rotate_batch(self.h, gate, _ptr(d_ct1), _ptr(d_ct2), d_ct1.shape[0],
+        b = d_ct1.shape[0]
+        _need(d_ct2, b * 4 * (self.p.n + 1), "blind_rotate ct2")
+        _need(d_lweN, b * 4 * (self.p.N + 1), "blind_rotate out")
+        check(lib().tfhe_gate_blind_rotate_batch(self.h, gate, _ptr(d_ct1), _ptr(d_ct2), b,
                                                  _ptr(d_lweN), _stream(stream)))
 
     def external_product(self, key_index, d_acc, d_out, stream=None):
+        _need(d_acc, 8 * self.p.N, "external_product acc")
+        _need(d_out, 8 * self.p.N, "external_product out")
         check(lib().tfhe_gate_external_product(self.h, key_index, _ptr(d_acc), _ptr(d_out), _stream(stream)))
 
     def keyswitch_batch(self, d_lweN, d_out, stream=None):
+        _need(d_out, d_lweN.shape[0] * 4 * (self.p.n + 1), "keyswitch out")
         check(lib().tfhe_gate_keyswitch_batch(self.h, _ptr(d_lweN), d_lweN.shape[0], _ptr(d_out), _stream(stream)))
 
 
 def polymul2048_batch(d_x, d_y, d_out, stream=None, approx=False):
     """Negacyclic products x * y (N = 2048, torus-64 y): exact, or in the PBS's
     one-product form (approx=True)."""
+    b = d_x.shape[0]
+    _need(d_x, b * 2048 * 4, "polymul2048 x")
+    _need(d_y, b * 2048 * 8, "polymul2048 y")
+    _need(d_out, b * 2048 * 8, "polymul2048 out")
     fn = lib().tfhe_polymul2048_approx_batch if approx else lib().tfhe_polymul2048_batch
-    check(fn(_ptr(d_x), _ptr(d_y), d_x.shape[0], _ptr(d_out), _stream(stream)))
+    check(fn(_ptr(d_x), _ptr(d_y), b, _ptr(d_out), _stream(stream)))
 
 
 def aes_encrypt_key_bits(sk, seed, key, nshift=15):
@@ -307,13 +345,20 @@ class CbsContext:
 
     def pbs_batch(self, d_lwe, d_out, batch=None, stream=None):
         b = d_lwe.shape[0] if batch is None else batch
+        _need(d_lwe, b * 4 * (self.p.n + 1), "pbs in")
+        _need(d_out, b * 2 * 8 * (self.p.N + 1), "pbs out")
         check(lib().tfhe_cbs_pbs_batch(self.h, _ptr(d_lwe), b, _ptr(d_out), _stream(stream)))
 
     def pfks_batch(self, u, d_lwe64, count, out_stride, d_out, stream=None):
+        if count:
+            _need(d_lwe64, count * 8 * (self.p.N + 1), "pfks in")
+            _need(d_out, ((count - 1) * out_stride + 2 * self.p.N) * 4, "pfks out")
         check(lib().tfhe_cbs_pfks_batch(self.h, u, _ptr(d_lwe64), count, out_stride, _ptr(d_out), _stream(stream)))
 
     def cbs_batch(self, d_lwe, d_rgsw, batch=None, stream=None):
         b = d_lwe.shape[0] if batch is None else batch
+        _need(d_lwe, b * 4 * (self.p.n + 1), "cbs in")
+        _need(d_rgsw, b * 4 * 2 * self.p.N * 4, "cbs out")
         check(lib().tfhe_cbs_batch(self.h, _ptr(d_lwe), b, _ptr(d_rgsw), _stream(stream)))
 
     def cbs_batch_from_device_lwe(self, d_a, d_b, batch, a_stride, stream=None):
@@ -326,18 +371,30 @@ class CbsContext:
         return v
 
     def rgsw_to_fourier(self, d_rgsw, count, d_rgsw_f, stream=None):
+        _need(d_rgsw, count * 8 * self.p.N * 4, "rgsw_to_fourier in")
+        _need(d_rgsw_f, count * 8 * 1024 * 16, "rgsw_to_fourier out")
         check(lib().tfhe_rgsw_to_fourier(self.h, _ptr(d_rgsw), count, _ptr(d_rgsw_f), _stream(stream)))
 
     def lut8(self, d_rgsw_f, nlut, d_table, table_stride, out_bits, d_out32, stream=None):
+        if nlut:
+            _need(d_rgsw_f, nlut * 8 * 8 * 1024 * 16, "lut8 rgsw")
+            _need(d_table, ((nlut - 1) * table_stride + self.p.N) * 4, "lut8 table")
+            _need(d_out32, nlut * out_bits * (self.p.N + 1) * 4, "lut8 out")
         check(lib().tfhe_lut8_eval_batch(self.h, _ptr(d_rgsw_f), nlut, _ptr(d_table), table_stride, out_bits,
                                          _ptr(d_out32), _stream(stream)))
 
     def lut8x32(self, d_rgsw_f, nbytes, d_tables, d_out32, stream=None):
         """8 -> 32-bit LUT (T-table lookup) of nbytes bytes; d_tables [4][N] (tfhe_lut8x32_eval_batch)."""
+        if nbytes:
+            _need(d_rgsw_f, nbytes * 8 * 8 * 1024 * 16, "lut8x32 rgsw")
+            _need(d_tables, 4 * self.p.N * 4, "lut8x32 tables")
+            _need(d_out32, nbytes * 32 * (self.p.N + 1) * 4, "lut8x32 out")
         check(lib().tfhe_lut8x32_eval_batch(self.h, _ptr(d_rgsw_f), nbytes, _ptr(d_tables), _ptr(d_out32),
                                             _stream(stream)))
 
     def keyswitch_batch(self, d_lwe32, count, d_out, stream=None):
+        _need(d_lwe32, count * (self.p.N + 1) * 4, "keyswitch in")
+        _need(d_out, count * (self.p.n + 1) * 4, "keyswitch out")
         check(lib().tfhe_cbs_keyswitch_batch(self.h, _ptr(d_lwe32), count, _ptr(d_out), _stream(stream)))
 
     def autotune(self, blocks, stream=None):
@@ -352,13 +409,17 @@ class CbsContext:
         check(lib().tfhe_aes_set_round_keys(self.h, np.ascontiguousarray(rk_bits, np.uint32).reshape(-1)))
 
     def aes_round(self, rnd, d_in, blocks, d_out, stream=None):
+        row = 128 * (self.p.n + 1) * 4
+        _need(d_in, blocks * row, "aes_round in")
+        _need(d_out, blocks * row, "aes_round out")
         check(lib().tfhe_aes_round_batch(self.h, rnd, _ptr(d_in), blocks, _ptr(d_out), _stream(stream)))
 
     def aes_ctr_keystream(self, ctr_blocks, d_out, stream=None, graph=False, plain=False):
         """CTR keystream bits of AES_k(ctr) for each 16-byte counter block; graph=True
         replays a captured hipGraph (tfhe_aes_ctr_keystream_graph), plain=True launches
         every bootstrap on its own (tfhe_aes_ctr_keystream_plain)."""
-        ctr = np.ascontiguousarray(np.frombuffer(b"".join(ctr_blocks), np.uint8))
+        ctr = _blocks16(ctr_blocks, "aes_ctr_keystream counters")
+        _need(d_out, len(ctr_blocks) * 128 * (self.p.n + 1) * 4, "aes_ctr_keystream out")
         fn = (lib().tfhe_aes_ctr_keystream_graph if graph else
               lib().tfhe_aes_ctr_keystream_plain if plain else lib().tfhe_aes_ctr_keystream_batch)
         check(fn(self.h, ctr, len(ctr_blocks), _ptr(d_out), _stream(stream)))
@@ -369,17 +430,12 @@ class CbsContext:
         mode "batch", "graph" or "plain"."""
         if len(cipher_blocks) != len(ctr_blocks):
             raise ValueError("one ciphertext block per counter block")
-        ctr = np.ascontiguousarray(np.frombuffer(b"".join(ctr_blocks), np.uint8))
-        ct = np.ascontiguousarray(np.frombuffer(b"".join(cipher_blocks), np.uint8))
+        ctr = _blocks16(ctr_blocks, "aes_ctr_transcipher counters")
+        ct = _blocks16(cipher_blocks, "aes_ctr_transcipher ciphertext")
+        _need(d_out, len(ctr_blocks) * 128 * (self.p.n + 1) * 4, "aes_ctr_transcipher out")
         check(lib().tfhe_aes_ctr_transcipher_batch(self.h, AES_MODES[mode], ctr, ct, len(ctr_blocks), _ptr(d_out),
                                                    _stream(stream)))
 
 
 AES_MODES = {"batch": 0, "graph": 1, "plain": 2}
 
-
-def probe_fma_tflops(fp64):
-    """Measured FMA throughput (TFLOP/s) of the library's FP64 or FP32 VALU chain kernel."""
-    v = ctypes.c_double(0.0)
-    check(lib().tfhe_probe_fma_tflops(int(fp64), ctypes.byref(v)))
-    return v.value

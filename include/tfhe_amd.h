@@ -298,12 +298,6 @@ int tfhe_polymul2048_batch(const int32_t* d_x, const uint64_t* d_y, uint32_t bat
 int tfhe_polymul2048_approx_batch(const int32_t* d_x, const uint64_t* d_y, uint32_t batch, uint64_t* d_out,
                                   void* stream);
 
-/* Measured VALU FMA throughput (TFLOP/s) of a chain kernel in FP64 (fp64 = 1)
- * or FP32 (fp64 = 0), built like the rest of the library (no packed FP32).  No
- * counterpart in the reference: it is the measured basis of the choice of FP64
- * over split-precision FP32 for the external product on this GPU. */
-int tfhe_probe_fma_tflops(int fp64, double* tflops);
-
 /* ------------------------------------------------------------------------- */
 /* AES-128 transciphering (reference bitwise/examples/aes_transcipher/main.cu */
 /* RunTranscipherBatch :1316-1619).  State bit 8*byte + j (byte = row + 4 col)*/
@@ -336,11 +330,12 @@ int tfhe_aes_ctr_keystream_plain(tfhe_cbs_ctx* ctx, const uint8_t* h_ctr, uint32
 /* The same keystream replayed from a captured hipGraph (the reference captures
  * its LUT and circuit-bootstrap launch sequences as CUDA graphs with thread-local
  * capture, gpu_lut.cuh:1859-2039, cirbts-batch.cu:559-581): the first
- * call for a (blocks, d_out) pair captures state init + the 10 rounds on the
- * context's own stream, later calls upload the counters on `stream` and replay,
- * ordered after and before `stream` by events.  Results equal
- * tfhe_aes_ctr_keystream_batch bit for bit.  Not thread-safe (the context caches
- * the graphs). */
+ * call for a block count captures state init + the 10 rounds on the context's own
+ * stream, writing the context's state buffers; every call uploads the counters on
+ * `stream`, replays, and copies the keystream to d_out, ordered after and before
+ * `stream` by events.  One graph per block count serves any d_out; the context keeps
+ * the 8 most recently used.  Results equal tfhe_aes_ctr_keystream_batch bit for
+ * bit.  Not thread-safe (the context caches the graphs). */
 int tfhe_aes_ctr_keystream_graph(tfhe_cbs_ctx* ctx, const uint8_t* h_ctr, uint32_t blocks, uint32_t* d_out,
                                  void* stream);
 /* AES-128 CTR transciphering (the CTR path of RunTranscipherBatch, main.cu:1555-1580):

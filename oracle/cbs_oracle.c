@@ -128,16 +128,12 @@ void orc_bsk64_to_fourier(const orc_cbs_params* p, const uint64_t* bsk, double* 
     for (size_t q = 0; q < npoly; ++q) orc_to_fourier64(bsk + q * N, N, bsk_f + q * 4 * M);
 }
 
-/* t = sum_r dec_r(acc) (*) C_r over Z/2^64, exact through two limb products.
- * rgsw_f: [2l rows][2 out][4M]; digit poly r = 2j + u holds digit j of component u. */
-double orc_external_product64(int N, int logB, int l, const double* rgsw_f, const uint64_t* acc,
-                              uint64_t* out) {
+/* Forward transforms of the 2l signed digit polynomials of acc = (a, b): dr, di
+ * [2l][M]; digit poly r = 2j + u holds digit j of component u (the row order of
+ * the reference's decomposition, kernel.cu:30-49). */
+static void ep64_digit_transforms(int N, int logB, int l, const uint64_t* acc, double* dr, double* di) {
     int M = N / 2, rows = 2 * l;
     double* x = (double*)malloc(sizeof(double) * N);
-    double* dr = (double*)malloc(sizeof(double) * M);
-    double* di = (double*)malloc(sizeof(double) * M);
-    double* sr = (double*)calloc((size_t)4 * M, sizeof(double)); /* [o][limb][M] */
-    double* si = (double*)calloc((size_t)4 * M, sizeof(double));
     int32_t* dig = (int32_t*)malloc(sizeof(int32_t) * (size_t)rows * N);
     int32_t tmp[64];
     for (int u = 0; u < 2; ++u)
@@ -147,7 +143,23 @@ double orc_external_product64(int N, int logB, int l, const double* rgsw_f, cons
         }
     for (int r = 0; r < rows; ++r) {
         for (int c = 0; c < N; ++c) x[c] = (double)dig[(size_t)r * N + c];
-        orc_fft_forward(x, dr, di, N);
+        orc_fft_forward(x, dr + (size_t)r * M, di + (size_t)r * M, N);
+    }
+    free(x);
+    free(dig);
+}
+
+/* t = sum_r D_r C_r from the digit transforms, two limbs per key polynomial,
+ * each limb product rounded to integers and recombined mod 2^64 (exact).
+ * rgsw_f: [2l rows][2 out][4M]. */
+static double ep64_mac_inverse(int N, int l, const double* dr, const double* di, const double* rgsw_f,
+                               uint64_t* out) {
+    int M = N / 2, rows = 2 * l;
+    double* sr = (double*)calloc((size_t)4 * M, sizeof(double)); /* [o][limb][M] */
+    double* si = (double*)calloc((size_t)4 * M, sizeof(double));
+    for (int r = 0; r < rows; ++r) {
+        const double* xr = dr + (size_t)r * M;
+        const double* xi = di + (size_t)r * M;
         for (int o = 0; o < 2; ++o)
             for (int lb = 0; lb < 2; ++lb) {
                 const double* kr = rgsw_f + ((size_t)r * 2 + o) * 4 * M + (size_t)lb * 2 * M;
@@ -155,12 +167,13 @@ double orc_external_product64(int N, int logB, int l, const double* rgsw_f, cons
                 double* ar = sr + (size_t)(o * 2 + lb) * M;
                 double* ai = si + (size_t)(o * 2 + lb) * M;
                 for (int k = 0; k < M; ++k) {
-                    ar[k] += dr[k] * kr[k] - di[k] * ki[k];
-                    ai[k] += dr[k] * ki[k] + di[k] * kr[k];
+                    ar[k] += xr[k] * kr[k] - xi[k] * ki[k];
+                    ai[k] += xr[k] * ki[k] + xi[k] * kr[k];
                 }
             }
     }
     double maxerr = 0.0;
+    double* x = (double*)malloc(sizeof(double) * N);
     double* y = (double*)malloc(sizeof(double) * N);
     for (int o = 0; o < 2; ++o) {
         orc_fft_inverse(sr + (size_t)(o * 2) * M, si + (size_t)(o * 2) * M, x, N);
@@ -171,8 +184,22 @@ double orc_external_product64(int N, int logB, int l, const double* rgsw_f, cons
             out[o * N + c] = p0 + (p1 << 32);
         }
     }
-    free(x); free(y); free(dr); free(di); free(sr); free(si); free(dig);
+    free(x); free(y); free(sr); free(si);
     return maxerr;
+}
+
+/* t = sum_r dec_r(acc) (*) C_r over Z/2^64, exact through two limb products.
+ * rgsw_f: [2l rows][2 out][4M]; digit poly r = 2j + u holds digit j of component u. */
+double orc_external_product64(int N, int logB, int l, const double* rgsw_f, const uint64_t* acc,
+                              uint64_t* out) {
+    int M = N / 2, rows = 2 * l;
+    double* dr = (double*)malloc(sizeof(double) * (size_t)rows * M);
+    double* di = (double*)malloc(sizeof(double) * (size_t)rows * M);
+    ep64_digit_transforms(N, logB, l, acc, dr, di);
+    double e = ep64_mac_inverse(N, l, dr, di, rgsw_f, out);
+    free(dr);
+    free(di);
+    return e;
 }
 
 /* acc += X^m t - t on both components (reference kernels_fft.cu:90-113, over Z/2^64). */
@@ -207,6 +234,82 @@ double orc_blind_rotate64(const orc_cbs_params* p, const double* bsk_f, const ui
     return maxerr;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Two key bits per blind-rotation step (the reference's multi-bit PBS,        */
+/* CIRBTS_PBS_MULTIBIT, main.cu:130; combination keys cirbts-init.cu:477-583;  */
+/* step kernel.cu:4585-4771, restated for two bits and the FFT domain).        */
+/* For the key-bit pair (s_2k, s_2k+1) three pattern keys RGSW(ind_p) are      */
+/* generated, p = 0, 1, 2 for the patterns (1,0), (0,1), (1,1); exactly one    */
+/* indicator is 1 (or none), so with m1, m2 the pair's rotations               */
+/*   acc += sum_p (X^(e_p) - 1) (dec(acc) (*) RGSW(ind_p)),                    */
+/*   e_p = m1, m2, m1 + m2,                                                    */
+/* rotates acc by X^(m1 s_2k + m2 s_2k+1) as two single-bit steps would.  One   */
+/* decomposition of acc serves all three products.                              */
+/* ------------------------------------------------------------------------- */
+uint32_t orc_mb2_indicator(const int32_t* s, int k, int pat) {
+    uint32_t b0 = (uint32_t)s[2 * k], b1 = (uint32_t)s[2 * k + 1];
+    return pat == 0 ? (b0 & (1u - b1)) : pat == 1 ? ((1u - b0) & b1) : (b0 & b1);
+}
+
+/* one row of the pattern key: RGSW q = 3 k + pat, row r = 2j + u, RLWE id q 2l + r of
+ * sampler domain ORC_DOM_BSK2MB, message ind_pat * 2^(64 - l logB + j logB) on
+ * component u ([2][N]) */
+void orc_gen_bsk64_mb2_row(uint64_t seed, const orc_cbs_params* p, const int32_t* s, const int32_t* K2,
+                           uint32_t q, int r, uint64_t* row) {
+    int N = p->N, rows = 2 * p->bk_l, j = r >> 1, u = r & 1;
+    uint64_t g = (uint64_t)1 << (64 - p->bk_l * p->bk_logB + j * p->bk_logB);
+    uint64_t* msg = (uint64_t*)calloc((size_t)N, sizeof(uint64_t));
+    msg[0] = (uint64_t)orc_mb2_indicator(s, (int)(q / 3), (int)(q % 3)) * g;
+    orc_rlwe_encrypt64(seed, ORC_DOM_BSK2MB, q * (uint32_t)rows + (uint32_t)r, N, K2, u == 0 ? msg : NULL,
+                       u == 1 ? msg : NULL, p->rlwe_nshift, row);
+    free(msg);
+}
+
+/* the whole pattern key [n/2][3][2l][2][N] (n even) */
+void orc_gen_bsk64_mb2(uint64_t seed, const orc_cbs_params* p, const int32_t* s, const int32_t* K2,
+                       uint64_t* bsk) {
+    int N = p->N, rows = 2 * p->bk_l;
+    long total = (long)(p->n / 2) * 3 * rows;
+#pragma omp parallel for schedule(dynamic, 8)
+    for (long id = 0; id < total; ++id)
+        orc_gen_bsk64_mb2_row(seed, p, s, K2, (uint32_t)(id / rows), (int)(id % rows), bsk + (size_t)id * 2 * N);
+}
+
+/* Fourier form of nrgsw RGSWs of 2l rows ([nrgsw][2l][2 out][4M], two limbs) */
+void orc_rgsw64_to_fourier(const orc_cbs_params* p, long nrgsw, const uint64_t* bsk, double* bsk_f) {
+    int N = p->N, M = N / 2;
+    long npoly = nrgsw * 2 * p->bk_l * 2;
+#pragma omp parallel for schedule(static)
+    for (long q = 0; q < npoly; ++q) orc_to_fourier64(bsk + (size_t)q * N, N, bsk_f + (size_t)q * 4 * M);
+}
+
+/* exact two-bit blind rotation with the pattern key (Fourier, [n/2][3][2l][2][4M]) */
+double orc_blind_rotate64_mb2(const orc_cbs_params* p, const double* bsk_f, const uint32_t* a_tilde,
+                              uint64_t* acc) {
+    int N = p->N, M = N / 2, rows = 2 * p->bk_l;
+    uint32_t Q2 = 2u * (uint32_t)N;
+    double* dr = (double*)malloc(sizeof(double) * (size_t)rows * M);
+    double* di = (double*)malloc(sizeof(double) * (size_t)rows * M);
+    uint64_t* t = (uint64_t*)malloc(sizeof(uint64_t) * 3 * 2 * N);
+    double maxerr = 0.0;
+    for (int k = 0; k < p->n / 2; ++k) {
+        uint32_t m1 = (Q2 - a_tilde[2 * k]) & (Q2 - 1), m2 = (Q2 - a_tilde[2 * k + 1]) & (Q2 - 1);
+        if ((m1 | m2) == 0) continue;
+        uint32_t e[3] = {m1, m2, (m1 + m2) & (Q2 - 1)};
+        ep64_digit_transforms(N, p->bk_logB, p->bk_l, acc, dr, di);
+        for (int pat = 0; pat < 3; ++pat) {
+            const double* key = bsk_f + ((size_t)k * 3 + pat) * rows * 2 * 4 * M;
+            double err = ep64_mac_inverse(N, p->bk_l, dr, di, key, t + (size_t)pat * 2 * N);
+            if (err > maxerr) maxerr = err;
+        }
+        /* all three products come from the accumulator before the step */
+        for (int pat = 0; pat < 3; ++pat)
+            if (e[pat]) orc_monic_update64(N, e[pat], t + (size_t)pat * 2 * N, acc);
+    }
+    free(dr); free(di); free(t);
+    return maxerr;
+}
+
 /* LWE of coefficient idx over Z/2^64 (extract.cuh:22-31). */
 void orc_sample_extract64(int N, const uint64_t* acc, uint32_t idx, uint64_t* lwe) {
     for (uint32_t j = 0; j < (uint32_t)N; ++j) lwe[j] = j <= idx ? acc[idx - j] : 0 - acc[(uint32_t)N + idx - j];
@@ -230,6 +333,13 @@ uint32_t orc_special_ms(uint32_t v, int N, int bw) {
  * with mu_j = 2^(64 - cb_l cb_logB + j cb_logB - 1); outputs out[j] = LWE_K2 of
  * b * 2^(64 - cb_l cb_logB + j cb_logB) (coefficient j plus mu_j), j = 0, 1. */
 double orc_pbs_manylut(const orc_cbs_params* p, const double* bsk_f, const uint32_t* ct32, uint64_t* out) {
+    return orc_pbs_manylut_mb(p, bsk_f, 1, ct32, out);
+}
+
+/* mb = 1: one key bit per step with the BSK2 of orc_gen_bsk64; mb = 2: two bits per
+ * step with the pattern key of orc_gen_bsk64_mb2 (n even) */
+double orc_pbs_manylut_mb(const orc_cbs_params* p, const double* bsk_f, int mb, const uint32_t* ct32,
+                          uint64_t* out) {
     int n = p->n, N = p->N;
     uint32_t* at = (uint32_t*)malloc(sizeof(uint32_t) * n);
     uint64_t* acc = (uint64_t*)calloc((size_t)2 * N, sizeof(uint64_t));
@@ -247,7 +357,7 @@ double orc_pbs_manylut(const orc_cbs_params* p, const double* bsk_f, const uint3
         if (arg >= (uint32_t)N) v = 0 - v;
         acc[N + k] = v;
     }
-    double err = orc_blind_rotate64(p, bsk_f, at, acc);
+    double err = mb == 2 ? orc_blind_rotate64_mb2(p, bsk_f, at, acc) : orc_blind_rotate64(p, bsk_f, at, acc);
     for (int j = 0; j < 2; ++j) {
         uint64_t* lwe = out + (size_t)j * (N + 1);
         orc_sample_extract64(N, acc, (uint32_t)j, lwe);

@@ -262,6 +262,13 @@ def _cbs_lib():
             "orc_sample_extract64": (None, [ctypes.c_int, _u64p, ctypes.c_uint32, _u64p]),
             "orc_special_ms": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_int, ctypes.c_int]),
             "orc_pbs_manylut": (ctypes.c_double, [P, _f64p, _u32p, _u64p]),
+            "orc_pbs_manylut_mb": (ctypes.c_double, [P, _f64p, ctypes.c_int, _u32p, _u64p]),
+            "orc_mb2_indicator": (ctypes.c_uint32, [_i32p, ctypes.c_int, ctypes.c_int]),
+            "orc_gen_bsk64_mb2_row": (None, [ctypes.c_uint64, P, _i32p, _i32p, ctypes.c_uint32, ctypes.c_int,
+                                             _u64p]),
+            "orc_gen_bsk64_mb2": (None, [ctypes.c_uint64, P, _i32p, _i32p, _u64p]),
+            "orc_rgsw64_to_fourier": (None, [P, ctypes.c_long, _u64p, _f64p]),
+            "orc_blind_rotate64_mb2": (ctypes.c_double, [P, _f64p, _u32p, _u64p]),
             "orc_pfks_id": (ctypes.c_uint32, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
             "orc_gen_pfks_entry": (None, [ctypes.c_uint64, P, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, _u32p]),
@@ -343,6 +350,44 @@ class CbsKeys:
             self.bsk_f = np.zeros(p.n * rows * 2 * 2 * p.N, np.float64)
             L.orc_bsk64_to_fourier(ctypes.byref(p), self.bsk, self.bsk_f)
         self._ksk2 = None
+        self._bsk_mb2 = self._bsk_mb2_f = None
+
+    def bsk_mb2(self):
+        """Pattern key of two key bits per step: [n/2][3][2l][2][N] u64 (RGSW q = 3k + p of
+        the indicator of pattern p of the key-bit pair k: (1,0), (0,1), (1,1))."""
+        if self._bsk_mb2 is None:
+            p = self.p
+            self._bsk_mb2 = np.zeros((p.n // 2) * 3 * 2 * p.bk_l * 2 * p.N, np.uint64)
+            _cbs_lib().orc_gen_bsk64_mb2(self.seed, ctypes.byref(p), self.s, self.K2, self._bsk_mb2)
+        return self._bsk_mb2
+
+    def bsk_mb2_fourier(self):
+        if self._bsk_mb2_f is None:
+            p = self.p
+            bsk = self.bsk_mb2()
+            self._bsk_mb2_f = np.zeros(bsk.size * 2, np.float64)
+            _cbs_lib().orc_rgsw64_to_fourier(ctypes.byref(p), (p.n // 2) * 3, bsk, self._bsk_mb2_f)
+        return self._bsk_mb2_f
+
+    def bsk_mb2_row(self, q, r):
+        p = self.p
+        row = np.zeros(2 * p.N, np.uint64)
+        _cbs_lib().orc_gen_bsk64_mb2_row(self.seed, ctypes.byref(p), self.s, self.K2, q, r, row)
+        return row
+
+    def pbs_mb2(self, ct32):
+        """Exact multi-value PBS with two key bits per step (the pattern key)."""
+        p = self.p
+        out = np.zeros(2 * (p.N + 1), np.uint64)
+        err = _cbs_lib().orc_pbs_manylut_mb(ctypes.byref(p), self.bsk_mb2_fourier(), 2,
+                                            np.ascontiguousarray(ct32, np.uint32), out)
+        return out.reshape(2, p.N + 1), err
+
+    def blind_rotate_mb2(self, a_tilde, acc):
+        """Exact two-bit blind rotation of acc [2N] u64 (in place) by the rotations a_tilde [n]."""
+        err = _cbs_lib().orc_blind_rotate64_mb2(ctypes.byref(self.p), self.bsk_mb2_fourier(),
+                                                np.ascontiguousarray(a_tilde, np.uint32), acc)
+        return err
 
     @property
     def ksk2(self):

@@ -41,7 +41,7 @@ int32_t orc_secret_bit(uint64_t seed, uint32_t dom, uint32_t i);
 enum {
     ORC_DOM_LWE_SK = 1, ORC_DOM_RLWE1_SK = 2, ORC_DOM_RLWE2_SK = 3,
     ORC_DOM_BSK1 = 4, ORC_DOM_KSK1 = 5, ORC_DOM_BSK2 = 6, ORC_DOM_PFKS = 7,
-    ORC_DOM_KSK2 = 8, ORC_DOM_PFKL = 10, ORC_DOM_ENC = 16
+    ORC_DOM_KSK2 = 8, ORC_DOM_BSK2MB = 9, ORC_DOM_PFKL = 10, ORC_DOM_ENC = 16
 };
 
 /* ---- parameter sets ---- */
@@ -122,6 +122,15 @@ double orc_blind_rotate64(const orc_cbs_params* p, const double* bsk_f, const ui
 void orc_sample_extract64(int N, const uint64_t* acc, uint32_t idx, uint64_t* lwe);
 uint32_t orc_special_ms(uint32_t v, int N, int bw);
 double orc_pbs_manylut(const orc_cbs_params* p, const double* bsk_f, const uint32_t* ct32, uint64_t* out);
+/* two key bits per step (pattern keys of the indicators of (1,0), (0,1), (1,1)), cbs_oracle.c */
+uint32_t orc_mb2_indicator(const int32_t* s, int k, int pat);
+void orc_gen_bsk64_mb2_row(uint64_t seed, const orc_cbs_params* p, const int32_t* s, const int32_t* K2,
+                           uint32_t q, int r, uint64_t* row);
+void orc_gen_bsk64_mb2(uint64_t seed, const orc_cbs_params* p, const int32_t* s, const int32_t* K2, uint64_t* bsk);
+void orc_rgsw64_to_fourier(const orc_cbs_params* p, long nrgsw, const uint64_t* bsk, double* bsk_f);
+double orc_blind_rotate64_mb2(const orc_cbs_params* p, const double* bsk_f, const uint32_t* a_tilde, uint64_t* acc);
+double orc_pbs_manylut_mb(const orc_cbs_params* p, const double* bsk_f, int mb, const uint32_t* ct32,
+                          uint64_t* out);
 uint32_t orc_pfks_id(const orc_cbs_params* p, int u, int i, int t, int v);
 void orc_gen_pfks_entry(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, int u, int i, int t, int v,
                         uint32_t* ct);

@@ -291,3 +291,96 @@ def test_ks2_preserves_phase(keys):
         assert keys.decrypt_bit(out) == bit
         e = ((keys.phase32(out) - (bit << 31) + 2 ** 31) % 2 ** 32) - 2 ** 31
         assert abs(e) < 2 ** 32 / 16
+
+
+# ---------------------------------------------------------------- two key bits per step
+def test_mb2_patterns_reproduce_the_pair_rotation():
+    """For every key-bit pair (b0, b1) and rotations (m1, m2): at most one pattern
+    indicator is 1, and 1 + sum_p ind_p (X^(e_p) - 1) with e_p = m1, m2, m1 + m2 is the
+    monomial X^(m1 b0 + m2 b1) -- the identity the two-bit step rests on.  Checked on
+    exponents mod 2N (X^(2N) = 1 in the negacyclic ring)."""
+    L = O._cbs_lib()
+    Q2 = 4096
+    rng = random.Random(3)
+    for b0 in (0, 1):
+        for b1 in (0, 1):
+            s = np.array([b0, b1], np.int32)
+            ind = [int(L.orc_mb2_indicator(s, 0, p)) for p in range(3)]
+            assert sum(ind) == (1 if (b0 or b1) else 0)
+            for _ in range(50):
+                m1, m2 = rng.randrange(Q2), rng.randrange(Q2)
+                e = [m1, m2, (m1 + m2) % Q2]
+                active = [e[p] for p in range(3) if ind[p]]
+                want = (m1 * b0 + m2 * b1) % Q2
+                assert (active[0] if active else 0) == want
+
+
+def test_mb2_pattern_rows_decrypt(keys):
+    """Pattern-key rows q = 3k + p, r = 2j + u decrypt to ind_p(s_2k, s_2k+1) * 2^(28 + 9j)
+    on component u (u = 1: the body; u = 0: the mask, i.e. -K2 times it)."""
+    p = keys.p
+    s = keys.s
+    for k in (0, 1, 200, p.n // 2 - 1):
+        for pat in range(3):
+            q = 3 * k + pat
+            b0, b1 = int(s[2 * k]), int(s[2 * k + 1])
+            ind = [b0 & (1 - b1), (1 - b0) & b1, b0 & b1][pat]
+            for r in (0, 1, 6, 7):
+                j, u = r >> 1, r & 1
+                g = 1 << (28 + 9 * j)
+                ph = O.rlwe_phase64(keys.K2, keys.bsk_mb2_row(q, r))
+                m = np.zeros(p.N, np.uint64)
+                m[0] = ind * g
+                want = m if u == 1 else (-O.nc_mul_exact64(keys.K2, m)).astype(np.uint64)
+                err = O.signed64((ph - want).astype(np.uint64))
+                assert np.abs(err).max() < 7 * 2 ** 13, (k, pat, r)
+
+
+def test_mb2_full_key_equals_rows(keys):
+    """The whole pattern key (OpenMP over rows) holds the rows of the row generator."""
+    p = keys.p
+    bsk = keys.bsk_mb2().reshape(p.n // 2 * 3, 2 * p.bk_l, 2 * p.N)
+    for q, r in ((0, 0), (5, 3), (3 * 317 + 2, 7)):
+        assert np.array_equal(bsk[q, r], keys.bsk_mb2_row(q, r))
+
+
+def test_mb2_blind_rotation_closed_form(keys):
+    """After the exact two-bit blind rotation of acc = (0, TV) by rotations a~ the
+    phase is X^(-<a~, s>) * TV within the noise (2^-24 of the torus); pairs with one
+    or both rotations 0 are included.  A wrong pattern order, indicator or skip rule
+    rotates by another exponent and fails by about the size of TV."""
+    p = keys.p
+    N = p.N
+    rng = np.random.default_rng(8)
+    at = rng.integers(0, 2 * N, p.n).astype(np.uint32)
+    at[0:2] = 0          # both rotations of pair 0 vanish: the step is skipped
+    at[2] = 0            # pair 1: first rotation 0
+    at[5] = 0            # pair 2: second rotation 0
+    tv = (rng.integers(-2 ** 20, 2 ** 20, N).astype(np.int64) << 40).astype(np.uint64)
+    acc = np.zeros(2 * N, np.uint64)
+    acc[N:] = tv
+    err = keys.blind_rotate_mb2(at, acc)
+    assert err < 0.25
+    phi = int(np.dot(at.astype(np.int64), keys.s.astype(np.int64))) % (2 * N)
+    mono = np.zeros(N, np.int32)
+    sh = (2 * N - phi) % (2 * N)
+    if sh < N:
+        mono[sh] = 1
+    else:
+        mono[sh - N] = -1
+    want = O.nc_mul_exact64(mono, tv)
+    ph = O.rlwe_phase64(keys.K2, acc)
+    e = O.signed64((ph - want).astype(np.uint64))
+    assert np.abs(e).max() < 2 ** 40
+
+
+@pytest.mark.parametrize("bit", [0, 1])
+def test_pbs_manylut_mb2_levels(keys, bit):
+    """The two-bit PBS gives both circuit-bootstrapping levels like the one-bit PBS:
+    LWE_K2(bit * 2^48) and LWE_K2(bit * 2^56), noise below 2^-24 of the torus."""
+    ct = keys.encrypt_bit(bit, 140 + bit)
+    out, err = keys.pbs_mb2(ct)
+    assert err < 0.25
+    for j in range(2):
+        e = centered64(keys.lwe64_phase(out[j]) - bit * 2 ** (48 + 8 * j))
+        assert abs(e) < 2 ** 40
