@@ -1,12 +1,24 @@
-"""Benchmark of the batched gate-bootstrapping workload on one MI355X.
+"""Benchmark of AES-128 CTR transciphering on one MI355X, with batched gate
+bootstraps as a secondary line.
 
-Workload (one step): 4096 independent gate bootstraps on seeded random bit
-pairs, 2048 NAND + 2048 XOR, n=636, N=1024, BSK 2^7 x 3, KS 2^2 x 8, FP64
-negacyclic FFT, one persistent blind-rotation launch per gate batch followed by
-one batched key switch.  Keys are generated on the GPU from seed 1; inputs are
-resident in HBM before the timed region.
+Headline workload (one step): homomorphic AES-128 CTR transciphering of 256
+blocks: the keystream of 256 counter blocks under an encrypted, seeded random
+128-bit key (10 rounds; SubBytes by circuit bootstrap of every state bit, an
+8-bit CMux vertical-packing S-box LUT and a key switch; ShiftRows, MixColumns and
+AddRoundKey as LWE additions), plus the public AES-CTR ciphertext bits, so the
+outputs are LWE encryptions of the plaintext bits.  Level-0 LWE n=636 (sigma
+2^-15), circuit bootstrap N=2048: one-product FP64 PBS (2^9 x 4) with two key bits
+per blind-rotation step, linear-key PFKS (2^4 x 8) as an int8 GEMM on the matrix
+cores, CB 2^8 x 2, KS 2^2 x 8.  Keys are generated on the GPU from seed 3; the
+counters and the public ciphertext (4 KB for 256 blocks) are the only host inputs
+of a step.  Every block of the last timed step is decrypted and compared with the
+seeded random plaintext.
 
-Prints one JSON line (see the README of the driver contract).
+Secondary (key "gate_bootstraps"): 4096 independent NAND/XOR gate bootstraps per
+step (n=636, N=1024, BSK 2^7 x 3, KS 2^2 x 8, FP64 FFT), with a decryption-failure
+campaign over 10^6 chained bootstraps.
+
+Prints one JSON line.
 """
 import argparse
 import json
@@ -33,6 +45,9 @@ PEAK_INT8_TOPS = 5000.0  # dense int8 matrix-core peak
 N_CB, M_CB, CB_ROWS = 2048, 1024, 8
 F_EP_CB = {1: (CB_ROWS + 2) * 5 * M_CB * 10 + 8 * 2 * CB_ROWS * M_CB,      # one product: 643,072
            2: (CB_ROWS + 4) * 5 * M_CB * 10 + 8 * 2 * 2 * CB_ROWS * M_CB}  # two limbs: 876,544
+# the PBS's algorithmic FLOPs per circuit bootstrap, priced at the FP64 one-product count for
+# every form (the two-key-bit form does more arithmetic per key bit; it is not credited)
+F_PBS_CB = N_LWE * F_EP_CB[1]  # 409.0 MFLOP
 
 
 def cpu_baseline(sample=24, threads=None):
@@ -89,6 +104,54 @@ def one_product_error(batch=256):
     worst = int(np.abs(d).max())
     return {"products": batch, "max_abs_error_log2_of_torus": round(float(np.log2(max(worst, 1))) - 64, 2),
             "rms_error_log2_of_torus": round(float(np.log2(np.sqrt(np.mean(d.astype(np.float64) ** 2)))) - 64, 2)}
+
+
+def split32_product_bench(batch=4096):
+    """The split-precision FP32 product forms (tfhe_polymul2048_split32_batch) beside the
+    FP64 ones, N = 2048: digit polynomials in [-256, 256] times uniform polynomials,
+    `batch` products per launch, each timed with HIP events (mean of 3 launches after a
+    warm-up).  Errors are against the one-FP64-product form, which the tests show to be
+    the exact product bit for bit."""
+    import numpy as np
+    import torch
+    import tfhe_amd
+    rng = np.random.default_rng(7)
+    x = torch.from_numpy(rng.integers(-256, 257, (batch, 2048)).astype(np.int32)).to("cuda")
+    y32 = torch.from_numpy(rng.integers(0, 2 ** 32, (batch, 2048), dtype=np.uint64).astype(np.uint32)
+                           .view(np.int32)).to("cuda")
+    y64 = torch.from_numpy(rng.integers(0, 2 ** 64, (batch, 2048), dtype=np.uint64).view(np.int64)).to("cuda")
+    o32 = {f: torch.empty_like(y32) for f in tfhe_amd.SPLIT32_FORMS}
+    o64 = torch.empty_like(y64)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(3):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 3
+
+    out = {"products_per_launch": batch}
+    for f in tfhe_amd.SPLIT32_FORMS:
+        ms = timed(lambda: tfhe_amd.polymul2048_split32_batch(x, y32, o32[f], f))
+        out[f] = {"ms": round(ms, 4), "us_per_product": round(1e3 * ms / batch, 4)}
+    ref = o32["fp64"].cpu().numpy().view(np.uint32).astype(np.int64)
+    for f in ("fp32_3x11", "fp32_exact_top"):
+        d = (o32[f].cpu().numpy().view(np.uint32).astype(np.int64) - ref) % 2 ** 32
+        d = np.where(d >= 2 ** 31, d - 2 ** 32, d)
+        worst = int(np.abs(d).max())
+        out[f]["max_abs_error_log2_of_torus"] = round(float(np.log2(max(worst, 1))) - 32, 2)
+        out[f]["rms_error_log2_of_torus"] = round(float(np.log2(max(np.sqrt(np.mean(d.astype(np.float64) ** 2)),
+                                                                    1e-30))) - 32, 2)
+        out[f]["coefficients_wrong"] = float(np.mean(d != 0))
+    for name, approx in (("fp64_torus64_two_limbs", False), ("fp64_torus64_one_product", True)):
+        ms = timed(lambda: tfhe_amd.polymul2048_batch(x, y64, o64, approx=approx))
+        out[name] = {"ms": round(ms, 4), "us_per_product": round(1e3 * ms / batch, 4)}
+    return out
 
 
 def aes_kernel_rooflines(ctx, pbs_approx, nb, pfks_linear=0):
@@ -235,45 +298,71 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0,
                     "(1-limb exact)", "keygen_s": round(keygen_s, 2), "batches": out}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--plain-steps", type=int, default=3, help="plain per-bootstrap form samples")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run three 512 MiB device copies (the HBM-counter calibration kernel)")
-    ap.add_argument("--campaign", type=int, default=1_000_000,
-                    help="gate bootstraps in the decryption-failure campaign (0 to skip)")
-    ap.add_argument("--valu-rates", action="store_true",
-                    help="also measure the FP64 and (unpacked) FP32 FMA throughput of the library's probe kernel")
-    ap.add_argument("--aes-blocks", default="1,17,256,1024",
-                    help="comma-separated AES-128 CTR batch sizes to time ('' to skip)")
-    ap.add_argument("--aes-plain-blocks", type=int, default=1,
-                    help="AES blocks timed in the plain per-bootstrap form (exact PBS; 0 to skip)")
-    ap.add_argument("--aes-pbs", default="exact,exact-lin,approx-auto-lin",
-                    help="comma-separated PBS forms of the AES runs: exact (two-limb), approx (one FP64 "
-                         "product), approx-mb2 (one product, two key bits per step); exact-lin / approx-lin "
-                         "with the linear-key PFKS on the matrix cores; approx-mb2-lin two key bits per step; "
-                         "approx-auto-lin both key forms, two bits per step unless tuned to one")
-    ap.add_argument("--no-aes-tune", action="store_true",
-                    help="keep the kernels' model launch shapes (no tfhe_cbs_autotune before each AES batch)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              f"(bootstraps are independent: a multi-GPU run would give each GPU a key replica and a slice "
-              f"of the batch)")
-        sys.exit(2)
+def aes_cpu_baseline(threads=None, cbs_sample=1):
+    """The oracle's AES-128 pieces per state bit on the host cores, scaled to blocks/s:
+    one block is 1280 circuit bootstraps (PBS + linear-key PFKS of both levels), 160
+    S-box LUTs (8 CMux each) and 1280 N -> n key switches.  A bounded sample is timed
+    on one thread (cbs_sample circuit bootstraps, 4 LUTs, 16 key switches), then
+    `threads` threads (the box's CPU share, at most 16) each run one circuit bootstrap
+    (the C calls release the GIL).  The oracle is the exact two-limb FP64 PBS with one
+    key bit per step; its PFKS regenerates every key entry it uses from Philox (the
+    8 GB key is never materialised), which dominates its time."""
+    import ctypes
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    if threads is None:
+        threads = max(1, min(16, os.cpu_count() or 1))
+    K = O.CbsKeys(3)
+    K.ksk2  # noqa: B018 (materialise the key-switching key before timing)
+    cts = [K.encrypt_bit(k & 1, 100 + k) for k in range(max(cbs_sample, threads))]
+    # the oracle's PFKS has OpenMP loops: every timing thread runs them on itself alone
+    gomp = ctypes.CDLL("libgomp.so.1")
+    gomp.omp_set_num_threads(1)
+    t0 = time.perf_counter()
+    for k in range(cbs_sample):
+        rgsw, _ = K.cbs(cts[k], linear=True)
+    t_cbs = (time.perf_counter() - t0) / cbs_sample
+    rf = K.rgsw_fourier(rgsw)
+    table = np.random.default_rng(2).integers(0, 2 ** 32, K.p.N, dtype=np.uint64).astype(np.uint32)
+    t0 = time.perf_counter()
+    for _ in range(4):
+        acc, _ = K.vertical_packing([rf] * 8, table)
+    t_lut = (time.perf_counter() - t0) / 4
+    lwe = K.extract64(np.zeros(2 * K.p.N, np.uint64), 0)
+    t0 = time.perf_counter()
+    for _ in range(16):
+        K.ks2(lwe)
+    t_ks = (time.perf_counter() - t0) / 16
+    per_block = 1280 * t_cbs + 160 * t_lut + 1280 * t_ks
+    t0 = time.perf_counter()
+    def one_cbs(c):
+        gomp.omp_set_num_threads(1)
+        return K.cbs(c, linear=True)
 
+    with ThreadPoolExecutor(threads) as pool:
+        list(pool.map(one_cbs, cts[:threads]))
+    t_cbs_many = (time.perf_counter() - t0) / threads  # wall time per circuit bootstrap on `threads` threads
+    per_block_many = 1280 * t_cbs_many + (160 * t_lut + 1280 * t_ks) / threads
+    return {"value": round(1.0 / per_block_many, 8), "unit": "blocks/s", "cores": threads, "kind": "port",
+            "single_thread": round(1.0 / per_block, 8),
+            "per_bit_s": {"circuit_bootstrap": round(t_cbs, 4), "sbox_lut_per_byte": round(t_lut, 5),
+                          "keyswitch": round(t_ks, 5), "circuit_bootstrap_on_all_threads": round(t_cbs_many, 4)},
+            "sample": f"scaled, not a whole block: {cbs_sample} circuit bootstraps, 4 S-box LUTs and 16 key "
+                      f"switches on one thread, then one circuit bootstrap per thread on {threads} threads; "
+                      f"blocks/s = 1 / (1280 CBS + 160 LUT + 1280 KS); oracle C restatement (exact two-limb "
+                      f"FP64 PBS, linear-key PFKS regenerating its key entries from Philox)"}
+
+
+def gate_bench(args):
+    """Batched gate bootstraps (secondary line): 4096 = 2048 NAND + 2048 XOR per step,
+    one persistent blind-rotation launch and one key switch per gate half."""
     import numpy as np
     import torch
     import tfhe_amd
-
-    torch.cuda.init()
     stream = torch.cuda.current_stream()
     sptr = stream.cuda_stream
-
     ctx = tfhe_amd.GateContext(max_batch=BATCH)
     sk = tfhe_amd.lwe_keygen(N_LWE, 1)
     ctx.bt_keygen(sk, 1)
@@ -288,13 +377,10 @@ def main():
     d_lweN = torch.empty((BATCH, N_RING + 1), dtype=torch.int32, device="cuda")
     half = BATCH // 2
     L = tfhe_amd.lib()
-
-    def ptr(t):
-        return tfhe_amd._ptr(t)
+    ptr = tfhe_amd._ptr
 
     def step(events=None):
-        # NAND on the first half, XOR on the second half
-        for g, lo in ((tfhe_amd.NAND, 0), (tfhe_amd.XOR, half)):
+        for g, lo in ((tfhe_amd.NAND, 0), (tfhe_amd.XOR, half)):  # NAND on one half, XOR on the other
             sl = slice(lo, lo + half)
             if events is not None:
                 events.append(torch.cuda.Event(enable_timing=True))
@@ -309,23 +395,21 @@ def main():
                 events.append(torch.cuda.Event(enable_timing=True))
                 events[-1].record(stream)
 
-    for _ in range(args.warmup):
+    steps = args.gate_steps
+    for _ in range(2):
         step()
     torch.cuda.synchronize()
     events = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         step(events)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-
-    # correctness of the timed output (last step)
     got = d_out.cpu().numpy().view(np.uint32)
     want = np.concatenate([1 - (a[:half] & b[:half]), a[half:] ^ b[half:]])
     s64 = sk.astype(np.uint64)
     ph = (got[:, -1].astype(np.uint64) - (got[:, :-1].astype(np.uint64) @ s64)) % 2 ** 32
-    dec = ((ph + (1 << 29)) >> 30) & 3
-    failures = int(np.count_nonzero(dec != want))
+    failures = int(np.count_nonzero((((ph + (1 << 29)) >> 30) & 3) != want))
 
     # decryption-failure campaign: batches chained through their own outputs (inputs of
     # batch r+1 are the outputs of batch r and a rotation of them), every output decrypted
@@ -333,8 +417,7 @@ def main():
     if args.campaign > 0:
         cur, cur_bits = d_out.clone(), want.copy()
         d_new = torch.empty_like(d_out)
-        done = fails = 0
-        worst = 0
+        done = fails = worst = 0
         while done < args.campaign:
             d1.copy_(cur)
             d2.copy_(torch.roll(cur, 1, 0))
@@ -347,8 +430,7 @@ def main():
             exp = np.concatenate([1 - (bits_a[:half] & bits_b[:half]), bits_a[half:] ^ bits_b[half:]])
             g_np = d_new.cpu().numpy().view(np.uint32)
             ph_c = (g_np[:, -1].astype(np.uint64) - (g_np[:, :-1].astype(np.uint64) @ s64)) % 2 ** 32
-            dec_c = ((ph_c + (1 << 29)) >> 30) & 3
-            fails += int(np.count_nonzero(dec_c != exp))
+            fails += int(np.count_nonzero((((ph_c + (1 << 29)) >> 30) & 3) != exp))
             err = ((ph_c - exp.astype(np.uint64) * (1 << 30) + 2 ** 31) % 2 ** 32).astype(np.int64) - 2 ** 31
             worst = max(worst, int(np.abs(err).max()))
             cur, d_new = d_new, cur
@@ -358,16 +440,12 @@ def main():
                     "max_abs_phase_error_log2": round(float(np.log2(max(worst, 1))), 2), "margin_log2": 29.0,
                     "inputs": "chained: each batch's inputs are the previous outputs and a rotation of them"}
 
-    br_ms, ks_ms = [], []
-    for k in range(0, len(events), 3):
-        br_ms.append(events[k].elapsed_time(events[k + 1]))
-        ks_ms.append(events[k + 1].elapsed_time(events[k + 2]))
-    br_avg = sum(br_ms) / len(br_ms)
-    ks_avg = sum(ks_ms) / len(ks_ms)
-    achieved_tflops = F_PBS * half / (br_avg * 1e-3) / 1e12
+    br_ms = [events[k].elapsed_time(events[k + 1]) for k in range(0, len(events), 3)]
+    ks_ms = [events[k + 1].elapsed_time(events[k + 2]) for k in range(0, len(events), 3)]
+    br_avg, ks_avg = sum(br_ms) / len(br_ms), sum(ks_ms) / len(ks_ms)
+    achieved = F_PBS * half / (br_avg * 1e-3) / 1e12
 
-    # plain per-bootstrap form (reference launch structure), a few single gates
-    plain_rate = None
+    plain_rate = None  # plain per-bootstrap form (the reference's launch structure), a few single gates
     if args.plain_steps > 0:
         one_out = torch.empty((1, N_LWE + 1), dtype=torch.int32, device="cuda")
         ctx.eval_plain(tfhe_amd.NAND, d1[0:1], d2[0:1], one_out, stream=sptr)
@@ -377,76 +455,238 @@ def main():
             ctx.eval_plain(tfhe_amd.NAND, d1[k:k + 1], d2[k:k + 1], one_out, stream=sptr)
         torch.cuda.synchronize()
         plain_rate = args.plain_steps / (time.perf_counter() - tp)
+    ctx.close()
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "gate_blind_rotate_4096_nand_xor_traffic.json")
+    if os.path.exists(prof):
+        traffic = json.load(open(prof)).get("hbm_bytes_per_launch")
+    out = {"value": round(BATCH * steps / elapsed, 1), "unit": "bootstraps/s", "steps": steps,
+           "ms_per_step": round(elapsed * 1e3 / steps, 3),
+           "config": {"workload": "batched gate bootstraps: 4096 = 2048 NAND + 2048 XOR per step",
+                      "n": N_LWE, "N": N_RING, "bsk_gadget": "2^7 x 3", "ks_gadget": "2^2 x 8",
+                      "torus_bits": 32, "fft": "fp64 negacyclic, wavefront-resident"},
+           "roofline": {"bound": "valu", "kernel": "k_gate_blind_rotate", "achieved": round(achieved, 3),
+                        "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+                        "traffic": traffic, "algorithmic_flop_per_launch": F_PBS * half,
+                        "avg_launch_ms": round(br_avg, 4)},
+           "breakdown_ms": {"blind_rotate_per_2048": round(br_avg, 4), "keyswitch_per_2048": round(ks_avg, 4)},
+           "plain_fp64_per_bootstrap_rate": None if plain_rate is None else round(plain_rate, 2),
+           "decrypt_failures": failures}
+    if campaign is not None:
+        out["failure_campaign"] = campaign
+    return out
 
+
+AES_FORMS = {"exact": ("aes128_ctr", 0, 1, 0), "approx": ("aes128_ctr_one_product_pbs", 1, 1, 0),
+             "approx-mb2": ("aes128_ctr_one_product_pbs_2bit", 1, 2, 0),
+             "exact-lin": ("aes128_ctr_linear_pfks", 0, 1, 1),
+             "approx-lin": ("aes128_ctr_one_product_pbs_linear_pfks", 1, 1, 1),
+             "approx-mb2-lin": ("aes128_ctr_one_product_pbs_2bit_linear_pfks", 1, 2, 1),
+             "approx-auto-lin": ("aes128_ctr_one_product_pbs_auto_bits_linear_pfks", 1, 3, 1)}
+
+
+def aes_headline(args):
+    """AES-128 CTR transciphering of args.blocks blocks per step in the form args.aes_form:
+    W untimed steps, then K timed steps with every PBS launch bracketed by HIP events
+    inside the library (tfhe_cbs_timing_*) on the stream it runs on; then one launch of
+    each side batch size.  Returns (headline dict, side dict)."""
+    import numpy as np
+    import torch
+    import tfhe_amd
+    key_name, approx, mb, lin = AES_FORMS[args.aes_form]
+    sides = [int(b) for b in args.side_blocks.split(",") if b]
+    maxb = max([args.blocks] + sides)
+    ctx = tfhe_amd.CbsContext(max_blocks=maxb, params=tfhe_amd.cbs_params(approx, mb, lin))
+    sk = tfhe_amd.lwe_keygen(N_LWE, 3)
+    t0 = time.perf_counter()
+    ctx.keygen(sk, 3)
+    torch.cuda.synchronize()
+    keygen_s = time.perf_counter() - t0
+    rng = np.random.default_rng(11)
+    key = bytes(rng.integers(0, 256, 16, dtype=np.uint8))
+    ctx.set_round_keys(tfhe_amd.aes_encrypt_key_bits(sk, 5, key))
+    nonce = bytes(rng.integers(0, 256, 8, dtype=np.uint8))
+    d_ks = torch.empty((maxb, 128, N_LWE + 1), dtype=torch.int32, device="cuda")
+    s64 = sk.astype(np.uint64)
+
+    def workload(blocks):  # counters, seeded random plaintexts and their public AES-CTR ciphertexts
+        ctrs = [nonce + int(b).to_bytes(8, "big") for b in range(blocks)]
+        plain = [bytes(rng.integers(0, 256, 16, dtype=np.uint8)) for _ in range(blocks)]
+        cipher = [bytes(p ^ k for p, k in zip(pt, tfhe_amd.aes128_encrypt_plain(key, c)))
+                  for pt, c in zip(plain, ctrs)]
+        return ctrs, plain, cipher
+
+    def wrong_blocks(blocks, plain):
+        got = d_ks[:blocks].cpu().numpy().view(np.uint32)
+        ph = (got[:, :, -1].astype(np.uint64) - (got[:, :, :-1].astype(np.uint64) @ s64)) % 2 ** 32
+        bits = ((ph + (1 << 30)) >> 31) & 1
+        bad = 0
+        for b in range(blocks):
+            v = bytes(int(sum(int(bits[b, 8 * k + j]) << j for j in range(8))) for k in range(16))
+            bad += v != plain[b]
+        return int(bad)
+
+    blocks = args.blocks
+    ctrs, plain, cipher = workload(blocks)
+    tuned = None
+    if not args.no_aes_tune:
+        t0 = time.perf_counter()
+        tuned = ctx.autotune(blocks)
+        tuned["tune_s"] = round(time.perf_counter() - t0, 3)
+    for _ in range(args.warmup):
+        ctx.aes_ctr_transcipher(ctrs, cipher, d_ks)
+    torch.cuda.synchronize()
+    ctx.timing_enable(True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ctx.aes_ctr_transcipher(ctrs, cipher, d_ks)
+        print(f"aes128 step {k + 1}/{args.steps} issued", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    pbs_ms, launches, inputs = ctx.timing_read()
+    ctx.timing_enable(False)
+    bad = wrong_blocks(blocks, plain)
+    print(f"aes128 {blocks} blocks x {args.steps} steps: {elapsed:.2f} s, {bad} wrong blocks", file=sys.stderr,
+          flush=True)
+    per_launch_inputs = inputs / max(launches, 1)
+    avg_ms = pbs_ms / max(launches, 1)
+    tflops = per_launch_inputs * F_PBS_CB / (avg_ms * 1e-3) / 1e12
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "pbs_two_bit_aes_256_blocks_traffic.json")
+    if os.path.exists(prof):
+        traffic = json.load(open(prof)).get("hbm_bytes_per_launch")
+    head = {"value": round(blocks * args.steps / elapsed, 3), "elapsed_s": elapsed, "blocks_wrong": bad,
+            "keygen_s": round(keygen_s, 2), "tuned_shapes": tuned,
+            "roofline": {"bound": "valu", "kernel": "k_pbs2_manylut (one-product, two key bits per step)",
+                         "achieved": round(tflops, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(tflops / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
+                         "algorithmic_flop_per_launch": int(per_launch_inputs * F_PBS_CB),
+                         "avg_launch_ms": round(avg_ms, 3), "launches_timed": launches,
+                         "inputs_per_launch": int(per_launch_inputs),
+                         "pbs_share_of_step": round(pbs_ms / (elapsed * 1e3), 4)}}
+    side = {}
+    for sb in sides:
+        s_ctrs, s_plain, s_cipher = workload(sb)
+        s_tuned = None
+        if not args.no_aes_tune:
+            s_tuned = ctx.autotune(sb)
+        ctx.aes_ctr_transcipher(s_ctrs, s_cipher, d_ks)  # first launch of this size
+        torch.cuda.synchronize()
+        reps = 3 if sb <= 32 else 1
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.aes_ctr_transcipher(s_ctrs, s_cipher, d_ks)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        entry = {"blocks_per_s": round(sb / dt, 3), "s_per_launch": round(dt, 4), "reps": reps,
+                 "circuit_bootstraps_per_s": round(sb * 1280 / dt, 1), "blocks_wrong": wrong_blocks(sb, s_plain),
+                 "tuned_shapes": s_tuned}
+        if sb <= 32:  # the keystream replayed from its captured hipGraph
+            eager = d_ks[:sb].clone()
+            ctx.aes_ctr_transcipher(s_ctrs, s_cipher, d_ks, mode="graph")  # capture + first replay
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ctx.aes_ctr_transcipher(s_ctrs, s_cipher, d_ks, mode="graph")
+            torch.cuda.synchronize()
+            entry["graph_s_per_launch"] = round(time.perf_counter() - t0, 4)
+            entry["graph_equals_eager"] = bool(torch.equal(d_ks[:sb], eager))
+        side[str(sb)] = entry
+        print(f"aes128 side batch {sb} blocks: {sb / dt:.2f} blocks/s, {entry['blocks_wrong']} wrong",
+              file=sys.stderr, flush=True)
+    side["kernels_at_17_blocks"] = aes_kernel_rooflines(ctx, approx, min(17, maxb) * 128, lin)
+    ctx.close()
+    return head, side
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5, help="timed AES steps (one transcipher call of --blocks blocks)")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--blocks", type=int, default=256, help="AES blocks per step (the headline batch)")
+    ap.add_argument("--side-blocks", default="1,17,1024",
+                    help="comma-separated AES batch sizes timed once each beside the headline ('' to skip)")
+    ap.add_argument("--aes-form", default="approx-auto-lin", choices=sorted(AES_FORMS),
+                    help="PBS / PFKS form of the headline: exact (two-limb) or approx (one FP64 product) PBS, "
+                         "-mb2 two key bits per step, -auto both key forms (two bits unless tuned to one), "
+                         "-lin the linear-key PFKS on the matrix cores")
+    ap.add_argument("--extra-forms", default="",
+                    help="comma-separated further AES forms timed at --extra-blocks (one launch per size)")
+    ap.add_argument("--extra-blocks", default="1,17,256")
+    ap.add_argument("--aes-plain-blocks", type=int, default=0,
+                    help="AES blocks timed in the per-bootstrap form of the exact PBS (a launch-overhead figure: "
+                         "every bootstrap, LUT and key switch is a launch of one; 0 to skip)")
+    ap.add_argument("--no-aes-tune", action="store_true",
+                    help="keep the kernels' model launch shapes (no tfhe_cbs_autotune before each AES batch)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--gate-steps", type=int, default=20, help="timed steps of the gate line (0 to skip)")
+    ap.add_argument("--plain-steps", type=int, default=3, help="plain per-bootstrap gate samples")
+    ap.add_argument("--campaign", type=int, default=1_000_000,
+                    help="gate bootstraps in the decryption-failure campaign (0 to skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run three 512 MiB device copies (the HBM-counter calibration kernel)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              f"(bootstraps and AES blocks are independent: a multi-GPU run would give each GPU a key replica "
+              f"and a slice of the batch)")
+        sys.exit(2)
+
+    import torch
+    import tfhe_amd
+    torch.cuda.init()
+    tfhe_amd.lib()
+
+    head, side = aes_headline(args)
+    elapsed = head.pop("elapsed_s")
+    line = {
+        "metric": METRIC,
+        "value": head.pop("value"),
+        "unit": "AES-128 blocks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"AES-128 CTR transciphering, {args.blocks} blocks per launch (uneven-batch config "
+                               f"1/17/256/1024 blocks; 256 the headline, the others side keys)",
+                   "blocks_per_step": args.blocks, "form": args.aes_form,
+                   "n": N_LWE, "N": N_CB, "pbs_gadget": "2^9 x 4", "cb_gadget": "2^8 x 2",
+                   "pfks_gadget": "2^4 x 8", "ks_gadget": "2^2 x 8",
+                   "pbs": "one FP64 product per digit polynomial, two key bits per blind-rotation step",
+                   "pfks": "linear 40-bit key, int8 GEMM on the matrix cores",
+                   "circuit_bootstraps_per_step": 1280 * args.blocks},
+        "roofline": head.pop("roofline"),
+        "blocks_wrong": head.pop("blocks_wrong"),
+        "aes128_side_batches": side,
+        "aes128_keygen_s": head.pop("keygen_s"),
+        "aes128_tuned_shapes": head.pop("tuned_shapes"),
+    }
+    if args.extra_forms:
+        sizes = [int(b) for b in args.extra_blocks.split(",") if b]
+        for f in [f for f in args.extra_forms.split(",") if f]:
+            key, approx, mb, lin = AES_FORMS[f]
+            line[key] = aes_ctr_bench(sizes, pbs_approx=approx, pbs_multibit=mb,
+                                      plain_blocks=args.aes_plain_blocks if f == "exact" else 0,
+                                      tune=not args.no_aes_tune, pfks_linear=lin)
+    line["one_product_vs_exact_product"] = one_product_error()
+    line["split_fp32_product"] = split32_product_bench()
+    if args.gate_steps > 0:
+        line["gate_bootstraps"] = gate_bench(args)
     if args.copy_calibration:
         src = torch.empty(512 << 20, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
-
-    traffic = None
-    prof = os.path.join(ROOT, "profiles", "gate_blind_rotate_4096_nand_xor_traffic.json")
-    if os.path.exists(prof):
-        traffic = json.load(open(prof)).get("hbm_bytes_per_launch")
-
-    ms_per_step = elapsed * 1e3 / args.steps
-    line = {
-        "metric": METRIC,
-        "value": round(BATCH * args.steps / elapsed, 1),
-        "unit": "bootstraps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": "batched gate bootstraps: 4096 = 2048 NAND + 2048 XOR per step",
-                   "n": N_LWE, "N": N_RING, "bsk_gadget": "2^7 x 3", "ks_gadget": "2^2 x 8",
-                   "torus_bits": 32, "fft": "fp64 negacyclic, wavefront-resident", "batch": BATCH},
-        "roofline": {"bound": "valu", "kernel": "k_gate_blind_rotate",
-                     "achieved": round(achieved_tflops, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved_tflops / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
-                     "algorithmic_flop_per_launch": F_PBS * half,
-                     "avg_launch_ms": round(br_avg, 4)},
-        "breakdown_ms": {"blind_rotate_per_2048": round(br_avg, 4), "keyswitch_per_2048": round(ks_avg, 4)},
-        "plain_fp64_per_bootstrap_rate": None if plain_rate is None else round(plain_rate, 2),
-        "decrypt_failures": failures,
-    }
-    if campaign is not None:
-        line["failure_campaign"] = campaign
-    ctx.close()
-    if args.valu_rates:
-        line["valu_fma_tflops"] = {"fp64": round(tfhe_amd.probe_fma_tflops(1), 2),
-                                   "fp32_unpacked": round(tfhe_amd.probe_fma_tflops(0), 2)}
-    if args.aes_blocks:
-        sizes = [int(b) for b in args.aes_blocks.split(",") if b]
-        forms = {"exact": ("aes128_ctr", 0, 1, 0), "approx": ("aes128_ctr_one_product_pbs", 1, 1, 0),
-                 "approx-mb2": ("aes128_ctr_one_product_pbs_2bit", 1, 2, 0),
-                 "exact-lin": ("aes128_ctr_linear_pfks", 0, 1, 1),
-                 "approx-lin": ("aes128_ctr_one_product_pbs_linear_pfks", 1, 1, 1),
-                 "approx-mb2-lin": ("aes128_ctr_one_product_pbs_2bit_linear_pfks", 1, 2, 1),
-                 "approx-auto-lin": ("aes128_ctr_one_product_pbs_auto_bits_linear_pfks", 1, 3, 1)}
-        for f in [f for f in args.aes_pbs.split(",") if f]:
-            if f not in forms:
-                sys.exit(f"--aes-pbs: unknown form {f!r} ({', '.join(forms)})")
-            key, approx, mb, lin = forms[f]
-            line[key] = aes_ctr_bench(sizes, pbs_approx=approx, pbs_multibit=mb,
-                                      plain_blocks=args.aes_plain_blocks if f == "exact" else 0,
-                                      tune=not args.no_aes_tune, pfks_linear=lin)
-            if approx:
-                line[key]["one_product_vs_exact_product"] = one_product_error()
-        # batched blocks/s over the plain FP64 per-bootstrap form's
-        plain = line.get("aes128_ctr", {}).get("batches", {}).get("plain_per_bootstrap")
-        if plain:
-            best = max((v["blocks_per_s"] for k in forms.values() if k[0] in line
-                        for b, v in line[k[0]]["batches"].items() if b.isdigit()), default=None)
-            line["aes128_batched_over_plain"] = None if best is None else round(best / plain["blocks_per_s"], 1)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline()
+        line["cpu_baseline"] = aes_cpu_baseline()
+        if args.gate_steps > 0:
+            line["gate_bootstraps"]["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
 
 

@@ -10,43 +10,12 @@
 
 #include "../../include/tfhe_amd.h"
 #include "capi_common.hpp"
+#include "cbs_layout.hpp"
 #include "kernels.hpp"
 
 using namespace tfhe;
 
 namespace {
-constexpr int kN = 2048;
-
-bool cbs_params_supported(const tfhe_cbs_params* p) {
-    return p && p->N == kN && p->bk_logB == 9 && p->bk_l == 4 && p->cb_logB == 8 && p->cb_l == 2 && p->pf_logB == 4 &&
-           p->pf_l == 8 && p->ks_logB == 2 && p->ks_l == 8 && p->n > 0 && p->n <= 1024 && p->lwe_nshift >= 1 &&
-           p->lwe_nshift <= 31 && p->rlwe_nshift >= -31 && p->rlwe_nshift <= 31 && p->pf_nshift >= 1 &&
-           p->pf_nshift <= 31 && (p->pbs_approx == 0 || p->pbs_approx == 1) && p->pbs_multibit >= 0 &&
-           p->pbs_multibit <= 3 && (p->pbs_multibit < 2 || (p->pbs_approx == 1 && p->n % 2 == 0)) &&
-           (p->pfks_linear == 0 || p->pfks_linear == 1);
-}
-
-int pbs_limbs(const tfhe_cbs_params& p) { return p.pbs_approx ? 1 : 2; }
-int pbs_mb(const tfhe_cbs_params& p) { return p.pbs_multibit == 2 ? 2 : 1; }
-// pbs_multibit = 3: both key forms, one or two key bits per step chosen per launch size
-bool pbs_auto(const tfhe_cbs_params& p) { return p.pbs_multibit == 3; }
-// RGSWs of the BSK2: one per key bit, three pattern keys per key-bit pair, or both
-size_t bsk2_rgsw(const tfhe_cbs_params& p) {
-    return pbs_auto(p) ? (size_t)p.n + (size_t)p.n / 2 * 3 : pbs_mb(p) == 2 ? (size_t)p.n / 2 * 3 : (size_t)p.n;
-}
-// first two-bit pattern RGSW in the BSK2 of an auto context (after the n one-bit RGSWs)
-size_t bsk2_mb2_first(const tfhe_cbs_params& p) { return pbs_auto(p) ? (size_t)p.n : 0; }
-// capacity bound: keeps every 32-bit count (2 x inputs for the PFKS) and every grid y
-// dimension (padded inputs / 64) of the path in range
-constexpr uint32_t kMaxBlocks = 4096;
-// captured keystream graphs kept per context
-constexpr size_t kMaxGraphs = 8;
-// inputs per linear-form PFKS launch (digit scratch pfkl_digit_bytes(kPfklChunk) = 134 MB)
-constexpr uint32_t kPfklChunk = 8192;
-size_t bsk2_cplx(const tfhe_cbs_params& p) { return bsk2_rgsw(p) * 8 * 2 * pbs_limbs(p) * 1024; }
-size_t pfks_words_per_u() { return (size_t)(kN + 1) * 8 * 15 * 2 * kN; }
-size_t ksk2_words(const tfhe_cbs_params& p) { return (size_t)kN * 8 * 4 * (p.n + 1); }
-
 // ---- AES-128 pieces computed by the library itself (client-side helpers and
 // the bit-level linear layer); the S-box is GF(2^8) inversion + affine map ----
 uint8_t gmul(uint8_t a, uint8_t b) {
@@ -196,6 +165,12 @@ struct tfhe_cbs_ctx {
     // launch shapes chosen by tfhe_cbs_autotune, keyed by the launch's size (absent: the
     // kernels' own models): PBS inputs -> FFT form, PFKS LWEs -> i-slices, key switches -> i-slices
     std::map<uint32_t, int> tune_pbs, tune_pfks, tune_ks;
+    // kernel timing (tfhe_cbs_timing_enable / _read): an event pair around every PBS launch
+    // made outside stream capture while enabled; the pairs are reused after each read
+    bool timing = false;
+    std::vector<hipEvent_t> tev;  // [2 k] before, [2 k + 1] after launch k
+    size_t tev_used = 0;          // events recorded since the last read
+    uint64_t t_inputs = 0;        // PBS inputs of the timed launches
 };
 
 extern "C" int tfhe_cbs_ctx_destroy(tfhe_cbs_ctx* c) {
@@ -215,6 +190,7 @@ extern "C" int tfhe_cbs_ctx_destroy(tfhe_cbs_ctx* c) {
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->cstream) (void)hipStreamDestroy(c->cstream);
+    for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
     c->tables.release();
     delete c;
     return TFHE_OK;
@@ -466,6 +442,32 @@ extern "C" int tfhe_cbs_keygen(tfhe_cbs_ctx* c, const int32_t* lwe_sk, uint64_t 
     return rc;
 }
 
+const tfhe_cbs_params* tfhe::cbs_ctx_params(const tfhe_cbs_ctx* c) { return &c->p; }
+
+extern "C" int tfhe_cbs_save_keys(const tfhe_cbs_ctx* c, const char* path) {
+    if (!c || !path) return fail(TFHE_E_PARAM, "tfhe_cbs_save_keys: bad argument");
+    if (!c->has_keys || c->lwe_sk.empty())
+        return fail(TFHE_E_STATE, "tfhe_cbs_save_keys: the context's keys were not made by tfhe_cbs_keygen "
+                                  "(uploaded keys are written from their sources with tfhe_cbs_write_keys)");
+    // the context holds the BSK2 in its Fourier form: its coefficient rows are made again
+    // from the seed; the PFKS key and the KSK2 are written as the context holds them
+    const auto& p = c->p;
+    int32_t* d_s = nullptr;
+    uint64_t* d_bsk = nullptr;
+    int rc = dmalloc((void**)&d_s, p.n * sizeof(int32_t));
+    rc = rc ? rc : dmalloc((void**)&d_bsk, bsk2_words(p) * sizeof(uint64_t));
+    if (!rc) rc = hipcheck(hipMemcpy(d_s, c->lwe_sk.data(), p.n * sizeof(int32_t), hipMemcpyHostToDevice), "sk");
+    if (!rc) rc = tfhe_gen_bsk64(&p, c->seed, d_s, c->d_K2, d_bsk, nullptr);
+    if (!rc) {
+        const void* const src[4] = {c->K2.data(), d_bsk,
+                                    p.pfks_linear ? (const void*)c->d_pfkl : (const void*)c->d_pfks, c->d_ksk2};
+        rc = write_key_blob(path, p, c->seed, src);
+    }
+    (void)hipFree(d_s);
+    (void)hipFree(d_bsk);
+    return rc;
+}
+
 extern "C" int tfhe_cbs_get_rlwe_sk(const tfhe_cbs_ctx* c, int32_t* out) {
     if (!c || !out || c->K2.empty()) return fail(TFHE_E_STATE, "tfhe_cbs_get_rlwe_sk: no keys");
     std::memcpy(out, c->K2.data(), kN * sizeof(int32_t));
@@ -535,8 +537,80 @@ extern "C" int tfhe_cbs_pbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32
         if (mb == 2) form -= 2;
     }
     const cplx* bsk = c->d_bsk2f + (mb == 2 ? bsk2_mb2_first(c->p) * 8 * 2 * 1024 : 0);
-    launch_pbs2(d_lwe, batch, c->p.n, bsk, pbs_limbs(c->p), mb, c->tables.T, d_out, (hipStream_t)stream, form);
-    return check_launch("pbs2");
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e1 = nullptr;
+    if (c->timing) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        rc = hipcheck(hipStreamIsCapturing(st, &cs), "pbs timing");
+        if (rc) return rc;
+        if (cs == hipStreamCaptureStatusNone) {
+            while (c->tev.size() < c->tev_used + 2) {
+                hipEvent_t e = nullptr;
+                rc = hipcheck(hipEventCreate(&e), "pbs timing event");
+                if (rc) return rc;
+                c->tev.push_back(e);
+            }
+            rc = hipcheck(hipEventRecord(c->tev[c->tev_used], st), "pbs timing");
+            if (rc) return rc;
+            e1 = c->tev[c->tev_used + 1];
+        }
+    }
+    launch_pbs2(d_lwe, batch, c->p.n, bsk, pbs_limbs(c->p), mb, c->tables.T, d_out, st, form);
+    rc = check_launch("pbs2");
+    if (!rc && e1) {
+        rc = hipcheck(hipEventRecord(e1, st), "pbs timing");
+        c->tev_used += 2;
+        c->t_inputs += batch;
+    }
+    return rc;
+}
+
+extern "C" int tfhe_cbs_bootstrap_many_lut(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t batch,
+                                           const uint64_t* d_lut, uint32_t bitwidth, uint64_t* d_out, void* stream) {
+    int rc = check_cbs(c, batch, c ? c->max_batch : 0);
+    if (rc) return rc;
+    if (!d_lwe || !d_lut || !d_out) return fail(TFHE_E_PARAM, "bootstrap_many_lut: NULL buffer");
+    if (bitwidth > 11) return fail(TFHE_E_PARAM, "bootstrap_many_lut: bitwidth must be at most 11 (2N = 4096)");
+    if (!batch) return TFHE_OK;
+    int mb = pbs_mb(c->p), form = tuned(c->tune_pbs, batch, -1);
+    if (pbs_auto(c->p)) {
+        if (form < 0) form = auto_pbs_model(batch);
+        mb = form >= 2 ? 2 : 1;
+        if (mb == 2) form -= 2;
+    }
+    const cplx* bsk = c->d_bsk2f + (mb == 2 ? bsk2_mb2_first(c->p) * 8 * 2 * 1024 : 0);
+    PbsLut io;
+    io.lut = d_lut;
+    io.rlwe = d_out;
+    io.bw = bitwidth;
+    launch_pbs2(d_lwe, batch, c->p.n, bsk, pbs_limbs(c->p), mb, c->tables.T, nullptr, (hipStream_t)stream, form, io);
+    return check_launch("bootstrap_many_lut");
+}
+
+extern "C" int tfhe_cbs_timing_enable(tfhe_cbs_ctx* c, int on) {
+    if (!c) return fail(TFHE_E_PARAM, "timing: NULL context");
+    c->timing = on != 0;
+    c->tev_used = 0;
+    c->t_inputs = 0;
+    return TFHE_OK;
+}
+
+extern "C" int tfhe_cbs_timing_read(tfhe_cbs_ctx* c, double* pbs_ms, uint32_t* launches, uint64_t* inputs) {
+    if (!c || !pbs_ms || !launches || !inputs) return fail(TFHE_E_PARAM, "timing: bad argument");
+    double total = 0.0;
+    for (size_t k = 0; k < c->tev_used; k += 2) {
+        int rc = hipcheck(hipEventSynchronize(c->tev[k + 1]), "timing read");
+        float ms = 0.0f;
+        if (!rc) rc = hipcheck(hipEventElapsedTime(&ms, c->tev[k], c->tev[k + 1]), "timing read");
+        if (rc) return rc;
+        total += ms;
+    }
+    *pbs_ms = total;
+    *launches = (uint32_t)(c->tev_used / 2);
+    *inputs = c->t_inputs;
+    c->tev_used = 0;
+    c->t_inputs = 0;
+    return TFHE_OK;
 }
 
 // linear-form PFKS in launches of at most kPfklChunk inputs (the digit scratch)
@@ -641,6 +715,28 @@ extern "C" int tfhe_cbs_batch_from_device_lwe(tfhe_cbs_ctx* c, const uint64_t* d
     // packed inputs in the spare AES state buffer (context scratch)
     launch_pack_lwe64(d_a, d_b, batch, c->p.n, a_stride, c->sc.state[1], (hipStream_t)stream);
     rc = check_launch("pack_lwe64");
+    if (!rc) rc = tfhe_cbs_batch(c, c->sc.state[1], batch, c->sc.rgsw, stream);
+    if (rc) return rc;
+    view->d_rgsw = c->sc.rgsw;
+    view->batch = batch;
+    view->digits = 2;
+    view->N = kN;
+    view->per_ciphertext_elems = (size_t)2 * 4 * kN;
+    view->stream = stream;
+    return TFHE_OK;
+}
+
+extern "C" int tfhe_cbs_batch_to_device(tfhe_cbs_ctx* c, const uint32_t* h_lwe, uint32_t batch,
+                                        tfhe_rgsw_batch_view* view, void* stream) {
+    if (!c || !view) return fail(TFHE_E_PARAM, "cbs_batch_to_device: bad argument");
+    *view = tfhe_rgsw_batch_view{};
+    if (!h_lwe || batch == 0) return TFHE_OK;  // the reference returns an empty view
+    int rc = check_cbs(c, batch, c->max_batch);
+    if (rc) return rc;
+    // staged in the spare AES state buffer (context scratch), as the device-LWE form packs there
+    rc = hipcheck(hipMemcpyAsync(c->sc.state[1], h_lwe, (size_t)batch * (c->p.n + 1) * sizeof(uint32_t),
+                                 hipMemcpyHostToDevice, (hipStream_t)stream),
+                  "cbs_batch_to_device upload");
     if (!rc) rc = tfhe_cbs_batch(c, c->sc.state[1], batch, c->sc.rgsw, stream);
     if (rc) return rc;
     view->d_rgsw = c->sc.rgsw;
@@ -865,6 +961,22 @@ extern "C" int tfhe_polymul2048_batch(const int32_t* d_x, const uint64_t* d_y, u
 extern "C" int tfhe_polymul2048_approx_batch(const int32_t* d_x, const uint64_t* d_y, uint32_t batch,
                                              uint64_t* d_out, void* stream) {
     return polymul2048(d_x, d_y, batch, d_out, stream, 1);
+}
+
+extern "C" int tfhe_polymul2048_split32_batch(const int32_t* d_x, const uint32_t* d_y, uint32_t batch, int form,
+                                              uint32_t* d_out, void* stream) {
+    if (!d_x || !d_y || !d_out) return fail(TFHE_E_PARAM, "polymul2048_split32: NULL argument");
+    if (form < 0 || form > 2) return fail(TFHE_E_PARAM, "polymul2048_split32: form must be 0, 1 or 2");
+    if (!batch) return TFHE_OK;
+    DeviceTables1024* tables = nullptr;
+    int rc = stateless_tables(tables);
+    if (rc) return rc;
+    DeviceTables1024F* ftables = nullptr;
+    rc = per_device_tables(ftables, [](DeviceTables1024F& t) { return t.init(); });
+    if (rc) return rc;
+    if (launch_polymul2048_split32(d_x, d_y, batch, ftables->T, tables->T, d_out, (hipStream_t)stream, form))
+        return fail(TFHE_E_PARAM, "polymul2048_split32: unsupported form");
+    return check_launch("polymul2048_split32");
 }
 
 // ---------------------------------------------------------------------------

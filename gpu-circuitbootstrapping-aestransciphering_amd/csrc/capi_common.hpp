@@ -63,7 +63,8 @@ struct DeviceTables1024 {
     FFT1024Tables T{};
     cplx* d_buf = nullptr;
     static constexpr int kCount = 1792 + 4096;
-    int init() {
+    // host values of the tables, in device order
+    static std::vector<cplx> host_values() {
         std::vector<cplx> h(kCount);
         const double PI = 3.14159265358979323846264338327950288;
         const int N = 2048;
@@ -82,9 +83,38 @@ struct DeviceTables1024 {
                     put(768 + (r * 4 + s) * 64 + l, PI * r / N - 2.0 * PI * (double)((r * k) % 1024) / 1024.0);
                 }
         for (int j = 0; j < 2 * N; ++j) put(1792 + j, PI * j / N);
+        return h;
+    }
+    int init() {
+        const std::vector<cplx> h = host_values();
         int rc = dmalloc((void**)&d_buf, h.size() * sizeof(cplx));
         if (rc) return rc;
         rc = hipcheck(hipMemcpy(d_buf, h.data(), h.size() * sizeof(cplx), hipMemcpyHostToDevice), "tables1024");
+        if (rc) return rc;
+        T.p1t = d_buf;
+        T.p2 = d_buf + 256;
+        T.p3 = d_buf + 512;
+        T.tr = d_buf + 768;
+        T.rot = d_buf + 1792;
+        return TFHE_OK;
+    }
+    void release() {
+        if (d_buf) (void)hipFree(d_buf);
+        d_buf = nullptr;
+    }
+};
+
+// The same tables rounded to FP32 (the split-precision product).
+struct DeviceTables1024F {
+    FFT1024TablesF T{};
+    cplxf* d_buf = nullptr;
+    int init() {
+        const std::vector<cplx> h = DeviceTables1024::host_values();
+        std::vector<cplxf> f(h.size());
+        for (size_t k = 0; k < h.size(); ++k) f[k] = {(float)h[k].re, (float)h[k].im};
+        int rc = dmalloc((void**)&d_buf, f.size() * sizeof(cplxf));
+        if (rc) return rc;
+        rc = hipcheck(hipMemcpy(d_buf, f.data(), f.size() * sizeof(cplxf), hipMemcpyHostToDevice), "tables1024f");
         if (rc) return rc;
         T.p1t = d_buf;
         T.p2 = d_buf + 256;

@@ -38,63 +38,6 @@ __device__ __forceinline__ void split_limbs(uint64_t v, double& lo, double& hi) 
     hi = (double)(int32_t)(uint32_t)((v - (uint64_t)(int64_t)l) >> 32);
 }
 
-// FFT1024 forward/inverse with the cross twiddles taken from registers.  The
-// cross-wave radix-4 output of wave w is either selected on w (BF = false:
-// uniform branches, which also bound the scheduler's reordering and keep the
-// kernel within 256 registers, i.e. two waves per SIMD) or computed branch-free
-// (BF = true: fewer instructions and a longer straight-line schedule, but more
-// registers; faster where there is one wave per SIMD anyway).  Both round
-// at the same points, so the outputs are identical.
-template <bool BF, bool RT = false>
-__device__ __forceinline__ void f1024_fwd(cplx (&x)[4], cplx* lds, cplx* xbuf, const FFT1024Tables& T,
-                                          const cplx (&trw)[4], int w, int lane, const Tw256Regs& tw = Tw256Regs{}) {
-    fft256_fwd_t<RT>(x, lds, T, tw, lane);
-#pragma unroll
-    for (int s = 0; s < 4; ++s) xbuf[(w * 4 + s) * 64 + lane] = cmul(x[s], trw[s]);
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        cplx y0 = xbuf[(0 * 4 + s) * 64 + lane], y1 = xbuf[(1 * 4 + s) * 64 + lane];
-        cplx y2 = xbuf[(2 * 4 + s) * 64 + lane], y3 = xbuf[(3 * 4 + s) * 64 + lane];
-        if constexpr (BF) {
-            x[s] = xwave_dft4<false>(y0, y1, y2, y3, w);
-        } else {
-            cplx s02 = cadd(y0, y2), d02 = csub(y0, y2), s13 = cadd(y1, y3), d13 = mul_mi<false>(csub(y1, y3));
-            x[s] = (w == 0) ? cadd(s02, s13) : (w == 1) ? cadd(d02, d13) : (w == 2) ? csub(s02, s13) : csub(d02, d13);
-        }
-    }
-}
-
-template <bool BF, bool RT = false>
-__device__ __forceinline__ void f1024_inv(cplx (&x)[4], cplx* lds, cplx* xbuf, const FFT1024Tables& T,
-                                          const cplx (&trw)[4], int w, int lane, const Tw256Regs& tw = Tw256Regs{}) {
-#pragma unroll
-    for (int s = 0; s < 4; ++s) xbuf[(w * 4 + s) * 64 + lane] = x[s];
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        cplx z0 = xbuf[(0 * 4 + s) * 64 + lane], z1 = xbuf[(1 * 4 + s) * 64 + lane];
-        cplx z2 = xbuf[(2 * 4 + s) * 64 + lane], z3 = xbuf[(3 * 4 + s) * 64 + lane];
-        cplx y;
-        if constexpr (BF) {
-            y = xwave_dft4<true>(z0, z1, z2, z3, w);
-        } else {
-            cplx s02 = cadd(z0, z2), d02 = csub(z0, z2), s13 = cadd(z1, z3), d13 = mul_mi<true>(csub(z1, z3));
-            y = (w == 0) ? cadd(s02, s13) : (w == 1) ? cadd(d02, d13) : (w == 2) ? csub(s02, s13) : csub(d02, d13);
-        }
-        x[s] = cmulc(y, trw[s]);
-    }
-    fft256_inv_t<RT>(x, lds, T, tw, lane);
-}
-
-__device__ __forceinline__ void load_trw(const FFT1024Tables& Tg, int w, int lane, cplx (&trw)[4]) {
-#pragma unroll
-    for (int s = 0; s < 4; ++s) trw[s] = Tg.tr[(w * 4 + s) * 64 + lane];
-}
-
-// coefficient owned by (wave w, lane, slot a): 4 (64 a + lane) + w; slot a + 4 is +1024
-__device__ __forceinline__ int own_coeff(int w, int lane, int a) { return 4 * (64 * a + lane) + w; }
-
 // ---------------------------------------------------------------------------
 // Fourier conversion: u64 polynomials -> [poly][limb][q][s][lane] (1024 cplx per limb)
 // ---------------------------------------------------------------------------
@@ -428,6 +371,10 @@ __device__ __forceinline__ double pbs_digit(uint64_t tp, int j) {
 __device__ __forceinline__ uint32_t special_ms_2048(uint32_t v) {
     return (uint32_t)((((uint64_t)v + (1ull << 20)) >> 21) << 1) & 4095u;
 }
+// the same for any bw in [0, 11] (the reference's gpu_SpecilMS, cirbts-utils.cu:52-61)
+__device__ __forceinline__ uint32_t special_ms_2048_bw(uint32_t v, uint32_t bw) {
+    return (uint32_t)((((uint64_t)v + (1ull << (19 + bw))) >> (20 + bw)) << bw) & 4095u;
+}
 
 // round(y * 2^64) mod 2^64 for a torus value held as a double.  f = y - rint(y)
 // in [-1/2, 1/2] is exact; its top 32 bits are rint(f 2^32), taken with the
@@ -461,10 +408,10 @@ __device__ __forceinline__ uint64_t torus64_from_double(double y) {
 // SIMD and issues each pattern's 8 values as one batch, the next pattern's before the
 // current pattern's MACs; WIDE = true (launches of at most one workgroup per CU)
 // issues all 24 at once with one wave per SIMD.
-template <bool BF, int LIMBS, int MB = 1, bool WIDE = false>
+template <bool BF, int LIMBS, int MB = 1, bool WIDE = false, bool LUT = false>
 __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylut(const uint32_t* __restrict__ lwe, uint32_t batch, int n,
                                                      const cplx* __restrict__ bskf, FFT1024Tables Tg,
-                                                     uint64_t* __restrict__ out) {
+                                                     uint64_t* __restrict__ out, PbsLut io) {
     __shared__ cplx tl[768];
     __shared__ cplx lds_all[4][F256_LDS_CPLX];
     __shared__ cplx xbuf[2][1024];
@@ -491,19 +438,39 @@ __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylu
     for (uint32_t cidx = blockIdx.x; cidx < batch; cidx += gridDim.x) {
         const uint32_t* ct = lwe + (size_t)cidx * (n + 1);
         __syncthreads();
-        for (int k = threadIdx.x; k < n; k += blockDim.x) at[k] = (uint16_t)special_ms_2048(ct[k]);
-        const uint32_t bt = special_ms_2048(ct[n] + (1u << 30));
+        uint32_t bt;
+        if constexpr (LUT) {
+            // rotation by X^{+a~_i s_i} (the step multiplies by X^{-at[i]}), so that acc ends
+            // as LUT X^{-phase~}
+            for (int k = threadIdx.x; k < n; k += blockDim.x)
+                at[k] = (uint16_t)((Q2 - special_ms_2048_bw(ct[k], io.bw)) & (Q2 - 1));
+            bt = special_ms_2048_bw(ct[n], io.bw);
+        } else {
+            for (int k = threadIdx.x; k < n; k += blockDim.x) at[k] = (uint16_t)special_ms_2048(ct[k]);
+            bt = special_ms_2048(ct[n] + (1u << 30));
+        }
         __syncthreads();
 
         uint64_t acc[2][8];
+        if constexpr (LUT) {  // acc = (0, LUT X^-bt): coefficient c is LUT[m], m = c + bt mod 2N, negated for m >= N
 #pragma unroll
-        for (int h = 0; h < 8; ++h) {
-            const uint32_t c = (uint32_t)own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0);
-            const uint32_t arg = (bt + Q2 - (c & ~1u)) & (Q2 - 1);
-            uint64_t v = 0 - ((c & 1) ? mu1 : mu0);
-            if (arg >= N) v = 0 - v;
-            acc[0][h] = 0;
-            acc[1][h] = v;
+            for (int h = 0; h < 8; ++h) {
+                const uint32_t c = (uint32_t)own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0);
+                const uint32_t m = (c + bt) & (Q2 - 1);
+                const uint64_t v = io.lut[m & (N - 1)];
+                acc[0][h] = 0;
+                acc[1][h] = m >= N ? 0 - v : v;
+            }
+        } else {
+#pragma unroll
+            for (int h = 0; h < 8; ++h) {
+                const uint32_t c = (uint32_t)own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0);
+                const uint32_t arg = (bt + Q2 - (c & ~1u)) & (Q2 - 1);
+                uint64_t v = 0 - ((c & 1) ? mu1 : mu0);
+                if (arg >= N) v = 0 - v;
+                acc[0][h] = 0;
+                acc[1][h] = v;
+            }
         }
         int buf = 0;
         for (int i = 0; i < n; i += MB) {
@@ -677,6 +644,16 @@ __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylu
                     }
                 __syncthreads();
             }
+        }
+        if constexpr (LUT) {  // the whole RLWE (mask, body), Gpow_j / 2 added to body coefficient j < 2
+            uint64_t* r = io.rlwe + (size_t)cidx * 2 * N;
+#pragma unroll
+            for (int h = 0; h < 8; ++h) {
+                const uint32_t c = (uint32_t)own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0);
+                r[c] = acc[0][h];
+                r[N + c] = acc[1][h] + (c == 0 ? mu0 : c == 1 ? mu1 : 0);
+            }
+            continue;
         }
         // sample extraction of coefficients 0 and 1
         uint64_t* o0 = out + (size_t)cidx * 2 * (N + 1);
@@ -1241,8 +1218,9 @@ int pbs_model_form(uint32_t batch, int limbs, int mb) {
     return 1;
 }
 
-void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, int limbs, int mb,
-                 const FFT1024Tables& T, uint64_t* out, hipStream_t st, int form) {
+template <bool LUT>
+static void launch_pbs2_t(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, int limbs, int mb,
+                          const FFT1024Tables& T, uint64_t* out, hipStream_t st, int form, const PbsLut& io) {
     // one workgroup per ciphertext; form: 1 = branch-free cross-wave FFT step,
     // 0 = branchy (two key bits per step: 1 = wide key batch, 0 = two waves per
     // SIMD), < 0 = pbs_model_form's choice
@@ -1250,19 +1228,29 @@ void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, i
     if (form < 0) form = pbs_model_form(batch, limbs, mb);
     if (limbs == 2) {
         if (form == 1)
-            k_pbs2_manylut<true, 2><<<batch <= 256 ? batch : big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+            k_pbs2_manylut<true, 2, 1, false, LUT><<<batch <= 256 ? batch : big, 256, 0, st>>>(lwe, batch, n, bskf, T,
+                                                                                               out, io);
         else
-            k_pbs2_manylut<false, 2><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+            k_pbs2_manylut<false, 2, 1, false, LUT><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out, io);
     } else if (mb == 1) {
         if (form == 0)
-            k_pbs2_manylut<false, 1><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+            k_pbs2_manylut<false, 1, 1, false, LUT><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out, io);
         else
-            k_pbs2_manylut<true, 1><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+            k_pbs2_manylut<true, 1, 1, false, LUT><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out, io);
     } else if (form == 1) {
-        k_pbs2_manylut<true, 1, 2, true><<<batch <= 256 ? batch : big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+        k_pbs2_manylut<true, 1, 2, true, LUT><<<batch <= 256 ? batch : big, 256, 0, st>>>(lwe, batch, n, bskf, T, out,
+                                                                                          io);
     } else {
-        k_pbs2_manylut<true, 1, 2><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out);
+        k_pbs2_manylut<true, 1, 2, false, LUT><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out, io);
     }
+}
+
+void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, int limbs, int mb,
+                 const FFT1024Tables& T, uint64_t* out, hipStream_t st, int form, const PbsLut& io) {
+    if (io.lut)
+        launch_pbs2_t<true>(lwe, batch, n, bskf, limbs, mb, T, out, st, form, io);
+    else
+        launch_pbs2_t<false>(lwe, batch, n, bskf, limbs, mb, T, out, st, form, io);
 }
 
 uint32_t pfks_bpad(uint32_t B) { return (B + PF_GROUP - 1) / PF_GROUP * PF_GROUP; }

@@ -111,6 +111,27 @@ __device__ __forceinline__ cplx mul_mi(cplx a) {
     else return {a.im, -a.re};
 }
 
+// Complex FP32 (the split-precision product): the same operations, single rounding
+// per fused multiply-add.  Scalar code only: the compiler is kept from packed FP32
+// forms by the build flags.
+struct cplxf { float re, im; };
+
+__device__ __forceinline__ cplxf cadd(cplxf a, cplxf b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cplxf csub(cplxf a, cplxf b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ cplxf cmul(cplxf a, cplxf b) {
+    return {fmaf(a.re, b.re, -a.im * b.im), fmaf(a.re, b.im, a.im * b.re)};
+}
+__device__ __forceinline__ cplxf cmulc(cplxf a, cplxf b) {
+    return {fmaf(a.re, b.re, a.im * b.im), fmaf(a.im, b.re, -a.re * b.im)};
+}
+template <bool INV>
+__device__ __forceinline__ cplxf mul_mi(cplxf a) {
+    if constexpr (INV) return {-a.im, a.re};
+    else return {a.im, -a.re};
+}
+__device__ __forceinline__ double sfma(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float sfma(float a, float b, float c) { return fmaf(a, b, c); }
+
 // 8-point DFT in registers, natural order in and out; forward uses exp(-2 pi i/8).
 template <bool INV>
 __device__ __forceinline__ void dft8(cplx (&x)[8]) {

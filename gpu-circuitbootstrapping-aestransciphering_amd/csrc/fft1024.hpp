@@ -18,28 +18,35 @@ namespace tfhe {
 
 constexpr int F256_LDS_CPLX = 320;  // per-wave transpose buffer (bank-staggered rows)
 
-struct FFT1024Tables {
-    const cplx* p1t;  // [4 k0][64 b]  (w^4)^b * W256^(b k0)
-    const cplx* p2;   // [4 e][64 L]   W64^((L mod 16) e)
-    const cplx* p3;   // [4 p][64 L]   W16^((L mod 4) p)
-    const cplx* tr;   // [4 r][4 s][64 lane]  w^r W1024^(r k(lane, s))
-    const cplx* rot;  // [2N] w^j: frequency k (wave q, slot s, lane) is the polynomial at w^(1 - 4k),
-                      // k = (lane >> 4) + 4 ((lane >> 2) & 3) + 16 (lane & 3) + 64 s + 256 q
+// The transform is written once for a complex type C: cplx (FP64, every form of the
+// bootstraps) or cplxf (FP32, the split-precision product); tables hold C values.
+template <class C>
+struct FFT1024TablesT {
+    const C* p1t;  // [4 k0][64 b]  (w^4)^b * W256^(b k0)
+    const C* p2;   // [4 e][64 L]   W64^((L mod 16) e)
+    const C* p3;   // [4 p][64 L]   W16^((L mod 4) p)
+    const C* tr;   // [4 r][4 s][64 lane]  w^r W1024^(r k(lane, s))
+    const C* rot;  // [2N] w^j: frequency k (wave q, slot s, lane) is the polynomial at w^(1 - 4k),
+                   // k = (lane >> 4) + 4 ((lane >> 2) & 3) + 16 (lane & 3) + 64 s + 256 q
 };
+using FFT1024Tables = FFT1024TablesT<cplx>;
+using FFT1024TablesF = FFT1024TablesT<cplxf>;
 
 // (w^4)^(64 a) = exp(i pi a / 8), a < 4
-__device__ __forceinline__ cplx twist4_const(int a) {
-    constexpr double C[4][2] = {{1.0, 0.0},
+template <class C = cplx>
+__device__ __forceinline__ C twist4_const(int a) {
+    using S = decltype(C{}.re);
+    constexpr double K[4][2] = {{1.0, 0.0},
                                 {0.92387953251128674, 0.38268343236508978},
                                 {0.70710678118654757, 0.70710678118654746},
                                 {0.38268343236508984, 0.92387953251128674}};
-    return {C[a][0], C[a][1]};
+    return {(S)K[a][0], (S)K[a][1]};
 }
 
-template <bool INV>
-__device__ __forceinline__ void dft4(cplx (&x)[4]) {
-    cplx a0 = cadd(x[0], x[2]), a1 = csub(x[0], x[2]);
-    cplx b0 = cadd(x[1], x[3]), b1 = mul_mi<INV>(csub(x[1], x[3]));
+template <bool INV, class C>
+__device__ __forceinline__ void dft4(C (&x)[4]) {
+    C a0 = cadd(x[0], x[2]), a1 = csub(x[0], x[2]);
+    C b0 = cadd(x[1], x[3]), b1 = mul_mi<INV>(csub(x[1], x[3]));
     x[0] = cadd(a0, b0);
     x[2] = csub(a0, b0);
     x[1] = cadd(a1, b1);
@@ -54,12 +61,15 @@ __device__ __forceinline__ int f256_a3(int k0, int e, int p, int h) { return k0 
 // p2 and p3 twiddles of one lane held in registers (RT = true below): p2[e - 1] =
 // T.p2[64 e + lane], p3[p - 1] = T.p3[64 p + lane]; they are the same for every
 // transform a lane runs, so a kernel with registers to spare loads them once.
-struct Tw256Regs {
-    cplx p2[3], p3[3];
+template <class C>
+struct Tw256RegsT {
+    C p2[3], p3[3];
 };
+using Tw256Regs = Tw256RegsT<cplx>;
 
-__device__ __forceinline__ Tw256Regs load_tw256(const FFT1024Tables& T, int lane) {
-    Tw256Regs r;
+template <class C>
+__device__ __forceinline__ Tw256RegsT<C> load_tw256(const FFT1024TablesT<C>& T, int lane) {
+    Tw256RegsT<C> r;
 #pragma unroll
     for (int e = 1; e < 4; ++e) {
         r.p2[e - 1] = T.p2[e * 64 + lane];
@@ -68,11 +78,11 @@ __device__ __forceinline__ Tw256Regs load_tw256(const FFT1024Tables& T, int lane
     return r;
 }
 
-template <bool RT>
-__device__ __forceinline__ void fft256_fwd_t(cplx (&x)[4], cplx* lds, const FFT1024Tables& T, const Tw256Regs& tw,
+template <bool RT, class C>
+__device__ __forceinline__ void fft256_fwd_t(C (&x)[4], C* lds, const FFT1024TablesT<C>& T, const Tw256RegsT<C>& tw,
                                              int lane) {
 #pragma unroll
-    for (int a = 1; a < 4; ++a) x[a] = cmul(x[a], twist4_const(a));
+    for (int a = 1; a < 4; ++a) x[a] = cmul(x[a], twist4_const<C>(a));
     dft4<false>(x);
 #pragma unroll
     for (int k = 0; k < 4; ++k) x[k] = cmul(x[k], T.p1t[k * 64 + lane]);
@@ -111,8 +121,8 @@ __device__ __forceinline__ void fft256_fwd(cplx (&x)[4], cplx* lds, const FFT102
 }
 
 // inverse 256-point: frequency layout in, 256 x folded points out
-template <bool RT>
-__device__ __forceinline__ void fft256_inv_t(cplx (&x)[4], cplx* lds, const FFT1024Tables& T, const Tw256Regs& tw,
+template <bool RT, class C>
+__device__ __forceinline__ void fft256_inv_t(C (&x)[4], C* lds, const FFT1024TablesT<C>& T, const Tw256RegsT<C>& tw,
                                              int lane) {
     dft4<true>(x);
     const int k0 = lane >> 4, e = (lane >> 2) & 3, low = lane & 3;
@@ -145,7 +155,7 @@ __device__ __forceinline__ void fft256_inv_t(cplx (&x)[4], cplx* lds, const FFT1
     for (int k = 0; k < 4; ++k) x[k] = cmulc(x[k], T.p1t[k * 64 + lane]);
     dft4<true>(x);
 #pragma unroll
-    for (int a = 1; a < 4; ++a) x[a] = cmulc(x[a], twist4_const(a));
+    for (int a = 1; a < 4; ++a) x[a] = cmulc(x[a], twist4_const<C>(a));
 }
 
 __device__ __forceinline__ void fft256_inv(cplx (&x)[4], cplx* lds, const FFT1024Tables& T, int lane) {
@@ -159,16 +169,79 @@ __device__ __forceinline__ void fft256_inv(cplx (&x)[4], cplx* lds, const FFT102
 // basic blocks; the constants e, c, t in {0, +-1} make it straight-line FMAs
 // with the same rounding points as the add/subtract form (products by 0 and +-1
 // are exact).
-template <bool INV>
-__device__ __forceinline__ cplx xwave_dft4(cplx y0, cplx y1, cplx y2, cplx y3, int w) {
-    const double e = (w & 1) ? -1.0 : 1.0;
-    const double c = (w == 0) ? 1.0 : (w == 2) ? -1.0 : 0.0;
-    const double tf = (w == 1) ? -1.0 : (w == 3) ? 1.0 : 0.0;  // sin(-pi w / 2)
-    const double t = INV ? -tf : tf;
-    const cplx A = {fma(e, y2.re, y0.re), fma(e, y2.im, y0.im)};
-    const cplx B = {fma(e, y3.re, y1.re), fma(e, y3.im, y1.im)};
-    return {fma(-t, B.im, fma(c, B.re, A.re)), fma(t, B.re, fma(c, B.im, A.im))};
+template <bool INV, class C>
+__device__ __forceinline__ C xwave_dft4(C y0, C y1, C y2, C y3, int w) {
+    using S = decltype(C{}.re);
+    const S e = (w & 1) ? S(-1) : S(1);
+    const S c = (w == 0) ? S(1) : (w == 2) ? S(-1) : S(0);
+    const S tf = (w == 1) ? S(-1) : (w == 3) ? S(1) : S(0);  // sin(-pi w / 2)
+    const S t = INV ? -tf : tf;
+    const C A = {sfma(e, y2.re, y0.re), sfma(e, y2.im, y0.im)};
+    const C B = {sfma(e, y3.re, y1.re), sfma(e, y3.im, y1.im)};
+    return {sfma(-t, B.im, sfma(c, B.re, A.re)), sfma(t, B.re, sfma(c, B.im, A.im))};
 }
+
+// Cross-wave step of the FFT1024 (wave w holds frequencies 256 w ..): forward and
+// inverse, with the cross twiddles trw taken from registers.  The cross-wave
+// radix-4 output of wave w is either selected on w (BF = false: uniform branches,
+// which also bound the scheduler's reordering and keep the one-limb / two-limb
+// kernels within 256 registers, i.e. two waves per SIMD) or computed branch-free
+// (BF = true: fewer instructions and a longer straight-line schedule, but more
+// registers; faster where there is one wave per SIMD anyway).  Both round at the
+// same points, so the outputs are identical.  xbuf: 1024 C of LDS shared by the
+// four waves.
+template <bool BF, bool RT = false, class C>
+__device__ __forceinline__ void f1024_fwd(C (&x)[4], C* lds, C* xbuf, const FFT1024TablesT<C>& T,
+                                          const C (&trw)[4], int w, int lane,
+                                          const Tw256RegsT<C>& tw = Tw256RegsT<C>{}) {
+    fft256_fwd_t<RT>(x, lds, T, tw, lane);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) xbuf[(w * 4 + s) * 64 + lane] = cmul(x[s], trw[s]);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        C y0 = xbuf[(0 * 4 + s) * 64 + lane], y1 = xbuf[(1 * 4 + s) * 64 + lane];
+        C y2 = xbuf[(2 * 4 + s) * 64 + lane], y3 = xbuf[(3 * 4 + s) * 64 + lane];
+        if constexpr (BF) {
+            x[s] = xwave_dft4<false>(y0, y1, y2, y3, w);
+        } else {
+            C s02 = cadd(y0, y2), d02 = csub(y0, y2), s13 = cadd(y1, y3), d13 = mul_mi<false>(csub(y1, y3));
+            x[s] = (w == 0) ? cadd(s02, s13) : (w == 1) ? cadd(d02, d13) : (w == 2) ? csub(s02, s13) : csub(d02, d13);
+        }
+    }
+}
+
+template <bool BF, bool RT = false, class C>
+__device__ __forceinline__ void f1024_inv(C (&x)[4], C* lds, C* xbuf, const FFT1024TablesT<C>& T,
+                                          const C (&trw)[4], int w, int lane,
+                                          const Tw256RegsT<C>& tw = Tw256RegsT<C>{}) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) xbuf[(w * 4 + s) * 64 + lane] = x[s];
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        C z0 = xbuf[(0 * 4 + s) * 64 + lane], z1 = xbuf[(1 * 4 + s) * 64 + lane];
+        C z2 = xbuf[(2 * 4 + s) * 64 + lane], z3 = xbuf[(3 * 4 + s) * 64 + lane];
+        C y;
+        if constexpr (BF) {
+            y = xwave_dft4<true>(z0, z1, z2, z3, w);
+        } else {
+            C s02 = cadd(z0, z2), d02 = csub(z0, z2), s13 = cadd(z1, z3), d13 = mul_mi<true>(csub(z1, z3));
+            y = (w == 0) ? cadd(s02, s13) : (w == 1) ? cadd(d02, d13) : (w == 2) ? csub(s02, s13) : csub(d02, d13);
+        }
+        x[s] = cmulc(y, trw[s]);
+    }
+    fft256_inv_t<RT>(x, lds, T, tw, lane);
+}
+
+template <class C>
+__device__ __forceinline__ void load_trw(const FFT1024TablesT<C>& Tg, int w, int lane, C (&trw)[4]) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) trw[s] = Tg.tr[(w * 4 + s) * 64 + lane];
+}
+
+// coefficient owned by (wave w, lane, slot a): 4 (64 a + lane) + w; slot a + 4 is +1024
+__device__ __forceinline__ int own_coeff(int w, int lane, int a) { return 4 * (64 * a + lane) + w; }
 
 // round(x * scale) mod 2^64 through two FP64 limbs is done by the callers;
 // exact int64 of a rounded double with |x| < 2^52.

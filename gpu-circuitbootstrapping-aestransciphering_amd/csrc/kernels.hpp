@@ -35,8 +35,16 @@ int ks_model_slices(uint32_t batch, int N, int n);
 int launch_keyswitch32(const uint32_t* lweN, uint32_t batch, int N, int n, int logB, int l, const uint32_t* ksk,
                        uint32_t* at, uint32_t* out, hipStream_t st, int slices = 0);
 // circuit-bootstrapping level (N = 2048; PBS torus-64, PFKS / RGSW / LUT torus-32)
-struct FFT1024Tables;
+struct cplxf;
+template <class C>
+struct FFT1024TablesT;
+using FFT1024Tables = FFT1024TablesT<cplx>;
+using FFT1024TablesF = FFT1024TablesT<cplxf>;
 void launch_to_fourier2048(const uint64_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st);
+// torus-32 product x * y mod (X^2048 + 1, 2^32): form 0 = FP32, limbs of 11/11/10 bits;
+// 1 = FP32, limbs of 11/6/5/5/5 bits; 2 = one FP64 product.  Returns -1 for another form.
+int launch_polymul2048_split32(const int32_t* x, const uint32_t* y, uint32_t batch, const FFT1024TablesF& TF,
+                               const FFT1024Tables& T, uint32_t* out, hipStream_t st, int form);
 // limbs = 2: exact product; 1: the one-product form of the PBS (rounded once)
 void launch_polymul2048(const int32_t* x, const uint64_t* y, uint32_t batch, const FFT1024Tables& T, uint64_t* out,
                         hipStream_t st, int limbs = 2);
@@ -52,8 +60,19 @@ void launch_to_fourier2048_t64(const uint64_t* coef, size_t npoly, const FFT1024
 // form: 1 = branch-free cross-wave FFT step, 0 = branchy (two key bits per step: 1 = wide key
 // batch at one wave per SIMD, 0 = two waves per SIMD), < 0 = pbs_model_form's choice
 int pbs_model_form(uint32_t batch, int limbs, int mb);
+// Test polynomial and output of the multi-value PBS.  lut == NULL (default): the circuit
+// bootstrap's two-level test vector, b + 2^30 switched with bw = 1, LWE outputs of
+// coefficients 0 and 1 into out.  lut != NULL (the reference's gpu_BootstrapManyLUT):
+// modulus switch with bw, acc starts as (0, lut X^-b~) (lut [N] torus-64), and the whole
+// RLWE [batch][2][N] goes to rlwe (Gpow_j / 2 added to body coefficient j < 2); out is not
+// written.
+struct PbsLut {
+    const uint64_t* lut = nullptr;
+    uint64_t* rlwe = nullptr;
+    uint32_t bw = 1;
+};
 void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, int limbs, int mb,
-                 const FFT1024Tables& T, uint64_t* out, hipStream_t st, int form = -1);
+                 const FFT1024Tables& T, uint64_t* out, hipStream_t st, int form = -1, const PbsLut& io = PbsLut{});
 // PFKS: inputs rounded and transposed to at [N+1][pfks_bpad(B)] u32, then the key switch for
 // functions u0 .. u0+nu-1 added into zero-filled outputs (out + (u-u0)*out_u_offset, rows out_stride apart)
 uint32_t pfks_bpad(uint32_t B);

@@ -109,13 +109,25 @@ SIGNATURES = {
     "tfhe_cbs_pfks_batch": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_uint32, ctypes.c_size_t, _vp, _vp]),
     "tfhe_cbs_batch_from_device_lwe": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32,
                                                       ctypes.POINTER(RgswBatchView), _vp]),
+    "tfhe_cbs_write_keys": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(CbsParams), ctypes.c_uint64, _i32p, _vp,
+                                           _vp, _vp]),
+    "tfhe_cbs_save_keys": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "tfhe_cbs_load_keys": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "tfhe_cbs_key_file_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(CbsParams),
+                                              ctypes.POINTER(ctypes.c_uint64)]),
+    "tfhe_cbs_batch_to_device": (ctypes.c_int, [_vp, _u32p, ctypes.c_uint32, ctypes.POINTER(RgswBatchView), _vp]),
+    "tfhe_cbs_bootstrap_many_lut": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_rgsw_to_fourier": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_lut8_eval_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_size_t, ctypes.c_int, _vp, _vp]),
     "tfhe_lut8x32_eval_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp, _vp]),
     "tfhe_cbs_keyswitch_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_cbs_autotune": (ctypes.c_int, [_vp, ctypes.c_uint32, _i32p, _vp]),
+    "tfhe_cbs_timing_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "tfhe_cbs_timing_read": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint32),
+                                            ctypes.POINTER(ctypes.c_uint64)]),
     "tfhe_polymul2048_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_polymul2048_approx_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
+    "tfhe_polymul2048_split32_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_int, _vp, _vp]),
     "tfhe_aes_encrypt_key_bits": (None, [ctypes.c_int, _i32p, ctypes.c_uint64,
                                          np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
                                          ctypes.c_int, _u32p]),
@@ -281,6 +293,37 @@ def polymul2048_batch(d_x, d_y, d_out, stream=None, approx=False):
     check(fn(_ptr(d_x), _ptr(d_y), b, _ptr(d_out), _stream(stream)))
 
 
+def write_keys(path, params, seed, rlwe_sk, bsk, pfks, ksk2):
+    """A key file from keys made elsewhere (tfhe_cbs_write_keys); bsk / pfks / ksk2 are
+    numpy arrays (host) or device tensors."""
+    def p_(a):
+        return ctypes.c_void_p(a.ctypes.data) if isinstance(a, np.ndarray) else _ptr(a)
+    check(lib().tfhe_cbs_write_keys(os.fsencode(path), ctypes.byref(params), seed,
+                                    np.ascontiguousarray(rlwe_sk, np.int32), p_(bsk), p_(pfks), p_(ksk2)))
+
+
+def key_file_info(path):
+    """(CbsParams, seed) of a key file (tfhe_cbs_key_file_info)."""
+    prm, seed = CbsParams(), ctypes.c_uint64()
+    check(lib().tfhe_cbs_key_file_info(os.fsencode(path), ctypes.byref(prm), ctypes.byref(seed)))
+    return prm, seed.value
+
+
+SPLIT32_FORMS = {"fp32_3x11": 0, "fp32_exact_top": 1, "fp64": 2}
+
+
+def polymul2048_split32_batch(d_x, d_y, d_out, form, stream=None):
+    """Negacyclic products x * y mod 2^32 (N = 2048, torus-32 y) in the split-precision
+    FP32 forms ("fp32_3x11": limbs 11/11/10, "fp32_exact_top": limbs 11/6/5/5/5) or one
+    FP64 product ("fp64"); tfhe_polymul2048_split32_batch."""
+    b = d_x.shape[0]
+    _need(d_x, b * 2048 * 4, "split32 x")
+    _need(d_y, b * 2048 * 4, "split32 y")
+    _need(d_out, b * 2048 * 4, "split32 out")
+    check(lib().tfhe_polymul2048_split32_batch(_ptr(d_x), _ptr(d_y), b, SPLIT32_FORMS[form], _ptr(d_out),
+                                               _stream(stream)))
+
+
 def aes_encrypt_key_bits(sk, seed, key, nshift=15):
     n = len(sk)
     out = np.zeros(11 * 128 * (n + 1), np.uint32)
@@ -332,6 +375,14 @@ class CbsContext:
             return a.ctypes.data if isinstance(a, np.ndarray) else _ptr(a)
         check(lib().tfhe_cbs_set_keys(self.h, np.ascontiguousarray(rlwe_sk, np.int32), p_(bsk), p_(pfks), p_(ksk2)))
 
+    def save_keys(self, path):
+        """Write the context's keys (made by keygen) to a key file (tfhe_cbs_save_keys)."""
+        check(lib().tfhe_cbs_save_keys(self.h, os.fsencode(path)))
+
+    def load_keys(self, path):
+        """Check a key file against the context's parameter set and upload it (tfhe_cbs_load_keys)."""
+        check(lib().tfhe_cbs_load_keys(self.h, os.fsencode(path)))
+
     def pfkl_entry(self, u, i, t):
         """Linear-key entry (u, i, t): 2N words mod 2^40."""
         out = np.zeros(2 * self.p.N, np.uint64)
@@ -370,6 +421,24 @@ class CbsContext:
                                                    ctypes.byref(v), _stream(stream)))
         return v
 
+    def cbs_batch_to_device(self, h_lwe, stream=None):
+        """Circuit bootstraps of host LWE ciphertexts [batch][n+1] (uint32); returns the
+        RgswBatchView of the outputs held in the context (tfhe_cbs_batch_to_device)."""
+        h = np.ascontiguousarray(h_lwe, np.uint32).reshape(-1, self.p.n + 1)
+        v = RgswBatchView()
+        check(lib().tfhe_cbs_batch_to_device(self.h, h.reshape(-1), h.shape[0], ctypes.byref(v), _stream(stream)))
+        return v
+
+    def bootstrap_many_lut(self, d_lwe, d_lut, bitwidth, d_out, batch=None, stream=None):
+        """gpu_BootstrapManyLUT over a batch: d_lwe [batch][n+1] uint32, d_lut [N] (torus-64),
+        d_out [batch][2][N] (torus-64 RLWE, mask then body)."""
+        b = d_lwe.shape[0] if batch is None else batch
+        _need(d_lwe, b * 4 * (self.p.n + 1), "many_lut in")
+        _need(d_lut, 8 * self.p.N, "many_lut lut")
+        _need(d_out, b * 2 * 8 * self.p.N, "many_lut out")
+        check(lib().tfhe_cbs_bootstrap_many_lut(self.h, _ptr(d_lwe), b, _ptr(d_lut), bitwidth, _ptr(d_out),
+                                                _stream(stream)))
+
     def rgsw_to_fourier(self, d_rgsw, count, d_rgsw_f, stream=None):
         _need(d_rgsw, count * 8 * self.p.N * 4, "rgsw_to_fourier in")
         _need(d_rgsw_f, count * 8 * 1024 * 16, "rgsw_to_fourier out")
@@ -404,6 +473,16 @@ class CbsContext:
         check(lib().tfhe_cbs_autotune(self.h, blocks, ch, _stream(stream)))
         return {"pbs_form": [int(ch[0]), int(ch[1])], "pfks_slices": [int(ch[2]), int(ch[3])],
                 "ks_slices": int(ch[4])}
+
+    def timing_enable(self, on=True):
+        """Bracket every PBS launch with HIP events from now on (tfhe_cbs_timing_enable)."""
+        check(lib().tfhe_cbs_timing_enable(self.h, int(bool(on))))
+
+    def timing_read(self):
+        """(summed PBS launch ms, launches, PBS inputs) since enabling or the last read."""
+        ms, n, inp = ctypes.c_double(), ctypes.c_uint32(), ctypes.c_uint64()
+        check(lib().tfhe_cbs_timing_read(self.h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(inp)))
+        return ms.value, n.value, inp.value
 
     def set_round_keys(self, rk_bits):
         check(lib().tfhe_aes_set_round_keys(self.h, np.ascontiguousarray(rk_bits, np.uint32).reshape(-1)))

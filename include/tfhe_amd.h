@@ -208,6 +208,27 @@ int tfhe_gen_pfks32(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_rl
 int tfhe_cbs_set_keys(tfhe_cbs_ctx* ctx, const int32_t* rlwe_sk, const uint64_t* bsk, const uint32_t* pfks,
                       const uint32_t* ksk2);
 
+/* Key files.  The reference has no file format: its keys come from the OpenFHE
+ * fork's CPU context and gpu_init_cirbsk uploads them (cirbts-init.cu:200-865).
+ * A key file holds what tfhe_cbs_set_keys takes, behind a versioned 256-byte header
+ * (magic "TFHECBSK", version 1, the tfhe_cbs_params, the key seed or 0, the four
+ * section sizes and a 64-bit content hash of each, and a hash of the header):
+ * rlwe_sk [N] int32, the BSK2 rows as tfhe_gen_bsk64 writes them, the PFKS key of the
+ * parameter set's form (as tfhe_gen_pfks32 or tfhe_gen_pfkl40 writes it) and ksk2
+ * [N][8][4][n+1] uint32.
+ * tfhe_cbs_write_keys: a file from keys made elsewhere (host or device buffers).
+ * tfhe_cbs_save_keys: a context's keys, if tfhe_cbs_keygen made them (the BSK2
+ * rows are generated again from the seed; TFHE_E_STATE otherwise).
+ * tfhe_cbs_load_keys: checks magic, version, header hash, that the parameter set
+ * equals the context's (TFHE_E_PARAM otherwise), section sizes and content hashes
+ * (TFHE_E_SHAPE), then uploads as tfhe_cbs_set_keys.
+ * tfhe_cbs_key_file_info: the parameter set and seed of a file. */
+int tfhe_cbs_write_keys(const char* path, const tfhe_cbs_params* params, uint64_t seed, const int32_t* rlwe_sk,
+                        const uint64_t* bsk, const void* pfks, const uint32_t* ksk2);
+int tfhe_cbs_save_keys(const tfhe_cbs_ctx* ctx, const char* path);
+int tfhe_cbs_load_keys(tfhe_cbs_ctx* ctx, const char* path);
+int tfhe_cbs_key_file_info(const char* path, tfhe_cbs_params* params, uint64_t* seed);
+
 /* gpu_CircuitBootstrappingBatchToDeviceFromDeviceLWE (cirbts.cuh:466-474,
  * cirbts-batch.cu:1416-1879): d_lwe [batch][n+1] -> d_rgsw [batch][4][2][N] uint32.
  * Pieces: the multi-value PBS (gpu_BootstrapLUT_inplace_batch_device,
@@ -244,6 +265,28 @@ typedef struct tfhe_rgsw_batch_view {
  * capacity TFHE_E_CAPACITY. */
 int tfhe_cbs_batch_from_device_lwe(tfhe_cbs_ctx* ctx, const uint64_t* d_a, const uint64_t* d_b, uint32_t batch,
                                    uint32_t a_stride, tfhe_rgsw_batch_view* view, void* stream);
+
+/* gpu_CircuitBootstrappingBatchToDevice(params, ek, vector<LWECiphertext>)
+ * (cirbts.cuh:464-466): host LWE ciphertexts h_lwe [batch][n+1] uint32 are staged
+ * into the context and bootstrapped; the outputs stay in the context and *view
+ * describes them, with the validity rule of tfhe_cbs_batch_from_device_lwe.  NULL
+ * input or batch 0 gives an empty view; batch above the capacity is TFHE_E_CAPACITY. */
+int tfhe_cbs_batch_to_device(tfhe_cbs_ctx* ctx, const uint32_t* h_lwe, uint32_t batch, tfhe_rgsw_batch_view* view,
+                             void* stream);
+
+/* gpu_BootstrapManyLUT(params, ek, ct, LUT, bitwidth) (cirbts.cuh:478-482,
+ * cirbts-bootstrap.cu:753-839) for a batch sharing one test polynomial: each level-0
+ * LWE (d_lwe [batch][n+1] uint32) is switched to 2N with the bitwidth rule of
+ * gpu_SpecilMS (cirbts-utils.cu:52-61: round(v 2N / (2^32 2^bw)) 2^bw), the
+ * accumulator starts as (0, LUT X^-b~) with d_lut [N] uint64 on the 64-bit torus,
+ * is blind-rotated by the context's PBS form to LUT X^-phase~ (phase~ = b~ - <a~, s>
+ * mod 2N; extracting coefficient 0 gives LUT[phase~], negated for phase~ >= N),
+ * and Gpow_j / 2 (2^47, 2^55) is added to
+ * body coefficient j < 2 (kernel_ModAddGpowScaled, kernel.cu:7459-7473).  d_out
+ * [batch][2][N] uint64: the RLWE_K2 (mask, then body) in the coefficient domain
+ * (the reference returns it in NTT evaluation form).  bitwidth <= 11. */
+int tfhe_cbs_bootstrap_many_lut(tfhe_cbs_ctx* ctx, const uint32_t* d_lwe, uint32_t batch, const uint64_t* d_lut,
+                                uint32_t bitwidth, uint64_t* d_out, void* stream);
 
 /* RGSW (torus-32 coefficients, [count][4][2][N] uint32) -> Fourier, one
  * exact FP64 transform per polynomial: d_rgsw_f [count][8][1024] complex doubles. */
@@ -287,6 +330,18 @@ int tfhe_cbs_keyswitch_batch(tfhe_cbs_ctx* ctx, const uint32_t* d_lwe32, uint32_
  * during stream capture. */
 int tfhe_cbs_autotune(tfhe_cbs_ctx* ctx, uint32_t blocks, int* choices, void* stream);
 
+/* Kernel timing of the multi-value PBS (the reference prints per-block
+ * compute_only_ms from events around its launches, main.cu:1694-1758): while
+ * enabled, every PBS launch made outside stream capture (tfhe_cbs_pbs_batch and
+ * everything built on it: circuit bootstraps, AES rounds, keystreams) is
+ * bracketed by a pair of HIP events on the stream it is launched on.
+ * tfhe_cbs_timing_read waits for the recorded events and returns the summed
+ * launch durations (ms), the number of launches and their PBS inputs since
+ * enabling or the last read, then starts a new count.  Enabling resets the
+ * count. */
+int tfhe_cbs_timing_enable(tfhe_cbs_ctx* ctx, int on);
+int tfhe_cbs_timing_read(tfhe_cbs_ctx* ctx, double* pbs_ms, uint32_t* launches, uint64_t* inputs);
+
 /* Exact negacyclic product through the 4-wave FP64 FFT, N = 2048:
  * d_out = d_x * d_y mod (X^N + 1, 2^64), small-integer d_x, torus-64 d_y. */
 int tfhe_polymul2048_batch(const int32_t* d_x, const uint64_t* d_y, uint32_t batch, uint64_t* d_out, void* stream);
@@ -297,6 +352,18 @@ int tfhe_polymul2048_batch(const int32_t* d_x, const uint64_t* d_y, uint32_t bat
  * split-precision FFT, cirbts-fft.cu:159-226). */
 int tfhe_polymul2048_approx_batch(const int32_t* d_x, const uint64_t* d_y, uint32_t batch, uint64_t* d_out,
                                   void* stream);
+/* Split-precision FP32 external-product arithmetic (the reference's split-FFT
+ * backend: limb rule cirbts-init.cu:1096-1120, limb extraction cirbts-fft.cu:345-355,
+ * exact integer recombination cirbts-fft.cu:159-226, carried to FP32):
+ * d_out = d_x * d_y mod (X^2048 + 1, 2^32) for digit polynomials d_x (|x| <= 256)
+ * and torus-32 d_y, all [batch][2048].  d_y is cut into balanced signed limbs, each
+ * limb's product taken through the FP32 FFT1024 and rounded to an integer, and the
+ * limbs recombined mod 2^32.  form 0: limbs of 11, 11, 10 bits (three limbs of at
+ * most 11 bits); form 1: 11, 6, 5, 5, 5 bits, so that only the lowest limb's product
+ * can round wrongly (the exact-top-limb form); form 2: one FP64 product (exact), the
+ * baseline.  The difference from the exact product is the form's coefficient error. */
+int tfhe_polymul2048_split32_batch(const int32_t* d_x, const uint32_t* d_y, uint32_t batch, int form,
+                                   uint32_t* d_out, void* stream);
 
 /* ------------------------------------------------------------------------- */
 /* AES-128 transciphering (reference bitwise/examples/aes_transcipher/main.cu */

@@ -253,6 +253,36 @@ public:
                                    static_cast<hipStream_t>(v.stream)};
     }
 
+    /* gpu_CircuitBootstrappingBatchToDevice(params, ek, cts) (cirbts.cuh:464-466): host
+     * ciphertexts in, a borrowed view of the outputs held in the context (valid until the
+     * next call that reuses the context's RGSW buffer).  An empty vector gives an empty view. */
+    DeviceRGSWBatchView gpu_CircuitBootstrappingBatchToDevice(const std::vector<LWECiphertext>& cts,
+                                                              hipStream_t s = hipStreamPerThread) const {
+        if (cts.size() > max_batch_)
+            throw Error(TFHE_E_CAPACITY, "gpu_CircuitBootstrappingBatchToDevice: batch size exceeds max_batch_size_");
+        std::vector<uint32_t> h = detail::pack(cts, p_.n);
+        tfhe_rgsw_batch_view v{};
+        check(tfhe_cbs_batch_to_device(h_, cts.empty() ? nullptr : h.data(), (uint32_t)cts.size(), &v, s));
+        return DeviceRGSWBatchView{v.d_rgsw, v.batch, v.digits, v.N, v.per_ciphertext_elems,
+                                   static_cast<hipStream_t>(v.stream)};
+    }
+
+    /* gpu_BootstrapManyLUT(params, ek, ct, LUT, bitwidth) (cirbts.cuh:478-482,
+     * cirbts-bootstrap.cu:753-839): an owned device buffer of 2N words, the RLWE_K2
+     * (mask, then body) on the 64-bit torus, coefficient domain; LUT: N torus-64 words. */
+    DeviceBuffer<uint64_t> gpu_BootstrapManyLUT(const LWECiphertext& ct, const std::vector<uint64_t>& LUT,
+                                               uint64_t bitwidth, hipStream_t s = hipStreamPerThread) const {
+        if (LUT.size() != (size_t)p_.N) throw Error(TFHE_E_SHAPE, "gpu_BootstrapManyLUT: LUT of the wrong size");
+        std::vector<uint32_t> h = detail::pack({ct}, p_.n);
+        DeviceBuffer<uint32_t> d(h.size());
+        DeviceBuffer<uint64_t> dl(LUT.size()), out((size_t)2 * p_.N);
+        d.upload(h.data(), h.size(), s);
+        dl.upload(LUT.data(), LUT.size(), s);
+        check(tfhe_cbs_bootstrap_many_lut(h_, d.get(), 1, dl.get(), (uint32_t)bitwidth, out.get(), s));
+        hip_check(hipStreamSynchronize(s), "gpu_BootstrapManyLUT");  // d and dl are freed on return
+        return out;
+    }
+
     /* gpu_CircuitBootstrappingBatch(params, ek, cts) (cirbts.cuh:460-462): host in, host out. */
     std::vector<RGSWCiphertext> gpu_CircuitBootstrappingBatch(const std::vector<LWECiphertext>& cts,
                                                               hipStream_t s = hipStreamPerThread) const {

@@ -368,6 +368,37 @@ double orc_pbs_manylut_mb(const orc_cbs_params* p, const double* bsk_f, int mb, 
     return err;
 }
 
+/* The reference's gpu_BootstrapManyLUT (cirbts-bootstrap.cu:753-839) with an
+ * arbitrary test polynomial: a~_i, b~ = gpu_SpecilMS with bitwidth bw
+ * (cirbts-utils.cu:52-61, no offset on b), acc = (0, LUT X^{-b~}) (the memcpy of LUT
+ * and kernel_ModMulScalar by the monomial X^{-b_MS}), the blind rotation (mb key bits
+ * per step) by X^{+a~_i s_i}, so that acc ends as LUT X^{-phase~} with
+ * phase~ = b~ - <a~, s> mod 2N (orc_blind_rotate64 multiplies by X^{-a_tilde_i}, so
+ * it is given -a~_i, the negation the reference applies in its LMKCDEY branch),
+ * then Gpow_j / 2 added to body coefficient j < cb_l (kernel_ModAddGpowScaled,
+ * kernel.cu:7459-7473; Gpow_j = 2^(64 - cb_l cb_logB + j cb_logB) here).
+ * out: RLWE [2][N] (mask, body), 64-bit torus, coefficient domain; coefficient c of
+ * its phase is LUT[c + phase~] (negated past N) plus the offsets and the noise. */
+double orc_bootstrap_many_lut(const orc_cbs_params* p, const double* bsk_f, int mb, const uint32_t* ct32,
+                              const uint64_t* lut, int bw, uint64_t* out) {
+    int n = p->n, N = p->N;
+    uint32_t Q2 = 2u * (uint32_t)N;
+    uint32_t* at = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    for (int i = 0; i < n; ++i) at[i] = (Q2 - orc_special_ms(ct32[i], N, bw)) & (Q2 - 1);
+    uint32_t bt = orc_special_ms(ct32[n], N, bw);
+    for (int c = 0; c < N; ++c) {
+        /* coefficient c of LUT X^{-bt}: LUT[m], m = c + bt mod 2N, negated when m >= N */
+        uint32_t m = ((uint32_t)c + bt) & (Q2 - 1);
+        out[c] = 0;
+        out[N + c] = m < (uint32_t)N ? lut[m] : 0 - lut[m - (uint32_t)N];
+    }
+    double err = mb == 2 ? orc_blind_rotate64_mb2(p, bsk_f, at, out) : orc_blind_rotate64(p, bsk_f, at, out);
+    for (int j = 0; j < p->cb_l && j < N; ++j)
+        out[N + j] += (uint64_t)1 << (64 - p->cb_l * p->cb_logB + j * p->cb_logB - 1);
+    free(at);
+    return err;
+}
+
 /* ------------------------------------------------------------------------- */
 /* Private functional key switching LWE_K2 (dim N, torus-64) -> RLWE_K2        */
 /* (torus-32) (CGGI 2020, PrivKS):                                             */

@@ -263,6 +263,8 @@ def _cbs_lib():
             "orc_special_ms": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_int, ctypes.c_int]),
             "orc_pbs_manylut": (ctypes.c_double, [P, _f64p, _u32p, _u64p]),
             "orc_pbs_manylut_mb": (ctypes.c_double, [P, _f64p, ctypes.c_int, _u32p, _u64p]),
+            "orc_bootstrap_many_lut": (ctypes.c_double, [P, _f64p, ctypes.c_int, _u32p, _u64p, ctypes.c_int,
+                                                         _u64p]),
             "orc_mb2_indicator": (ctypes.c_uint32, [_i32p, ctypes.c_int, ctypes.c_int]),
             "orc_gen_bsk64_mb2_row": (None, [ctypes.c_uint64, P, _i32p, _i32p, ctypes.c_uint32, ctypes.c_int,
                                              _u64p]),
@@ -306,6 +308,11 @@ def rlwe_phase32(K, ct):
     N = len(K)
     ct = np.asarray(ct, np.uint32)
     return (ct[N:] - nc_mul_exact32(K, ct[:N])).astype(np.uint32)
+
+
+def special_ms(v, N, bw):
+    """round(v 2N / (2^32 2^bw)) 2^bw mod 2N (orc_special_ms)."""
+    return int(_cbs_lib().orc_special_ms(int(v) & 0xFFFFFFFF, N, bw))
 
 
 def rlwe_phase64(K, ct):
@@ -382,6 +389,16 @@ class CbsKeys:
         err = _cbs_lib().orc_pbs_manylut_mb(ctypes.byref(p), self.bsk_mb2_fourier(), 2,
                                             np.ascontiguousarray(ct32, np.uint32), out)
         return out.reshape(2, p.N + 1), err
+
+    def bootstrap_many_lut(self, ct32, lut, bw, mb=1):
+        """The reference's gpu_BootstrapManyLUT: RLWE [2][N] (mask, body) of LUT X^-phase~
+        plus Gpow_j / 2 on body coefficient j < cb_l (mb = 2: the pattern key)."""
+        p = self.p
+        out = np.zeros(2 * p.N, np.uint64)
+        bskf = self.bsk_mb2_fourier() if mb == 2 else self.bsk_f
+        err = _cbs_lib().orc_bootstrap_many_lut(ctypes.byref(p), bskf, mb, np.ascontiguousarray(ct32, np.uint32),
+                                                np.ascontiguousarray(lut, np.uint64), int(bw), out)
+        return out.reshape(2, p.N), err
 
     def blind_rotate_mb2(self, a_tilde, acc):
         """Exact two-bit blind rotation of acc [2N] u64 (in place) by the rotations a_tilde [n]."""

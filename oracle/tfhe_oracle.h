@@ -131,6 +131,9 @@ void orc_rgsw64_to_fourier(const orc_cbs_params* p, long nrgsw, const uint64_t* 
 double orc_blind_rotate64_mb2(const orc_cbs_params* p, const double* bsk_f, const uint32_t* a_tilde, uint64_t* acc);
 double orc_pbs_manylut_mb(const orc_cbs_params* p, const double* bsk_f, int mb, const uint32_t* ct32,
                           uint64_t* out);
+/* gpu_BootstrapManyLUT with test polynomial lut [N] and bitwidth bw: out RLWE [2][N] */
+double orc_bootstrap_many_lut(const orc_cbs_params* p, const double* bsk_f, int mb, const uint32_t* ct32,
+                              const uint64_t* lut, int bw, uint64_t* out);
 uint32_t orc_pfks_id(const orc_cbs_params* p, int u, int i, int t, int v);
 void orc_gen_pfks_entry(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, int u, int i, int t, int v,
                         uint32_t* ct);

@@ -129,6 +129,46 @@ static int circuit_bootstrap_and_aes(const std::vector<int32_t>& sk) {
     if (expect_throw([&] { (void)cbs.gpu_CircuitBootstrappingBatch(many); }, TFHE_E_CAPACITY, "257 > 256")) return 1;
     std::printf("circuit bootstrap: FromDeviceLWE view == host batch == single, errors ok\n");
 
+    // gpu_CircuitBootstrappingBatchToDevice: host ciphertexts in, a view of the context's rows
+    const auto hview = cbs.gpu_CircuitBootstrappingBatchToDevice(cts, hipStreamPerThread);
+    EXPECT(hview.batch == batch && hview.per_ciphertext_elems == view.per_ciphertext_elems, "ToDevice view fields");
+    hip_check(hipMemcpyAsync(rows.data(), hview.d_rgsw, rows.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             hview.stream),
+              "view download");
+    hip_check(hipStreamSynchronize(hview.stream), "view sync");
+    for (uint32_t k = 0; k < batch; ++k)
+        EXPECT(std::memcmp(host[k].data(), rows.data() + k * hview.per_ciphertext_elems,
+                           hview.per_ciphertext_elems * sizeof(uint32_t)) == 0,
+               "ToDevice rows differ from gpu_CircuitBootstrappingBatch, ciphertext %u", k);
+    EXPECT(cbs.gpu_CircuitBootstrappingBatchToDevice({}).batch == 0, "empty ToDevice view");
+
+    // gpu_BootstrapManyLUT on a staircase polynomial LUT[c] = c 2^50: coefficient 0 of the
+    // output decrypts to LUT[phase~] (negated past N) + Gpow_0 / 2 = 2^47
+    {
+        std::vector<uint64_t> lut(N);
+        for (int c = 0; c < N; ++c) lut[c] = (uint64_t)c << 50;
+        const uint32_t bw = 1;
+        const LWECiphertext ct = LWEEncrypt(sk, 21, 5, 3, 8);
+        const DeviceBuffer<uint64_t> d_rl = cbs.gpu_BootstrapManyLUT(ct, lut, bw);
+        std::vector<uint64_t> rl((size_t)2 * N);
+        d_rl.download(rl.data(), rl.size(), hipStreamPerThread);
+        std::vector<int32_t> K2(N);
+        check(tfhe_cbs_get_rlwe_sk(cbs.handle(), K2.data()));
+        uint64_t ph = rl[N];  // b[0] - (a K2)[0], (a K2)[0] = a_0 K2_0 - sum_{i>0} a_i K2_{N-i}
+        ph -= rl[0] * (uint64_t)(int64_t)K2[0];
+        for (int i = 1; i < N; ++i) ph += rl[i] * (uint64_t)(int64_t)K2[N - i];
+        auto ms = [&](uint32_t v) { return (uint32_t)((((uint64_t)v + (1ull << (19 + bw))) >> (20 + bw)) << bw) & 4095u; };
+        uint32_t phi = ms(ct.b);
+        for (int i = 0; i < n; ++i) phi = (phi - ms(ct.a[i]) * (uint32_t)sk[i]) & 4095u;
+        const uint64_t want = (phi < (uint32_t)N ? lut[phi] : 0 - lut[phi - N]) + (1ull << 47);
+        const int64_t err = (int64_t)(ph - want);
+        EXPECT(err < (1ll << 40) && err > -(1ll << 40), "gpu_BootstrapManyLUT phase off by %lld", (long long)err);
+        std::vector<uint64_t> short_lut(N - 1);
+        if (expect_throw([&] { (void)cbs.gpu_BootstrapManyLUT(ct, short_lut, bw); }, TFHE_E_SHAPE, "LUT size"))
+            return 1;
+    }
+    std::printf("gpu_CircuitBootstrappingBatchToDevice == host batch; gpu_BootstrapManyLUT decrypts to LUT[phase]\n");
+
     // AES-128 CTR keystream of the FIPS-197 C.1 block
     std::array<uint8_t, 16> key{}, pt{};
     for (int i = 0; i < 16; ++i) key[i] = (uint8_t)i, pt[i] = (uint8_t)(0x11 * i);

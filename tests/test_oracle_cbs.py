@@ -384,3 +384,65 @@ def test_pbs_manylut_mb2_levels(keys, bit):
     for j in range(2):
         e = centered64(keys.lwe64_phase(out[j]) - bit * 2 ** (48 + 8 * j))
         assert abs(e) < 2 ** 40
+
+
+def _many_lut_message(lut, ph, N):
+    """LUT X^-ph (negacyclic) plus Gpow_j / 2 = 2^47, 2^55 on coefficients 0 and 1."""
+    m = (np.arange(N) + ph) % (2 * N)
+    msg = np.where(m < N, lut[m % N], (0 - lut[m % N].astype(np.uint64))).astype(np.uint64)
+    msg[0] += np.uint64(1 << 47)
+    msg[1] += np.uint64(1 << 55)
+    return msg
+
+
+@pytest.mark.parametrize("bw", [0, 1, 3])
+def test_many_lut_zero_mask_is_the_rotated_polynomial(keys, bw):
+    """gpu_BootstrapManyLUT restated: with a zero mask no rotation step acts, so the
+    output is exactly (0, LUT X^-b~ + offsets) with b~ = round(b 2N / 2^(32+bw)) 2^bw."""
+    N, n = keys.p.N, keys.p.n
+    rng = np.random.default_rng(90 + bw)
+    lut = rng.integers(0, 2 ** 64, N, dtype=np.uint64)
+    for b in [0, 1 << 31, int(rng.integers(0, 2 ** 32)), (1 << 32) - 1]:
+        ct = np.zeros(n + 1, np.uint32)
+        ct[n] = b
+        out, _ = keys.bootstrap_many_lut(ct, lut, bw)
+        bt = (((b + (1 << (19 + bw))) >> (20 + bw)) << bw) % (2 * N)
+        assert not out[0].any()
+        assert np.array_equal(out[1], _many_lut_message(lut, bt, N))
+
+
+@pytest.mark.parametrize("mb", [1, 2])
+def test_many_lut_phase_is_lut_at_the_switched_phase(keys, mb):
+    """Decryption pins the rotation direction and the test-polynomial placement: the
+    phase of every coefficient of the output equals LUT X^-phase~ + offsets, with
+    phase~ = b~ - <a~, s> mod 2N from the plaintext secret, within the PBS noise
+    (asserted 2^40 of 2^64); one and two key bits per step."""
+    N, n = keys.p.N, keys.p.n
+    rng = np.random.default_rng(95 + mb)
+    lut = rng.integers(0, 2 ** 64, N, dtype=np.uint64)
+    s = keys.s.astype(np.int64)
+    for k in range(2):
+        ct = keys.encrypt_bit(k, 700 + k)
+        ct[n] = (int(ct[n]) + int(rng.integers(0, 2 ** 32))) % 2 ** 32  # a random phase
+        out, err = keys.bootstrap_many_lut(ct, lut, 1, mb=mb)
+        assert err < 0.25
+        at = np.array([O.special_ms(int(a), N, 1) for a in ct[:n]], np.int64)
+        ph = (O.special_ms(int(ct[n]), N, 1) - int(np.dot(at, s))) % (2 * N)
+        e = (O.rlwe_phase64(keys.K2, out.reshape(-1)) - _many_lut_message(lut, ph, N)).view(np.int64)
+        assert np.abs(e).max() < 2 ** 40
+
+
+@pytest.mark.parametrize("bit", [0, 1])
+def test_many_lut_with_the_cb_polynomial_gives_the_levels(keys, bit):
+    """With the circuit bootstrap's polynomial (-2^47, -2^55 alternating) and b + 2^30,
+    coefficient j of gpu_BootstrapManyLUT's output decrypts to bit * 2^(48 + 8j), as the
+    multi-value PBS's level j does."""
+    N, n = keys.p.N, keys.p.n
+    lut = np.array([(0 - (1 << (47 + 8 * (k & 1)))) % 2 ** 64 for k in range(N)], np.uint64)
+    ct = keys.encrypt_bit(bit, 800 + bit)
+    ct[n] = (int(ct[n]) + (1 << 30)) % 2 ** 32
+    out, _ = keys.bootstrap_many_lut(ct, lut, 1)
+    ph = O.rlwe_phase64(keys.K2, out.reshape(-1))
+    for j in range(2):
+        e = centered64(int(ph[j]) - bit * (1 << (48 + 8 * j)))
+        assert abs(e) < 2 ** 40
