@@ -80,6 +80,64 @@ void shift_mix(const uint8_t in[16], uint8_t out[16], bool mix) {
 }
 constexpr int kLinSrc = 8;  // max sources per output bit of ShiftRows+MixColumns
 
+// S^-1(y): GF(2^8) inverse of the inverse affine map of y ^ 0x63 (rotations by 1, 3, 6)
+uint8_t inv_sbox(uint8_t y) {
+    const uint8_t v = (uint8_t)(y ^ 0x63);
+    auto rl = [](uint8_t x, int k) { return (uint8_t)((x << k) | (x >> (8 - k))); };
+    const uint8_t x = (uint8_t)(rl(v, 1) ^ rl(v, 3) ^ rl(v, 6));
+    uint8_t r = x ? 1 : 0, b = x;
+    for (int e = 254; x && e; e >>= 1) {
+        if (e & 1) r = gmul(r, b);
+        b = gmul(b, b);
+    }
+    return r;
+}
+// InvShiftRows then (optionally) InvMixColumns on 16 bytes, byte index = row + 4 col
+void inv_shift_mix(const uint8_t in[16], uint8_t out[16], bool mix) {
+    uint8_t t[16];
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) t[r + 4 * ((c + r) % 4)] = in[r + 4 * c];
+    if (!mix) {
+        std::memcpy(out, t, 16);
+        return;
+    }
+    for (int c = 0; c < 4; ++c) {
+        const uint8_t* a = t + 4 * c;
+        out[4 * c + 0] = (uint8_t)(gmul(a[0], 14) ^ gmul(a[1], 11) ^ gmul(a[2], 13) ^ gmul(a[3], 9));
+        out[4 * c + 1] = (uint8_t)(gmul(a[0], 9) ^ gmul(a[1], 14) ^ gmul(a[2], 11) ^ gmul(a[3], 13));
+        out[4 * c + 2] = (uint8_t)(gmul(a[0], 13) ^ gmul(a[1], 9) ^ gmul(a[2], 14) ^ gmul(a[3], 11));
+        out[4 * c + 3] = (uint8_t)(gmul(a[0], 11) ^ gmul(a[1], 13) ^ gmul(a[2], 9) ^ gmul(a[3], 14));
+    }
+}
+constexpr int kLinSrcDec = 20;  // max sources per output bit of InvShiftRows+InvMixColumns (19)
+// T-table word of row r (reference aes128.cuh:227-276): Te_r[x] = rotr(be(2 S, S, S, 3 S), 8 r),
+// Td_r[x] = rotr(be(14 S^-1, 9 S^-1, 13 S^-1, 11 S^-1), 8 r); byte p of the word (row p of the
+// output column) is bits 24 - 8 p .. 31 - 8 p
+uint32_t ttable_word(int dec, int r, uint8_t x) {
+    const uint8_t s = dec ? inv_sbox(x) : sbox(x);
+    const uint32_t w = dec ? ((uint32_t)gmul(s, 14) << 24) | ((uint32_t)gmul(s, 9) << 16) |
+                                 ((uint32_t)gmul(s, 13) << 8) | gmul(s, 11)
+                           : ((uint32_t)gmul(s, 2) << 24) | ((uint32_t)s << 16) | ((uint32_t)s << 8) | gmul(s, 3);
+    return r ? (w >> (8 * r)) | (w << (32 - 8 * r)) : w;
+}
+// GF(2)-linear byte map -> source bit lists [128][nsrc] (-1 padded), bit = 8 byte + j
+template <class F>
+int linear_sources(F map, int nsrc, std::vector<int16_t>& src) {
+    src.assign((size_t)128 * nsrc, -1);
+    std::vector<int> cnt(128, 0);
+    for (int ib = 0; ib < 128; ++ib) {
+        uint8_t in[16] = {0}, o[16];
+        in[ib / 8] = (uint8_t)(1u << (ib % 8));
+        map(in, o);
+        for (int ob = 0; ob < 128; ++ob)
+            if ((o[ob / 8] >> (ob % 8)) & 1) {
+                if (cnt[ob] >= nsrc) return fail(TFHE_E_PARAM, "linear layer: too many sources");
+                src[(size_t)ob * nsrc + cnt[ob]++] = (int16_t)ib;
+            }
+    }
+    return TFHE_OK;
+}
+
 // Per-call scratch of one group of AES blocks: the context's main set, or the small
 // set that the remainder group of an uneven batch uses on the second stream while the
 // full-round group runs on the caller's stream.
@@ -91,15 +149,25 @@ struct CbsScratch {
     uint32_t* rgsw = nullptr;   // [cap][4][2][N]
     cplx* rgsw_f = nullptr;     // [cap][8][1024]
     uint32_t* lut32 = nullptr;  // [cap][N+1]
+    uint32_t* lutw = nullptr;   // T-table rounds: [cap / 8 bytes][32][N+1] word bits, made on first use
     uint32_t* state[2] = {nullptr, nullptr};  // [cap][n+1]
+    size_t cap = 0;
 };
 
+// the T-table rounds' LUT output buffer of a scratch set (allocated on first use)
+int ensure_lutw(CbsScratch& s) {
+    return s.lutw ? TFHE_OK : dmalloc((void**)&s.lutw, s.cap / 8 * 32 * (kN + 1) * sizeof(uint32_t));
+}
+
 // allocates a scratch set for cap circuit bootstraps
-int alloc_scratch(CbsScratch& s, size_t cap, int n, bool linear) {
+// conv: the context's LWE -> RGSW form (0 lookup PFKS, 1 linear PFKS, 2 HomTrace + SchemeSwitch)
+int alloc_scratch(CbsScratch& s, size_t cap, int n, int conv) {
     int rc = TFHE_OK;
+    s.cap = cap;
+    const bool linear = conv == 1;
     rc = rc ? rc : dmalloc((void**)&s.lwe64, cap * 2 * (kN + 1) * sizeof(uint64_t));
     // + 256 columns: a batch split into full rounds and a tail pads each part separately
-    if (!linear)
+    if (conv == 0)
         rc = rc ? rc
                 : dmalloc((void**)&s.at, (size_t)(kN + 1) * (pfks_bpad(2 * (uint32_t)cap) + 256) * sizeof(uint32_t));
     rc = rc ? rc : dmalloc((void**)&s.ks_at, (size_t)kN * ks_bpad((uint32_t)cap) * sizeof(uint32_t));
@@ -114,7 +182,7 @@ int alloc_scratch(CbsScratch& s, size_t cap, int n, bool linear) {
 }
 
 void free_scratch(CbsScratch& s) {
-    void* bufs[] = {s.lwe64, s.at, s.pfkl_D, s.ks_at, s.rgsw, s.rgsw_f, s.lut32, s.state[0], s.state[1]};
+    void* bufs[] = {s.lwe64, s.at, s.pfkl_D, s.ks_at, s.rgsw, s.rgsw_f, s.lut32, s.lutw, s.state[0], s.state[1]};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     s = CbsScratch{};
@@ -132,6 +200,9 @@ struct tfhe_cbs_ctx {
     uint32_t* d_pfks = nullptr;  // [2][N+1][8][15][2N] torus-32 (lookup form)
     int8_t* d_pfkl = nullptr;    // [2][5][2N][pfkl_kp()] signed key bytes (linear form)
     uint32_t* d_ksk2 = nullptr;
+    cplx* d_htkf = nullptr;       // HomTrace key [11][5][2][2 limbs][1024] (lwe_to_rgsw = 1)
+    cplx* d_sskf = nullptr;       // SchemeSwitch key [5][2][2 limbs][1024]
+    uint64_t* d_htss_lut = nullptr;  // [N] test polynomial of the HomTrace form's PBS
     CbsScratch sc;                // per-call scratch for max_batch circuit bootstraps
     // scratch of the remainder group of an uneven AES batch (tail_blocks blocks at most),
     // run on cstream beside the full-round group
@@ -142,9 +213,20 @@ struct tfhe_cbs_ctx {
     // AES state
     uint32_t max_blocks = 0;
     uint32_t* d_sbox_poly = nullptr;            // [N]
+    uint32_t* d_inv_sbox_poly = nullptr;        // [N]
+    uint32_t* d_ttab = nullptr;                 // [2 (Te, Td)][4 rows][4 output rows][N] T-table polynomials
     int16_t* d_lin_mix = nullptr;               // [128][kLinSrc]
     int16_t* d_lin_last = nullptr;
+    int16_t* d_lin_inv_mix = nullptr;           // [128][kLinSrcDec] InvShiftRows + InvMixColumns
+    int16_t* d_lin_inv_last = nullptr;          // [128][kLinSrcDec] InvShiftRows
+    int16_t* d_lin_id = nullptr;                // [128] identity (the round key addition of T-table rounds)
     uint32_t* d_rk = nullptr;                   // [11][128][n+1] encrypted round-key bits
+    uint32_t* d_dk = nullptr;                   // [11][128][n+1] encrypted decryption round-key bits
+    bool has_dec_round_keys = false;
+    // AES round form: 0 = S-box LUT then ShiftRows / MixColumns / AddRoundKey as LWE
+    // additions; 1 = T-table rounds (rounds 1..9: four 8 -> 32 LUTs per column XORed at
+    // dimension N, one key switch per bit, the round key added after it), as the reference
+    int round_form = 0;
     uint8_t* d_ctr = nullptr;                   // [max_blocks][16]
     uint8_t* d_pub = nullptr;                   // [max_blocks][16] public AES-CTR ciphertext bytes
     bool has_round_keys = false;
@@ -175,8 +257,10 @@ struct tfhe_cbs_ctx {
 
 extern "C" int tfhe_cbs_ctx_destroy(tfhe_cbs_ctx* c) {
     if (!c) return TFHE_OK;
-    void* bufs[] = {c->d_K2, c->d_bsk2f, c->d_pfks, c->d_ksk2, c->d_sbox_poly, c->d_lin_mix, c->d_lin_last, c->d_rk,
-                    c->d_ctr, c->d_pub, c->d_pfkl};
+    void* bufs[] = {c->d_K2,          c->d_bsk2f,       c->d_pfks,          c->d_ksk2,           c->d_sbox_poly,
+                    c->d_inv_sbox_poly, c->d_ttab,     c->d_lin_mix,        c->d_lin_last,       c->d_lin_inv_mix,
+                    c->d_lin_inv_last,  c->d_lin_id,   c->d_rk,             c->d_dk,             c->d_ctr,
+                    c->d_pub,           c->d_pfkl,     c->d_htkf,           c->d_sskf,           c->d_htss_lut};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     free_scratch(c->sc);
@@ -215,15 +299,16 @@ extern "C" int tfhe_cbs_ctx_create(const tfhe_cbs_params* params, uint32_t max_b
     rc = rc ? rc : dmalloc((void**)&c->d_K2, kN * sizeof(int32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_bsk2f, bsk2_cplx(p) * sizeof(cplx));
     rc = rc ? rc : dmalloc((void**)&c->d_ksk2, ksk2_words(p) * sizeof(uint32_t));
-    rc = rc ? rc : alloc_scratch(c->sc, B, p.n, p.pfks_linear != 0);
+    const int conv = htss(p) ? 2 : p.pfks_linear ? 1 : 0;
+    rc = rc ? rc : alloc_scratch(c->sc, B, p.n, conv);
     {
         // uneven AES batches of a linear-PFKS context: blocks beyond the last full round of
         // resident PBS workgroups (2 x CUs inputs, rb blocks) run as their own group on the
         // second stream
         const uint32_t slots = 2 * device_cus(), rb = slots / 128;
-        if (!rc && p.pfks_linear && slots % 128 == 0 && rb >= 2 && max_blocks > rb) {
+        if (!rc && conv != 0 && slots % 128 == 0 && rb >= 2 && max_blocks > rb) {
             c->tail_blocks = rb - 1;
-            rc = alloc_scratch(c->tail, 128 * (size_t)c->tail_blocks, p.n, p.pfks_linear != 0);
+            rc = alloc_scratch(c->tail, 128 * (size_t)c->tail_blocks, p.n, conv);
         }
         // the second stream and its events exist before any call, so that no call made
         // under stream capture creates them
@@ -237,6 +322,21 @@ extern "C" int tfhe_cbs_ctx_create(const tfhe_cbs_params* params, uint32_t max_b
     rc = rc ? rc : dmalloc((void**)&c->d_lin_mix, 128 * kLinSrc * sizeof(int16_t));
     rc = rc ? rc : dmalloc((void**)&c->d_lin_last, 128 * kLinSrc * sizeof(int16_t));
     rc = rc ? rc : dmalloc((void**)&c->d_rk, (size_t)11 * 128 * (p.n + 1) * sizeof(uint32_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_dk, (size_t)11 * 128 * (p.n + 1) * sizeof(uint32_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_inv_sbox_poly, kN * sizeof(uint32_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_ttab, (size_t)2 * 4 * 4 * kN * sizeof(uint32_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_lin_inv_mix, 128 * kLinSrcDec * sizeof(int16_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_lin_inv_last, 128 * kLinSrcDec * sizeof(int16_t));
+    rc = rc ? rc : dmalloc((void**)&c->d_lin_id, 128 * sizeof(int16_t));
+    if (!rc && htss(p)) {
+        // LUT[k] = -(Gpow_(k mod 2) / 2) / N: after the trace, coefficient 0 of level j carries
+        // bit Gpow_j (Gpow_j = 2^(48 + 8 j) on the 64-bit torus)
+        std::vector<uint64_t> lut(kN);
+        for (int k = 0; k < kN; ++k) lut[k] = 0 - (1ull << (47 + 8 * (k & 1) - 11));
+        rc = dmalloc((void**)&c->d_htss_lut, kN * sizeof(uint64_t));
+        if (!rc)
+            rc = hipcheck(hipMemcpy(c->d_htss_lut, lut.data(), kN * sizeof(uint64_t), hipMemcpyHostToDevice), "htss lut");
+    }
     rc = rc ? rc : dmalloc((void**)&c->d_ctr, (size_t)max_blocks * 16);
     rc = rc ? rc : dmalloc((void**)&c->d_pub, (size_t)max_blocks * 16);
     rc = rc ? rc : c->tables.init();
@@ -246,6 +346,36 @@ extern "C" int tfhe_cbs_ctx_create(const tfhe_cbs_params* params, uint32_t max_b
         for (int x = 0; x < 256; ++x)
             for (int j = 0; j < 8; ++j) poly[8 * x + j] = (uint32_t)((sbox((uint8_t)x) >> j) & 1) << 31;
         rc = hipcheck(hipMemcpy(c->d_sbox_poly, poly.data(), kN * sizeof(uint32_t), hipMemcpyHostToDevice), "sbox");
+        for (int x = 0; !rc && x < 256; ++x)
+            for (int j = 0; j < 8; ++j) poly[8 * x + j] = (uint32_t)((inv_sbox((uint8_t)x) >> j) & 1) << 31;
+        if (!rc)
+            rc = hipcheck(hipMemcpy(c->d_inv_sbox_poly, poly.data(), kN * sizeof(uint32_t), hipMemcpyHostToDevice),
+                          "inverse sbox");
+        // T-table polynomials: (dir, row r, output row p) coefficient 8x + j = bit j of byte p
+        // of T_r[x], times 2^31
+        std::vector<uint32_t> tt((size_t)2 * 4 * 4 * kN, 0);
+        for (int d = 0; d < 2; ++d)
+            for (int r = 0; r < 4; ++r)
+                for (int x = 0; x < 256; ++x) {
+                    const uint32_t w = ttable_word(d, r, (uint8_t)x);
+                    for (int pr = 0; pr < 4; ++pr)
+                        for (int j = 0; j < 8; ++j)
+                            tt[(((size_t)d * 4 + r) * 4 + pr) * kN + 8 * x + j] = ((w >> (24 - 8 * pr + j)) & 1u) << 31;
+                }
+        if (!rc) rc = hipcheck(hipMemcpy(c->d_ttab, tt.data(), tt.size() * sizeof(uint32_t), hipMemcpyHostToDevice), "ttab");
+        std::vector<int16_t> src;
+        if (!rc) rc = linear_sources([](const uint8_t* i, uint8_t* o) { inv_shift_mix(i, o, true); }, kLinSrcDec, src);
+        if (!rc)
+            rc = hipcheck(hipMemcpy(c->d_lin_inv_mix, src.data(), src.size() * sizeof(int16_t), hipMemcpyHostToDevice),
+                          "inverse linear tables");
+        if (!rc) rc = linear_sources([](const uint8_t* i, uint8_t* o) { inv_shift_mix(i, o, false); }, kLinSrcDec, src);
+        if (!rc)
+            rc = hipcheck(hipMemcpy(c->d_lin_inv_last, src.data(), src.size() * sizeof(int16_t), hipMemcpyHostToDevice),
+                          "inverse linear tables");
+        if (!rc) rc = linear_sources([](const uint8_t* i, uint8_t* o) { std::memcpy(o, i, 16); }, 1, src);
+        if (!rc)
+            rc = hipcheck(hipMemcpy(c->d_lin_id, src.data(), src.size() * sizeof(int16_t), hipMemcpyHostToDevice),
+                          "identity table");
     }
     if (!rc) {
         // source bit lists of the GF(2)-linear ShiftRows(+MixColumns) map, bit = 8*byte + j
@@ -313,7 +443,8 @@ extern "C" int tfhe_gen_bsk64(const tfhe_cbs_params* p, uint64_t seed, const int
             launch_gen_bsk64(seed, p->n, p->rlwe_nshift, d_lwe_sk, d_rlwe_sk, d_K2f, tables->T,
                              d_bsk + bsk2_mb2_first(*p) * 8 * 2 * kN, st, 2);
         } else {
-            launch_gen_bsk64(seed, p->n, p->rlwe_nshift, d_lwe_sk, d_rlwe_sk, d_K2f, tables->T, d_bsk, st, pbs_mb(*p));
+            launch_gen_bsk64(seed, p->n, p->rlwe_nshift, d_lwe_sk, d_rlwe_sk, d_K2f, tables->T, d_bsk, st, pbs_mb(*p),
+                             p->bk_logB);
         }
     }
     if (!rc) rc = check_launch("gen_bsk64");
@@ -340,10 +471,74 @@ extern "C" int tfhe_gen_pfks32(const tfhe_cbs_params* p, uint64_t seed, const in
     return rc;
 }
 
-// the PFKS key of the context's form (allocated once)
+// the conversion key of the context's form (allocated once)
 static int alloc_pfks_key(tfhe_cbs_ctx* c) {
+    if (htss(c->p)) {
+        int rc = c->d_htkf ? TFHE_OK : dmalloc((void**)&c->d_htkf, (size_t)kHtRounds * kKsL * 2 * 2 * 1024 * sizeof(cplx));
+        return rc ? rc : c->d_sskf ? TFHE_OK : dmalloc((void**)&c->d_sskf, (size_t)kKsL * 2 * 2 * 1024 * sizeof(cplx));
+    }
     if (c->p.pfks_linear) return c->d_pfkl ? TFHE_OK : dmalloc((void**)&c->d_pfkl, pfkl_key_bytes());
     return c->d_pfks ? TFHE_OK : dmalloc((void**)&c->d_pfks, 2 * pfks_words_per_u() * sizeof(uint32_t));
+}
+
+// HomTrace + SchemeSwitch keys in coefficient form, d_keys [12][5][2][N] u64: message
+// polynomials -sigma_k(K2) (k < 11, sigma_k: X -> X^(N/2^k + 1)) and K2^2, each times the
+// gadget rows 2^(24 + 8d) (launch_gen_ks64); Philox domains DOM_HTKEY / DOM_SSKEY
+static int gen_htss_keys(const tfhe_cbs_params& p, uint64_t seed, const int32_t* hK2, const cplx* d_K2f,
+                         const FFT1024Tables& T, uint64_t* d_keys, hipStream_t st) {
+    std::vector<int64_t> msg((size_t)(kHtRounds + 1) * kN, 0);
+    for (int k = 0; k < kHtRounds; ++k) {
+        const uint32_t e = (uint32_t)((kN >> k) + 1);
+        for (int i = 0; i < kN; ++i) {
+            const uint32_t t = (uint32_t)(((uint64_t)i * e) & (2 * kN - 1));
+            if (t < (uint32_t)kN) msg[(size_t)k * kN + t] = -(int64_t)hK2[i];
+            else msg[(size_t)k * kN + t - kN] = (int64_t)hK2[i];
+        }
+    }
+    int64_t* sq = msg.data() + (size_t)kHtRounds * kN;  // K2^2 mod X^N + 1
+    for (int i = 0; i < kN; ++i) {
+        if (!hK2[i]) continue;
+        for (int j = 0; j < kN; ++j) {
+            const int64_t v = (int64_t)hK2[i] * hK2[j];
+            if (i + j < kN) sq[i + j] += v;
+            else sq[i + j - kN] -= v;
+        }
+    }
+    int64_t* d_msg = nullptr;
+    int rc = dmalloc((void**)&d_msg, msg.size() * sizeof(int64_t));
+    if (!rc) rc = hipcheck(hipMemcpy(d_msg, msg.data(), msg.size() * sizeof(int64_t), hipMemcpyHostToDevice), "htss msg");
+    if (!rc) {
+        launch_gen_ks64(seed, DOM_HTKEY, kHtRounds, p.rlwe_nshift, d_msg, d_K2f, T, d_keys, st);
+        launch_gen_ks64(seed, DOM_SSKEY, 1, p.rlwe_nshift, d_msg + (size_t)kHtRounds * kN, d_K2f, T,
+                        d_keys + (size_t)kHtRounds * kKsL * 2 * kN, st);
+        rc = check_launch("gen htss keys");
+    }
+    if (!rc) rc = hipcheck(hipStreamSynchronize(st), "gen htss keys sync");
+    (void)hipFree(d_msg);
+    return rc;
+}
+
+// coefficient-form HomTrace + SchemeSwitch keys (device) -> the context's two-limb Fourier keys
+static int htss_keys_to_fourier(tfhe_cbs_ctx* c, const uint64_t* d_keys, hipStream_t st) {
+    launch_to_fourier2048(d_keys, (size_t)kHtRounds * kKsL * 2, c->tables.T, c->d_htkf, st);
+    launch_to_fourier2048(d_keys + (size_t)kHtRounds * kKsL * 2 * kN, (size_t)kKsL * 2, c->tables.T, c->d_sskf, st);
+    return check_launch("htss keys fourier");
+}
+
+extern "C" int tfhe_gen_htss_keys(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_rlwe_sk, uint64_t* d_keys,
+                                  void* stream) {
+    if (!cbs_params_supported(p) || !d_rlwe_sk || !d_keys) return fail(TFHE_E_PARAM, "tfhe_gen_htss_keys: bad argument");
+    DeviceTables1024* tables = nullptr;
+    int rc = stateless_tables(tables);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<int32_t> hK(kN);
+    rc = hipcheck(hipMemcpy(hK.data(), d_rlwe_sk, kN * sizeof(int32_t), hipMemcpyDeviceToHost), "K2");
+    cplx* d_K2f = nullptr;
+    if (!rc) rc = k2_fourier(hK.data(), tables->T, &d_K2f, st);
+    if (!rc) rc = gen_htss_keys(*p, seed, hK.data(), d_K2f, tables->T, d_keys, st);
+    (void)hipFree(d_K2f);
+    return rc;
 }
 
 extern "C" int tfhe_gen_pfkl40(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_rlwe_sk, int8_t* d_pfkl,
@@ -373,10 +568,18 @@ extern "C" int tfhe_cbs_set_keys(tfhe_cbs_ctx* c, const int32_t* rlwe_sk, const 
     // host or device sources: hipMemcpyDefault follows the pointers
     if (!rc) rc = hipcheck(hipMemcpy(c->d_K2, c->K2.data(), kN * sizeof(int32_t), hipMemcpyHostToDevice), "K2");
     if (!rc) rc = hipcheck(hipMemcpy(d_bsk, bsk, bsk_words * sizeof(uint64_t), hipMemcpyDefault), "bsk2");
-    if (!rc)
+    if (!rc && htss(p)) {  // pfks: the HomTrace + SchemeSwitch keys as tfhe_gen_htss_keys writes them
+        uint64_t* d_keys = nullptr;
+        rc = dmalloc((void**)&d_keys, htss_key_words() * sizeof(uint64_t));
+        if (!rc) rc = hipcheck(hipMemcpy(d_keys, pfks, htss_key_words() * sizeof(uint64_t), hipMemcpyDefault), "htss keys");
+        if (!rc) rc = htss_keys_to_fourier(c, d_keys, nullptr);
+        if (!rc) rc = hipcheck(hipDeviceSynchronize(), "htss keys sync");
+        (void)hipFree(d_keys);
+    } else if (!rc) {
         rc = p.pfks_linear ? hipcheck(hipMemcpy(c->d_pfkl, pfks, pfkl_key_bytes(), hipMemcpyDefault), "pfkl")
                            : hipcheck(hipMemcpy(c->d_pfks, pfks, 2 * pfks_words_per_u() * sizeof(uint32_t),
                                                 hipMemcpyDefault), "pfks");
+    }
     if (!rc) rc = hipcheck(hipMemcpy(c->d_ksk2, ksk2, ksk2_words(p) * sizeof(uint32_t), hipMemcpyDefault), "ksk2");
     if (!rc) {
         if (pbs_limbs(p) == 2)
@@ -418,13 +621,21 @@ extern "C" int tfhe_cbs_keygen(tfhe_cbs_ctx* c, const int32_t* lwe_sk, uint64_t 
             launch_gen_bsk64(seed, p.n, p.rlwe_nshift, d_s, c->d_K2, d_K2f, c->tables.T,
                              d_bsk + bsk2_mb2_first(p) * 8 * 2 * kN, nullptr, 2);
         } else {
-            launch_gen_bsk64(seed, p.n, p.rlwe_nshift, d_s, c->d_K2, d_K2f, c->tables.T, d_bsk, nullptr, pbs_mb(p));
+            launch_gen_bsk64(seed, p.n, p.rlwe_nshift, d_s, c->d_K2, d_K2f, c->tables.T, d_bsk, nullptr, pbs_mb(p),
+                             p.bk_logB);
         }
         if (pbs_limbs(p) == 2)
             launch_to_fourier2048(d_bsk, bsk2_rgsw(p) * 8 * 2, c->tables.T, c->d_bsk2f, nullptr);
         else
             launch_to_fourier2048_t64(d_bsk, bsk2_rgsw(p) * 8 * 2, c->tables.T, c->d_bsk2f, nullptr);
-        if (p.pfks_linear) {
+        if (htss(p)) {
+            uint64_t* d_keys = nullptr;
+            rc = dmalloc((void**)&d_keys, htss_key_words() * sizeof(uint64_t));
+            if (!rc) rc = gen_htss_keys(p, seed, c->K2.data(), d_K2f, c->tables.T, d_keys, nullptr);
+            if (!rc) rc = htss_keys_to_fourier(c, d_keys, nullptr);
+            if (!rc) rc = hipcheck(hipDeviceSynchronize(), "htss keys sync");
+            (void)hipFree(d_keys);
+        } else if (p.pfks_linear) {
             rc = hipcheck(hipMemset(c->d_pfkl, 0, pfkl_key_bytes()), "pfkl zero");
             launch_gen_pfkl40(seed, c->d_K2, c->d_pfkl, nullptr);
         } else {
@@ -458,13 +669,19 @@ extern "C" int tfhe_cbs_save_keys(const tfhe_cbs_ctx* c, const char* path) {
     rc = rc ? rc : dmalloc((void**)&d_bsk, bsk2_words(p) * sizeof(uint64_t));
     if (!rc) rc = hipcheck(hipMemcpy(d_s, c->lwe_sk.data(), p.n * sizeof(int32_t), hipMemcpyHostToDevice), "sk");
     if (!rc) rc = tfhe_gen_bsk64(&p, c->seed, d_s, c->d_K2, d_bsk, nullptr);
+    uint64_t* d_keys = nullptr;  // HomTrace + SchemeSwitch keys, also held in Fourier form only
+    if (!rc && htss(p)) {
+        rc = dmalloc((void**)&d_keys, htss_key_words() * sizeof(uint64_t));
+        if (!rc) rc = tfhe_gen_htss_keys(&p, c->seed, c->d_K2, d_keys, nullptr);
+    }
     if (!rc) {
-        const void* const src[4] = {c->K2.data(), d_bsk,
-                                    p.pfks_linear ? (const void*)c->d_pfkl : (const void*)c->d_pfks, c->d_ksk2};
+        const void* conv = htss(p) ? (const void*)d_keys : p.pfks_linear ? (const void*)c->d_pfkl : (const void*)c->d_pfks;
+        const void* const src[4] = {c->K2.data(), d_bsk, conv, c->d_ksk2};
         rc = write_key_blob(path, p, c->seed, src);
     }
     (void)hipFree(d_s);
     (void)hipFree(d_bsk);
+    if (d_keys) (void)hipFree(d_keys);
     return rc;
 }
 
@@ -525,10 +742,26 @@ static int check_cbs(const tfhe_cbs_ctx* c, uint32_t batch, uint32_t cap) {
 // with that FFT form.
 static int auto_pbs_model(uint32_t batch) { return pbs_model_form(batch, 1, 2) == 1 ? 3 : 2; }
 
+// The HomTrace form's PBS: the accumulator (0, LUT X^-b~) with its test polynomial, b + 2^30
+// switched with bw = 1, the 2^10 gadget, written whole (no offsets) into rlwe [batch][2][N]
+static PbsLut htss_pbs_io(const tfhe_cbs_ctx* c, uint64_t* rlwe) {
+    PbsLut io;
+    io.lut = c->d_htss_lut;
+    io.rlwe = rlwe;
+    io.bw = 1;
+    io.b_add = 1u << 30;
+    io.offsets = 0;
+    io.logB = c->p.bk_logB;
+    return io;
+}
+
 extern "C" int tfhe_cbs_pbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t batch, uint64_t* d_out, void* stream) {
     int rc = check_cbs(c, batch, c ? c->max_batch : 0);
     if (rc) return rc;
     if (!d_lwe || !d_out) return fail(TFHE_E_PARAM, "pbs: NULL buffer");
+    if (htss(c->p))
+        return fail(TFHE_E_PARAM, "pbs: the HomTrace + SchemeSwitch form's PBS keeps whole accumulators "
+                                  "(tfhe_cbs_bootstrap_many_lut gives them)");
     if (!batch) return TFHE_OK;
     int mb = pbs_mb(c->p), form = tuned(c->tune_pbs, batch, -1);
     if (pbs_auto(c->p)) {  // forms 2 and 3 = two key bits per step, else one bit with that FFT form
@@ -583,6 +816,7 @@ extern "C" int tfhe_cbs_bootstrap_many_lut(tfhe_cbs_ctx* c, const uint32_t* d_lw
     io.lut = d_lut;
     io.rlwe = d_out;
     io.bw = bitwidth;
+    io.logB = c->p.bk_logB;
     launch_pbs2(d_lwe, batch, c->p.n, bsk, pbs_limbs(c->p), mb, c->tables.T, nullptr, (hipStream_t)stream, form, io);
     return check_launch("bootstrap_many_lut");
 }
@@ -629,6 +863,7 @@ extern "C" int tfhe_cbs_pfks_batch(tfhe_cbs_ctx* c, int u, const uint64_t* d_lwe
     if (rc) return rc;
     if (u < 0 || u > 1 || !d_lwe64 || !d_out || out_stride < 2 * (size_t)kN)
         return fail(TFHE_E_PARAM, "pfks: bad argument");
+    if (htss(c->p)) return fail(TFHE_E_PARAM, "pfks: the context converts by HomTrace + SchemeSwitch");
     if (!count) return TFHE_OK;
     hipStream_t st = (hipStream_t)stream;
     if (c->p.pfks_linear) {
@@ -666,6 +901,14 @@ static void cbs_pfks(tfhe_cbs_ctx* c, const uint64_t* lwe64, uint32_t count, uin
 // circuit bootstraps with scratch set sc (checked arguments)
 static int cbs_batch_impl(tfhe_cbs_ctx* c, const CbsScratch& sc, const uint32_t* d_lwe, uint32_t batch,
                           uint32_t* d_rgsw, hipStream_t st) {
+    if (htss(c->p)) {  // exact 2^10 x 4 PBS accumulators, then HomTrace + SchemeSwitch per CB level
+        launch_pbs2(d_lwe, batch, c->p.n, c->d_bsk2f, 2, 1, c->tables.T, nullptr, st, tuned(c->tune_pbs, batch, -1),
+                    htss_pbs_io(c, sc.lwe64));
+        int rc = check_launch("htss pbs");
+        if (rc) return rc;
+        launch_htss(sc.lwe64, batch, c->d_htkf, c->d_sskf, c->tables.T, d_rgsw, st);
+        return check_launch("htss");
+    }
     const size_t lwe64_words = 2 * (size_t)(kN + 1), rgsw_words = 4 * 2 * (size_t)kN;
     if (!c->p.pfks_linear) launch_zero_rows32(d_rgsw, 2 * batch, 4 * (size_t)kN, 4 * kN, st);  // lookup form adds
     // The PBS runs in rounds of as many workgroups as fit at once (two per CU).  A partial
@@ -864,6 +1107,8 @@ extern "C" int tfhe_cbs_autotune(tfhe_cbs_ctx* c, uint32_t blocks, int* choices,
     const uint32_t nb = 128 * blocks;
     const int n = c->p.n, limbs = pbs_limbs(c->p), mb = pbs_mb(c->p);
     const float budget_ms = 250.f;
+    // the HomTrace form's PBS writes whole accumulators with its own test polynomial
+    const PbsLut io = htss(c->p) ? htss_pbs_io(c, c->sc.lwe64) : PbsLut{};
     if (!rc) {
         launch_fill_hash32(c->sc.state[0], (size_t)nb * (n + 1), 0x9e3779b9u, st);
         launch_fill_hash32(c->sc.lut32, (size_t)nb * (kN + 1), 0x7f4a7c15u, st);
@@ -880,8 +1125,8 @@ extern "C" int tfhe_cbs_autotune(tfhe_cbs_ctx* c, uint32_t blocks, int* choices,
         if (!batch) continue;
         if (batch >= many) {
             choices[part] = pbs_auto(c->p) ? auto_pbs_model(batch) : pbs_model_form(batch, limbs, mb);
-            // the linear-form PFKS is one GEMM shape: no i-slices to report
-            choices[2 + part] = c->p.pfks_linear ? -1 : pfks_model_slices(2 * batch, 2);
+            // the linear-form PFKS is one GEMM shape, HomTrace has none: no i-slices to report
+            choices[2 + part] = (c->p.pfks_linear || htss(c->p)) ? -1 : pfks_model_slices(2 * batch, 2);
             continue;
         }
         const int model_form = pbs_auto(c->p) ? auto_pbs_model(batch) : pbs_model_form(batch, limbs, mb);
@@ -900,19 +1145,19 @@ extern "C" int tfhe_cbs_autotune(tfhe_cbs_ctx* c, uint32_t blocks, int* choices,
                                               f - 2);
                               else
                                   launch_pbs2(c->sc.state[0], batch, n, c->d_bsk2f, limbs, mb, c->tables.T,
-                                              c->sc.lwe64, st, f);
+                                              c->sc.lwe64, st, f, io);
                           },
                           form);
             if (!rc) c->tune_pbs[batch] = form;
         } else {
             // one form: its output still feeds the PFKS timing below
-            launch_pbs2(c->sc.state[0], batch, n, c->d_bsk2f, limbs, mb, c->tables.T, c->sc.lwe64, st);
+            launch_pbs2(c->sc.state[0], batch, n, c->d_bsk2f, limbs, mb, c->tables.T, c->sc.lwe64, st, -1, io);
             rc = check_launch("autotune pbs");
         }
         choices[part] = form;
         if (rc) break;
         const uint32_t count = 2 * batch;
-        if (c->p.pfks_linear) continue;  // one GEMM shape: nothing to choose
+        if (c->p.pfks_linear || htss(c->p)) continue;  // one GEMM shape or no PFKS: nothing to choose
         launch_pfks_digits(c->sc.lwe64, count, c->sc.at, st);
         int sl = 0;
         rc = tune_one(st, e0, e1, {pfks_model_slices(count, 2), 4, 6, 8, 12, 16, 24, 32}, budget_ms,
@@ -1015,21 +1260,45 @@ extern "C" int tfhe_aes_set_round_keys(tfhe_cbs_ctx* c, const uint32_t* rk_bits)
 }
 
 // one AES round of `blocks` blocks with scratch set sc (sc.state[0 or 1] may be d_in or
-// d_out; the other is the key switch's output)
-static int aes_round_impl(tfhe_cbs_ctx* c, const CbsScratch& sc, int round, const uint32_t* d_in, uint32_t blocks,
-                          uint32_t* d_out, hipStream_t st) {
+// d_out; the other is the key switch's output); dir 0 = the cipher (round keys d_rk),
+// 1 = the equivalent inverse cipher (decryption round keys d_dk)
+static int aes_round_impl(tfhe_cbs_ctx* c, CbsScratch& sc, int dir, int round, const uint32_t* d_in,
+                          uint32_t blocks, uint32_t* d_out, hipStream_t st) {
     const uint32_t nb = 128 * blocks;
     const int n = c->p.n;
+    const uint32_t* rk = (dir ? c->d_dk : c->d_rk) + (size_t)round * 128 * (n + 1);
     int rc = cbs_batch_impl(c, sc, d_in, nb, sc.rgsw, st);
     if (!rc) rc = tfhe_rgsw_to_fourier(c, sc.rgsw, nb, reinterpret_cast<double*>(sc.rgsw_f), st);
-    if (!rc) rc = tfhe_lut8_eval_batch(c, reinterpret_cast<const double*>(sc.rgsw_f), 16 * blocks, c->d_sbox_poly, 0, 8,
-                                       sc.lut32, st);
-    // SubBytes output into the spare state buffer, then the linear layer into d_out
     uint32_t* sb = (d_out == sc.state[0]) ? sc.state[1] : sc.state[0];
+    if (c->round_form == 1 && round < 10) {
+        // T-table round: per byte four table polynomials of its row's T-table (Te or Td),
+        // the four words of a column XORed at dimension N, one key switch per bit, then the
+        // round key (the reference's run_round_batch, main.cu:1438-1554)
+        if (!rc) rc = ensure_lutw(sc);
+        if (!rc) {
+            launch_lut_vertical(reinterpret_cast<const cplx*>(sc.rgsw_f), 64 * blocks,
+                                c->d_ttab + (size_t)dir * 16 * kN, kN, 8, 8, c->tables.T, sc.lutw, st, 4, 4);
+            launch_aes_ttable_xor(sc.lutw, blocks, dir, sc.lut32, st);
+            rc = check_launch("aes ttable");
+        }
+        if (!rc) rc = cbs_keyswitch(c, sc.ks_at, sc.lut32, nb, sb, st);
+        if (!rc) {
+            launch_aes_linear(sb, rk, c->d_lin_id, 1, blocks, n, d_out, st);
+            rc = check_launch("aes round key");
+        }
+        return rc;
+    }
+    if (!rc)
+        rc = tfhe_lut8_eval_batch(c, reinterpret_cast<const double*>(sc.rgsw_f), 16 * blocks,
+                                  dir ? c->d_inv_sbox_poly : c->d_sbox_poly, 0, 8, sc.lut32, st);
+    // SubBytes output into the spare state buffer, then the linear layer into d_out
     if (!rc) rc = cbs_keyswitch(c, sc.ks_at, sc.lut32, nb, sb, st);
     if (!rc) {
-        launch_aes_linear(sb, c->d_rk + (size_t)round * 128 * (n + 1), round == 10 ? c->d_lin_last : c->d_lin_mix,
-                          kLinSrc, blocks, n, d_out, st);
+        if (dir)
+            launch_aes_linear(sb, rk, round == 10 ? c->d_lin_inv_last : c->d_lin_inv_mix, kLinSrcDec, blocks, n, d_out,
+                              st);
+        else
+            launch_aes_linear(sb, rk, round == 10 ? c->d_lin_last : c->d_lin_mix, kLinSrc, blocks, n, d_out, st);
         rc = check_launch("aes_linear");
     }
     return rc;
@@ -1044,7 +1313,37 @@ extern "C" int tfhe_aes_round_batch(tfhe_cbs_ctx* c, int round, const uint32_t* 
     if (!c->has_round_keys) return fail(TFHE_E_STATE, "aes_round: no round keys");
     if (blocks > c->max_blocks) return fail(TFHE_E_CAPACITY, "aes_round: blocks exceed the context capacity");
     if (!blocks) return TFHE_OK;
-    return aes_round_impl(c, c->sc, round, d_in, blocks, d_out, (hipStream_t)stream);
+    return aes_round_impl(c, c->sc, 0, round, d_in, blocks, d_out, (hipStream_t)stream);
+}
+
+// One round (1..10) of the equivalent inverse cipher (FIPS-197 5.3.5; the reference's
+// Td-table decryption rounds): InvSubBytes, InvShiftRows, InvMixColumns (rounds 1..9) and
+// the decryption round key, in the context's round form.
+extern "C" int tfhe_aes_dec_round_batch(tfhe_cbs_ctx* c, int round, const uint32_t* d_in, uint32_t blocks,
+                                        uint32_t* d_out, void* stream) {
+    if (!c || !d_in || !d_out || round < 1 || round > 10) return fail(TFHE_E_PARAM, "aes_dec_round: bad argument");
+    if (!c->has_dec_round_keys) return fail(TFHE_E_STATE, "aes_dec_round: no decryption round keys");
+    if (blocks > c->max_blocks) return fail(TFHE_E_CAPACITY, "aes_dec_round: blocks exceed the context capacity");
+    if (!blocks) return TFHE_OK;
+    return aes_round_impl(c, c->sc, 1, round, d_in, blocks, d_out, (hipStream_t)stream);
+}
+
+extern "C" int tfhe_aes_set_round_form(tfhe_cbs_ctx* c, int form) {
+    if (!c || form < 0 || form > 1) return fail(TFHE_E_PARAM, "aes_set_round_form: form must be 0 or 1");
+    if (form == c->round_form) return TFHE_OK;
+    int rc = TFHE_OK;
+    // captured keystreams hold the launches of the old form
+    if (c->gstream) rc = hipcheck(hipStreamSynchronize(c->gstream), "round form sync");
+    if (rc) return rc;
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+    if (form == 1) {  // the LUT word buffers before any call that may be captured
+        rc = ensure_lutw(c->sc);
+        if (!rc && c->tail.cap) rc = ensure_lutw(c->tail);
+        if (rc) return rc;
+    }
+    c->round_form = form;
+    return TFHE_OK;
 }
 
 // the keystream's kernels after the counter upload: state init from round key 0
@@ -1078,9 +1377,9 @@ static int aes_round_plain(tfhe_cbs_ctx* c, int round, const uint32_t* d_in, uin
 
 // state init from round key 0 and the counter blocks d_ctr, then rounds 1..10 ping-ponging
 // the scratch set's state buffers
-static int aes_group(tfhe_cbs_ctx* c, const CbsScratch& sc, const uint8_t* d_ctr, uint32_t blocks, uint32_t* d_out,
-                     hipStream_t st, bool plain) {
-    launch_aes_init(c->d_rk, d_ctr, blocks, c->p.n, sc.state[0], st);
+static int aes_group(tfhe_cbs_ctx* c, CbsScratch& sc, const uint8_t* d_ctr, uint32_t blocks, uint32_t* d_out,
+                     hipStream_t st, bool plain, int dir = 0) {
+    launch_aes_init(dir ? c->d_dk : c->d_rk, d_ctr, blocks, c->p.n, sc.state[0], st);
     int rc = check_launch("aes_init");
     int cur = 0;
     for (int round = 1; round <= 10 && !rc; ++round) {
@@ -1089,7 +1388,7 @@ static int aes_group(tfhe_cbs_ctx* c, const CbsScratch& sc, const uint8_t* d_ctr
         // the caller's buffer
         uint32_t* dst = (round == 10 && d_out) ? d_out : sc.state[cur ^ 1];
         rc = plain ? aes_round_plain(c, round, sc.state[cur], blocks, dst, st)
-                   : aes_round_impl(c, sc, round, sc.state[cur], blocks, dst, st);
+                   : aes_round_impl(c, sc, dir, round, sc.state[cur], blocks, dst, st);
         cur ^= 1;
     }
     return rc;
@@ -1115,16 +1414,19 @@ static uint32_t aes_full_group_blocks(const tfhe_cbs_ctx* c, uint32_t blocks, bo
 }
 
 // d_out == NULL: each group's keystream stays in its scratch set's state[0]
+// d_in: the public input bytes of each block (counter blocks, or ciphertext blocks for
+// dir = 1, the decryption), already on the device
 static int aes_keystream_kernels(tfhe_cbs_ctx* c, uint32_t blocks, uint32_t* d_out, hipStream_t st,
-                                 bool plain = false) {
+                                 bool plain = false, int dir = 0, const uint8_t* d_in = nullptr) {
+    if (!d_in) d_in = c->d_ctr;
     const uint32_t b1 = aes_full_group_blocks(c, blocks, plain);
-    if (!b1) return aes_group(c, c->sc, c->d_ctr, blocks, d_out, st, plain);
+    if (!b1) return aes_group(c, c->sc, d_in, blocks, d_out, st, plain, dir);
     int rc = hipcheck(hipEventRecord(c->ev_gfork, st), "group fork");
     if (!rc) rc = hipcheck(hipStreamWaitEvent(c->cstream, c->ev_gfork, 0), "group fork wait");
-    if (!rc) rc = aes_group(c, c->sc, c->d_ctr, b1, d_out, st, false);
+    if (!rc) rc = aes_group(c, c->sc, d_in, b1, d_out, st, false, dir);
     if (!rc)
-        rc = aes_group(c, c->tail, c->d_ctr + (size_t)16 * b1, blocks - b1,
-                       d_out ? d_out + (size_t)b1 * 128 * (c->p.n + 1) : nullptr, c->cstream, false);
+        rc = aes_group(c, c->tail, d_in + (size_t)16 * b1, blocks - b1,
+                       d_out ? d_out + (size_t)b1 * 128 * (c->p.n + 1) : nullptr, c->cstream, false, dir);
     if (!rc) rc = hipcheck(hipEventRecord(c->ev_gjoin, c->cstream), "group join");
     if (!rc) rc = hipcheck(hipStreamWaitEvent(st, c->ev_gjoin, 0), "group join wait");
     return rc;
@@ -1151,6 +1453,7 @@ extern "C" int tfhe_aes_ctr_keystream_plain(tfhe_cbs_ctx* c, const uint8_t* h_ct
                                             void* stream) {
     int rc = aes_ctr_check(c, h_ctr, blocks, d_out);
     if (rc || !blocks) return rc;
+    if (c->round_form != 0) return fail(TFHE_E_PARAM, "aes_ctr_keystream_plain: the per-bootstrap form is the S-box round form");
     hipStream_t st = (hipStream_t)stream;
     rc = hipcheck(hipMemcpyAsync(c->d_ctr, h_ctr, (size_t)blocks * 16, hipMemcpyHostToDevice, st), "ctr");
     if (rc) return rc;
@@ -1227,6 +1530,53 @@ extern "C" int tfhe_aes_ctr_transcipher_batch(tfhe_cbs_ctx* c, int mode, const u
                      : tfhe_aes_ctr_keystream_batch(c, h_ctr, blocks, d_out, stream);
     if (rc) return rc;
     return tfhe_lwe_xor_public(d_out, c->d_pub, 128 * blocks, c->p.n, stream);
+}
+
+extern "C" void tfhe_aes_encrypt_dec_key_bits(int n, const int32_t* sk, uint64_t seed, const uint8_t key[16],
+                                              int nshift, uint32_t* out) {
+    // decryption schedule of the equivalent inverse cipher: dk[0] = rk[10],
+    // dk[r] = InvMixColumns(rk[10 - r]) (r = 1..9), dk[10] = rk[0]
+    uint8_t rk[176], dk[176];
+    key_expand(key, rk);
+    std::memcpy(dk, rk + 160, 16);
+    std::memcpy(dk + 160, rk, 16);
+    for (int r = 1; r <= 9; ++r) {
+        uint8_t t[16];  // InvMixColumns alone: undo the InvShiftRows of inv_shift_mix first
+        for (int cc = 0; cc < 4; ++cc)
+            for (int row = 0; row < 4; ++row) t[row + 4 * cc] = rk[16 * (10 - r) + row + 4 * ((cc + row) % 4)];
+        inv_shift_mix(t, dk + 16 * r, true);
+    }
+    for (int r = 0; r < 11; ++r)
+        for (int b = 0; b < 128; ++b) {
+            const uint32_t bit = (dk[16 * r + b / 8] >> (b % 8)) & 1u;
+            tfhe_lwe_encrypt(n, sk, seed, (uint32_t)(11 * 128 + r * 128 + b), bit, 2, nshift,
+                             out + ((size_t)r * 128 + b) * (n + 1));
+        }
+}
+
+extern "C" int tfhe_aes_set_dec_round_keys(tfhe_cbs_ctx* c, const uint32_t* dk_bits) {
+    if (!c || !dk_bits) return fail(TFHE_E_PARAM, "aes_set_dec_round_keys: bad argument");
+    int rc = hipcheck(hipMemcpy(c->d_dk, dk_bits, (size_t)11 * 128 * (c->p.n + 1) * sizeof(uint32_t),
+                                hipMemcpyHostToDevice),
+                      "dec round keys");
+    if (!rc) c->has_dec_round_keys = true;
+    return rc;
+}
+
+// Homomorphic AES-128 decryption of public ciphertext blocks under the encrypted key (the
+// reference's default transciphering path: Td tables and InvMixColumns round keys,
+// main.cu:279-317, key_material.cuh:12-47): the state starts as the ciphertext bits plus
+// dk[0], then 10 rounds of the equivalent inverse cipher in the context's round form.
+extern "C" int tfhe_aes_decrypt_batch(tfhe_cbs_ctx* c, const uint8_t* h_cipher, uint32_t blocks, uint32_t* d_out,
+                                      void* stream) {
+    if (!c || !h_cipher || !d_out) return fail(TFHE_E_PARAM, "aes_decrypt: bad argument");
+    if (!c->has_dec_round_keys) return fail(TFHE_E_STATE, "aes_decrypt: no decryption round keys");
+    if (blocks > c->max_blocks) return fail(TFHE_E_CAPACITY, "aes_decrypt: blocks exceed the context capacity");
+    if (!blocks) return TFHE_OK;
+    hipStream_t st = (hipStream_t)stream;
+    int rc = hipcheck(hipMemcpyAsync(c->d_pub, h_cipher, (size_t)blocks * 16, hipMemcpyHostToDevice, st), "cipher");
+    if (rc) return rc;
+    return aes_keystream_kernels(c, blocks, d_out, st, false, 1, c->d_pub);
 }
 
 extern "C" int tfhe_lwe_xor_public(uint32_t* d_ct, const uint8_t* d_pub, uint32_t nbits, int n, void* stream) {

@@ -12,6 +12,7 @@
 namespace tfhe {
 
 constexpr int CB_N = 2048, CB_M = 1024;
+constexpr int PBS_LOGB = 9, PBS_L = 4, PBS_ROWS = 8, PBS_IGN = 28;  // PBS gadget (see pbs_digit)
 
 // ---------------------------------------------------------------------------
 // shared helpers
@@ -223,9 +224,22 @@ __device__ __forceinline__ uint32_t bsk_message_bit(const int32_t* s, uint32_t q
     return pat == 0 ? (b0 & (1u - b1)) : pat == 1 ? ((1u - b0) & b1) : (b0 & b1);
 }
 
-__global__ __launch_bounds__(256) void k_gen_bsk64(uint64_t seed, uint32_t nrows, int nshift,
-                                                  const int32_t* __restrict__ s, const cplx* __restrict__ K2f,
-                                                  FFT1024Tables Tg, int mb, uint64_t* __restrict__ out) {
+// What row `row` of a generated torus-64 key encrypts (RLWE_K2 rows [2][N], Philox ids = row):
+//   msgpoly == NULL (BSK2): row = i * 8 + r, r = 2j + u, adds bsk_message_bit(s, i, mb) *
+//     2^(64 - 4 logB + j logB) to component u at coefficient 0 (domain DOM_BSK2 or DOM_BSK2MB);
+//   msgpoly != NULL (key-switching keys of HomTrace / SchemeSwitch): row = q * KS_L + d adds
+//     msgpoly[q][c] * 2^(64 - KS_L KS_LOGB + d KS_LOGB) to the body (domain dom).
+struct KeyRows {
+    const int32_t* s = nullptr;
+    int mb = 1, logB = 9;
+    const int64_t* msgpoly = nullptr;
+    uint32_t dom = 0;
+};
+constexpr int KS_LOGB = 8, KS_L = 5;  // HomTrace / SchemeSwitch gadget 2^8 x 5 on the 64-bit torus
+
+__global__ __launch_bounds__(256) void k_gen_bsk64(uint64_t seed, uint32_t nrows, int nshift, KeyRows kr,
+                                                  const cplx* __restrict__ K2f, FFT1024Tables Tg,
+                                                  uint64_t* __restrict__ out) {
     __shared__ cplx tl[768];
     __shared__ cplx lds_all[4][F256_LDS_CPLX];
     __shared__ cplx xbuf[2][1024];
@@ -238,11 +252,15 @@ __global__ __launch_bounds__(256) void k_gen_bsk64(uint64_t seed, uint32_t nrows
     for (int sl = 0; sl < 4; ++sl) kf[sl] = K2f[(w * 4 + sl) * 64 + lane];
     int buf = 0;
     for (uint32_t row = blockIdx.x; row < nrows; row += gridDim.x) {
+        const bool ks = kr.msgpoly != nullptr;
         const int r = row % 8, j = r >> 1;
         const int msg_comp = r & 1;
-        const uint64_t cval = (uint64_t)bsk_message_bit(s, row / 8, mb) << (28 + 9 * j);
+        const uint64_t cval =
+            ks ? 0 : (uint64_t)bsk_message_bit(kr.s, row / 8, kr.mb) << (64 - PBS_L * kr.logB + j * kr.logB);
+        const uint64_t* mp = ks ? reinterpret_cast<const uint64_t*>(kr.msgpoly) + (size_t)(row / KS_L) * CB_N : nullptr;
+        const int gsh = 64 - KS_L * KS_LOGB + (int)(row % KS_L) * KS_LOGB;
         const uint32_t id = row;
-        const uint32_t dom = mb == 1 ? DOM_BSK2 : DOM_BSK2MB;
+        const uint32_t dom = ks ? kr.dom : kr.mb == 1 ? DOM_BSK2 : DOM_BSK2MB;
         uint64_t av[8];
         cplx A0[4], A1[4];
 #pragma unroll
@@ -275,7 +293,9 @@ __global__ __launch_bounds__(256) void k_gen_bsk64(uint64_t seed, uint32_t nrows
             uint64_t b = (uint64_t)round_i64(re0, 1.0 / CB_M) + ((uint64_t)round_i64(re1, 1.0 / CB_M) << 32);
             b += (uint64_t)noise(seed, dom, id, (uint32_t)c, nshift);
             uint64_t aval = av[h];
-            if (c == 0) {
+            if (ks) {
+                b += mp[c] << gsh;
+            } else if (c == 0) {
                 if (msg_comp == 0) aval += cval; else b += cval;
             }
             oa[c] = aval;
@@ -354,17 +374,32 @@ __global__ __launch_bounds__(256) void k_gen_pfks32(uint64_t seed, uint32_t nrow
 // coefficients 0 and 1 and add mu_j = 2^(47 + 8j).
 // Output [batch][2][N+1] u64: LWE_K2 of b * 2^(48 + 8j).
 // ===========================================================================
-constexpr int PBS_LOGB = 9, PBS_L = 4, PBS_ROWS = 8, PBS_IGN = 28;
+// PBS gadget: PBS_L = 4 levels of 2^LOGB (LOGB = 9 for every PFKS form, 10 for the
+// HomTrace + SchemeSwitch form), the low 64 - 4 LOGB bits dropped with rounding
 
-__device__ __forceinline__ uint64_t pbs_digit_offset() {
-    uint64_t off = 1ull << (PBS_IGN - 1);
+// signed digits by the offset method on 64 bits: off = 2^(IGN-1) + sum_j (B/2) 2^(IGN + j logB)
+template <int LOGB, int L>
+__device__ __forceinline__ uint64_t digit_offset64() {
+    constexpr int IGN = 64 - L * LOGB;
+    uint64_t off = 1ull << (IGN - 1);
 #pragma unroll
-    for (int j = 0; j < PBS_L; ++j) off += (1ull << (PBS_LOGB - 1)) << (PBS_IGN + j * PBS_LOGB);
+    for (int j = 0; j < L; ++j) off += (1ull << (LOGB - 1)) << (IGN + j * LOGB);
     return off;
 }
+template <int LOGB, int L>
+__device__ __forceinline__ double digit64(uint64_t tp, int j) {
+    constexpr int IGN = 64 - L * LOGB;
+    return (double)((int32_t)((tp >> (IGN + j * LOGB)) & ((1u << LOGB) - 1)) - (1 << (LOGB - 1)));
+}
 
+template <int LOGB = PBS_LOGB>
+__device__ __forceinline__ uint64_t pbs_digit_offset() {
+    return digit_offset64<LOGB, PBS_L>();
+}
+
+template <int LOGB = PBS_LOGB>
 __device__ __forceinline__ double pbs_digit(uint64_t tp, int j) {
-    return (double)((int32_t)((tp >> (PBS_IGN + j * PBS_LOGB)) & ((1u << PBS_LOGB) - 1)) - (1 << (PBS_LOGB - 1)));
+    return digit64<LOGB, PBS_L>(tp, j);
 }
 
 // round(v * 2N / (2^32 2^bw)) * 2^bw mod 2N with N = 2048, bw = 1
@@ -408,7 +443,7 @@ __device__ __forceinline__ uint64_t torus64_from_double(double y) {
 // SIMD and issues each pattern's 8 values as one batch, the next pattern's before the
 // current pattern's MACs; WIDE = true (launches of at most one workgroup per CU)
 // issues all 24 at once with one wave per SIMD.
-template <bool BF, int LIMBS, int MB = 1, bool WIDE = false, bool LUT = false>
+template <bool BF, int LIMBS, int MB = 1, bool WIDE = false, bool LUT = false, int LOGB = PBS_LOGB>
 __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylut(const uint32_t* __restrict__ lwe, uint32_t batch, int n,
                                                      const cplx* __restrict__ bskf, FFT1024Tables Tg,
                                                      uint64_t* __restrict__ out, PbsLut io) {
@@ -427,7 +462,8 @@ __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylu
     const Tw256Regs tw = RT ? load_tw256(Tg, lane) : Tw256Regs{};
     cplx* lds = lds_all[w];
     uint64_t* x64 = reinterpret_cast<uint64_t*>(&xbuf[0][0]);  // [2][2048] u64 scratch for the monic exchange
-    const uint64_t OFF = pbs_digit_offset();
+    const uint64_t OFF = pbs_digit_offset<LOGB>();
+    static_assert(LOGB == PBS_LOGB || (LUT && LIMBS == 2 && MB == 1), "the 2^10 gadget: exact LUT-mode form only");
     constexpr uint32_t N = CB_N, Q2 = 2 * CB_N;
     const uint64_t mu0 = 1ull << 47, mu1 = 1ull << 55;
     uint32_t ek[4];  // evaluation exponents 1 - 4k (mod 2N) of this lane's frequencies
@@ -444,7 +480,7 @@ __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylu
             // as LUT X^{-phase~}
             for (int k = threadIdx.x; k < n; k += blockDim.x)
                 at[k] = (uint16_t)((Q2 - special_ms_2048_bw(ct[k], io.bw)) & (Q2 - 1));
-            bt = special_ms_2048_bw(ct[n], io.bw);
+            bt = special_ms_2048_bw(ct[n] + io.b_add, io.bw);
         } else {
             for (int k = threadIdx.x; k < n; k += blockDim.x) at[k] = (uint16_t)special_ms_2048(ct[k]);
             bt = special_ms_2048(ct[n] + (1u << 30));
@@ -506,7 +542,7 @@ __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylu
                     const int r = 2 * j + u;
                     cplx x[4];
 #pragma unroll
-                    for (int a = 0; a < 4; ++a) x[a] = {pbs_digit(tp[a], j), pbs_digit(tp[a + 4], j)};
+                    for (int a = 0; a < 4; ++a) x[a] = {pbs_digit<LOGB>(tp[a], j), pbs_digit<LOGB>(tp[a + 4], j)};
                     f1024_fwd<BF, RT>(x, lds, xbuf[buf], T, trw, w, lane, tw);
                     buf ^= 1;
                     if constexpr (MB == 2) {
@@ -647,11 +683,12 @@ __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylu
         }
         if constexpr (LUT) {  // the whole RLWE (mask, body), Gpow_j / 2 added to body coefficient j < 2
             uint64_t* r = io.rlwe + (size_t)cidx * 2 * N;
+            const uint64_t o0 = io.offsets ? mu0 : 0, o1 = io.offsets ? mu1 : 0;
 #pragma unroll
             for (int h = 0; h < 8; ++h) {
                 const uint32_t c = (uint32_t)own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0);
                 r[c] = acc[0][h];
-                r[N + c] = acc[1][h] + (c == 0 ? mu0 : c == 1 ? mu1 : 0);
+                r[N + c] = acc[1][h] + (c == 0 ? o0 : c == 1 ? o1 : 0);
             }
             continue;
         }
@@ -667,6 +704,148 @@ __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylu
         }
         if (lane == 0 && w == 0) o0[N] = acc[1][0] + mu0;
         if (lane == 0 && w == 1) o1[N] = acc[1][0] + mu1;
+    }
+}
+
+// ===========================================================================
+// HomTrace + SchemeSwitch LWE -> RGSW (the reference's conversion, cirbts-batch.cu:
+// 1612-1790; trace round kernel.cu:7832-7888, scheme switch 8407-8493, row save
+// 7776-7789), on the 64-bit torus with exact two-limb key-switching products.
+// One workgroup (4 waves) per (input, CB level j): the PBS accumulator times X^-j plus
+// (Gpow_j / 2) / N at coefficient 0, then log2 N trace rounds
+//   (a, b) += (ks_0, sigma_k(b) + ks_1),  ks = sum_d dig_d(sigma_k(a)) HT_k,d,
+//   sigma_k: X -> X^(N/2^k + 1),
+// then the scheme switch (b + ks_0, ks_1), ks = sum_d dig_d(a) SS_d; rows rounded to
+// the 32-bit torus: row 2j = SS, row 2j + 1 = HT.
+// Keys (two-limb Fourier, the BSK2's layout): HT [log2 N][KS_L][2 out][2 limbs][1024],
+// SS [KS_L][2 out][2 limbs][1024].
+// ===========================================================================
+// k^-1 mod 2^12 for odd k (Newton's iteration x <- x (2 - k x) doubles the correct bits)
+__device__ __forceinline__ uint32_t inv_mod_4096(uint32_t k) {
+    uint32_t x = k;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) x *= 2u - k * x;
+    return x & 4095u;
+}
+
+// exact sum_d dig_d(x) key_d for the wave-distributed coefficients x (8 per lane):
+// ks[o][h], o = output component
+__device__ __forceinline__ void htss_keyswitch(const uint64_t (&x)[8], const cplx* __restrict__ key, cplx* lds,
+                                               cplx (*xbuf)[1024], int& buf, const FFT1024Tables& T,
+                                               const cplx (&trw)[4], int w, int lane, uint64_t (&ks)[2][8]) {
+    const uint64_t OFF = digit_offset64<KS_LOGB, KS_L>();
+    uint64_t tp[8];
+#pragma unroll
+    for (int h = 0; h < 8; ++h) tp[h] = x[h] + OFF;
+    cplx macc[2][2][4];
+#pragma unroll
+    for (int o = 0; o < 2; ++o)
+#pragma unroll
+        for (int lb = 0; lb < 2; ++lb)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) macc[o][lb][s] = {0.0, 0.0};
+#pragma unroll 1
+    for (int d = 0; d < KS_L; ++d) {
+        cplx xd[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) xd[a] = {digit64<KS_LOGB, KS_L>(tp[a], d), digit64<KS_LOGB, KS_L>(tp[a + 4], d)};
+        f1024_fwd<false>(xd, lds, xbuf[buf], T, trw, w, lane);
+        buf ^= 1;
+        const cplx* kr = key + (size_t)d * 2 * 2 * CB_M + (w * 4) * 64 + lane;
+#pragma unroll
+        for (int o = 0; o < 2; ++o)
+#pragma unroll
+            for (int lb = 0; lb < 2; ++lb)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) cmac(macc[o][lb][s], xd[s], kr[(o * 2 + lb) * CB_M + s * 64]);
+    }
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+        cplx y0[4], y1[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) { y0[s] = macc[o][0][s]; y1[s] = macc[o][1][s]; }
+        f1024_inv<false>(y0, lds, xbuf[buf], T, trw, w, lane);
+        buf ^= 1;
+        f1024_inv<false>(y1, lds, xbuf[buf], T, trw, w, lane);
+        buf ^= 1;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            ks[o][a] = (uint64_t)round_i64(y0[a].re, 1.0 / CB_M) + ((uint64_t)round_i64(y1[a].re, 1.0 / CB_M) << 32);
+            ks[o][a + 4] = (uint64_t)round_i64(y0[a].im, 1.0 / CB_M) + ((uint64_t)round_i64(y1[a].im, 1.0 / CB_M) << 32);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_htss(const uint64_t* __restrict__ acc_in, uint32_t batch,
+                                             const cplx* __restrict__ htkey, const cplx* __restrict__ sskey,
+                                             FFT1024Tables Tg, uint32_t* __restrict__ rgsw) {
+    __shared__ cplx tl[768];
+    __shared__ cplx lds_all[4][F256_LDS_CPLX];
+    __shared__ cplx xbuf[2][1024];
+    const FFT1024Tables T = stage_tables(Tg, tl);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    cplx trw[4];
+    load_trw(Tg, w, lane, trw);
+    cplx* lds = lds_all[w];
+    uint64_t* x64 = reinterpret_cast<uint64_t*>(&xbuf[0][0]);  // [2][N] u64: the automorphism exchange
+    constexpr uint32_t N = CB_N, Q2 = 2 * CB_N;
+    constexpr int LOGN = 11;
+    int buf = 0;
+    for (uint32_t item = blockIdx.x; item < 2 * batch; item += gridDim.x) {
+        const uint32_t cidx = item >> 1, j = item & 1;
+        const uint64_t* src = acc_in + (size_t)cidx * 2 * N;
+        uint64_t lv[2][8];
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {  // acc X^-j
+            const uint32_t c = (uint32_t)own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0);
+            const uint32_t sidx = c + j;
+#pragma unroll
+            for (int o = 0; o < 2; ++o) lv[o][h] = sidx < N ? src[o * N + sidx] : 0 - src[o * N + sidx - N];
+            if (c == 0) lv[1][h] += 1ull << (47 + 8 * j - LOGN);  // (Gpow_j / 2) / N
+        }
+#pragma unroll 1
+        for (int k = 0; k < LOGN; ++k) {
+            const uint32_t einv = inv_mod_4096((N >> k) + 1);
+            __syncthreads();  // the previous round's transforms are done with xbuf
+#pragma unroll
+            for (int o = 0; o < 2; ++o)
+#pragma unroll
+                for (int h = 0; h < 8; ++h) x64[o * N + own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0)] = lv[o][h];
+            __syncthreads();
+            uint64_t sa[8], sb[8];  // sigma(a), sigma(b): coefficient t is +-c[t e^-1 mod 2N]
+#pragma unroll
+            for (int h = 0; h < 8; ++h) {
+                const uint32_t t = (uint32_t)own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0);
+                const uint32_t u = (t * einv) & (Q2 - 1);
+                const uint32_t si = u < N ? u : u - N;
+                const uint64_t va = x64[si], vb = x64[N + si];
+                sa[h] = u < N ? va : 0 - va;
+                sb[h] = u < N ? vb : 0 - vb;
+            }
+            __syncthreads();  // every wave has read x64 before the transforms reuse xbuf
+            uint64_t ks[2][8];
+            htss_keyswitch(sa, htkey + (size_t)k * KS_L * 2 * 2 * CB_M, lds, xbuf, buf, T, trw, w, lane, ks);
+#pragma unroll
+            for (int h = 0; h < 8; ++h) {
+                lv[0][h] += ks[0][h];
+                lv[1][h] += sb[h] + ks[1][h];
+            }
+        }
+        uint64_t ks[2][8];
+        htss_keyswitch(lv[0], sskey, lds, xbuf, buf, T, trw, w, lane, ks);
+        uint32_t* out = rgsw + (size_t)cidx * 4 * 2 * N;
+        uint32_t* ss_row = out + (size_t)(2 * j) * 2 * N;
+        uint32_t* ht_row = out + (size_t)(2 * j + 1) * 2 * N;
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+            const uint32_t c = (uint32_t)own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0);
+            constexpr uint64_t R = 1ull << 31;  // round to the top 32 bits
+            ss_row[c] = (uint32_t)((lv[1][h] + ks[0][h] + R) >> 32);
+            ss_row[N + c] = (uint32_t)((ks[1][h] + R) >> 32);
+            ht_row[c] = (uint32_t)((lv[0][h] + R) >> 32);
+            ht_row[N + c] = (uint32_t)((lv[1][h] + R) >> 32);
+        }
     }
 }
 
@@ -1043,8 +1222,8 @@ __device__ __forceinline__ uint32_t cb_digit_offset() {
 
 __global__ __launch_bounds__(256) void k_lut_vertical(const cplx* __restrict__ rgsw_f, uint32_t nlut,
                                                      const uint32_t* __restrict__ table, size_t table_stride,
-                                                     int stride, int out_bits, int parts, FFT1024Tables Tg,
-                                                     uint32_t* __restrict__ out) {
+                                                     int stride, int out_bits, int parts, int tperiod,
+                                                     FFT1024Tables Tg, uint32_t* __restrict__ out) {
     __shared__ cplx tl[768];
     __shared__ cplx lds_all[4][F256_LDS_CPLX];
     __shared__ cplx xbuf[2][1024];
@@ -1060,9 +1239,13 @@ __global__ __launch_bounds__(256) void k_lut_vertical(const cplx* __restrict__ r
 
     for (uint32_t lut = blockIdx.x; lut < nlut; lut += gridDim.x) {
         // parts > 1: `parts` table polynomials share one group of 8 RGSWs (an 8 -> 8 parts
-        // bit LUT); LUT lut reads group lut / parts and table polynomial lut % parts
+        // bit LUT); LUT lut reads group lut / parts and table polynomial lut % parts, of
+        // table set (group mod tperiod) when tperiod > 0 (the T-table round: one table per
+        // state row, byte = row + 4 col)
         const uint32_t group = parts > 1 ? lut / (uint32_t)parts : lut;
-        const uint32_t* tab = table + (size_t)(parts > 1 ? lut % (uint32_t)parts : lut) * table_stride;
+        const uint32_t tpoly = parts > 1 ? lut % (uint32_t)parts : lut;
+        const uint32_t tset = tperiod > 0 ? (group % (uint32_t)tperiod) * (uint32_t)parts : 0u;
+        const uint32_t* tab = table + (size_t)(tset + tpoly) * table_stride;
         uint32_t acc[2][8];
 #pragma unroll
         for (int h = 0; h < 8; ++h) {
@@ -1175,6 +1358,27 @@ __global__ void k_aes_linear(const uint32_t* __restrict__ in, const uint32_t* __
     }
 }
 
+// T-table round at dimension N (the reference's word XOR, kernel_RLWE_AddInplace_Batch with
+// its word map, main.cu:1480-1523): output bit 8 (p + 4 c) + j of a block (row p, column c)
+// is the sum of the four LUT outputs lutw [blocks 16 bytes][32 bits][N+1], bit 8 p + j, of
+// the input bytes (r, c + r) (encryption, ShiftRows) or (r, c - r) (decryption, InvShiftRows),
+// r = 0..3: the XOR of the four T-table words' bits.
+__global__ void k_aes_ttable_xor(const uint32_t* __restrict__ lutw, uint32_t blocks, int dec,
+                                 uint32_t* __restrict__ out) {
+    constexpr uint32_t W = CB_N + 1;
+    const uint32_t blk = blockIdx.y, ob = blockIdx.x;
+    if (blk >= blocks) return;
+    const uint32_t byte = ob >> 3, j = ob & 7, p = byte & 3, c = byte >> 2;
+    const uint32_t* src[4];
+#pragma unroll
+    for (uint32_t r = 0; r < 4; ++r) {
+        const uint32_t cin = dec ? (c + 4 - r) & 3 : (c + r) & 3;
+        src[r] = lutw + (((size_t)blk * 16 + r + 4 * cin) * 32 + 8 * p + j) * W;
+    }
+    uint32_t* dst = out + ((size_t)blk * 128 + ob) * W;
+    for (uint32_t k = threadIdx.x; k < W; k += blockDim.x) dst[k] = src[0][k] + src[1][k] + src[2][k] + src[3][k];
+}
+
 // public bits XOR: ct[bit] += (0, pub_bit * 2^31)
 __global__ void k_xor_public(uint32_t* __restrict__ ct, const uint8_t* __restrict__ pub, uint32_t nbits, int n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1188,9 +1392,18 @@ __global__ void k_xor_public(uint32_t* __restrict__ ct, const uint8_t* __restric
 static unsigned grid_for(size_t work, size_t cap) { return (unsigned)(work < cap ? work : cap); }
 
 void launch_lut_vertical(const cplx* rgsw_f, uint32_t nlut, const uint32_t* table, size_t table_stride, int stride,
-                         int out_bits, const FFT1024Tables& T, uint32_t* out, hipStream_t st, int parts) {
-    k_lut_vertical<<<grid_for(nlut, 2048), 256, 0, st>>>(rgsw_f, nlut, table, table_stride, stride, out_bits, parts, T,
-                                                           out);
+                         int out_bits, const FFT1024Tables& T, uint32_t* out, hipStream_t st, int parts, int tperiod) {
+    k_lut_vertical<<<grid_for(nlut, 2048), 256, 0, st>>>(rgsw_f, nlut, table, table_stride, stride, out_bits, parts,
+                                                           tperiod, T, out);
+}
+
+void launch_htss(const uint64_t* acc, uint32_t batch, const cplx* htkey, const cplx* sskey, const FFT1024Tables& T,
+                 uint32_t* rgsw, hipStream_t st) {
+    k_htss<<<grid_for(2 * batch, 2048), 256, 0, st>>>(acc, batch, htkey, sskey, T, rgsw);
+}
+
+void launch_aes_ttable_xor(const uint32_t* lutw, uint32_t blocks, int dec, uint32_t* out, hipStream_t st) {
+    k_aes_ttable_xor<<<dim3(128, blocks), 256, 0, st>>>(lutw, blocks, dec, out);
 }
 
 void launch_aes_init(const uint32_t* rk0, const uint8_t* ctr, uint32_t blocks, int n, uint32_t* state, hipStream_t st) {
@@ -1226,6 +1439,16 @@ static void launch_pbs2_t(const uint32_t* lwe, uint32_t batch, int n, const cplx
     // SIMD), < 0 = pbs_model_form's choice
     const unsigned big = grid_for(batch, 2048);
     if (form < 0) form = pbs_model_form(batch, limbs, mb);
+    if constexpr (LUT) {
+        if (limbs == 2 && io.logB == 10) {  // the HomTrace + SchemeSwitch form's 2^10 x 4 gadget
+            if (form == 1)
+                k_pbs2_manylut<true, 2, 1, false, true, 10><<<batch <= 256 ? batch : big, 256, 0, st>>>(
+                    lwe, batch, n, bskf, T, out, io);
+            else
+                k_pbs2_manylut<false, 2, 1, false, true, 10><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out, io);
+            return;
+        }
+    }
     if (limbs == 2) {
         if (form == 1)
             k_pbs2_manylut<true, 2, 1, false, LUT><<<batch <= 256 ? batch : big, 256, 0, st>>>(lwe, batch, n, bskf, T,
@@ -1345,10 +1568,23 @@ void launch_polymul2048(const int32_t* x, const uint64_t* y, uint32_t batch, con
 }
 
 void launch_gen_bsk64(uint64_t seed, int n, int nshift, const int32_t* s, const int32_t* K2, const cplx* K2f,
-                      const FFT1024Tables& T, uint64_t* bsk, hipStream_t st, int mb) {
+                      const FFT1024Tables& T, uint64_t* bsk, hipStream_t st, int mb, int logB) {
     (void)K2;
     const uint32_t rows = (uint32_t)(mb == 1 ? n : n / 2 * 3) * 8;
-    k_gen_bsk64<<<grid_for(rows, 8192), 256, 0, st>>>(seed, rows, nshift, s, K2f, T, mb, bsk);
+    KeyRows kr;
+    kr.s = s;
+    kr.mb = mb;
+    kr.logB = logB;
+    k_gen_bsk64<<<grid_for(rows, 8192), 256, 0, st>>>(seed, rows, nshift, kr, K2f, T, bsk);
+}
+
+void launch_gen_ks64(uint64_t seed, uint32_t dom, uint32_t npoly, int nshift, const int64_t* msgpoly, const cplx* K2f,
+                     const FFT1024Tables& T, uint64_t* key, hipStream_t st) {
+    const uint32_t rows = npoly * KS_L;
+    KeyRows kr;
+    kr.msgpoly = msgpoly;
+    kr.dom = dom;
+    k_gen_bsk64<<<grid_for(rows, 8192), 256, 0, st>>>(seed, rows, nshift, kr, K2f, T, key);
 }
 
 void launch_gen_pfks(uint64_t seed, int nshift, const int32_t* K2, const cplx* K2f, const FFT1024Tables& T,

@@ -10,14 +10,23 @@ namespace tfhe {
 
 constexpr int kN = 2048;
 
+// lwe_to_rgsw = 1: HomTrace + SchemeSwitch, which needs the exact PBS with a 2^10 x 4 gadget
+inline bool htss(const tfhe_cbs_params& p) { return p.lwe_to_rgsw == 1; }
+
 inline bool cbs_params_supported(const tfhe_cbs_params* p) {
-    return p && p->N == kN && p->bk_logB == 9 && p->bk_l == 4 && p->cb_logB == 8 && p->cb_l == 2 && p->pf_logB == 4 &&
+    return p && p->N == kN && p->bk_l == 4 && p->cb_logB == 8 && p->cb_l == 2 && p->pf_logB == 4 &&
            p->pf_l == 8 && p->ks_logB == 2 && p->ks_l == 8 && p->n > 0 && p->n <= 1024 && p->lwe_nshift >= 1 &&
            p->lwe_nshift <= 31 && p->rlwe_nshift >= -31 && p->rlwe_nshift <= 31 && p->pf_nshift >= 1 &&
            p->pf_nshift <= 31 && (p->pbs_approx == 0 || p->pbs_approx == 1) && p->pbs_multibit >= 0 &&
            p->pbs_multibit <= 3 && (p->pbs_multibit < 2 || (p->pbs_approx == 1 && p->n % 2 == 0)) &&
-           (p->pfks_linear == 0 || p->pfks_linear == 1);
+           (p->pfks_linear == 0 || p->pfks_linear == 1) && (p->lwe_to_rgsw == 0 || p->lwe_to_rgsw == 1) &&
+           (p->lwe_to_rgsw == 0 ? p->bk_logB == 9
+                                : p->bk_logB == 10 && p->pbs_approx == 0 && p->pbs_multibit <= 1 && p->pfks_linear == 0);
 }
+
+// HomTrace + SchemeSwitch keys: 11 trace rounds and the scheme switch, 5 gadget rows each
+constexpr int kHtRounds = 11, kKsL = 5;
+inline size_t htss_key_words() { return (size_t)(kHtRounds + 1) * kKsL * 2 * kN; }  // coefficient form, u64
 
 inline int pbs_limbs(const tfhe_cbs_params& p) { return p.pbs_approx ? 1 : 2; }
 inline int pbs_mb(const tfhe_cbs_params& p) { return p.pbs_multibit == 2 ? 2 : 1; }

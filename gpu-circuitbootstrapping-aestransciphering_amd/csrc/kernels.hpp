@@ -50,7 +50,11 @@ void launch_polymul2048(const int32_t* x, const uint64_t* y, uint32_t batch, con
                         hipStream_t st, int limbs = 2);
 // mb = 1: [n][8][2][N] rows of RGSW(s_i); mb = 2: [n/2][3][8][2][N] rows of RGSW(pattern indicators)
 void launch_gen_bsk64(uint64_t seed, int n, int nshift, const int32_t* s, const int32_t* K2, const cplx* K2f,
-                      const FFT1024Tables& T, uint64_t* bsk, hipStream_t st, int mb = 1);
+                      const FFT1024Tables& T, uint64_t* bsk, hipStream_t st, int mb = 1, int logB = 9);
+// key-switching keys of HomTrace / SchemeSwitch (gadget 2^8 x 5): npoly x 5 RLWE_K2 rows [2][N]
+// u64, row q * 5 + d = RLWE(msgpoly[q] 2^(24 + 8 d)) in Philox domain dom
+void launch_gen_ks64(uint64_t seed, uint32_t dom, uint32_t npoly, int nshift, const int64_t* msgpoly, const cplx* K2f,
+                     const FFT1024Tables& T, uint64_t* key, hipStream_t st);
 void launch_gen_pfks(uint64_t seed, int nshift, const int32_t* K2, const cplx* K2f, const FFT1024Tables& T,
                      uint32_t* key, hipStream_t st);
 void launch_to_fourier2048_32(const uint32_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st);
@@ -70,6 +74,9 @@ struct PbsLut {
     const uint64_t* lut = nullptr;
     uint64_t* rlwe = nullptr;
     uint32_t bw = 1;
+    uint32_t b_add = 0;    // added to b before the modulus switch
+    int offsets = 1;       // 0: no Gpow_j / 2 added (the HomTrace + SchemeSwitch form adds its own)
+    int logB = 9;          // PBS gadget base log: 9, or 10 (exact form only) for HomTrace + SchemeSwitch
 };
 void launch_pbs2(const uint32_t* lwe, uint32_t batch, int n, const cplx* bskf, int limbs, int mb,
                  const FFT1024Tables& T, uint64_t* out, hipStream_t st, int form = -1, const PbsLut& io = PbsLut{});
@@ -100,9 +107,18 @@ int pfks_model_slices(uint32_t B, int nu);
 void launch_pfks32(const uint32_t* at, uint32_t B, const uint32_t* key, size_t key_u_stride, int u0, int nu,
                    uint32_t* out, size_t out_stride, size_t out_u_offset, hipStream_t st, int slices = 0);
 // parts = 1: LUT k reads RGSW group k and table k; parts > 1: `parts` table polynomials
-// (table_stride apart) share each RGSW group (LUT k: group k / parts, table k % parts)
+// (table_stride apart) share each RGSW group (LUT k: group k / parts, table k % parts, of
+// table set (group mod tperiod) when tperiod > 0, sets parts polynomials apart)
 void launch_lut_vertical(const cplx* rgsw_f, uint32_t nlut, const uint32_t* table, size_t table_stride, int stride,
-                         int out_bits, const FFT1024Tables& T, uint32_t* out, hipStream_t st, int parts = 1);
+                         int out_bits, const FFT1024Tables& T, uint32_t* out, hipStream_t st, int parts = 1,
+                         int tperiod = 0);
+// HomTrace + SchemeSwitch of the PBS accumulators acc [batch][2][N] (LUT form, no offsets)
+// into torus-32 RGSW rows [batch][4][2][N] (row 2j SS, 2j + 1 HT); two-limb Fourier keys
+void launch_htss(const uint64_t* acc, uint32_t batch, const cplx* htkey, const cplx* sskey, const FFT1024Tables& T,
+                 uint32_t* rgsw, hipStream_t st);
+// T-table round: XOR of the four T-table words per column at dimension N, lutw
+// [blocks 16][32][N+1] -> out [blocks][128][N+1]; dec = 1: the InvShiftRows word map
+void launch_aes_ttable_xor(const uint32_t* lutw, uint32_t blocks, int dec, uint32_t* out, hipStream_t st);
 void launch_aes_init(const uint32_t* rk0, const uint8_t* ctr, uint32_t blocks, int n, uint32_t* state, hipStream_t st);
 void launch_aes_linear(const uint32_t* in, const uint32_t* rk, const int16_t* src, int nsrc, uint32_t blocks, int n,
                        uint32_t* out, hipStream_t st);

@@ -7,8 +7,9 @@
 //
 // Layout (little-endian): a 256-byte header, then the four sections back to back:
 //   rlwe_sk [N] int32 | bsk2 rows as tfhe_gen_bsk64 writes them (uint64) |
-//   PFKS key (lookup form [2][N+1][8][15][2N] uint32, or the linear form's bytes as
-//   tfhe_gen_pfkl40 writes them) | ksk2 [N][8][4][n+1] uint32.
+//   conversion key (lookup PFKS [2][N+1][8][15][2N] uint32, the linear PFKS key's bytes as
+//   tfhe_gen_pfkl40 writes them, or the HomTrace + SchemeSwitch keys as
+//   tfhe_gen_htss_keys writes them) | ksk2 [N][8][4][n+1] uint32.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -37,9 +38,10 @@ struct BlobHeader {
     uint64_t bytes[4];    // section sizes: rlwe_sk, bsk2, pfks, ksk2
     uint64_t hash[4];     // content hash of each section
     uint64_t header_hash; // hash of all fields above
-    uint8_t pad[96];
+    uint8_t pad[88];
 };
 static_assert(sizeof(BlobHeader) == 256, "key file header is 256 bytes");
+static_assert(sizeof(tfhe_cbs_params) == 68, "the header holds the 17-field parameter set");
 
 // Streaming 64-bit content hash over little-endian 64-bit words (an integrity check,
 // not a cryptographic one): h = mix(h ^ w) per word, with the multiply-xorshift
@@ -81,7 +83,9 @@ uint64_t header_hash(const BlobHeader& b) {
 void section_sizes(const tfhe_cbs_params& p, uint64_t (&bytes)[4]) {
     bytes[0] = (uint64_t)kN * sizeof(int32_t);
     bytes[1] = (uint64_t)bsk2_words(p) * sizeof(uint64_t);
-    bytes[2] = p.pfks_linear ? (uint64_t)pfkl_key_bytes() : (uint64_t)2 * pfks_words_per_u() * sizeof(uint32_t);
+    bytes[2] = htss(p)          ? (uint64_t)htss_key_words() * sizeof(uint64_t)
+               : p.pfks_linear ? (uint64_t)pfkl_key_bytes()
+                               : (uint64_t)2 * pfks_words_per_u() * sizeof(uint32_t);
     bytes[3] = (uint64_t)ksk2_words(p) * sizeof(uint32_t);
 }
 

@@ -46,18 +46,22 @@ class CbsParams(ctypes.Structure):
                 ("cb_logB", ctypes.c_int), ("cb_l", ctypes.c_int), ("pf_logB", ctypes.c_int), ("pf_l", ctypes.c_int),
                 ("ks_logB", ctypes.c_int), ("ks_l", ctypes.c_int), ("lwe_nshift", ctypes.c_int),
                 ("rlwe_nshift", ctypes.c_int), ("pf_nshift", ctypes.c_int), ("pbs_approx", ctypes.c_int),
-                ("pbs_multibit", ctypes.c_int), ("pfks_linear", ctypes.c_int)]
+                ("pbs_multibit", ctypes.c_int), ("pfks_linear", ctypes.c_int), ("lwe_to_rgsw", ctypes.c_int)]
 
 
-def cbs_params(pbs_approx=0, pbs_multibit=1, pfks_linear=0):
+def cbs_params(pbs_approx=0, pbs_multibit=1, pfks_linear=0, lwe_to_rgsw=0):
     """n=636, N=2048; PBS 2^9 x 4 on the 64-bit torus (sigma 2^-51), PFKS 2^4 x 8 with
     a torus-32 key (sigma 2^-30), CB 2^8 x 2 (torus-32 RGSW), KS 2^2 x 8 to the
     level-0 key (sigma 2^-15).  pbs_approx=1 selects the one-product FP64 PBS
     (rounding error about 2^-33 of the torus) instead of the exact two-limb one;
     pbs_multibit=2 (with pbs_approx=1) blind-rotates two key bits per step; pfks_linear=1
-    switches keys through the linear 40-bit PFKS key on the matrix cores."""
+    switches keys through the linear 40-bit PFKS key on the matrix cores.  lwe_to_rgsw=1
+    converts by HomTrace + SchemeSwitch (the reference's conversion): exact PBS with a
+    2^10 x 4 gadget and BSK2 / HT / SS keys at sigma 2^-54, no PFKS key."""
+    if lwe_to_rgsw:
+        return CbsParams(636, 2048, 10, 4, 8, 2, 4, 8, 2, 8, 15, 22, 30, 0, min(int(pbs_multibit), 1), 0, 1)
     return CbsParams(636, 2048, 9, 4, 8, 2, 4, 8, 2, 8, 15, 19, 30, int(pbs_approx), int(pbs_multibit),
-                     int(pfks_linear))
+                     int(pfks_linear), 0)
 
 
 def build():
@@ -103,6 +107,7 @@ SIGNATURES = {
     "tfhe_cbs_get_pfkl_entry": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")]),
     "tfhe_gen_pfks32": (ctypes.c_int, [ctypes.POINTER(CbsParams), ctypes.c_uint64, _vp, _vp, _vp]),
+    "tfhe_gen_htss_keys": (ctypes.c_int, [ctypes.POINTER(CbsParams), ctypes.c_uint64, _vp, _vp, _vp]),
     "tfhe_cbs_set_keys": (ctypes.c_int, [_vp, _i32p, _vp, _vp, _vp]),
     "tfhe_cbs_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_cbs_pbs_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
@@ -141,6 +146,14 @@ SIGNATURES = {
     "tfhe_aes_ctr_keystream_plain": (ctypes.c_int, [_vp, np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
                                                     ctypes.c_uint32, _vp, _vp]),
     "tfhe_lwe_xor_public": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_int, _vp]),
+    "tfhe_aes_set_round_form": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "tfhe_aes_encrypt_dec_key_bits": (None, [ctypes.c_int, _i32p, ctypes.c_uint64,
+                                             np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
+                                             ctypes.c_int, _u32p]),
+    "tfhe_aes_set_dec_round_keys": (ctypes.c_int, [_vp, _u32p]),
+    "tfhe_aes_dec_round_batch": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_uint32, _vp, _vp]),
+    "tfhe_aes_decrypt_batch": (ctypes.c_int, [_vp, np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
+                                              ctypes.c_uint32, _vp, _vp]),
     "tfhe_aes_ctr_transcipher_batch": (ctypes.c_int, [_vp, ctypes.c_int,
                                                       np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
                                                       np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"),
@@ -181,6 +194,20 @@ def _need(t, nbytes, what):
     have = t.numel() * t.element_size()
     if have < nbytes:
         raise TfheError(TFHE_E_SHAPE, f"{what}: {have} bytes, {nbytes} needed")
+
+
+def _need_span(t, nbytes, what):
+    """Raise unless t is a device tensor whose storage holds nbytes from t's first
+    element: for outputs written at a stride (rows out_stride words apart), where t may
+    be a strided view into a larger buffer."""
+    if t is None:
+        raise TfheError(TFHE_E_PARAM, f"{what}: buffer is None")
+    if not getattr(t, "is_cuda", False):
+        raise TfheError(TFHE_E_PARAM, f"{what}: expected a device tensor")
+    st = t.untyped_storage()
+    have = st.nbytes() - (t.data_ptr() - st.data_ptr())
+    if have < nbytes:
+        raise TfheError(TFHE_E_SHAPE, f"{what}: {have} bytes from the first element, {nbytes} needed")
 
 
 def _blocks16(blocks, what):
@@ -324,6 +351,15 @@ def polymul2048_split32_batch(d_x, d_y, d_out, form, stream=None):
                                                _stream(stream)))
 
 
+def aes_encrypt_dec_key_bits(sk, seed, key, nshift=15):
+    """The equivalent inverse cipher's round-key bits encrypted under sk, [11][128][n+1]."""
+    n = len(sk)
+    out = np.zeros(11 * 128 * (n + 1), np.uint32)
+    lib().tfhe_aes_encrypt_dec_key_bits(n, np.ascontiguousarray(sk, np.int32), seed,
+                                        np.frombuffer(bytes(key), np.uint8).copy(), nshift, out)
+    return out.reshape(11, 128, n + 1)
+
+
 def aes_encrypt_key_bits(sk, seed, key, nshift=15):
     n = len(sk)
     out = np.zeros(11 * 128 * (n + 1), np.uint32)
@@ -403,7 +439,7 @@ class CbsContext:
     def pfks_batch(self, u, d_lwe64, count, out_stride, d_out, stream=None):
         if count:
             _need(d_lwe64, count * 8 * (self.p.N + 1), "pfks in")
-            _need(d_out, ((count - 1) * out_stride + 2 * self.p.N) * 4, "pfks out")
+            _need_span(d_out, ((count - 1) * out_stride + 2 * self.p.N) * 4, "pfks out")
         check(lib().tfhe_cbs_pfks_batch(self.h, u, _ptr(d_lwe64), count, out_stride, _ptr(d_out), _stream(stream)))
 
     def cbs_batch(self, d_lwe, d_rgsw, batch=None, stream=None):
@@ -483,6 +519,25 @@ class CbsContext:
         ms, n, inp = ctypes.c_double(), ctypes.c_uint32(), ctypes.c_uint64()
         check(lib().tfhe_cbs_timing_read(self.h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(inp)))
         return ms.value, n.value, inp.value
+
+    def set_round_form(self, form):
+        """0 = S-box LUT + LWE linear layers, 1 = T-table rounds (tfhe_aes_set_round_form)."""
+        check(lib().tfhe_aes_set_round_form(self.h, int(form)))
+
+    def set_dec_round_keys(self, dk_bits):
+        check(lib().tfhe_aes_set_dec_round_keys(self.h, np.ascontiguousarray(dk_bits, np.uint32).reshape(-1)))
+
+    def aes_dec_round(self, rnd, d_in, blocks, d_out, stream=None):
+        row = 128 * (self.p.n + 1) * 4
+        _need(d_in, blocks * row, "aes_dec_round in")
+        _need(d_out, blocks * row, "aes_dec_round out")
+        check(lib().tfhe_aes_dec_round_batch(self.h, rnd, _ptr(d_in), blocks, _ptr(d_out), _stream(stream)))
+
+    def aes_decrypt(self, cipher_blocks, d_out, stream=None):
+        """LWE bits [blocks][128][n+1] of AES-128^-1 of public ciphertext blocks (tfhe_aes_decrypt_batch)."""
+        ct = _blocks16(cipher_blocks, "aes_decrypt ciphertext")
+        _need(d_out, len(cipher_blocks) * 128 * (self.p.n + 1) * 4, "aes_decrypt out")
+        check(lib().tfhe_aes_decrypt_batch(self.h, ct, len(cipher_blocks), _ptr(d_out), _stream(stream)))
 
     def set_round_keys(self, rk_bits):
         check(lib().tfhe_aes_set_round_keys(self.h, np.ascontiguousarray(rk_bits, np.uint32).reshape(-1)))

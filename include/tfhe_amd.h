@@ -166,6 +166,19 @@ typedef struct tfhe_cbs_params {
      *     rounded to the 32-bit torus.  Output noise 2^-23.8 of the torus against
      *     2^-23.0 for the lookup key; the two forms give different words. */
     int pfks_linear;
+    /* LWE -> RGSW conversion after the PBS:
+     * 0 = private functional key switching in the pfks_linear form (default);
+     * 1 = HomTrace + SchemeSwitch, the reference's own conversion (cirbts-batch.cu:
+     *     1612-1790): per CB level the PBS accumulator's level coefficient, prescaled by
+     *     1/N, is traced over log2 N = 11 automorphism rounds (each a key switch with
+     *     gadget 2^8 x 5) into RLWE(bit g_j) (row 2j + 1), and scheme-switched with the
+     *     key RLWE(K2^2 g_d) into RLWE(-K2 bit g_j) (row 2j), exact two-limb FFT products
+     *     throughout.  The trace multiplies the PBS noise by N at the constant coefficient,
+     *     so this form needs a quieter PBS: exact (pbs_approx = 0, pbs_multibit <= 1) with
+     *     a 2^10 x 4 gadget (bk_logB = 10) and, for the stated noise, rlwe_nshift = 22
+     *     (BSK2, HT and SS keys at sigma 2^-54).  No PFKS key is held (keys: 3.7 MB for
+     *     HT + SS).  Rows are the same RGSW(bit) the PFKS forms give, other words. */
+    int lwe_to_rgsw;
 } tfhe_cbs_params;
 
 typedef struct tfhe_cbs_ctx tfhe_cbs_ctx;
@@ -192,6 +205,12 @@ int tfhe_gen_bsk64(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_lwe
                    uint64_t* d_bsk, void* stream);
 /* stateless PFKS key generator into d_pfks [2][N+1][8][15][2N] uint32 (the
  * context's key; tfhe_cbs_get_pfks_entry reads entries of it) */
+/* stateless HomTrace + SchemeSwitch key generator (lwe_to_rgsw = 1) into d_keys
+ * [12][5][2][N] uint64: rows (k, d), k < 11, = RLWE_K2(-sigma_k(K2) 2^(24 + 8d)) with
+ * sigma_k: X -> X^(N/2^k + 1), then rows (11, d) = RLWE_K2(K2^2 2^(24 + 8d)); Philox
+ * domains 11 and 12, noise rlwe_nshift; synchronises the stream */
+int tfhe_gen_htss_keys(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_rlwe_sk, uint64_t* d_keys,
+                       void* stream);
 /* stateless linear PFKS key generator (pfks_linear = 1) into d_pfkl, [2][5][2N][16416]
  * int8 (2 x 336 MB): entry (u, i, t) at k = 8 i + t of every column, as 5 signed bytes
  * of its 40-bit words; synchronises the stream */
@@ -412,6 +431,39 @@ int tfhe_aes_ctr_keystream_graph(tfhe_cbs_ctx* ctx, const uint8_t* h_ctr, uint32
  * mode 0 = batched keystream, 1 = hipGraph replay, 2 = per-bootstrap form. */
 int tfhe_aes_ctr_transcipher_batch(tfhe_cbs_ctx* ctx, int mode, const uint8_t* h_ctr, const uint8_t* h_cipher,
                                    uint32_t blocks, uint32_t* d_out, void* stream);
+/* AES round form of the context (both directions):
+ * 0 (default) = SubBytes by the 8 -> 8 S-box LUT, then ShiftRows / MixColumns /
+ *     AddRoundKey as LWE additions after the key switch;
+ * 1 = T-table rounds, the reference's data flow (run_round_batch, main.cu:1438-1554;
+ *     EvalLUT8x32_RLWE_GPU_DeviceCtrl_Batch, gpu_lut.cuh:1783-2214): rounds 1..9 look
+ *     each byte up in its row's T-table (Te_r, or Td_r for decryption; 8 -> 32 bits,
+ *     four vertical-packing polynomials on the byte's 8 RGSWs), XOR the four words of
+ *     each output column at dimension N (the ShiftRows / InvShiftRows word map), key-
+ *     switch each of the 128 bits and add the round key; round 10 is the S-box form.
+ *     Each bit then carries the noise of four LUTs, against one LUT plus up to 7 (19
+ *     for decryption) key-switched LWEs in form 0.
+ * The two forms give different words that decrypt to the same bits.  Changing the
+ * form drops the context's captured keystream graphs.  The per-bootstrap keystream
+ * (tfhe_aes_ctr_keystream_plain, mode 2) is form 0 only. */
+int tfhe_aes_set_round_form(tfhe_cbs_ctx* ctx, int form);
+/* AES-128 decryption (the reference's default transciphering direction: the server
+ * decrypts public AES ciphertext under the encrypted key with Td tables and
+ * InvMixColumns round keys, main.cu:279-317, key_material.cuh:12-47 rk_imc_words).
+ * Client side: the decryption schedule of the equivalent inverse cipher (FIPS-197
+ * 5.3.5: dk[0] = rk[10], dk[r] = InvMixColumns(rk[10-r]), dk[10] = rk[0]), its bits
+ * encrypted under sk, [11][128][n+1]. */
+void tfhe_aes_encrypt_dec_key_bits(int n, const int32_t* sk, uint64_t seed, const uint8_t key[16], int nshift,
+                                   uint32_t* out);
+int tfhe_aes_set_dec_round_keys(tfhe_cbs_ctx* ctx, const uint32_t* dk_bits);
+/* one round (1..10) of the equivalent inverse cipher on [blocks][128][n+1] */
+int tfhe_aes_dec_round_batch(tfhe_cbs_ctx* ctx, int round, const uint32_t* d_in, uint32_t blocks, uint32_t* d_out,
+                             void* stream);
+/* d_out [blocks][128][n+1] = LWE bits of AES-128^-1_k(h_cipher[blk]) for public
+ * ciphertext blocks h_cipher [blocks][16] (host): the state starts as the public bits
+ * plus dk[0], then the 10 rounds.  Uneven batches run as two block groups as the
+ * keystream does. */
+int tfhe_aes_decrypt_batch(tfhe_cbs_ctx* ctx, const uint8_t* h_cipher, uint32_t blocks, uint32_t* d_out,
+                           void* stream);
 /* XOR of public bits into LWE bits: d_ct[i] += (0, bit_i * 2^31), d_pub packed bits */
 int tfhe_lwe_xor_public(uint32_t* d_ct, const uint8_t* d_pub, uint32_t nbits, int n, void* stream);
 

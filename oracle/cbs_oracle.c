@@ -368,7 +368,8 @@ double orc_pbs_manylut_mb(const orc_cbs_params* p, const double* bsk_f, int mb, 
     return err;
 }
 
-/* The reference's gpu_BootstrapManyLUT (cirbts-bootstrap.cu:753-839) with an
+/* orc_blind_rotate_lut64: the accumulator part, without the offsets.
+ * orc_bootstrap_many_lut: the reference's gpu_BootstrapManyLUT (cirbts-bootstrap.cu:753-839) with an
  * arbitrary test polynomial: a~_i, b~ = gpu_SpecilMS with bitwidth bw
  * (cirbts-utils.cu:52-61, no offset on b), acc = (0, LUT X^{-b~}) (the memcpy of LUT
  * and kernel_ModMulScalar by the monomial X^{-b_MS}), the blind rotation (mb key bits
@@ -379,7 +380,7 @@ double orc_pbs_manylut_mb(const orc_cbs_params* p, const double* bsk_f, int mb, 
  * kernel.cu:7459-7473; Gpow_j = 2^(64 - cb_l cb_logB + j cb_logB) here).
  * out: RLWE [2][N] (mask, body), 64-bit torus, coefficient domain; coefficient c of
  * its phase is LUT[c + phase~] (negated past N) plus the offsets and the noise. */
-double orc_bootstrap_many_lut(const orc_cbs_params* p, const double* bsk_f, int mb, const uint32_t* ct32,
+double orc_blind_rotate_lut64(const orc_cbs_params* p, const double* bsk_f, int mb, const uint32_t* ct32,
                               const uint64_t* lut, int bw, uint64_t* out) {
     int n = p->n, N = p->N;
     uint32_t Q2 = 2u * (uint32_t)N;
@@ -393,9 +394,16 @@ double orc_bootstrap_many_lut(const orc_cbs_params* p, const double* bsk_f, int 
         out[N + c] = m < (uint32_t)N ? lut[m] : 0 - lut[m - (uint32_t)N];
     }
     double err = mb == 2 ? orc_blind_rotate64_mb2(p, bsk_f, at, out) : orc_blind_rotate64(p, bsk_f, at, out);
+    free(at);
+    return err;
+}
+
+double orc_bootstrap_many_lut(const orc_cbs_params* p, const double* bsk_f, int mb, const uint32_t* ct32,
+                              const uint64_t* lut, int bw, uint64_t* out) {
+    int N = p->N;
+    double err = orc_blind_rotate_lut64(p, bsk_f, mb, ct32, lut, bw, out);
     for (int j = 0; j < p->cb_l && j < N; ++j)
         out[N + j] += (uint64_t)1 << (64 - p->cb_l * p->cb_logB + j * p->cb_logB - 1);
-    free(at);
     return err;
 }
 
@@ -694,4 +702,169 @@ void orc_ks2(const orc_cbs_params* p, const uint32_t* ksk2, const uint64_t* lwe6
 void orc_gen_ksk2(uint64_t seed, const orc_cbs_params* p, const int32_t* s, const int32_t* K2, uint32_t* ksk) {
     orc_gate_params g = {p->n, p->N, 0, 0, p->ks_logB, p->ks_l, p->lwe_nshift, 0};
     orc_gen_ksk32_dom(seed, ORC_DOM_KSK2, &g, s, K2, ksk);
+}
+
+/* ------------------------------------------------------------------------- */
+/* HomTrace + SchemeSwitch LWE -> RGSW (the reference's conversion,            */
+/* cirbts-batch.cu:1612-1790; kernels kernel.cu:7536-7569 (automorphism maps), */
+/* 7776-7789 (row save), 7832-7888 (trace round), 8407-8493 (scheme switch);   */
+/* keys cirbts-init.cu:734-864), restated on the 64-bit torus with exact FFT    */
+/* products.  RLWE ct = (a, b), phase b - a K2.                                 */
+/*   trace round k = 0 .. log2 N - 1, automorphism sigma_k: X -> X^(N/2^k + 1): */
+/*     (a, b) += (ks_0, sigma_k(b) + ks_1),  ks = sum_d dig_d(sigma_k(a)) HT_k,d */
+/*     HT_k,d = RLWE_K2(-sigma_k(K2) g_d): (ks_0, sigma(b) + ks_1) is sigma(ct)  */
+/*     switched back to K2, so each round adds the automorphic image and the    */
+/*     trace multiplies the constant coefficient by N and kills the others.     */
+/*   scheme switch: (b + ks_0, ks_1), ks = sum_d dig_d(a) SS_d,                 */
+/*     SS_d = RLWE_K2(K2^2 g_d): phase a K2^2 - b K2 = -K2 (b - a K2).          */
+/* Gadget 2^8 x 5 (40 of 64 bits, g_d = 2^(24 + 8 d)) for both; key noise as the */
+/* BSK2's (rlwe_nshift).                                                        */
+/* ------------------------------------------------------------------------- */
+#define ORC_HT_LOGB 8
+#define ORC_HT_L 5
+
+/* out(X) = in(X^k) mod X^N + 1, k odd: coefficient i goes to i k mod 2N (negated past N) */
+void orc_automorphism64(int N, const uint64_t* in, uint32_t k, uint64_t* out) {
+    uint32_t Q2 = 2u * (uint32_t)N;
+    for (int i = 0; i < N; ++i) {
+        uint32_t t = (uint32_t)(((uint64_t)i * k) & (Q2 - 1));
+        if (t < (uint32_t)N) out[t] = in[i];
+        else out[t - (uint32_t)N] = 0 - in[i];
+    }
+}
+
+static int orc_log2(int N) {
+    int lg = 0;
+    while ((1 << lg) < N) ++lg;
+    return lg;
+}
+
+/* HT key [log2 N][HT_L][2][N]: entry (k, d) = RLWE_K2(-sigma_k(K2) g_d), Philox domain
+ * ORC_DOM_HTKEY, id k HT_L + d */
+void orc_gen_ht_key(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, uint64_t* key) {
+    int N = p->N, R = orc_log2(N);
+    uint64_t* k64 = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    uint64_t* sk = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    uint64_t* msg = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    for (int i = 0; i < N; ++i) k64[i] = (uint64_t)(int64_t)K2[i];
+    for (int k = 0; k < R; ++k) {
+        orc_automorphism64(N, k64, (uint32_t)((N >> k) + 1), sk);
+        for (int d = 0; d < ORC_HT_L; ++d) {
+            uint64_t g = (uint64_t)1 << (64 - ORC_HT_L * ORC_HT_LOGB + d * ORC_HT_LOGB);
+            for (int i = 0; i < N; ++i) msg[i] = 0 - sk[i] * g;
+            orc_rlwe_encrypt64(seed, ORC_DOM_HTKEY, (uint32_t)(k * ORC_HT_L + d), N, K2, NULL, msg, p->rlwe_nshift,
+                               key + ((size_t)k * ORC_HT_L + d) * 2 * N);
+        }
+    }
+    free(k64); free(sk); free(msg);
+}
+
+/* SS key [HT_L][2][N]: entry d = RLWE_K2(K2^2 g_d), domain ORC_DOM_SSKEY, id d */
+void orc_gen_ss_key(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, uint64_t* key) {
+    int N = p->N;
+    uint64_t* k64 = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    uint64_t* sq = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    uint64_t* msg = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    for (int i = 0; i < N; ++i) k64[i] = (uint64_t)(int64_t)K2[i];
+    mul_binary64(K2, k64, sq, N);
+    for (int d = 0; d < ORC_HT_L; ++d) {
+        uint64_t g = (uint64_t)1 << (64 - ORC_HT_L * ORC_HT_LOGB + d * ORC_HT_LOGB);
+        for (int i = 0; i < N; ++i) msg[i] = sq[i] * g;
+        orc_rlwe_encrypt64(seed, ORC_DOM_SSKEY, (uint32_t)d, N, K2, NULL, msg, p->rlwe_nshift,
+                           key + (size_t)d * 2 * N);
+    }
+    free(k64); free(sq); free(msg);
+}
+
+/* Fourier form of nkeys key-switching keys of HT_L rows each, in the external-product
+ * layout [2 HT_L rows][2 out][4M] with row 2d = entry d and row 2d + 1 zero (so that
+ * orc_external_product64 of (x, 0) is sum_d dig_d(x) entry_d) */
+void orc_ksk64_to_fourier(const orc_cbs_params* p, int nkeys, const uint64_t* key, double* key_f) {
+    int N = p->N, M = N / 2;
+    size_t per = (size_t)2 * ORC_HT_L * 2 * 4 * M;
+    memset(key_f, 0, sizeof(double) * per * (size_t)nkeys);
+    for (int q = 0; q < nkeys; ++q)
+        for (int d = 0; d < ORC_HT_L; ++d)
+            for (int o = 0; o < 2; ++o)
+                orc_to_fourier64(key + (((size_t)q * ORC_HT_L + d) * 2 + o) * N, N,
+                                 key_f + (size_t)q * per + ((size_t)(2 * d) * 2 + o) * 4 * M);
+}
+
+/* sum_d dig_d(x) key_d, exact (x [N], out [2][N]) */
+static double ks64(int N, const double* key_f, const uint64_t* x, uint64_t* out) {
+    uint64_t* acc = (uint64_t*)calloc((size_t)2 * N, sizeof(uint64_t));
+    memcpy(acc, x, sizeof(uint64_t) * N);
+    double e = orc_external_product64(N, ORC_HT_LOGB, ORC_HT_L, key_f, acc, out);
+    free(acc);
+    return e;
+}
+
+/* HomTrace of ct [2][N] in place, all log2 N rounds (htkey_f: orc_ksk64_to_fourier of
+ * the HT key) */
+double orc_homtrace64(const orc_cbs_params* p, const double* htkey_f, uint64_t* ct) {
+    int N = p->N, M = N / 2, R = orc_log2(N);
+    size_t per = (size_t)2 * ORC_HT_L * 2 * 4 * M;
+    uint64_t* sa = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    uint64_t* sb = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    uint64_t* ks = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N);
+    double maxerr = 0.0;
+    for (int k = 0; k < R; ++k) {
+        uint32_t e = (uint32_t)((N >> k) + 1);
+        orc_automorphism64(N, ct, e, sa);
+        orc_automorphism64(N, ct + N, e, sb);
+        double er = ks64(N, htkey_f + (size_t)k * per, sa, ks);
+        if (er > maxerr) maxerr = er;
+        for (int i = 0; i < N; ++i) {
+            ct[i] += ks[i];
+            ct[N + i] += sb[i] + ks[N + i];
+        }
+    }
+    free(sa); free(sb); free(ks);
+    return maxerr;
+}
+
+/* SchemeSwitch: out = (b + ks_0, ks_1), ks = sum_d dig_d(a) SS_d: an RLWE of -K2 m */
+double orc_schemeswitch64(const orc_cbs_params* p, const double* sskey_f, const uint64_t* ct, uint64_t* out) {
+    int N = p->N;
+    double e = ks64(N, sskey_f, ct, out);
+    for (int i = 0; i < N; ++i) out[i] += ct[N + i];
+    return e;
+}
+
+/* Circuit bootstrap through HomTrace + SchemeSwitch: the accumulator of the blind
+ * rotation of LUT[k] = -2^(47 + 8 (k mod 2)) / N (b + 2^30 switched with bw = 1), then
+ * for each CB level j: acc X^-j (coefficient j to 0) plus 2^(47 + 8j) / N at coefficient
+ * 0 (the level's value bit 2^(48 + 8j) / N there), HomTrace (times N: an RLWE of
+ * bit 2^(48 + 8j)), SchemeSwitch; rows rounded to the 32-bit torus (top 32 bits):
+ * row 2j = SS (RLWE of -K2 bit 2^(16 + 8j)), row 2j + 1 = HT (bit 2^(16 + 8j)), the
+ * reference's row order (kernel.cu:7776-7789) and the PFKS forms' row meaning. */
+double orc_cbs_htss(const orc_cbs_params* p, const double* bsk_f, const double* htkey_f, const double* sskey_f,
+                    const uint32_t* ct32, uint32_t* rgsw) {
+    int N = p->N, lg = orc_log2(N);
+    uint64_t* lut = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    uint64_t* acc = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N);
+    uint64_t* lv = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N);
+    uint64_t* ss = (uint64_t*)malloc(sizeof(uint64_t) * 2 * N);
+    uint32_t* in = (uint32_t*)malloc(sizeof(uint32_t) * (p->n + 1));
+    memcpy(in, ct32, sizeof(uint32_t) * p->n);
+    in[p->n] = ct32[p->n] + (1u << 30);
+    for (int k = 0; k < N; ++k)
+        lut[k] = 0 - ((uint64_t)1 << (64 - p->cb_l * p->cb_logB + (k & 1) * p->cb_logB - 1 - lg));
+    double err = orc_blind_rotate_lut64(p, bsk_f, 1, in, lut, 1, acc);
+    for (int j = 0; j < p->cb_l; ++j) {
+        for (int o = 0; o < 2; ++o)
+            for (int i = 0; i < N; ++i)  /* acc X^-j */
+                lv[o * N + i] = i + j < N ? acc[o * N + i + j] : 0 - acc[o * N + i + j - N];
+        lv[N] += (uint64_t)1 << (64 - p->cb_l * p->cb_logB + j * p->cb_logB - 1 - lg);
+        double e = orc_homtrace64(p, htkey_f, lv);
+        if (e > err) err = e;
+        e = orc_schemeswitch64(p, sskey_f, lv, ss);
+        if (e > err) err = e;
+        for (int i = 0; i < 2 * N; ++i) {
+            rgsw[(size_t)(2 * j) * 2 * N + i] = (uint32_t)((ss[i] + ((uint64_t)1 << 31)) >> 32);
+            rgsw[(size_t)(2 * j + 1) * 2 * N + i] = (uint32_t)((lv[i] + ((uint64_t)1 << 31)) >> 32);
+        }
+    }
+    free(lut); free(acc); free(lv); free(ss); free(in);
+    return err;
 }

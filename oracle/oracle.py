@@ -87,6 +87,12 @@ def lib():
             "orc_aes_sbox": (ctypes.c_uint8, [ctypes.c_uint8]),
             "orc_aes128_expand": (None, [_u8p, _u8p]),
             "orc_aes128_encrypt": (None, [_u8p, _u8p, _u8p]),
+            "orc_aes_inv_sbox": (ctypes.c_uint8, [ctypes.c_uint8]),
+            "orc_aes_ttable": (ctypes.c_uint32, [ctypes.c_int, ctypes.c_int, ctypes.c_uint8]),
+            "orc_aes128_decrypt": (None, [_u8p, _u8p, _u8p]),
+            "orc_aes128_dec_schedule": (None, [_u8p, _u8p]),
+            "orc_aes128_decrypt_ttable": (None, [_u8p, _u8p, _u8p]),
+            "orc_aes128_encrypt_ttable": (None, [_u8p, _u8p, _u8p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -189,6 +195,43 @@ def aes128_expand(key):
     return rk
 
 
+def aes_inv_sbox(x):
+    return int(lib().orc_aes_inv_sbox(x))
+
+
+def aes_ttable(dec, r, x):
+    """Te_r[x] (dec = 0) or Td_r[x] (dec = 1) as a big-endian 32-bit word."""
+    return int(lib().orc_aes_ttable(int(dec), r, x))
+
+
+def _u8(b):
+    return np.frombuffer(bytes(b), np.uint8).copy()
+
+
+def aes128_decrypt(key, block):
+    out = np.zeros(16, np.uint8)
+    lib().orc_aes128_decrypt(aes128_expand(key), _u8(block), out)
+    return bytes(out)
+
+
+def aes128_dec_schedule(key):
+    dk = np.zeros(176, np.uint8)
+    lib().orc_aes128_dec_schedule(aes128_expand(key), dk)
+    return dk
+
+
+def aes128_decrypt_ttable(key, block):
+    out = np.zeros(16, np.uint8)
+    lib().orc_aes128_decrypt_ttable(aes128_dec_schedule(key), _u8(block), out)
+    return bytes(out)
+
+
+def aes128_encrypt_ttable(key, block):
+    out = np.zeros(16, np.uint8)
+    lib().orc_aes128_encrypt_ttable(aes128_expand(key), _u8(block), out)
+    return bytes(out)
+
+
 def aes128_encrypt(key, block):
     rk = aes128_expand(key)
     out = np.zeros(16, np.uint8)
@@ -232,11 +275,21 @@ class CbsParams(ctypes.Structure):
                 ("rlwe_nshift", ctypes.c_int), ("pf_nshift", ctypes.c_int)]
 
 
-def cbs_params(n=636, N=2048):
+def cbs_params(n=636, N=2048, bk_logB=9, rlwe_nshift=19):
     """Circuit-bootstrapping level: N=2048, PBS 2^9 x 4 (torus-64, sigma 2^-51), PFKS 2^4 x 8
     (torus-32 key, sigma 2^-30), CB 2^8 x 2 (torus-32 RGSW), KS 2^2 x 8 to the level-0 key
     (sigma 2^-15)."""
-    return CbsParams(n, N, 9, 4, 8, 2, 4, 8, 2, 8, 15, 19, 30)
+    return CbsParams(n, N, bk_logB, 4, 8, 2, 4, 8, 2, 8, 15, rlwe_nshift, 30)
+
+
+def cbs_params_htss(n=636, N=2048):
+    """The HomTrace + SchemeSwitch form: PBS 2^10 x 4 with BSK2 sigma 2^-54 (PBS noise near
+    2^-32, which the trace multiplies by N at the constant coefficient), HT / SS gadget
+    2^8 x 5 with the same key noise."""
+    return cbs_params(n, N, bk_logB=10, rlwe_nshift=22)
+
+
+HT_LOGB, HT_L = 8, 5
 
 
 _cbs_sigs_done = False
@@ -265,6 +318,15 @@ def _cbs_lib():
             "orc_pbs_manylut_mb": (ctypes.c_double, [P, _f64p, ctypes.c_int, _u32p, _u64p]),
             "orc_bootstrap_many_lut": (ctypes.c_double, [P, _f64p, ctypes.c_int, _u32p, _u64p, ctypes.c_int,
                                                          _u64p]),
+            "orc_blind_rotate_lut64": (ctypes.c_double, [P, _f64p, ctypes.c_int, _u32p, _u64p, ctypes.c_int,
+                                                         _u64p]),
+            "orc_automorphism64": (None, [ctypes.c_int, _u64p, ctypes.c_uint32, _u64p]),
+            "orc_gen_ht_key": (None, [ctypes.c_uint64, P, _i32p, _u64p]),
+            "orc_gen_ss_key": (None, [ctypes.c_uint64, P, _i32p, _u64p]),
+            "orc_ksk64_to_fourier": (None, [P, ctypes.c_int, _u64p, _f64p]),
+            "orc_homtrace64": (ctypes.c_double, [P, _f64p, _u64p]),
+            "orc_schemeswitch64": (ctypes.c_double, [P, _f64p, _u64p, _u64p]),
+            "orc_cbs_htss": (ctypes.c_double, [P, _f64p, _f64p, _f64p, _u32p, _u32p]),
             "orc_mb2_indicator": (ctypes.c_uint32, [_i32p, ctypes.c_int, ctypes.c_int]),
             "orc_gen_bsk64_mb2_row": (None, [ctypes.c_uint64, P, _i32p, _i32p, ctypes.c_uint32, ctypes.c_int,
                                              _u64p]),
@@ -313,6 +375,14 @@ def rlwe_phase32(K, ct):
 def special_ms(v, N, bw):
     """round(v 2N / (2^32 2^bw)) 2^bw mod 2N (orc_special_ms)."""
     return int(_cbs_lib().orc_special_ms(int(v) & 0xFFFFFFFF, N, bw))
+
+
+def automorphism64(poly, k):
+    """poly(X^k) mod X^N + 1 (k odd), uint64 coefficients."""
+    poly = np.ascontiguousarray(poly, np.uint64)
+    out = np.zeros_like(poly)
+    _cbs_lib().orc_automorphism64(len(poly), poly, k, out)
+    return out
 
 
 def rlwe_phase64(K, ct):
@@ -399,6 +469,45 @@ class CbsKeys:
         err = _cbs_lib().orc_bootstrap_many_lut(ctypes.byref(p), bskf, mb, np.ascontiguousarray(ct32, np.uint32),
                                                 np.ascontiguousarray(lut, np.uint64), int(bw), out)
         return out.reshape(2, p.N), err
+
+    def _htss_keys(self):
+        if getattr(self, "_htk", None) is None:
+            p, L = self.p, _cbs_lib()
+            R = int(np.log2(p.N))
+            self.ht_key = np.zeros(R * HT_L * 2 * p.N, np.uint64)
+            L.orc_gen_ht_key(self.seed, ctypes.byref(p), self.K2, self.ht_key)
+            self.ss_key = np.zeros(HT_L * 2 * p.N, np.uint64)
+            L.orc_gen_ss_key(self.seed, ctypes.byref(p), self.K2, self.ss_key)
+            self._htk = np.zeros(R * 2 * HT_L * 2 * 2 * p.N, np.float64)
+            L.orc_ksk64_to_fourier(ctypes.byref(p), R, self.ht_key, self._htk)
+            self._ssk = np.zeros(2 * HT_L * 2 * 2 * p.N, np.float64)
+            L.orc_ksk64_to_fourier(ctypes.byref(p), 1, self.ss_key, self._ssk)
+        return self._htk, self._ssk
+
+    def homtrace(self, ct):
+        """HomTrace of an RLWE [2][N] (torus-64): all log2 N rounds; returns (ct', err)."""
+        htk, _ = self._htss_keys()
+        out = np.ascontiguousarray(ct, np.uint64).reshape(-1).copy()
+        err = _cbs_lib().orc_homtrace64(ctypes.byref(self.p), htk, out)
+        return out.reshape(2, self.p.N), err
+
+    def schemeswitch(self, ct):
+        """RLWE of -K2 m from an RLWE of m (torus-64)."""
+        _, ssk = self._htss_keys()
+        out = np.zeros(2 * self.p.N, np.uint64)
+        err = _cbs_lib().orc_schemeswitch64(ctypes.byref(self.p), ssk, np.ascontiguousarray(ct, np.uint64).reshape(-1),
+                                            out)
+        return out.reshape(2, self.p.N), err
+
+    def cbs_htss(self, ct32):
+        """Circuit bootstrap through HomTrace + SchemeSwitch: torus-32 RGSW [4][2][N] (row 2j SS,
+        2j + 1 HT)."""
+        htk, ssk = self._htss_keys()
+        p = self.p
+        rgsw = np.zeros(2 * p.cb_l * 2 * p.N, np.uint32)
+        err = _cbs_lib().orc_cbs_htss(ctypes.byref(p), self.bsk_f, htk, ssk, np.ascontiguousarray(ct32, np.uint32),
+                                      rgsw)
+        return rgsw, err
 
     def blind_rotate_mb2(self, a_tilde, acc):
         """Exact two-bit blind rotation of acc [2N] u64 (in place) by the rotations a_tilde [n]."""

@@ -633,3 +633,121 @@ void orc_aes128_encrypt(const uint8_t rk[176], const uint8_t in[16], uint8_t out
     }
     memcpy(out, s, 16);
 }
+
+/* ------------------------------------------------------------------------- */
+/* AES-128 decryption and the T-tables of the reference's AES data flow        */
+/* (aes128.cuh:227-276 BuildAesDecTables / BuildAesEncTables; the round-key     */
+/* words of the Td path, key_material.cuh:12-47, rk_imc_words).                 */
+/* ------------------------------------------------------------------------- */
+static uint8_t rotl8(uint8_t x, int k) { return (uint8_t)((x << k) | (x >> (8 - k))); }
+
+/* S^-1(y) = inverse in GF(2^8) of the inverse affine map of y: x = rotl(y', 1) ^
+ * rotl(y', 3) ^ rotl(y', 6) with y' = y ^ 0x63 (FIPS-197 5.3.2). */
+uint8_t orc_aes_inv_sbox(uint8_t y) {
+    uint8_t v = (uint8_t)(y ^ 0x63);
+    uint8_t x = (uint8_t)(rotl8(v, 1) ^ rotl8(v, 3) ^ rotl8(v, 6));
+    if (!x) return 0;
+    uint8_t r = 1, b = x;
+    int e = 254;
+    while (e) {
+        if (e & 1) r = gf_mul(r, b);
+        b = gf_mul(b, b);
+        e >>= 1;
+    }
+    return r;
+}
+
+static uint32_t be32(uint8_t b0, uint8_t b1, uint8_t b2, uint8_t b3) {
+    return ((uint32_t)b0 << 24) | ((uint32_t)b1 << 16) | ((uint32_t)b2 << 8) | b3;
+}
+static uint32_t rotr32(uint32_t x, int k) { return k ? (x >> k) | (x << (32 - k)) : x; }
+
+/* T-table word: dec = 0: Te_r[x] = rotr(be(2S, S, S, 3S), 8r); dec = 1:
+ * Td_r[x] = rotr(be(14 S^-1, 9 S^-1, 13 S^-1, 11 S^-1), 8r), S = S(x) / S^-1(x). */
+uint32_t orc_aes_ttable(int dec, int r, uint8_t x) {
+    uint32_t w;
+    if (dec) {
+        uint8_t s = orc_aes_inv_sbox(x);
+        w = be32(gf_mul(s, 14), gf_mul(s, 9), gf_mul(s, 13), gf_mul(s, 11));
+    } else {
+        uint8_t s = orc_aes_sbox(x);
+        w = be32(gf_mul(s, 2), s, s, gf_mul(s, 3));
+    }
+    return rotr32(w, 8 * r);
+}
+
+static void inv_mix_column(const uint8_t a[4], uint8_t o[4]) {
+    o[0] = (uint8_t)(gf_mul(a[0], 14) ^ gf_mul(a[1], 11) ^ gf_mul(a[2], 13) ^ gf_mul(a[3], 9));
+    o[1] = (uint8_t)(gf_mul(a[0], 9) ^ gf_mul(a[1], 14) ^ gf_mul(a[2], 11) ^ gf_mul(a[3], 13));
+    o[2] = (uint8_t)(gf_mul(a[0], 13) ^ gf_mul(a[1], 9) ^ gf_mul(a[2], 14) ^ gf_mul(a[3], 11));
+    o[3] = (uint8_t)(gf_mul(a[0], 11) ^ gf_mul(a[1], 13) ^ gf_mul(a[2], 9) ^ gf_mul(a[3], 14));
+}
+
+/* The straightforward inverse cipher (FIPS-197 5.3): AddRoundKey(10), then per round
+ * InvShiftRows, InvSubBytes, AddRoundKey, InvMixColumns (rounds 9..1), final round
+ * without InvMixColumns.  Byte index = row + 4 col. */
+void orc_aes128_decrypt(const uint8_t rk[176], const uint8_t in[16], uint8_t out[16]) {
+    uint8_t s[16], t[16];
+    for (int i = 0; i < 16; ++i) s[i] = (uint8_t)(in[i] ^ rk[160 + i]);
+    for (int round = 9; round >= 0; --round) {
+        for (int c = 0; c < 4; ++c)
+            for (int r = 0; r < 4; ++r) t[r + 4 * ((c + r) % 4)] = s[r + 4 * c];  /* InvShiftRows */
+        for (int i = 0; i < 16; ++i) s[i] = (uint8_t)(orc_aes_inv_sbox(t[i]) ^ rk[16 * round + i]);
+        if (round > 0)
+            for (int c = 0; c < 4; ++c) {
+                uint8_t o[4];
+                inv_mix_column(s + 4 * c, o);
+                memcpy(s + 4 * c, o, 4);
+            }
+    }
+    memcpy(out, s, 16);
+}
+
+/* Decryption key schedule of the equivalent inverse cipher (FIPS-197 5.3.5):
+ * dk[0] = rk[10], dk[r] = InvMixColumns(rk[10 - r]) for r = 1..9, dk[10] = rk[0]. */
+void orc_aes128_dec_schedule(const uint8_t rk[176], uint8_t dk[176]) {
+    memcpy(dk, rk + 160, 16);
+    for (int r = 1; r <= 9; ++r)
+        for (int c = 0; c < 4; ++c) inv_mix_column(rk + 16 * (10 - r) + 4 * c, dk + 16 * r + 4 * c);
+    memcpy(dk + 160, rk, 16);
+}
+
+/* The equivalent inverse cipher on Td tables, as the reference's GPU rounds evaluate
+ * it: s = in ^ dk[0]; rounds 1..9: column c of the new state = Td0[s(0, c)] ^
+ * Td1[s(1, c - 1)] ^ Td2[s(2, c - 2)] ^ Td3[s(3, c - 3)] ^ dk[r] column c (big-endian
+ * word: row 0 in the top byte); final round: S^-1, InvShiftRows, ^ dk[10]. */
+void orc_aes128_decrypt_ttable(const uint8_t dk[176], const uint8_t in[16], uint8_t out[16]) {
+    uint8_t s[16], t[16];
+    for (int i = 0; i < 16; ++i) s[i] = (uint8_t)(in[i] ^ dk[i]);
+    for (int round = 1; round <= 9; ++round) {
+        for (int c = 0; c < 4; ++c) {
+            uint32_t w = 0;
+            for (int r = 0; r < 4; ++r) w ^= orc_aes_ttable(1, r, s[r + 4 * ((c - r + 4) % 4)]);
+            for (int r = 0; r < 4; ++r) t[r + 4 * c] = (uint8_t)((w >> (24 - 8 * r)) ^ dk[16 * round + r + 4 * c]);
+        }
+        memcpy(s, t, 16);
+    }
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r)
+            t[r + 4 * c] = (uint8_t)(orc_aes_inv_sbox(s[r + 4 * ((c - r + 4) % 4)]) ^ dk[160 + r + 4 * c]);
+    memcpy(out, t, 16);
+}
+
+/* The cipher on Te tables (reference GPU rounds of the CTR path): column c = Te0[s(0, c)]
+ * ^ Te1[s(1, c + 1)] ^ Te2[s(2, c + 2)] ^ Te3[s(3, c + 3)] ^ rk[r] column c; final round
+ * S, ShiftRows, ^ rk[10]. */
+void orc_aes128_encrypt_ttable(const uint8_t rk[176], const uint8_t in[16], uint8_t out[16]) {
+    uint8_t s[16], t[16];
+    for (int i = 0; i < 16; ++i) s[i] = (uint8_t)(in[i] ^ rk[i]);
+    for (int round = 1; round <= 9; ++round) {
+        for (int c = 0; c < 4; ++c) {
+            uint32_t w = 0;
+            for (int r = 0; r < 4; ++r) w ^= orc_aes_ttable(0, r, s[r + 4 * ((c + r) % 4)]);
+            for (int r = 0; r < 4; ++r) t[r + 4 * c] = (uint8_t)((w >> (24 - 8 * r)) ^ rk[16 * round + r + 4 * c]);
+        }
+        memcpy(s, t, 16);
+    }
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) t[r + 4 * c] = (uint8_t)(orc_aes_sbox(s[r + 4 * ((c + r) % 4)]) ^ rk[160 + r + 4 * c]);
+    memcpy(out, t, 16);
+}

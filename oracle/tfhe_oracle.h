@@ -41,7 +41,8 @@ int32_t orc_secret_bit(uint64_t seed, uint32_t dom, uint32_t i);
 enum {
     ORC_DOM_LWE_SK = 1, ORC_DOM_RLWE1_SK = 2, ORC_DOM_RLWE2_SK = 3,
     ORC_DOM_BSK1 = 4, ORC_DOM_KSK1 = 5, ORC_DOM_BSK2 = 6, ORC_DOM_PFKS = 7,
-    ORC_DOM_KSK2 = 8, ORC_DOM_BSK2MB = 9, ORC_DOM_PFKL = 10, ORC_DOM_ENC = 16
+    ORC_DOM_KSK2 = 8, ORC_DOM_BSK2MB = 9, ORC_DOM_PFKL = 10, ORC_DOM_HTKEY = 11, ORC_DOM_SSKEY = 12,
+    ORC_DOM_ENC = 16
 };
 
 /* ---- parameter sets ---- */
@@ -134,6 +135,17 @@ double orc_pbs_manylut_mb(const orc_cbs_params* p, const double* bsk_f, int mb, 
 /* gpu_BootstrapManyLUT with test polynomial lut [N] and bitwidth bw: out RLWE [2][N] */
 double orc_bootstrap_many_lut(const orc_cbs_params* p, const double* bsk_f, int mb, const uint32_t* ct32,
                               const uint64_t* lut, int bw, uint64_t* out);
+double orc_blind_rotate_lut64(const orc_cbs_params* p, const double* bsk_f, int mb, const uint32_t* ct32,
+                              const uint64_t* lut, int bw, uint64_t* out);
+/* HomTrace + SchemeSwitch (gadget 2^8 x 5) */
+void orc_automorphism64(int N, const uint64_t* in, uint32_t k, uint64_t* out);
+void orc_gen_ht_key(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, uint64_t* key);
+void orc_gen_ss_key(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, uint64_t* key);
+void orc_ksk64_to_fourier(const orc_cbs_params* p, int nkeys, const uint64_t* key, double* key_f);
+double orc_homtrace64(const orc_cbs_params* p, const double* htkey_f, uint64_t* ct);
+double orc_schemeswitch64(const orc_cbs_params* p, const double* sskey_f, const uint64_t* ct, uint64_t* out);
+double orc_cbs_htss(const orc_cbs_params* p, const double* bsk_f, const double* htkey_f, const double* sskey_f,
+                    const uint32_t* ct32, uint32_t* rgsw);
 uint32_t orc_pfks_id(const orc_cbs_params* p, int u, int i, int t, int v);
 void orc_gen_pfks_entry(uint64_t seed, const orc_cbs_params* p, const int32_t* K2, int u, int i, int t, int v,
                         uint32_t* ct);
@@ -160,6 +172,12 @@ void orc_gen_ksk2(uint64_t seed, const orc_cbs_params* p, const int32_t* s, cons
 uint8_t orc_aes_sbox(uint8_t x);
 void orc_aes128_expand(const uint8_t key[16], uint8_t rk[176]);
 void orc_aes128_encrypt(const uint8_t rk[176], const uint8_t in[16], uint8_t out[16]);
+uint8_t orc_aes_inv_sbox(uint8_t y);
+uint32_t orc_aes_ttable(int dec, int r, uint8_t x);
+void orc_aes128_decrypt(const uint8_t rk[176], const uint8_t in[16], uint8_t out[16]);
+void orc_aes128_dec_schedule(const uint8_t rk[176], uint8_t dk[176]);
+void orc_aes128_decrypt_ttable(const uint8_t dk[176], const uint8_t in[16], uint8_t out[16]);
+void orc_aes128_encrypt_ttable(const uint8_t rk[176], const uint8_t in[16], uint8_t out[16]);
 
 #ifdef __cplusplus
 }

@@ -1156,3 +1156,86 @@ def test_key_file_round_trip(gpu, okeys, tmp_path):
     assert e.value.code == tfhe_amd.TFHE_E_SHAPE
     for c in (c1, c2, c3, c4):
         c.close()
+
+
+def _phase_margin_log2(okeys, got, blocks):
+    """log2 of the largest |phase - bit 2^31| over the output bits (torus-32)."""
+    s64 = okeys.s.astype(np.uint64)
+    ph = (got[:blocks, :, -1].astype(np.uint64) - (got[:blocks, :, :-1].astype(np.uint64) @ s64)) % 2 ** 32
+    err = ((ph + 2 ** 30) % 2 ** 31).astype(np.int64) - 2 ** 30
+    return float(np.log2(max(int(np.abs(err).max()), 1)))
+
+
+@pytest.mark.parametrize("form", [0, 1], ids=["sbox_rounds", "ttable_rounds"])
+def test_aes128_decrypt_fips197_and_random(lctx, okeys, form):
+    """Homomorphic AES-128 decryption (the reference's Td direction, tfhe_aes_decrypt_batch)
+    in both round forms: FIPS-197 C.1 and Appendix-B ciphertexts and three seeded random
+    blocks (5 blocks: a full-round group plus a tail group in this linear-PFKS context)
+    decrypt to the plaintexts of the oracle's inverse cipher, every phase inside the p = 2
+    margin (2^30), with the decryption round keys of the equivalent inverse cipher."""
+    import torch
+    import tfhe_amd
+    key = bytes.fromhex("2b7e151628aed2a6abf7158809cf4f3c")
+    lctx.set_dec_round_keys(tfhe_amd.aes_encrypt_dec_key_bits(okeys.s, 91, key))
+    lctx.set_round_form(form)
+    try:
+        rng = np.random.default_rng(12)
+        cts = [O.aes128_encrypt(key, bytes.fromhex("00112233445566778899aabbccddeeff")),
+               bytes.fromhex("3925841d02dc09fbdc118597196a0b32")] + \
+              [bytes(rng.integers(0, 256, 16, dtype=np.uint8)) for _ in range(3)]
+        d_out = torch.empty((5, 128, okeys.p.n + 1), dtype=torch.int32, device="cuda")
+        lctx.aes_decrypt(cts, d_out)
+        torch.cuda.synchronize()
+        got = from_dev(d_out)
+        dec = _decrypt_blocks(okeys, got, 5)
+        assert dec[1].hex() == "3243f6a8885a308d313198a2e0370734"
+        assert dec == [O.aes128_decrypt(key, c) for c in cts]
+        m = _phase_margin_log2(okeys, got, 5)
+        print(f"AES decryption, round form {form}: max |phase error| 2^{m:.2f} of 2^32")
+        assert m < 30
+    finally:
+        lctx.set_round_form(0)
+
+
+def test_aes128_ttable_keystream_and_rounds(lctx, okeys):
+    """T-table rounds (tfhe_aes_set_round_form(1), the reference's data flow) in the
+    encryption direction: the CTR keystream of FIPS-197 C.1 and a random block decrypts
+    to plain AES, eager and graph-replayed (equal words); one decryption round and one
+    encryption round in each form decrypt to the same state bits."""
+    import torch
+    import tfhe_amd
+    key = bytes(range(16))
+    lctx.set_round_keys(tfhe_amd.aes_encrypt_key_bits(okeys.s, 77, key))
+    lctx.set_dec_round_keys(tfhe_amd.aes_encrypt_dec_key_bits(okeys.s, 78, key))
+    blocks = [bytes.fromhex("00112233445566778899aabbccddeeff"),
+              bytes(np.random.default_rng(6).integers(0, 256, 16, dtype=np.uint8))]
+    lctx.set_round_form(1)
+    try:
+        d_ks = torch.empty((2, 128, okeys.p.n + 1), dtype=torch.int32, device="cuda")
+        lctx.aes_ctr_keystream(blocks, d_ks)
+        torch.cuda.synchronize()
+        got = from_dev(d_ks)
+        dec = _decrypt_blocks(okeys, got, 2)
+        assert dec[0].hex() == "69c4e0d86a7b0430d8cdb78070b4c55a"
+        assert dec[1] == O.aes128_encrypt(key, blocks[1])
+        assert _phase_margin_log2(okeys, got, 2) < 30
+        d_g = torch.empty_like(d_ks)
+        lctx.aes_ctr_keystream(blocks, d_g, graph=True)
+        torch.cuda.synchronize()
+        assert np.array_equal(from_dev(d_g), got)
+        # single rounds from a fresh state of random bits, both forms
+        rng = np.random.default_rng(7)
+        state = np.stack([okeys.encrypt_bit(int(b), 1200 + k) for k, b in enumerate(rng.integers(0, 2, 128))])
+        d_in = to_dev32(state[None])
+        outs = {}
+        for form in (1, 0):
+            lctx.set_round_form(form)
+            for name, fn in (("enc", lctx.aes_round), ("dec", lctx.aes_dec_round)):
+                d_o = torch.empty((1, 128, okeys.p.n + 1), dtype=torch.int32, device="cuda")
+                fn(3, d_in, 1, d_o)
+                torch.cuda.synchronize()
+                outs[(name, form)] = _decrypt_blocks(okeys, from_dev(d_o), 1)[0]
+        assert outs[("enc", 0)] == outs[("enc", 1)]
+        assert outs[("dec", 0)] == outs[("dec", 1)]
+    finally:
+        lctx.set_round_form(0)

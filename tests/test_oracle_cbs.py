@@ -446,3 +446,124 @@ def test_many_lut_with_the_cb_polynomial_gives_the_levels(keys, bit):
     for j in range(2):
         e = centered64(int(ph[j]) - bit * (1 << (48 + 8 * j)))
         assert abs(e) < 2 ** 40
+
+
+# ---------------------------------------------------------------- HomTrace + SchemeSwitch
+@pytest.fixture(scope="module")
+def hkeys():
+    """Keys of the HomTrace + SchemeSwitch form: PBS 2^10 x 4, BSK2 / HT / SS sigma 2^-54."""
+    return O.CbsKeys(5, p=O.cbs_params_htss())
+
+
+def _centered64(v):
+    return np.asarray(v, np.uint64).view(np.int64)
+
+
+def test_automorphism_is_a_ring_map():
+    """sigma_k(x y) = sigma_k(x) sigma_k(y) in Z/2^64[X]/(X^N + 1) (checked against the
+    exact schoolbook product), sigma_k sigma_k' = sigma_(k k' mod 2N), sigma_1 = id."""
+    N = 2048
+    rng = np.random.default_rng(4)
+    x = rng.integers(-8, 8, N).astype(np.int64).astype(np.uint64)
+    y = rng.integers(0, 2 ** 64, N, dtype=np.uint64)
+    xy = O.nc_mul_exact64(x.view(np.int64).astype(np.int32), y)
+    for k in (3, 5, 1025, 2049, 4095):
+        sx, sy = O.automorphism64(x, k), O.automorphism64(y, k)
+        assert np.array_equal(O.automorphism64(xy, k), O.nc_mul_exact64(sx.view(np.int64).astype(np.int32), sy))
+    assert np.array_equal(O.automorphism64(O.automorphism64(y, 3), 5), O.automorphism64(y, 15))
+    assert np.array_equal(O.automorphism64(y, 1), y)
+
+
+def test_homtrace_of_a_trivial_ciphertext_is_the_trace(hkeys):
+    """With a zero mask every key switch of the trace adds exactly zero, so HomTrace of
+    (0, m) is exactly (0, N m_0 at coefficient 0): the trace identity, which pins the
+    automorphism sequence X -> X^(N/2^k + 1), k = 0 .. 10."""
+    N = hkeys.p.N
+    m = np.random.default_rng(8).integers(0, 2 ** 64, N, dtype=np.uint64)
+    out, _ = hkeys.homtrace(np.concatenate([np.zeros(N, np.uint64), m]))
+    want = np.zeros(N, np.uint64)
+    want[0] = np.uint64((int(m[0]) * N) % 2 ** 64)
+    assert not out[0].any() and np.array_equal(out[1], want)
+
+
+def test_ht_and_ss_key_rows_decrypt(hkeys):
+    """HT key entry (k, d) decrypts to -sigma_k(K2) 2^(24 + 8d), SS entry d to K2^2 2^(24 + 8d),
+    within the key noise (Irwin-Hall, sigma 2^10 of 2^64, bounded at 6 sigma)."""
+    N = hkeys.p.N
+    hkeys._htss_keys()
+    K64 = hkeys.K2.astype(np.int64).astype(np.uint64)
+    sq = O.nc_mul_exact64(hkeys.K2, K64)
+    for k, d in [(0, 0), (3, 2), (10, 4)]:
+        ent = hkeys.ht_key[((k * O.HT_L + d) * 2) * N:((k * O.HT_L + d) * 2 + 2) * N]
+        want = (0 - O.automorphism64(K64, (N >> k) + 1) * np.uint64(1 << (24 + 8 * d))).astype(np.uint64)
+        assert np.abs(_centered64(O.rlwe_phase64(hkeys.K2, ent) - want)).max() <= 6 * 2 ** 10
+    for d in range(O.HT_L):
+        ent = hkeys.ss_key[d * 2 * N:(d + 1) * 2 * N]
+        want = (sq * np.uint64(1 << (24 + 8 * d))).astype(np.uint64)
+        assert np.abs(_centered64(O.rlwe_phase64(hkeys.K2, ent) - want)).max() <= 6 * 2 ** 10
+
+
+def _rlwe_encrypt64(keys, m, seed):
+    rng = np.random.default_rng(seed)
+    N = keys.p.N
+    a = rng.integers(0, 2 ** 64, N, dtype=np.uint64)
+    e = rng.integers(-2 ** 10, 2 ** 10, N).astype(np.int64).astype(np.uint64)
+    return np.concatenate([a, (O.nc_mul_exact64(keys.K2, a) + m + e).astype(np.uint64)])
+
+
+def test_homtrace_and_schemeswitch_decrypt(hkeys):
+    """HomTrace of an RLWE of m decrypts to N m_0 at coefficient 0 and 0 elsewhere;
+    SchemeSwitch of an RLWE of m decrypts to -K2 m; both within 2^40 of 2^64 (the
+    trace doubles the input noise of coefficient 0 per round, 2^10 x 2^11, and adds the
+    key switches' rounding, 2^-41 of the torus through K2)."""
+    N = hkeys.p.N
+    rng = np.random.default_rng(12)
+    m = rng.integers(0, 2 ** 64, N, dtype=np.uint64)
+    ht, err = hkeys.homtrace(_rlwe_encrypt64(hkeys, m, 1))
+    assert err < 0.25
+    want = np.zeros(N, np.uint64)
+    want[0] = np.uint64((int(m[0]) * N) % 2 ** 64)
+    e = _centered64(O.rlwe_phase64(hkeys.K2, ht.reshape(-1)) - want)
+    assert np.abs(e).max() < 2 ** 40
+    ss, err = hkeys.schemeswitch(_rlwe_encrypt64(hkeys, m, 2))
+    assert err < 0.25
+    want = (0 - O.nc_mul_exact64(hkeys.K2, m)).astype(np.uint64)
+    e = _centered64(O.rlwe_phase64(hkeys.K2, ss.reshape(-1)) - want)
+    assert np.abs(e).max() < 2 ** 40
+
+
+@pytest.mark.parametrize("bit", [0, 1])
+def test_cbs_htss_rows_and_cmux(hkeys, bit):
+    """The circuit bootstrap through HomTrace + SchemeSwitch: row 2j + 1 (HT) decrypts to
+    bit 2^(16 + 8j) (constant), row 2j (SS) to -K2 bit 2^(16 + 8j), on the 32-bit torus,
+    within 2^14 at coefficient 0 and 2^10 elsewhere (HT), 2^15 (SS); the RGSW then selects
+    in a CMux: the SS rows' noise (about 2^-21 per coefficient, the HT noise times K2)
+    meets digits of up to 2^7 over 2 rows x N coefficients, about 2^23.2 of 2^32 rms per
+    CMux; asserted below 2^26 (the decision margin is 2^30)."""
+    p = hkeys.p
+    N = p.N
+    ct = hkeys.encrypt_bit(bit, 40 + bit)
+    rgsw, err = hkeys.cbs_htss(ct)
+    assert err < 0.25
+    K = hkeys.K2.astype(np.int64)
+    for j in range(p.cb_l):
+        g = bit << (16 + 8 * j)
+        ht = O.rlwe_phase32(hkeys.K2, rgsw[(2 * j + 1) * 2 * N:(2 * j + 2) * 2 * N]).view(np.int32).astype(np.int64)
+        want = np.zeros(N, np.int64)
+        want[0] = g
+        e = ht - want
+        assert abs(e[0]) < 2 ** 14 and np.abs(e[1:]).max() < 2 ** 10
+        ss = O.rlwe_phase32(hkeys.K2, rgsw[2 * j * 2 * N:(2 * j + 1) * 2 * N]).view(np.int32).astype(np.int64)
+        e = ((ss + K * g + 2 ** 31) % 2 ** 32) - 2 ** 31
+        assert np.abs(e).max() < 2 ** 15
+    C = hkeys.rgsw_fourier(rgsw)
+    rng = np.random.default_rng(60 + bit)
+    msgs = [(rng.integers(0, 2, N).astype(np.uint32) << np.uint32(31)).astype(np.uint32) for _ in range(2)]
+    ds = []
+    for msg in msgs:
+        a = rng.integers(0, 2 ** 32, N, dtype=np.uint64).astype(np.uint32)
+        ds.append(np.concatenate([a, (O.nc_mul_exact32(hkeys.K2, a) + msg).astype(np.uint32)]))
+    out = np.zeros(2 * N, np.uint32)
+    assert O._cbs_lib().orc_cmux32(N, p.cb_logB, p.cb_l, C, ds[0], ds[1], out) < 0.25
+    e = ((O.rlwe_phase32(hkeys.K2, out) - msgs[bit]).astype(np.uint32)).view(np.int32)
+    assert np.abs(e).max() < 2 ** 26

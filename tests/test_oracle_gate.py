@@ -53,6 +53,52 @@ def test_aes_sbox_properties():
     assert all(S[x] != x and S[x] != x ^ 0xFF for x in range(256))  # no fixed / opposite fixed points
 
 
+def test_aes_inverse_sbox_and_decryption_vectors():
+    """S^-1 inverts S everywhere (FIPS-197 Fig. 14 values at 0x00, 0x63, 0xff); the
+    inverse cipher (FIPS-197 5.3) and the Td-table equivalent inverse cipher (5.3.5, the
+    reference's decryption data flow) both decrypt Appendix C.1 and B, and random blocks,
+    to the plaintext; the Te-table cipher equals the plain cipher."""
+    assert all(O.aes_inv_sbox(O.aes_sbox(x)) == x for x in range(256))
+    assert O.aes_inv_sbox(0x00) == 0x52 and O.aes_inv_sbox(0x63) == 0x00 and O.aes_inv_sbox(0xFF) == 0x7D
+    k1, c1, p1 = bytes(range(16)), "69c4e0d86a7b0430d8cdb78070b4c55a", "00112233445566778899aabbccddeeff"
+    k2 = bytes.fromhex("2b7e151628aed2a6abf7158809cf4f3c")
+    c2, p2 = "3925841d02dc09fbdc118597196a0b32", "3243f6a8885a308d313198a2e0370734"
+    for k, c, p in ((k1, c1, p1), (k2, c2, p2)):
+        assert O.aes128_decrypt(k, bytes.fromhex(c)).hex() == p
+        assert O.aes128_decrypt_ttable(k, bytes.fromhex(c)).hex() == p
+        assert O.aes128_encrypt_ttable(k, bytes.fromhex(p)).hex() == c
+    rng = random.Random(17)
+    for _ in range(20):
+        k = bytes(rng.randrange(256) for _ in range(16))
+        m = bytes(rng.randrange(256) for _ in range(16))
+        ct = O.aes128_encrypt(k, m)
+        assert O.aes128_encrypt_ttable(k, m) == ct
+        assert O.aes128_decrypt(k, ct) == m and O.aes128_decrypt_ttable(k, ct) == m
+
+
+def test_aes_ttables():
+    """Te0[x] = (2 S, S, S, 3 S) and Td0[x] = (14 S^-1, 9 S^-1, 13 S^-1, 11 S^-1) big-endian
+    (the identities of the reference's aes128.cuh:227-276), Te_r / Td_r = Te0 / Td0 rotated
+    right by 8 r bits; the published first entries Te0[0] = c66363a5, Td0[0] = 51f4a750."""
+    assert O.aes_ttable(0, 0, 0) == 0xC66363A5 and O.aes_ttable(1, 0, 0) == 0x51F4A750
+
+    def gm(a, b):
+        r = 0
+        while b:
+            if b & 1:
+                r ^= a
+            a = ((a << 1) ^ (0x1B if a & 0x80 else 0)) & 0xFF
+            b >>= 1
+        return r
+    for x in range(256):
+        s, si = O.aes_sbox(x), O.aes_inv_sbox(x)
+        te0 = (gm(s, 2) << 24) | (s << 16) | (s << 8) | gm(s, 3)
+        td0 = (gm(si, 14) << 24) | (gm(si, 9) << 16) | (gm(si, 13) << 8) | gm(si, 11)
+        for r in range(4):
+            rot = lambda w: ((w >> (8 * r)) | (w << (32 - 8 * r))) & 0xFFFFFFFF if r else w
+            assert O.aes_ttable(0, r, x) == rot(te0) and O.aes_ttable(1, r, x) == rot(td0)
+
+
 # ---------------------------------------------------------------- decomposition
 @pytest.mark.parametrize("logB,l", [(7, 3), (8, 4), (8, 2), (2, 8)])
 def test_decompose32_reconstruction(logB, l):
