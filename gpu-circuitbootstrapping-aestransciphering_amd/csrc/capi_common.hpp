@@ -150,4 +150,12 @@ int per_device_tables(Tables*& out, Init init) {
     return TFHE_OK;
 }
 
+// Batched gate bootstrap with the scratch passed in: lweN holds [batch][N+1] u32 (blind-rotation
+// outputs), ks_at [N][ks_bpad(batch)] u32 (key-switch digits); cap is the largest batch they hold.
+int gate_eval_with_scratch(tfhe_gate_ctx* c, int gate, const uint32_t* d_ct1, const uint32_t* d_ct2, uint32_t batch,
+                           uint32_t cap, uint32_t* d_out, uint32_t* lweN, uint32_t* ks_at, void* stream);
+
+// Looks up the no-allocation entry point's device array on the current GPU (once per device).
+int noalloc_prepare_current_device();
+
 }  // namespace tfhe

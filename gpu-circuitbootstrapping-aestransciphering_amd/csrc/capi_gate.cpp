@@ -78,6 +78,7 @@ extern "C" int tfhe_gate_ctx_create(const tfhe_gate_params* params, uint32_t max
     rc = rc ? rc : dmalloc((void**)&c->d_digf, (size_t)2 * p.bk_l * 512 * sizeof(cplx));
     rc = rc ? rc : dmalloc((void**)&c->d_t, 2 * (size_t)p.N * sizeof(uint32_t));
     rc = rc ? rc : c->tables.init(p.N);
+    rc = rc ? rc : noalloc_prepare_current_device();
     if (rc) {
         tfhe_gate_ctx_destroy(c);
         return rc;
@@ -166,14 +167,18 @@ extern "C" int tfhe_gate_get_rlwe_sk(const tfhe_gate_ctx* c, int32_t* out) {
     return TFHE_OK;
 }
 
-static int check_gate_call(tfhe_gate_ctx* c, int gate, const uint32_t* a, const uint32_t* b, uint32_t batch) {
+static int check_gate_call(tfhe_gate_ctx* c, int gate, const uint32_t* a, const uint32_t* b, uint32_t batch,
+                           uint32_t cap) {
     if (!c) return fail(TFHE_E_PARAM, "gate call: ctx is NULL");
     if (!c->has_keys) return fail(TFHE_E_STATE, "gate call: no bootstrapping keys (call tfhe_gate_bt_keygen)");
     if (gate_const(gate) < 0) return fail(TFHE_E_PARAM, "gate call: unsupported gate (AND, OR, NAND, NOR, XOR, XNOR)");
     if (!a || !b) return fail(TFHE_E_PARAM, "gate call: NULL ciphertext");
     if (a == b) return fail(TFHE_E_PARAM, "Input ciphertexts should be independent");
-    if (batch > c->max_batch) return fail(TFHE_E_CAPACITY, "gate call: batch exceeds the context's max_batch");
+    if (batch > cap) return fail(TFHE_E_CAPACITY, "gate call: batch exceeds the context's max_batch");
     return TFHE_OK;
+}
+static int check_gate_call(tfhe_gate_ctx* c, int gate, const uint32_t* a, const uint32_t* b, uint32_t batch) {
+    return check_gate_call(c, gate, a, b, batch, c ? c->max_batch : 0);
 }
 
 extern "C" int tfhe_gate_blind_rotate_batch(tfhe_gate_ctx* c, int gate, const uint32_t* d_ct1, const uint32_t* d_ct2,
@@ -200,15 +205,30 @@ extern "C" int tfhe_gate_keyswitch_batch(tfhe_gate_ctx* c, const uint32_t* d_lwe
     return check_launch("keyswitch32");
 }
 
-extern "C" int tfhe_gate_eval_batch(tfhe_gate_ctx* c, int gate, const uint32_t* d_ct1, const uint32_t* d_ct2,
-                                    uint32_t batch, uint32_t* d_out, void* stream) {
-    int rc = check_gate_call(c, gate, d_ct1, d_ct2, batch);
+namespace tfhe {
+
+int gate_eval_with_scratch(tfhe_gate_ctx* c, int gate, const uint32_t* d_ct1, const uint32_t* d_ct2, uint32_t batch,
+                           uint32_t cap, uint32_t* d_out, uint32_t* lweN, uint32_t* ks_at, void* stream) {
+    int rc = check_gate_call(c, gate, d_ct1, d_ct2, batch, cap);
     if (rc) return rc;
     if (!d_out) return fail(TFHE_E_PARAM, "eval_batch: NULL output");
     if (batch == 0) return TFHE_OK;
-    rc = tfhe_gate_blind_rotate_batch(c, gate, d_ct1, d_ct2, batch, c->d_lweN, stream);
+    const hipStream_t st = (hipStream_t)stream;
+    const uint32_t q1 = (uint32_t)gate_const(gate) * (2u * c->p.N / 8u);
+    launch_gate_blind_rotate(d_ct1, d_ct2, batch, c->p.n, gate_doubled(gate), q1, c->d_bskf, c->tables.T, lweN, st);
+    rc = check_launch("gate_blind_rotate");
     if (rc) return rc;
-    return tfhe_gate_keyswitch_batch(c, c->d_lweN, batch, d_out, stream);
+    if (launch_keyswitch32(lweN, batch, c->p.N, c->p.n, c->p.ks_logB, c->p.ks_l, c->d_ksk, ks_at, d_out, st))
+        return fail(TFHE_E_PARAM, "keyswitch: unsupported shape");
+    return check_launch("keyswitch32");
+}
+
+}  // namespace tfhe
+
+extern "C" int tfhe_gate_eval_batch(tfhe_gate_ctx* c, int gate, const uint32_t* d_ct1, const uint32_t* d_ct2,
+                                    uint32_t batch, uint32_t* d_out, void* stream) {
+    if (!c) return fail(TFHE_E_PARAM, "gate call: ctx is NULL");
+    return gate_eval_with_scratch(c, gate, d_ct1, d_ct2, batch, c->max_batch, d_out, c->d_lweN, c->d_ks_at, stream);
 }
 
 static uint32_t host_modswitch_2N(uint32_t v, int N) {

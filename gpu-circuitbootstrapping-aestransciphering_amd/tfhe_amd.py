@@ -83,6 +83,7 @@ SIGNATURES = {
     "tfhe_gate_set_keys": (ctypes.c_int, [_vp, _u32p, _u32p]),
     "tfhe_gate_get_rlwe_sk": (ctypes.c_int, [_vp, _i32p]),
     "tfhe_gate_eval_batch": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_uint32, _vp, _vp]),
+    "tfhe_gate_eval_batch_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_gate_eval_plain": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]),
     "tfhe_gate_blind_rotate_batch": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_gate_keyswitch_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
@@ -285,6 +286,16 @@ class GateContext:
         for t, w in ((d_ct1, "ct1"), (d_ct2, "ct2"), (d_out, "out")):
             _need(t, b * row, f"eval_batch {w}")
         check(lib().tfhe_gate_eval_batch(self.h, gate, _ptr(d_ct1), _ptr(d_ct2), b, _ptr(d_out), _stream(stream)))
+
+    def eval_batch_noalloc(self, gate, d_ct1, d_ct2, d_out, batch=None, stream=None):
+        """tfhe_gate_eval_batch_noalloc: the same words as eval_batch, with the library's fixed
+        device scratch (at most NOALLOC_MAX_GATES gates; the call allocates nothing)."""
+        b = d_ct1.shape[0] if batch is None else batch
+        row = 4 * (self.p.n + 1)
+        for t, w in ((d_ct1, "ct1"), (d_ct2, "ct2"), (d_out, "out")):
+            _need(t, b * row, f"eval_batch_noalloc {w}")
+        check(lib().tfhe_gate_eval_batch_noalloc(self.h, gate, _ptr(d_ct1), _ptr(d_ct2), b, _ptr(d_out),
+                                                 _stream(stream)))
 
     def eval_plain(self, gate, d_ct1, d_ct2, d_out, stream=None):
         row = 4 * (self.p.n + 1)
@@ -572,4 +583,5 @@ class CbsContext:
 
 
 AES_MODES = {"batch": 0, "graph": 1, "plain": 2}
+NOALLOC_MAX_GATES = 1024  # TFHE_NOALLOC_MAX_GATES
 

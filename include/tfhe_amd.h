@@ -86,6 +86,19 @@ int tfhe_gate_get_rlwe_sk(const tfhe_gate_ctx* ctx, int32_t* out);
 int tfhe_gate_eval_batch(tfhe_gate_ctx* ctx, int gate, const uint32_t* d_ct1, const uint32_t* d_ct2,
                          uint32_t batch, uint32_t* d_out, void* stream);
 
+/* tfhe_gate_eval_batch that allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and the same output words as
+ * tfhe_gate_eval_batch; its scratch (the blind rotation's [batch][N+1] outputs and the key
+ * switch's [N][batch] digits, 8.0 MiB) is a fixed device array of the library, one per GPU,
+ * held until the process ends, instead of the context's buffers.  batch is capped by
+ * TFHE_NOALLOC_MAX_GATES, not by the context's max_batch (TFHE_E_CAPACITY beyond it).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one GPU (two streams
+ * or two host threads) race and must be serialised by the caller. */
+#define TFHE_NOALLOC_MAX_GATES 1024u
+int tfhe_gate_eval_batch_noalloc(tfhe_gate_ctx* ctx, int gate, const uint32_t* d_ct1, const uint32_t* d_ct2,
+                                 uint32_t batch, uint32_t* d_out, void* stream);
+
 /* GPUEvalBinGate(gate, ct1, ct2) (binfhecontext.cuh:48-51, binfhe-base-scheme.cu:151-252)
  * in the reference's plain per-step form: one ciphertext, three kernel launches
  * per blind-rotation step.  Synchronises the stream (the rotation indices are
