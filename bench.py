@@ -45,9 +45,16 @@ PEAK_INT8_TOPS = 5000.0  # dense int8 matrix-core peak
 N_CB, M_CB, CB_ROWS = 2048, 1024, 8
 F_EP_CB = {1: (CB_ROWS + 2) * 5 * M_CB * 10 + 8 * 2 * CB_ROWS * M_CB,      # one product: 643,072
            2: (CB_ROWS + 4) * 5 * M_CB * 10 + 8 * 2 * 2 * CB_ROWS * M_CB}  # two limbs: 876,544
-# the PBS's algorithmic FLOPs per circuit bootstrap, priced at the FP64 one-product count for
-# every form (the two-key-bit form does more arithmetic per key bit; it is not credited)
+# the PBS's algorithmic FLOPs per circuit bootstrap: one FFT external product per key bit at the
+# one-product count, for every form.  The two-key-bit form executes less by the same convention
+# (10 FFTs + 3 pattern MACs + 196,608 FLOP of pattern factors per key-bit pair = 550,912 per key
+# bit, 0.857 of this count), so its rate by executed arithmetic is 0.857 x the one reported.
 F_PBS_CB = N_LWE * F_EP_CB[1]  # 409.0 MFLOP
+F_MB2_PER_BIT = (CB_ROWS + 2) * 5 * M_CB * 10 + 3 * 8 * 2 * CB_ROWS * M_CB + 196608  # 1,101,824 per pair
+# HomTrace + SchemeSwitch per circuit bootstrap: 2 CB levels x (11 trace rounds + 1 scheme switch)
+# key switches, each 5 forward + 4 inverse FFT1024s and a 5-digit, 2-output, 2-limb MAC
+F_HTSS_KS = 9 * 5 * M_CB * 10 + 8 * 5 * 2 * 2 * M_CB  # 624,640
+F_HTSS_CB = 2 * 12 * F_HTSS_KS  # 14.99 MFLOP
 
 
 def cpu_baseline(sample=24, threads=None):
@@ -200,7 +207,8 @@ def aes_kernel_rooflines(ctx, pbs_approx, nb, pfks_linear=0):
     return out
 
 
-def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0, tune=True, pfks_linear=0):
+def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0, tune=True, pfks_linear=0,
+                  lwe_to_rgsw=0):
     """AES-128 CTR transciphering: the keystream (circuit bootstrap + S-box LUT per
     round, linear layers as LWE additions) plus the public AES-CTR ciphertext bits, so
     the outputs encrypt the plaintext bits; every block decrypted and compared with the
@@ -211,7 +219,8 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0,
     import torch
     import tfhe_amd
     maxb = max(batches)
-    ctx = tfhe_amd.CbsContext(max_blocks=maxb, params=tfhe_amd.cbs_params(pbs_approx, pbs_multibit, pfks_linear))
+    ctx = tfhe_amd.CbsContext(max_blocks=maxb, params=tfhe_amd.cbs_params(pbs_approx, pbs_multibit, pfks_linear,
+                                                                          lwe_to_rgsw))
     sk = tfhe_amd.lwe_keygen(N_LWE, 3)
     t0 = time.perf_counter()
     ctx.keygen(sk, 3)
@@ -259,7 +268,7 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0,
         if tuned is not None:
             out[str(blocks)]["tuned_shapes"] = tuned
         print(f"aes128 ctr transcipher ({'one-product' if pbs_approx else 'exact'} PBS, {pbs_multibit} key bit(s) "
-              f"per step): "
+              f"per step{', HT+SS' if lwe_to_rgsw else ''}): "
               f"{blocks} blocks in {dt:.3f} s, "
               f"{int(bad)} wrong", file=sys.stderr, flush=True)
         if blocks <= 32:  # the same transcipher with the keystream replayed from its captured hipGraph
@@ -272,7 +281,9 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0,
             gdt = time.perf_counter() - t0
             out[str(blocks)]["graph_s_per_launch"] = round(gdt, 4)
             out[str(blocks)]["graph_equals_eager"] = bool(torch.equal(d_ks[:blocks], eager))
-    if pbs_multibit in (1, 3):  # the round's two big kernels on their own, at 17 blocks' shape (or less)
+    if lwe_to_rgsw:  # PBS and HT+SS of one round's circuit bootstraps, at 17 blocks' shape (or less)
+        out["kernels"] = htss_kernel_rooflines(ctx, min(17, maxb) * 128)
+    elif pbs_multibit in (1, 3):  # the round's two big kernels on their own, at 17 blocks' shape (or less)
         out["kernels"] = aes_kernel_rooflines(ctx, pbs_approx, min(17, maxb) * 128, pfks_linear)
     if plain_blocks:  # the per-bootstrap form: every bootstrap, LUT and key switch a launch of one
         ctrs, plain, cipher = workload(plain_blocks)
@@ -289,13 +300,9 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0,
         print(f"aes128 ctr plain per-bootstrap form: {plain_blocks} block(s) in {dt:.2f} s", file=sys.stderr,
               flush=True)
     ctx.close()
-    pbs = ("one FP64 product per digit polynomial, rounding error ~2^-33" if pbs_approx
-           else "2-limb exact") + (", two key bits per blind-rotation step" if pbs_multibit == 2 else
-                                   ", two key bits per step unless the autotuner measured one bit faster" if pbs_multibit == 3 else "")
-    pf = ("linear 40-bit key as an int8 GEMM on the matrix cores" if pfks_linear
-          else "lookup key, LDS selects")
-    return {"form": f"fp64 FFT: PBS on the 64-bit torus ({pbs}), PFKS ({pf}) and CMux LUT on the 32-bit torus "
-                    "(1-limb exact)", "keygen_s": round(keygen_s, 2), "batches": out}
+    pbs, conv = form_text(pbs_approx, pbs_multibit, pfks_linear, lwe_to_rgsw)
+    return {"form": f"fp64 FFT: PBS on the 64-bit torus ({pbs}), LWE->RGSW by {conv}, CMux LUT on the 32-bit "
+                    "torus (1-limb exact)", "keygen_s": round(keygen_s, 2), "batches": out}
 
 
 def aes_cpu_baseline(threads=None, cbs_sample=1):
@@ -477,12 +484,60 @@ def gate_bench(args):
     return out
 
 
-AES_FORMS = {"exact": ("aes128_ctr", 0, 1, 0), "approx": ("aes128_ctr_one_product_pbs", 1, 1, 0),
-             "approx-mb2": ("aes128_ctr_one_product_pbs_2bit", 1, 2, 0),
-             "exact-lin": ("aes128_ctr_linear_pfks", 0, 1, 1),
-             "approx-lin": ("aes128_ctr_one_product_pbs_linear_pfks", 1, 1, 1),
-             "approx-mb2-lin": ("aes128_ctr_one_product_pbs_2bit_linear_pfks", 1, 2, 1),
-             "approx-auto-lin": ("aes128_ctr_one_product_pbs_auto_bits_linear_pfks", 1, 3, 1)}
+# form -> (bench key, pbs_approx, pbs_multibit, pfks_linear, lwe_to_rgsw)
+AES_FORMS = {"exact": ("aes128_ctr", 0, 1, 0, 0), "approx": ("aes128_ctr_one_product_pbs", 1, 1, 0, 0),
+             "approx-mb2": ("aes128_ctr_one_product_pbs_2bit", 1, 2, 0, 0),
+             "exact-lin": ("aes128_ctr_linear_pfks", 0, 1, 1, 0),
+             "approx-lin": ("aes128_ctr_one_product_pbs_linear_pfks", 1, 1, 1, 0),
+             "approx-mb2-lin": ("aes128_ctr_one_product_pbs_2bit_linear_pfks", 1, 2, 1, 0),
+             "approx-auto-lin": ("aes128_ctr_one_product_pbs_auto_bits_linear_pfks", 1, 3, 1, 0),
+             "htss": ("aes128_ctr_homtrace_schemeswitch", 0, 1, 0, 1)}
+
+
+def form_text(pbs_approx, pbs_multibit, pfks_linear, lwe_to_rgsw):
+    """(PBS, LWE->RGSW conversion) of a form, in words."""
+    if lwe_to_rgsw:
+        return ("2-limb exact, gadget 2^10 x 4, LUT accumulator (BSK sigma 2^-54)",
+                "HomTrace (11 automorphism key switches) + SchemeSwitch, gadget 2^8 x 5, torus-64 keys")
+    pbs = ("one FP64 product per digit polynomial, rounding error ~2^-33" if pbs_approx else "2-limb exact") + \
+        (", two key bits per blind-rotation step" if pbs_multibit == 2 else
+         ", two key bits per step unless the autotuner measured one bit faster" if pbs_multibit == 3 else "")
+    return pbs, ("PFKS, linear 40-bit key as an int8 GEMM on the matrix cores" if pfks_linear
+                 else "PFKS, lookup key, LDS selects")
+
+
+def htss_kernel_rooflines(ctx, nb):
+    """One circuit-bootstrap launch of nb inputs in the HomTrace + SchemeSwitch form: the PBS
+    (events inside the library) and the HT+SS kernel (the rest of the launch) against the FP64
+    peak by their algorithmic FLOPs (2-limb exact PBS count; 2 levels x 12 key switches)."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(13)
+    lwe = torch.from_numpy(rng.integers(0, 2 ** 32, (nb, N_LWE + 1), dtype=np.uint64).astype(np.uint32)
+                           .view(np.int32)).to("cuda")
+    rgsw = torch.empty((nb, 4, 2, N_CB), dtype=torch.int32, device="cuda")
+    ctx.cbs_batch(lwe, rgsw)
+    torch.cuda.synchronize()
+    ctx.timing_read()
+    ctx.timing_enable(True)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    ctx.cbs_batch(lwe, rgsw)
+    e1.record()
+    torch.cuda.synchronize()
+    total = e0.elapsed_time(e1)
+    pbs_ms, _, _ = ctx.timing_read()
+    ctx.timing_enable(False)
+    ht_ms = max(total - pbs_ms, 1e-6)
+    pbs_tf = nb * N_LWE * F_EP_CB[2] / (pbs_ms * 1e-3) / 1e12
+    ht_tf = nb * F_HTSS_CB / (ht_ms * 1e-3) / 1e12
+    return {"inputs": nb,
+            "pbs": {"ms": round(pbs_ms, 3), "bound": "fp64", "achieved": round(pbs_tf, 2),
+                    "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(pbs_tf / PEAK_FP64_TFLOPS, 4),
+                    "flop_per_input": N_LWE * F_EP_CB[2]},
+            "homtrace_schemeswitch": {"ms": round(ht_ms, 3), "bound": "fp64", "achieved": round(ht_tf, 2),
+                                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                                      "frac": round(ht_tf / PEAK_FP64_TFLOPS, 4), "flop_per_input": F_HTSS_CB}}
 
 
 def aes_headline(args):
@@ -493,10 +548,10 @@ def aes_headline(args):
     import numpy as np
     import torch
     import tfhe_amd
-    key_name, approx, mb, lin = AES_FORMS[args.aes_form]
+    key_name, approx, mb, lin, l2r = AES_FORMS[args.aes_form]
     sides = [int(b) for b in args.side_blocks.split(",") if b]
     maxb = max([args.blocks] + sides)
-    ctx = tfhe_amd.CbsContext(max_blocks=maxb, params=tfhe_amd.cbs_params(approx, mb, lin))
+    ctx = tfhe_amd.CbsContext(max_blocks=maxb, params=tfhe_amd.cbs_params(approx, mb, lin, l2r))
     sk = tfhe_amd.lwe_keygen(N_LWE, 3)
     t0 = time.perf_counter()
     ctx.keygen(sk, 3)
@@ -550,17 +605,19 @@ def aes_headline(args):
           flush=True)
     per_launch_inputs = inputs / max(launches, 1)
     avg_ms = pbs_ms / max(launches, 1)
-    tflops = per_launch_inputs * F_PBS_CB / (avg_ms * 1e-3) / 1e12
+    f_unit = N_LWE * F_EP_CB[2] if l2r else F_PBS_CB  # the exact form's count for the HT+SS PBS
+    tflops = per_launch_inputs * f_unit / (avg_ms * 1e-3) / 1e12
     traffic = None
     prof = os.path.join(ROOT, "profiles", "pbs_two_bit_aes_256_blocks_traffic.json")
     if os.path.exists(prof):
         traffic = json.load(open(prof)).get("hbm_bytes_per_launch")
     head = {"value": round(blocks * args.steps / elapsed, 3), "elapsed_s": elapsed, "blocks_wrong": bad,
             "keygen_s": round(keygen_s, 2), "tuned_shapes": tuned,
-            "roofline": {"bound": "valu", "kernel": "k_pbs2_manylut (one-product, two key bits per step)",
+            "roofline": {"bound": "valu", "kernel": "k_pbs2_manylut (" + form_text(approx, mb, lin, l2r)[0] + ")",
                          "achieved": round(tflops, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                         "frac": round(tflops / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
-                         "algorithmic_flop_per_launch": int(per_launch_inputs * F_PBS_CB),
+                         "frac": round(tflops / PEAK_FP64_TFLOPS, 4),
+                         "traffic": traffic if args.aes_form == "approx-auto-lin" else None,
+                         "algorithmic_flop_per_launch": int(per_launch_inputs * f_unit),
                          "avg_launch_ms": round(avg_ms, 3), "launches_timed": launches,
                          "inputs_per_launch": int(per_launch_inputs),
                          "pbs_share_of_step": round(pbs_ms / (elapsed * 1e3), 4)}}
@@ -593,7 +650,8 @@ def aes_headline(args):
         side[str(sb)] = entry
         print(f"aes128 side batch {sb} blocks: {sb / dt:.2f} blocks/s, {entry['blocks_wrong']} wrong",
               file=sys.stderr, flush=True)
-    side["kernels_at_17_blocks"] = aes_kernel_rooflines(ctx, approx, min(17, maxb) * 128, lin)
+    side["kernels_at_17_blocks"] = (htss_kernel_rooflines(ctx, min(17, maxb) * 128) if l2r else
+                                    aes_kernel_rooflines(ctx, approx, min(17, maxb) * 128, lin))
     ctx.close()
     return head, side
 
@@ -610,8 +668,9 @@ def main():
                     help="PBS / PFKS form of the headline: exact (two-limb) or approx (one FP64 product) PBS, "
                          "-mb2 two key bits per step, -auto both key forms (two bits unless tuned to one), "
                          "-lin the linear-key PFKS on the matrix cores")
-    ap.add_argument("--extra-forms", default="",
-                    help="comma-separated further AES forms timed at --extra-blocks (one launch per size)")
+    ap.add_argument("--extra-forms", default="htss",
+                    help="comma-separated further AES forms timed at --extra-blocks (one launch per size; "
+                         "'' to skip)")
     ap.add_argument("--extra-blocks", default="1,17,256")
     ap.add_argument("--aes-plain-blocks", type=int, default=0,
                     help="AES blocks timed in the per-bootstrap form of the exact PBS (a launch-overhead figure: "
@@ -655,10 +714,12 @@ def main():
         "config": {"workload": f"AES-128 CTR transciphering, {args.blocks} blocks per launch (uneven-batch config "
                                f"1/17/256/1024 blocks; 256 the headline, the others side keys)",
                    "blocks_per_step": args.blocks, "form": args.aes_form,
-                   "n": N_LWE, "N": N_CB, "pbs_gadget": "2^9 x 4", "cb_gadget": "2^8 x 2",
-                   "pfks_gadget": "2^4 x 8", "ks_gadget": "2^2 x 8",
-                   "pbs": "one FP64 product per digit polynomial, two key bits per blind-rotation step",
-                   "pfks": "linear 40-bit key, int8 GEMM on the matrix cores",
+                   "n": N_LWE, "N": N_CB, "pbs_gadget": "2^10 x 4" if AES_FORMS[args.aes_form][4] else "2^9 x 4",
+                   "cb_gadget": "2^8 x 2",
+                   "conversion_gadget": "2^8 x 5" if AES_FORMS[args.aes_form][4] else "2^4 x 8",
+                   "ks_gadget": "2^2 x 8",
+                   "pbs": form_text(*AES_FORMS[args.aes_form][1:])[0],
+                   "lwe_to_rgsw": form_text(*AES_FORMS[args.aes_form][1:])[1],
                    "circuit_bootstraps_per_step": 1280 * args.blocks},
         "roofline": head.pop("roofline"),
         "blocks_wrong": head.pop("blocks_wrong"),
@@ -669,10 +730,10 @@ def main():
     if args.extra_forms:
         sizes = [int(b) for b in args.extra_blocks.split(",") if b]
         for f in [f for f in args.extra_forms.split(",") if f]:
-            key, approx, mb, lin = AES_FORMS[f]
+            key, approx, mb, lin, l2r = AES_FORMS[f]
             line[key] = aes_ctr_bench(sizes, pbs_approx=approx, pbs_multibit=mb,
                                       plain_blocks=args.aes_plain_blocks if f == "exact" else 0,
-                                      tune=not args.no_aes_tune, pfks_linear=lin)
+                                      tune=not args.no_aes_tune, pfks_linear=lin, lwe_to_rgsw=l2r)
     line["one_product_vs_exact_product"] = one_product_error()
     line["split_fp32_product"] = split32_product_bench()
     if args.gate_steps > 0:

@@ -755,6 +755,32 @@ static PbsLut htss_pbs_io(const tfhe_cbs_ctx* c, uint64_t* rlwe) {
     return io;
 }
 
+// Event pair around one PBS launch while timing is on and the stream is not capturing:
+// begin records the first event and returns the second (or null), end records it.
+static int pbs_timing_begin(tfhe_cbs_ctx* c, hipStream_t st, hipEvent_t* e1) {
+    *e1 = nullptr;
+    if (!c->timing) return TFHE_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    int rc = hipcheck(hipStreamIsCapturing(st, &cs), "pbs timing");
+    if (rc || cs != hipStreamCaptureStatusNone) return rc;
+    while (c->tev.size() < c->tev_used + 2) {
+        hipEvent_t e = nullptr;
+        rc = hipcheck(hipEventCreate(&e), "pbs timing event");
+        if (rc) return rc;
+        c->tev.push_back(e);
+    }
+    rc = hipcheck(hipEventRecord(c->tev[c->tev_used], st), "pbs timing");
+    if (!rc) *e1 = c->tev[c->tev_used + 1];
+    return rc;
+}
+static int pbs_timing_end(tfhe_cbs_ctx* c, hipStream_t st, hipEvent_t e1, uint32_t batch) {
+    if (!e1) return TFHE_OK;
+    int rc = hipcheck(hipEventRecord(e1, st), "pbs timing");
+    c->tev_used += 2;
+    c->t_inputs += batch;
+    return rc;
+}
+
 extern "C" int tfhe_cbs_pbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t batch, uint64_t* d_out, void* stream) {
     int rc = check_cbs(c, batch, c ? c->max_batch : 0);
     if (rc) return rc;
@@ -772,30 +798,11 @@ extern "C" int tfhe_cbs_pbs_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32
     const cplx* bsk = c->d_bsk2f + (mb == 2 ? bsk2_mb2_first(c->p) * 8 * 2 * 1024 : 0);
     hipStream_t st = (hipStream_t)stream;
     hipEvent_t e1 = nullptr;
-    if (c->timing) {
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        rc = hipcheck(hipStreamIsCapturing(st, &cs), "pbs timing");
-        if (rc) return rc;
-        if (cs == hipStreamCaptureStatusNone) {
-            while (c->tev.size() < c->tev_used + 2) {
-                hipEvent_t e = nullptr;
-                rc = hipcheck(hipEventCreate(&e), "pbs timing event");
-                if (rc) return rc;
-                c->tev.push_back(e);
-            }
-            rc = hipcheck(hipEventRecord(c->tev[c->tev_used], st), "pbs timing");
-            if (rc) return rc;
-            e1 = c->tev[c->tev_used + 1];
-        }
-    }
+    rc = pbs_timing_begin(c, st, &e1);
+    if (rc) return rc;
     launch_pbs2(d_lwe, batch, c->p.n, bsk, pbs_limbs(c->p), mb, c->tables.T, d_out, st, form);
     rc = check_launch("pbs2");
-    if (!rc && e1) {
-        rc = hipcheck(hipEventRecord(e1, st), "pbs timing");
-        c->tev_used += 2;
-        c->t_inputs += batch;
-    }
-    return rc;
+    return rc ? rc : pbs_timing_end(c, st, e1, batch);
 }
 
 extern "C" int tfhe_cbs_bootstrap_many_lut(tfhe_cbs_ctx* c, const uint32_t* d_lwe, uint32_t batch,
@@ -902,9 +909,13 @@ static void cbs_pfks(tfhe_cbs_ctx* c, const uint64_t* lwe64, uint32_t count, uin
 static int cbs_batch_impl(tfhe_cbs_ctx* c, const CbsScratch& sc, const uint32_t* d_lwe, uint32_t batch,
                           uint32_t* d_rgsw, hipStream_t st) {
     if (htss(c->p)) {  // exact 2^10 x 4 PBS accumulators, then HomTrace + SchemeSwitch per CB level
+        hipEvent_t e1 = nullptr;
+        int rc = pbs_timing_begin(c, st, &e1);
+        if (rc) return rc;
         launch_pbs2(d_lwe, batch, c->p.n, c->d_bsk2f, 2, 1, c->tables.T, nullptr, st, tuned(c->tune_pbs, batch, -1),
                     htss_pbs_io(c, sc.lwe64));
-        int rc = check_launch("htss pbs");
+        rc = check_launch("htss pbs");
+        if (!rc) rc = pbs_timing_end(c, st, e1, batch);
         if (rc) return rc;
         launch_htss(sc.lwe64, batch, c->d_htkf, c->d_sskf, c->tables.T, d_rgsw, st);
         return check_launch("htss");

@@ -189,8 +189,8 @@ typedef struct tfhe_cbs_params {
      *     throughout.  The trace multiplies the PBS noise by N at the constant coefficient,
      *     so this form needs a quieter PBS: exact (pbs_approx = 0, pbs_multibit <= 1) with
      *     a 2^10 x 4 gadget (bk_logB = 10) and, for the stated noise, rlwe_nshift = 22
-     *     (BSK2, HT and SS keys at sigma 2^-54).  No PFKS key is held (keys: 3.7 MB for
-     *     HT + SS).  Rows are the same RGSW(bit) the PFKS forms give, other words. */
+     *     (BSK2, HT and SS keys at sigma 2^-54).  No PFKS key is held (HT + SS keys:
+     *     3.9 MB in Fourier form).  Rows are the same RGSW(bit) the PFKS forms give, other words. */
     int lwe_to_rgsw;
 } tfhe_cbs_params;
 
