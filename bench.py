@@ -186,6 +186,10 @@ def aes_kernel_rooflines(ctx, pbs_approx, nb, pfks_linear=0):
         return e0.elapsed_time(e1)
 
     t_pbs = timed(lambda: ctx.pbs_batch(lwe, pbs_out))
+    lwe32 = torch.from_numpy(rng.integers(0, 2 ** 32, (nb, N_CB + 1), dtype=np.uint64).astype(np.uint32)
+                             .view(np.int32)).to("cuda")
+    ks_out = torch.empty((nb, N_LWE + 1), dtype=torch.int32, device="cuda")
+    t_ks = timed(lambda: ctx.keyswitch_batch(lwe32, nb, ks_out))
     t_pfks = timed(lambda: [ctx.pfks_batch(u, pbs_out, 2 * nb, 2 * N_CB, rows) for u in (0, 1)])
     limbs = 1 if pbs_approx else 2
     tflops = nb * N_LWE * F_EP_CB[limbs] / (t_pbs * 1e-3) / 1e12
@@ -204,11 +208,22 @@ def aes_kernel_rooflines(ctx, pbs_approx, nb, pfks_linear=0):
         out["pfks_both_functions"] = {"ms": round(t_pfks, 3), "form": "lookup key", "bound": "lds",
                                       "achieved": round(lds_tbs, 2), "peak": PEAK_LDS_TBS, "unit": "TB/s",
                                       "frac": round(lds_tbs / PEAK_LDS_TBS, 4)}
+    if ctx.p.ks_linear:
+        ops = 2 * nb * N_CB * 8 * (N_LWE + 1) * 4  # 2 ops x inputs x digits x columns x key bytes
+        tops = ops / (t_ks * 1e-3) / 1e12
+        out["keyswitch"] = {"ms": round(t_ks, 3), "form": "linear key, int8 GEMM", "bound": "mfma",
+                            "achieved": round(tops, 1), "peak": PEAK_INT8_TOPS, "unit": "TOP/s",
+                            "frac": round(tops / PEAK_INT8_TOPS, 4)}
+    else:
+        lds_bytes = nb * N_CB * 8 * (N_LWE + 1) * 4  # inputs x digits x columns x 4 B of selected key rows
+        lds_tbs = lds_bytes / (t_ks * 1e-3) / 1e12
+        out["keyswitch"] = {"ms": round(t_ks, 3), "form": "lookup key", "bound": "lds", "achieved": round(lds_tbs, 2),
+                            "peak": PEAK_LDS_TBS, "unit": "TB/s", "frac": round(lds_tbs / PEAK_LDS_TBS, 4)}
     return out
 
 
 def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0, tune=True, pfks_linear=0,
-                  lwe_to_rgsw=0):
+                  lwe_to_rgsw=0, ks_linear=0):
     """AES-128 CTR transciphering: the keystream (circuit bootstrap + S-box LUT per
     round, linear layers as LWE additions) plus the public AES-CTR ciphertext bits, so
     the outputs encrypt the plaintext bits; every block decrypted and compared with the
@@ -220,7 +235,7 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0,
     import tfhe_amd
     maxb = max(batches)
     ctx = tfhe_amd.CbsContext(max_blocks=maxb, params=tfhe_amd.cbs_params(pbs_approx, pbs_multibit, pfks_linear,
-                                                                          lwe_to_rgsw))
+                                                                          lwe_to_rgsw, ks_linear))
     sk = tfhe_amd.lwe_keygen(N_LWE, 3)
     t0 = time.perf_counter()
     ctx.keygen(sk, 3)
@@ -300,9 +315,9 @@ def aes_ctr_bench(batches, reps=1, pbs_approx=0, pbs_multibit=1, plain_blocks=0,
         print(f"aes128 ctr plain per-bootstrap form: {plain_blocks} block(s) in {dt:.2f} s", file=sys.stderr,
               flush=True)
     ctx.close()
-    pbs, conv = form_text(pbs_approx, pbs_multibit, pfks_linear, lwe_to_rgsw)
+    pbs, conv, ks = form_text(pbs_approx, pbs_multibit, pfks_linear, lwe_to_rgsw, ks_linear)
     return {"form": f"fp64 FFT: PBS on the 64-bit torus ({pbs}), LWE->RGSW by {conv}, CMux LUT on the 32-bit "
-                    "torus (1-limb exact)", "keygen_s": round(keygen_s, 2), "batches": out}
+                    f"torus (1-limb exact), N -> n key switch ({ks})", "keygen_s": round(keygen_s, 2), "batches": out}
 
 
 def aes_cpu_baseline(threads=None, cbs_sample=1):
@@ -484,26 +499,29 @@ def gate_bench(args):
     return out
 
 
-# form -> (bench key, pbs_approx, pbs_multibit, pfks_linear, lwe_to_rgsw)
-AES_FORMS = {"exact": ("aes128_ctr", 0, 1, 0, 0), "approx": ("aes128_ctr_one_product_pbs", 1, 1, 0, 0),
-             "approx-mb2": ("aes128_ctr_one_product_pbs_2bit", 1, 2, 0, 0),
-             "exact-lin": ("aes128_ctr_linear_pfks", 0, 1, 1, 0),
-             "approx-lin": ("aes128_ctr_one_product_pbs_linear_pfks", 1, 1, 1, 0),
-             "approx-mb2-lin": ("aes128_ctr_one_product_pbs_2bit_linear_pfks", 1, 2, 1, 0),
-             "approx-auto-lin": ("aes128_ctr_one_product_pbs_auto_bits_linear_pfks", 1, 3, 1, 0),
-             "htss": ("aes128_ctr_homtrace_schemeswitch", 0, 1, 0, 1)}
+# form -> (bench key, pbs_approx, pbs_multibit, pfks_linear, lwe_to_rgsw, ks_linear)
+AES_FORMS = {"exact": ("aes128_ctr", 0, 1, 0, 0, 0), "approx": ("aes128_ctr_one_product_pbs", 1, 1, 0, 0, 0),
+             "approx-mb2": ("aes128_ctr_one_product_pbs_2bit", 1, 2, 0, 0, 0),
+             "exact-lin": ("aes128_ctr_linear_pfks", 0, 1, 1, 0, 0),
+             "approx-lin": ("aes128_ctr_one_product_pbs_linear_pfks", 1, 1, 1, 0, 0),
+             "approx-mb2-lin": ("aes128_ctr_one_product_pbs_2bit_linear_pfks", 1, 2, 1, 0, 0),
+             "approx-auto-lin": ("aes128_ctr_one_product_pbs_auto_bits_linear_pfks", 1, 3, 1, 0, 0),
+             "approx-auto-links": ("aes128_ctr_one_product_pbs_auto_bits_linear_pfks_linear_ks", 1, 3, 1, 0, 1),
+             "htss": ("aes128_ctr_homtrace_schemeswitch", 0, 1, 0, 1, 0)}
 
 
-def form_text(pbs_approx, pbs_multibit, pfks_linear, lwe_to_rgsw):
-    """(PBS, LWE->RGSW conversion) of a form, in words."""
+def form_text(pbs_approx, pbs_multibit, pfks_linear, lwe_to_rgsw, ks_linear=0):
+    """(PBS, LWE->RGSW conversion, N -> n key switch) of a form, in words."""
+    ks = ("linear key, signed digits, int8 GEMM on the matrix cores" if ks_linear
+          else "lookup key, LDS-staged rows")
     if lwe_to_rgsw:
         return ("2-limb exact, gadget 2^10 x 4, LUT accumulator (BSK sigma 2^-54)",
-                "HomTrace (11 automorphism key switches) + SchemeSwitch, gadget 2^8 x 5, torus-64 keys")
+                "HomTrace (11 automorphism key switches) + SchemeSwitch, gadget 2^8 x 5, torus-64 keys", ks)
     pbs = ("one FP64 product per digit polynomial, rounding error ~2^-33" if pbs_approx else "2-limb exact") + \
         (", two key bits per blind-rotation step" if pbs_multibit == 2 else
          ", two key bits per step unless the autotuner measured one bit faster" if pbs_multibit == 3 else "")
     return pbs, ("PFKS, linear 40-bit key as an int8 GEMM on the matrix cores" if pfks_linear
-                 else "PFKS, lookup key, LDS selects")
+                 else "PFKS, lookup key, LDS selects"), ks
 
 
 def htss_kernel_rooflines(ctx, nb):
@@ -548,10 +566,10 @@ def aes_headline(args):
     import numpy as np
     import torch
     import tfhe_amd
-    key_name, approx, mb, lin, l2r = AES_FORMS[args.aes_form]
+    key_name, approx, mb, lin, l2r, ksl = AES_FORMS[args.aes_form]
     sides = [int(b) for b in args.side_blocks.split(",") if b]
     maxb = max([args.blocks] + sides)
-    ctx = tfhe_amd.CbsContext(max_blocks=maxb, params=tfhe_amd.cbs_params(approx, mb, lin, l2r))
+    ctx = tfhe_amd.CbsContext(max_blocks=maxb, params=tfhe_amd.cbs_params(approx, mb, lin, l2r, ksl))
     sk = tfhe_amd.lwe_keygen(N_LWE, 3)
     t0 = time.perf_counter()
     ctx.keygen(sk, 3)
@@ -613,10 +631,10 @@ def aes_headline(args):
         traffic = json.load(open(prof)).get("hbm_bytes_per_launch")
     head = {"value": round(blocks * args.steps / elapsed, 3), "elapsed_s": elapsed, "blocks_wrong": bad,
             "keygen_s": round(keygen_s, 2), "tuned_shapes": tuned,
-            "roofline": {"bound": "valu", "kernel": "k_pbs2_manylut (" + form_text(approx, mb, lin, l2r)[0] + ")",
+            "roofline": {"bound": "valu", "kernel": "k_pbs2_manylut (" + form_text(approx, mb, lin, l2r, ksl)[0] + ")",
                          "achieved": round(tflops, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                          "frac": round(tflops / PEAK_FP64_TFLOPS, 4),
-                         "traffic": traffic if args.aes_form == "approx-auto-lin" else None,
+                         "traffic": traffic if args.aes_form in ("approx-auto-lin", "approx-auto-links") else None,
                          "algorithmic_flop_per_launch": int(per_launch_inputs * f_unit),
                          "avg_launch_ms": round(avg_ms, 3), "launches_timed": launches,
                          "inputs_per_launch": int(per_launch_inputs),
@@ -720,6 +738,7 @@ def main():
                    "ks_gadget": "2^2 x 8",
                    "pbs": form_text(*AES_FORMS[args.aes_form][1:])[0],
                    "lwe_to_rgsw": form_text(*AES_FORMS[args.aes_form][1:])[1],
+                   "keyswitch": form_text(*AES_FORMS[args.aes_form][1:])[2],
                    "circuit_bootstraps_per_step": 1280 * args.blocks},
         "roofline": head.pop("roofline"),
         "blocks_wrong": head.pop("blocks_wrong"),
@@ -730,10 +749,10 @@ def main():
     if args.extra_forms:
         sizes = [int(b) for b in args.extra_blocks.split(",") if b]
         for f in [f for f in args.extra_forms.split(",") if f]:
-            key, approx, mb, lin, l2r = AES_FORMS[f]
+            key, approx, mb, lin, l2r, ksl = AES_FORMS[f]
             line[key] = aes_ctr_bench(sizes, pbs_approx=approx, pbs_multibit=mb,
                                       plain_blocks=args.aes_plain_blocks if f == "exact" else 0,
-                                      tune=not args.no_aes_tune, pfks_linear=lin, lwe_to_rgsw=l2r)
+                                      tune=not args.no_aes_tune, pfks_linear=lin, lwe_to_rgsw=l2r, ks_linear=ksl)
     line["one_product_vs_exact_product"] = one_product_error()
     line["split_fp32_product"] = split32_product_bench()
     if args.gate_steps > 0:

@@ -144,7 +144,7 @@ int linear_sources(F map, int nsrc, std::vector<int16_t>& src) {
 struct CbsScratch {
     uint64_t* lwe64 = nullptr;  // [cap][2][N+1] PBS outputs
     uint32_t* at = nullptr;     // lookup form: [N+1][pfks_bpad(2 cap) + 256] PFKS inputs, rounded and transposed
-    int8_t* pfkl_D = nullptr;   // linear form: digits of min(2 cap, kPfklChunk) PFKS inputs
+    int8_t* pfkl_D = nullptr;   // linear PFKS or key switch: digits of min(2 cap, kPfklChunk) inputs
     uint32_t* ks_at = nullptr;  // [N][ks_bpad(cap)] key-switch digits
     uint32_t* rgsw = nullptr;   // [cap][4][2][N]
     cplx* rgsw_f = nullptr;     // [cap][8][1024]
@@ -160,11 +160,12 @@ int ensure_lutw(CbsScratch& s) {
 }
 
 // allocates a scratch set for cap circuit bootstraps
-// conv: the context's LWE -> RGSW form (0 lookup PFKS, 1 linear PFKS, 2 HomTrace + SchemeSwitch)
-int alloc_scratch(CbsScratch& s, size_t cap, int n, int conv) {
+// conv: the context's LWE -> RGSW form (0 lookup PFKS, 1 linear PFKS, 2 HomTrace + SchemeSwitch);
+// ks_linear: the N -> n key switch runs as a GEMM (it shares the linear PFKS's digit scratch)
+int alloc_scratch(CbsScratch& s, size_t cap, int n, int conv, int ks_linear) {
     int rc = TFHE_OK;
     s.cap = cap;
-    const bool linear = conv == 1;
+    const bool linear = conv == 1 || ks_linear;
     rc = rc ? rc : dmalloc((void**)&s.lwe64, cap * 2 * (kN + 1) * sizeof(uint64_t));
     // + 256 columns: a batch split into full rounds and a tail pads each part separately
     if (conv == 0)
@@ -199,7 +200,8 @@ struct tfhe_cbs_ctx {
     cplx* d_bsk2f = nullptr;
     uint32_t* d_pfks = nullptr;  // [2][N+1][8][15][2N] torus-32 (lookup form)
     int8_t* d_pfkl = nullptr;    // [2][5][2N][pfkl_kp()] signed key bytes (linear form)
-    uint32_t* d_ksk2 = nullptr;
+    uint32_t* d_ksk2 = nullptr;  // key-switching key words (the form of p.ks_linear)
+    int8_t* d_kskl = nullptr;    // linear key switch: the key as signed byte slices in fragment order
     cplx* d_htkf = nullptr;       // HomTrace key [11][5][2][2 limbs][1024] (lwe_to_rgsw = 1)
     cplx* d_sskf = nullptr;       // SchemeSwitch key [5][2][2 limbs][1024]
     uint64_t* d_htss_lut = nullptr;  // [N] test polynomial of the HomTrace form's PBS
@@ -260,7 +262,8 @@ extern "C" int tfhe_cbs_ctx_destroy(tfhe_cbs_ctx* c) {
     void* bufs[] = {c->d_K2,          c->d_bsk2f,       c->d_pfks,          c->d_ksk2,           c->d_sbox_poly,
                     c->d_inv_sbox_poly, c->d_ttab,     c->d_lin_mix,        c->d_lin_last,       c->d_lin_inv_mix,
                     c->d_lin_inv_last,  c->d_lin_id,   c->d_rk,             c->d_dk,             c->d_ctr,
-                    c->d_pub,           c->d_pfkl,     c->d_htkf,           c->d_sskf,           c->d_htss_lut};
+                    c->d_pub,           c->d_pfkl,     c->d_htkf,           c->d_sskf,           c->d_htss_lut,
+                    c->d_kskl};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     free_scratch(c->sc);
@@ -299,8 +302,9 @@ extern "C" int tfhe_cbs_ctx_create(const tfhe_cbs_params* params, uint32_t max_b
     rc = rc ? rc : dmalloc((void**)&c->d_K2, kN * sizeof(int32_t));
     rc = rc ? rc : dmalloc((void**)&c->d_bsk2f, bsk2_cplx(p) * sizeof(cplx));
     rc = rc ? rc : dmalloc((void**)&c->d_ksk2, ksk2_words(p) * sizeof(uint32_t));
+    if (p.ks_linear) rc = rc ? rc : dmalloc((void**)&c->d_kskl, kskl_key_bytes());
     const int conv = htss(p) ? 2 : p.pfks_linear ? 1 : 0;
-    rc = rc ? rc : alloc_scratch(c->sc, B, p.n, conv);
+    rc = rc ? rc : alloc_scratch(c->sc, B, p.n, conv, p.ks_linear);
     {
         // uneven AES batches of a linear-PFKS context: blocks beyond the last full round of
         // resident PBS workgroups (2 x CUs inputs, rb blocks) run as their own group on the
@@ -308,7 +312,7 @@ extern "C" int tfhe_cbs_ctx_create(const tfhe_cbs_params* params, uint32_t max_b
         const uint32_t slots = 2 * device_cus(), rb = slots / 128;
         if (!rc && conv != 0 && slots % 128 == 0 && rb >= 2 && max_blocks > rb) {
             c->tail_blocks = rb - 1;
-            rc = alloc_scratch(c->tail, 128 * (size_t)c->tail_blocks, p.n, conv);
+            rc = alloc_scratch(c->tail, 128 * (size_t)c->tail_blocks, p.n, conv, p.ks_linear);
         }
         // the second stream and its events exist before any call, so that no call made
         // under stream capture creates them
@@ -554,6 +558,14 @@ extern "C" int tfhe_gen_pfkl40(const tfhe_cbs_params* p, uint64_t seed, const in
     return rc;
 }
 
+// linear key switch: the key words d_ksk2 as signed byte slices in fragment order (padding zero)
+static int pack_kskl(tfhe_cbs_ctx* c) {
+    if (!c->p.ks_linear) return TFHE_OK;
+    int rc = hipcheck(hipMemset(c->d_kskl, 0, kskl_key_bytes()), "kskl zero");
+    if (!rc && launch_kskl_pack(c->d_ksk2, c->p.n, c->d_kskl, nullptr)) rc = fail(TFHE_E_PARAM, "kskl: n too large");
+    return rc ? rc : check_launch("kskl pack");
+}
+
 extern "C" int tfhe_cbs_set_keys(tfhe_cbs_ctx* c, const int32_t* rlwe_sk, const uint64_t* bsk, const uint32_t* pfks,
                                  const uint32_t* ksk2) {
     if (!c || !rlwe_sk || !bsk || !pfks || !ksk2) return fail(TFHE_E_PARAM, "tfhe_cbs_set_keys: bad argument");
@@ -581,6 +593,7 @@ extern "C" int tfhe_cbs_set_keys(tfhe_cbs_ctx* c, const int32_t* rlwe_sk, const 
                                                 hipMemcpyDefault), "pfks");
     }
     if (!rc) rc = hipcheck(hipMemcpy(c->d_ksk2, ksk2, ksk2_words(p) * sizeof(uint32_t), hipMemcpyDefault), "ksk2");
+    if (!rc) rc = pack_kskl(c);
     if (!rc) {
         if (pbs_limbs(p) == 2)
             launch_to_fourier2048(d_bsk, bsk2_rgsw(p) * 8 * 2, c->tables.T, c->d_bsk2f, nullptr);
@@ -641,8 +654,10 @@ extern "C" int tfhe_cbs_keygen(tfhe_cbs_ctx* c, const int32_t* lwe_sk, uint64_t 
         } else {
             launch_gen_pfks(seed, p.pf_nshift, c->d_K2, d_K2f, c->tables.T, c->d_pfks, nullptr);
         }
-        launch_gen_ksk32(seed, p.n, kN, p.ks_logB, p.ks_l, p.lwe_nshift, d_s, c->d_K2, c->d_ksk2, nullptr, DOM_KSK2);
+        launch_gen_ksk32(seed, p.n, kN, p.ks_logB, p.ks_l, p.lwe_nshift, d_s, c->d_K2, c->d_ksk2, nullptr,
+                         p.ks_linear ? DOM_KSKL : DOM_KSK2, p.ks_linear);
         if (!rc) rc = check_launch("cbs keygen");
+        if (!rc) rc = pack_kskl(c);
     }
     if (!rc) rc = hipcheck(hipDeviceSynchronize(), "cbs keygen sync");
     (void)hipFree(d_s);
@@ -1031,9 +1046,15 @@ extern "C" int tfhe_lut8x32_eval_batch(tfhe_cbs_ctx* c, const double* d_rgsw_f, 
     return check_launch("lut8x32");
 }
 
-static int cbs_keyswitch(tfhe_cbs_ctx* c, uint32_t* ks_at, const uint32_t* d_lwe32, uint32_t count, uint32_t* d_out,
-                         hipStream_t st) {
-    if (launch_keyswitch32(d_lwe32, count, kN, c->p.n, c->p.ks_logB, c->p.ks_l, c->d_ksk2, ks_at, d_out, st,
+static int cbs_keyswitch(tfhe_cbs_ctx* c, const CbsScratch& sc, const uint32_t* d_lwe32, uint32_t count,
+                         uint32_t* d_out, hipStream_t st) {
+    if (c->p.ks_linear) {  // the digit sum as an int8 GEMM, in chunks the scratch holds
+        const uint32_t chunk = std::min<uint32_t>(2 * (uint32_t)sc.cap, kPfklChunk);
+        if (launch_ks_linear(d_lwe32, count, c->p.n, c->d_kskl, sc.pfkl_D, chunk, d_out, st))
+            return fail(TFHE_E_PARAM, "cbs keyswitch: unsupported shape");
+        return check_launch("linear keyswitch N=2048");
+    }
+    if (launch_keyswitch32(d_lwe32, count, kN, c->p.n, c->p.ks_logB, c->p.ks_l, c->d_ksk2, sc.ks_at, d_out, st,
                            tuned(c->tune_ks, count, 0)))
         return fail(TFHE_E_PARAM, "cbs keyswitch: unsupported shape");
     return check_launch("keyswitch32 N=2048");
@@ -1044,7 +1065,7 @@ extern "C" int tfhe_cbs_keyswitch_batch(tfhe_cbs_ctx* c, const uint32_t* d_lwe32
     if (!c || !c->has_keys || !d_lwe32 || !d_out) return fail(TFHE_E_PARAM, "cbs keyswitch: bad argument");
     if (count > c->max_batch) return fail(TFHE_E_CAPACITY, "cbs keyswitch: count exceeds the context capacity");
     if (!count) return TFHE_OK;
-    return cbs_keyswitch(c, c->sc.ks_at, d_lwe32, count, d_out, (hipStream_t)stream);
+    return cbs_keyswitch(c, c->sc, d_lwe32, count, d_out, (hipStream_t)stream);
 }
 
 namespace {
@@ -1094,6 +1115,31 @@ int tune_one(hipStream_t st, hipEvent_t e0, hipEvent_t e1, const std::vector<int
 // the N=2048 key switch's i-slices.  Inputs are hash-filled context buffers; the round
 // state, RGSW and LUT buffers of the context are overwritten.  The choices change
 // launch shapes only: the exact PBS, the PFKS and the key switch give the same words.
+// Captured keystreams hold the launch shapes they were captured with: wait for the
+// context's streams and drop them, so that the next graph call captures the current shapes.
+static int drop_graphs(tfhe_cbs_ctx* c, const char* what) {
+    int rc = TFHE_OK;
+    if (c->gstream) rc = hipcheck(hipStreamSynchronize(c->gstream), what);
+    if (!rc && c->cstream) rc = hipcheck(hipStreamSynchronize(c->cstream), what);
+    if (rc) return rc;
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+    return TFHE_OK;
+}
+
+extern "C" int tfhe_cbs_set_pbs_form(tfhe_cbs_ctx* c, uint32_t inputs, int form) {
+    if (!c) return fail(TFHE_E_PARAM, "set_pbs_form: NULL context");
+    const int nforms = pbs_auto(c->p) ? 4 : 2;
+    if (form >= nforms) return fail(TFHE_E_PARAM, "set_pbs_form: no such PBS form for this context");
+    const int rc = drop_graphs(c, "set_pbs_form sync");
+    if (rc) return rc;
+    if (form < 0)
+        c->tune_pbs.erase(inputs);
+    else
+        c->tune_pbs[inputs] = form;
+    return TFHE_OK;
+}
+
 extern "C" int tfhe_cbs_autotune(tfhe_cbs_ctx* c, uint32_t blocks, int* choices, void* stream) {
     if (!c || !choices) return fail(TFHE_E_PARAM, "autotune: bad argument");
     if (!c->has_keys) return fail(TFHE_E_STATE, "autotune: no keys");
@@ -1105,13 +1151,9 @@ extern "C" int tfhe_cbs_autotune(tfhe_cbs_ctx* c, uint32_t blocks, int* choices,
     int rc = hipcheck(hipStreamIsCapturing(st, &cap), "autotune capture status");
     if (rc) return rc;
     if (cap != hipStreamCaptureStatusNone) return fail(TFHE_E_STATE, "autotune: the stream is capturing");
-    // captured keystreams keep the shapes they were captured with: drop them so that the
-    // next graph call captures the tuned shapes
-    if (c->gstream) rc = hipcheck(hipStreamSynchronize(c->gstream), "autotune sync");
-    if (!rc && c->cstream) rc = hipcheck(hipStreamSynchronize(c->cstream), "autotune sync");
+    // captured keystreams keep the shapes they were captured with
+    rc = drop_graphs(c, "autotune sync");
     if (rc) return rc;
-    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
-    c->graphs.clear();
     hipEvent_t e0 = nullptr, e1 = nullptr;
     rc = hipcheck(hipEventCreate(&e0), "autotune event");
     if (!rc) rc = hipcheck(hipEventCreate(&e1), "autotune event");
@@ -1181,7 +1223,9 @@ extern "C" int tfhe_cbs_autotune(tfhe_cbs_ctx* c, uint32_t blocks, int* choices,
         c->tune_pfks[count] = sl;
         choices[2 + part] = sl;
     }
-    if (!rc && nb >= many) {
+    if (!rc && c->p.ks_linear) {
+        choices[4] = -1;  // one GEMM shape: no i-slices
+    } else if (!rc && nb >= many) {
         choices[4] = ks_model_slices(nb, kN, n);
     } else if (!rc) {
         int sl = 0;
@@ -1292,7 +1336,7 @@ static int aes_round_impl(tfhe_cbs_ctx* c, CbsScratch& sc, int dir, int round, c
             launch_aes_ttable_xor(sc.lutw, blocks, dir, sc.lut32, st);
             rc = check_launch("aes ttable");
         }
-        if (!rc) rc = cbs_keyswitch(c, sc.ks_at, sc.lut32, nb, sb, st);
+        if (!rc) rc = cbs_keyswitch(c, sc, sc.lut32, nb, sb, st);
         if (!rc) {
             launch_aes_linear(sb, rk, c->d_lin_id, 1, blocks, n, d_out, st);
             rc = check_launch("aes round key");
@@ -1303,7 +1347,7 @@ static int aes_round_impl(tfhe_cbs_ctx* c, CbsScratch& sc, int dir, int round, c
         rc = tfhe_lut8_eval_batch(c, reinterpret_cast<const double*>(sc.rgsw_f), 16 * blocks,
                                   dir ? c->d_inv_sbox_poly : c->d_sbox_poly, 0, 8, sc.lut32, st);
     // SubBytes output into the spare state buffer, then the linear layer into d_out
-    if (!rc) rc = cbs_keyswitch(c, sc.ks_at, sc.lut32, nb, sb, st);
+    if (!rc) rc = cbs_keyswitch(c, sc, sc.lut32, nb, sb, st);
     if (!rc) {
         if (dir)
             launch_aes_linear(sb, rk, round == 10 ? c->d_lin_inv_last : c->d_lin_inv_mix, kLinSrcDec, blocks, n, d_out,

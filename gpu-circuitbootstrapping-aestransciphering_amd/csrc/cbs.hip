@@ -515,14 +515,18 @@ __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylu
             if constexpr (MB == 2) mrot2 = (Q2 - (uint32_t)at[i + 1]) & (Q2 - 1);
             if ((mrot | mrot2) == 0) continue;
             const cplx* key = bskf + (size_t)(i / MB) * (MB == 2 ? 3 : 1) * PBS_ROWS * 2 * LIMBS * CB_M;
-            // MB = 2: z1 = w^(m1 (1 - 4k)), z2 = w^(m2 (1 - 4k)); the pattern factors are
-            // z1 - 1, z2 - 1 and z1 z2 - 1
-            cplx zp[MB == 2 ? 2 : 1][4];
+            // MB = 2: z1 = w^(m1 (1 - 4k)), z2 = w^(m2 (1 - 4k)); the pattern factors
+            // z1 - 1, z1 z2 - 1 and z2 - 1 (patterns (1,0), (1,1), (0,1)), formed once per step
+            cplx fp[MB == 2 ? 3 : 1][4];
             if constexpr (MB == 2) {
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
-                    zp[0][s] = T.rot[(mrot * ek[s]) & (Q2 - 1)];
-                    zp[1][s] = T.rot[(mrot2 * ek[s]) & (Q2 - 1)];
+                    const cplx z1 = T.rot[(mrot * ek[s]) & (Q2 - 1)];
+                    const cplx z2 = T.rot[(mrot2 * ek[s]) & (Q2 - 1)];
+                    const cplx z12 = cmul(z1, z2);
+                    fp[0][s] = {z1.re - 1.0, z1.im};
+                    fp[1][s] = {z12.re - 1.0, z12.im};
+                    fp[2][s] = {z2.re - 1.0, z2.im};
                 }
             }
             cplx macc[2][LIMBS][4];
@@ -546,10 +550,7 @@ __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylu
                     f1024_fwd<BF, RT>(x, lds, xbuf[buf], T, trw, w, lane, tw);
                     buf ^= 1;
                     if constexpr (MB == 2) {
-                        // patterns in the order (1,0), (1,1), (0,1): x z1 serves the first two
-                        cplx t1[4], y[4];
-#pragma unroll
-                        for (int s = 0; s < 4; ++s) t1[s] = cmul(x[s], zp[0][s]);
+                        cplx y[4];
                         // key values of pattern q (rows in the order (1,0), (1,1), (0,1)) for
                         // both outputs.  The empty asm statements close a batch of loads, which
                         // the compiler would otherwise split, waiting on each load in turn.
@@ -569,8 +570,7 @@ __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylu
                         };
                         auto mac_pattern = [&](const cplx(&kv)[2][4], int q) {
 #pragma unroll
-                            for (int s = 0; s < 4; ++s)
-                                y[s] = csub(q == 0 ? t1[s] : q == 1 ? cmul(t1[s], zp[1][s]) : cmul(x[s], zp[1][s]), x[s]);
+                            for (int s = 0; s < 4; ++s) y[s] = cmul(x[s], fp[q][s]);
 #pragma unroll
                             for (int o = 0; o < 2; ++o)
 #pragma unroll
@@ -585,21 +585,16 @@ __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylu
 #pragma unroll
                             for (int q = 0; q < 3; ++q) mac_pattern(kv[q], q);
                         } else {  // 8 per batch, pattern q + 1 loaded before pattern q's MACs
-                            cplx kc[2][4], kn[2][4];
-                            load_keys(kc, 0);
-                            close_batch(kc);
-#pragma unroll
-                            for (int q = 0; q < 3; ++q) {
-                                if (q < 2) {
-                                    load_keys(kn, q + 1);
-                                    close_batch(kn);
-                                }
-                                mac_pattern(kc, q);
-#pragma unroll
-                                for (int o = 0; o < 2; ++o)
-#pragma unroll
-                                    for (int s = 0; s < 4; ++s) kc[o][s] = kn[o][s];
-                            }
+                            cplx ka[2][4], kb[2][4];
+                            load_keys(ka, 0);
+                            close_batch(ka);
+                            load_keys(kb, 1);
+                            close_batch(kb);
+                            mac_pattern(ka, 0);
+                            load_keys(ka, 2);
+                            close_batch(ka);
+                            mac_pattern(kb, 1);
+                            mac_pattern(ka, 2);
                         }
                     } else {
                         const cplx* kr = key + (size_t)r * 2 * LIMBS * CB_M + (w * 4) * 64 + lane;
@@ -1203,6 +1198,136 @@ __global__ __launch_bounds__(256, 1) void k_pfkl_gemm(const int8_t* __restrict__
 }
 
 // ===========================================================================
+// N -> n key switch with a linear key on the matrix cores (the switch of
+// aes_gpu_ks.cuh:341-370, base 2^2 x 8 on the rounded top 16 bits, with one key row
+// per (i, j) and signed digits instead of the lookup key's row per digit value):
+//   out[m] = (0, b_m) - sum_{k = 8 i + j} d[m][k] KL[k]   (mod 2^32),
+//   d_j = ((a~ + 0xAAAA) >> 2 j mod 4) - 2,  KL[i][j] = LWE_s(K2_i 2^(16 + 2 j)).
+// The digit sum is a GEMM [inputs x 8N] x [8N x (n + 1)] with k padded to PFKL_KP
+// = 513 x 32, in the linear PFKS's fragment geometry (pfkl_frag_off): digits
+// D [Mpad / 32][PFKL_KS][1 KB], key KT [4 slices][640 / 32 column blocks][PFKL_KS][1 KB],
+// each 32-bit key word held as 4 signed bytes (sum s_b 2^(8 b) = the word as an int32),
+// |sum| <= 8N x 2 x 128 < 2^23 per int32 accumulator.  Columns n + 1 .. 639 and
+// k >= 8N are zero.
+// ===========================================================================
+constexpr int KSL_SLICES = 4, KSL_COLS = 640;
+static_assert(PFKL_KP >= 8 * CB_N, "padded digit count of the key switch");
+
+// key words kl [N][8][n+1] (= [k][n+1]) -> signed byte slices of KT (zeroed by the caller)
+__global__ __launch_bounds__(256) void k_kskl_pack(const uint32_t* __restrict__ kl, int n, int8_t* __restrict__ KT) {
+    const size_t total = (size_t)8 * CB_N * (n + 1);
+    const size_t sstride = (size_t)KSL_COLS * PFKL_KP;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t k = (uint32_t)(e / (size_t)(n + 1));
+        const int c = (int)(e - (size_t)k * (n + 1));
+        int64_t x = (int32_t)kl[e];
+        int8_t* o = KT + pfkl_frag_off((size_t)c, k);
+#pragma unroll
+        for (int sl = 0; sl < KSL_SLICES; ++sl) {
+            const int8_t b = (int8_t)(uint8_t)(x & 255);
+            o[sl * sstride] = b;
+            x = (x - b) >> 8;
+        }
+    }
+}
+
+// signed base-4 digits of the rounded top 16 bits: D[m][8 q + j] for q < N and m < B,
+// zero elsewhere in [Mpad][PFKL_KP]
+__global__ __launch_bounds__(256) void k_ksl_digits(const uint32_t* __restrict__ lwe32, uint32_t B, uint32_t Mpad,
+                                                   int8_t* __restrict__ D) {
+    constexpr int N = CB_N, Q = PFKL_KP / 8;
+    const size_t total = (size_t)Mpad * Q;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t m = e / Q;
+        const int q = (int)(e - m * Q);
+        uint64_t word = 0;
+        if (m < B && q < N) {
+            const uint32_t at = (uint32_t)(((uint64_t)lwe32[m * (N + 1) + q] + (1u << 15)) >> 16);
+            const uint32_t w = at + 0xAAAAu;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int d = (int)((w >> (2 * j)) & 3u) - 2;
+                word |= (uint64_t)(uint8_t)(int8_t)d << (8 * j);
+            }
+        }
+        *reinterpret_cast<uint64_t*>(D + pfkl_frag_off(m, 8u * q)) = word;  // 8 bytes of one 16-byte run
+    }
+}
+
+// Grid x = Mpad / 128 input tiles, y = 640 / 64 column tiles.  Wave w computes 64 inputs
+// (two 32-row blocks) x 32 columns x 4 slices: eight 16-register accumulators, three
+// operand sets in flight as in k_pfkl_gemm.
+__global__ __launch_bounds__(256, 1) void k_ksl_gemm(const int8_t* __restrict__ D, uint32_t B,
+                                                 const int8_t* __restrict__ KT, const uint32_t* __restrict__ lwe32,
+                                                 int n, uint32_t* __restrict__ out) {
+    constexpr int N = CB_N;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t m0 = blockIdx.x * 128 + (w & 1) * 64;
+    const int col0 = (int)blockIdx.y * 64 + (w >> 1) * 32;
+    const int8_t* a0 = D + ((size_t)(m0 / 32) * PFKL_KS * 64 + lane) * 16;
+    const int8_t* a1 = a0 + (size_t)PFKL_KS * 1024;
+    const size_t sstride = (size_t)KSL_COLS * PFKL_KP;
+    const int8_t* bp = KT + ((size_t)(col0 / 32) * PFKL_KS * 64 + lane) * 16;
+    pfkl_v16i acc[2][KSL_SLICES];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int sl = 0; sl < KSL_SLICES; ++sl) acc[r][sl] = pfkl_v16i{};
+    // three operand sets rotate over the steps (513 = 3 x 171), each loaded two steps ahead
+    // of its products; loads past the last step reread step 0 (in bounds, unused)
+    struct Ops { pfkl_v4i a[2], b[KSL_SLICES]; };
+    auto load = [&](Ops& x, int ks) {
+        const size_t fo = (size_t)(ks < PFKL_KS ? ks : 0) * 1024;
+        x.a[0] = *reinterpret_cast<const pfkl_v4i*>(a0 + fo);
+        x.a[1] = *reinterpret_cast<const pfkl_v4i*>(a1 + fo);
+#pragma unroll
+        for (int sl = 0; sl < KSL_SLICES; ++sl) x.b[sl] = *reinterpret_cast<const pfkl_v4i*>(bp + sl * sstride + fo);
+    };
+    auto mma = [&](const Ops& x) {
+#pragma unroll
+        for (int sl = 0; sl < KSL_SLICES; ++sl) {
+            acc[0][sl] = __builtin_amdgcn_mfma_i32_32x32x32_i8(x.a[0], x.b[sl], acc[0][sl], 0, 0, 0);
+            acc[1][sl] = __builtin_amdgcn_mfma_i32_32x32x32_i8(x.a[1], x.b[sl], acc[1][sl], 0, 0, 0);
+        }
+    };
+    Ops X, Y, Z;
+    load(X, 0);
+    load(Y, 1);
+#pragma unroll 1
+    for (int ks = 0; ks < PFKL_KS; ks += 3) {
+        load(Z, ks + 2);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(X);
+        __builtin_amdgcn_sched_barrier(0);
+        load(X, ks + 3);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(Y);
+        __builtin_amdgcn_sched_barrier(0);
+        load(Y, ks + 4);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(Z);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    const int col = col0 + (lane & 31);
+    if (col > n) return;
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const uint32_t m = m0 + 32 * r + 8 * (j / 4) + 4 * (lane >> 5) + (j & 3);
+            if (m < B) {
+                uint32_t t = 0;
+#pragma unroll
+                for (int sl = 0; sl < KSL_SLICES; ++sl) t += (uint32_t)acc[r][sl][j] << (8 * sl);
+                uint32_t v = 0u - t;
+                if (col == n) v += lwe32[(size_t)m * (N + 1) + N];
+                out[(size_t)m * (n + 1) + col] = v;
+            }
+        }
+}
+
+// ===========================================================================
 // Vertical-packing LUT on the 32-bit torus (reference CMux semantics,
 // rlwe_tables.cuh:79-108): acc = (0, table); for bit i = 0..7:
 // acc = CMux(C_i, acc, X^{-stride 2^i} acc) = acc + C_i (*) (X^{-stride 2^i} acc - acc),
@@ -1545,6 +1670,27 @@ void launch_pfkl(const uint64_t* lwe64, uint32_t B, int8_t* D, const int8_t* KT,
     k_pfkl_digits<<<grid_for((size_t)Mpad * (PFKL_KP / 8) / 256 + 1, 8192), 256, 0, st>>>(lwe64, B, Mpad, D);
     k_pfkl_gemm<<<dim3(Mpad / 128, (unsigned)nu * 2 * CB_N / 64), 256, 0, st>>>(D, B, KT, u0, out, out_stride,
                                                                               out_u_offset);
+}
+
+size_t kskl_key_bytes() { return (size_t)KSL_SLICES * KSL_COLS * PFKL_KP; }
+
+int launch_kskl_pack(const uint32_t* kl, int n, int8_t* KT, hipStream_t st) {
+    if (n + 1 > KSL_COLS) return -1;
+    k_kskl_pack<<<grid_for(((size_t)8 * CB_N * (n + 1) + 255) / 256, 16384), 256, 0, st>>>(kl, n, KT);
+    return 0;
+}
+
+int launch_ks_linear(const uint32_t* lwe32, uint32_t count, int n, const int8_t* KT, int8_t* D, uint32_t chunk,
+                     uint32_t* out, hipStream_t st) {
+    if (n + 1 > KSL_COLS || chunk == 0) return -1;
+    for (uint32_t off = 0; off < count; off += chunk) {
+        const uint32_t B = count - off < chunk ? count - off : chunk;
+        const uint32_t Mpad = (B + 127) / 128 * 128;
+        const uint32_t* in = lwe32 + (size_t)off * (CB_N + 1);
+        k_ksl_digits<<<grid_for((size_t)Mpad * (PFKL_KP / 8) / 256 + 1, 8192), 256, 0, st>>>(in, B, Mpad, D);
+        k_ksl_gemm<<<dim3(Mpad / 128, KSL_COLS / 64), 256, 0, st>>>(D, B, KT, in, n, out + (size_t)off * (n + 1));
+    }
+    return 0;
 }
 
 void launch_to_fourier2048(const uint64_t* coef, size_t npoly, const FFT1024Tables& T, cplx* out, hipStream_t st) {

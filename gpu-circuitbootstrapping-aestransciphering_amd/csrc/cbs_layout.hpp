@@ -20,6 +20,7 @@ inline bool cbs_params_supported(const tfhe_cbs_params* p) {
            p->pf_nshift <= 31 && (p->pbs_approx == 0 || p->pbs_approx == 1) && p->pbs_multibit >= 0 &&
            p->pbs_multibit <= 3 && (p->pbs_multibit < 2 || (p->pbs_approx == 1 && p->n % 2 == 0)) &&
            (p->pfks_linear == 0 || p->pfks_linear == 1) && (p->lwe_to_rgsw == 0 || p->lwe_to_rgsw == 1) &&
+           (p->ks_linear == 0 || (p->ks_linear == 1 && p->n + 1 <= 640)) &&
            (p->lwe_to_rgsw == 0 ? p->bk_logB == 9
                                 : p->bk_logB == 10 && p->pbs_approx == 0 && p->pbs_multibit <= 1 && p->pfks_linear == 0);
 }
@@ -49,7 +50,8 @@ inline size_t bsk2_cplx(const tfhe_cbs_params& p) { return bsk2_rgsw(p) * 8 * 2 
 // coefficient-form BSK2 words (uint64), as tfhe_gen_bsk64 writes them
 inline size_t bsk2_words(const tfhe_cbs_params& p) { return bsk2_rgsw(p) * 8 * 2 * kN; }
 inline size_t pfks_words_per_u() { return (size_t)(kN + 1) * 8 * 15 * 2 * kN; }
-inline size_t ksk2_words(const tfhe_cbs_params& p) { return (size_t)kN * 8 * 4 * (p.n + 1); }
+// N -> n key-switching key words (uint32): [N][8][4][n+1] lookup form, [N][8][n+1] linear form
+inline size_t ksk2_words(const tfhe_cbs_params& p) { return (size_t)kN * 8 * (p.ks_linear ? 1 : 4) * (p.n + 1); }
 
 // the context's parameter set (capi_cbs.cpp)
 const tfhe_cbs_params* cbs_ctx_params(const tfhe_cbs_ctx* c);

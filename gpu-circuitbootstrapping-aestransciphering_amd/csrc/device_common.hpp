@@ -32,6 +32,7 @@ __host__ __device__ inline u32x4 philox4x32_10(uint64_t seed, uint32_t c0, uint3
 enum : uint32_t {
     DOM_LWE_SK = 1, DOM_RLWE1_SK = 2, DOM_RLWE2_SK = 3, DOM_BSK1 = 4, DOM_KSK1 = 5,
     DOM_BSK2 = 6, DOM_PFKS = 7, DOM_KSK2 = 8, DOM_BSK2MB = 9, DOM_PFKL = 10, DOM_HTKEY = 11, DOM_SSKEY = 12,
+    DOM_KSKL = 13,
     DOM_ENC = 16
 };
 

@@ -51,15 +51,17 @@ __global__ __launch_bounds__(256) void k_gen_bsk32(uint64_t seed, int n, int N, 
     }
 }
 
-// KSK entry (i, j, v) = LWE_s(v * K_i * 2^(32 - l*logB + j*logB)), layout [N][l][B][n+1].
+// KSK entry (i, j, v) = LWE_s(v * K_i * 2^(32 - l*logB + j*logB)), layout [N][l][B][n+1];
+// linear = 1: one entry per (i, j) with v = 1 (the linear key of the signed-digit switch),
+// layout [N][l][n+1].
 __global__ __launch_bounds__(256) void k_gen_ksk32(uint64_t seed, uint32_t dom, int n, int N, int logB, int l,
-                                                   int nshift, const int32_t* __restrict__ s,
+                                                   int nshift, int linear, const int32_t* __restrict__ s,
                                                    const int32_t* __restrict__ K,
                                                    uint32_t* __restrict__ ksk) {
     __shared__ uint32_t red[256];
-    const int B = 1 << logB;
+    const int B = linear ? 1 : 1 << logB;  // entries per (i, j)
     const uint32_t id = blockIdx.x;  // ((i*l + j)*B + v)
-    const int v = id % B;
+    const int v = linear ? 1 : id % B;
     const int j = (id / B) % l;
     const int i = id / (B * l);
     uint32_t* ct = ksk + (size_t)id * (n + 1);
@@ -491,8 +493,9 @@ void launch_gen_bsk32(uint64_t seed, int n, int N, int logB, int l, int nshift, 
 }
 
 void launch_gen_ksk32(uint64_t seed, int n, int N, int logB, int l, int nshift, const int32_t* s,
-                      const int32_t* K, uint32_t* ksk, hipStream_t st, uint32_t dom) {
-    k_gen_ksk32<<<N * l * (1 << logB), 256, 0, st>>>(seed, dom, n, N, logB, l, nshift, s, K, ksk);
+                      const int32_t* K, uint32_t* ksk, hipStream_t st, uint32_t dom, int linear) {
+    k_gen_ksk32<<<N * l * (linear ? 1 : 1 << logB), 256, 0, st>>>(seed, dom, n, N, logB, l, nshift, linear, s, K,
+                                                                   ksk);
 }
 
 void launch_to_fourier1024(const uint32_t* coef, size_t npoly, const FFTTables& T, cplx* out, hipStream_t st) {

@@ -12,7 +12,7 @@ struct FFTTables;
 void launch_gen_bsk32(uint64_t seed, int n, int N, int logB, int l, int nshift, const int32_t* s,
                       const int32_t* K, uint32_t* bsk, hipStream_t st);
 void launch_gen_ksk32(uint64_t seed, int n, int N, int logB, int l, int nshift, const int32_t* s,
-                      const int32_t* K, uint32_t* ksk, hipStream_t st, uint32_t dom = 5);
+                      const int32_t* K, uint32_t* ksk, hipStream_t st, uint32_t dom = 5, int linear = 0);
 void launch_to_fourier1024(const uint32_t* coef, size_t npoly, const FFTTables& T, cplx* out, hipStream_t st);
 void launch_gate_blind_rotate(const uint32_t* ct1, const uint32_t* ct2, uint32_t batch, int n, int doubled,
                               uint32_t q1, const cplx* bskf, const FFTTables& T, uint32_t* lweN,
@@ -99,6 +99,15 @@ uint32_t pfkl_kp();
 void launch_gen_pfkl40(uint64_t seed, const int32_t* K2, int8_t* KT, hipStream_t st);
 void launch_pfkl(const uint64_t* lwe64, uint32_t B, int8_t* D, const int8_t* KT, int u0, int nu, uint32_t* out,
                  size_t out_stride, size_t out_u_offset, hipStream_t st);
+// Linear-key N -> n key switch (int8 GEMM on the matrix cores): key KT of kskl_key_bytes()
+// int8 (zero the buffer, then launch_kskl_pack from the key words [N][8][n+1] u32);
+// launch_ks_linear switches count torus-32 LWE_K2 [count][N+1] into out [count][n+1] in
+// chunks of at most `chunk` inputs, with digit scratch D of pfkl_digit_bytes(chunk) bytes.
+// Both return -1 for n + 1 > 640.
+size_t kskl_key_bytes();
+int launch_kskl_pack(const uint32_t* kl, int n, int8_t* KT, hipStream_t st);
+int launch_ks_linear(const uint32_t* lwe32, uint32_t count, int n, const int8_t* KT, int8_t* D, uint32_t chunk,
+                     uint32_t* out, hipStream_t st);
 // LWEs as a u64 mask array (rows a_stride apart) and a u64 body array -> [batch][n+1] u32
 void launch_pack_lwe64(const uint64_t* a, const uint64_t* b, uint32_t batch, int n, uint32_t a_stride, uint32_t* out,
                        hipStream_t st);

@@ -9,7 +9,8 @@
 //   rlwe_sk [N] int32 | bsk2 rows as tfhe_gen_bsk64 writes them (uint64) |
 //   conversion key (lookup PFKS [2][N+1][8][15][2N] uint32, the linear PFKS key's bytes as
 //   tfhe_gen_pfkl40 writes them, or the HomTrace + SchemeSwitch keys as
-//   tfhe_gen_htss_keys writes them) | ksk2 [N][8][4][n+1] uint32.
+//   tfhe_gen_htss_keys writes them) | ksk2 [N][8][4][n+1] uint32 ([N][8][n+1] for the linear
+//   key switch).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -27,7 +28,7 @@ using namespace tfhe;
 namespace {
 
 constexpr char kMagic[8] = {'T', 'F', 'H', 'E', 'C', 'B', 'S', 'K'};
-constexpr uint32_t kVersion = 1;
+constexpr uint32_t kVersion = 2;  // 2: the parameter set holds ks_linear (18 fields)
 
 struct BlobHeader {
     char magic[8];
@@ -41,7 +42,7 @@ struct BlobHeader {
     uint8_t pad[88];
 };
 static_assert(sizeof(BlobHeader) == 256, "key file header is 256 bytes");
-static_assert(sizeof(tfhe_cbs_params) == 68, "the header holds the 17-field parameter set");
+static_assert(sizeof(tfhe_cbs_params) == 72, "the header holds the 18-field parameter set");
 
 // Streaming 64-bit content hash over little-endian 64-bit words (an integrity check,
 // not a cryptographic one): h = mix(h ^ w) per word, with the multiply-xorshift

@@ -46,10 +46,11 @@ class CbsParams(ctypes.Structure):
                 ("cb_logB", ctypes.c_int), ("cb_l", ctypes.c_int), ("pf_logB", ctypes.c_int), ("pf_l", ctypes.c_int),
                 ("ks_logB", ctypes.c_int), ("ks_l", ctypes.c_int), ("lwe_nshift", ctypes.c_int),
                 ("rlwe_nshift", ctypes.c_int), ("pf_nshift", ctypes.c_int), ("pbs_approx", ctypes.c_int),
-                ("pbs_multibit", ctypes.c_int), ("pfks_linear", ctypes.c_int), ("lwe_to_rgsw", ctypes.c_int)]
+                ("pbs_multibit", ctypes.c_int), ("pfks_linear", ctypes.c_int), ("lwe_to_rgsw", ctypes.c_int),
+                ("ks_linear", ctypes.c_int)]
 
 
-def cbs_params(pbs_approx=0, pbs_multibit=1, pfks_linear=0, lwe_to_rgsw=0):
+def cbs_params(pbs_approx=0, pbs_multibit=1, pfks_linear=0, lwe_to_rgsw=0, ks_linear=0):
     """n=636, N=2048; PBS 2^9 x 4 on the 64-bit torus (sigma 2^-51), PFKS 2^4 x 8 with
     a torus-32 key (sigma 2^-30), CB 2^8 x 2 (torus-32 RGSW), KS 2^2 x 8 to the
     level-0 key (sigma 2^-15).  pbs_approx=1 selects the one-product FP64 PBS
@@ -57,11 +58,14 @@ def cbs_params(pbs_approx=0, pbs_multibit=1, pfks_linear=0, lwe_to_rgsw=0):
     pbs_multibit=2 (with pbs_approx=1) blind-rotates two key bits per step; pfks_linear=1
     switches keys through the linear 40-bit PFKS key on the matrix cores.  lwe_to_rgsw=1
     converts by HomTrace + SchemeSwitch (the reference's conversion): exact PBS with a
-    2^10 x 4 gadget and BSK2 / HT / SS keys at sigma 2^-54, no PFKS key."""
+    2^10 x 4 gadget and BSK2 / HT / SS keys at sigma 2^-54, no PFKS key.  ks_linear=1 runs
+    the N -> n key switch with a linear key and signed digits as an int8 GEMM on the
+    matrix cores."""
     if lwe_to_rgsw:
-        return CbsParams(636, 2048, 10, 4, 8, 2, 4, 8, 2, 8, 15, 22, 30, 0, min(int(pbs_multibit), 1), 0, 1)
+        return CbsParams(636, 2048, 10, 4, 8, 2, 4, 8, 2, 8, 15, 22, 30, 0, min(int(pbs_multibit), 1), 0, 1,
+                         int(ks_linear))
     return CbsParams(636, 2048, 9, 4, 8, 2, 4, 8, 2, 8, 15, 19, 30, int(pbs_approx), int(pbs_multibit),
-                     int(pfks_linear), 0)
+                     int(pfks_linear), 0, int(ks_linear))
 
 
 def build():
@@ -128,6 +132,7 @@ SIGNATURES = {
     "tfhe_lut8x32_eval_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp, _vp]),
     "tfhe_cbs_keyswitch_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_cbs_autotune": (ctypes.c_int, [_vp, ctypes.c_uint32, _i32p, _vp]),
+    "tfhe_cbs_set_pbs_form": (ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_int]),
     "tfhe_cbs_timing_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "tfhe_cbs_timing_read": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint32),
                                             ctypes.POINTER(ctypes.c_uint64)]),
@@ -520,6 +525,11 @@ class CbsContext:
         check(lib().tfhe_cbs_autotune(self.h, blocks, ch, _stream(stream)))
         return {"pbs_form": [int(ch[0]), int(ch[1])], "pfks_slices": [int(ch[2]), int(ch[3])],
                 "ks_slices": int(ch[4])}
+
+    def set_pbs_form(self, inputs, form):
+        """Pin the PBS launch form for launches of `inputs` ciphertexts (-1: back to the
+        size model); tfhe_cbs_set_pbs_form."""
+        check(lib().tfhe_cbs_set_pbs_form(self.h, int(inputs), int(form)))
 
     def timing_enable(self, on=True):
         """Bracket every PBS launch with HIP events from now on (tfhe_cbs_timing_enable)."""

@@ -192,6 +192,15 @@ typedef struct tfhe_cbs_params {
      *     (BSK2, HT and SS keys at sigma 2^-54).  No PFKS key is held (HT + SS keys:
      *     3.9 MB in Fourier form).  Rows are the same RGSW(bit) the PFKS forms give, other words. */
     int lwe_to_rgsw;
+    /* N -> n key switch form (base 2^2 x 8 on the rounded top 16 bits, as the
+     * reference's kernel_KeySwitchU16, aes_gpu_ks.cuh:341-370):
+     * 0 = lookup key, one LWE row per (i, digit j, digit value v) ([N][8][4][n+1] u32),
+     *     one LDS-staged row selected and subtracted per digit (default);
+     * 1 = linear key, one row per (i, j) ([N][8][n+1] u32, LWE_s(K2_i 2^(16 + 2 j))),
+     *     signed digits in [-2, 2), the digit sum an int8 GEMM on the matrix cores
+     *     (each key word as 4 signed bytes).  Same phase, other words; a quarter of
+     *     the key bytes; noise variance 1.5 x 8N key noises against 0.75 x 8N. */
+    int ks_linear;
 } tfhe_cbs_params;
 
 typedef struct tfhe_cbs_ctx tfhe_cbs_ctx;
@@ -234,7 +243,7 @@ int tfhe_gen_pfks32(const tfhe_cbs_params* p, uint64_t seed, const int32_t* d_rl
  * uploads the CPU context's keys (cirbts.cuh:420-426, gpu_init_cirbsk
  * cirbts-init.cu:200-865): rlwe_sk host [N] int32; bsk coefficient rows as
  * tfhe_gen_bsk64 writes them, pfks as tfhe_gen_pfks32, ksk2 [N][8][4][n+1]
- * uint32.  The three key buffers may be host or device memory; the BSK2 is
+ * uint32 ([N][8][n+1] with ks_linear = 1).  The three key buffers may be host or device memory; the BSK2 is
  * transformed to the context's PBS form.  With pfks_linear = 1, pfks points at the
  * linear key as tfhe_gen_pfkl40 writes it. */
 int tfhe_cbs_set_keys(tfhe_cbs_ctx* ctx, const int32_t* rlwe_sk, const uint64_t* bsk, const uint32_t* pfks,
@@ -361,6 +370,15 @@ int tfhe_cbs_keyswitch_batch(tfhe_cbs_ctx* ctx, const uint32_t* d_lwe32, uint32_
  * shapes change no output word of the exact PBS, the PFKS or the key switch.  Not
  * during stream capture. */
 int tfhe_cbs_autotune(tfhe_cbs_ctx* ctx, uint32_t blocks, int* choices, void* stream);
+
+/* Pin the PBS launch form used for launches of `inputs` ciphertexts (the numbering of
+ * tfhe_cbs_autotune's choices[0]; -1 returns that size to the size model), e.g. to
+ * restore a stored autotune result without timing again.  No counterpart in the
+ * reference (its shapes are compile-time).  TFHE_E_PARAM for a form the context's PBS
+ * does not have.  Drops captured keystream graphs.  The FFT and key-batch forms of one
+ * PBS give the same words; in a pbs_multibit = 3 context, choosing one or two key bits
+ * per step changes them (both decrypt within the PBS noise). */
+int tfhe_cbs_set_pbs_form(tfhe_cbs_ctx* ctx, uint32_t inputs, int form);
 
 /* Kernel timing of the multi-value PBS (the reference prints per-block
  * compute_only_ms from events around its launches, main.cu:1694-1758): while

@@ -83,11 +83,12 @@ struct LWECiphertext {
  * KS 2^2 x 8; circuit bootstrapping N=2048, PBS 2^9 x 4, CB 2^8 x 2, PFKS 2^4 x 8. */
 inline tfhe_gate_params GateParams() { return tfhe_gate_params{636, 1024, 7, 3, 2, 8, 15, 25}; }
 inline tfhe_cbs_params CirBTSParams(int pbs_approx = 0, int pbs_multibit = 1, int pfks_linear = 0,
-                                    int lwe_to_rgsw = 0) {
+                                    int lwe_to_rgsw = 0, int ks_linear = 0) {
     if (lwe_to_rgsw)  // HomTrace + SchemeSwitch: exact 2^10 x 4 PBS, keys at sigma 2^-54
         return tfhe_cbs_params{636, 2048, 10, 4, 8, 2, 4, 8, 2, 8, 15, 22, 30, 0, pbs_multibit > 1 ? 1 : pbs_multibit,
-                               0, 1};
-    return tfhe_cbs_params{636, 2048, 9, 4, 8, 2, 4, 8, 2, 8, 15, 19, 30, pbs_approx, pbs_multibit, pfks_linear, 0};
+                               0, 1, ks_linear};
+    return tfhe_cbs_params{636, 2048, 9, 4, 8, 2, 4, 8, 2, 8, 15, 19, 30, pbs_approx, pbs_multibit, pfks_linear, 0,
+                           ks_linear};
 }
 
 /* Client-side helpers standing in for the OpenFHE fork's KeyGen / Encrypt / Decrypt. */

@@ -705,6 +705,56 @@ void orc_gen_ksk2(uint64_t seed, const orc_cbs_params* p, const int32_t* s, cons
 }
 
 /* ------------------------------------------------------------------------- */
+/* N -> n key switch with a linear key: the same switch as orc_ks2 (reference   */
+/* kernel_KeySwitchU16, aes_gpu_ks.cuh:341-370: base 2^2, 8 digits of the       */
+/* rounded top 16 bits), with one key row per (i, j),                          */
+/*   KL[i][j] = LWE_s(K2_i 2^(32 - ks_l ks_logB + j ks_logB)),                  */
+/* and signed digits d_j in [-B/2, B/2) by the offset method:                  */
+/*   a~ = round(a_i / 2^(32 - 16)), d_j = ((a~ + sum_j (B/2) B^j) >> j logB mod B) - B/2, */
+/* so sum_j d_j B^j = a~ mod 2^16 and                                          */
+/*   out = (0, b) - sum_i sum_j d_j KL[i][j]  (mod 2^32)                        */
+/* has the lookup form's phase (the wrap of the top digit is a multiple of     */
+/* 2^32).  The sum is an integer GEMM (digits x key), which the GPU runs on the */
+/* matrix cores.  Key rows: domain ORC_DOM_KSKL, id i l + j, noise lwe_nshift.  */
+/* ------------------------------------------------------------------------- */
+void orc_gen_kskl_entry(uint64_t seed, const orc_cbs_params* p, const int32_t* s, const int32_t* K2, int i, int j,
+                        uint32_t* row) {
+    const int kbits = p->ks_l * p->ks_logB;
+    const uint32_t mu = (uint32_t)K2[i] * ((uint32_t)1 << (32 - kbits + j * p->ks_logB));
+    orc_lwe_encrypt32(seed, ORC_DOM_KSKL, (uint32_t)(i * p->ks_l + j), p->n, s, mu, p->lwe_nshift, row);
+}
+
+void orc_gen_kskl(uint64_t seed, const orc_cbs_params* p, const int32_t* s, const int32_t* K2, uint32_t* kl) {
+    for (int i = 0; i < p->N; ++i)
+        for (int j = 0; j < p->ks_l; ++j)
+            orc_gen_kskl_entry(seed, p, s, K2, i, j, kl + ((size_t)i * p->ks_l + j) * (p->n + 1));
+}
+
+void orc_ks2_linear32(const orc_cbs_params* p, const uint32_t* kl, const uint32_t* lwe32, uint32_t* out) {
+    const int n = p->n, B = 1 << p->ks_logB, kbits = p->ks_l * p->ks_logB;
+    uint32_t off = 0;
+    for (int j = 0; j < p->ks_l; ++j) off += (uint32_t)(B / 2) << (j * p->ks_logB);
+    for (int k = 0; k < n; ++k) out[k] = 0;
+    out[n] = lwe32[p->N];
+    for (int i = 0; i < p->N; ++i) {
+        const uint32_t at = (uint32_t)(((uint64_t)lwe32[i] + ((uint64_t)1 << (31 - kbits))) >> (32 - kbits));
+        for (int j = 0; j < p->ks_l; ++j) {
+            const int32_t d = (int32_t)(((at + off) >> (j * p->ks_logB)) & (uint32_t)(B - 1)) - B / 2;
+            const uint32_t* row = kl + ((size_t)i * p->ks_l + j) * (n + 1);
+            for (int k = 0; k <= n; ++k) out[k] -= (uint32_t)d * row[k];
+        }
+    }
+}
+
+void orc_ks2_linear(const orc_cbs_params* p, const uint32_t* kl, const uint64_t* lwe64, uint32_t* out) {
+    const int N = p->N;
+    uint32_t* l32 = (uint32_t*)malloc(sizeof(uint32_t) * (N + 1));
+    for (int i = 0; i <= N; ++i) l32[i] = (uint32_t)((lwe64[i] + ((uint64_t)1 << 31)) >> 32);
+    orc_ks2_linear32(p, kl, l32, out);
+    free(l32);
+}
+
+/* ------------------------------------------------------------------------- */
 /* HomTrace + SchemeSwitch LWE -> RGSW (the reference's conversion,            */
 /* cirbts-batch.cu:1612-1790; kernels kernel.cu:7536-7569 (automorphism maps), */
 /* 7776-7789 (row save), 7832-7888 (trace round), 8407-8493 (scheme switch);   */

@@ -347,6 +347,10 @@ def _cbs_lib():
             "orc_vertical_packing": (ctypes.c_double, [P, ctypes.c_int, _f64p, _u32p, _u32p]),
             "orc_ks2": (None, [P, _u32p, _u64p, _u32p]),
             "orc_gen_ksk2": (None, [ctypes.c_uint64, P, _i32p, _i32p, _u32p]),
+            "orc_gen_kskl_entry": (None, [ctypes.c_uint64, P, _i32p, _i32p, ctypes.c_int, ctypes.c_int, _u32p]),
+            "orc_gen_kskl": (None, [ctypes.c_uint64, P, _i32p, _i32p, _u32p]),
+            "orc_ks2_linear32": (None, [P, _u32p, _u32p, _u32p]),
+            "orc_ks2_linear": (None, [P, _u32p, _u64p, _u32p]),
             "orc_gen_ksk32_dom": (None, [ctypes.c_uint64, ctypes.c_uint32, G, _i32p, _i32p, _u32p]),
         }
         for name, (res, args) in sig.items():
@@ -524,6 +528,21 @@ class CbsKeys:
             _cbs_lib().orc_gen_ksk2(self.seed, ctypes.byref(p), self.s, self.K2, self._ksk2)
         return self._ksk2
 
+    @property
+    def kskl(self):
+        """Linear N -> n key switching key [N][ks_l][n+1] u32 (one row per (i, j))."""
+        if getattr(self, "_kskl", None) is None:
+            p = self.p
+            self._kskl = np.zeros(p.N * p.ks_l * (p.n + 1), np.uint32)
+            _cbs_lib().orc_gen_kskl(self.seed, ctypes.byref(p), self.s, self.K2, self._kskl)
+        return self._kskl
+
+    def kskl_entry(self, i, j):
+        p = self.p
+        row = np.zeros(p.n + 1, np.uint32)
+        _cbs_lib().orc_gen_kskl_entry(self.seed, ctypes.byref(p), self.s, self.K2, int(i), int(j), row)
+        return row
+
     def encrypt_bit(self, bit, id_, seed=None):
         """Level-0 LWE of bit * 2^31 (p = 2)."""
         p = self.p
@@ -609,6 +628,18 @@ class CbsKeys:
     def ks2(self, lwe64):
         out = np.zeros(self.p.n + 1, np.uint32)
         _cbs_lib().orc_ks2(ctypes.byref(self.p), self.ksk2, np.ascontiguousarray(lwe64, np.uint64), out)
+        return out
+
+    def ks2_linear(self, lwe64):
+        """The N -> n key switch of a torus-64 LWE_K2 with the linear key and signed digits."""
+        out = np.zeros(self.p.n + 1, np.uint32)
+        _cbs_lib().orc_ks2_linear(ctypes.byref(self.p), self.kskl, np.ascontiguousarray(lwe64, np.uint64), out)
+        return out
+
+    def ks2_linear32(self, lwe32):
+        """The same for a torus-32 LWE_K2 [N+1] u32 (the GPU key switch's input)."""
+        out = np.zeros(self.p.n + 1, np.uint32)
+        _cbs_lib().orc_ks2_linear32(ctypes.byref(self.p), self.kskl, np.ascontiguousarray(lwe32, np.uint32), out)
         return out
 
     def extract32(self, acc, idx):

@@ -42,6 +42,7 @@ enum {
     ORC_DOM_LWE_SK = 1, ORC_DOM_RLWE1_SK = 2, ORC_DOM_RLWE2_SK = 3,
     ORC_DOM_BSK1 = 4, ORC_DOM_KSK1 = 5, ORC_DOM_BSK2 = 6, ORC_DOM_PFKS = 7,
     ORC_DOM_KSK2 = 8, ORC_DOM_BSK2MB = 9, ORC_DOM_PFKL = 10, ORC_DOM_HTKEY = 11, ORC_DOM_SSKEY = 12,
+    ORC_DOM_KSKL = 13,
     ORC_DOM_ENC = 16
 };
 
@@ -167,6 +168,12 @@ double orc_vertical_packing(const orc_cbs_params* p, int stride, const double* r
                             uint32_t* acc);
 void orc_ks2(const orc_cbs_params* p, const uint32_t* ksk2, const uint64_t* lwe64, uint32_t* out);
 void orc_gen_ksk2(uint64_t seed, const orc_cbs_params* p, const int32_t* s, const int32_t* K2, uint32_t* ksk);
+/* the N -> n key switch with a linear key (one row per (i, j)) and signed digits, cbs_oracle.c */
+void orc_gen_kskl_entry(uint64_t seed, const orc_cbs_params* p, const int32_t* s, const int32_t* K2, int i, int j,
+                        uint32_t* row);
+void orc_gen_kskl(uint64_t seed, const orc_cbs_params* p, const int32_t* s, const int32_t* K2, uint32_t* kl);
+void orc_ks2_linear32(const orc_cbs_params* p, const uint32_t* kl, const uint32_t* lwe32, uint32_t* out);
+void orc_ks2_linear(const orc_cbs_params* p, const uint32_t* kl, const uint64_t* lwe64, uint32_t* out);
 
 /* ---- plain AES-128 (FIPS-197) ---- */
 uint8_t orc_aes_sbox(uint8_t x);

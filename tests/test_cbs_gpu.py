@@ -849,6 +849,91 @@ def lctx(gpu, okeys):
     c.close()
 
 
+@pytest.fixture(scope="module")
+def kctx(gpu, okeys):
+    """The same keys with the linear-key N -> n key switch (ks_linear = 1, int8 GEMM on the
+    matrix cores) and the linear PFKS; 65 blocks, so that one key switch of 8320 inputs
+    runs as two GEMM chunks (8192 + 128)."""
+    import tfhe_amd
+    c = tfhe_amd.CbsContext(max_blocks=65, params=tfhe_amd.cbs_params(pfks_linear=1, ks_linear=1))
+    c.keygen(okeys.s, SEED)
+    yield c
+    c.close()
+
+
+def test_ks2_linear_bit_exact(kctx, okeys):
+    """The linear-key key switch equals the oracle's restatement (orc_ks2_linear32) bit for
+    bit: 8320 random torus-32 inputs in one call (checked on both sides of the 8192-input
+    chunk boundary and at random), then calls of 1 and 129 inputs.  The oracle's key rows
+    are regenerated from the seed, so this also pins the GPU key generator."""
+    import torch
+    rng = np.random.default_rng(43)
+    nb = 8320
+    lwe32 = rng.integers(0, 2 ** 32, (nb, okeys.p.N + 1), dtype=np.uint64).astype(np.uint32)
+    d_in = to_dev32(lwe32)
+    d_out = torch.empty((nb, okeys.p.n + 1), dtype=torch.int32, device="cuda")
+    kctx.keyswitch_batch(d_in, nb, d_out)
+    torch.cuda.synchronize()
+    got = from_dev(d_out)
+    idx = [0, 1, 127, 128, 4095, 8190, 8191, 8192, 8193, 8319] + [int(k) for k in rng.integers(0, nb, 6)]
+    for k in idx:
+        assert np.array_equal(got[k], okeys.ks2_linear32(lwe32[k])), k
+    for cnt in (1, 129):
+        d_small = torch.full((cnt, okeys.p.n + 1), 5, dtype=torch.int32, device="cuda")
+        kctx.keyswitch_batch(d_in[nb - cnt:].contiguous(), cnt, d_small)
+        torch.cuda.synchronize()
+        assert np.array_equal(from_dev(d_small), got[nb - cnt:])
+
+
+def test_ks2_linear_key_file_and_aes(kctx, okeys, tmp_path):
+    """A key file of the linear-KS context (version 2: the parameter set holds ks_linear)
+    carries the key-switching key as the oracle's rows [N][8][n+1]; loaded into a fresh
+    context it gives the same key switch words; AES-128 (FIPS-197 C.1 and random blocks)
+    through the linear key switch decrypts exactly."""
+    import struct
+    import torch
+    import tfhe_amd
+    path = tmp_path / "kskl.tfhe"
+    kctx.save_keys(path)
+    data = open(path, "rb").read()
+    assert data[:8] == b"TFHECBSK" and struct.unpack_from("<I", data, 8)[0] == 2
+    sizes = struct.unpack_from("<4Q", data, 96)
+    assert sizes[3] == okeys.p.N * 8 * (okeys.p.n + 1) * 4
+    off = 256 + sum(sizes[:3])
+    assert np.array_equal(np.frombuffer(data, np.uint32, okeys.p.N * 8 * (okeys.p.n + 1), off), okeys.kskl)
+    c2 = tfhe_amd.CbsContext(max_blocks=1, params=tfhe_amd.cbs_params(pfks_linear=1, ks_linear=1))
+    try:
+        c2.load_keys(path)
+        rng = np.random.default_rng(47)
+        lwe32 = rng.integers(0, 2 ** 32, (5, okeys.p.N + 1), dtype=np.uint64).astype(np.uint32)
+        outs = []
+        for c in (kctx, c2):
+            d = torch.empty((5, okeys.p.n + 1), dtype=torch.int32, device="cuda")
+            c.keyswitch_batch(to_dev32(lwe32), 5, d)
+            torch.cuda.synchronize()
+            outs.append(from_dev(d))
+        assert np.array_equal(outs[0], outs[1])
+    finally:
+        c2.close()
+    key = bytes(range(16))
+    kctx.set_round_keys(tfhe_amd.aes_encrypt_key_bits(okeys.s, 77, key))
+    rng = np.random.default_rng(11)
+    blocks = [bytes.fromhex("00112233445566778899aabbccddeeff")] + \
+        [bytes(rng.integers(0, 256, 16, dtype=np.uint8)) for _ in range(4)]
+    d_ks = torch.empty((5, 128, 637), dtype=torch.int32, device="cuda")
+    kctx.aes_ctr_keystream(blocks, d_ks)
+    torch.cuda.synchronize()
+    got = from_dev(d_ks)
+    dec = _decrypt_blocks(okeys, got, 5)
+    assert dec[0].hex() == "69c4e0d86a7b0430d8cdb78070b4c55a"
+    assert dec[1:] == [O.aes128_encrypt(key, b) for b in blocks[1:]]
+    s64 = okeys.s.astype(np.uint64)
+    ph = (got[:, :, -1].astype(np.uint64) - (got[:, :, :-1].astype(np.uint64) @ s64)) % 2 ** 32
+    err = ((ph + 2 ** 30) % 2 ** 31).astype(np.int64) - 2 ** 30
+    print(f"AES with the linear key switch: max |phase error| 2^{np.log2(np.abs(err).max()):.2f} of 2^32", flush=True)
+    assert np.abs(err).max() < 2 ** 29
+
+
 def test_pfkl_key_entries_match_oracle(lctx, okeys):
     """GPU linear-key entries (5 signed bytes per 40-bit word, k = 8 i + t) equal the
     oracle's, including the b row (i = N) and both functions."""
@@ -993,6 +1078,43 @@ def test_pbs_auto_multibit_matches_fixed_forms(gpu, okeys):
     assert not np.array_equal(outs[(1, 128)], outs[(2, 128)])  # one and two bits per step differ in their words
     (two_bits, got), = [(k[1], v) for k, v in outs.items() if k[0] == "tuned"]
     assert np.array_equal(got, outs[(2, 128)] if two_bits else outs[(1, 128)])
+
+
+def test_pinned_pbs_forms_give_the_same_words(actx, okeys):
+    """tfhe_cbs_set_pbs_form pins the launch form per size.  The two forms give the same
+    words on the same launch sizes (301, 300 and 3 inputs) whichever the size model would
+    pick: with one key bit per step the branchy and branch-free cross-wave FFT steps
+    (rounding at the same points), with two the two-waves-per-SIMD and wide key batches.
+    Inputs mix encryptions, zero masks (every step a zero rotation) and masks with one
+    rotating key-bit pair.  A form the context's PBS does not have is refused."""
+    import torch
+    import tfhe_amd
+    n = okeys.p.n
+    rng = np.random.default_rng(37)
+    batch = 301
+    cts = np.zeros((batch, n + 1), np.uint32)
+    for k in range(batch):
+        if k % 5 == 1:  # zero mask, random body
+            cts[k, n] = rng.integers(0, 1 << 32)
+        elif k % 7 == 3:  # one rotating key-bit pair
+            q = int(rng.integers(0, n // 2))
+            cts[k, 2 * q] = rng.integers(1 << 22, (1 << 32) - (1 << 22))
+            cts[k, n] = rng.integers(0, 1 << 32)
+        else:
+            cts[k] = tfhe_amd.lwe_encrypt(okeys.s, 41, k, int(rng.integers(0, 2)), p=2)
+    d_cts = to_dev32(cts)
+    with pytest.raises(tfhe_amd.TfheError):
+        actx.set_pbs_form(batch, 2)  # forms 0 and 1 only
+    for nb in (301, 300, 3):
+        outs = {}
+        for form in (0, 1):
+            actx.set_pbs_form(nb, form)
+            d_out = torch.full((nb, 2, 2049), 7, dtype=torch.int64, device="cuda")
+            actx.pbs_batch(d_cts[:nb].contiguous(), d_out)
+            torch.cuda.synchronize()
+            outs[form] = from_dev(d_out, np.uint64)
+        actx.set_pbs_form(nb, -1)
+        assert np.array_equal(outs[0], outs[1])
 
 
 def test_empty_batches_leave_outputs_untouched(ctx, okeys):

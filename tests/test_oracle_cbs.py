@@ -293,6 +293,68 @@ def test_ks2_preserves_phase(keys):
         assert abs(e) < 2 ** 32 / 16
 
 
+def test_kskl_rows_decrypt(keys):
+    """Linear key-switching rows KL[i][j] decrypt under the level-0 key to
+    K2_i 2^(16 + 2 j) (mod 2^32) within 6 sigma (sigma = 2^-15 of the torus), and the
+    full key holds the same rows."""
+    p = keys.p
+    rng = np.random.default_rng(8)
+    kl = keys.kskl.reshape(p.N, p.ks_l, p.n + 1)
+    for i, j in [(0, 0), (p.N - 1, p.ks_l - 1)] + [(int(rng.integers(0, p.N)), int(rng.integers(0, p.ks_l)))
+                                                  for _ in range(20)]:
+        row = keys.kskl_entry(i, j)
+        assert np.array_equal(row, kl[i, j])
+        mu = (int(keys.K2[i]) * (1 << (16 + 2 * j))) % 2 ** 32
+        e = ((keys.phase32(row) - mu + 2 ** 31) % 2 ** 32) - 2 ** 31
+        assert abs(e) < 6 * 2 ** 17
+
+
+def _balanced_digits(at, B, l):
+    """Digits d_j in [-B/2, B/2) with sum_j d_j B^j = at (mod B^l), by carrying."""
+    out = []
+    for _ in range(l):
+        d = ((at + B // 2) % B) - B // 2
+        out.append(d)
+        at = (at - d) // B
+    return out
+
+
+def test_ks2_linear_equals_signed_digit_sum(keys):
+    """The oracle's linear key switch equals an explicit sum over the key rows with digits
+    found by carrying (not the offset method): out = (0, b) - sum_ij d_ij KL[i][j] mod 2^32,
+    on random torus-32 inputs."""
+    p = keys.p
+    rng = np.random.default_rng(9)
+    kl = keys.kskl.reshape(p.N * p.ks_l, p.n + 1).astype(np.int64)
+    B, kbits = 1 << p.ks_logB, p.ks_l * p.ks_logB
+    for _ in range(3):
+        lwe = rng.integers(0, 2 ** 32, p.N + 1, dtype=np.uint64).astype(np.uint32)
+        d = np.zeros(p.N * p.ks_l, np.int64)
+        for i in range(p.N):
+            at = ((int(lwe[i]) + (1 << (31 - kbits))) >> (32 - kbits)) % (1 << kbits)
+            d[i * p.ks_l:(i + 1) * p.ks_l] = _balanced_digits(at, B, p.ks_l)
+        assert d.min() >= -B // 2 and d.max() < B // 2
+        want = (-(d @ kl)) % 2 ** 32
+        want[p.n] = (want[p.n] + int(lwe[p.N])) % 2 ** 32
+        assert np.array_equal(keys.ks2_linear32(lwe), want.astype(np.uint32))
+
+
+def test_ks2_linear_preserves_phase(keys):
+    """The linear key switch keeps the phase within the key-switching noise, as the lookup
+    form does (the same digit sum, other key rows and signed digits)."""
+    p = keys.p
+    N = p.N
+    rng = np.random.default_rng(5)
+    for bit in (0, 1):
+        a = rng.integers(0, 2 ** 64, N, dtype=np.uint64)
+        b = (int(np.dot(a, keys.K2.astype(np.uint64))) + (bit << 63) + 12345) % 2 ** 64
+        lwe = np.concatenate([a, np.array([b], np.uint64)])
+        for out in (keys.ks2_linear(lwe), keys.ks2(lwe)):
+            assert keys.decrypt_bit(out) == bit
+            e = ((keys.phase32(out) - (bit << 31) + 2 ** 31) % 2 ** 32) - 2 ** 31
+            assert abs(e) < 2 ** 32 / 16
+
+
 # ---------------------------------------------------------------- two key bits per step
 def test_mb2_patterns_reproduce_the_pair_rotation():
     """For every key-bit pair (b0, b1) and rotations (m1, m2): at most one pattern
