@@ -682,10 +682,11 @@ def main():
     ap.add_argument("--blocks", type=int, default=256, help="AES blocks per step (the headline batch)")
     ap.add_argument("--side-blocks", default="1,17,1024",
                     help="comma-separated AES batch sizes timed once each beside the headline ('' to skip)")
-    ap.add_argument("--aes-form", default="approx-auto-lin", choices=sorted(AES_FORMS),
-                    help="PBS / PFKS form of the headline: exact (two-limb) or approx (one FP64 product) PBS, "
-                         "-mb2 two key bits per step, -auto both key forms (two bits unless tuned to one), "
-                         "-lin the linear-key PFKS on the matrix cores")
+    ap.add_argument("--aes-form", default="approx-auto-links", choices=sorted(AES_FORMS),
+                    help="PBS / PFKS / key-switch form of the headline: exact (two-limb) or approx (one FP64 "
+                         "product) PBS, -mb2 two key bits per step, -auto both key forms (two bits unless tuned to "
+                         "one), -lin the linear-key PFKS on the matrix cores, -links also the linear-key N -> n key "
+                         "switch; htss: HomTrace + SchemeSwitch")
     ap.add_argument("--extra-forms", default="htss",
                     help="comma-separated further AES forms timed at --extra-blocks (one launch per size; "
                          "'' to skip)")
