@@ -84,6 +84,16 @@ def lib():
             "orc_keyswitch32": (None, [P, _u32p, _u32p, _u32p]),
             "orc_gate_prep": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _u32p, _u32p, _u32p]),
             "orc_gate_bootstrap": (ctypes.c_double, [P, _f64p, _u32p, ctypes.c_int, _u32p, _u32p, _u32p]),
+            "orc_func_property": (ctypes.c_int, [_u32p, ctypes.c_uint64]),
+            "orc_bootstrap_func": (ctypes.c_int, [P, _f64p, _u32p, _u32p, ctypes.c_uint64, _u32p, ctypes.c_uint64,
+                                                  _u32p]),
+            "orc_eval_func": (ctypes.c_int, [P, _f64p, _u32p, _u32p, ctypes.c_uint64, _u32p, ctypes.c_uint32, _u32p]),
+            "orc_eval_floor": (ctypes.c_int, [P, _f64p, _u32p, _u32p, ctypes.c_uint64, ctypes.c_uint32,
+                                              ctypes.c_uint32, ctypes.c_uint64, _u32p]),
+            "orc_eval_sign": (ctypes.c_int, [P, _f64p, _u32p, _u32p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64,
+                                             ctypes.c_int, _u32p]),
+            "orc_eval_decomp": (ctypes.c_int, [P, _f64p, _u32p, _u32p, ctypes.c_uint64, ctypes.c_uint32,
+                                               ctypes.c_uint64, _u32p, ctypes.c_int]),
             "orc_aes_sbox": (ctypes.c_uint8, [ctypes.c_uint8]),
             "orc_aes128_expand": (None, [_u8p, _u8p]),
             "orc_aes128_encrypt": (None, [_u8p, _u8p, _u8p]),
@@ -137,6 +147,18 @@ def nc_mul_fft(x, y):
     return out, err
 
 
+# the large-precision functions' small LWE modulus q and offset beta (this build's choice:
+# q = N, so that arbitrary functions fit the doubled modulus 2q = 2N; beta = q / 16, about 10
+# sigma of the bootstrap output noise at q; the plaintext space of EvalFunc is q / (2 beta) = 8)
+FUNC_Q = 1024
+FUNC_BETA = 64
+
+
+def func_property(lut, q):
+    """0 negacyclic, 1 periodic, 2 arbitrary (orc_func_property)."""
+    return int(lib().orc_func_property(np.ascontiguousarray(lut, np.uint32), int(q)))
+
+
 class GateKeys:
     """Gate-level key material generated by the oracle from a seed."""
 
@@ -183,6 +205,58 @@ class GateKeys:
                                        np.ascontiguousarray(ct1, np.uint32),
                                        np.ascontiguousarray(ct2, np.uint32), out)
         return out, err
+
+    # ---- programmable bootstrap and large-precision functions (func_oracle.c) ----
+    def encrypt_mod(self, value, mod, id_, seed=None):
+        """LWE over Z_mod (a power of two <= 2^32) of `value` (in Z_mod): the 2^32-torus
+        encryption of value * 2^32 / mod, every element rounded to Z_mod."""
+        p = self.p
+        ct = np.zeros(p.n + 1, np.uint32)
+        sc = (1 << 32) // int(mod)
+        lib().orc_lwe_encrypt32(self.seed if seed is None else seed, DOM_ENC, id_, p.n, self.s,
+                                (int(value) * sc) & 0xFFFFFFFF, p.lwe_nshift, ct)
+        if int(mod) == 1 << 32:
+            return ct
+        return (((ct.astype(np.uint64) + sc // 2) // sc) % int(mod)).astype(np.uint32)
+
+    def phase_mod(self, ct, mod):
+        ct = np.asarray(ct, np.uint64)
+        return int((int(ct[-1]) - int(np.dot(ct[:-1], self.s.astype(np.uint64)))) % int(mod))
+
+    def bootstrap_func(self, ct, ctmod, f, fmod):
+        out = np.zeros(self.p.n + 1, np.uint32)
+        rc = lib().orc_bootstrap_func(ctypes.byref(self.p), self.bsk_f, self.ksk, np.ascontiguousarray(ct, np.uint32),
+                                      int(ctmod), np.ascontiguousarray(f, np.uint32), int(fmod), out)
+        assert rc == 0
+        return out
+
+    def eval_func(self, ct, q, lut, beta=FUNC_BETA):
+        out = np.zeros(self.p.n + 1, np.uint32)
+        rc = lib().orc_eval_func(ctypes.byref(self.p), self.bsk_f, self.ksk, np.ascontiguousarray(ct, np.uint32),
+                                 int(q), np.ascontiguousarray(lut, np.uint32), beta, out)
+        assert rc == 0
+        return out
+
+    def eval_floor(self, ct, mod, roundbits=0, beta=FUNC_BETA, qsmall=FUNC_Q):
+        out = np.zeros(self.p.n + 1, np.uint32)
+        rc = lib().orc_eval_floor(ctypes.byref(self.p), self.bsk_f, self.ksk, np.ascontiguousarray(ct, np.uint32),
+                                  int(mod), beta, roundbits, qsmall, out)
+        assert rc == 0
+        return out
+
+    def eval_sign(self, ct, mod, scheme_switch=False, beta=FUNC_BETA, qsmall=FUNC_Q):
+        out = np.zeros(self.p.n + 1, np.uint32)
+        rc = lib().orc_eval_sign(ctypes.byref(self.p), self.bsk_f, self.ksk, np.ascontiguousarray(ct, np.uint32),
+                                 int(mod), beta, qsmall, int(bool(scheme_switch)), out)
+        assert rc == 0
+        return out
+
+    def eval_decomp(self, ct, mod, beta=FUNC_BETA, qsmall=FUNC_Q, max_digits=32):
+        out = np.zeros((max_digits, self.p.n + 1), np.uint32)
+        cnt = lib().orc_eval_decomp(ctypes.byref(self.p), self.bsk_f, self.ksk, np.ascontiguousarray(ct, np.uint32),
+                                    int(mod), beta, qsmall, out, max_digits)
+        assert cnt > 0
+        return out[:cnt]
 
 
 def aes_sbox(x):

@@ -99,6 +99,19 @@ double orc_gate_bootstrap(const orc_gate_params* p, const double* bsk_f, const u
 void orc_gen_ksk32_dom(uint64_t seed, uint32_t dom, const orc_gate_params* p, const int32_t* s,
                        const int32_t* K, uint32_t* ksk);
 
+/* programmable bootstrap and large-precision functions of the gate level, func_oracle.c */
+int orc_func_property(const uint32_t* lut, uint64_t q);
+int orc_bootstrap_func(const orc_gate_params* p, const double* bsk_f, const uint32_t* ksk, const uint32_t* ct,
+                       uint64_t ctmod, const uint32_t* f, uint64_t fmod, uint32_t* out);
+int orc_eval_func(const orc_gate_params* p, const double* bsk_f, const uint32_t* ksk, const uint32_t* ct,
+                  uint64_t q, const uint32_t* lut, uint32_t beta, uint32_t* out);
+int orc_eval_floor(const orc_gate_params* p, const double* bsk_f, const uint32_t* ksk, const uint32_t* ct,
+                   uint64_t mod, uint32_t beta, uint32_t roundbits, uint64_t qsmall, uint32_t* out);
+int orc_eval_sign(const orc_gate_params* p, const double* bsk_f, const uint32_t* ksk, const uint32_t* ct,
+                  uint64_t mod, uint32_t beta, uint64_t qsmall, int scheme_switch, uint32_t* out);
+int orc_eval_decomp(const orc_gate_params* p, const double* bsk_f, const uint32_t* ksk, const uint32_t* ct,
+                    uint64_t mod, uint32_t beta, uint64_t qsmall, uint32_t* out, int max_digits);
+
 /* ---- circuit-bootstrapping level (N = 2048; PBS torus-64, PFKS/RGSW/LUT torus-32), cbs_oracle.c ---- */
 typedef struct {
     int n, N;             /* 636, 2048 */
