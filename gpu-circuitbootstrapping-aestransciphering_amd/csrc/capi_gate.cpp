@@ -10,25 +10,10 @@
 #include "../../include/tfhe_amd.h"
 #include "capi_common.hpp"
 #include "device_common.hpp"
+#include "gate_ctx.hpp"
 #include "kernels.hpp"
 
 using namespace tfhe;
-
-struct tfhe_gate_ctx {
-    tfhe_gate_params p{};
-    uint32_t max_batch = 0;
-    bool has_keys = false;
-    std::vector<int32_t> lwe_sk, rlwe_sk;
-    cplx* d_bskf = nullptr;       // [n][2l][2][8][64]
-    uint32_t* d_ksk = nullptr;    // [N][ks_l][B][n+1]
-    uint32_t* d_lweN = nullptr;   // [max_batch][N+1]
-    uint32_t* d_ks_at = nullptr;  // [N][ks_bpad(max_batch)] key-switch digits scratch
-    // plain-form scratch
-    uint32_t* d_acc = nullptr;    // [2][N]
-    cplx* d_digf = nullptr;       // [2l][8][64]
-    uint32_t* d_t = nullptr;      // [2][N]
-    DeviceTables tables;
-};
 
 static bool gate_params_supported(const tfhe_gate_params* p) {
     return p && p->N == 1024 && p->bk_logB == 7 && p->bk_l == 3 && p->ks_logB == 2 && p->ks_l == 8 &&
@@ -96,6 +81,7 @@ extern "C" int tfhe_gate_ctx_destroy(tfhe_gate_ctx* c) {
     (void)hipFree(c->d_acc);
     (void)hipFree(c->d_digf);
     (void)hipFree(c->d_t);
+    gate_func_release(c);
     c->tables.release();
     delete c;
     return TFHE_OK;

@@ -135,10 +135,15 @@ __device__ __forceinline__ uint32_t gate_tv(uint32_t q1, uint32_t q2, uint32_t b
 // ===========================================================================
 constexpr int GATE_N = 1024, GATE_LOGB = 7, GATE_L = 3, GATE_ROWS = 6;
 
+// FUNC = true is the programmable bootstrap (reference binfhe-base-scheme.cu:259-378): one input
+// ciphertext over Z_ctmod (ctmod = 2N >> logf, a power of two), rotations a_i 2^logf (exact, no
+// rounding), test vector m[j 2^logf] = tv[(b - j) mod ctmod] for j < ctmod / 2 (tv already scaled
+// to the torus by the host) and zero elsewhere, no offset after extraction.
+template <bool FUNC>
 __global__ __launch_bounds__(256) void k_gate_blind_rotate(
     const uint32_t* __restrict__ ct1, const uint32_t* __restrict__ ct2, uint32_t batch, int n,
     int doubled, uint32_t q1, const cplx* __restrict__ bskf, FFTTables Tg,
-    uint32_t* __restrict__ lweN) {
+    uint32_t* __restrict__ lweN, int logf, const uint32_t* __restrict__ tv) {
     constexpr int N = GATE_N, M = N / 2;
     __shared__ cplx lds_all[4][FFT512_LDS_CPLX];
     __shared__ uint16_t at_all[4][1024];
@@ -158,24 +163,36 @@ __global__ __launch_bounds__(256) void k_gate_blind_rotate(
     cplx* lds = lds_all[w];
     uint16_t* at = at_all[w];
     const uint32_t* c1 = ct1 + (size_t)cidx * (n + 1);
-    const uint32_t* c2 = ct2 + (size_t)cidx * (n + 1);
     constexpr int LOG2N2 = 11;
-    for (int k = lane; k < n; k += 64) {
-        uint32_t v = c1[k] + c2[k];
-        if (doubled) v *= 2u;
-        at[k] = (uint16_t)modswitch_2N(v, LOG2N2);
-    }
-    uint32_t vb = c1[n] + c2[n];
-    if (doubled) vb *= 2u;
-    const uint32_t bt = modswitch_2N(vb, LOG2N2);
     const uint32_t q = 2u * N;
-    const uint32_t q2 = (q1 + N) & (q - 1);
-
     uint32_t acc[2][16];
+    if constexpr (FUNC) {
+        const uint32_t cmask = (q >> logf) - 1u;
+        for (int k = lane; k < n; k += 64) at[k] = (uint16_t)((c1[k] & cmask) << logf);
+        const uint32_t b = c1[n] & cmask;
+        const uint32_t fmask = (1u << logf) - 1u;
 #pragma unroll
-    for (int s = 0; s < 16; ++s) {
-        acc[0][s] = 0u;
-        acc[1][s] = gate_tv(q1, q2, bt, (uint32_t)(64 * s + lane), q);
+        for (int s = 0; s < 16; ++s) {
+            const uint32_t c = (uint32_t)(64 * s + lane);  // < N, so j = c >> logf < ctmod / 2
+            acc[0][s] = 0u;
+            acc[1][s] = (c & fmask) ? 0u : tv[(b - (c >> logf)) & cmask];
+        }
+    } else {
+        const uint32_t* c2 = ct2 + (size_t)cidx * (n + 1);
+        for (int k = lane; k < n; k += 64) {
+            uint32_t v = c1[k] + c2[k];
+            if (doubled) v *= 2u;
+            at[k] = (uint16_t)modswitch_2N(v, LOG2N2);
+        }
+        uint32_t vb = c1[n] + c2[n];
+        if (doubled) vb *= 2u;
+        const uint32_t bt = modswitch_2N(vb, LOG2N2);
+        const uint32_t q2 = (q1 + N) & (q - 1);
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            acc[0][s] = 0u;
+            acc[1][s] = gate_tv(q1, q2, bt, (uint32_t)(64 * s + lane), q);
+        }
     }
     wave_sync();
 
@@ -248,7 +265,7 @@ __global__ __launch_bounds__(256) void k_gate_blind_rotate(
         const uint32_t v = acc[0][s];
         out[(N - c) & (N - 1)] = c ? 0u - v : v;
     }
-    if (lane == 0) out[N] = acc[1][0] + (1u << 29) + 1u;
+    if (lane == 0) out[N] = FUNC ? acc[1][0] : acc[1][0] + (1u << 29) + 1u;
 }
 
 // ===========================================================================
@@ -505,7 +522,55 @@ void launch_to_fourier1024(const uint32_t* coef, size_t npoly, const FFTTables& 
 void launch_gate_blind_rotate(const uint32_t* ct1, const uint32_t* ct2, uint32_t batch, int n, int doubled,
                               uint32_t q1, const cplx* bskf, const FFTTables& T, uint32_t* lweN,
                               hipStream_t st) {
-    k_gate_blind_rotate<<<(batch + 3) / 4, 256, 0, st>>>(ct1, ct2, batch, n, doubled, q1, bskf, T, lweN);
+    k_gate_blind_rotate<false><<<(batch + 3) / 4, 256, 0, st>>>(ct1, ct2, batch, n, doubled, q1, bskf, T, lweN,
+                                                                  0, nullptr);
+}
+
+void launch_func_blind_rotate(const uint32_t* ct, uint32_t batch, int n, int logf, const uint32_t* tv,
+                              const cplx* bskf, const FFTTables& T, uint32_t* lweN, hipStream_t st) {
+    k_gate_blind_rotate<true><<<(batch + 3) / 4, 256, 0, st>>>(ct, nullptr, batch, n, 0, 0u, bskf, T, lweN, logf,
+                                                                 tv);
+}
+
+// LWE arithmetic over power-of-two moduli, element-wise on [batch][n+1] (reference
+// binfhe-base-scheme.cu:380-698 through the OpenFHE LWE operations): out = ((x & mx) - (y & my)
+// + [k = n] c) & mo, with y optional.
+__global__ __launch_bounds__(256) void k_lwe_axpb(const uint32_t* __restrict__ x, uint32_t mx,
+                                                 const uint32_t* __restrict__ y, uint32_t my, uint32_t c,
+                                                 uint32_t mo, int n, size_t count, uint32_t* __restrict__ out) {
+    const uint32_t w = (uint32_t)n + 1u;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += (size_t)gridDim.x * blockDim.x) {
+        uint32_t v = x[k] & mx;
+        if (y) v -= y[k] & my;
+        if (k % w == (uint32_t)n) v += c;
+        out[k] = v & mo;
+    }
+}
+
+// round(x 2^-shift) & mo on every element (the ModSwitch between power-of-two moduli)
+__global__ __launch_bounds__(256) void k_lwe_modswitch(const uint32_t* __restrict__ x, int shift, uint32_t mo,
+                                                      size_t count, uint32_t* __restrict__ out) {
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t v = x[k];
+        out[k] = (uint32_t)(shift ? (v + (1ull << (shift - 1))) >> shift : v) & mo;
+    }
+}
+
+static unsigned elementwise_blocks(size_t count) {
+    const size_t b = (count + 255) / 256;
+    return (unsigned)(b < 4096 ? (b ? b : 1) : 4096);
+}
+
+void launch_lwe_axpb(const uint32_t* x, uint32_t mx, const uint32_t* y, uint32_t my, uint32_t c, uint32_t mo, int n,
+                     uint32_t batch, uint32_t* out, hipStream_t st) {
+    const size_t count = (size_t)batch * (n + 1);
+    k_lwe_axpb<<<elementwise_blocks(count), 256, 0, st>>>(x, mx, y, my, c, mo, n, count, out);
+}
+
+void launch_lwe_modswitch(const uint32_t* x, int shift, uint32_t mo, int n, uint32_t batch, uint32_t* out,
+                          hipStream_t st) {
+    const size_t count = (size_t)batch * (n + 1);
+    k_lwe_modswitch<<<elementwise_blocks(count), 256, 0, st>>>(x, shift, mo, count, out);
 }
 
 void launch_plain_init(uint32_t* acc, uint32_t q1, uint32_t bt, hipStream_t st) {

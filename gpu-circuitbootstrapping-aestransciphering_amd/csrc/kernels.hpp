@@ -17,6 +17,15 @@ void launch_to_fourier1024(const uint32_t* coef, size_t npoly, const FFTTables& 
 void launch_gate_blind_rotate(const uint32_t* ct1, const uint32_t* ct2, uint32_t batch, int n, int doubled,
                               uint32_t q1, const cplx* bskf, const FFTTables& T, uint32_t* lweN,
                               hipStream_t st);
+// programmable bootstrap: ct over Z_(2N >> logf), tv [2N >> logf] torus-32 test-vector entries
+void launch_func_blind_rotate(const uint32_t* ct, uint32_t batch, int n, int logf, const uint32_t* tv,
+                              const cplx* bskf, const FFTTables& T, uint32_t* lweN, hipStream_t st);
+// out = ((x & mx) - (y & my) + [b] c) & mo on [batch][n+1]; y may be NULL
+void launch_lwe_axpb(const uint32_t* x, uint32_t mx, const uint32_t* y, uint32_t my, uint32_t c, uint32_t mo, int n,
+                     uint32_t batch, uint32_t* out, hipStream_t st);
+// out = round(x 2^-shift) & mo on [batch][n+1]
+void launch_lwe_modswitch(const uint32_t* x, int shift, uint32_t mo, int n, uint32_t batch, uint32_t* out,
+                          hipStream_t st);
 void launch_plain_init(uint32_t* acc, uint32_t q1, uint32_t bt, hipStream_t st);
 void launch_plain_step(uint32_t* acc, cplx* digf, uint32_t* t, const cplx* key, uint32_t m,
                        const FFTTables& T, hipStream_t st);

@@ -91,6 +91,18 @@ SIGNATURES = {
     "tfhe_gate_eval_plain": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]),
     "tfhe_gate_blind_rotate_batch": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_gate_keyswitch_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
+    "tfhe_func_property": (ctypes.c_int, [_u32p, ctypes.c_uint64]),
+    "tfhe_gate_bootstrap_func_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint64, _u32p,
+                                                      ctypes.c_uint64, _vp, _vp]),
+    "tfhe_gate_eval_func_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint64, _u32p, _vp, _vp]),
+    "tfhe_gate_eval_floor_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32,
+                                                  _vp, _vp]),
+    "tfhe_gate_eval_sign_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, _vp,
+                                                 _vp]),
+    "tfhe_gate_eval_decomp_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, _vp,
+                                                   ctypes.POINTER(ctypes.c_uint32),
+                                                   np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS"),
+                                                   _vp]),
     "tfhe_polymul1024_batch": (ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _vp, _vp]),
     "tfhe_gate_external_product": (ctypes.c_int, [_vp, ctypes.c_uint32, _vp, _vp, _vp]),
     "tfhe_gen_bsk32": (ctypes.c_int, [ctypes.POINTER(GateParams), ctypes.c_uint64, _vp, _vp, _vp, _vp]),
@@ -323,6 +335,59 @@ class GateContext:
     def keyswitch_batch(self, d_lweN, d_out, stream=None):
         _need(d_out, d_lweN.shape[0] * 4 * (self.p.n + 1), "keyswitch out")
         check(lib().tfhe_gate_keyswitch_batch(self.h, _ptr(d_lweN), d_lweN.shape[0], _ptr(d_out), _stream(stream)))
+
+    # ---- programmable bootstrap and large-precision functions ----
+    def _func_io(self, d_ct, d_out, batch, what, digits=1):
+        b = d_ct.shape[0] if batch is None else batch
+        row = 4 * (self.p.n + 1)
+        _need(d_ct, b * row, f"{what} ct")
+        _need(d_out, digits * b * row, f"{what} out")
+        return b
+
+    def bootstrap_func(self, d_ct, ctmod, lut, fmod, d_out, batch=None, stream=None):
+        """BootstrapFunc on a batch over Z_ctmod with a table over Z_fmod (lut: ctmod entries)."""
+        b = self._func_io(d_ct, d_out, batch, "bootstrap_func")
+        lut = np.ascontiguousarray(lut, np.uint32)
+        if lut.size != int(ctmod):
+            raise ValueError("bootstrap_func: the table needs ctmod entries")
+        check(lib().tfhe_gate_bootstrap_func_batch(self.h, _ptr(d_ct), b, int(ctmod), lut, int(fmod), _ptr(d_out),
+                                                   _stream(stream)))
+
+    def eval_func(self, d_ct, q, lut, d_out, batch=None, stream=None):
+        """EvalFunc on a batch over Z_q with a table of q entries over Z_q."""
+        b = self._func_io(d_ct, d_out, batch, "eval_func")
+        lut = np.ascontiguousarray(lut, np.uint32)
+        if lut.size != int(q):
+            raise ValueError("eval_func: the table needs q entries")
+        check(lib().tfhe_gate_eval_func_batch(self.h, _ptr(d_ct), b, int(q), lut, _ptr(d_out), _stream(stream)))
+
+    def eval_floor(self, d_ct, mod, d_out, roundbits=0, batch=None, stream=None):
+        b = self._func_io(d_ct, d_out, batch, "eval_floor")
+        check(lib().tfhe_gate_eval_floor_batch(self.h, _ptr(d_ct), b, int(mod), roundbits, _ptr(d_out),
+                                               _stream(stream)))
+
+    def eval_sign(self, d_ct, mod, d_out, scheme_switch=False, batch=None, stream=None):
+        b = self._func_io(d_ct, d_out, batch, "eval_sign")
+        check(lib().tfhe_gate_eval_sign_batch(self.h, _ptr(d_ct), b, int(mod), int(bool(scheme_switch)),
+                                              _ptr(d_out), _stream(stream)))
+
+    def eval_decomp(self, d_ct, mod, d_out, max_digits, batch=None, stream=None):
+        """EvalDecomp into d_out [max_digits][batch][n+1]; returns the digit moduli (one per digit)."""
+        b = self._func_io(d_ct, d_out, batch, "eval_decomp", digits=max_digits)
+        cnt = ctypes.c_uint32(0)
+        mods = np.zeros(max(int(max_digits), 1), np.uint64)
+        check(lib().tfhe_gate_eval_decomp_batch(self.h, _ptr(d_ct), b, int(mod), max_digits, _ptr(d_out),
+                                                ctypes.byref(cnt), mods, _stream(stream)))
+        return [int(m) for m in mods[:cnt.value]]
+
+
+def func_property(lut, q):
+    """0 negacyclic, 1 periodic, 2 arbitrary (tfhe_func_property)."""
+    return lib().tfhe_func_property(np.ascontiguousarray(lut, np.uint32), int(q))
+
+
+FUNC_Q = 1024
+FUNC_BETA = 64
 
 
 def polymul2048_batch(d_x, d_y, d_out, stream=None, approx=False):

@@ -114,6 +114,62 @@ int tfhe_gate_blind_rotate_batch(tfhe_gate_ctx* ctx, int gate, const uint32_t* d
 int tfhe_gate_keyswitch_batch(tfhe_gate_ctx* ctx, const uint32_t* d_lweN, uint32_t batch, uint32_t* d_out,
                               void* stream);
 
+/* ------------------------------------------------------------------------- */
+/* Programmable bootstrap and large-precision functions on the gate context  */
+/* (Liu-Micciancio-Polyakov, eprint 2021/1337).  LWE ciphertexts are         */
+/* [batch][n+1] uint32 over a power-of-two modulus (2 .. 2^32): residues     */
+/* under the same LWE key, a message m over Z_mod encrypted as the 2^32-torus */
+/* phase m 2^32 / mod rounded to Z_mod.  Every flow is a fixed sequence of   */
+/* launches on `stream` (nothing is read back); the flows share the context's */
+/* scratch, so calls on one context must not overlap (one stream at a time). */
+/* The table pointers are host memory, read before the call returns.         */
+/* ------------------------------------------------------------------------- */
+#define TFHE_FUNC_Q 1024u   /* q_small: the modulus EvalFloor/EvalSign/EvalDecomp work at */
+#define TFHE_FUNC_BETA 64u  /* beta: the offset against bootstrapping noise; q_small/(2 beta) = 8 levels per floor */
+enum { TFHE_FUNC_NEGACYCLIC = 0, TFHE_FUNC_PERIODIC = 1, TFHE_FUNC_ARBITRARY = 2 };
+
+/* The class of a table over Z_q (q a power of two): negacyclic when lut[x + q/2] = -lut[x]
+ * mod q, periodic when lut[x + q/2] = lut[x], arbitrary otherwise; -1 for a bad argument.
+ * The test EvalFunc makes (OpenFHE BinFHEScheme::checkInputFunction). */
+int tfhe_func_property(const uint32_t* lut, uint64_t q);
+
+/* BatchGPUBootstrapFunc(params, EK, v_ct, f, fmod) (binfhe-base-scheme-batch.cu:267-290,
+ * single form binfhe-base-scheme.cu:259-378): d_ct over Z_ctmod (ctmod a power of two <= 2N),
+ * lut [ctmod] over Z_fmod (fmod a power of two <= 2^32).  Test vector m[j 2N/ctmod] =
+ * lut[(b - j) mod ctmod] 2^32/fmod for j < ctmod/2; the output over Z_fmod encrypts
+ * lut[phase] when <a,s> mod ctmod < ctmod/2 and -lut[phase + ctmod/2] otherwise. */
+int tfhe_gate_bootstrap_func_batch(tfhe_gate_ctx* ctx, const uint32_t* d_ct, uint32_t batch, uint64_t ctmod,
+                                   const uint32_t* lut, uint64_t fmod, uint32_t* d_out, void* stream);
+
+/* BatchGPUEvalFunc(v_ct, LUT) (binfhecontext.cuh:70-72, binfhe-base-scheme.cu:380-481): d_ct
+ * and d_out over Z_q (q a power of two <= 2N; <= N when the table is arbitrary), lut [q] over
+ * Z_q; the output encrypts lut at the input phase plus beta.  Tables built as
+ * f(floor(x p / q)) q / p (OpenFHE GenerateLUTviaFunction) absorb that offset and the noise for
+ * inputs m q / p + e.  One bootstrap for a negacyclic table, two otherwise. */
+int tfhe_gate_eval_func_batch(tfhe_gate_ctx* ctx, const uint32_t* d_ct, uint32_t batch, uint64_t q,
+                              const uint32_t* lut, uint32_t* d_out, void* stream);
+
+/* BatchGPUEvalFloor(v_ct, roundbits) (binfhecontext.cuh:86-88, binfhe-base-scheme.cu:484-531):
+ * d_ct and d_out over Z_mod; clears the low part of the message modulo the inner modulus
+ * q = TFHE_FUNC_Q (roundbits 0) or TFHE_FUNC_BETA 2^(roundbits+1) (q <= min(2N, mod)). */
+int tfhe_gate_eval_floor_batch(tfhe_gate_ctx* ctx, const uint32_t* d_ct, uint32_t batch, uint64_t mod,
+                               uint32_t roundbits, uint32_t* d_out, void* stream);
+
+/* BatchGPUEvalSign(v_ct, schemeSwitch) (binfhecontext.cuh:102-104, binfhe-base-scheme.cu:533-608):
+ * d_ct over Z_mod (TFHE_FUNC_Q < mod <= 2^32), d_out over Z_TFHE_FUNC_Q encrypting 0 for a
+ * message below mod/2 and q_small/2 above (scheme_switch: q_small/4 and -q_small/4 instead). */
+int tfhe_gate_eval_sign_batch(tfhe_gate_ctx* ctx, const uint32_t* d_ct, uint32_t batch, uint64_t mod,
+                              int scheme_switch, uint32_t* d_out, void* stream);
+
+/* BatchGPUEvalDecomp(v_ct) (binfhecontext.cuh:116-118, binfhe-base-scheme.cu:633-698): digits
+ * of d_ct (over Z_mod, TFHE_FUNC_Q < mod <= 2^32) into d_out [digits][batch][n+1], digit k over
+ * Z_digit_mods[k] (TFHE_FUNC_Q for all but the last); *n_digits and digit_mods (host, optional,
+ * max_digits entries) are written before any launch.  TFHE_E_CAPACITY when max_digits is too
+ * small (mod 2^32 takes 9). */
+int tfhe_gate_eval_decomp_batch(tfhe_gate_ctx* ctx, const uint32_t* d_ct, uint32_t batch, uint64_t mod,
+                                uint32_t max_digits, uint32_t* d_out, uint32_t* n_digits, uint64_t* digit_mods,
+                                void* stream);
+
 /* Exact negacyclic products through the wavefront FP64 FFT, N = 1024:
  * d_out[k] = d_x[k] * d_y[k] mod (X^N + 1, 2^32) for small-integer d_x (|x| < 2^10)
  * and torus-32 d_y, all [batch][1024].  The FFT of the reference

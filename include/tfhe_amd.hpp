@@ -73,10 +73,12 @@ public:
     }
 };
 
-/* An LWE ciphertext on the 32-bit torus: a (n words) and b. */
+/* An LWE ciphertext: a (n words) and b over Z_modulus (a power of two; 2^32, the torus,
+ * for gate and circuit-bootstrapping inputs), as lbcrypto::LWECiphertext carries GetModulus(). */
 struct LWECiphertext {
     std::vector<uint32_t> a;
     uint32_t b = 0;
+    uint64_t modulus = 1ull << 32;
 };
 
 /* The build's parameter sets: gate bootstrapping n=636, N=1024, BSK 2^7 x 3,
@@ -112,6 +114,28 @@ inline uint32_t LWEDecrypt(const std::vector<int32_t>& sk, const LWECiphertext& 
     std::vector<uint32_t> ct(c.a);
     ct.push_back(c.b);
     return tfhe_lwe_decrypt((int)sk.size(), sk.data(), ct.data(), p);
+}
+
+/* m over Z_mod (a power of two <= 2^32): the torus encryption of m 2^32 / mod, every element
+ * rounded to Z_mod (the inputs of the programmable bootstrap and the large-precision functions). */
+inline LWECiphertext LWEEncryptMod(const std::vector<int32_t>& sk, uint64_t seed, uint32_t id, uint64_t m,
+                                   uint64_t mod, int nshift = 15) {
+    LWECiphertext c = LWEEncrypt(sk, seed, id, 0, 2, nshift);
+    const int sh = [&] { int l = 0; while ((1ull << l) < mod) ++l; return 32 - l; }();
+    c.b += (uint32_t)((m % mod) << sh);
+    if (sh) {
+        const uint64_t half = 1ull << (sh - 1), mask = mod - 1;
+        for (auto& x : c.a) x = (uint32_t)((((uint64_t)x + half) >> sh) & mask);
+        c.b = (uint32_t)((((uint64_t)c.b + half) >> sh) & mask);
+    }
+    c.modulus = mod;
+    return c;
+}
+/* b - <a, s> over Z_modulus */
+inline uint64_t LWEPhase(const std::vector<int32_t>& sk, const LWECiphertext& c) {
+    uint64_t ph = c.b;
+    for (size_t i = 0; i < c.a.size(); ++i) ph -= (uint64_t)c.a[i] * (uint64_t)(uint32_t)sk[i];
+    return ph & (c.modulus - 1);
 }
 
 namespace detail {
@@ -201,6 +225,102 @@ public:
     void BatchGPUEvalBinGateDevice(int gate, const uint32_t* d_ct1, const uint32_t* d_ct2, uint32_t batch,
                                    uint32_t* d_out, hipStream_t s = hipStreamPerThread) const {
         check(tfhe_gate_eval_batch(h_, gate, d_ct1, d_ct2, batch, d_out, s));
+    }
+
+    /* GPUEvalFunc(ct, LUT, stream) / BatchGPUEvalFunc(v_ct, LUT, stream) (binfhecontext.cuh:66-72):
+     * the table has one entry per element of Z_q, q the ciphertexts' modulus. */
+    LWECiphertext GPUEvalFunc(const LWECiphertext& ct, const std::vector<uint32_t>& LUT,
+                              hipStream_t s = hipStreamPerThread) const {
+        return BatchGPUEvalFunc({ct}, LUT, s)[0];
+    }
+    std::vector<LWECiphertext> BatchGPUEvalFunc(const std::vector<LWECiphertext>& v_ct,
+                                                const std::vector<uint32_t>& LUT,
+                                                hipStream_t s = hipStreamPerThread) const {
+        const uint64_t q = common_modulus(v_ct, "BatchGPUEvalFunc");
+        if (LUT.size() != q) throw Error(TFHE_E_SHAPE, "BatchGPUEvalFunc: the table needs one entry per element of Z_q");
+        return run(v_ct, 1, q, s, [&](const uint32_t* in, uint32_t B, uint32_t* out) {
+            check(tfhe_gate_eval_func_batch(h_, in, B, q, LUT.data(), out, s));
+        })[0];
+    }
+
+    /* GPUEvalFloor(ct, roundbits, stream) / BatchGPUEvalFloor (binfhecontext.cuh:82-88). */
+    LWECiphertext GPUEvalFloor(const LWECiphertext& ct, uint32_t roundbits = 0,
+                               hipStream_t s = hipStreamPerThread) const {
+        return BatchGPUEvalFloor({ct}, roundbits, s)[0];
+    }
+    std::vector<LWECiphertext> BatchGPUEvalFloor(const std::vector<LWECiphertext>& v_ct, uint32_t roundbits = 0,
+                                                 hipStream_t s = hipStreamPerThread) const {
+        const uint64_t mod = common_modulus(v_ct, "BatchGPUEvalFloor");
+        return run(v_ct, 1, mod, s, [&](const uint32_t* in, uint32_t B, uint32_t* out) {
+            check(tfhe_gate_eval_floor_batch(h_, in, B, mod, roundbits, out, s));
+        })[0];
+    }
+
+    /* GPUEvalSign(ct, schemeSwitch, stream) / BatchGPUEvalSign (binfhecontext.cuh:98-104): the
+     * outputs are over Z_TFHE_FUNC_Q. */
+    LWECiphertext GPUEvalSign(const LWECiphertext& ct, bool schemeSwitch = false,
+                              hipStream_t s = hipStreamPerThread) const {
+        return BatchGPUEvalSign({ct}, schemeSwitch, s)[0];
+    }
+    std::vector<LWECiphertext> BatchGPUEvalSign(const std::vector<LWECiphertext>& v_ct, bool schemeSwitch = false,
+                                                hipStream_t s = hipStreamPerThread) const {
+        const uint64_t mod = common_modulus(v_ct, "BatchGPUEvalSign");
+        return run(v_ct, 1, TFHE_FUNC_Q, s, [&](const uint32_t* in, uint32_t B, uint32_t* out) {
+            check(tfhe_gate_eval_sign_batch(h_, in, B, mod, schemeSwitch ? 1 : 0, out, s));
+        })[0];
+    }
+
+    /* GPUEvalDecomp(ct, stream) / BatchGPUEvalDecomp(v_ct, stream) (binfhecontext.cuh:113-118): the
+     * digits from the least significant up; the batch form returns them per ciphertext, as the
+     * reference's vector of vectors (binfhe-base-scheme-batch.cu:665-672). */
+    std::vector<LWECiphertext> GPUEvalDecomp(const LWECiphertext& ct, hipStream_t s = hipStreamPerThread) const {
+        return BatchGPUEvalDecomp({ct}, s)[0];
+    }
+    std::vector<std::vector<LWECiphertext>> BatchGPUEvalDecomp(const std::vector<LWECiphertext>& v_ct,
+                                                               hipStream_t s = hipStreamPerThread) const {
+        const uint64_t mod = common_modulus(v_ct, "BatchGPUEvalDecomp");
+        constexpr uint32_t kMax = 16;  // 2^32 takes 9
+        uint32_t nd = 0;
+        uint64_t mods[kMax] = {};
+        auto digits = run(v_ct, kMax, 0, s, [&](const uint32_t* in, uint32_t B, uint32_t* out) {
+            check(tfhe_gate_eval_decomp_batch(h_, in, B, mod, kMax, out, &nd, mods, s));
+        });
+        std::vector<std::vector<LWECiphertext>> ret(v_ct.size(), std::vector<LWECiphertext>(nd));
+        for (size_t k = 0; k < v_ct.size(); ++k)
+            for (uint32_t d = 0; d < nd; ++d) {
+                ret[k][d] = std::move(digits[d][k]);
+                ret[k][d].modulus = mods[d];
+            }
+        return ret;
+    }
+
+private:
+    static uint64_t common_modulus(const std::vector<LWECiphertext>& v, const char* what) {
+        if (v.empty()) throw Error(TFHE_E_SHAPE, std::string(what) + ": no ciphertexts");
+        for (const auto& c : v)
+            if (c.modulus != v[0].modulus) throw Error(TFHE_E_SHAPE, std::string(what) + ": ciphertexts of different moduli");
+        return v[0].modulus;
+    }
+    /* stage v through device memory, run f(d_in, batch, d_out[digits][batch][n+1]) and return
+     * the outputs per digit, each over out_mod (0: set by the caller) */
+    template <class F>
+    std::vector<std::vector<LWECiphertext>> run(const std::vector<LWECiphertext>& v, uint32_t digits,
+                                                uint64_t out_mod, hipStream_t s, F f) const {
+        if (v.size() > max_batch_) throw Error(TFHE_E_CAPACITY, "function call: batch exceeds the context");
+        const size_t w = (size_t)p_.n + 1, B = v.size();
+        std::vector<uint32_t> h = detail::pack(v, p_.n);
+        DeviceBuffer<uint32_t> din(B * w), dout((size_t)digits * B * w);
+        din.upload(h.data(), B * w, s);
+        f(din.get(), (uint32_t)B, dout.get());
+        std::vector<uint32_t> o((size_t)digits * B * w);
+        dout.download(o.data(), o.size(), s);
+        std::vector<std::vector<LWECiphertext>> ret(digits);
+        for (uint32_t d = 0; d < digits; ++d) {
+            std::vector<uint32_t> part(o.begin() + d * B * w, o.begin() + (d + 1) * B * w);
+            ret[d] = detail::unpack(part, B, p_.n);
+            for (auto& c : ret[d]) c.modulus = out_mod;
+        }
+        return ret;
     }
 };
 

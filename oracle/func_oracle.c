@@ -15,8 +15,8 @@
  * f(phase) or -f(phase + q/2) chosen by <a,s> mod q; EvalFunc on its three branches against the
  * plaintext table; EvalFloor against floor to a multiple of q; EvalSign against the top bit;
  * EvalDecomp by rebuilding the input phase from its digits.  The floor family is exact only
- * while (x + beta) mod q keeps several sigma of the bootstrap output noise (2^-8.2 of the
- * output modulus on these parameters) away from 0 and q/2; nearer, a floor moves by one level.
+ * while (x + beta) mod q keeps several sigma of the bootstrap output noise (2^-8.1 of the
+ * output modulus with the gate keys: BSK noise over 636 steps, then the key switch) away from 0 and q/2; nearer, a floor moves by one level.
  */
 #include <stdlib.h>
 #include <string.h>

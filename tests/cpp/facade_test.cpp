@@ -227,10 +227,67 @@ static int circuit_bootstrap_and_aes(const std::vector<int32_t>& sk) {
     return 0;
 }
 
+// EvalFunc, EvalFloor, EvalSign and EvalDecomp through the facade.  Inputs over Z_2^12 keep
+// (v + beta) mod q_small near q/4 or 3q/4, far from the floor's branch points 0 and q/2 against
+// the bootstrap noise (about 15 units at 2^12); the floor is checked to 6 sigma, the digits to
+// 6 sigma of the switch to Z_512 (its rounding adds about 5 units there, 40 at 2^12).
+static int functions(const std::vector<int32_t>& sk) {
+    GPUBinFHEContext cc(GateParams(), 64);
+    cc.GPUBTKeyGen(sk, 7);
+    const uint32_t q = TFHE_FUNC_Q, beta = TFHE_FUNC_BETA, p = q / (2 * beta);
+    const int F[8] = {1, 4, 7, 2, 5, 0, 3, 6};  // 3m + 1 mod 8: neither negacyclic nor periodic
+    std::vector<uint32_t> lut(q);
+    for (uint32_t x = 0; x < q; ++x) lut[x] = (uint32_t)F[x * p / q] * (q / p);
+    EXPECT(tfhe_func_property(lut.data(), q) == TFHE_FUNC_ARBITRARY, "table class");
+    std::vector<LWECiphertext> in;
+    for (uint32_t m = 0; m < p; ++m) in.push_back(LWEEncryptMod(sk, 21, m, m * (q / p), q));
+    const std::vector<LWECiphertext> fo = cc.BatchGPUEvalFunc(in, lut);
+    for (uint32_t m = 0; m < p; ++m) {
+        EXPECT(fo[m].modulus == q, "EvalFunc output modulus");
+        const uint64_t ph = LWEPhase(sk, fo[m]);
+        EXPECT(((ph + q / (2 * p)) / (q / p)) % p == (uint64_t)F[m], "BatchGPUEvalFunc m=%u", m);
+    }
+    {
+        const LWECiphertext one = cc.GPUEvalFunc(in[3], lut);
+        EXPECT(one.a == fo[3].a && one.b == fo[3].b, "GPUEvalFunc equals the batch form");
+    }
+    std::printf("BatchGPUEvalFunc: 8 messages of Z_8 ok\n");
+
+    const uint64_t mod = 1u << 12;
+    std::vector<LWECiphertext> v;
+    std::vector<uint64_t> val;
+    for (uint32_t k = 0; k < 8; ++k) {
+        const uint64_t y = (k & 1) ? 3 * q / 4 : q / 4;  // (v + beta) mod q
+        val.push_back(((k >> 1) * q + y - beta + 17 * k) % mod);
+        v.push_back(LWEEncryptMod(sk, 23, k, val.back(), mod));
+    }
+    const auto fl = cc.BatchGPUEvalFloor(v);
+    const auto sg = cc.BatchGPUEvalSign(v);
+    const auto dc = cc.BatchGPUEvalDecomp(v);
+    for (uint32_t k = 0; k < 8; ++k) {
+        const uint64_t want = ((val[k] + beta) / q * q) % mod;
+        const int64_t e = (int64_t)((LWEPhase(sk, fl[k]) - want + mod / 2) % mod) - (int64_t)(mod / 2);
+        EXPECT(e > -128 && e < 128, "BatchGPUEvalFloor k=%u: error %lld", k, (long long)e);
+        EXPECT(sg[k].modulus == q, "EvalSign output modulus");
+        const uint64_t sph = LWEPhase(sk, sg[k]);
+        EXPECT(((sph + q / 4) / (q / 2)) % 2 == val[k] / (mod / 2), "BatchGPUEvalSign k=%u", k);
+        EXPECT(dc[k].size() == 2 && dc[k][0].modulus == q && dc[k][1].modulus == 512, "EvalDecomp digits");
+        const uint64_t x = (8 * LWEPhase(sk, dc[k][1]) + (LWEPhase(sk, dc[k][0]) + beta) % q - beta) % mod;
+        const int64_t d = (int64_t)((x - val[k] + mod / 2) % mod) - (int64_t)(mod / 2);
+        EXPECT(d > -256 && d < 256, "BatchGPUEvalDecomp k=%u: error %lld", k, (long long)d);
+    }
+    std::printf("BatchGPUEvalFloor / Sign / Decomp over Z_2^12: 8 ciphertexts ok\n");
+    if (expect_throw([&] { (void)cc.GPUEvalSign(LWEEncryptMod(sk, 1, 1, 1, q, 15)); }, TFHE_E_PARAM,
+                     "EvalSign of a small-precision ciphertext"))
+        return 1;
+    return 0;
+}
+
 int main() {
     try {
         const std::vector<int32_t> sk = LWEKeyGen(636, 3);
         if (gates(sk)) return 1;
+        if (functions(sk)) return 1;
         if (circuit_bootstrap_and_aes(sk)) return 1;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "FAIL: unexpected exception: %s\n", e.what());

@@ -1,7 +1,7 @@
 """The C++ interface (include/tfhe_amd.hpp: GPUBinFHEContext, GPUCirBTSContext with the
 reference's method names) compiles with a plain host compiler, and its test program
 (tests/cpp/facade_test.cpp) passes on the GPU: gate truth tables, batched gates,
-circuit bootstraps from device LWEs in the reference's layout, AES-128 FIPS-197 C.1."""
+EvalFunc / EvalFloor / EvalSign / EvalDecomp, circuit bootstraps from device LWEs in the reference's layout, AES-128 FIPS-197 C.1."""
 import os
 import subprocess
 

@@ -112,8 +112,8 @@ def test_eval_decomp(gk):
     floor(x) = x + beta - ((x + beta) mod q) and x_1 = floor(x_0) / 8, the digits give the input
     phase back through x_0 = 8 x_1 + ((d_0 + beta) mod q) - beta within the bootstrap noise.
     The inputs keep (x + beta) mod q away from the floor's branch points 0 and q / 2 by more
-    than 8 sigma of the bootstrap output noise at 2^13 (2^-8.2 of the modulus, from the key
-    switch): nearer, the floor moves by one level (q), which the gate parameters' key-switch
+    than 8 sigma of the bootstrap output noise at 2^13 (2^-8.1 of the modulus, from the blind
+    rotation and the key switch): nearer, the floor moves by one level (q), which the gate parameters' key-switch
     noise allows at large moduli."""
     mod = 2 ** 13
     rng = np.random.default_rng(11)
