@@ -752,10 +752,10 @@ static int check_cbs(const tfhe_cbs_ctx* c, uint32_t batch, uint32_t cap) {
 
 // pbs_multibit = 3 without a tuned choice: two key bits per step at every size (with the key
 // loads batched it measured faster than one bit at 128, 2048 and 8192 inputs: 5.9 / 34.4 /
-// 136 ms against 8.1 / 44.4 / 178), in the wide form (3) for launches of at most one
-// workgroup per CU, else at two waves per SIMD (2).  Forms 0 and 1 are one bit per step
-// with that FFT form.
-static int auto_pbs_model(uint32_t batch) { return pbs_model_form(batch, 1, 2) == 1 ? 3 : 2; }
+// 136 ms against 8.1 / 44.4 / 178), split over 8 waves (4) for launches of at most one
+// workgroup per CU, else at two waves per SIMD (2); 3 is the wide key batch.  Forms 0 and 1
+// are one bit per step with that FFT form.
+static int auto_pbs_model(uint32_t batch) { return 2 + pbs_model_form(batch, 1, 2); }
 
 // The HomTrace form's PBS: the accumulator (0, LUT X^-b~) with its test polynomial, b + 2^30
 // switched with bw = 1, the 2^10 gadget, written whole (no offsets) into rlwe [batch][2][N]
@@ -1129,7 +1129,7 @@ static int drop_graphs(tfhe_cbs_ctx* c, const char* what) {
 
 extern "C" int tfhe_cbs_set_pbs_form(tfhe_cbs_ctx* c, uint32_t inputs, int form) {
     if (!c) return fail(TFHE_E_PARAM, "set_pbs_form: NULL context");
-    const int nforms = pbs_auto(c->p) ? 4 : 2;
+    const int nforms = pbs_auto(c->p) ? 5 : pbs_mb(c->p) == 2 ? 3 : 2;
     if (form >= nforms) return fail(TFHE_E_PARAM, "set_pbs_form: no such PBS form for this context");
     const int rc = drop_graphs(c, "set_pbs_form sync");
     if (rc) return rc;
@@ -1185,10 +1185,12 @@ extern "C" int tfhe_cbs_autotune(tfhe_cbs_ctx* c, uint32_t blocks, int* choices,
         const int model_form = pbs_auto(c->p) ? auto_pbs_model(batch) : pbs_model_form(batch, limbs, mb);
         int form = model_form;
         if (model_form >= 0) {
-            // auto contexts: 2 / 3 = two key bits per step (two waves per SIMD / wide),
-            // 0 / 1 = one bit with that FFT form
+            // auto contexts: 2 / 3 / 4 = two key bits per step (two waves per SIMD / wide /
+            // split over 8 waves), 0 / 1 = one bit with that FFT form
             std::vector<int> cand{model_form};
-            for (int f : pbs_auto(c->p) ? std::vector<int>{3, 2, 1, 0} : std::vector<int>{1 - model_form})
+            for (int f : pbs_auto(c->p) ? std::vector<int>{4, 3, 2, 1, 0}
+                         : mb == 2        ? std::vector<int>{2, 1, 0}
+                                          : std::vector<int>{1 - model_form})
                 if (f != model_form) cand.push_back(f);
             const cplx* bsk2 = c->d_bsk2f + bsk2_mb2_first(c->p) * 8 * 2 * 1024;
             rc = tune_one(st, e0, e1, cand, budget_ms,

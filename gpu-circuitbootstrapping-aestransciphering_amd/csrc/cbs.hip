@@ -702,6 +702,188 @@ __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylu
     }
 }
 
+// Two key bits per step with one ciphertext per 8-wave workgroup (the one-product form):
+// wave group g (waves 4g .. 4g + 3) owns accumulator polynomial g.  Per step each group
+// decomposes its polynomial into the digit rows r = 2j + g, transforms them and sums both
+// outputs' products over its rows; the partial sum of the other group's output goes through
+// LDS, and each group inverse-transforms its own output and updates its polynomial.  A wave
+// runs half the forward transforms and MACs of the 4-wave form per step, and a CU holds two
+// waves per SIMD with one workgroup: the form for launches of at most one workgroup per CU,
+// where the 4-wave forms leave a wave per SIMD.  Each output's sum is the two groups' partial
+// sums added once, where the 4-wave form runs one sum over all eight rows, so the words differ
+// from that form's by FP64 rounding (the phases agree within the one-product form's bound).
+template <bool LUT>
+__global__ __launch_bounds__(512, 1) void k_pbs2_split(const uint32_t* __restrict__ lwe, uint32_t batch, int n,
+                                                      const cplx* __restrict__ bskf, FFT1024Tables Tg,
+                                                      uint64_t* __restrict__ out, PbsLut io) {
+    __shared__ cplx tl[768];
+    __shared__ cplx lds_all[8][F256_LDS_CPLX];
+    __shared__ cplx xbuf[2][2][1024];  // [group][buffer]
+    __shared__ uint16_t at[1024];
+    const FFT1024Tables T = stage_tables(Tg, tl);
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int g = wv >> 2, w = wv & 3;
+    cplx trw[4];
+    load_trw(Tg, w, lane, trw);
+    cplx* lds = lds_all[wv];
+    const uint64_t OFF = pbs_digit_offset<PBS_LOGB>();
+    constexpr uint32_t N = CB_N, Q2 = 2 * CB_N;
+    const uint64_t mu0 = 1ull << 47, mu1 = 1ull << 55;
+    uint32_t ek[4];
+    const uint32_t kq = (uint32_t)((lane >> 4) + 4 * ((lane >> 2) & 3) + 16 * (lane & 3) + 256 * w);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) ek[s] = (1u - 4u * (kq + 64u * (uint32_t)s)) & (Q2 - 1);
+
+    for (uint32_t cidx = blockIdx.x; cidx < batch; cidx += gridDim.x) {
+        const uint32_t* ct = lwe + (size_t)cidx * (n + 1);
+        __syncthreads();
+        uint32_t bt;
+        if constexpr (LUT) {
+            for (int k = threadIdx.x; k < n; k += blockDim.x)
+                at[k] = (uint16_t)((Q2 - special_ms_2048_bw(ct[k], io.bw)) & (Q2 - 1));
+            bt = special_ms_2048_bw(ct[n] + io.b_add, io.bw);
+        } else {
+            for (int k = threadIdx.x; k < n; k += blockDim.x) at[k] = (uint16_t)special_ms_2048(ct[k]);
+            bt = special_ms_2048(ct[n] + (1u << 30));
+        }
+        __syncthreads();
+
+        uint64_t acc[8];  // polynomial g
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+            const uint32_t c = (uint32_t)own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0);
+            uint64_t v;
+            if constexpr (LUT) {
+                const uint32_t m = (c + bt) & (Q2 - 1);
+                v = io.lut[m & (N - 1)];
+                if (m >= N) v = 0 - v;
+            } else {
+                const uint32_t arg = (bt + Q2 - (c & ~1u)) & (Q2 - 1);
+                v = 0 - ((c & 1) ? mu1 : mu0);
+                if (arg >= N) v = 0 - v;
+            }
+            acc[h] = g ? v : 0;
+        }
+        int buf = 0;
+        for (int i = 0; i < n; i += 2) {
+            const uint32_t mrot = (Q2 - (uint32_t)at[i]) & (Q2 - 1);
+            const uint32_t mrot2 = (Q2 - (uint32_t)at[i + 1]) & (Q2 - 1);
+            if ((mrot | mrot2) == 0) continue;
+            const cplx* key = bskf + (size_t)(i / 2) * 3 * PBS_ROWS * 2 * CB_M;
+            cplx fp[3][4];  // z1 - 1, z1 z2 - 1, z2 - 1 (patterns (1,0), (1,1), (0,1))
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const cplx z1 = T.rot[(mrot * ek[s]) & (Q2 - 1)];
+                const cplx z2 = T.rot[(mrot2 * ek[s]) & (Q2 - 1)];
+                const cplx z12 = cmul(z1, z2);
+                fp[0][s] = {z1.re - 1.0, z1.im};
+                fp[1][s] = {z12.re - 1.0, z12.im};
+                fp[2][s] = {z2.re - 1.0, z2.im};
+            }
+            // sums for this group's output (mown) and the other group's (moth); the key rows
+            // of each are addressed by offset, so no array is indexed by the group
+            cplx mown[4], moth[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) mown[s] = moth[s] = {0.0, 0.0};
+            const int own_off = g * CB_M, oth_off = (1 - g) * CB_M;
+            uint64_t tp[8];
+#pragma unroll
+            for (int h = 0; h < 8; ++h) tp[h] = acc[h] + OFF;
+#pragma unroll
+            for (int j = 0; j < PBS_L; ++j) {
+                const int r = 2 * j + g;
+                cplx x[4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) x[a] = {pbs_digit<PBS_LOGB>(tp[a], j), pbs_digit<PBS_LOGB>(tp[a + 4], j)};
+                f1024_fwd<true>(x, lds, xbuf[g][buf], T, trw, w, lane);
+                buf ^= 1;
+                cplx y[4];
+                auto load_keys = [&](cplx(&kv)[2][4], int q) {
+                    const int pt = q == 0 ? 0 : q == 1 ? 2 : 1;
+                    const cplx* kr = key + ((size_t)pt * PBS_ROWS + r) * 2 * CB_M + (w * 4) * 64 + lane;
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        kv[0][s] = kr[own_off + s * 64];
+                        kv[1][s] = kr[oth_off + s * 64];
+                    }
+                };
+                auto close_batch = [](cplx(&kv)[2][4]) {
+#pragma unroll
+                    for (int o = 0; o < 2; ++o)
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) asm volatile("" : "+v"(kv[o][k].re), "+v"(kv[o][k].im));
+                };
+                auto mac_pattern = [&](const cplx(&kv)[2][4], int q) {
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) y[s] = cmul(x[s], fp[q][s]);
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        cmac(mown[s], y[s], kv[0][s]);
+                        cmac(moth[s], y[s], kv[1][s]);
+                    }
+                };
+                cplx ka[2][4], kb[2][4];
+                load_keys(ka, 0);
+                close_batch(ka);
+                load_keys(kb, 1);
+                close_batch(kb);
+                mac_pattern(ka, 0);
+                load_keys(ka, 2);
+                close_batch(ka);
+                mac_pattern(kb, 1);
+                mac_pattern(ka, 2);
+            }
+            // the other group's output: this group's partial through LDS (the buffer the
+            // third transform used; the fourth transform's barrier ended its reads)
+            cplx* ex = xbuf[g][buf];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) ex[(w * 4 + s) * 64 + lane] = moth[s];
+            __syncthreads();
+            const cplx* oth = xbuf[1 - g][buf];
+            cplx y0[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                y0[s] = cadd(mown[s], oth[(w * 4 + s) * 64 + lane]);
+            }
+            buf ^= 1;
+            // the exchange buffer is read by the other group until that group's inverse
+            // transform reaches its barrier: this transform uses the other buffer
+            f1024_inv<true>(y0, lds, xbuf[g][buf], T, trw, w, lane);
+            buf ^= 1;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                acc[a] += torus64_from_double(y0[a].re);
+                acc[a + 4] += torus64_from_double(y0[a].im);
+            }
+        }
+        if constexpr (LUT) {
+            uint64_t* r = io.rlwe + (size_t)cidx * 2 * N;
+            const uint64_t o0 = io.offsets ? mu0 : 0, o1 = io.offsets ? mu1 : 0;
+#pragma unroll
+            for (int h = 0; h < 8; ++h) {
+                const uint32_t c = (uint32_t)own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0);
+                r[g * N + c] = acc[h] + (g && c == 0 ? o0 : g && c == 1 ? o1 : 0);
+            }
+            continue;
+        }
+        uint64_t* o0 = out + (size_t)cidx * 2 * (N + 1);
+        uint64_t* o1 = o0 + (N + 1);
+        if (g == 0) {  // sample extraction of coefficients 0 and 1 from the mask polynomial
+#pragma unroll
+            for (int h = 0; h < 8; ++h) {
+                const uint32_t c = (uint32_t)own_coeff(w, lane, h & 3) + (h >= 4 ? CB_M : 0);
+                const uint64_t v = acc[h];
+                o0[(0u - c) & (N - 1)] = (c == 0) ? v : 0 - v;
+                o1[(1u - c) & (N - 1)] = (c <= 1) ? v : 0 - v;
+            }
+        } else {  // bodies: coefficient 0 sits in wave 0, coefficient 1 in wave 1 (lane 0, h = 0)
+            if (lane == 0 && w == 0) o0[N] = acc[0] + mu0;
+            if (lane == 0 && w == 1) o1[N] = acc[0] + mu1;
+        }
+    }
+}
+
 // ===========================================================================
 // HomTrace + SchemeSwitch LWE -> RGSW (the reference's conversion, cirbts-batch.cu:
 // 1612-1790; trace round kernel.cu:7832-7888, scheme switch 8407-8493, row save
@@ -1549,10 +1731,12 @@ int pbs_model_form(uint32_t batch, int limbs, int mb) {
     // which pays only up to 256 ciphertexts; beyond, the branchy form keeps two waves
     // per SIMD.  One limb: the branch-free form fits two waves per SIMD (220
     // registers) and was the faster one at every size measured.  Two key bits per
-    // step (branch-free FFT): 1 = the wide key batch at one wave per SIMD, for
-    // launches of at most one workgroup per CU (128 inputs 5.9 against 6.6 ms), else
-    // 0 = two waves per SIMD (2048 inputs 34.4 against 48.0 ms).
-    if (limbs == 2 || mb == 2) return batch <= 256 ? 1 : 0;
+    // step (branch-free FFT): 2 = split over 8 waves, for launches of at most one
+    // workgroup per CU (128 inputs 4.67 ms against 6.20 wide and 6.47 at two waves per
+    // SIMD; 256 inputs 5.08 against 6.22), else 0 = two waves per SIMD (2176 inputs 37.9
+    // against 43.8 split and 54.7 wide); 1 = the wide key batch at one wave per SIMD.
+    if (mb == 2) return batch <= 256 ? 2 : 0;
+    if (limbs == 2) return batch <= 256 ? 1 : 0;
     return 1;
 }
 
@@ -1561,7 +1745,7 @@ static void launch_pbs2_t(const uint32_t* lwe, uint32_t batch, int n, const cplx
                           const FFT1024Tables& T, uint64_t* out, hipStream_t st, int form, const PbsLut& io) {
     // one workgroup per ciphertext; form: 1 = branch-free cross-wave FFT step,
     // 0 = branchy (two key bits per step: 1 = wide key batch, 0 = two waves per
-    // SIMD), < 0 = pbs_model_form's choice
+    // SIMD, 2 = split over 8 waves), < 0 = pbs_model_form's choice
     const unsigned big = grid_for(batch, 2048);
     if (form < 0) form = pbs_model_form(batch, limbs, mb);
     if constexpr (LUT) {
@@ -1588,6 +1772,8 @@ static void launch_pbs2_t(const uint32_t* lwe, uint32_t batch, int n, const cplx
     } else if (form == 1) {
         k_pbs2_manylut<true, 1, 2, true, LUT><<<batch <= 256 ? batch : big, 256, 0, st>>>(lwe, batch, n, bskf, T, out,
                                                                                           io);
+    } else if (form == 2) {
+        k_pbs2_split<LUT><<<batch <= 256 ? batch : big, 512, 0, st>>>(lwe, batch, n, bskf, T, out, io);
     } else {
         k_pbs2_manylut<true, 1, 2, false, LUT><<<big, 256, 0, st>>>(lwe, batch, n, bskf, T, out, io);
     }

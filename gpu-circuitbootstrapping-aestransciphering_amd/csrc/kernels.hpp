@@ -71,7 +71,8 @@ void launch_to_fourier2048_t64(const uint64_t* coef, size_t npoly, const FFT1024
 // limbs = 2: exact two-limb BSK2 rows (launch_to_fourier2048); 1: one torus-scaled row (launch_to_fourier2048_t64)
 // mb = 2 (limbs = 1, n even): two key bits per step with the pattern keys of launch_gen_bsk64(mb = 2)
 // form: 1 = branch-free cross-wave FFT step, 0 = branchy (two key bits per step: 1 = wide key
-// batch at one wave per SIMD, 0 = two waves per SIMD), < 0 = pbs_model_form's choice
+// batch at one wave per SIMD, 0 = two waves per SIMD, 2 = split over 8 waves), < 0 =
+// pbs_model_form's choice
 int pbs_model_form(uint32_t batch, int limbs, int mb);
 // Test polynomial and output of the multi-value PBS.  lut == NULL (default): the circuit
 // bootstrap's two-level test vector, b + 2^30 switched with bw = 1, LWE outputs of

@@ -415,8 +415,8 @@ int tfhe_cbs_keyswitch_batch(tfhe_cbs_ctx* ctx, const uint32_t* d_lwe32, uint32_
  * round over `blocks` blocks and keeps the fastest in the context for later calls
  * of those sizes.  choices[5] receives the PBS form (0 branchy, 1 branch-free FFT step;
  * with two key bits per step 1 = the wide key batch at one wave per SIMD, 0 = two
- * waves per SIMD; in a pbs_multibit = 3 context 0 / 1 are one bit per step and
- * 2 / 3 two bits at two waves per SIMD / wide) of the
+ * waves per SIMD, 2 = split over 8 waves; in a pbs_multibit = 3 context 0 / 1 are one
+ * bit per step and 2 / 3 / 4 two bits at two waves per SIMD / wide / split) of the
  * full-round part and of the rest of tfhe_cbs_batch's split,
  * the PFKS i-slices of the same two parts, and the N=2048 key switch's i-slices;
  * -1 where a part does not exist.  A part of 16 or more rounds of resident
@@ -432,8 +432,9 @@ int tfhe_cbs_autotune(tfhe_cbs_ctx* ctx, uint32_t blocks, int* choices, void* st
  * restore a stored autotune result without timing again.  No counterpart in the
  * reference (its shapes are compile-time).  TFHE_E_PARAM for a form the context's PBS
  * does not have.  Drops captured keystream graphs.  The FFT and key-batch forms of one
- * PBS give the same words; in a pbs_multibit = 3 context, choosing one or two key bits
- * per step changes them (both decrypt within the PBS noise). */
+ * PBS give the same words, except the split form, whose two wave groups' partial sums
+ * are added once (words differ by FP64 rounding); in a pbs_multibit = 3 context, choosing
+ * one or two key bits per step changes them too (all decrypt within the PBS noise). */
 int tfhe_cbs_set_pbs_form(tfhe_cbs_ctx* ctx, uint32_t inputs, int form);
 
 /* Kernel timing of the multi-value PBS (the reference prints per-block

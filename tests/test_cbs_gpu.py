@@ -523,9 +523,9 @@ def test_aes_plain_form_equals_batched(ctx, okeys):
     assert _decrypt_blocks(okeys, from_dev(d_plain), 1) == [O.aes128_encrypt(key, ctr[0])]
 
 
-def _check_choices(ch):
+def _check_choices(ch, forms=(0, 1)):
     for k in range(2):
-        assert ch["pbs_form"][k] in (-1, 0, 1)  # an exact-PBS context: no two-bit forms
+        assert ch["pbs_form"][k] == -1 or ch["pbs_form"][k] in forms  # the forms the context's PBS has
         assert ch["pfks_slices"][k] == -1 or 4 <= ch["pfks_slices"][k] <= 32
     assert ch["pfks_slices"][1] >= 4  # the whole batch or its tail always exists
     assert 1 <= ch["ks_slices"] <= 64
@@ -614,8 +614,9 @@ def test_pbs_approx_within_noise_bound(actx, okeys):
     1.5-1.9 (two bits); asserted: below 2.2 and above 1/1.5, with the GPU's rms
     below 2^37 (2^-27 of the torus).  The per-step agreement that pins the pattern
     factors and key rows is test_pbs_few_steps_phase_matches_oracle.  Two bits: the
-    4-input wide launch equals the first 4 outputs of the 300-input launch bit for
-    bit."""
+    4-input launch runs split over 8 waves (the size model's form for launches of at most
+    one workgroup per CU), inside the same bounds; the 4-input wide launch equals the
+    first 4 outputs of the 300-input launch bit for bit."""
     import torch
     import tfhe_amd
     mb = actx.p.pbs_multibit
@@ -646,7 +647,12 @@ def test_pbs_approx_within_noise_bound(actx, okeys):
     assert rms(gpu_e) < 2 ** 37
     assert rms(gpu_e) < 2.2 * rms(ora_e) and rms(ora_e) < 1.5 * rms(gpu_e)
     if mb == 2:
-        assert np.array_equal(small, big[:4])
+        actx.set_pbs_form(4, 1)  # the wide key batch: the same sums as two waves per SIMD
+        d_wide = torch.empty((4, 2, 2049), dtype=torch.int64, device="cuda")
+        actx.pbs_batch(d_cts[:4].contiguous(), d_wide)
+        torch.cuda.synchronize()
+        actx.set_pbs_form(4, -1)
+        assert np.array_equal(from_dev(d_wide, np.uint64), big[:4])
 
 
 def test_pbs_few_steps_phase_matches_oracle(actx, okeys):
@@ -659,8 +665,9 @@ def test_pbs_few_steps_phase_matches_oracle(actx, okeys):
     error of 2^-38.4 per mask coefficient through K2's N/2 ones), up to 6 key bits here:
     rms 2^32.9, max 2^34.6 over 192 phases.  Asserted: every phase within 2^36 of 2^64
     (2^-28 of the torus; beyond 8 sigma of 6 steps).  A wrong pattern factor, key row or
-    key pair would miss by at least a gadget level, 2^44 and more.  Both launch forms
-    (96 inputs, and 4 in the small-launch form)."""
+    key pair would miss by at least a gadget level, 2^44 and more.  Every launch form: 96
+    inputs in each form of the context (one bit: branchy and branch-free FFT steps; two bits:
+    two waves per SIMD, wide, split over 8 waves), and 4 in the size model's form."""
     import torch
     mb = actx.p.pbs_multibit
     n = okeys.p.n
@@ -676,11 +683,13 @@ def test_pbs_few_steps_phase_matches_oracle(actx, okeys):
         cts[k, n] = rng.integers(0, 1 << 32)
     d_cts = to_dev32(cts)
     outs = []
-    for b in (batch, 4):
+    for b, form in [(batch, f) for f in ((0, 1, 2) if mb == 2 else (0, 1))] + [(4, -1)]:
+        actx.set_pbs_form(b, form)
         d_o = torch.empty((b, 2, 2049), dtype=torch.int64, device="cuda")
         actx.pbs_batch(d_cts[:b].contiguous(), d_o)
         torch.cuda.synchronize()
-        outs.append(from_dev(d_o, np.uint64))
+        actx.set_pbs_form(b, -1)
+        outs.append((from_dev(d_o, np.uint64), b))
     exact = _oracle_pbs_many(okeys, cts, mb)
     K = okeys.K2.astype(np.int64).astype(np.uint64)
 
@@ -688,7 +697,7 @@ def test_pbs_few_steps_phase_matches_oracle(actx, okeys):
         return [(int(o[j, -1]) - int(np.dot(o[j, :-1], K))) % 2 ** 64 for j in range(2)]
 
     worst = 0
-    for got, cnt in ((outs[0], batch), (outs[1], 4)):
+    for got, cnt in outs:
         for k in range(cnt):
             for pg, pe in zip(phases(got[k]), phases(exact[k])):
                 d = (pg - pe) % 2 ** 64
@@ -832,7 +841,7 @@ def test_aes128_approx_pbs_form(actx, okeys):
         assert np.array_equal(from_dev(d_graph), got)
     # tuned shapes (the one-product PBS may change FFT form, which moves rounding
     # but not decryption): the same blocks again decrypt to the same AES output
-    _check_choices(actx.autotune(2))
+    _check_choices(actx.autotune(2), (0, 1, 2) if actx.p.pbs_multibit == 2 else (0, 1))
     d_tuned = torch.empty_like(d_ks)
     actx.aes_ctr_keystream(blocks, d_tuned)
     torch.cuda.synchronize()
@@ -1031,12 +1040,12 @@ def test_aes128_linear_pfks(lctx, okeys):
 
 
 def test_pbs_auto_multibit_matches_fixed_forms(gpu, okeys):
-    """Two key bits per step: the wide form (a 128-input launch, one wave per SIMD and
-    all pattern keys in one batch) and the two-waves-per-SIMD form (8192 inputs) give
-    the same words on the same inputs.  pbs_multibit = 3 holds both key forms and
-    picks two bits untuned (its launches equal the two-bit context's bit for bit);
-    tuned for 1 block it equals the fixed context of the form it picked; AES through
-    it decrypts."""
+    """Two key bits per step: the wide form (a 128-input launch pinned to it, one wave per
+    SIMD and all pattern keys in one batch) and the two-waves-per-SIMD form (8192 inputs)
+    give the same words on the same inputs.  pbs_multibit = 3 holds both key forms and
+    picks two bits untuned (its launches equal the two-bit context's bit for bit); tuned
+    for 1 block it equals the fixed context of the form it picked (the split form: the same
+    phases within 2^40 of 2^64); AES through it decrypts."""
     import torch
     import tfhe_amd
     outs = {}
@@ -1050,18 +1059,21 @@ def test_pbs_auto_multibit_matches_fixed_forms(gpu, okeys):
                                                                            pfks_linear=1))
         try:
             c.keygen(okeys.s, SEED)
+            if mb >= 2:
+                c.set_pbs_form(128, {2: 1, 3: 3}[mb])  # the wide key batch
             for nb in (128, 8192):
                 d_out = torch.empty((nb, 2, 2049), dtype=torch.int64, device="cuda")
                 c.pbs_batch(d_cts[:nb].contiguous(), d_out)
                 torch.cuda.synchronize()
                 outs[(mb, nb)] = from_dev(d_out, np.uint64)
+            c.set_pbs_form(128, -1)
             if mb == 3:
                 tuned = c.autotune(1)["pbs_form"][1]  # 1 block: no full round, all 128 in the rest
-                assert tuned in (0, 1, 2, 3)
+                assert tuned in (0, 1, 2, 3, 4)
                 d_out = torch.empty((128, 2, 2049), dtype=torch.int64, device="cuda")
                 c.pbs_batch(d_cts[:128].contiguous(), d_out)
                 torch.cuda.synchronize()
-                outs[("tuned", tuned >= 2)] = from_dev(d_out, np.uint64)
+                outs[("tuned", tuned)] = from_dev(d_out, np.uint64)
                 key = bytes(range(16))
                 c.set_round_keys(tfhe_amd.aes_encrypt_key_bits(okeys.s, 77, key))
                 ctr = [bytes.fromhex("00112233445566778899aabbccddeeff")]
@@ -1076,15 +1088,22 @@ def test_pbs_auto_multibit_matches_fixed_forms(gpu, okeys):
     assert np.array_equal(outs[(3, 8192)], outs[(2, 8192)])
     assert np.array_equal(outs[(1, 128)], outs[(1, 8192)][:128])
     assert not np.array_equal(outs[(1, 128)], outs[(2, 128)])  # one and two bits per step differ in their words
-    (two_bits, got), = [(k[1], v) for k, v in outs.items() if k[0] == "tuned"]
-    assert np.array_equal(got, outs[(2, 128)] if two_bits else outs[(1, 128)])
+    (tuned, got), = [(k[1], v) for k, v in outs.items() if k[0] == "tuned"]
+    if tuned == 4:  # split over 8 waves: other FP64 rounding, the same phases
+        for k in range(0, 128, 9):
+            for e1, e2 in zip(_phase_err(okeys.K2, got[k], int(bits[k % 64])),
+                              _phase_err(okeys.K2, outs[(2, 128)][k], int(bits[k % 64]))):
+                assert abs(e1 - e2) < 2 ** 40, k
+    else:
+        assert np.array_equal(got, outs[(2, 128)] if tuned >= 2 else outs[(1, 128)])
 
 
 def test_pinned_pbs_forms_give_the_same_words(actx, okeys):
     """tfhe_cbs_set_pbs_form pins the launch form per size.  The two forms give the same
     words on the same launch sizes (301, 300 and 3 inputs) whichever the size model would
     pick: with one key bit per step the branchy and branch-free cross-wave FFT steps
-    (rounding at the same points), with two the two-waves-per-SIMD and wide key batches.
+    (rounding at the same points), with two the two-waves-per-SIMD and wide key batches;
+    with two bits the split form (8 waves) gives the same phases within 2^40 of 2^64.
     Inputs mix encryptions, zero masks (every step a zero rotation) and masks with one
     rotating key-bit pair.  A form the context's PBS does not have is refused."""
     import torch
@@ -1103,8 +1122,9 @@ def test_pinned_pbs_forms_give_the_same_words(actx, okeys):
         else:
             cts[k] = tfhe_amd.lwe_encrypt(okeys.s, 41, k, int(rng.integers(0, 2)), p=2)
     d_cts = to_dev32(cts)
+    mb = actx.p.pbs_multibit
     with pytest.raises(tfhe_amd.TfheError):
-        actx.set_pbs_form(batch, 2)  # forms 0 and 1 only
+        actx.set_pbs_form(batch, 3 if mb == 2 else 2)  # forms 0, 1 (and 2: split, two bits)
     for nb in (301, 300, 3):
         outs = {}
         for form in (0, 1):
@@ -1115,6 +1135,17 @@ def test_pinned_pbs_forms_give_the_same_words(actx, okeys):
             outs[form] = from_dev(d_out, np.uint64)
         actx.set_pbs_form(nb, -1)
         assert np.array_equal(outs[0], outs[1])
+        if mb == 2:  # the split form: the same phases (other FP64 rounding of the sums)
+            actx.set_pbs_form(nb, 2)
+            d_out = torch.full((nb, 2, 2049), 7, dtype=torch.int64, device="cuda")
+            actx.pbs_batch(d_cts[:nb].contiguous(), d_out)
+            torch.cuda.synchronize()
+            actx.set_pbs_form(nb, -1)
+            split = from_dev(d_out, np.uint64)
+            for k in range(0, nb, 7):
+                for e1, e2 in zip(_phase_err(okeys.K2, split[k], 0), _phase_err(okeys.K2, outs[0][k], 0)):
+                    d = (e1 - e2) % 2 ** 64
+                    assert min(d, 2 ** 64 - d) < 2 ** 40, (nb, k)
 
 
 def test_empty_batches_leave_outputs_untouched(ctx, okeys):
