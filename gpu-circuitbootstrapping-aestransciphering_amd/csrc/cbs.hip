@@ -771,15 +771,13 @@ __global__ __launch_bounds__(512, 1) void k_pbs2_split(const uint32_t* __restric
             const uint32_t mrot2 = (Q2 - (uint32_t)at[i + 1]) & (Q2 - 1);
             if ((mrot | mrot2) == 0) continue;
             const cplx* key = bskf + (size_t)(i / 2) * 3 * PBS_ROWS * 2 * CB_M;
-            cplx fp[3][4];  // z1 - 1, z1 z2 - 1, z2 - 1 (patterns (1,0), (1,1), (0,1))
+            // z1, z2 per frequency; the pattern factors z1 - 1, z1 z2 - 1, z2 - 1 (patterns
+            // (1,0), (1,1), (0,1)) are formed where they are used, which keeps 16 registers free
+            cplx z1[4], z2[4];
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                const cplx z1 = T.rot[(mrot * ek[s]) & (Q2 - 1)];
-                const cplx z2 = T.rot[(mrot2 * ek[s]) & (Q2 - 1)];
-                const cplx z12 = cmul(z1, z2);
-                fp[0][s] = {z1.re - 1.0, z1.im};
-                fp[1][s] = {z12.re - 1.0, z12.im};
-                fp[2][s] = {z2.re - 1.0, z2.im};
+                z1[s] = T.rot[(mrot * ek[s]) & (Q2 - 1)];
+                z2[s] = T.rot[(mrot2 * ek[s]) & (Q2 - 1)];
             }
             // sums for this group's output (mown) and the other group's (moth); the key rows
             // of each are addressed by offset, so no array is indexed by the group
@@ -816,7 +814,10 @@ __global__ __launch_bounds__(512, 1) void k_pbs2_split(const uint32_t* __restric
                 };
                 auto mac_pattern = [&](const cplx(&kv)[2][4], int q) {
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) y[s] = cmul(x[s], fp[q][s]);
+                    for (int s = 0; s < 4; ++s) {
+                        const cplx z = q == 0 ? z1[s] : q == 2 ? z2[s] : cmul(z1[s], z2[s]);
+                        y[s] = cmul(x[s], cplx{z.re - 1.0, z.im});
+                    }
 #pragma unroll
                     for (int s = 0; s < 4; ++s) {
                         cmac(mown[s], y[s], kv[0][s]);
@@ -1439,9 +1440,12 @@ __global__ __launch_bounds__(256) void k_ksl_digits(const uint32_t* __restrict__
 // Grid x = Mpad / 128 input tiles, y = 640 / 64 column tiles.  Wave w computes 64 inputs
 // (two 32-row blocks) x 32 columns x 4 slices: eight 16-register accumulators, three
 // operand sets in flight as in k_pfkl_gemm.
+// blockIdx.z = K-slice of kstep steps (a multiple of 3): with more than one slice
+// (gridDim.z > 1) each adds its partial sum into the zero-filled output with 32-bit atomics
+// (the sums are mod 2^32, so the order does not change a word), slice 0 adding b
 __global__ __launch_bounds__(256, 1) void k_ksl_gemm(const int8_t* __restrict__ D, uint32_t B,
                                                  const int8_t* __restrict__ KT, const uint32_t* __restrict__ lwe32,
-                                                 int n, uint32_t* __restrict__ out) {
+                                                 int n, uint32_t* __restrict__ out, int kstep) {
     constexpr int N = CB_N;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1457,10 +1461,11 @@ __global__ __launch_bounds__(256, 1) void k_ksl_gemm(const int8_t* __restrict__ 
 #pragma unroll
         for (int sl = 0; sl < KSL_SLICES; ++sl) acc[r][sl] = pfkl_v16i{};
     // three operand sets rotate over the steps (513 = 3 x 171), each loaded two steps ahead
-    // of its products; loads past the last step reread step 0 (in bounds, unused)
+    // of its products; loads past the slice's last step reread its first (in bounds, unused)
+    const int kbeg = (int)blockIdx.z * kstep, kend = kbeg + kstep;
     struct Ops { pfkl_v4i a[2], b[KSL_SLICES]; };
     auto load = [&](Ops& x, int ks) {
-        const size_t fo = (size_t)(ks < PFKL_KS ? ks : 0) * 1024;
+        const size_t fo = (size_t)(ks < kend ? ks : kbeg) * 1024;
         x.a[0] = *reinterpret_cast<const pfkl_v4i*>(a0 + fo);
         x.a[1] = *reinterpret_cast<const pfkl_v4i*>(a1 + fo);
 #pragma unroll
@@ -1474,10 +1479,10 @@ __global__ __launch_bounds__(256, 1) void k_ksl_gemm(const int8_t* __restrict__ 
         }
     };
     Ops X, Y, Z;
-    load(X, 0);
-    load(Y, 1);
+    load(X, kbeg);
+    load(Y, kbeg + 1);
 #pragma unroll 1
-    for (int ks = 0; ks < PFKL_KS; ks += 3) {
+    for (int ks = kbeg; ks < kend; ks += 3) {
         load(Z, ks + 2);
         __builtin_amdgcn_sched_barrier(0);
         mma(X);
@@ -1503,8 +1508,11 @@ __global__ __launch_bounds__(256, 1) void k_ksl_gemm(const int8_t* __restrict__ 
 #pragma unroll
                 for (int sl = 0; sl < KSL_SLICES; ++sl) t += (uint32_t)acc[r][sl][j] << (8 * sl);
                 uint32_t v = 0u - t;
-                if (col == n) v += lwe32[(size_t)m * (N + 1) + N];
-                out[(size_t)m * (n + 1) + col] = v;
+                if (col == n && blockIdx.z == 0) v += lwe32[(size_t)m * (N + 1) + N];
+                if (gridDim.z > 1)
+                    atomicAdd(&out[(size_t)m * (n + 1) + col], v);
+                else
+                    out[(size_t)m * (n + 1) + col] = v;
             }
         }
 }
@@ -1874,7 +1882,17 @@ int launch_ks_linear(const uint32_t* lwe32, uint32_t count, int n, const int8_t*
         const uint32_t Mpad = (B + 127) / 128 * 128;
         const uint32_t* in = lwe32 + (size_t)off * (CB_N + 1);
         k_ksl_digits<<<grid_for((size_t)Mpad * (PFKL_KP / 8) / 256 + 1, 8192), 256, 0, st>>>(in, B, Mpad, D);
-        k_ksl_gemm<<<dim3(Mpad / 128, KSL_COLS / 64), 256, 0, st>>>(D, B, KT, in, n, out + (size_t)off * (n + 1));
+        // small batches: the K steps split over 3, 9, 19 or 57 workgroups (513 = 27 x 19), so
+        // that at least 160 workgroups stream the key (128 inputs: 10 column tiles x 19)
+        const unsigned tiles = (Mpad / 128) * (KSL_COLS / 64);
+        int kslices = 1;
+        for (int ks : {1, 3, 9, 19, 57}) {
+            kslices = ks;
+            if (tiles * ks >= 160) break;
+        }
+        uint32_t* o = out + (size_t)off * (n + 1);
+        if (kslices > 1) launch_zero_rows32(o, B, n + 1, n + 1, st);
+        k_ksl_gemm<<<dim3(Mpad / 128, KSL_COLS / 64, kslices), 256, 0, st>>>(D, B, KT, in, n, o, PFKL_KS / kslices);
     }
     return 0;
 }
