@@ -16,7 +16,8 @@ seeded random plaintext.
 
 Secondary (key "gate_bootstraps"): 4096 independent NAND/XOR gate bootstraps per
 step (n=636, N=1024, BSK 2^7 x 3, KS 2^2 x 8, FP64 FFT), with a decryption-failure
-campaign over 10^6 chained bootstraps.
+campaign over 10^6 chained bootstraps.  Side line "gate_functions": the gate context's
+programmable bootstrap, EvalFunc and EvalSign over Z_2^32 on 1024 ciphertexts.
 
 Prints one JSON line.
 """
@@ -499,6 +500,60 @@ def gate_bench(args):
     return out
 
 
+def gate_functions_bench(reps, batch=1024):
+    """Programmable bootstrap and the large-precision functions on the gate context (side
+    line): BootstrapFunc over Z_1024, EvalFunc of an arbitrary table of Z_8 over Z_1024 (two
+    bootstraps through Z_2048; every output decrypted) and EvalSign over Z_2^32 (eight floors
+    and the sign bootstrap, 17 bootstraps per ciphertext), `batch` ciphertexts per call."""
+    import numpy as np
+    import torch
+    import tfhe_amd
+    Q, P = tfhe_amd.FUNC_Q, 8
+    ctx = tfhe_amd.GateContext(max_batch=batch)
+    sk = tfhe_amd.lwe_keygen(N_LWE, 5)
+    ctx.bt_keygen(sk, 5)
+    s64 = sk.astype(np.uint64)
+    rng = np.random.default_rng(3)
+
+    def enc(vals, mod, seed):
+        sh = 32 - int(mod).bit_length() + 1
+        cts = np.stack([tfhe_amd.lwe_encrypt(sk, seed, k, 0, p=2) for k in range(len(vals))]).astype(np.uint64)
+        cts[:, -1] = (cts[:, -1] + (np.asarray(vals, np.uint64) << np.uint64(sh))) % 2 ** 32
+        if sh:
+            cts = ((cts + (1 << (sh - 1))) >> sh) % int(mod)
+        return torch.from_numpy(cts.astype(np.uint32).view(np.int32)).cuda()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    F = [(3 * m + 1) % P for m in range(P)]
+    lut = np.array([F[(x * P) // Q] * (Q // P) for x in range(Q)], np.uint32)
+    msgs = rng.integers(0, P, batch)
+    d_f = enc(msgs * (Q // P), Q, 11)
+    d_out = torch.empty_like(d_f)
+    t_bf = timed(lambda: ctx.bootstrap_func(d_f, Q, lut, Q, d_out))
+    t_ef = timed(lambda: ctx.eval_func(d_f, Q, lut, d_out))
+    got = d_out.cpu().numpy().view(np.uint32)
+    ph = ((got[:, -1].astype(np.uint64) - got[:, :-1].astype(np.uint64) @ s64) % Q).astype(np.int64)
+    ef_fail = int(np.count_nonzero(((ph + Q // (2 * P)) // (Q // P)) % P != np.array(F)[msgs]))
+    d_s = enc(rng.integers(0, 2 ** 32, batch, dtype=np.uint64), 2 ** 32, 12)
+    t_sg = timed(lambda: ctx.eval_sign(d_s, 2 ** 32, d_out))
+    ctx.close()
+    return {"batch": batch, "reps": reps,
+            "bootstrap_func_per_s": round(batch / t_bf, 1),
+            "eval_func_per_s": round(batch / t_ef, 1), "eval_func_decrypt_failures": ef_fail,
+            "eval_sign_mod_2_32_per_s": round(batch / t_sg, 1),
+            "config": {"workload": "gate-context programmable bootstrap (n 636, N 1024, BSK 2^7 x 3, KS 2^2 x 8)",
+                       "eval_func": "arbitrary table of Z_8 over Z_1024 (two bootstraps)",
+                       "eval_sign": "Z_2^32 to Z_1024: 8 floors + 1 bootstrap = 17 bootstraps"}}
+
+
 # form -> (bench key, pbs_approx, pbs_multibit, pfks_linear, lwe_to_rgsw, ks_linear)
 AES_FORMS = {"exact": ("aes128_ctr", 0, 1, 0, 0, 0), "approx": ("aes128_ctr_one_product_pbs", 1, 1, 0, 0, 0),
              "approx-mb2": ("aes128_ctr_one_product_pbs_2bit", 1, 2, 0, 0, 0),
@@ -699,6 +754,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--gate-steps", type=int, default=20, help="timed steps of the gate line (0 to skip)")
     ap.add_argument("--plain-steps", type=int, default=3, help="plain per-bootstrap gate samples")
+    ap.add_argument("--func-reps", type=int, default=3,
+                    help="timed calls of the gate-context function line (0 to skip)")
     ap.add_argument("--campaign", type=int, default=1_000_000,
                     help="gate bootstraps in the decryption-failure campaign (0 to skip)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -758,6 +815,8 @@ def main():
     line["split_fp32_product"] = split32_product_bench()
     if args.gate_steps > 0:
         line["gate_bootstraps"] = gate_bench(args)
+    if args.func_reps > 0:
+        line["gate_functions"] = gate_functions_bench(args.func_reps)
     if args.copy_calibration:
         src = torch.empty(512 << 20, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)

@@ -735,6 +735,16 @@ __global__ __launch_bounds__(512, 1) void k_pbs2_split(const uint32_t* __restric
 #pragma unroll
     for (int s = 0; s < 4; ++s) ek[s] = (1u - 4u * (kq + 64u * (uint32_t)s)) & (Q2 - 1);
 
+    // L2 prefetch of the next step's pattern keys (786 KB): workgroups are dealt to the 8 XCDs
+    // in turn and run the steps nearly in lockstep, so the up to 16 workgroups of one XCD each
+    // touch one share of the step's 128-byte lines (one word per line), issued after the step's
+    // last key loads (loads return in order: earlier, the step's own key waits would include it)
+    // and consumed a step later
+    constexpr uint32_t KLINES = 3 * PBS_ROWS * 2 * CB_M * sizeof(cplx) / 128;
+    const uint32_t per_xcd = (gridDim.x + 7) / 8 < 16 ? (gridDim.x + 7) / 8 : 16;
+    const uint32_t share = (KLINES + per_xcd - 1) / per_xcd, part = (blockIdx.x >> 3) % per_xcd;
+    uint32_t pf0 = 0, pf1 = 0, sink = 0;
+
     for (uint32_t cidx = blockIdx.x; cidx < batch; cidx += gridDim.x) {
         const uint32_t* ct = lwe + (size_t)cidx * (n + 1);
         __syncthreads();
@@ -835,6 +845,14 @@ __global__ __launch_bounds__(512, 1) void k_pbs2_split(const uint32_t* __restric
                 mac_pattern(kb, 1);
                 mac_pattern(ka, 2);
             }
+            {  // branch-free: the last step re-touches its own key, lines past the share are clamped
+                sink ^= pf0 ^ pf1;  // the previous step's prefetch, long landed
+                const char* nk = reinterpret_cast<const char*>(i + 2 < n ? key + 3 * PBS_ROWS * 2 * CB_M : key);
+                const uint32_t end = (part + 1) * share < KLINES ? (part + 1) * share : KLINES;
+                const uint32_t l0 = part * share + threadIdx.x, l1 = l0 + blockDim.x;
+                pf0 = *reinterpret_cast<const uint32_t*>(nk + (size_t)(l0 < end ? l0 : end - 1) * 128);
+                pf1 = *reinterpret_cast<const uint32_t*>(nk + (size_t)(l1 < end ? l1 : end - 1) * 128);
+            }
             // the other group's output: this group's partial through LDS (the buffer the
             // third transform used; the fourth transform's barrier ended its reads)
             cplx* ex = xbuf[g][buf];
@@ -883,6 +901,7 @@ __global__ __launch_bounds__(512, 1) void k_pbs2_split(const uint32_t* __restric
             if (lane == 0 && w == 1) o1[N] = acc[0] + mu1;
         }
     }
+    if (n < 0) out[threadIdx.x] = sink;  // never (n > 0): keeps the prefetch loads
 }
 
 // ===========================================================================
@@ -1308,12 +1327,16 @@ typedef int pfkl_v16i __attribute__((ext_vector_type(16)));
 __global__ __launch_bounds__(256, 1) void k_pfkl_gemm(const int8_t* __restrict__ D, uint32_t B,
                                                   const int8_t* __restrict__ KT, int u0,
                                                   uint32_t* __restrict__ out, size_t out_stride,
-                                                  size_t out_u_offset) {
+                                                  size_t out_u_offset, int cols_first) {
     constexpr int COLS = 2 * CB_N;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t m0 = blockIdx.x * 128 + (w & 1) * 64;
-    const uint32_t cg = blockIdx.y * 64 + (w >> 1) * 32;
+    // cols_first: blockIdx.x = column tile, blockIdx.y = row block, so that the row blocks of one
+    // key tile have linear workgroup ids 128 apart, on one XCD (a launch of at most two row
+    // blocks runs whole, and the second read of each key tile hits that XCD's L2)
+    const uint32_t rb = cols_first ? blockIdx.y : blockIdx.x, ct = cols_first ? blockIdx.x : blockIdx.y;
+    const uint32_t m0 = rb * 128 + (w & 1) * 64;
+    const uint32_t cg = ct * 64 + (w >> 1) * 32;
     const int uu = (int)(cg / COLS), col0 = (int)(cg % COLS);
     // fragment (block, step) of lane l at ((block * PFKL_KS + step) * 64 + l) * 16: step k / 32
     // advances a pointer by 1 KB
@@ -1862,8 +1885,11 @@ void launch_pfkl(const uint64_t* lwe64, uint32_t B, int8_t* D, const int8_t* KT,
                  size_t out_stride, size_t out_u_offset, hipStream_t st) {
     const uint32_t Mpad = (B + 127) / 128 * 128;
     k_pfkl_digits<<<grid_for((size_t)Mpad * (PFKL_KP / 8) / 256 + 1, 8192), 256, 0, st>>>(lwe64, B, Mpad, D);
-    k_pfkl_gemm<<<dim3(Mpad / 128, (unsigned)nu * 2 * CB_N / 64), 256, 0, st>>>(D, B, KT, u0, out, out_stride,
-                                                                              out_u_offset);
+    const unsigned rows = Mpad / 128, tiles = (unsigned)nu * 2 * CB_N / 64;
+    if (rows <= 2)
+        k_pfkl_gemm<<<dim3(tiles, rows), 256, 0, st>>>(D, B, KT, u0, out, out_stride, out_u_offset, 1);
+    else
+        k_pfkl_gemm<<<dim3(rows, tiles), 256, 0, st>>>(D, B, KT, u0, out, out_stride, out_u_offset, 0);
 }
 
 size_t kskl_key_bytes() { return (size_t)KSL_SLICES * KSL_COLS * PFKL_KP; }
