@@ -552,8 +552,8 @@ __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylu
                     if constexpr (MB == 2) {
                         cplx y[4];
                         // key values of pattern q (rows in the order (1,0), (1,1), (0,1)) for
-                        // both outputs.  The empty asm statements close a batch of loads, which
-                        // the compiler would otherwise split, waiting on each load in turn.
+                        // both outputs, issued as batches, which the compiler would otherwise
+                        // split, waiting on each load in turn.
                         auto load_keys = [&](cplx(&kv)[2][4], int q) {
                             const int pt = q == 0 ? 0 : q == 1 ? 2 : 1;
                             const cplx* kr = key + ((size_t)pt * PBS_ROWS + r) * 2 * CB_M + (w * 4) * 64 + lane;
@@ -562,11 +562,22 @@ __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylu
 #pragma unroll
                                 for (int s = 0; s < 4; ++s) kv[o][s] = kr[o * CB_M + s * 64];
                         };
+                        // WIDE (one wave per SIMD, registers to spare): a scheduling fence keeps a
+                        // batch's loads together and leaves their waits at the first use, so later
+                        // batches are in flight meanwhile.  Two waves per SIMD: the empty asm
+                        // statements wait for the batch at once (both batches in flight at 256
+                        // registers would spill, which measured slower: 2176 inputs 50-51 ms
+                        // against 38)
                         auto close_batch = [](cplx(&kv)[2][4]) {
+                            if constexpr (WIDE) {
+                                __builtin_amdgcn_sched_barrier(0);
+                            } else {
 #pragma unroll
-                            for (int o = 0; o < 2; ++o)
+                                for (int o = 0; o < 2; ++o)
 #pragma unroll
-                                for (int k = 0; k < 4; ++k) asm volatile("" : "+v"(kv[o][k].re), "+v"(kv[o][k].im));
+                                    for (int k = 0; k < 4; ++k)
+                                        asm volatile("" : "+v"(kv[o][k].re), "+v"(kv[o][k].im));
+                            }
                         };
                         auto mac_pattern = [&](const cplx(&kv)[2][4], int q) {
 #pragma unroll
@@ -816,12 +827,10 @@ __global__ __launch_bounds__(512, 1) void k_pbs2_split(const uint32_t* __restric
                         kv[1][s] = kr[oth_off + s * 64];
                     }
                 };
-                auto close_batch = [](cplx(&kv)[2][4]) {
-#pragma unroll
-                    for (int o = 0; o < 2; ++o)
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) asm volatile("" : "+v"(kv[o][k].re), "+v"(kv[o][k].im));
-                };
+                // a scheduling fence keeps a batch's loads together and leaves their waits at
+                // the first use, so the next batch is in flight meanwhile (1-block AES 18.5 ->
+                // 21.1 blocks/s against waiting for each batch at once)
+                auto close_batch = [](cplx(&)[2][4]) { __builtin_amdgcn_sched_barrier(0); };
                 auto mac_pattern = [&](const cplx(&kv)[2][4], int q) {
 #pragma unroll
                     for (int s = 0; s < 4; ++s) {
