@@ -54,6 +54,32 @@ __device__ __forceinline__ void dft4(C (&x)[4]) {
 }
 
 __device__ __forceinline__ int f256_a1(int k0, int b) { return k0 * 80 + (b >> 4) * 20 + (b & 15); }
+
+// The first transpose of the forward 256-point step (and the last of the inverse, its own
+// inverse): slot c of lane 16 q + d takes slot q of lane 16 c + d, a 4 x 4 transpose of
+// (slot, 16-lane row) done in registers by gfx950's row swaps: v_permlane32_swap exchanges
+// rows 2, 3 of one register with rows 0, 1 of another, v_permlane16_swap rows 1, 3 with
+// rows 0, 2.  Slots (0, 2), (1, 3) through the first, then (0, 1), (2, 3) through the second.
+template <class C>
+__device__ __forceinline__ void row_transpose4(C (&x)[4]) {
+    constexpr int W = (int)(sizeof(C) / 4);  // dwords per value
+    uint32_t d[4][W];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) __builtin_memcpy(d[s], &x[s], sizeof(C));
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const auto a = __builtin_amdgcn_permlane32_swap(d[0][w], d[2][w], false, false);
+        const auto b = __builtin_amdgcn_permlane32_swap(d[1][w], d[3][w], false, false);
+        const auto c = __builtin_amdgcn_permlane16_swap(a[0], b[0], false, false);
+        const auto e = __builtin_amdgcn_permlane16_swap(a[1], b[1], false, false);
+        d[0][w] = c[0];
+        d[1][w] = c[1];
+        d[2][w] = e[0];
+        d[3][w] = e[1];
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) __builtin_memcpy(&x[s], d[s], sizeof(C));
+}
 __device__ __forceinline__ int f256_a2(int k0, int e, int d) { return k0 * 80 + e * 20 + d; }
 __device__ __forceinline__ int f256_a3(int k0, int e, int p, int h) { return k0 * 80 + e * 20 + p * 5 + h; }
 
@@ -86,13 +112,8 @@ __device__ __forceinline__ void fft256_fwd_t(C (&x)[4], C* lds, const FFT1024Tab
     dft4<false>(x);
 #pragma unroll
     for (int k = 0; k < 4; ++k) x[k] = cmul(x[k], T.p1t[k * 64 + lane]);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) lds[f256_a1(k, lane)] = x[k];
-    wave_sync();
+    row_transpose4(x);
     const int k0 = lane >> 4, d = lane & 15;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) x[c] = lds[f256_a1(k0, 16 * c + d)];
-    wave_sync();
     dft4<false>(x);
 #pragma unroll
     for (int e = 1; e < 4; ++e) x[e] = cmul(x[e], RT ? tw.p2[e - 1] : T.p2[e * 64 + lane]);
@@ -145,12 +166,7 @@ __device__ __forceinline__ void fft256_inv_t(C (&x)[4], C* lds, const FFT1024Tab
 #pragma unroll
     for (int ee = 1; ee < 4; ++ee) x[ee] = cmulc(x[ee], RT ? tw.p2[ee - 1] : T.p2[ee * 64 + lane]);
     dft4<true>(x);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) lds[f256_a1(k0, 16 * c + d)] = x[c];
-    wave_sync();
-#pragma unroll
-    for (int k = 0; k < 4; ++k) x[k] = lds[f256_a1(k, lane)];
-    wave_sync();
+    row_transpose4(x);
 #pragma unroll
     for (int k = 0; k < 4; ++k) x[k] = cmulc(x[k], T.p1t[k * 64 + lane]);
     dft4<true>(x);
