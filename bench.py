@@ -17,7 +17,8 @@ seeded random plaintext.
 Secondary (key "gate_bootstraps"): 4096 independent NAND/XOR gate bootstraps per
 step (n=636, N=1024, BSK 2^7 x 3, KS 2^2 x 8, FP64 FFT), with a decryption-failure
 campaign over 10^6 chained bootstraps.  Side line "gate_functions": the gate context's
-programmable bootstrap, EvalFunc and EvalSign over Z_2^32 on 1024 ciphertexts.
+programmable bootstrap, EvalFunc and EvalSign over Z_2^32 on 1024 ciphertexts.  Side line
+"cbs_lut_256_bits": 256 LWE bits through circuit bootstrapping and 32 random-table byte LUTs.
 
 Prints one JSON line.
 """
@@ -500,6 +501,69 @@ def gate_bench(args):
     return out
 
 
+def cbs_lut_bench(reps, nbits=256):
+    """Circuit bootstrapping + LUT (the 256-bit workload): 32 seeded random bytes as 256 LWE
+    bits -> 256 circuit bootstraps -> RGSW to the Fourier domain -> 32 CMux vertical-packing
+    lookups in a seeded random byte table (8 output bits each) -> N -> n key switch, every
+    output byte decrypted against the table; in the exact FP64 form (2-limb PBS, lookup
+    PFKS) and in the headline form (one-product two-bit PBS, linear PFKS and key switch)."""
+    import numpy as np
+    import torch
+    import tfhe_amd
+    rng = np.random.default_rng(17)
+    table = rng.integers(0, 256, 256)
+    poly = np.zeros(N_CB, np.uint32)
+    for x in range(256):
+        for j in range(8):
+            poly[8 * x + j] = np.uint32(((int(table[x]) >> j) & 1) << 31)
+    xs = rng.integers(0, 256, nbits // 8)
+    bits = [(int(x) >> i) & 1 for x in xs for i in range(8)]
+    sk = tfhe_amd.lwe_keygen(N_LWE, 3)
+    cts = np.stack([tfhe_amd.lwe_encrypt(sk, 5, k, b, p=2) for k, b in enumerate(bits)])
+    s64 = sk.astype(np.uint64)
+    out = {"bits": nbits, "luts": nbits // 8, "reps": reps}
+    forms = {"exact_fp64": tfhe_amd.cbs_params(),
+             "one_product_two_bit_linear": tfhe_amd.cbs_params(pbs_approx=1, pbs_multibit=3, pfks_linear=1,
+                                                               ks_linear=1)}
+    for name, params in forms.items():
+        ctx = tfhe_amd.CbsContext(max_blocks=(nbits + 127) // 128, params=params)
+        ctx.keygen(sk, 3)
+        d_cts = torch.from_numpy(cts.view(np.int32).copy()).cuda()
+        d_poly = torch.from_numpy(poly.view(np.int32).copy()).cuda()
+        d_rgsw = torch.empty((nbits, 4, 2, N_CB), dtype=torch.int32, device="cuda")
+        d_f = torch.empty(nbits * 8 * M_CB * 2, dtype=torch.float64, device="cuda")
+        d_l = torch.empty((nbits // 8, 8, N_CB + 1), dtype=torch.int32, device="cuda")
+        d_o = torch.empty((nbits, N_LWE + 1), dtype=torch.int32, device="cuda")
+
+        def call():
+            ctx.cbs_batch(d_cts, d_rgsw)
+            ctx.rgsw_to_fourier(d_rgsw, nbits, d_f)
+            ctx.lut8(d_f, nbits // 8, d_poly, 0, 8, d_l)
+            ctx.keyswitch_batch(d_l, nbits, d_o)
+
+        call()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            call()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / reps
+        t = time.perf_counter()
+        for _ in range(reps):
+            ctx.cbs_batch(d_cts, d_rgsw)
+        torch.cuda.synchronize()
+        dt_cbs = (time.perf_counter() - t) / reps
+        g = d_o.cpu().numpy().view(np.uint32)
+        ph = (g[:, -1].astype(np.uint64) - g[:, :-1].astype(np.uint64) @ s64) % 2 ** 32
+        got_bits = ((ph + (1 << 30)) >> 31) & 1
+        fails = sum(int(sum(int(got_bits[8 * k + j]) << j for j in range(8)) != int(table[x])) for k, x in enumerate(xs))
+        ctx.close()
+        out[name] = {"ms_per_call": round(dt * 1e3, 3), "circuit_bootstraps_per_s": round(nbits / dt_cbs, 1),
+                     "cbs_ms": round(dt_cbs * 1e3, 3), "luts_per_s_end_to_end": round(nbits // 8 / dt, 1),
+                     "wrong_bytes": fails}
+    return out
+
+
 def gate_functions_bench(reps, batch=1024):
     """Programmable bootstrap and the large-precision functions on the gate context (side
     line): BootstrapFunc over Z_1024, EvalFunc of an arbitrary table of Z_8 over Z_1024 (two
@@ -754,6 +818,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--gate-steps", type=int, default=20, help="timed steps of the gate line (0 to skip)")
     ap.add_argument("--plain-steps", type=int, default=3, help="plain per-bootstrap gate samples")
+    ap.add_argument("--cbs-lut-reps", type=int, default=5,
+                    help="timed calls of the 256-bit circuit bootstrap + LUT line (0 to skip)")
     ap.add_argument("--func-reps", type=int, default=3,
                     help="timed calls of the gate-context function line (0 to skip)")
     ap.add_argument("--campaign", type=int, default=1_000_000,
@@ -815,6 +881,8 @@ def main():
     line["split_fp32_product"] = split32_product_bench()
     if args.gate_steps > 0:
         line["gate_bootstraps"] = gate_bench(args)
+    if args.cbs_lut_reps > 0:
+        line["cbs_lut_256_bits"] = cbs_lut_bench(args.cbs_lut_reps)
     if args.func_reps > 0:
         line["gate_functions"] = gate_functions_bench(args.func_reps)
     if args.copy_calibration:
