@@ -55,3 +55,22 @@ def test_host_encrypt_matches_oracle():
         O.lib().orc_lwe_encrypt32(9, O.DOM_ENC, 100 + m, 636, s2, m << 30, 15, ct2)
         assert np.array_equal(ct, ct2)
         assert tfhe_amd.lwe_decrypt(sk, ct) == m
+
+
+def test_func_property_matches_oracle_on_cpu():
+    """tfhe_func_property (host code, the test EvalFunc branches on) classifies tables as the
+    oracle does: hand-made negacyclic / periodic / arbitrary tables, the all-zero table
+    (both properties: negacyclic wins), random tables over several moduli, and bad moduli."""
+    import numpy as np
+    import oracle as O
+    if not os.path.exists(tfhe_amd.LIB_PATH):
+        pytest.skip("library not built")
+    rng = np.random.default_rng(3)
+    for q in (2, 16, 1024, 2048):
+        half = rng.integers(0, q, q // 2)
+        neg = np.concatenate([half, (-half) % q]).astype(np.uint32)
+        per = np.concatenate([half, half]).astype(np.uint32)
+        arb = rng.integers(0, q, q).astype(np.uint32)
+        for t in (neg, per, arb, np.zeros(q, np.uint32)):
+            assert tfhe_amd.func_property(t, q) == O.func_property(t, q), q
+    assert tfhe_amd.func_property(np.zeros(12, np.uint32), 12) == -1  # not a power of two
