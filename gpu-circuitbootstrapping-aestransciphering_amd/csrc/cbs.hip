@@ -471,6 +471,12 @@ __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylu
 #pragma unroll
     for (int s = 0; s < 4; ++s) ek[s] = (1u - 4u * (kq + 64u * (uint32_t)s)) & (Q2 - 1);
 
+    // MB = 2, two waves per SIMD: the 64 resident workgroups of an XCD each touch one share of
+    // the next step's pattern-key lines after a step's last key load (see k_pbs2_split)
+    constexpr uint32_t KLINES = 3 * PBS_ROWS * 2 * CB_M * sizeof(cplx) / 128;
+    const uint32_t per_xcd = (gridDim.x + 7) / 8 < 64 ? (gridDim.x + 7) / 8 : 64;
+    const uint32_t share = (KLINES + per_xcd - 1) / per_xcd, part = (blockIdx.x >> 3) % per_xcd;
+    uint32_t pf0 = 0, sink = 0;
     for (uint32_t cidx = blockIdx.x; cidx < batch; cidx += gridDim.x) {
         const uint32_t* ct = lwe + (size_t)cidx * (n + 1);
         __syncthreads();
@@ -619,6 +625,13 @@ __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylu
                     }
                 }
             }
+            if constexpr (MB == 2 && !WIDE) {
+                sink ^= pf0;
+                const char* nk = reinterpret_cast<const char*>(i + 2 < n ? key + 3 * PBS_ROWS * 2 * CB_M : key);
+                const uint32_t end = (part + 1) * share < KLINES ? (part + 1) * share : KLINES;
+                const uint32_t l0 = part * share + threadIdx.x;
+                pf0 = *reinterpret_cast<const uint32_t*>(nk + (size_t)(l0 < end ? l0 : end - 1) * 128);
+            }
             if constexpr (LIMBS == 1) {
                 // acc += (X^mrot - 1) t in the Fourier domain: frequency k is the polynomial
                 // at w^(1 - 4k), so t's transform is scaled by w^(mrot (1 - 4k)) - 1 before
@@ -711,6 +724,7 @@ __global__ __launch_bounds__(256, (MB == 2 && !WIDE) ? 2 : 1) void k_pbs2_manylu
         if (lane == 0 && w == 0) o0[N] = acc[1][0] + mu0;
         if (lane == 0 && w == 1) o1[N] = acc[1][0] + mu1;
     }
+    if (n < 0) out[threadIdx.x] = sink;  // never (n > 0): keeps the prefetch loads
 }
 
 // Two key bits per step with one ciphertext per 8-wave workgroup (the one-product form):
