@@ -1436,6 +1436,124 @@ __global__ __launch_bounds__(256, 1) void k_pfkl_gemm(const int8_t* __restrict__
         }
 }
 
+// The same product with the operands staged through LDS, for launches of many input tiles:
+// 8 waves per workgroup, 256 inputs (eight 32-row blocks) x 64 columns (two 32-column
+// blocks) x 5 slices; wave w computes row blocks 2 (w % 4) + 0, 1 of column block w / 4, as
+// a wave of k_pfkl_gemm does.  Each step's 18 fragments (8 digit blocks, 2 x 5 key blocks,
+// 1 KB each) are read from global memory once per workgroup: wave w stages fragments w,
+// w + 8 and (waves 0, 1) w + 16 through registers three steps ahead, into a ring of three
+// LDS buffers (54 KB).  Step t stores step t + 1's fragments into buffer (t + 1) % 3, whose
+// last reads (step t - 2's operands, read at step t - 3) are complete at step t - 2's
+// barrier, then passes its own barrier and reads step t + 1's operands, which are in flight
+// while step t's products issue: one barrier per step.  The operands live in LDS, so a wave
+// holds its 160 accumulators and about 90 VGPRs, and a CU holds two waves per SIMD.  Same
+// sums, same words.
+__global__ __launch_bounds__(512, 1) void k_pfkl_gemm_lds(const int8_t* __restrict__ D, uint32_t B,
+                                                      const int8_t* __restrict__ KT, int u0,
+                                                      uint32_t* __restrict__ out, size_t out_stride,
+                                                      size_t out_u_offset) {
+    constexpr int COLS = 2 * CB_N, NF = 8 + 2 * PFKL_SLICES, NBUF = 3;
+    static_assert(PFKL_KS % NBUF == 0, "the step count splits into ring rounds");
+    __shared__ pfkl_v4i ring[NBUF][NF][64];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t m0 = blockIdx.x * 256;
+    const uint32_t cg = blockIdx.y * 64;
+    const int uu = (int)(cg / COLS), col0 = (int)(cg % COLS);
+    const size_t sstride = (size_t)COLS * PFKL_KP;
+    // fragment f at step 0 (a step advances the pointer by 1 KB): f < 8 digit row block f,
+    // else key column block (f - 8) / 5, slice (f - 8) % 5
+    auto src = [&](int f) -> const pfkl_v4i* {
+        const int8_t* p;
+        if (f < 8) {
+            p = D + ((size_t)(m0 / 32 + f) * PFKL_KS * 64 + lane) * 16;
+        } else {
+            const int j = (f - 8) / PFKL_SLICES, sl = (f - 8) % PFKL_SLICES;
+            p = KT + (size_t)(u0 + uu) * PFKL_SLICES * sstride + (size_t)sl * sstride +
+                ((size_t)(col0 / 32 + j) * PFKL_KS * 64 + lane) * 16;
+        }
+        return reinterpret_cast<const pfkl_v4i*>(p);
+    };
+    const bool third = w + 16 < NF;  // waves 0 and 1 stage a third fragment
+    const pfkl_v4i* g0 = src(w);
+    const pfkl_v4i* g1 = src(w + 8);
+    const pfkl_v4i* g2 = src(third ? w + 16 : w);
+    pfkl_v4i st[NBUF][3];
+    auto fetch = [&](pfkl_v4i(&r)[3], int ks) {  // step ks (past the last: step 0, unused)
+        const size_t fo = (size_t)(ks < PFKL_KS ? ks : 0) * 64;
+        r[0] = g0[fo];
+        r[1] = g1[fo];
+        if (third) r[2] = g2[fo];
+    };
+    auto stage = [&](const pfkl_v4i(&r)[3], int b) {
+        ring[b][w][lane] = r[0];
+        ring[b][w + 8][lane] = r[1];
+        if (third) ring[b][w + 16][lane] = r[2];
+    };
+    pfkl_v16i acc[2][PFKL_SLICES];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int sl = 0; sl < PFKL_SLICES; ++sl) acc[r][sl] = pfkl_v16i{};
+    const int ra = 2 * (w & 3), cb = 8 + (w >> 2) * PFKL_SLICES;
+    struct Opnd { pfkl_v4i a[2], b[PFKL_SLICES]; };
+    Opnd op[2];
+    auto read = [&](Opnd& x, int b) {
+        x.a[0] = ring[b][ra][lane];
+        x.a[1] = ring[b][ra + 1][lane];
+#pragma unroll
+        for (int sl = 0; sl < PFKL_SLICES; ++sl) x.b[sl] = ring[b][cb + sl][lane];
+    };
+    auto mma = [&](const Opnd& x) {
+#pragma unroll
+        for (int sl = 0; sl < PFKL_SLICES; ++sl) {
+            acc[0][sl] = __builtin_amdgcn_mfma_i32_32x32x32_i8(x.a[0], x.b[sl], acc[0][sl], 0, 0, 0);
+            acc[1][sl] = __builtin_amdgcn_mfma_i32_32x32x32_i8(x.a[1], x.b[sl], acc[1][sl], 0, 0, 0);
+        }
+    };
+    // step s (s % 6 = u, so that the register set, buffer and operand indices are constants):
+    // stage step s + 1, fetch step s + 4, barrier, read step s + 1's operands (staged before
+    // this barrier, so they are in flight during step s's products), products of step s
+    auto step = [&](int u, int s) {
+        const int nb = (u + 1) % NBUF;
+        stage(st[nb], nb);
+        fetch(st[nb], s + 1 + NBUF);
+        __syncthreads();
+        read(op[(u + 1) & 1], nb);
+        mma(op[u & 1]);
+    };
+    fetch(st[0], 0);
+    fetch(st[1], 1);
+    fetch(st[2], 2);
+    stage(st[0], 0);
+    fetch(st[0], 3);
+    __syncthreads();
+    read(op[0], 0);
+    static_assert(PFKL_KS % 6 == 3, "six-step rounds and a tail of three");
+#pragma unroll 1
+    for (int t = 0; t < PFKL_KS - 3; t += 6) {
+#pragma unroll
+        for (int u = 0; u < 6; ++u) step(u, t + u);
+    }
+#pragma unroll
+    for (int u = 0; u < 3; ++u) step(u, PFKL_KS - 3 + u);
+    const uint32_t mw = m0 + 64 * (w & 3);
+    uint32_t* o = out + (size_t)uu * out_u_offset + col0 + 32 * (w >> 2) + (lane & 31);
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const uint32_t m = mw + 32 * r + 8 * (j / 4) + 4 * (lane >> 5) + (j & 3);
+            if (m < B) {
+                int64_t tsum = 0;
+#pragma unroll
+                for (int sl = 0; sl < PFKL_SLICES; ++sl) tsum += (int64_t)acc[r][sl][j] * ((int64_t)1 << (8 * sl));
+                const uint64_t o40 = (0 - (uint64_t)tsum) & PFKL_MASK;
+                o[(size_t)m * out_stride] = (uint32_t)((o40 + 128) >> 8);
+            }
+        }
+}
+
 // ===========================================================================
 // N -> n key switch with a linear key on the matrix cores (the switch of
 // aes_gpu_ks.cuh:341-370, base 2^2 x 8 on the rounded top 16 bits, with one key row
@@ -1907,7 +2025,7 @@ size_t pfkl_key_bytes() { return (size_t)2 * PFKL_SLICES * 2 * CB_N * PFKL_KP; }
 size_t pfkl_key_offset(int u, int slice, int col, uint32_t k) {
     return ((size_t)u * PFKL_SLICES + slice) * 2 * CB_N * PFKL_KP + pfkl_frag_off((size_t)col, k);
 }
-size_t pfkl_digit_bytes(uint32_t rows) { return (size_t)((rows + 127) / 128 * 128) * PFKL_KP; }
+size_t pfkl_digit_bytes(uint32_t rows) { return (size_t)((rows + 255) / 256 * 256) * PFKL_KP; }
 uint32_t pfkl_kp() { return PFKL_KP; }
 
 void launch_gen_pfkl40(uint64_t seed, const int32_t* K2, int8_t* KT, hipStream_t st) {
@@ -1916,10 +2034,15 @@ void launch_gen_pfkl40(uint64_t seed, const int32_t* K2, int8_t* KT, hipStream_t
 
 void launch_pfkl(const uint64_t* lwe64, uint32_t B, int8_t* D, const int8_t* KT, int u0, int nu, uint32_t* out,
                  size_t out_stride, size_t out_u_offset, hipStream_t st) {
-    const uint32_t Mpad = (B + 127) / 128 * 128;
+    // many input tiles: the LDS-staged form (256-input tiles, one 8-wave workgroup per CU);
+    // few: 128-input tiles, for twice the workgroups
+    const bool staged = B > 1792;
+    const uint32_t Mpad = staged ? (B + 255) / 256 * 256 : (B + 127) / 128 * 128;
     k_pfkl_digits<<<grid_for((size_t)Mpad * (PFKL_KP / 8) / 256 + 1, 8192), 256, 0, st>>>(lwe64, B, Mpad, D);
     const unsigned rows = Mpad / 128, tiles = (unsigned)nu * 2 * CB_N / 64;
-    if (rows <= 2)
+    if (staged)
+        k_pfkl_gemm_lds<<<dim3(Mpad / 256, tiles), 512, 0, st>>>(D, B, KT, u0, out, out_stride, out_u_offset);
+    else if (rows <= 2)
         k_pfkl_gemm<<<dim3(tiles, rows), 256, 0, st>>>(D, B, KT, u0, out, out_stride, out_u_offset, 1);
     else
         k_pfkl_gemm<<<dim3(rows, tiles), 256, 0, st>>>(D, B, KT, u0, out, out_stride, out_u_offset, 0);
