@@ -198,3 +198,22 @@ def test_func_argument_errors(fctx, gk):
     with pytest.raises(tfhe_amd.TfheError) as e:
         fctx.eval_floor(d_in, 2 ** 16, d_out, batch=fctx.max_batch + 1)
     assert e.value.code in (tfhe_amd.TFHE_E_CAPACITY, tfhe_amd.TFHE_E_SHAPE)
+
+
+def test_func_empty_batches_leave_outputs_untouched(fctx, gk):
+    """A batch of 0 returns OK and launches nothing in every flow (outputs keep a sentinel);
+    EvalDecomp still reports its digit moduli (2^32: 8 digits of q_small, then 2^8)."""
+    import torch
+    SENT = 0x5A5A5A5A
+    d_in = to_dev(_cts(gk, [1], 2 ** 16, 1))
+    out = torch.full((1, O.gate_params().n + 1), SENT, dtype=torch.int32, device="cuda")
+    dig = torch.full((9, 1, O.gate_params().n + 1), SENT, dtype=torch.int32, device="cuda")
+    lut = _lut([(3 * m + 1) % 8 for m in range(8)])
+    fctx.bootstrap_func(d_in, Q, lut, Q, out, batch=0)
+    fctx.eval_func(d_in, Q, lut, out, batch=0)
+    fctx.eval_floor(d_in, 2 ** 16, out, batch=0)
+    fctx.eval_sign(d_in, 2 ** 16, out, batch=0)
+    mods = fctx.eval_decomp(d_in, 2 ** 32, dig, 9, batch=0)
+    torch.cuda.synchronize()
+    assert mods == [Q] * 8 + [2 ** 8]
+    assert (from_dev(out) == SENT).all() and (from_dev(dig) == SENT).all()
