@@ -970,27 +970,29 @@ def test_pfkl_switch_bit_exact(lctx, okeys):
         assert not got[:, 2 * N:].any()  # nothing written past the row
 
 
-def test_pfkl_staged_gemm_equals_tiled_and_oracle(lctx, okeys):
+@pytest.mark.parametrize("nb", [2000, 8192])
+def test_pfkl_staged_gemm_equals_tiled_and_oracle(lctx, okeys, nb):
     """Launches of more than 1792 PFKS inputs take the LDS-staged GEMM (256-input tiles,
-    8 waves); smaller ones the 128-input tiles.  2000 random inputs in one call equal two
-    calls of 1000 word for word (integer sums: the same words whatever the tiling), and
-    sampled rows, including the last one in the zero-padded tile, equal the oracle."""
+    8 waves); smaller ones the 128-input tiles.  nb random inputs in one call (2000: a
+    zero-padded last tile; 8192: a whole launch chunk of 32 tiles) equal calls of 1000
+    word for word (integer sums: the same words whatever the tiling), and sampled rows,
+    including the last one, equal the oracle."""
     import torch
     N = okeys.p.N
-    nb = 2000
-    rng = np.random.default_rng(61)
+    rng = np.random.default_rng(61 + nb)
     lwe = rng.integers(0, 2 ** 64, (nb, N + 1), dtype=np.uint64)
     d_lwe = to_dev64(lwe)
     for u in (0, 1):
         d_big = torch.empty((nb, 2 * N), dtype=torch.int32, device="cuda")
         lctx.pfks_batch(u, d_lwe, nb, 2 * N, d_big)
         d_small = torch.empty((nb, 2 * N), dtype=torch.int32, device="cuda")
-        for c0 in (0, nb // 2):
-            lctx.pfks_batch(u, d_lwe[c0:c0 + nb // 2].contiguous(), nb // 2, 2 * N, d_small[c0:c0 + nb // 2])
+        for c0 in range(0, nb, 1000):
+            c = min(1000, nb - c0)
+            lctx.pfks_batch(u, d_lwe[c0:c0 + c].contiguous(), c, 2 * N, d_small[c0:c0 + c])
         torch.cuda.synchronize()
         big = from_dev(d_big)
         assert np.array_equal(big, from_dev(d_small)), u
-        for k in (0, 255, 256, 1999):
+        for k in (0, 255, 256, nb - 1):
             assert np.array_equal(big[k], okeys.pfks_linear(u, lwe[k])), (u, k)
 
 
