@@ -1031,6 +1031,28 @@ def test_cbs_linear_chunked_batch_equals_parts(lctx, okeys):
     del d_all, d_part
 
 
+def test_cbs_linear_staged_pfks_equals_tiled(lctx, okeys):
+    """1000 inputs give 2000 PFKS inputs, which take the LDS-staged GEMM with both functions
+    in one launch; calls of 250 (500 PFKS inputs) take the 128-input tiles.  The RGSW rows
+    are equal bit for bit (the same PBS words, integer key-switch sums)."""
+    import torch
+    import tfhe_amd
+    batch = 1000
+    rng = np.random.default_rng(71)
+    bits = rng.integers(0, 2, batch)
+    cts = np.stack([tfhe_amd.lwe_encrypt(okeys.s, 39, k, int(bits[k]), p=2) for k in range(batch)])
+    d_cts = to_dev32(cts)
+    d_all = torch.empty((batch, 4, 2, 2048), dtype=torch.int32, device="cuda")
+    lctx.cbs_batch(d_cts, d_all)
+    torch.cuda.synchronize()
+    whole = from_dev(d_all)
+    d_part = torch.empty((batch // 4, 4, 2, 2048), dtype=torch.int32, device="cuda")
+    for lo in range(0, batch, batch // 4):
+        lctx.cbs_batch(d_cts[lo:lo + batch // 4].contiguous(), d_part)
+        torch.cuda.synchronize()
+        assert np.array_equal(from_dev(d_part), whole[lo:lo + batch // 4]), lo
+
+
 def test_aes128_linear_pfks(lctx, okeys):
     """Homomorphic AES-128 with the linear PFKS: FIPS-197 C.1 and random blocks
     decrypt exactly with phases inside half the margin; the graph replay equals the
